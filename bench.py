@@ -1,0 +1,224 @@
+#!/usr/bin/env python3
+"""Benchmark of the MI355X radio projection (c8mi) on the standalone radio
+workload: 2^20 straight e+- segments along a 60 deg zenith axis
+(Gaisser-Hillas depth, Xmax 750 g/cm^2, lambda 60 g/cm^2, lateral sigma 40 m,
+0.5-5 m, beta 0.99-1, 20 % negative excess), 160 ground antennas on a star of
+8 arms x 20 radii (25-500 m), 2048 bins at 0.2 ns, n(h) = 1 + 3.25e-4
+exp(-h / 8 km), CoREAS + ZHS endpoint formulae, int64 fixed-point waveforms.
+
+One step = one pass of the projection over the whole resident track list
+(reset of the waveforms + CoREAS+ZHS projection of 2^20 x 160 pairs).
+Prints one JSON line (see --help for options).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "corsika8-kokkos_amd"))
+
+import numpy as np  # noqa: E402
+
+METRIC = "EM particle-steps/s; radio track·antenna/s at ≤1e-3 err; p99 shower latency"
+UNIT = "track·antenna updates/s"
+# Algorithmic FP64 operations per track-antenna update of the fused CoREAS+ZHS
+# projection (add/mul/div/sqrt = 1, fma = 2), the per-unit figure of the
+# reference's RadioProjection.inl: CoREAS ~130 + ZHS ~155.
+FLOPS_PER_UPDATE = {"coreas_zhs": 285.0, "zhs": 155.0, "coreas": 130.0}
+FP64_PEAK_TFLOPS = 78.6
+HBM_PEAK_GBS = 8000.0
+TRACK_BYTES = 72
+
+
+def parse():
+    p = argparse.ArgumentParser(description=__doc__)
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=20)
+    p.add_argument("--warmup", type=int, default=3)
+    p.add_argument("--tracks", type=int, default=1 << 20)
+    p.add_argument("--mode", choices=("coreas_zhs", "zhs", "coreas"), default="coreas_zhs")
+    p.add_argument("--cpu-seconds", type=float, default=8.0,
+                   help="target wall time of the CPU-baseline sample")
+    p.add_argument("--no-cpu-baseline", action="store_true")
+    p.add_argument("--no-error-check", action="store_true")
+    p.add_argument("--copy-probe", type=int, default=0,
+                   help="only run a device-to-device copy of this many bytes (PMC calibration)")
+    p.add_argument("--chunk-tracks", type=int, default=0)
+    p.add_argument("--tile-bins", type=int, default=0)
+    p.add_argument("--target-workgroups", type=int, default=0)
+    return p.parse_args()
+
+
+def copy_probe(nbytes):
+    import c8mi
+
+    src = c8mi.DeviceArray(nbytes)
+    dst = c8mi.DeviceArray(nbytes)
+    for _ in range(3):
+        c8mi._check(c8mi.library().c8mi_memcpy_dtod(dst.pointer, src.pointer, c8mi.c_uint64(nbytes)))
+    c8mi.device_synchronize()
+    print(json.dumps({"copy_probe_bytes": nbytes}))
+
+
+def cpu_baseline(tracks, observers, propagation, mode, target_seconds):
+    """FP64 oracle over a bounded prefix of the same track list, all host
+    cores (OpenMP), timed as updates/s."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib
+
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count() or 1
+    threads = max(1, min(threads, os.cpu_count() or 1))
+    coreas = mode in ("coreas_zhs", "coreas")
+    zhs = mode in ("coreas_zhs", "zhs")
+    n = min(4096, tracks.shape[1])
+    while True:
+        sample = np.ascontiguousarray(tracks[:, :n])
+        t0 = time.perf_counter()
+        oracle_lib.radio_project(propagation, observers, sample, coreas=coreas, zhs=zhs,
+                                 floating=False, threads=threads)
+        dt = time.perf_counter() - t0
+        if dt >= target_seconds or n >= tracks.shape[1]:
+            break
+        n = min(tracks.shape[1], int(n * max(2.0, 1.2 * target_seconds / max(dt, 1e-3))))
+    updates = n * observers.size
+    return {"value": updates / dt, "unit": UNIT, "cores": threads, "kind": "port",
+            "sample": f"first {n} of the {tracks.shape[1]} tracks x {observers.size} antennas "
+                      f"({updates:.3e} updates, {dt:.2f} s wall), FP64 oracle, OpenMP"}
+
+
+def field_error(tracks, observers, propagation, mode, limit):
+    """Relative L2 error of the decoded GPU waveforms against the unquantised
+    FP64 oracle on 8 of the antennas (every 20th), full track list."""
+    import c8mi
+
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib
+
+    subset = np.ascontiguousarray(observers[::20])
+    coreas = mode in ("coreas_zhs", "coreas")
+    zhs = mode in ("coreas_zhs", "zhs")
+    threads = max(1, min(int(os.environ.get("OMP_NUM_THREADS", "0")) or 16, os.cpu_count() or 1))
+    ref = oracle_lib.radio_project(propagation, subset, tracks, coreas=coreas, zhs=zhs,
+                                   fixed=False, threads=threads)
+    acc = c8mi.RadioAccumulator(subset, propagation, coreas=coreas, zhs=zhs,
+                                fixed_point_limit=limit)
+    dev = c8mi.DeviceArray.from_host(tracks)
+    acc.accumulate_tracks(dev, tracks.shape[1], tracks.shape[1])
+    c, z, _ = acc.download()
+    acc.close()
+    dev.free()
+    errors = {}
+    if coreas:
+        errors["coreas"] = float(np.linalg.norm(c - ref["coreas_float"]) / np.linalg.norm(ref["coreas_float"]))
+    if zhs:
+        errors["zhs"] = float(np.linalg.norm(z - ref["zhs_float"]) / np.linalg.norm(ref["zhs_float"]))
+    return errors
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one GPU only (asked for --gpus {args.gpus}); "
+              "the radio and EM paths shard by observers/sub-showers without exchange, "
+              "which this single-GPU run does not exercise.", file=sys.stderr)
+        return 2
+    if args.copy_probe:
+        copy_probe(args.copy_probe)
+        return 0
+
+    import c8mi
+    from c8mi import workloads
+
+    if c8mi.device_count() < 1:
+        print("bench.py: no HIP device visible", file=sys.stderr)
+        return 1
+    tracks, observers, refractivity, (lo, hi) = workloads.config2_setup(count=args.tracks)
+    propagation = c8mi.Propagation(refractivity=refractivity, minimum_height_m=lo,
+                                   maximum_height_m=hi, step_m=1.0)
+    coreas = args.mode in ("coreas_zhs", "coreas")
+    zhs = args.mode in ("coreas_zhs", "zhs")
+    limit = 1.0
+    acc = c8mi.RadioAccumulator(observers, propagation, coreas=coreas, zhs=zhs,
+                                fixed_point_limit=limit)
+    if args.chunk_tracks or args.tile_bins or args.target_workgroups:
+        acc.set_tuning(args.chunk_tracks, args.tile_bins, args.target_workgroups)
+    n = tracks.shape[1]
+    device_tracks = c8mi.DeviceArray.from_host(tracks)
+
+    def step():
+        acc.reset()
+        acc.accumulate_tracks(device_tracks, n, n)
+
+    for _ in range(args.warmup):
+        step()
+    c8mi.device_synchronize()
+    acc.enable_kernel_timing(True)
+    c8mi.device_synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    c8mi.device_synchronize()
+    elapsed = time.perf_counter() - t0
+    kernel_ms, launches = acc.kernel_time()
+    _, _, counters = acc.download_fixed()
+    acc.close()
+
+    ms_per_step = elapsed * 1e3 / args.steps
+    updates = n * observers.size
+    value = updates / (elapsed / args.steps)
+    kernel_avg_ms = kernel_ms / max(launches, 1)
+    flops = FLOPS_PER_UPDATE[args.mode] * updates
+    achieved = flops / (kernel_avg_ms * 1e-3) / 1e12
+    result = {
+        "metric": METRIC,
+        "value": value,
+        "unit": UNIT,
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": f"radio standalone: {n} tracks x {observers.size} antennas, "
+                        f"{int(observers['number_of_bins'][0])} bins @ 0.2 ns, "
+                        f"{args.mode.replace('_', '+').upper()}, int64 fixed point",
+            "tracks": n,
+            "antennas": int(observers.size),
+            "bins": int(observers["number_of_bins"][0]),
+            "sample_rate_hz": float(observers["sample_rate_hz"][0]),
+            "mode": args.mode,
+        },
+        "roofline": {
+            "bound": "mfma",
+            "pipe": "fp64 VALU (no MFMA instruction is issued; MI355X's FP64 vector and "
+                    "matrix peaks are both 78.6 TFLOP/s)",
+            "achieved": achieved,
+            "peak": FP64_PEAK_TFLOPS,
+            "unit": "TFLOP/s",
+            "frac": achieved / FP64_PEAK_TFLOPS,
+            "traffic": None,
+            "kernel": f"radio_project_kernel<{int(coreas)},{int(zhs)},1>",
+            "kernel_avg_ms": kernel_avg_ms,
+            "flops_per_update": FLOPS_PER_UPDATE[args.mode],
+            "algorithmic_hbm_gbs": (n * TRACK_BYTES) / (kernel_avg_ms * 1e-3) / 1e9,
+        },
+        "counters": counters,
+    }
+    if not args.no_error_check:
+        result["field_error_vs_fp64"] = field_error(tracks, observers, propagation, args.mode, limit)
+    if not args.no_cpu_baseline:
+        base = cpu_baseline(tracks, observers, propagation, args.mode, args.cpu_seconds)
+        result["cpu_baseline"] = base
+        result["speedup_vs_cpu_baseline"] = value / base["value"]
+    print(json.dumps(result), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

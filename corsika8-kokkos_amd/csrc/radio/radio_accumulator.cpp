@@ -1,0 +1,464 @@
+// Host side of the MI355X radio accumulator. See radio_accumulator.hpp.
+#include "radio_accumulator.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <limits>
+#include <stdexcept>
+#include <string>
+
+#include "radio_launch.hpp"
+
+namespace c8mi::radio {
+namespace {
+
+constexpr double kHeadroom = 4611686018427387904.;  // 2^62, RadioProjectionData.hpp:34
+
+// RadioAccumulator::validate (corsika/detail/kokkos/radio/RadioAccumulator.inl:934-975).
+void validate(c8mi_radio_config const& config) {
+  auto const& p = config.propagation;
+  if (!config.coreas_enabled && !config.zhs_enabled)
+    throw std::invalid_argument("enabled radio has no algorithm");
+  if (!std::isfinite(config.fixed_point_field_limit_v_per_m) ||
+      !(config.fixed_point_field_limit_v_per_m > 0.))
+    throw std::invalid_argument("invalid radio fixed-point limit");
+  if (config.zhs_subtrack_refinement < 1 || config.zhs_subtrack_refinement > 64)
+    throw std::invalid_argument("ZHS subtrack refinement must be in [1,64]");
+  if (!std::isfinite(p.homogeneous_refractive_index) || p.homogeneous_refractive_index < 0.)
+    throw std::invalid_argument("invalid homogeneous radio refractive index");
+  bool const homogeneous = p.homogeneous_refractive_index > 0.;
+  if (!homogeneous &&
+      (p.table_size < 11 || p.refractivity == nullptr || p.integrated_refractivity == nullptr ||
+       !std::isfinite(p.minimum_height_m) || !std::isfinite(p.maximum_height_m) ||
+       !(p.maximum_height_m > p.minimum_height_m) || !std::isfinite(p.step_m) ||
+       !(p.step_m > 0.) || !std::isfinite(p.inverse_step_per_m) ||
+       !(p.inverse_step_per_m > 0.)))
+    throw std::invalid_argument("invalid flat-atmosphere radio snapshot");
+  auto check = [&](c8mi_radio_observer const* observers, std::uint64_t count) {
+    if (count != 0 && observers == nullptr)
+      throw std::invalid_argument("null radio observer list");
+    for (std::uint64_t i = 0; i < count; ++i) {
+      auto const& o = observers[i];
+      if (!std::isfinite(o.start_time_s) || !std::isfinite(o.duration_s) ||
+          !(o.duration_s >= 0.) || !std::isfinite(o.sample_rate_hz) ||
+          !(o.sample_rate_hz > 0.) || o.number_of_bins == 0)
+        throw std::invalid_argument("invalid radio observer");
+      double const height = (o.position_m[2] - p.minimum_height_m) * p.inverse_step_per_m;
+      if (!homogeneous && (!std::isfinite(height) || height < 0. ||
+                           height + 0.5 >= static_cast<double>(p.table_size)))
+        throw std::invalid_argument("radio observer outside propagation table");
+      for (double coordinate : o.position_m)
+        if (!std::isfinite(coordinate))
+          throw std::invalid_argument("radio observer position is not finite");
+    }
+  };
+  check(config.coreas_observers, config.coreas_enabled ? config.coreas_observer_count : 0);
+  check(config.zhs_observers, config.zhs_enabled ? config.zhs_observer_count : 0);
+}
+
+// RadioAccumulator::makeObservers (RadioAccumulator.inl:591-617).
+std::vector<ObserverDev> make_observers(c8mi_radio_observer const* input, std::uint64_t count,
+                                        double limit, bool zhs_potential,
+                                        std::uint64_t& total_bins, std::uint64_t& max_bins) {
+  std::vector<ObserverDev> result;
+  result.reserve(count);
+  total_bins = 0;
+  max_bins = 0;
+  for (std::uint64_t i = 0; i < count; ++i) {
+    auto const& o = input[i];
+    ObserverDev d{};
+    for (int a = 0; a < 3; ++a) d.position[a] = o.position_m[a];
+    d.start_time = o.start_time_s;
+    d.duration = o.duration_s;
+    d.sample_rate = o.sample_rate_hz;
+    double const potential_factor = zhs_potential ? o.sample_rate_hz : 1.;
+    d.fixed_scale = kHeadroom * potential_factor / limit;
+    d.number_of_bins = o.number_of_bins;
+    d.waveform_offset = total_bins;
+    if (o.number_of_bins > std::numeric_limits<std::uint64_t>::max() - total_bins)
+      throw std::overflow_error("radio waveform bin overflow");
+    total_bins += o.number_of_bins;
+    max_bins = std::max<std::uint64_t>(max_bins, o.number_of_bins);
+    result.push_back(d);
+  }
+  return result;
+}
+
+// observersCanShareTrackKinematics (RadioAccumulator.inl:619-632).
+bool can_fuse(std::vector<ObserverDev> const& a, std::vector<ObserverDev> const& b) {
+  if (a.size() != b.size() || a.empty()) return false;
+  for (std::size_t i = 0; i < a.size(); ++i) {
+    if (a[i].start_time != b[i].start_time || a[i].duration != b[i].duration ||
+        a[i].sample_rate != b[i].sample_rate || a[i].number_of_bins != b[i].number_of_bins ||
+        a[i].position[0] != b[i].position[0] || a[i].position[1] != b[i].position[1] ||
+        a[i].position[2] != b[i].position[2])
+      return false;
+  }
+  return true;
+}
+
+template <class T>
+void upload(DeviceBuffer<T>& device, T const* host, std::size_t count, hipStream_t stream) {
+  device.allocate(count);
+  if (count)
+    C8MI_HIP(hipMemcpyAsync(device.data(), host, count * sizeof(T), hipMemcpyHostToDevice,
+                            stream));
+}
+
+}  // namespace
+
+RadioAccumulator::RadioAccumulator(c8mi_radio_config const& config, hipStream_t stream)
+    : config_(config) {
+  validate(config);
+  if (stream == nullptr) {
+    C8MI_HIP(hipStreamCreate(&stream_));
+    owns_stream_ = true;
+  } else {
+    stream_ = stream;
+  }
+  try {
+    double const limit = config.fixed_point_field_limit_v_per_m;
+    if (config.coreas_enabled) {
+      coreas_snapshots_.assign(config.coreas_observers,
+                               config.coreas_observers + config.coreas_observer_count);
+      coreas_host_ = make_observers(config.coreas_observers, config.coreas_observer_count,
+                                    limit, false, coreas_bins_, coreas_max_bins_);
+    }
+    if (config.zhs_enabled) {
+      zhs_snapshots_.assign(config.zhs_observers,
+                            config.zhs_observers + config.zhs_observer_count);
+      zhs_host_ = make_observers(config.zhs_observers, config.zhs_observer_count, limit, true,
+                                 zhs_bins_, zhs_max_bins_);
+    }
+    // The config keeps borrowed pointers only for validation; the snapshots
+    // above own the observer data from here on.
+    config_.coreas_observers = nullptr;
+    config_.zhs_observers = nullptr;
+    upload(coreas_observers_, coreas_host_.data(), coreas_host_.size(), stream_);
+    upload(zhs_observers_, zhs_host_.data(), zhs_host_.size(), stream_);
+    auto const& p = config.propagation;
+    propagation_.homogeneous_index = p.homogeneous_refractive_index;
+    propagation_.minimum_height = p.minimum_height_m;
+    propagation_.maximum_height = p.maximum_height_m;
+    propagation_.step = p.step_m;
+    propagation_.inverse_step = p.inverse_step_per_m;
+    propagation_.slope_n_lower = p.slope_refractivity_lower;
+    propagation_.slope_i_lower = p.slope_integrated_refractivity_lower;
+    propagation_.slope_n_upper = p.slope_refractivity_upper;
+    propagation_.slope_i_upper = p.slope_integrated_refractivity_upper;
+    if (!(p.homogeneous_refractive_index > 0.)) {
+      upload(refractivity_, p.refractivity, p.table_size, stream_);
+      upload(integrated_, p.integrated_refractivity, p.table_size, stream_);
+      propagation_.table_size = p.table_size;
+    }
+    propagation_.refractivity = refractivity_.data();
+    propagation_.integrated = integrated_.data();
+    config_.propagation.refractivity = nullptr;
+    config_.propagation.integrated_refractivity = nullptr;
+    if (config.deterministic) {
+      coreas_fixed_.allocate(3 * coreas_bins_);
+      zhs_fixed_.allocate(3 * zhs_bins_);
+    } else {
+      coreas_floating_.allocate(3 * coreas_bins_);
+      zhs_floating_.allocate(3 * zhs_bins_);
+    }
+    counters_.allocate(1);
+    host_batch_ = PinnedBuffer<c8mi_lepton_transport_record>(kHostTrackBatchCapacity);
+    host_batch_device_.allocate(kHostTrackBatchCapacity);
+    fuse_ = config.coreas_enabled && config.zhs_enabled && can_fuse(coreas_host_, zhs_host_);
+    C8MI_HIP(hipStreamSynchronize(stream_));
+    reset();
+  } catch (...) {
+    if (owns_stream_) (void)hipStreamDestroy(stream_);
+    throw;
+  }
+}
+
+RadioAccumulator::~RadioAccumulator() {
+  if (stream_) (void)hipStreamSynchronize(stream_);
+  for (auto& e : pending_events_) {
+    (void)hipEventDestroy(e.first);
+    (void)hipEventDestroy(e.second);
+  }
+  for (auto& e : free_events_) {
+    (void)hipEventDestroy(e.first);
+    (void)hipEventDestroy(e.second);
+  }
+  if (owns_stream_) (void)hipStreamDestroy(stream_);
+}
+
+void RadioAccumulator::reset() {
+  if (host_count_ != 0)
+    throw std::logic_error("cannot reset radio with uncommitted host tracks");
+  if (coreas_fixed_.size())
+    C8MI_HIP(hipMemsetAsync(coreas_fixed_.data(), 0, coreas_fixed_.bytes(), stream_));
+  if (zhs_fixed_.size())
+    C8MI_HIP(hipMemsetAsync(zhs_fixed_.data(), 0, zhs_fixed_.bytes(), stream_));
+  if (coreas_floating_.size())
+    C8MI_HIP(hipMemsetAsync(coreas_floating_.data(), 0, coreas_floating_.bytes(), stream_));
+  if (zhs_floating_.size())
+    C8MI_HIP(hipMemsetAsync(zhs_floating_.data(), 0, zhs_floating_.bytes(), stream_));
+  C8MI_HIP(hipMemsetAsync(counters_.data(), 0, counters_.bytes(), stream_));
+  C8MI_HIP(hipStreamSynchronize(stream_));
+  statistics_ = {};
+  downloaded_ = false;
+  sealed_ = false;
+}
+
+void RadioAccumulator::enqueue_host_lepton_tracks(c8mi_lepton_transport_record const* records,
+                                                  std::size_t count) {
+  if (downloaded_ || sealed_)
+    throw std::logic_error("radio already downloaded; reset before host tracks");
+  if (count != 0 && records == nullptr) throw std::invalid_argument("null host radio track input");
+  while (count != 0) {
+    std::size_t const copied = std::min(count, kHostTrackBatchCapacity - host_count_);
+    std::memcpy(host_batch_.data() + host_count_, records,
+                copied * sizeof(c8mi_lepton_transport_record));
+    host_count_ += copied;
+    statistics_.host_tracks_enqueued += copied;
+    records += copied;
+    count -= copied;
+    if (host_count_ == kHostTrackBatchCapacity) flush_host_lepton_tracks();
+  }
+}
+
+void RadioAccumulator::flush_host_lepton_tracks() {
+  if (host_count_ == 0) return;
+  C8MI_HIP(hipMemcpyAsync(host_batch_device_.data(), host_batch_.data(),
+                          host_count_ * sizeof(c8mi_lepton_transport_record),
+                          hipMemcpyHostToDevice, stream_));
+  // The pinned batch is refilled by the next enqueue; finish its upload first
+  // (RadioAccumulator.inl:277-281). Projection stays asynchronous.
+  C8MI_HIP(hipStreamSynchronize(stream_));
+  statistics_.host_to_device_bytes += host_count_ * sizeof(c8mi_lepton_transport_record);
+  ++statistics_.host_track_batches;
+  std::size_t const count = host_count_;
+  host_count_ = 0;
+  accumulate_lepton_records(host_batch_device_.data(), count);
+}
+
+void RadioAccumulator::ensure_track_capacity(std::uint64_t count) {
+  if (track_capacity_ >= count) return;
+  std::uint64_t const grown = track_capacity_ == 0 ? count : std::max(count, 2 * track_capacity_);
+  if (track_capacity_ != 0) C8MI_HIP(hipStreamSynchronize(stream_));
+  tracks_.allocate(C8MI_RADIO_TRACK_FIELDS * grown);
+  track_capacity_ = grown;
+}
+
+void RadioAccumulator::accumulate_lepton_records(
+    c8mi_lepton_transport_record const* device_records, std::size_t count) {
+  if (count == 0) return;
+  if (sealed_) throw std::logic_error("radio endpoint already committed; reset before new tracks");
+  if (device_records == nullptr) throw std::invalid_argument("null device radio records");
+  ensure_track_capacity(count);
+  launch_records_to_tracks(device_records, count, tracks_.data(), track_capacity_, stream_);
+  project(tracks_.data(), track_capacity_, count);
+}
+
+void RadioAccumulator::accumulate_tracks(double const* device_tracks, std::uint64_t stride,
+                                         std::uint64_t count) {
+  if (count == 0) return;
+  if (sealed_) throw std::logic_error("radio endpoint already committed; reset before new tracks");
+  if (device_tracks == nullptr) throw std::invalid_argument("null device radio tracks");
+  if (stride < count) throw std::invalid_argument("radio track stride below track count");
+  project(device_tracks, stride, count);
+}
+
+ProjectionLaunch RadioAccumulator::base_launch(bool coreas, bool zhs, std::uint32_t observers,
+                                               std::uint64_t max_bins,
+                                               std::uint64_t count) const {
+  ProjectionLaunch L{};
+  L.observer_count = observers;
+  L.zhs_refinement = config_.zhs_subtrack_refinement;
+  L.propagation = propagation_;
+  L.counters = counters_.data();
+  // Window tile: the whole waveform when it fits the LDS budget.
+  std::size_t const words = kMaxDynamicLdsBytes / sizeof(long long);
+  std::size_t const per_bin = (coreas ? 3u : 0u) + (zhs ? 3u : 0u);
+  std::uint64_t capacity = (words - 3) / per_bin;
+  if (tile_bins_limit_ != 0) capacity = std::min<std::uint64_t>(capacity, tile_bins_limit_);
+  std::uint64_t const tile = std::min<std::uint64_t>(std::max<std::uint64_t>(max_bins, 1), capacity);
+  L.tile_bins = static_cast<std::uint32_t>(tile);
+  L.bin_tiles = static_cast<std::uint32_t>((max_bins + tile - 1) / tile);
+  if (L.bin_tiles == 0) L.bin_tiles = 1;
+  std::uint64_t chunk = chunk_tracks_;
+  if (chunk == 0) {
+    std::uint64_t const per_chunk = std::max<std::uint64_t>(1, std::uint64_t(observers) * L.bin_tiles);
+    std::uint64_t const chunks_wanted = std::max<std::uint64_t>(1, target_workgroups_ / per_chunk);
+    chunk = (count + chunks_wanted - 1) / chunks_wanted;
+    chunk = std::max<std::uint64_t>(kProjectionBlock, (chunk + kProjectionBlock - 1) /
+                                                          kProjectionBlock * kProjectionBlock);
+  }
+  L.chunk_size = chunk;
+  std::uint64_t const chunks = (count + chunk - 1) / chunk;
+  if (chunks > std::numeric_limits<std::uint32_t>::max())
+    throw std::length_error("radio track batch too large for one launch");
+  L.chunks = static_cast<std::uint32_t>(chunks);
+  std::uint64_t const grid =
+      (chunks + 7) / 8 * 8 * std::uint64_t(observers) * L.bin_tiles;
+  if (grid > std::numeric_limits<std::uint32_t>::max())
+    throw std::length_error("radio projection grid too large");
+  return L;
+}
+
+void RadioAccumulator::project(double const* tracks, std::uint64_t stride, std::uint64_t count) {
+  auto run = [&](bool coreas, bool zhs, bool count_valid) {
+    std::uint32_t const observers = static_cast<std::uint32_t>(
+        coreas ? coreas_host_.size() : zhs_host_.size());
+    if (observers == 0) return;
+    std::uint64_t const max_bins = std::max(coreas ? coreas_max_bins_ : 0, zhs ? zhs_max_bins_ : 0);
+    ProjectionLaunch L = base_launch(coreas, zhs, observers, max_bins, count);
+    L.tracks = tracks;
+    L.stride = stride;
+    L.track_count = count;
+    L.coreas_observers = coreas_observers_.data();
+    L.zhs_observers = zhs_observers_.data();
+    bool const fixed = config_.deterministic != 0;
+    L.coreas_waves = fixed ? static_cast<void*>(coreas_fixed_.data())
+                           : static_cast<void*>(coreas_floating_.data());
+    L.zhs_waves = fixed ? static_cast<void*>(zhs_fixed_.data())
+                        : static_cast<void*>(zhs_floating_.data());
+    L.coreas_total_bins = coreas_bins_;
+    L.zhs_total_bins = zhs_bins_;
+    L.count_valid = count_valid ? 1u : 0u;
+    std::pair<hipEvent_t, hipEvent_t> events{nullptr, nullptr};
+    if (timing_) {
+      if (!free_events_.empty()) {
+        events = free_events_.back();
+        free_events_.pop_back();
+      } else {
+        C8MI_HIP(hipEventCreate(&events.first));
+        C8MI_HIP(hipEventCreate(&events.second));
+      }
+      C8MI_HIP(hipEventRecord(events.first, stream_));
+    }
+    launch_projection(L, coreas, zhs, fixed, stream_);
+    if (timing_) {
+      C8MI_HIP(hipEventRecord(events.second, stream_));
+      pending_events_.push_back(events);
+    }
+    ++statistics_.projection_launches;
+    statistics_.track_observer_pairs += count * observers * (std::uint64_t(coreas) + zhs);
+    if (coreas && zhs) statistics_.fused_track_observer_pairs += count * observers;
+  };
+  if (fuse_) {
+    run(true, true, true);
+  } else {
+    bool first = true;
+    if (config_.coreas_enabled) {
+      run(true, false, first);
+      first = false;
+    }
+    if (config_.zhs_enabled) run(false, true, first);
+  }
+  downloaded_ = false;
+}
+
+void RadioAccumulator::copy_counters(c8mi_radio_counters* out) {
+  CountersDev host{};
+  C8MI_HIP(hipMemcpyAsync(&host, counters_.data(), sizeof(host), hipMemcpyDeviceToHost, stream_));
+  C8MI_HIP(hipStreamSynchronize(stream_));
+  statistics_.device_to_host_bytes += sizeof(host);
+  if (out) {
+    out->coreas_contributions = host.coreas_contributions;
+    out->zhs_contributions = host.zhs_contributions;
+    out->zhs_subtracks = host.zhs_subtracks;
+    out->valid_tracks = host.valid_tracks;
+    out->fixed_point_overflows = host.fixed_point_overflows;
+  }
+}
+
+void RadioAccumulator::download(double* coreas, double* zhs, c8mi_radio_counters* counters) {
+  if (downloaded_ || sealed_) throw std::logic_error("radio waveforms downloaded twice");
+  flush_host_lepton_tracks();
+  c8mi_radio_counters c{};
+  copy_counters(&c);
+  if (counters) *counters = c;
+  if (c.fixed_point_overflows != 0)
+    throw OverflowError(
+        "radio fixed-point accumulator overflowed; increase fixed_point_field_limit_v_per_m");
+  double const limit = config_.fixed_point_field_limit_v_per_m;
+  auto fetch = [&](double* out, std::vector<c8mi_radio_observer> const& snapshots,
+                   std::uint64_t total, DeviceBuffer<long long> const& fixed,
+                   DeviceBuffer<double> const& floating, bool potential) {
+    if (out == nullptr || total == 0) return;
+    if (fixed.size()) {
+      std::vector<long long> raw(3 * total);
+      C8MI_HIP(hipMemcpyAsync(raw.data(), fixed.data(), fixed.bytes(), hipMemcpyDeviceToHost,
+                              stream_));
+      C8MI_HIP(hipStreamSynchronize(stream_));
+      statistics_.device_to_host_bytes += fixed.bytes();
+      // downloadSet (RadioAccumulator.inl:850-899).
+      std::uint64_t offset = 0;
+      for (auto const& o : snapshots) {
+        double const factor = potential ? o.sample_rate_hz : 1.;
+        double const inverse = limit / (kHeadroom * factor);
+        for (std::uint64_t b = 0; b < o.number_of_bins; ++b) {
+          std::uint64_t const j = offset + b;
+          out[j] = static_cast<double>(raw[j]) * inverse;
+          out[total + j] = static_cast<double>(raw[total + j]) * inverse;
+          out[2 * total + j] = static_cast<double>(raw[2 * total + j]) * inverse;
+        }
+        offset += o.number_of_bins;
+      }
+    } else {
+      C8MI_HIP(hipMemcpyAsync(out, floating.data(), floating.bytes(), hipMemcpyDeviceToHost,
+                              stream_));
+      C8MI_HIP(hipStreamSynchronize(stream_));
+      statistics_.device_to_host_bytes += floating.bytes();
+    }
+  };
+  fetch(coreas, coreas_snapshots_, coreas_bins_, coreas_fixed_, coreas_floating_, false);
+  fetch(zhs, zhs_snapshots_, zhs_bins_, zhs_fixed_, zhs_floating_, true);
+  downloaded_ = true;
+}
+
+void RadioAccumulator::download_fixed(std::int64_t* coreas, std::int64_t* zhs,
+                                      c8mi_radio_counters* counters) {
+  if (!config_.deterministic) throw std::logic_error("fixed radio snapshot needs deterministic mode");
+  if (downloaded_ || sealed_) throw std::logic_error("fixed radio snapshot already committed");
+  flush_host_lepton_tracks();
+  sealed_ = true;
+  copy_counters(counters);
+  if (coreas && coreas_fixed_.size())
+    C8MI_HIP(hipMemcpyAsync(coreas, coreas_fixed_.data(), coreas_fixed_.bytes(),
+                            hipMemcpyDeviceToHost, stream_));
+  if (zhs && zhs_fixed_.size())
+    C8MI_HIP(hipMemcpyAsync(zhs, zhs_fixed_.data(), zhs_fixed_.bytes(), hipMemcpyDeviceToHost,
+                            stream_));
+  C8MI_HIP(hipStreamSynchronize(stream_));
+  statistics_.device_to_host_bytes += coreas_fixed_.bytes() + zhs_fixed_.bytes();
+  downloaded_ = true;
+}
+
+void RadioAccumulator::set_tuning(std::uint64_t chunk_tracks, std::uint32_t tile_bins,
+                                  std::uint32_t target_workgroups) {
+  if (chunk_tracks % kProjectionBlock != 0)
+    throw std::invalid_argument("radio chunk size must be a multiple of the workgroup size");
+  chunk_tracks_ = chunk_tracks;
+  tile_bins_limit_ = tile_bins;
+  if (target_workgroups != 0) target_workgroups_ = target_workgroups;
+}
+
+void RadioAccumulator::enable_kernel_timing(bool enable) {
+  timing_ = enable;
+  timed_ms_ = 0.;
+  timed_launches_ = 0;
+}
+
+double RadioAccumulator::kernel_time_ms(std::uint64_t* launches) {
+  C8MI_HIP(hipStreamSynchronize(stream_));
+  for (auto& e : pending_events_) {
+    float ms = 0.f;
+    C8MI_HIP(hipEventElapsedTime(&ms, e.first, e.second));
+    timed_ms_ += ms;
+    ++timed_launches_;
+    free_events_.push_back(e);
+  }
+  pending_events_.clear();
+  if (launches) *launches = timed_launches_;
+  return timed_ms_;
+}
+
+}  // namespace c8mi::radio

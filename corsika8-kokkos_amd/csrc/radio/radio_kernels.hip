@@ -1,0 +1,761 @@
+// CoREAS / ZHS endpoint projection for gfx950.
+//
+// Physics: corsika/detail/kokkos/radio/algorithms/RadioProjection.inl (propagate
+// :170-238, accumulate_co_reas :338-409, accumulate_zhs_segment :411-480,
+// accumulate_zhs :482-529, add_sample :262-301). Every floating-point expression
+// keeps the reference's operation order and the file is built without
+// contraction, so the int64 fixed-point waveforms equal the FP64 restatement's.
+//
+// MI355X design (differs from the reference's track-tile x observer-tile teams,
+// RadioAccumulator.inl:15-85):
+//  * one workgroup = one observer x one chunk of tracks x one window tile; the
+//    observer's whole waveform tile lives in LDS as int64 fixed point, so the
+//    per-sample atomics are ds_add_u64, and HBM sees one flush per workgroup;
+//  * ZHS middle bins all receive the same increment, so they are added to an
+//    LDS difference array (two atomics per component instead of one per bin)
+//    and recovered by a block prefix scan before the flush; integer addition
+//    makes this bit-identical to the per-bin loop;
+//  * threads stride over tracks (coalesced 9 x FP64 structure-of-arrays loads);
+//  * blockIdx -> (chunk, observer) is XCD-aware: the eight XCDs take disjoint
+//    chunk sets and walk all observers of a chunk back to back, so a chunk is
+//    fetched from HBM once per XCD and re-read from that XCD's L2.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdint>
+#include <type_traits>
+
+#include "radio_device.hpp"
+#include "radio_launch.hpp"
+#include "../runtime.hpp"
+
+namespace c8mi::radio {
+namespace {
+
+constexpr double kC = 299792458.;
+constexpr double kE = 1.6021766208e-19;
+constexpr double kEps0 = 8.8541878128e-12;
+constexpr double kPi = 3.141592653589793238462643383279502884;
+constexpr double kEmConstant = 1. / (4. * kPi * kEps0 * kC);
+constexpr double kChargeEm = kE * kEmConstant;
+constexpr double kCoreasThreshold = 1.e-3;
+constexpr double kIntLimit = 9223372036854775808.;  // 2^63
+
+constexpr int kXcds = 8;
+
+struct V3 {
+  double x, y, z;
+};
+__device__ __forceinline__ V3 vadd(V3 a, V3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+__device__ __forceinline__ V3 vsub(V3 a, V3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+__device__ __forceinline__ V3 vmul(V3 a, double s) { return {a.x * s, a.y * s, a.z * s}; }
+__device__ __forceinline__ V3 vdiv(V3 a, double s) { return {a.x / s, a.y / s, a.z / s}; }
+__device__ __forceinline__ double vdot(V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ __forceinline__ V3 vcross(V3 a, V3 b) {
+  return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+
+__device__ __forceinline__ double nearest_bin(double scaled) {
+  double const integral = floor(scaled);
+  return integral + (scaled - integral >= 0.5 ? 1. : 0.);
+}
+
+// Device form of co_reas_doppler: error-free products and sums when the
+// ordinary expression cancels to exactly zero.
+__device__ __forceinline__ double doppler(double index, V3 beta, V3 emit) {
+  double const ordinary = 1. - index * vdot(beta, emit);
+  if (ordinary != 0.) return ordinary;
+  double const px = beta.x * emit.x, py = beta.y * emit.y, pz = beta.z * emit.z;
+  double const ex = fma(beta.x, emit.x, -px);
+  double const ey = fma(beta.y, emit.y, -py);
+  double const ez = fma(beta.z, emit.z, -pz);
+  double const xy = px + py;
+  double const xyv = xy - px;
+  double const xye = (px - (xy - xyv)) + (py - xyv);
+  double const xyz = xy + pz;
+  double const xyzv = xyz - xy;
+  double const xyze = (xy - (xyz - xyzv)) + (pz - xyzv);
+  double const low = ex + ey + ez + xye + xyze;
+  double const product = index * xyz;
+  double const perr = fma(index, xyz, -product) + index * low;
+  return (1. - product) - perr;
+}
+
+struct Track {
+  V3 start, end, disp, beta;
+  double t0, t1, duration, length, beta_module, constant;
+};
+
+__device__ __forceinline__ bool load_track(double const* __restrict__ tracks,
+                                           std::uint64_t stride, std::uint64_t i, Track& t) {
+  t.start = {tracks[i], tracks[stride + i], tracks[2 * stride + i]};
+  t.end = {tracks[3 * stride + i], tracks[4 * stride + i], tracks[5 * stride + i]};
+  t.t0 = tracks[6 * stride + i];
+  t.t1 = tracks[7 * stride + i];
+  double const signed_weight = tracks[8 * stride + i];
+  t.duration = t.t1 - t.t0;
+  t.disp = vsub(t.end, t.start);
+  t.length = sqrt(vdot(t.disp, t.disp));
+  if (!(t.duration > 0.) || !(t.length > 0.)) return false;
+  t.beta = vdiv(t.disp, kC * t.duration);
+  t.beta_module = t.length / (kC * t.duration);
+  t.constant = kChargeEm * signed_weight;
+  return true;
+}
+
+// Observer quantities that do not depend on the track, evaluated once per
+// thread with the same expressions propagate() and add_sample() use.
+struct Obs {
+  V3 position;
+  double start, duration, rate, scale, grid, wavelength;
+  double nbins_d;
+  std::uint64_t nbins, offset;
+  double destination_height;
+  std::uint64_t destination_index;
+  double n_destination_table, integrated_destination;
+};
+
+struct Path {
+  double time, n_source, n_destination, distance;
+  V3 emit;
+};
+
+__device__ __forceinline__ Path propagate(PropagationDev const& p, V3 source, Obs const& o) {
+  Path r;
+  V3 const disp = vsub(o.position, source);
+  double const distance = sqrt(vdot(disp, disp));
+  V3 const emit = vdiv(disp, distance);
+  r.distance = distance;
+  r.emit = emit;
+  if (p.homogeneous_index > 0.) {
+    double const index = p.homogeneous_index;
+    r.time = index * distance / kC;
+    r.n_source = index;
+    r.n_destination = index;
+    return r;
+  }
+  double const source_height = (source.z - p.minimum_height) * p.inverse_step;
+  double const destination_height = o.destination_height;
+  std::uint64_t const last = p.table_size - 1;
+  double n_source = 1.;
+  double integrated_source = 1.;
+  double n_destination = o.n_destination_table;
+  double height = 1.;
+  if (source_height + 0.5 >= static_cast<double>(last)) {
+    n_source = p.refractivity[last] +
+               p.slope_n_upper * fabs((source.z - p.maximum_height) * p.inverse_step) + 1.;
+    integrated_source =
+        p.integrated[last] + p.slope_i_upper * fabs(p.maximum_height - source.z);
+    height = source_height - destination_height;
+  } else if (source_height + 0.5 < static_cast<double>(last) && source_height > 0.) {
+    std::uint64_t const source_index = static_cast<std::uint64_t>(source_height + 0.5);
+    n_source = p.refractivity[source_index] + 1.;
+    integrated_source = p.integrated[source_index];
+    height = static_cast<double>(static_cast<std::int64_t>(source_index) -
+                                 static_cast<std::int64_t>(o.destination_index)) *
+             p.step;
+    if (height == 0.) height = 1.;
+  } else if (source_height == 0.) {
+    n_source = 1.;
+    n_destination = 1.;
+    integrated_source = p.integrated[0];
+    height = destination_height - source_height;
+  } else {
+    n_source = p.refractivity[0] + p.slope_n_lower * fabs(source_height) + 1.;
+    integrated_source = p.integrated[0] + p.slope_i_lower * fabs(source_height);
+    n_destination = 1.;
+    height = destination_height - fabs(source_height);
+  }
+  r.time = (1. + (integrated_source - o.integrated_destination) / height) * distance / kC;
+  r.n_source = n_source;
+  r.n_destination = n_destination;
+  return r;
+}
+
+// Per-thread tallies, folded into the block and then the global counters.
+struct Tally {
+  unsigned long long coreas = 0, zhs = 0, subtracks = 0, valid = 0, overflow = 0;
+  double abs_sum = 0.;  // sum of |increments|: bounds every LDS partial sum
+};
+
+// LDS accumulator for one mode: `Fixed` selects int64 fixed point (the
+// production default) or FP64. `Diff` stores a difference array.
+template <bool Fixed>
+struct Tile {
+  using Acc = std::conditional_t<Fixed, long long, double>;
+  Acc* data;                 // 3 components x stride
+  std::uint32_t stride;      // bins per component row
+  std::uint64_t begin;       // first global bin of this tile
+  std::uint32_t bins;        // bins owned by this tile
+};
+
+// add_radio_fixed_point on an LDS cell; returns false on a non-representable
+// contribution (the reference's overflow case), in which case nothing is added.
+__device__ __forceinline__ bool fixed_increment(double value, double scale, long long& inc,
+                                                double& abs_sum) {
+  double const scaled = value * scale;
+  if (!isfinite(scaled) || fabs(scaled) >= kIntLimit) return false;
+  inc = static_cast<long long>(rint(scaled));
+  abs_sum += fabs(scaled);
+  return true;
+}
+
+template <class T>
+__device__ __forceinline__ void lds_add(T* address, T value) {
+  if constexpr (std::is_same_v<T, long long>)
+    atomicAdd(reinterpret_cast<unsigned long long*>(address),
+              static_cast<unsigned long long>(value));
+  else
+    atomicAdd(address, value);
+}
+
+// CoREAS sample (add_sample with time gating, IsVectorPotential = false).
+template <bool Fixed>
+__device__ __forceinline__ void add_field_sample(Obs const& o, Tile<Fixed> const& tile,
+                                                 double time, V3 c, Tally& tally) {
+  if (time < o.start || time > o.start + o.duration) return;
+  double const bin_value = nearest_bin((time - o.start) * o.rate);
+  if (!(bin_value >= 0. && bin_value < o.nbins_d)) return;
+  std::uint64_t const bin = static_cast<std::uint64_t>(bin_value);
+  if (bin >= o.nbins) return;
+  if (bin < tile.begin || bin >= tile.begin + tile.bins) return;
+  std::uint32_t const local = static_cast<std::uint32_t>(bin - tile.begin);
+  if constexpr (Fixed) {
+    long long ix = 0, iy = 0, iz = 0;
+    bool const vx = fixed_increment(c.x, o.scale, ix, tally.abs_sum);
+    bool const vy = fixed_increment(c.y, o.scale, iy, tally.abs_sum);
+    bool const vz = fixed_increment(c.z, o.scale, iz, tally.abs_sum);
+    if (vx) lds_add(tile.data + local, ix);
+    if (vy) lds_add(tile.data + tile.stride + local, iy);
+    if (vz) lds_add(tile.data + 2 * tile.stride + local, iz);
+    if (!(vx && vy && vz)) {
+      ++tally.overflow;
+      return;
+    }
+  } else {
+    lds_add(tile.data + local, c.x);
+    lds_add(tile.data + tile.stride + local, c.y);
+    lds_add(tile.data + 2 * tile.stride + local, c.z);
+  }
+  ++tally.coreas;
+}
+
+// Adds `c` to the ZHS bins [first, last] (global indices, already known to be
+// inside [0, nbins)) through the difference array, clipped to the tile.
+// Returns the number of bins of this tile that received it.
+template <bool Fixed>
+__device__ __forceinline__ std::uint64_t add_potential_range(Obs const& o,
+                                                             Tile<Fixed> const& tile,
+                                                             std::uint64_t first,
+                                                             std::uint64_t last, V3 c,
+                                                             Tally& tally, bool& valid) {
+  std::uint64_t const lo = first > tile.begin ? first : tile.begin;
+  std::uint64_t const tile_end = tile.begin + tile.bins;  // exclusive
+  std::uint64_t const hi = last + 1 < tile_end ? last + 1 : tile_end;  // exclusive
+  valid = true;
+  if (lo >= hi) return 0;
+  std::uint32_t const a = static_cast<std::uint32_t>(lo - tile.begin);
+  std::uint32_t const b = static_cast<std::uint32_t>(hi - tile.begin);  // <= bins
+  if constexpr (Fixed) {
+    long long ix = 0, iy = 0, iz = 0;
+    bool const vx = fixed_increment(c.x, o.scale, ix, tally.abs_sum);
+    bool const vy = fixed_increment(c.y, o.scale, iy, tally.abs_sum);
+    bool const vz = fixed_increment(c.z, o.scale, iz, tally.abs_sum);
+    if (vx) {
+      lds_add(tile.data + a, ix);
+      lds_add(tile.data + b, -ix);
+    }
+    if (vy) {
+      lds_add(tile.data + tile.stride + a, iy);
+      lds_add(tile.data + tile.stride + b, -iy);
+    }
+    if (vz) {
+      lds_add(tile.data + 2 * tile.stride + a, iz);
+      lds_add(tile.data + 2 * tile.stride + b, -iz);
+    }
+    valid = vx && vy && vz;
+  } else {
+    lds_add(tile.data + a, c.x);
+    lds_add(tile.data + b, -c.x);
+    lds_add(tile.data + tile.stride + a, c.y);
+    lds_add(tile.data + tile.stride + b, -c.y);
+    lds_add(tile.data + 2 * tile.stride + a, c.z);
+    lds_add(tile.data + 2 * tile.stride + b, -c.z);
+  }
+  return hi - lo;
+}
+
+// ZHS sample (add_sample with IsVectorPotential = true): no time gate.
+template <bool Fixed>
+__device__ __forceinline__ void add_potential_sample(Obs const& o, Tile<Fixed> const& tile,
+                                                     double time, V3 c, Tally& tally) {
+  double const bin_value = nearest_bin((time - o.start) * o.rate);
+  if (!(bin_value >= 0. && bin_value < o.nbins_d)) return;
+  std::uint64_t const bin = static_cast<std::uint64_t>(bin_value);
+  if (bin >= o.nbins) return;
+  bool valid = true;
+  std::uint64_t const added = add_potential_range(o, tile, bin, bin, c, tally, valid);
+  if (added == 0) return;
+  if (!valid) {
+    ++tally.overflow;
+    return;
+  }
+  ++tally.zhs;
+}
+
+template <bool Fixed>
+__device__ void coreas_pair(Track const& t, PropagationDev const& p, Obs const& o,
+                            Tile<Fixed> const& tile, Tally& tally) {
+  Path const ps = propagate(p, t.start, o);
+  Path const pe = propagate(p, t.end, o);
+  double const pre = doppler(ps.n_source, t.beta, ps.emit);
+  double const post = doppler(pe.n_source, t.beta, pe.emit);
+  double start_receive = t.t0 + ps.time;
+  double end_receive = t.t1 + pe.time;
+  if (ps.n_destination > 1. &&
+      (fabs(pre) < kCoreasThreshold || fabs(post) < kCoreasThreshold)) {
+    V3 const midpoint = vmul(vadd(t.start, t.end), 0.5);
+    double const middle_time = (t.t0 + t.t1) * 0.5;
+    Path const pm = propagate(p, midpoint, o);
+    double const middle_receive = middle_time + pm.time;
+    double const md = doppler(pm.n_source, t.beta, pm.emit);
+    V3 field_start = vmul(vcross(pm.emit, vcross(pm.emit, t.beta)),
+                          t.constant * o.rate / (md * pm.distance));
+    V3 field_end = vmul(field_start, -1.);
+    double delta = t.length / (kC * t.beta_module) * fabs(md);
+    if (start_receive < end_receive) {
+      start_receive = middle_receive - 0.5 * delta;
+      end_receive = middle_receive + 0.5 * delta;
+    } else {
+      start_receive = middle_receive + 0.5 * delta;
+      end_receive = middle_receive - 0.5 * delta;
+    }
+    double const grid = o.grid;
+    delta = end_receive - start_receive;
+    if (fabs(delta) < grid) {
+      double const scale = fabs(delta / grid);
+      field_start = vmul(field_start, scale);
+      field_end = vmul(field_end, scale);
+      // separate_same_bin_endpoints, signed order (RadioProjection.inl:303-336)
+      long long const sb = static_cast<long long>(nearest_bin(start_receive / grid));
+      long long const eb = static_cast<long long>(nearest_bin(end_receive / grid));
+      if (sb == eb) {
+        double const sf = start_receive / grid - floor(start_receive / grid);
+        double const ef = end_receive / grid - floor(end_receive / grid);
+        bool const forward = end_receive >= start_receive;
+        if (forward) {
+          if (sf >= 0.5 && ef >= 0.5) start_receive -= grid;
+          else if (sf < 0.5 && ef < 0.5) end_receive += grid;
+          else if (ef >= 1. - sf) end_receive += grid;
+          else start_receive -= grid;
+        } else {
+          if (sf >= 0.5 && ef >= 0.5) end_receive -= grid;
+          else if (sf < 0.5 && ef < 0.5) start_receive += grid;
+          else if (sf >= 1. - ef) start_receive += grid;
+          else end_receive -= grid;
+        }
+      }
+    }
+    add_field_sample(o, tile, start_receive, field_start, tally);
+    add_field_sample(o, tile, end_receive, field_end, tally);
+    return;
+  }
+  V3 field_start = vmul(vcross(ps.emit, vcross(ps.emit, t.beta)),
+                        t.constant * o.rate / (pre * ps.distance));
+  V3 field_end = vmul(vcross(pe.emit, vcross(pe.emit, t.beta)),
+                      -t.constant * o.rate / (post * pe.distance));
+  if (pre < 1.e-9 || post < 1.e-9) {
+    double const grid = o.grid;
+    double const delta = end_receive - start_receive;
+    if (fabs(delta) < grid) {
+      double const scale = fabs(delta / grid);
+      field_start = vmul(field_start, scale);
+      field_end = vmul(field_end, scale);
+      // separate_same_bin_endpoints, unsigned order: always "forward"
+      long long const sb = static_cast<long long>(nearest_bin(start_receive / grid));
+      long long const eb = static_cast<long long>(nearest_bin(end_receive / grid));
+      if (sb == eb) {
+        double const sf = start_receive / grid - floor(start_receive / grid);
+        double const ef = end_receive / grid - floor(end_receive / grid);
+        if (sf >= 0.5 && ef >= 0.5) start_receive -= grid;
+        else if (sf < 0.5 && ef < 0.5) end_receive += grid;
+        else if (ef >= 1. - sf) end_receive += grid;
+        else start_receive -= grid;
+      }
+    }
+  }
+  add_field_sample(o, tile, start_receive, field_start, tally);
+  add_field_sample(o, tile, end_receive, field_end, tally);
+}
+
+template <bool Fixed>
+__device__ void zhs_segment(V3 p1, V3 p2, double time1, double time2, Track const& t,
+                            PropagationDev const& p, Obs const& o, Tile<Fixed> const& tile,
+                            Tally& tally, bool subdivided) {
+  V3 const midpoint = vmul(vadd(p1, p2), 0.5);
+  Path const path = propagate(p, midpoint, o);
+  double const n_source = path.n_source;
+  double const bk = vdot(t.beta, path.emit);
+  double const middle_time = (time1 + time2) * 0.5;
+  double d1 = time1 + path.time - n_source * bk * (time1 - middle_time);
+  double d2 = time2 + path.time - n_source * bk * (time2 - middle_time);
+  double sign = 1.;
+  if (d1 > d2) {
+    d1 = time2 + path.time - n_source * bk * (time2 - middle_time);
+    d2 = time1 + path.time - n_source * bk * (time1 - middle_time);
+    sign = -1.;
+  }
+  double const start_bin = floor((d1 - o.start) * o.rate + 0.5);
+  double const end_bin = floor((d2 - o.start) * o.rate + 0.5);
+  V3 const bperp = vcross(path.emit, vcross(t.beta, path.emit));
+  double const den = 1. - n_source * bk;
+  if (start_bin == end_bin) {
+    V3 potential;
+    if (fabs(den) > 1.e-15) {
+      double const fraction = fabs(d2 * o.rate - d1 * o.rate);
+      potential = vmul(bperp, sign * t.constant * fraction / (den * path.distance));
+    } else {
+      double const fraction = (time2 - time1) * o.rate;
+      potential = vmul(bperp, sign * t.constant * fraction / path.distance);
+    }
+    add_potential_sample(o, tile, d2, potential, tally);
+    return;
+  }
+  int const nb = static_cast<int>(end_bin - start_bin);
+  double fraction = fabs(start_bin + 0.5 - (d1 - o.start) * o.rate);
+  V3 potential = vmul(bperp, sign * t.constant * fraction / (den * path.distance));
+  add_potential_sample(o, tile, d1, potential, tally);
+  if (nb > 1) {
+    V3 const middle =
+        vmul(bperp, (subdivided ? 1. : sign) * t.constant / (den * path.distance));
+    // Bins of the first and last middle sample, exactly as the per-bin loop
+    // computes them; in between the bins are consecutive.
+    double const b_first = nearest_bin((d1 + 1. / o.rate - o.start) * o.rate);
+    double const b_last =
+        nearest_bin((d1 + static_cast<double>(nb - 1) / o.rate - o.start) * o.rate);
+    if (b_last - b_first == static_cast<double>(nb - 2)) {
+      double const lo = b_first < 0. ? 0. : b_first;
+      double const hi = b_last < o.nbins_d - 1. ? b_last : o.nbins_d - 1.;
+      if (lo <= hi) {
+        bool valid = true;
+        std::uint64_t const added = add_potential_range(
+            o, tile, static_cast<std::uint64_t>(lo), static_cast<std::uint64_t>(hi), middle,
+            tally, valid);
+        if (added != 0) {
+          if (valid) tally.zhs += added;
+          else tally.overflow += added;
+        }
+      }
+    } else {
+      for (int index = 1; index < nb; ++index)
+        add_potential_sample(o, tile, d1 + static_cast<double>(index) / o.rate, middle, tally);
+    }
+  }
+  fraction = fabs((d2 - o.start) * o.rate + 0.5 - end_bin);
+  potential = vmul(bperp, sign * t.constant * fraction / (den * path.distance));
+  add_potential_sample(o, tile, d2, potential, tally);
+}
+
+template <bool Fixed>
+__device__ void zhs_pair(Track const& t, PropagationDev const& p, std::uint32_t refinement_u,
+                         Obs const& o, Tile<Fixed> const& tile, Tally& tally,
+                         bool count_subtracks) {
+  V3 const midpoint = vmul(vadd(t.start, t.end), 0.5);
+  Path const mp = propagate(p, midpoint, o);
+  double const u_k = vdot(t.beta, mp.emit) / t.beta_module;
+  double const sin2 = 1. - u_k * u_k;
+  double const fraunhofer = sin2 * t.length * t.length / mp.distance / o.wavelength * 2. * kPi;
+  double divisor = 1.;
+  int nsub = 1;
+  if (fraunhofer > 1.) {
+    divisor = sqrt(fraunhofer) + 1.;
+    nsub = static_cast<int>(ceil(divisor));
+  }
+  int const refinement = static_cast<int>(refinement_u);
+  if (count_subtracks) {
+    tally.subtracks += static_cast<unsigned long long>(nsub);
+    if (refinement > 1)
+      tally.subtracks += static_cast<unsigned long long>(nsub * (refinement - 1));
+  }
+  V3 const spatial_step = vdiv(t.disp, divisor);
+  double const time_step = t.duration / divisor;
+  V3 const rs = vdiv(spatial_step, static_cast<double>(refinement));
+  double const rt = time_step / static_cast<double>(refinement);
+  bool const subdivided = nsub > 1 || refinement > 1;
+  if (!subdivided) {
+    zhs_segment(t.start, vadd(t.start, rs), t.t0, t.t0 + rt, t, p, o, tile, tally, false);
+    return;
+  }
+  V3 p1 = t.start;
+  double time1 = t.t0;
+  for (int i = 0; i < nsub; ++i) {
+    for (int r = 0; r < refinement; ++r) {
+      V3 const p2 = vadd(p1, rs);
+      double const time2 = time1 + rt;
+      zhs_segment(p1, p2, time1, time2, t, p, o, tile, tally, true);
+      p1 = p2;
+      time1 = time2;
+    }
+  }
+}
+
+__device__ __forceinline__ Obs make_obs(ObserverDev const& d, PropagationDev const& p) {
+  Obs o;
+  o.position = {d.position[0], d.position[1], d.position[2]};
+  o.start = d.start_time;
+  o.duration = d.duration;
+  o.rate = d.sample_rate;
+  o.scale = d.fixed_scale;
+  o.grid = 1. / d.sample_rate;
+  o.wavelength = kC / d.sample_rate;
+  o.nbins = d.number_of_bins;
+  o.nbins_d = static_cast<double>(d.number_of_bins);
+  o.offset = d.waveform_offset;
+  if (p.homogeneous_index > 0.) {
+    o.destination_height = 0.;
+    o.destination_index = 0;
+    o.n_destination_table = 1.;
+    o.integrated_destination = 0.;
+  } else {
+    o.destination_height = (o.position.z - p.minimum_height) * p.inverse_step;
+    o.destination_index = static_cast<std::uint64_t>(o.destination_height + 0.5);
+    o.n_destination_table = p.refractivity[o.destination_index] + 1.;
+    o.integrated_destination = p.integrated[o.destination_index];
+  }
+  return o;
+}
+
+// Inclusive prefix sum over `count` entries of an LDS row, whole block.
+template <class T>
+__device__ void block_inclusive_scan(T* row, std::uint32_t count, T* scratch) {
+  std::uint32_t const tid = threadIdx.x;
+  std::uint32_t const threads = blockDim.x;
+  std::uint32_t const per = (count + threads - 1) / threads;
+  std::uint32_t const begin = tid * per < count ? tid * per : count;
+  std::uint32_t const end = begin + per < count ? begin + per : count;
+  T running = 0;
+  for (std::uint32_t i = begin; i < end; ++i) {
+    running += row[i];
+    row[i] = running;
+  }
+  scratch[tid] = running;
+  __syncthreads();
+  if (tid == 0) {
+    T acc = 0;
+    for (std::uint32_t i = 0; i < threads; ++i) {
+      T const v = scratch[i];
+      scratch[i] = acc;
+      acc += v;
+    }
+  }
+  __syncthreads();
+  T const offset = scratch[tid];
+  for (std::uint32_t i = begin; i < end; ++i) row[i] += offset;
+  __syncthreads();
+}
+
+constexpr std::uint32_t kMaxBlock = 1024;
+
+template <bool Coreas, bool Zhs, bool Fixed>
+__global__ void __launch_bounds__(256) radio_project_kernel(ProjectionLaunch L) {
+  using Acc = std::conditional_t<Fixed, long long, double>;
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  __shared__ Acc scan_scratch[kMaxBlock];
+  __shared__ unsigned long long block_counts[5];
+  __shared__ double block_abs[2];
+
+  // XCD-aware decode of blockIdx.x into (chunk, observer, window tile).
+  std::uint32_t const linear = blockIdx.x;
+  std::uint32_t const xcd = linear % kXcds;
+  std::uint32_t const sequence = linear / kXcds;
+  std::uint32_t const per_chunk = L.observer_count * L.bin_tiles;
+  std::uint32_t const chunk = (sequence / per_chunk) * kXcds + xcd;
+  if (chunk >= L.chunks) return;
+  std::uint32_t const within = sequence % per_chunk;
+  std::uint32_t const observer = within / L.bin_tiles;
+  std::uint32_t const bin_tile = within % L.bin_tiles;
+
+  std::uint32_t const W = L.tile_bins;
+  Acc* const coreas_lds = reinterpret_cast<Acc*>(lds_raw);
+  Acc* const zhs_lds = coreas_lds + (Coreas ? 3 * W : 0);
+  std::uint32_t const lds_count = (Coreas ? 3 * W : 0) + (Zhs ? 3 * (W + 1) : 0);
+  for (std::uint32_t i = threadIdx.x; i < lds_count; i += blockDim.x) coreas_lds[i] = 0;
+  if (threadIdx.x < 5) block_counts[threadIdx.x] = 0;
+  if (threadIdx.x < 2) block_abs[threadIdx.x] = 0.;
+  __syncthreads();
+
+  PropagationDev const prop = L.propagation;
+  std::uint64_t const tile_begin = static_cast<std::uint64_t>(bin_tile) * W;
+  Obs co{}, zo{};
+  Tile<Fixed> ctile{coreas_lds, W, tile_begin, 0};
+  Tile<Fixed> ztile{zhs_lds, W + 1, tile_begin, 0};
+  if constexpr (Coreas) {
+    co = make_obs(L.coreas_observers[observer], prop);
+    ctile.bins = tile_begin < co.nbins
+                     ? static_cast<std::uint32_t>(co.nbins - tile_begin < W ? co.nbins - tile_begin : W)
+                     : 0u;
+  }
+  if constexpr (Zhs) {
+    zo = make_obs(L.zhs_observers[observer], prop);
+    ztile.bins = tile_begin < zo.nbins
+                     ? static_cast<std::uint32_t>(zo.nbins - tile_begin < W ? zo.nbins - tile_begin : W)
+                     : 0u;
+  }
+  bool const first_observer_tile = L.count_valid && observer == 0 && bin_tile == 0;
+
+  std::uint64_t const chunk_begin = static_cast<std::uint64_t>(chunk) * L.chunk_size;
+  std::uint64_t const chunk_end =
+      chunk_begin + L.chunk_size < L.track_count ? chunk_begin + L.chunk_size : L.track_count;
+  Tally ctally, ztally;
+  for (std::uint64_t i = chunk_begin + threadIdx.x; i < chunk_end; i += blockDim.x) {
+    Track t;
+    if (!load_track(L.tracks, L.stride, i, t)) continue;
+    if (first_observer_tile) ++ctally.valid;
+    if constexpr (Coreas) {
+      if (ctile.bins) coreas_pair(t, prop, co, ctile, ctally);
+    }
+    if constexpr (Zhs) {
+      zhs_pair(t, prop, L.zhs_refinement, zo, ztile, ztally, bin_tile == 0);
+    }
+  }
+
+  atomicAdd(&block_counts[0], ctally.coreas);
+  atomicAdd(&block_counts[1], ztally.zhs);
+  atomicAdd(&block_counts[2], ztally.subtracks);
+  atomicAdd(&block_counts[3], ctally.valid);
+  atomicAdd(&block_counts[4], ctally.overflow + ztally.overflow);
+  if constexpr (Fixed) {
+    atomicAdd(&block_abs[0], ctally.abs_sum);
+    atomicAdd(&block_abs[1], ztally.abs_sum);
+  }
+  __syncthreads();
+
+  if constexpr (Zhs) {
+    for (int c = 0; c < 3; ++c) block_inclusive_scan(zhs_lds + c * (W + 1), W, scan_scratch);
+  }
+
+  // Flush the tile into the global waveforms (checked int64 atomics, as
+  // checked_radio_atomic_add; or FP64 atomics in floating mode).
+  unsigned long long flush_overflows = 0;
+  auto flush = [&](Acc const* lds, std::uint32_t row, Obs const& o, std::uint32_t bins,
+                   void* global, std::uint64_t total_bins) {
+    Acc* const g = static_cast<Acc*>(global);
+    for (std::uint32_t idx = threadIdx.x; idx < 3 * bins; idx += blockDim.x) {
+      std::uint32_t const c = idx / bins;
+      std::uint32_t const b = idx % bins;
+      Acc const v = lds[c * row + b];
+      if (v == 0) continue;
+      std::uint64_t const at = c * total_bins + o.offset + tile_begin + b;
+      if constexpr (Fixed) {
+        long long const previous = static_cast<long long>(
+            atomicAdd(reinterpret_cast<unsigned long long*>(g + at),
+                      static_cast<unsigned long long>(v)));
+        if ((v > 0 && previous > 0x7fffffffffffffffLL - v) ||
+            (v < 0 && previous < (-0x7fffffffffffffffLL - 1) - v))
+          ++flush_overflows;
+      } else {
+        atomicAdd(g + at, v);
+      }
+    }
+  };
+  if constexpr (Coreas) flush(coreas_lds, W, co, ctile.bins, L.coreas_waves, L.coreas_total_bins);
+  if constexpr (Zhs) flush(zhs_lds, W + 1, zo, ztile.bins, L.zhs_waves, L.zhs_total_bins);
+  if (flush_overflows) atomicAdd(&block_counts[4], flush_overflows);
+  __syncthreads();
+
+  if (threadIdx.x == 0) {
+    unsigned long long overflow = block_counts[4];
+    // Integer sums are exact modulo 2^64; every LDS partial is bounded by the
+    // sum of |increments|, so a block whose bound reaches 2^63 may have lost
+    // its result and is reported as an overflow (conservatively).
+    if constexpr (Fixed) {
+      if (block_abs[0] >= kIntLimit || block_abs[1] >= kIntLimit) ++overflow;
+    }
+    if (block_counts[0]) atomicAdd(&L.counters->coreas_contributions, block_counts[0]);
+    if (block_counts[1]) atomicAdd(&L.counters->zhs_contributions, block_counts[1]);
+    if (block_counts[2]) atomicAdd(&L.counters->zhs_subtracks, block_counts[2]);
+    if (block_counts[3]) atomicAdd(&L.counters->valid_tracks, block_counts[3]);
+    if (overflow) atomicAdd(&L.counters->fixed_point_overflows, overflow);
+  }
+}
+
+// LeptonTransportRecord -> compact track (make_radio_track_kinematics input
+// selection, RadioProjection.inl:145-168). Non-e+- records become zero-length
+// tracks, which the projection rejects exactly as the reference does.
+__global__ void __launch_bounds__(256)
+    records_to_tracks_kernel(c8mi_lepton_transport_record const* __restrict__ records,
+                             std::uint64_t count, double* __restrict__ tracks,
+                             std::uint64_t stride) {
+  std::uint64_t const i = static_cast<std::uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= count) return;
+  c8mi_lepton_transport_record const& r = records[i];
+  int const pid = r.start.pid;
+  bool const lepton = pid == C8MI_PID_ELECTRON || pid == C8MI_PID_POSITRON;
+  double const w = r.start.weight;
+  tracks[i] = lepton ? r.start.position_m[0] : 0.;
+  tracks[stride + i] = lepton ? r.start.position_m[1] : 0.;
+  tracks[2 * stride + i] = lepton ? r.start.position_m[2] : 0.;
+  tracks[3 * stride + i] = lepton ? r.end.position_m[0] : 0.;
+  tracks[4 * stride + i] = lepton ? r.end.position_m[1] : 0.;
+  tracks[5 * stride + i] = lepton ? r.end.position_m[2] : 0.;
+  tracks[6 * stride + i] = lepton ? r.start.time_s : 0.;
+  tracks[7 * stride + i] = lepton ? r.end.time_s : 0.;
+  tracks[8 * stride + i] = pid == C8MI_PID_ELECTRON ? -w : w;
+}
+
+template <bool Coreas, bool Zhs, bool Fixed>
+void launch_typed(ProjectionLaunch const& launch, hipStream_t stream, std::size_t lds_bytes,
+                  std::uint32_t grid) {
+  auto kernel = radio_project_kernel<Coreas, Zhs, Fixed>;
+  static bool attribute_set = false;
+  if (!attribute_set) {
+    C8MI_HIP(hipFuncSetAttribute(reinterpret_cast<void const*>(kernel),
+                                 hipFuncAttributeMaxDynamicSharedMemorySize,
+                                 static_cast<int>(kMaxDynamicLdsBytes)));
+    attribute_set = true;
+  }
+  hipLaunchKernelGGL(kernel, dim3(grid), dim3(kProjectionBlock), lds_bytes, stream, launch);
+  C8MI_HIP(hipGetLastError());
+}
+
+}  // namespace
+
+std::size_t projection_lds_bytes(bool coreas, bool zhs, std::uint32_t tile_bins) {
+  std::size_t const words = (coreas ? 3u * tile_bins : 0u) + (zhs ? 3u * (tile_bins + 1u) : 0u);
+  return words * sizeof(long long);
+}
+
+std::uint32_t projection_grid(ProjectionLaunch const& launch) {
+  std::uint64_t const per_chunk =
+      static_cast<std::uint64_t>(launch.observer_count) * launch.bin_tiles;
+  std::uint64_t const chunk_rounds = (launch.chunks + kXcds - 1) / kXcds;
+  return static_cast<std::uint32_t>(chunk_rounds * kXcds * per_chunk);
+}
+
+void launch_projection(ProjectionLaunch const& launch, bool coreas, bool zhs, bool fixed,
+                       hipStream_t stream) {
+  if (launch.track_count == 0 || launch.observer_count == 0) return;
+  std::size_t const lds = projection_lds_bytes(coreas, zhs, launch.tile_bins);
+  std::uint32_t const grid = projection_grid(launch);
+  if (coreas && zhs) {
+    if (fixed) launch_typed<true, true, true>(launch, stream, lds, grid);
+    else launch_typed<true, true, false>(launch, stream, lds, grid);
+  } else if (coreas) {
+    if (fixed) launch_typed<true, false, true>(launch, stream, lds, grid);
+    else launch_typed<true, false, false>(launch, stream, lds, grid);
+  } else if (zhs) {
+    if (fixed) launch_typed<false, true, true>(launch, stream, lds, grid);
+    else launch_typed<false, true, false>(launch, stream, lds, grid);
+  }
+}
+
+void launch_records_to_tracks(c8mi_lepton_transport_record const* records, std::uint64_t count,
+                              double* tracks, std::uint64_t stride, hipStream_t stream) {
+  if (count == 0) return;
+  std::uint32_t const blocks = static_cast<std::uint32_t>((count + 255) / 256);
+  hipLaunchKernelGGL(records_to_tracks_kernel, dim3(blocks), dim3(256), 0, stream, records,
+                     count, tracks, stride);
+  C8MI_HIP(hipGetLastError());
+}
+
+}  // namespace c8mi::radio

@@ -1,0 +1,31 @@
+// Host-callable launchers for the radio kernels (defined in radio_kernels.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+
+#include "c8mi/c8mi_types.h"
+#include "radio_device.hpp"
+
+namespace c8mi::radio {
+
+// Threads per projection workgroup (one observer x one track chunk).
+inline constexpr std::uint32_t kProjectionBlock = 256;
+// Dynamic LDS a projection workgroup may request (gfx950: 160 KiB per CU).
+inline constexpr std::size_t kMaxDynamicLdsBytes = 144 * 1024;
+
+// LDS bytes for one workgroup holding `tile_bins` bins of each enabled mode.
+std::size_t projection_lds_bytes(bool coreas, bool zhs, std::uint32_t tile_bins);
+// Workgroups launched for `launch` (chunks rounded up to whole XCD rounds).
+std::uint32_t projection_grid(ProjectionLaunch const& launch);
+
+void launch_projection(ProjectionLaunch const& launch, bool coreas, bool zhs, bool fixed,
+                       hipStream_t stream);
+
+void launch_records_to_tracks(c8mi_lepton_transport_record const* records,
+                              std::uint64_t count, double* tracks, std::uint64_t stride,
+                              hipStream_t stream);
+
+}  // namespace c8mi::radio

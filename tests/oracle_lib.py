@@ -1,0 +1,122 @@
+"""ctypes access to the oracle (oracle/liboracle.so) — test infrastructure.
+
+The oracle is the checker: only tests, __graft_entry__.smoke() and bench.py's
+cpu_baseline leg use it. It is built from oracle/*.c by oracle/Makefile.
+"""
+import ctypes
+import os
+import subprocess
+from ctypes import POINTER, byref, c_double, c_int, c_int64, c_uint32, c_uint64, c_void_p
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE_DIR = os.path.join(ROOT, "oracle")
+ORACLE_PATH = os.path.join(ORACLE_DIR, "liboracle.so")
+
+_LIB = None
+
+
+class OrRadioCounters(ctypes.Structure):
+    _fields_ = [
+        ("coreas_contributions", c_uint64),
+        ("zhs_contributions", c_uint64),
+        ("zhs_subtracks", c_uint64),
+        ("valid_tracks", c_uint64),
+        ("fixed_point_overflows", c_uint64),
+    ]
+
+    def as_dict(self):
+        return {name: int(getattr(self, name)) for name, _ in self._fields_}
+
+
+def load():
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(ORACLE_PATH):
+            subprocess.run(["make", "-C", ORACLE_DIR], check=True, capture_output=True)
+        lib = ctypes.CDLL(ORACLE_PATH)
+        lib.oracle_radio_nearest_bin.restype = c_double
+        lib.oracle_radio_nearest_bin.argtypes = [c_double]
+        lib.oracle_radio_doppler.restype = c_double
+        lib.oracle_radio_doppler.argtypes = [c_double, POINTER(c_double), POINTER(c_double), c_int]
+        _LIB = lib
+    return _LIB
+
+
+def _d(a):
+    return a.ctypes.data_as(POINTER(c_double))
+
+
+def nearest_bin(x):
+    return load().oracle_radio_nearest_bin(x)
+
+
+def doppler(index, beta, emit, device_variant=False):
+    b = np.ascontiguousarray(beta, dtype=np.float64)
+    e = np.ascontiguousarray(emit, dtype=np.float64)
+    return load().oracle_radio_doppler(index, _d(b), _d(e), 1 if device_variant else 0)
+
+
+def propagate(propagation, source, destination):
+    """propagation: c8mi.Propagation; returns (time, n_src, n_dst, distance, emit[3])."""
+    out = np.zeros(7)
+    s = np.ascontiguousarray(source, dtype=np.float64)
+    d = np.ascontiguousarray(destination, dtype=np.float64)
+    load().oracle_radio_propagate(byref(propagation.struct), _d(s), _d(d), _d(out))
+    return out[0], out[1], out[2], out[3], out[4:7].copy()
+
+
+def separate_same_bin(start, end, grid, signed_order):
+    s, e = c_double(start), c_double(end)
+    load().oracle_radio_separate_same_bin(byref(s), byref(e), c_double(grid), c_int(1 if signed_order else 0))
+    return s.value, e.value
+
+
+def build_flat_table(refractivity, inverse_step):
+    r = np.ascontiguousarray(refractivity, dtype=np.float64)
+    integrated = np.zeros_like(r)
+    slopes = np.zeros(4)
+    load().oracle_radio_build_flat_table(_d(r), c_uint64(r.size), c_double(inverse_step), _d(integrated), _d(slopes))
+    return integrated, slopes
+
+
+def radio_project(propagation, observers, tracks, coreas=True, zhs=True, refinement=1,
+                  limit=1.0, fixed=True, floating=True, threads=1):
+    """Runs the FP64 restatement. Returns dict with coreas/zhs fixed [3][total],
+    floating [3][total] and counters."""
+    obs = np.ascontiguousarray(observers)
+    tracks = np.ascontiguousarray(tracks, dtype=np.float64)
+    total = int(obs["number_of_bins"].sum())
+    out = {}
+    cf = np.zeros((3, total), dtype=np.int64) if (coreas and fixed) else None
+    cd = np.zeros((3, total)) if (coreas and floating) else None
+    zf = np.zeros((3, total), dtype=np.int64) if (zhs and fixed) else None
+    zd = np.zeros((3, total)) if (zhs and floating) else None
+    counters = OrRadioCounters()
+
+    def p64(a):
+        return None if a is None else a.ctypes.data_as(POINTER(c_int64))
+
+    def pd(a):
+        return None if a is None else _d(a)
+
+    status = load().oracle_radio_project(
+        byref(propagation.struct), c_uint32(refinement), c_double(limit),
+        obs.ctypes.data_as(c_void_p), c_uint64(obs.size), c_int(1 if coreas else 0),
+        c_int(1 if zhs else 0), _d(tracks), c_uint64(tracks.shape[1]), c_uint64(tracks.shape[1]),
+        p64(cf), pd(cd), p64(zf), pd(zd), byref(counters), c_int(threads))
+    assert status == 0
+    out.update(coreas_fixed=cf, coreas_float=cd, zhs_fixed=zf, zhs_float=zd, counters=counters.as_dict())
+    return out
+
+
+def radio_decode(fixed, observers, limit=1.0, potential=False):
+    obs = np.ascontiguousarray(observers)
+    fixed = np.ascontiguousarray(fixed, dtype=np.int64)
+    total = fixed.shape[1]
+    out = np.zeros((3, total))
+    load().oracle_radio_decode(fixed.ctypes.data_as(POINTER(c_int64)), c_uint64(total),
+                               obs.ctypes.data_as(c_void_p), c_uint64(obs.size), c_double(limit),
+                               c_int(1 if potential else 0), _d(out))
+    return out

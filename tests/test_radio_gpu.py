@@ -1,0 +1,253 @@
+"""GPU parity of the HIP radio projection against the FP64 oracle.
+
+Waveforms are compared as the int64 fixed-point sums the reference accumulates
+by default (RadioConfig.hpp:30, RadioProjection.inl:245-301): the HIP kernels
+keep the reference's FP64 operation order, so the bar is bit equality of every
+bin and equality of the contribution counters. Floating mode (FP64 atomics,
+order-dependent rounding) is compared with a tolerance of 1e-10 of the peak.
+At the full 2^20-track size the checks are size-independent: superposition,
+permutation invariance, and the field error against the FP64 oracle on a
+subset of antennas (1e-3 relative).
+"""
+import numpy as np
+import pytest
+
+import c8mi
+import oracle_lib
+from c8mi import workloads
+from radio_cases import (all_reference_cases, random_tracks, reference_case, uniform_table,
+                         window_observers)
+
+pytestmark = pytest.mark.gpu
+
+
+def gpu_project(tracks, observers, propagation, coreas=True, zhs=True, refinement=1,
+                deterministic=True, tuning=None, zhs_observers=None, limit=1.0):
+    acc = c8mi.RadioAccumulator(observers, propagation, coreas=coreas, zhs=zhs,
+                                deterministic=deterministic, zhs_refinement=refinement,
+                                zhs_observers=zhs_observers, fixed_point_limit=limit)
+    if tuning:
+        acc.set_tuning(**tuning)
+    n = tracks.shape[1]
+    dev = c8mi.DeviceArray.from_host(np.ascontiguousarray(tracks))
+    acc.accumulate_tracks(dev, n, n)
+    if deterministic:
+        out = acc.download_fixed()
+    else:
+        out = acc.download()
+    dev.free()
+    acc.close()
+    return out
+
+
+def assert_same(gpu, ref, coreas=True, zhs=True):
+    c, z, counters = gpu
+    if coreas:
+        assert np.array_equal(c, ref["coreas_fixed"])
+        assert counters["coreas_contributions"] == ref["counters"]["coreas_contributions"]
+    if zhs:
+        assert np.array_equal(z, ref["zhs_fixed"])
+        assert counters["zhs_contributions"] == ref["counters"]["zhs_contributions"]
+        assert counters["zhs_subtracks"] == ref["counters"]["zhs_subtracks"]
+    assert counters["valid_tracks"] == ref["counters"]["valid_tracks"]
+    assert counters["fixed_point_overflows"] == ref["counters"]["fixed_point_overflows"] == 0
+
+
+@pytest.mark.parametrize("name,step", all_reference_cases())
+def test_reference_cases_bit_exact(name, step):
+    print(f"case {name} step {step}", flush=True)
+    track, obs, propagation = reference_case(name, step)
+    ref = oracle_lib.radio_project(propagation, obs, track)
+    assert_same(gpu_project(track, obs, propagation), ref)
+
+
+POSITIONS = ((0., 0., 0.), (150., -40., 0.), (-300., 250., 0.), (30., 30., 0.),
+             (500., 0., 0.), (-80., -420., 0.))
+
+
+def exponential_table(max_h=2500.0):
+    refr = workloads.exponential_refractivity(-1000.0, max_h, 1.0)
+    return c8mi.Propagation(refractivity=refr, minimum_height_m=-1000.0, maximum_height_m=max_h,
+                            step_m=1.0)
+
+
+@pytest.mark.parametrize("rate,nbins", [(5.0e9, 2048), (1.0e9, 1024), (1.0e10, 4096)])
+def test_random_tracks_bit_exact(rate, nbins):
+    print(f"random tracks rate {rate} bins {nbins}", flush=True)
+    tracks = random_tracks(4096, 5)
+    obs = window_observers(tracks, POSITIONS, rate, nbins)
+    prop = exponential_table()
+    ref = oracle_lib.radio_project(prop, obs, tracks, threads=8)
+    assert_same(gpu_project(tracks, obs, prop), ref)
+
+
+def test_refined_zhs_bit_exact():
+    tracks = random_tracks(1024, 9, height=(20.0, 200.0))
+    obs = window_observers(tracks, POSITIONS[:3], 5.0e9, 2048)
+    prop = exponential_table()
+    ref = oracle_lib.radio_project(prop, obs, tracks, coreas=False, refinement=3)
+    assert ref["counters"]["zhs_subtracks"] > 3 * 1024
+    assert_same(gpu_project(tracks, obs, prop, coreas=False, refinement=3), ref, coreas=False)
+
+
+@pytest.fixture(scope="module")
+def config2_small():
+    tracks, obs, refr, (lo, hi) = workloads.config2_setup(count=8192, seed=77)
+    prop = c8mi.Propagation(refractivity=refr, minimum_height_m=lo, maximum_height_m=hi, step_m=1.0)
+    return tracks, obs, prop
+
+
+def test_config2_sample_bit_exact(config2_small):
+    tracks, obs, prop = config2_small
+    print("config2 sample 8192 x 160", flush=True)
+    ref = oracle_lib.radio_project(prop, obs, tracks, threads=8)
+    assert ref["counters"]["coreas_contributions"] > 0
+    assert_same(gpu_project(tracks, obs, prop), ref)
+
+
+def test_tiling_and_chunking_do_not_change_results(config2_small):
+    tracks, obs, prop = config2_small
+    base = gpu_project(tracks, obs, prop)
+    for tuning in (dict(tile_bins=300), dict(chunk_tracks=256), dict(target_workgroups=64),
+                   dict(tile_bins=1000, chunk_tracks=1024)):
+        print(f"tuning {tuning}", flush=True)
+        other = gpu_project(tracks, obs, prop, tuning=tuning)
+        assert np.array_equal(other[0], base[0]) and np.array_equal(other[1], base[1])
+        assert other[2] == base[2]
+
+
+def test_single_algorithm_launches_match_fused(config2_small):
+    tracks, obs, prop = config2_small
+    fused = gpu_project(tracks, obs, prop)
+    coreas_only = gpu_project(tracks, obs, prop, zhs=False)
+    zhs_only = gpu_project(tracks, obs, prop, coreas=False)
+    assert np.array_equal(coreas_only[0], fused[0])
+    assert np.array_equal(zhs_only[1], fused[1])
+
+
+def test_unfused_observer_lists(config2_small):
+    tracks, obs, prop = config2_small
+    zobs = obs[:40].copy()
+    acc = c8mi.RadioAccumulator(obs, prop, zhs_observers=zobs)
+    assert not acc.fused
+    acc.close()
+    gpu = gpu_project(tracks, obs, prop, zhs_observers=zobs)
+    ref_c = oracle_lib.radio_project(prop, obs, tracks, zhs=False, threads=8)
+    ref_z = oracle_lib.radio_project(prop, zobs, tracks, coreas=False, threads=8)
+    assert np.array_equal(gpu[0], ref_c["coreas_fixed"])
+    assert np.array_equal(gpu[1], ref_z["zhs_fixed"])
+    assert gpu[2]["valid_tracks"] == tracks.shape[1]
+
+
+def test_floating_mode_matches_oracle(config2_small):
+    tracks, obs, prop = config2_small
+    ref = oracle_lib.radio_project(prop, obs, tracks, threads=8)
+    c, z, counters = gpu_project(tracks, obs, prop, deterministic=False)
+    for got, want in ((c, ref["coreas_float"]), (z, ref["zhs_float"])):
+        peak = np.abs(want).max()
+        assert peak > 0
+        assert np.abs(got - want).max() <= 1e-10 * peak
+
+
+def test_lepton_records_paths_match_compact_tracks():
+    tracks = random_tracks(10000, 21)
+    obs = window_observers(tracks, POSITIONS[:4], 5.0e9, 2048)
+    prop = exponential_table()
+    records = workloads.tracks_to_records(tracks)
+    # photons and muons are not radio tracks (make_radio_track_kinematics)
+    records["start"]["pid"][::97] = 22
+    records["start"]["pid"][5::113] = 13
+    keep = np.ones(tracks.shape[1], dtype=bool)
+    keep[::97] = False
+    keep[5::113] = False
+    ref = oracle_lib.radio_project(prop, obs, tracks[:, keep], threads=8)
+    # host enqueue in uneven pieces (batches of 4096 inside)
+    acc = c8mi.RadioAccumulator(obs, prop)
+    for lo, hi in ((0, 1000), (1000, 5000), (5000, 5001), (5001, 10000)):
+        acc.enqueue_host_tracks(records[lo:hi])
+    assert_same(acc.download_fixed(), ref)
+    acc.close()
+    # device-resident records
+    acc = c8mi.RadioAccumulator(obs, prop)
+    dev = c8mi.DeviceArray.from_host(records)
+    acc.accumulate_records(dev, records.size)
+    assert_same(acc.download_fixed(), ref)
+    acc.close()
+
+
+def test_empty_input_and_lifecycle():
+    tracks = random_tracks(512, 2)
+    obs = window_observers(tracks, POSITIONS[:2], 5.0e9, 512)
+    prop = exponential_table()
+    acc = c8mi.RadioAccumulator(obs, prop)
+    acc.accumulate_tracks(0, 0, 0)
+    c, z, counters = acc.download()
+    assert not c.any() and not z.any() and counters["valid_tracks"] == 0
+    with pytest.raises(c8mi.C8miError) as err:
+        acc.download()
+    assert err.value.status == 2
+    acc.reset()
+    dev = c8mi.DeviceArray.from_host(tracks)
+    acc.accumulate_tracks(dev, 512, 512)
+    first = acc.download_fixed()
+    acc.reset()
+    acc.accumulate_tracks(dev, 512, 512)
+    second = acc.download_fixed()
+    assert np.array_equal(first[0], second[0]) and np.array_equal(first[1], second[1])
+    acc.close()
+
+
+def test_fixed_point_overflow_is_reported():
+    tracks = random_tracks(256, 4)
+    obs = window_observers(tracks, POSITIONS[:2], 5.0e9, 1024)
+    prop = exponential_table()
+    acc = c8mi.RadioAccumulator(obs, prop, fixed_point_limit=1e-40)
+    dev = c8mi.DeviceArray.from_host(tracks)
+    acc.accumulate_tracks(dev, 256, 256)
+    with pytest.raises(c8mi.C8miError) as err:
+        acc.download()
+    assert err.value.status == 6
+    acc.close()
+
+
+@pytest.fixture(scope="module")
+def config2_full():
+    tracks, obs, refr, (lo, hi) = workloads.config2_setup()
+    prop = c8mi.Propagation(refractivity=refr, minimum_height_m=lo, maximum_height_m=hi, step_m=1.0)
+    return tracks, obs, prop
+
+
+def test_full_size_superposition_and_permutation(config2_full):
+    tracks, obs, prop = config2_full
+    n = tracks.shape[1]
+    print(f"full config2 {n} x {obs.size}", flush=True)
+    whole = gpu_project(tracks, obs, prop)
+    acc = c8mi.RadioAccumulator(obs, prop)
+    dev = c8mi.DeviceArray.from_host(tracks)
+    half = n // 2
+    first = c8mi.DeviceArray.from_host(np.ascontiguousarray(tracks[:, :half]))
+    second = c8mi.DeviceArray.from_host(np.ascontiguousarray(tracks[:, half:]))
+    acc.accumulate_tracks(first, half, half)
+    acc.accumulate_tracks(second, n - half, n - half)
+    split = acc.download_fixed()
+    acc.close()
+    assert np.array_equal(split[0], whole[0]) and np.array_equal(split[1], whole[1])
+    perm = np.random.default_rng(1).permutation(n)
+    shuffled = gpu_project(np.ascontiguousarray(tracks[:, perm]), obs, prop)
+    assert np.array_equal(shuffled[0], whole[0]) and np.array_equal(shuffled[1], whole[1])
+    assert whole[2]["valid_tracks"] == n
+    del dev
+
+
+def test_full_size_error_against_fp64_oracle(config2_full):
+    tracks, obs, prop = config2_full
+    subset = obs[::20].copy()  # 8 antennas, one per arm position class
+    print(f"full config2 error check on {subset.size} antennas", flush=True)
+    ref = oracle_lib.radio_project(prop, subset, tracks, threads=16, fixed=False)
+    c, z, counters = gpu_project(tracks, subset, prop, deterministic=True)
+    cd = oracle_lib.radio_decode(c, subset)
+    zd = oracle_lib.radio_decode(z, subset, potential=True)
+    for got, want in ((cd, ref["coreas_float"]), (zd, ref["zhs_float"])):
+        err = np.linalg.norm(got - want) / np.linalg.norm(want)
+        assert err <= 1e-3
+        assert err <= 1e-6  # FP64 path: only fixed-point quantisation remains
