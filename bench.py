@@ -48,6 +48,11 @@ def parse():
     p.add_argument("--chunk-tracks", type=int, default=0)
     p.add_argument("--tile-bins", type=int, default=0)
     p.add_argument("--target-workgroups", type=int, default=0)
+    p.add_argument("--block-threads", type=int, default=0)
+    p.add_argument("--split", action="store_true",
+                   help="run CoREAS and ZHS as two launches instead of one fused launch")
+    p.add_argument("--exact", action="store_true",
+                   help="reference FP64 operation order (bit-identical to the FP64 oracle)")
     return p.parse_args()
 
 
@@ -88,7 +93,7 @@ def cpu_baseline(tracks, observers, propagation, mode, target_seconds):
                       f"({updates:.3e} updates, {dt:.2f} s wall), FP64 oracle, OpenMP"}
 
 
-def field_error(tracks, observers, propagation, mode, limit):
+def field_error(tracks, observers, propagation, mode, limit, exact):
     """Relative L2 error of the decoded GPU waveforms against the unquantised
     FP64 oracle on 8 of the antennas (every 20th), full track list."""
     import c8mi
@@ -103,7 +108,7 @@ def field_error(tracks, observers, propagation, mode, limit):
     ref = oracle_lib.radio_project(propagation, subset, tracks, coreas=coreas, zhs=zhs,
                                    fixed=False, threads=threads)
     acc = c8mi.RadioAccumulator(subset, propagation, coreas=coreas, zhs=zhs,
-                                fixed_point_limit=limit)
+                                fixed_point_limit=limit, exact=exact)
     dev = c8mi.DeviceArray.from_host(tracks)
     acc.accumulate_tracks(dev, tracks.shape[1], tracks.shape[1])
     c, z, _ = acc.download()
@@ -115,6 +120,18 @@ def field_error(tracks, observers, propagation, mode, limit):
     if zhs:
         errors["zhs"] = float(np.linalg.norm(z - ref["zhs_float"]) / np.linalg.norm(ref["zhs_float"]))
     return errors
+
+
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 PMC summary
+    (FETCH_SIZE + WRITE_SIZE, copy-calibrated), or None."""
+    path = os.path.join(ROOT, "profiles", "radio_traffic.json")
+    try:
+        with open(path) as f:
+            data = json.load(f)
+    except (OSError, ValueError):
+        return None
+    return data["traffic_bytes"] if data.get("kernel") == kernel else None
 
 
 def main():
@@ -141,9 +158,10 @@ def main():
     zhs = args.mode in ("coreas_zhs", "zhs")
     limit = 1.0
     acc = c8mi.RadioAccumulator(observers, propagation, coreas=coreas, zhs=zhs,
-                                fixed_point_limit=limit)
-    if args.chunk_tracks or args.tile_bins or args.target_workgroups:
-        acc.set_tuning(args.chunk_tracks, args.tile_bins, args.target_workgroups)
+                                fixed_point_limit=limit, exact=args.exact)
+    if args.chunk_tracks or args.tile_bins or args.target_workgroups or args.block_threads or args.split:
+        acc.set_tuning(args.chunk_tracks, args.tile_bins, args.target_workgroups,
+                       args.block_threads, args.split)
     n = tracks.shape[1]
     device_tracks = c8mi.DeviceArray.from_host(tracks)
 
@@ -161,16 +179,21 @@ def main():
         step()
     c8mi.device_synchronize()
     elapsed = time.perf_counter() - t0
-    kernel_ms, launches = acc.kernel_time()
+    timing = acc.kernel_time()
     _, _, counters = acc.download_fixed()
     acc.close()
 
     ms_per_step = elapsed * 1e3 / args.steps
     updates = n * observers.size
     value = updates / (elapsed / args.steps)
-    kernel_avg_ms = kernel_ms / max(launches, 1)
-    flops = FLOPS_PER_UPDATE[args.mode] * updates
-    achieved = flops / (kernel_avg_ms * 1e-3) / 1e12
+    # Dominant kernel: the launch kind with the largest total time.
+    kind = max(timing, key=lambda k: timing[k][0])
+    kernel_total_ms, launches = timing[kind]
+    kernel_avg_ms = kernel_total_ms / max(launches, 1)
+    achieved = FLOPS_PER_UPDATE[kind] * updates / (kernel_avg_ms * 1e-3) / 1e12
+    labels = {"coreas_zhs": (1, 1), "coreas": (1, 0), "zhs": (0, 1)}
+    kernel_name = "radio_project_kernel<coreas=%d,zhs=%d,fixed=1,exact=%d>" % (
+        labels[kind] + (int(args.exact),))
     result = {
         "metric": METRIC,
         "value": value,
@@ -183,6 +206,7 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
+        "precision": "exact FP64 order" if args.exact else "FP64, hardware reciprocal/rsqrt + Newton",
         "data": "synthetic",
         "config": {
             "workload": f"radio standalone: {n} tracks x {observers.size} antennas, "
@@ -202,16 +226,18 @@ def main():
             "peak": FP64_PEAK_TFLOPS,
             "unit": "TFLOP/s",
             "frac": achieved / FP64_PEAK_TFLOPS,
-            "traffic": None,
-            "kernel": f"radio_project_kernel<{int(coreas)},{int(zhs)},1>",
+            "traffic": measured_traffic(kernel_name),
+            "kernel": kernel_name,
             "kernel_avg_ms": kernel_avg_ms,
-            "flops_per_update": FLOPS_PER_UPDATE[args.mode],
+            "kernel_ms_per_step": {k: v[0] / args.steps for k, v in timing.items() if v[1]},
+            "flops_per_update": FLOPS_PER_UPDATE[kind],
             "algorithmic_hbm_gbs": (n * TRACK_BYTES) / (kernel_avg_ms * 1e-3) / 1e9,
         },
         "counters": counters,
     }
     if not args.no_error_check:
-        result["field_error_vs_fp64"] = field_error(tracks, observers, propagation, args.mode, limit)
+        result["field_error_vs_fp64"] = field_error(tracks, observers, propagation, args.mode, limit,
+                                                     args.exact)
     if not args.no_cpu_baseline:
         base = cpu_baseline(tracks, observers, propagation, args.mode, args.cpu_seconds)
         result["cpu_baseline"] = base

@@ -113,6 +113,8 @@ class RadioConfig(ctypes.Structure):
         ("zhs_enabled", c_int32),
         ("deterministic", c_int32),
         ("zhs_subtrack_refinement", c_uint32),
+        ("exact_fp64", c_int32),
+        ("reserved0", c_int32),
         ("fixed_point_field_limit_v_per_m", c_double),
         ("propagation", RadioPropagation),
         ("coreas_observers", POINTER(RadioObserver)),
@@ -220,7 +222,7 @@ class RadioAccumulator:
 
     def __init__(self, observers, propagation: Propagation, coreas: bool = True, zhs: bool = True,
                  deterministic: bool = True, fixed_point_limit: float = 1.0,
-                 zhs_refinement: int = 1, zhs_observers=None, stream=None):
+                 zhs_refinement: int = 1, zhs_observers=None, stream=None, exact: bool = False):
         self._observers = observers_array(observers)
         self._zhs_observers = (self._observers if zhs_observers is None
                                else observers_array(zhs_observers))
@@ -230,6 +232,7 @@ class RadioAccumulator:
         cfg.zhs_enabled = 1 if zhs else 0
         cfg.deterministic = 1 if deterministic else 0
         cfg.zhs_subtrack_refinement = zhs_refinement
+        cfg.exact_fp64 = 1 if exact else 0
         cfg.fixed_point_field_limit_v_per_m = fixed_point_limit
         cfg.propagation = propagation.struct
         cfg.coreas_observers = self._observers.ctypes.data_as(POINTER(RadioObserver))
@@ -292,14 +295,17 @@ class RadioAccumulator:
             None if zhs is None else zhs.ctypes.data_as(POINTER(c_int64)), byref(counters)))
         return coreas, zhs, counters.as_dict()
 
-    def set_tuning(self, chunk_tracks: int = 0, tile_bins: int = 0, target_workgroups: int = 0) -> None:
+    def set_tuning(self, chunk_tracks: int = 0, tile_bins: int = 0, target_workgroups: int = 0,
+                   block_threads: int = 0, split: bool = True) -> None:
         _check(library().c8mi_radio_set_tuning(self.handle, c_uint64(chunk_tracks),
-                                               c_uint32(tile_bins), c_uint32(target_workgroups)))
+                                               c_uint32(tile_bins), c_uint32(target_workgroups),
+                                               c_uint32(block_threads), c_int(1 if split else 0)))
 
     def enable_kernel_timing(self, enable: bool = True) -> None:
         _check(library().c8mi_radio_enable_kernel_timing(self.handle, c_int(1 if enable else 0)))
 
     def kernel_time(self):
-        ms, launches = c_double(), c_uint64()
-        _check(library().c8mi_radio_kernel_time(self.handle, byref(ms), byref(launches)))
-        return ms.value, launches.value
+        """{kind: (total ms, launches)} for kinds 'coreas_zhs', 'coreas', 'zhs'."""
+        ms, launches = (c_double * 3)(), (c_uint64 * 3)()
+        _check(library().c8mi_radio_kernel_time(self.handle, ms, launches))
+        return {kind: (ms[k], launches[k]) for k, kind in enumerate(("coreas_zhs", "coreas", "zhs"))}

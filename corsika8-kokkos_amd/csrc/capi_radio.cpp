@@ -145,10 +145,11 @@ int c8mi_radio_download_fixed(c8mi_radio* radio, int64_t* coreas, int64_t* zhs,
 }
 
 int c8mi_radio_set_tuning(c8mi_radio* radio, uint64_t chunk_tracks, uint32_t tile_bins,
-                          uint32_t target_workgroups) {
+                          uint32_t target_workgroups, uint32_t block_threads, int split) {
   return guarded([&] {
     if (!radio) throw std::invalid_argument("null radio handle");
-    radio->accumulator.set_tuning(chunk_tracks, tile_bins, target_workgroups);
+    radio->accumulator.set_tuning(chunk_tracks, tile_bins, target_workgroups, block_threads,
+                                  split != 0);
   });
 }
 
@@ -159,11 +160,10 @@ int c8mi_radio_enable_kernel_timing(c8mi_radio* radio, int enable) {
   });
 }
 
-int c8mi_radio_kernel_time(c8mi_radio* radio, double* total_ms, uint64_t* launches) {
+int c8mi_radio_kernel_time(c8mi_radio* radio, double ms[3], uint64_t launches[3]) {
   return guarded([&] {
     if (!radio) throw std::invalid_argument("null radio handle");
-    double const ms = radio->accumulator.kernel_time_ms(launches);
-    if (total_ms) *total_ms = ms;
+    radio->accumulator.kernel_time_ms(ms, launches);
   });
 }
 

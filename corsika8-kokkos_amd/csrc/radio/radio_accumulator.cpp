@@ -35,6 +35,8 @@ void validate(c8mi_radio_config const& config) {
        !(p.step_m > 0.) || !std::isfinite(p.inverse_step_per_m) ||
        !(p.inverse_step_per_m > 0.)))
     throw std::invalid_argument("invalid flat-atmosphere radio snapshot");
+  if (!homogeneous && p.table_size >= (std::uint64_t{1} << 32))
+    throw std::length_error("radio propagation table above 2^32 entries");
   auto check = [&](c8mi_radio_observer const* observers, std::uint64_t count) {
     if (count != 0 && observers == nullptr)
       throw std::invalid_argument("null radio observer list");
@@ -44,6 +46,8 @@ void validate(c8mi_radio_config const& config) {
           !(o.duration_s >= 0.) || !std::isfinite(o.sample_rate_hz) ||
           !(o.sample_rate_hz > 0.) || o.number_of_bins == 0)
         throw std::invalid_argument("invalid radio observer");
+      if (o.number_of_bins >= (std::uint64_t{1} << 31))
+        throw std::length_error("radio observer with 2^31 or more bins");
       double const height = (o.position_m[2] - p.minimum_height_m) * p.inverse_step_per_m;
       if (!homogeneous && (!std::isfinite(height) || height < 0. ||
                            height + 0.5 >= static_cast<double>(p.table_size)))
@@ -178,8 +182,8 @@ RadioAccumulator::RadioAccumulator(c8mi_radio_config const& config, hipStream_t 
 RadioAccumulator::~RadioAccumulator() {
   if (stream_) (void)hipStreamSynchronize(stream_);
   for (auto& e : pending_events_) {
-    (void)hipEventDestroy(e.first);
-    (void)hipEventDestroy(e.second);
+    (void)hipEventDestroy(e.start);
+    (void)hipEventDestroy(e.stop);
   }
   for (auto& e : free_events_) {
     (void)hipEventDestroy(e.first);
@@ -287,8 +291,8 @@ ProjectionLaunch RadioAccumulator::base_launch(bool coreas, bool zhs, std::uint3
     std::uint64_t const per_chunk = std::max<std::uint64_t>(1, std::uint64_t(observers) * L.bin_tiles);
     std::uint64_t const chunks_wanted = std::max<std::uint64_t>(1, target_workgroups_ / per_chunk);
     chunk = (count + chunks_wanted - 1) / chunks_wanted;
-    chunk = std::max<std::uint64_t>(kProjectionBlock, (chunk + kProjectionBlock - 1) /
-                                                          kProjectionBlock * kProjectionBlock);
+    std::uint64_t const block = block_threads_;
+    chunk = std::max<std::uint64_t>(block, (chunk + block - 1) / block * block);
   }
   L.chunk_size = chunk;
   std::uint64_t const chunks = (count + chunk - 1) / chunk;
@@ -333,16 +337,16 @@ void RadioAccumulator::project(double const* tracks, std::uint64_t stride, std::
       }
       C8MI_HIP(hipEventRecord(events.first, stream_));
     }
-    launch_projection(L, coreas, zhs, fixed, stream_);
+    launch_projection(L, coreas, zhs, fixed, config_.exact_fp64 != 0, block_threads_, stream_);
     if (timing_) {
       C8MI_HIP(hipEventRecord(events.second, stream_));
-      pending_events_.push_back(events);
+      pending_events_.push_back({events.first, events.second, coreas && zhs ? 0 : coreas ? 1 : 2});
     }
     ++statistics_.projection_launches;
     statistics_.track_observer_pairs += count * observers * (std::uint64_t(coreas) + zhs);
     if (coreas && zhs) statistics_.fused_track_observer_pairs += count * observers;
   };
-  if (fuse_) {
+  if (fuse_ && !split_) {
     run(true, true, true);
   } else {
     bool first = true;
@@ -433,32 +437,42 @@ void RadioAccumulator::download_fixed(std::int64_t* coreas, std::int64_t* zhs,
 }
 
 void RadioAccumulator::set_tuning(std::uint64_t chunk_tracks, std::uint32_t tile_bins,
-                                  std::uint32_t target_workgroups) {
-  if (chunk_tracks % kProjectionBlock != 0)
+                                  std::uint32_t target_workgroups, std::uint32_t block_threads,
+                                  bool split) {
+  std::uint32_t const block = block_threads == 0 ? kProjectionBlock : block_threads;
+  if (block != 256 && block != 512 && block != 1024)
+    throw std::invalid_argument("radio workgroup size must be 256, 512 or 1024");
+  if (chunk_tracks % block != 0)
     throw std::invalid_argument("radio chunk size must be a multiple of the workgroup size");
   chunk_tracks_ = chunk_tracks;
   tile_bins_limit_ = tile_bins;
   if (target_workgroups != 0) target_workgroups_ = target_workgroups;
+  block_threads_ = block;
+  split_ = split;
 }
 
 void RadioAccumulator::enable_kernel_timing(bool enable) {
   timing_ = enable;
-  timed_ms_ = 0.;
-  timed_launches_ = 0;
+  for (int k = 0; k < 3; ++k) {
+    timed_ms_[k] = 0.;
+    timed_launches_[k] = 0;
+  }
 }
 
-double RadioAccumulator::kernel_time_ms(std::uint64_t* launches) {
+void RadioAccumulator::kernel_time_ms(double ms[3], std::uint64_t launches[3]) {
   C8MI_HIP(hipStreamSynchronize(stream_));
   for (auto& e : pending_events_) {
-    float ms = 0.f;
-    C8MI_HIP(hipEventElapsedTime(&ms, e.first, e.second));
-    timed_ms_ += ms;
-    ++timed_launches_;
-    free_events_.push_back(e);
+    float elapsed = 0.f;
+    C8MI_HIP(hipEventElapsedTime(&elapsed, e.start, e.stop));
+    timed_ms_[e.kind] += elapsed;
+    ++timed_launches_[e.kind];
+    free_events_.push_back({e.start, e.stop});
   }
   pending_events_.clear();
-  if (launches) *launches = timed_launches_;
-  return timed_ms_;
+  for (int k = 0; k < 3; ++k) {
+    if (ms) ms[k] = timed_ms_[k];
+    if (launches) launches[k] = timed_launches_[k];
+  }
 }
 
 }  // namespace c8mi::radio

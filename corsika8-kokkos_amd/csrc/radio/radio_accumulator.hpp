@@ -61,12 +61,16 @@ class RadioAccumulator {
   // Raw fixed-point sums (downloadFixed); deterministic mode only.
   void download_fixed(std::int64_t* coreas, std::int64_t* zhs, c8mi_radio_counters* counters);
 
+  // Launch shape: tracks per workgroup (0 = automatic), LDS window tile in bins
+  // (0 = automatic), target workgroups per launch, threads per workgroup
+  // (256/512/1024) and whether fusable CoREAS+ZHS work is split into two launches.
   void set_tuning(std::uint64_t chunk_tracks, std::uint32_t tile_bins,
-                  std::uint32_t target_workgroups);
+                  std::uint32_t target_workgroups, std::uint32_t block_threads, bool split);
   void enable_kernel_timing(bool enable);
-  // Sum of projection-kernel durations measured with HIP events on this
-  // accumulator's stream since timing was enabled.
-  double kernel_time_ms(std::uint64_t* launches);
+  // Projection-kernel durations measured with HIP events on this
+  // accumulator's stream since timing was enabled, per launch kind:
+  // [0] fused CoREAS+ZHS, [1] CoREAS only, [2] ZHS only.
+  void kernel_time_ms(double ms[3], std::uint64_t launches[3]);
 
   std::uint64_t coreas_total_bins() const noexcept { return coreas_bins_; }
   std::uint64_t zhs_total_bins() const noexcept { return zhs_bins_; }
@@ -115,13 +119,22 @@ class RadioAccumulator {
 
   std::uint64_t chunk_tracks_ = 0;     // 0: automatic
   std::uint32_t tile_bins_limit_ = 0;  // 0: automatic
-  std::uint32_t target_workgroups_ = 2048;
+  // Defaults measured on MI355X for 2^20 tracks x 160 antennas x 2048 bins:
+  // separate CoREAS and ZHS launches (48 KiB LDS each, three workgroups per
+  // CU), 512 threads, about 8192 workgroups per launch.
+  std::uint32_t target_workgroups_ = 8192;
+  std::uint32_t block_threads_ = 512;
+  bool split_ = true;
 
   bool timing_ = false;
-  std::vector<std::pair<hipEvent_t, hipEvent_t>> pending_events_;
+  struct TimedLaunch {
+    hipEvent_t start, stop;
+    int kind;
+  };
+  std::vector<TimedLaunch> pending_events_;
   std::vector<std::pair<hipEvent_t, hipEvent_t>> free_events_;
-  double timed_ms_ = 0.;
-  std::uint64_t timed_launches_ = 0;
+  double timed_ms_[3] = {0., 0., 0.};
+  std::uint64_t timed_launches_[3] = {0, 0, 0};
 
   RadioStatistics statistics_{};
 };

@@ -23,6 +23,7 @@
 
 #include <cmath>
 #include <cstdint>
+#include <stdexcept>
 #include <type_traits>
 
 #include "radio_device.hpp"
@@ -81,11 +82,49 @@ __device__ __forceinline__ double doppler(double index, V3 beta, V3 emit) {
   return (1. - product) - perr;
 }
 
+// ---- arithmetic of the two precision modes --------------------------------
+//
+// Exact: the reference's FP64 expressions in its operation order (correctly
+// rounded IEEE division and square root, no contraction): the fixed-point
+// waveforms equal the FP64 restatement's bit for bit.
+// Fast: FP64 throughout, but every division and square root of the pair loop
+// becomes a hardware reciprocal / reciprocal square root refined by two Newton
+// steps (about one ulp), and the inverse distance is shared by the emission
+// direction and the amplitudes. Arrival times must stay at FP64 accuracy: an
+// endpoint sample is a spike in one bin, and a delay error of even 1e-15 s at
+// 15 km (an FP32 refractivity ratio) moves a few samples per million across a
+// bin edge, which shows up as per-bin errors of order 1e-3.
+
+__device__ __forceinline__ double recip_fast(double b) {
+  double const r0 = __builtin_amdgcn_rcp(b);
+  double e = fma(-b, r0, 1.);
+  double r = fma(r0, e, r0);
+  e = fma(-b, r, 1.);
+  r = fma(r, e, r);
+  // 0 and +-inf: keep the hardware result (+-inf, +-0) instead of NaN.
+  return isfinite(r0) && r0 != 0. ? r : r0;
+}
+
+// Returns sqrt(x) and 1/sqrt(x) for x > 0 (NaN propagates).
+__device__ __forceinline__ double rsqrt_fast(double x, double& root) {
+  double y = __builtin_amdgcn_rsq(x);
+  double h = x * y;
+  double e = fma(-h, y, 1.);
+  y = fma(0.5 * y, e, y);
+  h = x * y;
+  e = fma(-h, y, 1.);
+  y = fma(0.5 * y, e, y);
+  root = x * y;
+  return y;
+}
+
 struct Track {
   V3 start, end, disp, beta;
   double t0, t1, duration, length, beta_module, constant;
+  double inv_beta_module;  // fast mode only
 };
 
+template <bool Exact>
 __device__ __forceinline__ bool load_track(double const* __restrict__ tracks,
                                            std::uint64_t stride, std::uint64_t i, Track& t) {
   t.start = {tracks[i], tracks[stride + i], tracks[2 * stride + i]};
@@ -95,10 +134,21 @@ __device__ __forceinline__ bool load_track(double const* __restrict__ tracks,
   double const signed_weight = tracks[8 * stride + i];
   t.duration = t.t1 - t.t0;
   t.disp = vsub(t.end, t.start);
-  t.length = sqrt(vdot(t.disp, t.disp));
-  if (!(t.duration > 0.) || !(t.length > 0.)) return false;
-  t.beta = vdiv(t.disp, kC * t.duration);
-  t.beta_module = t.length / (kC * t.duration);
+  if constexpr (Exact) {
+    t.length = sqrt(vdot(t.disp, t.disp));
+    if (!(t.duration > 0.) || !(t.length > 0.)) return false;
+    t.beta = vdiv(t.disp, kC * t.duration);
+    t.beta_module = t.length / (kC * t.duration);
+  } else {
+    double const l2 = fma(t.disp.x, t.disp.x, fma(t.disp.y, t.disp.y, t.disp.z * t.disp.z));
+    if (!(t.duration > 0.) || !(l2 > 0.)) return false;
+    double const inv_length = rsqrt_fast(l2, t.length);
+    double const c_duration = kC * t.duration;
+    double const inv_c_duration = recip_fast(c_duration);
+    t.beta = vmul(t.disp, inv_c_duration);
+    t.beta_module = t.length * inv_c_duration;
+    t.inv_beta_module = c_duration * inv_length;
+  }
   t.constant = kChargeEm * signed_weight;
   return true;
 }
@@ -107,9 +157,10 @@ __device__ __forceinline__ bool load_track(double const* __restrict__ tracks,
 // thread with the same expressions propagate() and add_sample() use.
 struct Obs {
   V3 position;
-  double start, duration, rate, scale, grid, wavelength;
+  double start, duration, rate, grid, wavelength, two_pi_over_wavelength;
   double nbins_d;
-  std::uint64_t nbins, offset;
+  std::int32_t nbins;
+  std::uint64_t offset;
   double destination_height;
   std::uint64_t destination_index;
   double n_destination_table, integrated_destination;
@@ -117,19 +168,27 @@ struct Obs {
 
 struct Path {
   double time, n_source, n_destination, distance;
+  double inv_distance;  // fast mode only
   V3 emit;
 };
 
+template <bool Exact>
 __device__ __forceinline__ Path propagate(PropagationDev const& p, V3 source, Obs const& o) {
   Path r;
   V3 const disp = vsub(o.position, source);
-  double const distance = sqrt(vdot(disp, disp));
-  V3 const emit = vdiv(disp, distance);
-  r.distance = distance;
-  r.emit = emit;
+  if constexpr (Exact) {
+    r.distance = sqrt(vdot(disp, disp));
+    r.emit = vdiv(disp, r.distance);
+  } else {
+    double const r2 = fma(disp.x, disp.x, fma(disp.y, disp.y, disp.z * disp.z));
+    r.inv_distance = rsqrt_fast(r2, r.distance);
+    r.emit = vmul(disp, r.inv_distance);
+  }
+  double const distance = r.distance;
   if (p.homogeneous_index > 0.) {
     double const index = p.homogeneous_index;
-    r.time = index * distance / kC;
+    if constexpr (Exact) r.time = index * distance / kC;
+    else r.time = index * distance * (1. / kC);
     r.n_source = index;
     r.n_destination = index;
     return r;
@@ -148,7 +207,8 @@ __device__ __forceinline__ Path propagate(PropagationDev const& p, V3 source, Ob
         p.integrated[last] + p.slope_i_upper * fabs(p.maximum_height - source.z);
     height = source_height - destination_height;
   } else if (source_height + 0.5 < static_cast<double>(last) && source_height > 0.) {
-    std::uint64_t const source_index = static_cast<std::uint64_t>(source_height + 0.5);
+    // The table has fewer than 2^32 entries (checked on the host).
+    std::uint32_t const source_index = static_cast<std::uint32_t>(source_height + 0.5);
     n_source = p.refractivity[source_index] + 1.;
     integrated_source = p.integrated[source_index];
     height = static_cast<double>(static_cast<std::int64_t>(source_index) -
@@ -166,7 +226,12 @@ __device__ __forceinline__ Path propagate(PropagationDev const& p, V3 source, Ob
     n_destination = 1.;
     height = destination_height - fabs(source_height);
   }
-  r.time = (1. + (integrated_source - o.integrated_destination) / height) * distance / kC;
+  if constexpr (Exact) {
+    r.time = (1. + (integrated_source - o.integrated_destination) / height) * distance / kC;
+  } else {
+    double const ratio = (integrated_source - o.integrated_destination) * recip_fast(height);
+    r.time = (1. + ratio) * (distance * (1. / kC));
+  }
   r.n_source = n_source;
   r.n_destination = n_destination;
   return r;
@@ -179,24 +244,32 @@ struct Tally {
 };
 
 // LDS accumulator for one mode: `Fixed` selects int64 fixed point (the
-// production default) or FP64. `Diff` stores a difference array.
+// production default) or FP64. The ZHS tile stores a difference array.
 template <bool Fixed>
 struct Tile {
   using Acc = std::conditional_t<Fixed, long long, double>;
-  Acc* data;                 // 3 components x stride
-  std::uint32_t stride;      // bins per component row
-  std::uint64_t begin;       // first global bin of this tile
-  std::uint32_t bins;        // bins owned by this tile
+  Acc* data;             // 3 components x stride
+  std::uint32_t stride;  // bins per component row
+  std::int32_t begin;    // first global bin of this tile
+  std::int32_t bins;     // bins owned by this tile
 };
 
 // add_radio_fixed_point on an LDS cell; returns false on a non-representable
 // contribution (the reference's overflow case), in which case nothing is added.
+// Below 2^51 the round-to-nearest-even integer comes from adding 1.5 * 2^52
+// and reading the low mantissa bits: the same value as (long long) rint().
 __device__ __forceinline__ bool fixed_increment(double value, double scale, long long& inc,
                                                 double& abs_sum) {
   double const scaled = value * scale;
-  if (!isfinite(scaled) || fabs(scaled) >= kIntLimit) return false;
-  inc = static_cast<long long>(rint(scaled));
-  abs_sum += fabs(scaled);
+  double const magnitude = fabs(scaled);
+  if (magnitude < 2251799813685248.) {  // 2^51
+    double const shifted = scaled + 6755399441055744.;  // 1.5 * 2^52
+    inc = __double_as_longlong(shifted) - 0x4338000000000000LL;
+  } else {
+    if (!isfinite(scaled) || magnitude >= kIntLimit) return false;
+    inc = static_cast<long long>(rint(scaled));
+  }
+  abs_sum += magnitude;
   return true;
 }
 
@@ -209,29 +282,38 @@ __device__ __forceinline__ void lds_add(T* address, T value) {
     atomicAdd(address, value);
 }
 
+// Bin of `time` per observer_nearest_bin; -1 when outside [0, nbins).
+__device__ __forceinline__ std::int32_t sample_bin(Obs const& o, double time) {
+  double const bin_value = nearest_bin((time - o.start) * o.rate);
+  if (!(bin_value >= 0. && bin_value < o.nbins_d)) return -1;
+  return static_cast<std::int32_t>(bin_value);
+}
+
 // CoREAS sample (add_sample with time gating, IsVectorPotential = false).
 template <bool Fixed>
-__device__ __forceinline__ void add_field_sample(Obs const& o, Tile<Fixed> const& tile,
-                                                 double time, V3 c, Tally& tally) {
+__device__ __forceinline__ void add_field_sample(Obs const& o, double scale,
+                                                 Tile<Fixed> const& tile, double time, V3 c,
+                                                 Tally& tally) {
   if (time < o.start || time > o.start + o.duration) return;
-  double const bin_value = nearest_bin((time - o.start) * o.rate);
-  if (!(bin_value >= 0. && bin_value < o.nbins_d)) return;
-  std::uint64_t const bin = static_cast<std::uint64_t>(bin_value);
-  if (bin >= o.nbins) return;
-  if (bin < tile.begin || bin >= tile.begin + tile.bins) return;
-  std::uint32_t const local = static_cast<std::uint32_t>(bin - tile.begin);
+  std::int32_t const bin = sample_bin(o, time);
+  if (bin < 0) return;
+  std::int32_t const local = bin - tile.begin;
+  if (local < 0 || local >= tile.bins) return;
   if constexpr (Fixed) {
     long long ix = 0, iy = 0, iz = 0;
-    bool const vx = fixed_increment(c.x, o.scale, ix, tally.abs_sum);
-    bool const vy = fixed_increment(c.y, o.scale, iy, tally.abs_sum);
-    bool const vz = fixed_increment(c.z, o.scale, iz, tally.abs_sum);
+    bool const vx = fixed_increment(c.x, scale, ix, tally.abs_sum);
+    bool const vy = fixed_increment(c.y, scale, iy, tally.abs_sum);
+    bool const vz = fixed_increment(c.z, scale, iz, tally.abs_sum);
     if (vx) lds_add(tile.data + local, ix);
     if (vy) lds_add(tile.data + tile.stride + local, iy);
     if (vz) lds_add(tile.data + 2 * tile.stride + local, iz);
-    if (!(vx && vy && vz)) {
-      ++tally.overflow;
-      return;
-    }
+    // Counter updates are value selects: a branch between two tally fields
+    // can become a store through a selected pointer, which pins the tally to
+    // scratch memory.
+    bool const valid = vx && vy && vz;
+    tally.overflow += valid ? 0ull : 1ull;
+    tally.coreas += valid ? 1ull : 0ull;
+    return;
   } else {
     lds_add(tile.data + local, c.x);
     lds_add(tile.data + tile.stride + local, c.y);
@@ -240,74 +322,107 @@ __device__ __forceinline__ void add_field_sample(Obs const& o, Tile<Fixed> const
   ++tally.coreas;
 }
 
-// Adds `c` to the ZHS bins [first, last] (global indices, already known to be
-// inside [0, nbins)) through the difference array, clipped to the tile.
-// Returns the number of bins of this tile that received it.
+// Adds `c` to the ZHS bins [first, last] (inside [0, nbins)) through the
+// difference array, clipped to the tile. Returns the number of this tile's
+// bins that received it.
 template <bool Fixed>
-__device__ __forceinline__ std::uint64_t add_potential_range(Obs const& o,
-                                                             Tile<Fixed> const& tile,
-                                                             std::uint64_t first,
-                                                             std::uint64_t last, V3 c,
-                                                             Tally& tally, bool& valid) {
-  std::uint64_t const lo = first > tile.begin ? first : tile.begin;
-  std::uint64_t const tile_end = tile.begin + tile.bins;  // exclusive
-  std::uint64_t const hi = last + 1 < tile_end ? last + 1 : tile_end;  // exclusive
+__device__ __forceinline__ std::int32_t add_potential_range(double scale,
+                                                            Tile<Fixed> const& tile,
+                                                            std::int32_t first,
+                                                            std::int32_t last, V3 c,
+                                                            Tally& tally, bool& valid) {
+  std::int32_t const lo = (first > tile.begin ? first : tile.begin) - tile.begin;
+  std::int32_t const hi =
+      (last + 1 < tile.begin + tile.bins ? last + 1 : tile.begin + tile.bins) - tile.begin;
   valid = true;
   if (lo >= hi) return 0;
-  std::uint32_t const a = static_cast<std::uint32_t>(lo - tile.begin);
-  std::uint32_t const b = static_cast<std::uint32_t>(hi - tile.begin);  // <= bins
   if constexpr (Fixed) {
     long long ix = 0, iy = 0, iz = 0;
-    bool const vx = fixed_increment(c.x, o.scale, ix, tally.abs_sum);
-    bool const vy = fixed_increment(c.y, o.scale, iy, tally.abs_sum);
-    bool const vz = fixed_increment(c.z, o.scale, iz, tally.abs_sum);
+    bool const vx = fixed_increment(c.x, scale, ix, tally.abs_sum);
+    bool const vy = fixed_increment(c.y, scale, iy, tally.abs_sum);
+    bool const vz = fixed_increment(c.z, scale, iz, tally.abs_sum);
     if (vx) {
-      lds_add(tile.data + a, ix);
-      lds_add(tile.data + b, -ix);
+      lds_add(tile.data + lo, ix);
+      lds_add(tile.data + hi, -ix);
     }
     if (vy) {
-      lds_add(tile.data + tile.stride + a, iy);
-      lds_add(tile.data + tile.stride + b, -iy);
+      lds_add(tile.data + tile.stride + lo, iy);
+      lds_add(tile.data + tile.stride + hi, -iy);
     }
     if (vz) {
-      lds_add(tile.data + 2 * tile.stride + a, iz);
-      lds_add(tile.data + 2 * tile.stride + b, -iz);
+      lds_add(tile.data + 2 * tile.stride + lo, iz);
+      lds_add(tile.data + 2 * tile.stride + hi, -iz);
     }
     valid = vx && vy && vz;
   } else {
-    lds_add(tile.data + a, c.x);
-    lds_add(tile.data + b, -c.x);
-    lds_add(tile.data + tile.stride + a, c.y);
-    lds_add(tile.data + tile.stride + b, -c.y);
-    lds_add(tile.data + 2 * tile.stride + a, c.z);
-    lds_add(tile.data + 2 * tile.stride + b, -c.z);
+    lds_add(tile.data + lo, c.x);
+    lds_add(tile.data + hi, -c.x);
+    lds_add(tile.data + tile.stride + lo, c.y);
+    lds_add(tile.data + tile.stride + hi, -c.y);
+    lds_add(tile.data + 2 * tile.stride + lo, c.z);
+    lds_add(tile.data + 2 * tile.stride + hi, -c.z);
   }
   return hi - lo;
 }
 
 // ZHS sample (add_sample with IsVectorPotential = true): no time gate.
 template <bool Fixed>
-__device__ __forceinline__ void add_potential_sample(Obs const& o, Tile<Fixed> const& tile,
-                                                     double time, V3 c, Tally& tally) {
-  double const bin_value = nearest_bin((time - o.start) * o.rate);
-  if (!(bin_value >= 0. && bin_value < o.nbins_d)) return;
-  std::uint64_t const bin = static_cast<std::uint64_t>(bin_value);
-  if (bin >= o.nbins) return;
+__device__ __forceinline__ void add_potential_sample(Obs const& o, double scale,
+                                                     Tile<Fixed> const& tile, double time, V3 c,
+                                                     Tally& tally) {
+  std::int32_t const bin = sample_bin(o, time);
+  if (bin < 0) return;
   bool valid = true;
-  std::uint64_t const added = add_potential_range(o, tile, bin, bin, c, tally, valid);
-  if (added == 0) return;
-  if (!valid) {
-    ++tally.overflow;
-    return;
-  }
-  ++tally.zhs;
+  std::int32_t const added = add_potential_range(scale, tile, bin, bin, c, tally, valid);
+  unsigned long long const n = static_cast<unsigned long long>(added);
+  tally.zhs += valid ? n : 0ull;
+  tally.overflow += valid ? 0ull : n;
 }
 
-template <bool Fixed>
-__device__ void coreas_pair(Track const& t, PropagationDev const& p, Obs const& o,
-                            Tile<Fixed> const& tile, Tally& tally) {
-  Path const ps = propagate(p, t.start, o);
-  Path const pe = propagate(p, t.end, o);
+// separate_same_bin_endpoints (RadioProjection.inl:303-336); `to_bins` is the
+// reference's division by the grid (exact) or the product with the rate (fast).
+template <bool Exact>
+__device__ __forceinline__ double to_bins(double time, Obs const& o) {
+  if constexpr (Exact) return time / o.grid;
+  else return time * o.rate;
+}
+
+template <bool Exact>
+__device__ __forceinline__ void separate_same_bin(double& start_time, double& end_time,
+                                                  Obs const& o, bool signed_order) {
+  double const s = to_bins<Exact>(start_time, o);
+  double const e = to_bins<Exact>(end_time, o);
+  if (static_cast<long long>(nearest_bin(s)) != static_cast<long long>(nearest_bin(e))) return;
+  double const grid = o.grid;
+  double const sf = s - floor(s);
+  double const ef = e - floor(e);
+  bool const forward = !signed_order || end_time >= start_time;
+  if (forward) {
+    if (sf >= 0.5 && ef >= 0.5) start_time -= grid;
+    else if (sf < 0.5 && ef < 0.5) end_time += grid;
+    else if (ef >= 1. - sf) end_time += grid;
+    else start_time -= grid;
+  } else {
+    if (sf >= 0.5 && ef >= 0.5) end_time -= grid;
+    else if (sf < 0.5 && ef < 0.5) start_time += grid;
+    else if (sf >= 1. - ef) start_time += grid;
+    else end_time -= grid;
+  }
+}
+
+// t.constant * rate / (doppler * R)
+template <bool Exact>
+__device__ __forceinline__ double field_scale(Track const& t, Obs const& o, double doppler_factor,
+                                              Path const& path) {
+  if constexpr (Exact) return t.constant * o.rate / (doppler_factor * path.distance);
+  else return (t.constant * o.rate) * path.inv_distance * recip_fast(doppler_factor);
+}
+
+template <bool Exact, bool Fixed>
+__device__ __forceinline__ void coreas_pair(Track const& t, PropagationDev const& p, Obs const& o,
+                            double scale, Tile<Fixed> const& tile, Tally& tally) {
+  Path const ps = propagate<Exact>(p, t.start, o);
+  Path const pe = propagate<Exact>(p, t.end, o);
   double const pre = doppler(ps.n_source, t.beta, ps.emit);
   double const post = doppler(pe.n_source, t.beta, pe.emit);
   double start_receive = t.t0 + ps.time;
@@ -316,13 +431,15 @@ __device__ void coreas_pair(Track const& t, PropagationDev const& p, Obs const& 
       (fabs(pre) < kCoreasThreshold || fabs(post) < kCoreasThreshold)) {
     V3 const midpoint = vmul(vadd(t.start, t.end), 0.5);
     double const middle_time = (t.t0 + t.t1) * 0.5;
-    Path const pm = propagate(p, midpoint, o);
+    Path const pm = propagate<Exact>(p, midpoint, o);
     double const middle_receive = middle_time + pm.time;
     double const md = doppler(pm.n_source, t.beta, pm.emit);
-    V3 field_start = vmul(vcross(pm.emit, vcross(pm.emit, t.beta)),
-                          t.constant * o.rate / (md * pm.distance));
+    V3 field_start =
+        vmul(vcross(pm.emit, vcross(pm.emit, t.beta)), field_scale<Exact>(t, o, md, pm));
     V3 field_end = vmul(field_start, -1.);
-    double delta = t.length / (kC * t.beta_module) * fabs(md);
+    double delta;
+    if constexpr (Exact) delta = t.length / (kC * t.beta_module) * fabs(md);
+    else delta = t.duration * fabs(md);  // length / (c beta) is the duration
     if (start_receive < end_receive) {
       start_receive = middle_receive - 0.5 * delta;
       end_receive = middle_receive + 0.5 * delta;
@@ -330,70 +447,43 @@ __device__ void coreas_pair(Track const& t, PropagationDev const& p, Obs const& 
       start_receive = middle_receive + 0.5 * delta;
       end_receive = middle_receive - 0.5 * delta;
     }
-    double const grid = o.grid;
     delta = end_receive - start_receive;
-    if (fabs(delta) < grid) {
-      double const scale = fabs(delta / grid);
-      field_start = vmul(field_start, scale);
-      field_end = vmul(field_end, scale);
-      // separate_same_bin_endpoints, signed order (RadioProjection.inl:303-336)
-      long long const sb = static_cast<long long>(nearest_bin(start_receive / grid));
-      long long const eb = static_cast<long long>(nearest_bin(end_receive / grid));
-      if (sb == eb) {
-        double const sf = start_receive / grid - floor(start_receive / grid);
-        double const ef = end_receive / grid - floor(end_receive / grid);
-        bool const forward = end_receive >= start_receive;
-        if (forward) {
-          if (sf >= 0.5 && ef >= 0.5) start_receive -= grid;
-          else if (sf < 0.5 && ef < 0.5) end_receive += grid;
-          else if (ef >= 1. - sf) end_receive += grid;
-          else start_receive -= grid;
-        } else {
-          if (sf >= 0.5 && ef >= 0.5) end_receive -= grid;
-          else if (sf < 0.5 && ef < 0.5) start_receive += grid;
-          else if (sf >= 1. - ef) start_receive += grid;
-          else end_receive -= grid;
-        }
-      }
+    if (fabs(delta) < o.grid) {
+      double const shrink = fabs(to_bins<Exact>(delta, o));
+      field_start = vmul(field_start, shrink);
+      field_end = vmul(field_end, shrink);
+      separate_same_bin<Exact>(start_receive, end_receive, o, true);
     }
-    add_field_sample(o, tile, start_receive, field_start, tally);
-    add_field_sample(o, tile, end_receive, field_end, tally);
+    add_field_sample(o, scale, tile, start_receive, field_start, tally);
+    add_field_sample(o, scale, tile, end_receive, field_end, tally);
     return;
   }
-  V3 field_start = vmul(vcross(ps.emit, vcross(ps.emit, t.beta)),
-                        t.constant * o.rate / (pre * ps.distance));
-  V3 field_end = vmul(vcross(pe.emit, vcross(pe.emit, t.beta)),
-                      -t.constant * o.rate / (post * pe.distance));
+  V3 field_start =
+      vmul(vcross(ps.emit, vcross(ps.emit, t.beta)), field_scale<Exact>(t, o, pre, ps));
+  V3 field_end =
+      vmul(vcross(pe.emit, vcross(pe.emit, t.beta)), -field_scale<Exact>(t, o, post, pe));
   if (pre < 1.e-9 || post < 1.e-9) {
-    double const grid = o.grid;
     double const delta = end_receive - start_receive;
-    if (fabs(delta) < grid) {
-      double const scale = fabs(delta / grid);
-      field_start = vmul(field_start, scale);
-      field_end = vmul(field_end, scale);
-      // separate_same_bin_endpoints, unsigned order: always "forward"
-      long long const sb = static_cast<long long>(nearest_bin(start_receive / grid));
-      long long const eb = static_cast<long long>(nearest_bin(end_receive / grid));
-      if (sb == eb) {
-        double const sf = start_receive / grid - floor(start_receive / grid);
-        double const ef = end_receive / grid - floor(end_receive / grid);
-        if (sf >= 0.5 && ef >= 0.5) start_receive -= grid;
-        else if (sf < 0.5 && ef < 0.5) end_receive += grid;
-        else if (ef >= 1. - sf) end_receive += grid;
-        else start_receive -= grid;
-      }
+    if (fabs(delta) < o.grid) {
+      double const shrink = fabs(to_bins<Exact>(delta, o));
+      field_start = vmul(field_start, shrink);
+      field_end = vmul(field_end, shrink);
+      separate_same_bin<Exact>(start_receive, end_receive, o, false);
     }
   }
-  add_field_sample(o, tile, start_receive, field_start, tally);
-  add_field_sample(o, tile, end_receive, field_end, tally);
+  add_field_sample(o, scale, tile, start_receive, field_start, tally);
+  add_field_sample(o, scale, tile, end_receive, field_end, tally);
 }
 
-template <bool Fixed>
-__device__ void zhs_segment(V3 p1, V3 p2, double time1, double time2, Track const& t,
-                            PropagationDev const& p, Obs const& o, Tile<Fixed> const& tile,
-                            Tally& tally, bool subdivided) {
-  V3 const midpoint = vmul(vadd(p1, p2), 0.5);
-  Path const path = propagate(p, midpoint, o);
+template <bool Exact, bool Fixed>
+__device__ __forceinline__ void zhs_segment(V3 p1, V3 p2, double time1, double time2, Track const& t,
+                            PropagationDev const& p, Obs const& o, double scale,
+                            Tile<Fixed> const& tile, Tally& tally, bool subdivided,
+                            bool use_cached, Path cached_midpoint_path) {
+  // The cached path travels by value: a pointer to the caller's local would
+  // pin it to scratch memory.
+  Path const path = use_cached ? cached_midpoint_path
+                               : propagate<Exact>(p, vmul(vadd(p1, p2), 0.5), o);
   double const n_source = path.n_source;
   double const bk = vdot(t.beta, path.emit);
   double const middle_time = (time1 + time2) * 0.5;
@@ -405,66 +495,99 @@ __device__ void zhs_segment(V3 p1, V3 p2, double time1, double time2, Track cons
     d2 = time1 + path.time - n_source * bk * (time1 - middle_time);
     sign = -1.;
   }
-  double const start_bin = floor((d1 - o.start) * o.rate + 0.5);
+  double const x1 = (d1 - o.start) * o.rate;
+  double const start_bin = floor(x1 + 0.5);
   double const end_bin = floor((d2 - o.start) * o.rate + 0.5);
   V3 const bperp = vcross(path.emit, vcross(t.beta, path.emit));
   double const den = 1. - n_source * bk;
+  // constant / (den * R)
+  double k;
+  if constexpr (Exact) k = 0.;  // divisions below keep the reference's form
+  else k = t.constant * path.inv_distance * recip_fast(den);
   if (start_bin == end_bin) {
     V3 potential;
     if (fabs(den) > 1.e-15) {
       double const fraction = fabs(d2 * o.rate - d1 * o.rate);
-      potential = vmul(bperp, sign * t.constant * fraction / (den * path.distance));
+      if constexpr (Exact)
+        potential = vmul(bperp, sign * t.constant * fraction / (den * path.distance));
+      else
+        potential = vmul(bperp, sign * k * fraction);
     } else {
       double const fraction = (time2 - time1) * o.rate;
-      potential = vmul(bperp, sign * t.constant * fraction / path.distance);
+      if constexpr (Exact)
+        potential = vmul(bperp, sign * t.constant * fraction / path.distance);
+      else
+        potential = vmul(bperp, sign * t.constant * fraction * path.inv_distance);
     }
-    add_potential_sample(o, tile, d2, potential, tally);
+    add_potential_sample(o, scale, tile, d2, potential, tally);
     return;
   }
   int const nb = static_cast<int>(end_bin - start_bin);
-  double fraction = fabs(start_bin + 0.5 - (d1 - o.start) * o.rate);
-  V3 potential = vmul(bperp, sign * t.constant * fraction / (den * path.distance));
-  add_potential_sample(o, tile, d1, potential, tally);
+  double fraction = fabs(start_bin + 0.5 - x1);
+  V3 potential;
+  if constexpr (Exact)
+    potential = vmul(bperp, sign * t.constant * fraction / (den * path.distance));
+  else
+    potential = vmul(bperp, sign * k * fraction);
+  add_potential_sample(o, scale, tile, d1, potential, tally);
   if (nb > 1) {
-    V3 const middle =
-        vmul(bperp, (subdivided ? 1. : sign) * t.constant / (den * path.distance));
+    V3 middle;
+    if constexpr (Exact)
+      middle = vmul(bperp, (subdivided ? 1. : sign) * t.constant / (den * path.distance));
+    else
+      middle = vmul(bperp, (subdivided ? 1. : sign) * k);
     // Bins of the first and last middle sample, exactly as the per-bin loop
     // computes them; in between the bins are consecutive.
-    double const b_first = nearest_bin((d1 + 1. / o.rate - o.start) * o.rate);
-    double const b_last =
-        nearest_bin((d1 + static_cast<double>(nb - 1) / o.rate - o.start) * o.rate);
+    double b_first, b_last;
+    if constexpr (Exact) {
+      b_first = nearest_bin((d1 + 1. / o.rate - o.start) * o.rate);
+      b_last = nearest_bin((d1 + static_cast<double>(nb - 1) / o.rate - o.start) * o.rate);
+    } else {
+      b_first = nearest_bin(x1 + 1.);
+      b_last = nearest_bin(x1 + static_cast<double>(nb - 1));
+    }
     if (b_last - b_first == static_cast<double>(nb - 2)) {
       double const lo = b_first < 0. ? 0. : b_first;
       double const hi = b_last < o.nbins_d - 1. ? b_last : o.nbins_d - 1.;
       if (lo <= hi) {
         bool valid = true;
-        std::uint64_t const added = add_potential_range(
-            o, tile, static_cast<std::uint64_t>(lo), static_cast<std::uint64_t>(hi), middle,
-            tally, valid);
-        if (added != 0) {
-          if (valid) tally.zhs += added;
-          else tally.overflow += added;
-        }
+        std::int32_t const added =
+            add_potential_range(scale, tile, static_cast<std::int32_t>(lo),
+                                static_cast<std::int32_t>(hi), middle, tally, valid);
+        unsigned long long const n = static_cast<unsigned long long>(added);
+        tally.zhs += valid ? n : 0ull;
+        tally.overflow += valid ? 0ull : n;
       }
     } else {
       for (int index = 1; index < nb; ++index)
-        add_potential_sample(o, tile, d1 + static_cast<double>(index) / o.rate, middle, tally);
+        add_potential_sample(o, scale, tile, d1 + static_cast<double>(index) / o.rate, middle,
+                             tally);
     }
   }
   fraction = fabs((d2 - o.start) * o.rate + 0.5 - end_bin);
-  potential = vmul(bperp, sign * t.constant * fraction / (den * path.distance));
-  add_potential_sample(o, tile, d2, potential, tally);
+  if constexpr (Exact)
+    potential = vmul(bperp, sign * t.constant * fraction / (den * path.distance));
+  else
+    potential = vmul(bperp, sign * k * fraction);
+  add_potential_sample(o, scale, tile, d2, potential, tally);
 }
 
-template <bool Fixed>
-__device__ void zhs_pair(Track const& t, PropagationDev const& p, std::uint32_t refinement_u,
-                         Obs const& o, Tile<Fixed> const& tile, Tally& tally,
+template <bool Exact, bool Fixed>
+__device__ __forceinline__ void zhs_pair(Track const& t, PropagationDev const& p, std::uint32_t refinement_u,
+                         Obs const& o, double scale, Tile<Fixed> const& tile, Tally& tally,
                          bool count_subtracks) {
   V3 const midpoint = vmul(vadd(t.start, t.end), 0.5);
-  Path const mp = propagate(p, midpoint, o);
-  double const u_k = vdot(t.beta, mp.emit) / t.beta_module;
-  double const sin2 = 1. - u_k * u_k;
-  double const fraunhofer = sin2 * t.length * t.length / mp.distance / o.wavelength * 2. * kPi;
+  Path const mp = propagate<Exact>(p, midpoint, o);
+  double u_k, fraunhofer;
+  if constexpr (Exact) {
+    u_k = vdot(t.beta, mp.emit) / t.beta_module;
+    double const sin2 = 1. - u_k * u_k;
+    fraunhofer = sin2 * t.length * t.length / mp.distance / o.wavelength * 2. * kPi;
+  } else {
+    u_k = vdot(t.beta, mp.emit) * t.inv_beta_module;
+    double const sin2 = 1. - u_k * u_k;
+    fraunhofer = sin2 * t.length * t.length * mp.inv_distance * o.two_pi_over_wavelength;
+  }
   double divisor = 1.;
   int nsub = 1;
   if (fraunhofer > 1.) {
@@ -477,25 +600,36 @@ __device__ void zhs_pair(Track const& t, PropagationDev const& p, std::uint32_t 
     if (refinement > 1)
       tally.subtracks += static_cast<unsigned long long>(nsub * (refinement - 1));
   }
-  V3 const spatial_step = vdiv(t.disp, divisor);
-  double const time_step = t.duration / divisor;
-  V3 const rs = vdiv(spatial_step, static_cast<double>(refinement));
-  double const rt = time_step / static_cast<double>(refinement);
   bool const subdivided = nsub > 1 || refinement > 1;
-  if (!subdivided) {
-    zhs_segment(t.start, vadd(t.start, rs), t.t0, t.t0 + rt, t, p, o, tile, tally, false);
-    return;
+  // Unsubdivided, divisor == refinement == 1 and x / 1. is x exactly: the
+  // single segment runs from start to start + displacement over the full
+  // duration, so the divisions are skipped. Its midpoint is the track
+  // midpoint whenever start + displacement == end bit for bit, and the
+  // propagation above is then reused unchanged.
+  V3 rs = t.disp;
+  double rt = t.duration;
+  bool use_cached = false;
+  if (subdivided) {
+    V3 const spatial_step = vdiv(t.disp, divisor);
+    double const time_step = t.duration / divisor;
+    rs = vdiv(spatial_step, static_cast<double>(refinement));
+    rt = time_step / static_cast<double>(refinement);
+  } else {
+    V3 const p2 = vadd(t.start, t.disp);
+    use_cached = __double_as_longlong(p2.x) == __double_as_longlong(t.end.x) &&
+                 __double_as_longlong(p2.y) == __double_as_longlong(t.end.y) &&
+                 __double_as_longlong(p2.z) == __double_as_longlong(t.end.z);
   }
+  int const segments = nsub * refinement;
   V3 p1 = t.start;
   double time1 = t.t0;
-  for (int i = 0; i < nsub; ++i) {
-    for (int r = 0; r < refinement; ++r) {
-      V3 const p2 = vadd(p1, rs);
-      double const time2 = time1 + rt;
-      zhs_segment(p1, p2, time1, time2, t, p, o, tile, tally, true);
-      p1 = p2;
-      time1 = time2;
-    }
+  for (int i = 0; i < segments; ++i) {
+    V3 const p2 = vadd(p1, rs);
+    double const time2 = time1 + rt;
+    zhs_segment<Exact>(p1, p2, time1, time2, t, p, o, scale, tile, tally, subdivided, use_cached,
+                       mp);
+    p1 = p2;
+    time1 = time2;
   }
 }
 
@@ -505,10 +639,10 @@ __device__ __forceinline__ Obs make_obs(ObserverDev const& d, PropagationDev con
   o.start = d.start_time;
   o.duration = d.duration;
   o.rate = d.sample_rate;
-  o.scale = d.fixed_scale;
   o.grid = 1. / d.sample_rate;
   o.wavelength = kC / d.sample_rate;
-  o.nbins = d.number_of_bins;
+  o.two_pi_over_wavelength = 2. * kPi / o.wavelength;
+  o.nbins = static_cast<std::int32_t>(d.number_of_bins);
   o.nbins_d = static_cast<double>(d.number_of_bins);
   o.offset = d.waveform_offset;
   if (p.homogeneous_index > 0.) {
@@ -554,13 +688,11 @@ __device__ void block_inclusive_scan(T* row, std::uint32_t count, T* scratch) {
   __syncthreads();
 }
 
-constexpr std::uint32_t kMaxBlock = 1024;
-
-template <bool Coreas, bool Zhs, bool Fixed>
-__global__ void __launch_bounds__(256) radio_project_kernel(ProjectionLaunch L) {
+template <bool Coreas, bool Zhs, bool Fixed, bool Exact, int Block>
+__global__ void __launch_bounds__(Block, 4) radio_project_kernel(ProjectionLaunch L) {
   using Acc = std::conditional_t<Fixed, long long, double>;
   extern __shared__ __align__(16) unsigned char lds_raw[];
-  __shared__ Acc scan_scratch[kMaxBlock];
+  __shared__ Acc scan_scratch[Zhs ? Block : 1];
   __shared__ unsigned long long block_counts[5];
   __shared__ double block_abs[2];
 
@@ -586,21 +718,20 @@ __global__ void __launch_bounds__(256) radio_project_kernel(ProjectionLaunch L) 
 
   PropagationDev const prop = L.propagation;
   std::uint64_t const tile_begin = static_cast<std::uint64_t>(bin_tile) * W;
-  Obs co{}, zo{};
-  Tile<Fixed> ctile{coreas_lds, W, tile_begin, 0};
-  Tile<Fixed> ztile{zhs_lds, W + 1, tile_begin, 0};
-  if constexpr (Coreas) {
-    co = make_obs(L.coreas_observers[observer], prop);
-    ctile.bins = tile_begin < co.nbins
-                     ? static_cast<std::uint32_t>(co.nbins - tile_begin < W ? co.nbins - tile_begin : W)
-                     : 0u;
-  }
-  if constexpr (Zhs) {
-    zo = make_obs(L.zhs_observers[observer], prop);
-    ztile.bins = tile_begin < zo.nbins
-                     ? static_cast<std::uint32_t>(zo.nbins - tile_begin < W ? zo.nbins - tile_begin : W)
-                     : 0u;
-  }
+  // A fused launch has identical CoREAS and ZHS observers (position, window,
+  // rate, bins and hence offsets); only their fixed-point scales differ.
+  ObserverDev const& view = Coreas ? L.coreas_observers[observer] : L.zhs_observers[observer];
+  Obs const o = make_obs(view, prop);
+  double const coreas_scale = Coreas ? L.coreas_observers[observer].fixed_scale : 0.;
+  double const zhs_scale = Zhs ? L.zhs_observers[observer].fixed_scale : 0.;
+  std::uint64_t const nbins = static_cast<std::uint64_t>(o.nbins);
+  std::uint32_t const tile_bins =
+      tile_begin < nbins ? static_cast<std::uint32_t>(nbins - tile_begin < W ? nbins - tile_begin : W)
+                         : 0u;
+  Tile<Fixed> const ctile{coreas_lds, W, static_cast<std::int32_t>(tile_begin),
+                          static_cast<std::int32_t>(tile_bins)};
+  Tile<Fixed> const ztile{zhs_lds, W + 1, static_cast<std::int32_t>(tile_begin),
+                          static_cast<std::int32_t>(tile_bins)};
   bool const first_observer_tile = L.count_valid && observer == 0 && bin_tile == 0;
 
   std::uint64_t const chunk_begin = static_cast<std::uint64_t>(chunk) * L.chunk_size;
@@ -609,13 +740,13 @@ __global__ void __launch_bounds__(256) radio_project_kernel(ProjectionLaunch L) 
   Tally ctally, ztally;
   for (std::uint64_t i = chunk_begin + threadIdx.x; i < chunk_end; i += blockDim.x) {
     Track t;
-    if (!load_track(L.tracks, L.stride, i, t)) continue;
+    if (!load_track<Exact>(L.tracks, L.stride, i, t)) continue;
     if (first_observer_tile) ++ctally.valid;
     if constexpr (Coreas) {
-      if (ctile.bins) coreas_pair(t, prop, co, ctile, ctally);
+      if (tile_bins) coreas_pair<Exact>(t, prop, o, coreas_scale, ctile, ctally);
     }
     if constexpr (Zhs) {
-      zhs_pair(t, prop, L.zhs_refinement, zo, ztile, ztally, bin_tile == 0);
+      zhs_pair<Exact>(t, prop, L.zhs_refinement, o, zhs_scale, ztile, ztally, bin_tile == 0);
     }
   }
 
@@ -637,8 +768,8 @@ __global__ void __launch_bounds__(256) radio_project_kernel(ProjectionLaunch L) 
   // Flush the tile into the global waveforms (checked int64 atomics, as
   // checked_radio_atomic_add; or FP64 atomics in floating mode).
   unsigned long long flush_overflows = 0;
-  auto flush = [&](Acc const* lds, std::uint32_t row, Obs const& o, std::uint32_t bins,
-                   void* global, std::uint64_t total_bins) {
+  auto flush = [&](Acc const* lds, std::uint32_t row, std::uint32_t bins, void* global,
+                   std::uint64_t total_bins) {
     Acc* const g = static_cast<Acc*>(global);
     for (std::uint32_t idx = threadIdx.x; idx < 3 * bins; idx += blockDim.x) {
       std::uint32_t const c = idx / bins;
@@ -658,8 +789,8 @@ __global__ void __launch_bounds__(256) radio_project_kernel(ProjectionLaunch L) 
       }
     }
   };
-  if constexpr (Coreas) flush(coreas_lds, W, co, ctile.bins, L.coreas_waves, L.coreas_total_bins);
-  if constexpr (Zhs) flush(zhs_lds, W + 1, zo, ztile.bins, L.zhs_waves, L.zhs_total_bins);
+  if constexpr (Coreas) flush(coreas_lds, W, tile_bins, L.coreas_waves, L.coreas_total_bins);
+  if constexpr (Zhs) flush(zhs_lds, W + 1, tile_bins, L.zhs_waves, L.zhs_total_bins);
   if (flush_overflows) atomicAdd(&block_counts[4], flush_overflows);
   __syncthreads();
 
@@ -703,10 +834,10 @@ __global__ void __launch_bounds__(256)
   tracks[8 * stride + i] = pid == C8MI_PID_ELECTRON ? -w : w;
 }
 
-template <bool Coreas, bool Zhs, bool Fixed>
+template <bool Coreas, bool Zhs, bool Fixed, bool Exact, int Block>
 void launch_typed(ProjectionLaunch const& launch, hipStream_t stream, std::size_t lds_bytes,
                   std::uint32_t grid) {
-  auto kernel = radio_project_kernel<Coreas, Zhs, Fixed>;
+  auto kernel = radio_project_kernel<Coreas, Zhs, Fixed, Exact, Block>;
   static bool attribute_set = false;
   if (!attribute_set) {
     C8MI_HIP(hipFuncSetAttribute(reinterpret_cast<void const*>(kernel),
@@ -714,8 +845,19 @@ void launch_typed(ProjectionLaunch const& launch, hipStream_t stream, std::size_
                                  static_cast<int>(kMaxDynamicLdsBytes)));
     attribute_set = true;
   }
-  hipLaunchKernelGGL(kernel, dim3(grid), dim3(kProjectionBlock), lds_bytes, stream, launch);
+  hipLaunchKernelGGL(kernel, dim3(grid), dim3(Block), lds_bytes, stream, launch);
   C8MI_HIP(hipGetLastError());
+}
+
+template <bool Coreas, bool Zhs, bool Fixed, bool Exact>
+void launch_block(ProjectionLaunch const& launch, std::uint32_t block, hipStream_t stream,
+                  std::size_t lds_bytes, std::uint32_t grid) {
+  switch (block) {
+    case 256: launch_typed<Coreas, Zhs, Fixed, Exact, 256>(launch, stream, lds_bytes, grid); break;
+    case 512: launch_typed<Coreas, Zhs, Fixed, Exact, 512>(launch, stream, lds_bytes, grid); break;
+    case 1024: launch_typed<Coreas, Zhs, Fixed, Exact, 1024>(launch, stream, lds_bytes, grid); break;
+    default: throw std::invalid_argument("radio workgroup size must be 256, 512 or 1024");
+  }
 }
 
 }  // namespace
@@ -732,21 +874,23 @@ std::uint32_t projection_grid(ProjectionLaunch const& launch) {
   return static_cast<std::uint32_t>(chunk_rounds * kXcds * per_chunk);
 }
 
+template <bool Coreas, bool Zhs>
+void launch_modes(ProjectionLaunch const& launch, bool fixed, bool exact, std::uint32_t block,
+                  hipStream_t stream, std::size_t lds, std::uint32_t grid) {
+  // FP64 (non-deterministic) accumulation is offered in the exact form only.
+  if (!fixed) launch_block<Coreas, Zhs, false, true>(launch, block, stream, lds, grid);
+  else if (exact) launch_block<Coreas, Zhs, true, true>(launch, block, stream, lds, grid);
+  else launch_block<Coreas, Zhs, true, false>(launch, block, stream, lds, grid);
+}
+
 void launch_projection(ProjectionLaunch const& launch, bool coreas, bool zhs, bool fixed,
-                       hipStream_t stream) {
+                       bool exact, std::uint32_t block, hipStream_t stream) {
   if (launch.track_count == 0 || launch.observer_count == 0) return;
   std::size_t const lds = projection_lds_bytes(coreas, zhs, launch.tile_bins);
   std::uint32_t const grid = projection_grid(launch);
-  if (coreas && zhs) {
-    if (fixed) launch_typed<true, true, true>(launch, stream, lds, grid);
-    else launch_typed<true, true, false>(launch, stream, lds, grid);
-  } else if (coreas) {
-    if (fixed) launch_typed<true, false, true>(launch, stream, lds, grid);
-    else launch_typed<true, false, false>(launch, stream, lds, grid);
-  } else if (zhs) {
-    if (fixed) launch_typed<false, true, true>(launch, stream, lds, grid);
-    else launch_typed<false, true, false>(launch, stream, lds, grid);
-  }
+  if (coreas && zhs) launch_modes<true, true>(launch, fixed, exact, block, stream, lds, grid);
+  else if (coreas) launch_modes<true, false>(launch, fixed, exact, block, stream, lds, grid);
+  else if (zhs) launch_modes<false, true>(launch, fixed, exact, block, stream, lds, grid);
 }
 
 void launch_records_to_tracks(c8mi_lepton_transport_record const* records, std::uint64_t count,

@@ -11,7 +11,8 @@
 
 namespace c8mi::radio {
 
-// Threads per projection workgroup (one observer x one track chunk).
+// Default threads per projection workgroup (one observer x one track chunk);
+// 256, 512 and 1024 are instantiated.
 inline constexpr std::uint32_t kProjectionBlock = 256;
 // Dynamic LDS a projection workgroup may request (gfx950: 160 KiB per CU).
 inline constexpr std::size_t kMaxDynamicLdsBytes = 144 * 1024;
@@ -21,8 +22,10 @@ std::size_t projection_lds_bytes(bool coreas, bool zhs, std::uint32_t tile_bins)
 // Workgroups launched for `launch` (chunks rounded up to whole XCD rounds).
 std::uint32_t projection_grid(ProjectionLaunch const& launch);
 
+// exact: the reference's FP64 operation order (bit-identical fixed-point
+// sums); otherwise the fast FP64 form. Floating accumulation is always exact.
 void launch_projection(ProjectionLaunch const& launch, bool coreas, bool zhs, bool fixed,
-                       hipStream_t stream);
+                       bool exact, std::uint32_t block, hipStream_t stream);
 
 void launch_records_to_tracks(c8mi_lepton_transport_record const* records,
                               std::uint64_t count, double* tracks, std::uint64_t stride,

@@ -75,15 +75,19 @@ int c8mi_radio_download(c8mi_radio* radio, double* coreas, double* zhs,
 int c8mi_radio_download_fixed(c8mi_radio* radio, int64_t* coreas, int64_t* zhs,
                               c8mi_radio_counters* counters);
 
-/* TuningParameters (track/observer tiles, team size) analogue: tracks per
- * workgroup (multiple of 256, 0 = automatic), LDS window tile in bins
- * (0 = automatic) and target workgroups per launch (0 = default 2048). */
+/* TuningParameters (track/observer tiles, team size; TuningParameters.hpp)
+ * analogue: tracks per workgroup (multiple of the workgroup size,
+ * 0 = automatic), LDS window tile in bins (0 = automatic), target workgroups
+ * per launch (0 = default 8192), threads per workgroup (256, 512 or 1024;
+ * 0 = default 512) and split: 1 runs fusable CoREAS+ZHS work as two launches
+ * (the default), 0 as one fused launch. Results do not depend on any of these. */
 int c8mi_radio_set_tuning(c8mi_radio* radio, uint64_t chunk_tracks, uint32_t tile_bins,
-                          uint32_t target_workgroups);
+                          uint32_t target_workgroups, uint32_t block_threads, int split);
 
-/* Projection-kernel timing with HIP events on the handle's stream. */
+/* Projection-kernel timing with HIP events on the handle's stream. Times and
+ * launch counts per kind: [0] fused CoREAS+ZHS, [1] CoREAS only, [2] ZHS only. */
 int c8mi_radio_enable_kernel_timing(c8mi_radio* radio, int enable);
-int c8mi_radio_kernel_time(c8mi_radio* radio, double* total_ms, uint64_t* launches);
+int c8mi_radio_kernel_time(c8mi_radio* radio, double ms[3], uint64_t launches[3]);
 
 /* The straight-ray table arithmetic of make_radio_config
  * (corsika/detail/kokkos/radio/atmosphere/FlatAtmosphereConfig.inl:31-73):

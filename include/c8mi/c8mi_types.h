@@ -161,6 +161,12 @@ typedef struct c8mi_radio_config {
   int32_t zhs_enabled;
   int32_t deterministic;             /* 1: int64 fixed point (production default) */
   uint32_t zhs_subtrack_refinement;  /* 1..64, default 1 */
+  /* 1: evaluate in the reference's FP64 operation order, so the fixed-point
+   * waveforms are bit-identical to its scalar projection. 0 (default): FP64
+   * with hardware reciprocal / reciprocal-square-root refinement; field error
+   * against the FP64 result below 1e-6 relative. */
+  int32_t exact_fp64;
+  int32_t reserved0;
   double fixed_point_field_limit_v_per_m; /* default 1 V/m */
   c8mi_radio_propagation propagation;
   const c8mi_radio_observer* coreas_observers;
