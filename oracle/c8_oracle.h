@@ -15,6 +15,7 @@
 #include <stddef.h>
 #include <stdint.h>
 
+#include "c8mi/c8mi_tables.h"
 #include "c8mi/c8mi_types.h"
 
 #ifdef __cplusplus
@@ -86,6 +87,21 @@ void oracle_radio_separate_same_bin(double* start_time, double* end_time,
 void oracle_radio_decode(const int64_t* fixed, uint64_t total_bins,
                          const c8mi_radio_observer* observers, uint64_t observer_count,
                          double fixed_point_limit, int zhs_potential, double* out);
+
+/* ---------------- Philox (Philox.inl:15-57) ---------------- */
+void oracle_philox4x32_10(const uint32_t counter[4], const uint32_t key[2], uint32_t out[4]);
+void oracle_random_words(uint64_t seed, uint64_t shower, uint64_t history, uint64_t step,
+                         uint32_t process, uint64_t draw, uint32_t out[4]);
+double oracle_uniform_open01(uint64_t seed, uint64_t shower, uint64_t history, uint64_t step,
+                             uint32_t process, uint64_t draw, uint32_t lane);
+
+/* ---------------- native tables (ProposalNativeQueries.inl) ---------------- */
+void oracle_native_queries(const c8mi_native_table_set* table, const c8mi_native_query* queries,
+                           uint64_t count, c8mi_native_result* out);
+double oracle_native_physical_loss(const c8mi_native_table_set* table, uint32_t column,
+                                   double energy_mev, double transformed_loss);
+double oracle_native_transformed_loss(const c8mi_native_table_set* table, uint32_t column,
+                                      double energy_mev, double physical_loss);
 
 #ifdef __cplusplus
 }

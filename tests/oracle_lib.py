@@ -120,3 +120,61 @@ def radio_decode(fixed, observers, limit=1.0, potential=False):
                                obs.ctypes.data_as(c_void_p), c_uint64(obs.size), c_double(limit),
                                c_int(1 if potential else 0), _d(out))
     return out
+
+
+# ---- EM: Philox and native tables -------------------------------------------
+
+REF_PATH = os.path.join(ORACLE_DIR, "_ref", "libref.so")
+_REF = None
+
+
+def load_ref():
+    """The reference's own header functions compiled in place (oracle/ref.mk);
+    None when the library was not built (no reference tree at build time)."""
+    global _REF
+    if _REF is None and os.path.exists(REF_PATH):
+        lib = ctypes.CDLL(REF_PATH)
+        lib.ref_uniform_open01.restype = c_double
+        lib.ref_uniform_open01.argtypes = [c_uint64, c_uint64, c_uint64, c_uint64, c_uint32,
+                                           c_uint64, c_uint32]
+        _REF = lib
+    return _REF
+
+
+def philox(counter, key, lib=None):
+    lib = lib or load()
+    c = (c_uint32 * 4)(*counter)
+    k = (c_uint32 * 2)(*key)
+    out = (c_uint32 * 4)()
+    fn = lib.oracle_philox4x32_10 if lib is _LIB else lib.ref_philox4x32_10
+    fn(c, k, out)
+    return list(out)
+
+
+def random_words(seed, shower, history, step, process, draw, lib=None):
+    lib = lib or load()
+    out = (c_uint32 * 4)()
+    fn = lib.oracle_random_words if lib is _LIB else lib.ref_random_words
+    fn(c_uint64(seed), c_uint64(shower), c_uint64(history), c_uint64(step), c_uint32(process),
+       c_uint64(draw), out)
+    return list(out)
+
+
+def uniform_open01(seed, shower, history, step, process, draw, lane=0):
+    lib = load()
+    lib.oracle_uniform_open01.restype = c_double
+    return lib.oracle_uniform_open01(c_uint64(seed), c_uint64(shower), c_uint64(history),
+                                     c_uint64(step), c_uint32(process), c_uint64(draw),
+                                     c_uint32(lane))
+
+
+def native_queries(tables, queries, reference=False):
+    """Runs the oracle (or the reference probe) on a c8mi.tables.NativeTables."""
+    from c8mi.tables import RESULT_DTYPE
+
+    lib = load_ref() if reference else load()
+    out = np.zeros(queries.size, dtype=RESULT_DTYPE)
+    fn = lib.ref_native_queries if reference else lib.oracle_native_queries
+    fn(byref(tables.struct), queries.ctypes.data_as(c_void_p), c_uint64(queries.size),
+       out.ctypes.data_as(c_void_p))
+    return out
