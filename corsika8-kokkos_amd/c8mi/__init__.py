@@ -309,3 +309,70 @@ class RadioAccumulator:
         ms, launches = (c_double * 3)(), (c_uint64 * 3)()
         _check(library().c8mi_radio_kernel_time(self.handle, ms, launches))
         return {kind: (ms[k], launches[k]) for k, kind in enumerate(("coreas_zhs", "coreas", "zhs"))}
+
+
+# ---- EM: device tables, batched queries, Philox (c8mi_em.h) -----------------
+
+RANDOM_KEY_DTYPE = np.dtype(
+    {
+        "names": ["seed", "shower_id", "history_id", "step_id", "process_id", "draw_id"],
+        "formats": ["<u8", "<u8", "<u8", "<u8", "<u4", "<u8"],
+        "offsets": [0, 8, 16, 24, 32, 40],
+        "itemsize": 48,
+    }
+)
+
+
+class EmTables:
+    """Device copy of a format-v7 native table set (c8mi_em_tables)."""
+
+    def __init__(self, tables):
+        """tables: a c8mi.tables.NativeTables (host arrays + ctypes struct)."""
+        self.host = tables
+        self.handle = c_void_p()
+        _check(library().c8mi_em_tables_create(byref(tables.struct), byref(self.handle)))
+
+    def close(self) -> None:
+        if self.handle:
+            _check(library().c8mi_em_tables_destroy(self.handle))
+            self.handle = c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def bytes(self) -> int:
+        n = c_uint64()
+        _check(library().c8mi_em_tables_bytes(self.handle, byref(n)))
+        return n.value
+
+    def query(self, queries: np.ndarray) -> np.ndarray:
+        """Runs a batch of c8mi_native_query records on the device."""
+        from .tables import QUERY_DTYPE, RESULT_DTYPE
+
+        queries = np.ascontiguousarray(queries, dtype=QUERY_DTYPE)
+        count = queries.size
+        if count == 0:
+            return np.zeros(0, dtype=RESULT_DTYPE)
+        dq = DeviceArray.from_host(queries)
+        dr = DeviceArray(count * RESULT_DTYPE.itemsize)
+        _check(library().c8mi_em_native_queries(self.handle, dq.pointer, c_uint64(count),
+                                                dr.pointer, None))
+        device_synchronize()
+        return dr.to_host(RESULT_DTYPE, count)
+
+
+def random_draws(keys: np.ndarray):
+    """(words [n,4] uint32, uniforms [n,4] float64) for a batch of random keys."""
+    keys = np.ascontiguousarray(keys, dtype=RANDOM_KEY_DTYPE)
+    n = keys.size
+    if n == 0:
+        return np.zeros((0, 4), np.uint32), np.zeros((0, 4))
+    dk = DeviceArray.from_host(keys)
+    dw = DeviceArray(16 * n)
+    du = DeviceArray(32 * n)
+    _check(library().c8mi_em_random(dk.pointer, c_uint64(n), dw.pointer, du.pointer, None))
+    device_synchronize()
+    return dw.to_host(np.uint32, (n, 4)), du.to_host(np.float64, (n, 4))

@@ -345,7 +345,7 @@ def random_table_set(seed: int = 1, polynomial_every: int = 3, noise: float = 0.
         g = g0 if wiggle == 0.0 else (lambda t, g0=g0, w=wiggle, m=n % 3 + 2:
                                       g0(t) + w * np.sin(2 * math.pi * m * np.asarray(t) + 0.7))
         spline = b.bicubic(ea, la, lambda e, t, s=scale, g=g: s * (1.0 + 0.2 * np.log(e)) * g(t),
-                           polynomial=(n % polynomial_every == 0), noise=noise or None)
+                           polynomial=(polynomial_every > 0 and n % polynomial_every == 0), noise=noise or None)
         b.add_dndx(pdg_id=pdg, process_id=process, component_hash=comp, kinematic_model=KIN[kin],
                    loss_transform=lt, rate_model=rate, spline=spline,
                    component_weight=rng.uniform(0.5, 1.5), particle_mass_mev=mass,

@@ -10,6 +10,7 @@
 
 #include "c8mi/c8mi_radio.h"
 #include "radio/radio_accumulator.hpp"
+#include "capi_guard.hpp"
 #include "runtime.hpp"
 
 struct c8mi_radio {
@@ -25,39 +26,6 @@ thread_local std::string g_last_error;
 
 void set_last_error(std::string message) { g_last_error = std::move(message); }
 char const* last_error() { return g_last_error.c_str(); }
-
-template <class F>
-int guarded(F&& body) {
-  try {
-    body();
-    set_last_error("");
-    return kOk;
-  } catch (OverflowError const& e) {
-    set_last_error(e.what());
-    return kOverflowError;
-  } catch (DeviceError const& e) {
-    set_last_error(e.what());
-    return kDeviceError;
-  } catch (std::invalid_argument const& e) {
-    set_last_error(e.what());
-    return kInvalidArgument;
-  } catch (std::length_error const& e) {
-    set_last_error(e.what());
-    return kLengthError;
-  } catch (std::logic_error const& e) {
-    set_last_error(e.what());
-    return kLogicError;
-  } catch (std::bad_alloc const&) {
-    set_last_error("host allocation failed");
-    return kRuntimeError;
-  } catch (std::exception const& e) {
-    set_last_error(e.what());
-    return kRuntimeError;
-  } catch (...) {
-    set_last_error("unknown error");
-    return kRuntimeError;
-  }
-}
 
 }  // namespace c8mi
 

@@ -98,3 +98,61 @@ def test_record_dtype_matches_c_layout():
     assert c8mi.LEPTON_TRANSPORT_RECORD_DTYPE.fields["start"][1] == 320
     assert c8mi.LEPTON_TRANSPORT_RECORD_DTYPE.fields["end"][1] == 448
     assert ctypes.sizeof(c8mi.RadioObserver) == c8mi.OBSERVER_DTYPE.itemsize == 56
+
+
+# ---- EM tables: bounds validation runs before any device call ---------------
+
+def _bicubic_column(t):
+    from c8mi import tables
+    return int(np.flatnonzero(t.dndx["rate_model"] == tables.RATE_BICUBIC)[0])
+
+
+def _break_bicubic_offset(t):
+    t.dndx["spline"]["coefficient_offset"][_bicubic_column(t)] = t.bicubic_values.size
+
+
+def _break_bicubic_rows(t):
+    t.dndx["spline"]["rows"][_bicubic_column(t)] += 1
+
+
+def _break_selection(t):
+    t.selection[0] = t.dndx.size
+
+
+def _break_cubic(t):
+    t.totals["spline"]["coefficient_count"][0] = t.cubic_values.size + 1
+
+
+def _break_order(t):
+    t.dndx[[0, 1]] = t.dndx[[1, 0]]
+
+
+def _break_dedx_index(t):
+    t.dedx_order[0] = t.dedx.size
+
+
+def _break_polynomial(t):
+    i = int(np.flatnonzero(t.dndx["spline"]["polynomial_coefficient_offset"] != 2**64 - 1)[0])
+    t.dndx["spline"]["polynomial_coefficient_offset"][i] = t.polynomial.size - 8
+
+
+@pytest.mark.parametrize("breaker", [_break_bicubic_offset, _break_bicubic_rows, _break_selection,
+                                     _break_cubic, _break_order, _break_dedx_index,
+                                     _break_polynomial])
+def test_em_tables_reject_out_of_bounds_descriptors(c8mi_lib, breaker):
+    import c8mi
+    from c8mi import tables
+
+    t = tables.random_table_set(1)
+    breaker(t)
+    with pytest.raises(c8mi.C8miError) as err:
+        c8mi.EmTables(t)
+    assert err.value.status == 1, err.value.message
+    assert "native tables" in err.value.message
+
+
+def test_random_key_dtype_matches_c_layout():
+    import c8mi
+
+    assert c8mi.RANDOM_KEY_DTYPE.itemsize == 48
+    assert c8mi.RANDOM_KEY_DTYPE.fields["draw_id"][1] == 40
