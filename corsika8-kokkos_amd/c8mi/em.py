@@ -1,0 +1,430 @@
+"""ctypes layer of the EM backend (include/c8mi/c8mi_em.h) and the host-side
+builders of its inputs: the layered spherical atmosphere, the profile
+projection along the shower axis and the backend configuration.
+
+The device path is libc8mi.so; this module marshals arguments and decodes
+results only.
+"""
+from __future__ import annotations
+
+import ctypes
+import math
+from ctypes import POINTER, byref, c_double, c_int, c_int32, c_uint32, c_uint64, c_void_p
+
+import numpy as np
+
+from . import (PARTICLE_STATE_DTYPE, _check, library)
+
+EARTH_RADIUS_M = 6371000.0
+C_LIGHT = 299792458.0
+
+# USStdBK (corsika/media/CORSIKA7Atmospheres.hpp:182-186): upper altitude [m],
+# b [g/cm^2], c [cm]; the last layer is linear in the overburden (homogeneous).
+USSTDBK = [(7000.0, 1183.6071, 954248.34), (11400.0, 1143.0425, 800005.34),
+           (37000.0, 1322.9748, 629568.93), (100000.0, 655.67307, 737521.77),
+           (112800.0, 1.0, 1e9)]
+
+DENSITY_HOMOGENEOUS, DENSITY_SLIDING, DENSITY_FLAT = 1, 2, 3
+SHAPE_UNIVERSE, SHAPE_SPHERE = 1, 2
+
+
+class EnvModel(ctypes.Structure):
+    _fields_ = [("density_kind", c_uint32), ("medium_id", c_int32),
+                ("reference_density_g_per_cm3", c_double), ("scale_length_m", c_double),
+                ("inverse_scale_length_per_m", c_double), ("reference_height_m", c_double),
+                ("anchor_m", c_double * 3), ("axis", c_double * 3),
+                ("magnetic_field_t", c_double * 3)]
+
+
+class EnvNode(ctypes.Structure):
+    _fields_ = [("shape_kind", c_uint32), ("parent", c_int32), ("model", c_int32),
+                ("child_begin", c_uint32), ("child_count", c_uint32), ("pad_", c_uint32),
+                ("center_m", c_double * 3), ("radius_m", c_double)]
+
+
+class Environment(ctypes.Structure):
+    _fields_ = [("nodes", POINTER(EnvNode)), ("node_count", c_uint32),
+                ("child_index_count", c_uint32), ("children", POINTER(c_int32)),
+                ("models", POINTER(EnvModel)), ("model_count", c_uint32),
+                ("observation_enabled", c_uint32), ("observation_point_m", c_double * 3),
+                ("observation_normal", c_double * 3), ("earth_center_m", c_double * 3),
+                ("maximum_magnetic_deflection_rad", c_double)]
+
+
+class PairLpmComponent(ctypes.Structure):
+    _fields_ = [("component_hash", c_uint64), ("nuclear_charge", c_double),
+                ("radiation_log_constant", c_double)]
+
+
+class PairLpm(ctypes.Structure):
+    _fields_ = [("baseline_mass_density_g_per_cm3", c_double), ("molecular_density_per_cm3", c_double),
+                ("sum_charge", c_double), ("e_lpm_mev", c_double),
+                ("classical_electron_radius_cm", c_double), ("fine_structure_constant", c_double),
+                ("component_count", c_uint32), ("reserved", c_uint32),
+                ("components", PairLpmComponent * 16)]
+
+
+class BremsLpmComponent(ctypes.Structure):
+    _fields_ = [("component_hash", c_uint64), ("nuclear_charge", c_double),
+                ("atomic_mass_number", c_double), ("radiation_log_constant", c_double)]
+
+
+class BremsLpm(ctypes.Structure):
+    _fields_ = [("baseline_mass_density_g_per_cm3", c_double), ("molecular_density_per_cm3", c_double),
+                ("sum_charge", c_double), ("e_lpm_mev", c_double), ("lepton_mass_mev", c_double),
+                ("electron_mass_mev", c_double), ("muon_mass_mev", c_double),
+                ("classical_electron_radius_cm", c_double), ("fine_structure_constant", c_double),
+                ("component_count", c_uint32), ("reserved", c_uint32),
+                ("components", BremsLpmComponent * 16),
+                ("muon_baseline_mass_density_g_per_cm3", c_double),
+                ("muon_molecular_density_per_cm3", c_double), ("muon_sum_charge", c_double),
+                ("muon_e_lpm_mev", c_double)]
+
+
+class Thinning(ctypes.Structure):
+    _fields_ = [("threshold_gev", c_double), ("maximum_weight", c_double), ("enabled", c_uint32),
+                ("erase_zero_weight", c_uint32)]
+
+
+class EmConfig(ctypes.Structure):
+    _fields_ = [("random_seed", c_uint64), ("em_transport_cut_mev", c_double),
+                ("muon_transport_cut_mev", c_double), ("queue_capacity", c_uint64),
+                ("observation_capacity", c_uint64), ("fallback_capacity", c_uint64),
+                ("photon_pair_lpm", PairLpm), ("brems_lpm", BremsLpm), ("thinning", Thinning),
+                ("profile_bins", c_uint32), ("profile_crossing_mode", c_uint32),
+                ("profile_bin_width_g_per_cm2", c_double),
+                ("profile_energy_loss_threshold_g_per_cm2", c_double),
+                ("profile_weight_scale", c_double), ("profile_energy_scale", c_double),
+                ("axis_start_m", c_double * 3), ("axis_direction", c_double * 3),
+                ("axis_step_m", c_double), ("axis_grammage_g_per_cm2", POINTER(c_double)),
+                ("axis_support_count", c_uint64)]
+
+
+class RunResult(ctypes.Structure):
+    _fields_ = [(k, c_uint64) for k in (
+        "photon_fronts", "lepton_fronts", "photon_steps", "lepton_steps", "photon_interactions",
+        "lepton_interactions", "secondaries", "observations", "fallbacks", "pending_photons",
+        "pending_leptons", "next_history_id", "max_front")]
+
+    def as_dict(self):
+        return {k: int(getattr(self, k)) for k, _ in self._fields_}
+
+
+LEDGER_ENERGY_TERMS = ("injected_energy", "continuous_deposit", "cut_deposit", "cut_rest_mass",
+                       "observed_energy", "escaped_energy", "observation_cut_overlap",
+                       "mass_convention_correction", "medium_rest_mass_input",
+                       "photoelectric_binding", "fallback_energy", "thinning_discarded_energy",
+                       "pending_energy")
+
+
+class Ledger(ctypes.Structure):
+    _fields_ = ([(k, c_double) for k in LEDGER_ENERGY_TERMS] +
+                [("steps", c_uint64), ("photon_cuts", c_uint64), ("lepton_limits", c_uint64 * 8),
+                 ("thinning_hillas_vertices", c_uint64), ("thinning_statistical_vertices", c_uint64),
+                 ("thinning_particles_discarded", c_uint64), ("deposited_steps", c_uint64),
+                 ("fixed_point_overflows", c_uint64), ("invalid_records", c_uint64),
+                 ("fallback_reasons", c_uint64 * 32)])
+
+    def as_dict(self):
+        out = {}
+        for k, _ in self._fields_:
+            v = getattr(self, k)
+            out[k] = list(v) if hasattr(v, "__len__") else v
+        return out
+
+
+FALLBACK_DTYPE = np.dtype({"names": ["particle", "reason", "process_id", "stage"],
+                           "formats": [PARTICLE_STATE_DTYPE, "<i4", "<i4", "<u4"],
+                           "offsets": [0, 128, 132, 136], "itemsize": 144})
+
+# Counter layout of the device ledger (and of the oracle's counter array).
+COUNTER_INDEX = {name: k for k, name in enumerate(LEDGER_ENERGY_TERMS)}
+COUNTER_INDEX.update(steps=13, photon_cuts=14, lepton_limit0=15, thin_hillas=23,
+                     thin_statistical=24, thin_discarded=25, deposited_steps=26, overflows=27,
+                     invalid=28, fallback_reason0=29)
+COUNTERS = 64
+
+
+def ledger_residual(ledger: dict) -> float:
+    """Energy closure of a shower (GeV): injected minus every sink, with the
+    native/transport mass convention, atomic-electron rest mass, photoelectric
+    binding and thinning terms (ShowerReport.inl:1236-1276)."""
+    sinks = (ledger["continuous_deposit"] + ledger["cut_deposit"] + ledger["cut_rest_mass"] +
+             ledger["observed_energy"] + ledger["escaped_energy"] - ledger["observation_cut_overlap"] +
+             ledger["fallback_energy"] + ledger["pending_energy"] + ledger["thinning_discarded_energy"] +
+             ledger["photoelectric_binding"] - ledger["medium_rest_mass_input"] -
+             ledger["mass_convention_correction"])
+    return ledger["injected_energy"] - sinks
+
+
+class Atmosphere:
+    """Layered spherical atmosphere (LayeredSphericalAtmosphereBuilder) as an
+    environment tree: universe > outermost layer > ... > innermost layer; each
+    layer a sphere around the Earth centre with an exponential (sliding planar)
+    or homogeneous medium model."""
+
+    def __init__(self, layers=USSTDBK, observation_height_m: float = 0.0,
+                 earth_radius_m: float = EARTH_RADIUS_M, observation: bool = True):
+        self.earth_center = np.array([0.0, 0.0, -earth_radius_m])
+        n = len(layers)
+        self.models = (EnvModel * n)()
+        self.nodes = (EnvNode * (n + 1))()
+        self.children = (c_int32 * n)()
+        self.nodes[0].shape_kind = SHAPE_UNIVERSE
+        self.nodes[0].parent = -1
+        self.nodes[0].model = -1
+        # node k (1..n) is layer n-k (outermost first)
+        for k in range(1, n + 1):
+            li = n - k
+            top, b, c = layers[li]
+            m = self.models[k - 1]
+            rho0 = b / c
+            if li == n - 1:
+                m.density_kind = DENSITY_HOMOGENEOUS
+                m.reference_density_g_per_cm3 = rho0
+            else:
+                m.density_kind = DENSITY_SLIDING
+                m.reference_density_g_per_cm3 = rho0
+                scale = -c / 100.0
+                m.scale_length_m = scale
+                m.inverse_scale_length_per_m = 1.0 / scale
+                m.reference_height_m = earth_radius_m
+            for a in range(3):
+                m.anchor_m[a] = self.earth_center[a]
+            m.axis[2] = 1.0
+            m.medium_id = 1
+            node = self.nodes[k]
+            node.shape_kind = SHAPE_SPHERE
+            node.parent = k - 1
+            node.model = k - 1
+            for a in range(3):
+                node.center_m[a] = self.earth_center[a]
+            node.radius_m = earth_radius_m + top
+            self.children[k - 1] = k
+            self.nodes[k - 1].child_begin = k - 1
+            self.nodes[k - 1].child_count = 1
+        self.nodes[n].child_count = 0
+        self.struct = Environment()
+        self.struct.nodes = self.nodes
+        self.struct.node_count = n + 1
+        self.struct.child_index_count = n
+        self.struct.children = self.children
+        self.struct.models = self.models
+        self.struct.model_count = n
+        self.struct.observation_enabled = 1 if observation else 0
+        self.struct.observation_point_m[2] = observation_height_m
+        self.struct.observation_normal[2] = 1.0
+        for a in range(3):
+            self.struct.earth_center_m[a] = self.earth_center[a]
+        self.struct.maximum_magnetic_deflection_rad = 0.2
+        self.layers = layers
+        self.earth_radius = earth_radius_m
+
+    def density(self, p):
+        """Density (g/cm^3) at points p [..., 3] (host reference for the projection)."""
+        p = np.asarray(p, dtype=np.float64)
+        h = np.linalg.norm(p - self.earth_center, axis=-1) - self.earth_radius
+        rho = np.zeros(h.shape)
+        lower = -np.inf
+        for k, (top, b, c) in enumerate(self.layers):
+            sel = (h >= lower) & (h < top)
+            if k == len(self.layers) - 1:
+                rho[sel] = b / c
+            else:
+                rho[sel] = b / c * np.exp(-h[sel] * 100.0 / c)
+            lower = top
+        return rho
+
+    def axis_table(self, start, direction, step_m=10.0, length_m=None):
+        """Grammage along a straight axis (trapezoidal integration on 1 m cells)."""
+        start = np.asarray(start, dtype=np.float64)
+        d = np.asarray(direction, dtype=np.float64)
+        d = d / np.linalg.norm(d)
+        if length_m is None:
+            # to the observation plane
+            length_m = (start[2] - self.struct.observation_point_m[2]) / max(-d[2], 1e-9)
+        n = int(math.ceil(length_m / step_m)) + 2
+        fine = 10
+        s = np.arange(n * fine + 1) * (step_m / fine)
+        rho = self.density(start[None, :] + s[:, None] * d[None, :])
+        cum = np.concatenate(([0.0], np.cumsum(0.5 * (rho[1:] + rho[:-1]) * np.diff(s)))) * 100.0
+        return np.ascontiguousarray(cum[::fine][:n])
+
+
+def particle(pid, energy_gev, position, direction, history_id=1, weight=1.0, node=-3):
+    p = np.zeros((), dtype=PARTICLE_STATE_DTYPE)
+    p["pid"], p["energy_gev"] = pid, energy_gev
+    p["position_m"], p["direction"] = position, direction
+    p["weight"], p["history_id"], p["environment_node"] = weight, history_id, node
+    return p
+
+
+def make_config(tables_module, seed=1, cut_mev=1.0, start=(0.0, 0.0, 112750.0),
+                direction=(0.0, 0.0, -1.0), atmosphere=None, bins=128, bin_width=10.0,
+                queue_capacity=1 << 22, observation_capacity=1 << 20, fallback_capacity=1 << 16,
+                thinning=None, crossing_mode=0):
+    """EmConfig + the arrays it points to (kept alive on the returned object)."""
+    cfg = EmConfig()
+    cfg.random_seed = seed
+    cfg.em_transport_cut_mev = cut_mev
+    cfg.muon_transport_cut_mev = cut_mev
+    cfg.queue_capacity = queue_capacity
+    cfg.observation_capacity = observation_capacity
+    cfg.fallback_capacity = fallback_capacity
+    s = tables_module.photon_pair_lpm_snapshot()
+    for k in ("baseline_mass_density_g_per_cm3", "molecular_density_per_cm3", "sum_charge",
+              "e_lpm_mev", "classical_electron_radius_cm", "fine_structure_constant"):
+        setattr(cfg.photon_pair_lpm, k, s[k])
+    cfg.photon_pair_lpm.component_count = len(s["components"])
+    for i, (h, z, r) in enumerate(s["components"]):
+        cfg.photon_pair_lpm.components[i].component_hash = h
+        cfg.photon_pair_lpm.components[i].nuclear_charge = z
+        cfg.photon_pair_lpm.components[i].radiation_log_constant = r
+    s = tables_module.brems_lpm_snapshot()
+    for k in ("baseline_mass_density_g_per_cm3", "molecular_density_per_cm3", "sum_charge",
+              "e_lpm_mev", "lepton_mass_mev", "electron_mass_mev", "muon_mass_mev",
+              "classical_electron_radius_cm", "fine_structure_constant"):
+        setattr(cfg.brems_lpm, k, s[k])
+    cfg.brems_lpm.component_count = len(s["components"])
+    for i, (h, z, a, r) in enumerate(s["components"]):
+        c = cfg.brems_lpm.components[i]
+        c.component_hash, c.nuclear_charge, c.atomic_mass_number, c.radiation_log_constant = h, z, a, r
+    if thinning is not None:
+        cfg.thinning.threshold_gev, cfg.thinning.maximum_weight = thinning
+        cfg.thinning.enabled = 1
+    cfg.thinning.erase_zero_weight = 1
+    cfg.profile_bins = bins
+    cfg.profile_crossing_mode = crossing_mode
+    cfg.profile_bin_width_g_per_cm2 = bin_width
+    cfg.profile_energy_loss_threshold_g_per_cm2 = 1e-3
+    cfg.profile_weight_scale = float(1 << 24)
+    cfg.profile_energy_scale = float(1 << 40)
+    atmosphere = atmosphere or Atmosphere()
+    table = atmosphere.axis_table(start, direction)
+    d = np.asarray(direction, dtype=np.float64)
+    d = d / np.linalg.norm(d)
+    for a in range(3):
+        cfg.axis_start_m[a] = start[a]
+        cfg.axis_direction[a] = d[a]
+    cfg.axis_step_m = 10.0
+    cfg.axis_grammage_g_per_cm2 = table.ctypes.data_as(POINTER(c_double))
+    cfg.axis_support_count = table.size
+
+    class Holder:
+        pass
+
+    h = Holder()
+    h.struct, h.axis_table, h.atmosphere = cfg, table, atmosphere
+    return h
+
+
+def decode_counters(counters, energy_scale):
+    counters = np.asarray(counters, dtype=np.int64)
+    out = {name: float(counters[k]) / energy_scale for k, name in enumerate(LEDGER_ENERGY_TERMS)}
+    ci = COUNTER_INDEX
+    out["steps"] = int(counters[ci["steps"]])
+    out["photon_cuts"] = int(counters[ci["photon_cuts"]])
+    out["lepton_limits"] = [int(x) for x in counters[ci["lepton_limit0"]:ci["lepton_limit0"] + 8]]
+    out["thinning_hillas_vertices"] = int(counters[ci["thin_hillas"]])
+    out["thinning_statistical_vertices"] = int(counters[ci["thin_statistical"]])
+    out["thinning_particles_discarded"] = int(counters[ci["thin_discarded"]])
+    out["deposited_steps"] = int(counters[ci["deposited_steps"]])
+    out["fixed_point_overflows"] = int(counters[ci["overflows"]])
+    out["invalid_records"] = int(counters[ci["invalid"]])
+    out["fallback_reasons"] = [int(x) for x in counters[ci["fallback_reason0"]:ci["fallback_reason0"] + 32]]
+    return out
+
+
+class EmBackend:
+    """Binding of the c8mi EM handle (c8mi_em.h; reference IEmBackend)."""
+
+    def __init__(self, config, tables, environment, stream=None):
+        self.config = config
+        self.tables = tables
+        self.environment = environment
+        cfg = config.struct if hasattr(config, "struct") else config
+        env = environment.struct if hasattr(environment, "struct") else environment
+        self.bins = cfg.profile_bins
+        self.observation_capacity = cfg.observation_capacity
+        self.fallback_capacity = cfg.fallback_capacity
+        self.handle = c_void_p()
+        _check(library().c8mi_em_create(byref(cfg), byref(tables.struct), byref(env), c_void_p(stream),
+                                        byref(self.handle)))
+
+    def close(self):
+        if self.handle:
+            _check(library().c8mi_em_destroy(self.handle))
+            self.handle = c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def begin_shower(self, shower_id=1, first_history_id=2):
+        _check(library().c8mi_em_begin_shower(self.handle, c_uint64(shower_id), c_uint64(first_history_id)))
+
+    def can_transport(self, particle_state) -> bool:
+        p = np.ascontiguousarray(particle_state, dtype=PARTICLE_STATE_DTYPE).reshape(-1)[:1]
+        out = c_int(0)
+        _check(library().c8mi_em_can_transport(self.handle, p.ctypes.data_as(c_void_p), byref(out)))
+        return bool(out.value)
+
+    def enqueue(self, particles):
+        p = np.ascontiguousarray(particles, dtype=PARTICLE_STATE_DTYPE).reshape(-1)
+        _check(library().c8mi_em_enqueue(self.handle, p.ctypes.data_as(c_void_p), c_uint64(p.size)))
+
+    def run(self, maximum_fronts=1 << 40) -> dict:
+        r = RunResult()
+        _check(library().c8mi_em_run(self.handle, c_uint64(maximum_fronts), byref(r)))
+        return r.as_dict()
+
+    def run_photon_wavefront(self, particles, maximum_fronts=16) -> dict:
+        p = np.ascontiguousarray(particles, dtype=PARTICLE_STATE_DTYPE).reshape(-1)
+        r = RunResult()
+        _check(library().c8mi_em_run_photon_wavefront(self.handle, p.ctypes.data_as(c_void_p),
+                                                      c_uint64(p.size), c_uint64(maximum_fronts), byref(r)))
+        return r.as_dict()
+
+    def run_lepton_wavefront(self, particles, maximum_fronts=1024) -> dict:
+        p = np.ascontiguousarray(particles, dtype=PARTICLE_STATE_DTYPE).reshape(-1)
+        r = RunResult()
+        _check(library().c8mi_em_run_lepton_wavefront(self.handle, p.ctypes.data_as(c_void_p),
+                                                      c_uint64(p.size), c_uint64(maximum_fronts), byref(r)))
+        return r.as_dict()
+
+    def pending(self):
+        a, b = c_uint64(), c_uint64()
+        _check(library().c8mi_em_pending_counts(self.handle, byref(a), byref(b)))
+        return a.value, b.value
+
+    def download_profile(self):
+        hist = {k: np.zeros(self.bins) for k in ("photons", "electrons", "positrons", "energy_loss")}
+        led = Ledger()
+        dp = lambda a: a.ctypes.data_as(POINTER(c_double))
+        _check(library().c8mi_em_download_profile(self.handle, dp(hist["photons"]), dp(hist["electrons"]),
+                                                  dp(hist["positrons"]), dp(hist["energy_loss"]), byref(led)))
+        return hist, led.as_dict()
+
+    def observations(self):
+        out = np.zeros(self.observation_capacity, dtype=PARTICLE_STATE_DTYPE)
+        n = c_uint64()
+        _check(library().c8mi_em_download_observations(self.handle, out.ctypes.data_as(c_void_p),
+                                                       c_uint64(out.size), byref(n)))
+        return out[:min(n.value, out.size)], n.value
+
+    def fallbacks(self):
+        out = np.zeros(self.fallback_capacity, dtype=FALLBACK_DTYPE)
+        n = c_uint64()
+        _check(library().c8mi_em_download_fallbacks(self.handle, out.ctypes.data_as(c_void_p),
+                                                    c_uint64(out.size), byref(n)))
+        return out[:min(n.value, out.size)], n.value
+
+    def enable_kernel_timing(self, enable=True):
+        _check(library().c8mi_em_enable_kernel_timing(self.handle, c_int(1 if enable else 0)))
+
+    def kernel_time(self):
+        ms, n = (c_double * 4)(), (c_uint64 * 4)()
+        _check(library().c8mi_em_kernel_time(self.handle, ms, n))
+        return {k: (ms[i], n[i]) for i, k in enumerate(("photon_step", "lepton_step", "scan", "scatter"))}

@@ -7,6 +7,7 @@
 #include "c8mi/c8mi_em.h"
 #include "capi_guard.hpp"
 #include "em/device_tables.hpp"
+#include "em/em_backend.hpp"
 #include "em/em_launch.hpp"
 #include "runtime.hpp"
 
@@ -15,7 +16,20 @@ struct c8mi_em_tables {
   explicit c8mi_em_tables(c8mi_native_table_set const& host) : tables(host) {}
 };
 
+struct c8mi_em {
+  c8mi::em::EmBackend backend;
+  c8mi_em(c8mi_em_config const& config, c8mi_native_table_set const& tables,
+          c8mi_environment const& environment, hipStream_t stream)
+      : backend(config, tables, environment, stream) {}
+};
+
 using c8mi::guarded;
+
+namespace {
+void require_handle(void const* p) {
+  if (p == nullptr) throw std::invalid_argument("null handle");
+}
+}  // namespace
 
 extern "C" {
 
@@ -62,6 +76,125 @@ int c8mi_em_random(c8mi_random_key const* device_keys, uint64_t count, uint32_t*
     if (count > 0 && device_keys == nullptr) throw std::invalid_argument("null key array");
     c8mi::em::launch_random(device_keys, count, device_words, device_uniforms,
                             static_cast<hipStream_t>(stream));
+  });
+}
+
+int c8mi_em_create(c8mi_em_config const* config, c8mi_native_table_set const* tables,
+                   c8mi_environment const* environment, void* stream, c8mi_em** out) {
+  return guarded([&] {
+    if (!config || !tables || !environment || !out) throw std::invalid_argument("null argument");
+    *out = nullptr;
+    *out = new c8mi_em(*config, *tables, *environment, static_cast<hipStream_t>(stream));
+  });
+}
+
+int c8mi_em_destroy(c8mi_em* em) {
+  return guarded([&] { delete em; });
+}
+
+int c8mi_em_begin_shower(c8mi_em* em, uint64_t shower_id, uint64_t first_history_id) {
+  return guarded([&] {
+    require_handle(em);
+    em->backend.begin_shower(shower_id, first_history_id);
+  });
+}
+
+int c8mi_em_can_transport(c8mi_em const* em, c8mi_particle_state const* particle, int* out) {
+  return guarded([&] {
+    require_handle(em);
+    if (!particle || !out) throw std::invalid_argument("null argument");
+    *out = em->backend.can_transport(*particle) ? 1 : 0;
+  });
+}
+
+int c8mi_em_enqueue(c8mi_em* em, c8mi_particle_state const* particles, uint64_t count) {
+  return guarded([&] {
+    require_handle(em);
+    em->backend.enqueue(particles, count);
+  });
+}
+
+int c8mi_em_run_photon_wavefront(c8mi_em* em, c8mi_particle_state const* particles, uint64_t count,
+                                 uint64_t maximum_fronts, c8mi_em_run_result* out) {
+  return guarded([&] {
+    require_handle(em);
+    if (!out) throw std::invalid_argument("null result");
+    for (uint64_t i = 0; i < count; ++i)
+      if (particles[i].pid != 22) throw std::invalid_argument("runPhotonWavefront takes photons only");
+    em->backend.enqueue(particles, count);
+    em->backend.run(1, maximum_fronts, *out);
+  });
+}
+
+int c8mi_em_run_lepton_wavefront(c8mi_em* em, c8mi_particle_state const* particles, uint64_t count,
+                                 uint64_t maximum_fronts, c8mi_em_run_result* out) {
+  return guarded([&] {
+    require_handle(em);
+    if (!out) throw std::invalid_argument("null result");
+    for (uint64_t i = 0; i < count; ++i)
+      if (particles[i].pid != 11 && particles[i].pid != -11)
+        throw std::invalid_argument("runLeptonWavefront takes e+- only");
+    em->backend.enqueue(particles, count);
+    em->backend.run(2, maximum_fronts, *out);
+  });
+}
+
+int c8mi_em_run(c8mi_em* em, uint64_t maximum_fronts, c8mi_em_run_result* out) {
+  return guarded([&] {
+    require_handle(em);
+    if (!out) throw std::invalid_argument("null result");
+    em->backend.run(0, maximum_fronts, *out);
+  });
+}
+
+int c8mi_em_pending_counts(c8mi_em const* em, uint64_t* photons, uint64_t* leptons) {
+  return guarded([&] {
+    require_handle(em);
+    if (!photons || !leptons) throw std::invalid_argument("null argument");
+    uint64_t p = 0, l = 0;
+    em->backend.pending(p, l);
+    *photons = p;
+    *leptons = l;
+  });
+}
+
+int c8mi_em_download_profile(c8mi_em* em, double* photons, double* electrons, double* positrons,
+                             double* energy_loss, c8mi_em_ledger* ledger) {
+  return guarded([&] {
+    require_handle(em);
+    em->backend.download_profile(photons, electrons, positrons, energy_loss, ledger);
+  });
+}
+
+int c8mi_em_download_observations(c8mi_em* em, c8mi_particle_state* out, uint64_t capacity,
+                                  uint64_t* count) {
+  return guarded([&] {
+    require_handle(em);
+    uint64_t const n = em->backend.download_observations(out, capacity);
+    if (count) *count = n;
+  });
+}
+
+int c8mi_em_download_fallbacks(c8mi_em* em, c8mi_em_fallback* out, uint64_t capacity, uint64_t* count) {
+  return guarded([&] {
+    require_handle(em);
+    uint64_t const n = em->backend.download_fallbacks(out, capacity);
+    if (count) *count = n;
+  });
+}
+
+int c8mi_em_enable_kernel_timing(c8mi_em* em, int enable) {
+  return guarded([&] {
+    require_handle(em);
+    em->backend.enable_timing(enable != 0);
+  });
+}
+
+int c8mi_em_kernel_time(c8mi_em const* em, double ms[4], uint64_t launches[4]) {
+  return guarded([&] {
+    require_handle(em);
+    if (!ms || !launches) throw std::invalid_argument("null argument");
+    em->backend.kernel_time(ms, launches);
   });
 }
 

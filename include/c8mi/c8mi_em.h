@@ -58,6 +58,255 @@ int c8mi_em_native_queries(const c8mi_em_tables* tables, const c8mi_native_query
 int c8mi_em_random(const c8mi_random_key* device_keys, uint64_t count, uint32_t* device_words,
                    double* device_uniforms, void* stream);
 
+/* ---- Environment (corsika/kokkos/environment) ------------------------------ */
+
+/* EnvironmentModel::Density (corsika/kokkos/environment/EnvironmentModel.hpp:29-65). */
+enum {
+  C8MI_DENSITY_HOMOGENEOUS = 1,
+  C8MI_DENSITY_SLIDING_PLANAR_EXPONENTIAL = 2, /* height = |x - anchor| */
+  C8MI_DENSITY_FLAT_EXPONENTIAL = 3            /* height = (x - anchor) . axis */
+};
+/* EnvironmentShape::Kind (corsika/kokkos/environment/EnvironmentShape.hpp:27-36). */
+enum { C8MI_SHAPE_UNIVERSE = 1, C8MI_SHAPE_SPHERE = 2 };
+
+/* One medium model: rho = rho0 * exp(inverse_scale_length * (height - reference_height)). */
+typedef struct c8mi_env_model {
+  uint32_t density_kind;
+  int32_t medium_id;
+  double reference_density_g_per_cm3;
+  double scale_length_m; /* signed; 0 for homogeneous */
+  double inverse_scale_length_per_m;
+  double reference_height_m;
+  double anchor_m[3];
+  double axis[3];
+  double magnetic_field_t[3]; /* uniform B; must be zero in this build */
+} c8mi_env_model;
+
+/* One volume of the environment tree (EnvironmentTreeView): its shape, parent,
+ * ordered children (a range of c8mi_environment.children) and medium model
+ * (-1: no medium, the particle leaves the environment there). */
+typedef struct c8mi_env_node {
+  uint32_t shape_kind;
+  int32_t parent; /* -1 for the root */
+  int32_t model;
+  uint32_t child_begin;
+  uint32_t child_count;
+  uint32_t pad_;
+  double center_m[3];
+  double radius_m;
+} c8mi_env_node;
+
+/* Environment snapshot + transport geometry (TransportGeometryConfig): tree,
+ * models, observation plane (point, unit normal) and Earth centre. */
+typedef struct c8mi_environment {
+  const c8mi_env_node* nodes;
+  uint32_t node_count;
+  uint32_t child_index_count;
+  const int32_t* children;
+  const c8mi_env_model* models;
+  uint32_t model_count;
+  uint32_t observation_enabled;
+  double observation_point_m[3];
+  double observation_normal[3];
+  double earth_center_m[3];
+  double maximum_magnetic_deflection_rad;
+} c8mi_environment;
+
+/* ---- Physics auxiliaries ------------------------------------------------------ */
+
+/* PhotonPairLpmSnapshot (corsika/kokkos/em/models/PhotonPairLpm.hpp:20-40). */
+typedef struct c8mi_photon_pair_lpm_component {
+  uint64_t component_hash;
+  double nuclear_charge;
+  double radiation_log_constant;
+} c8mi_photon_pair_lpm_component;
+
+typedef struct c8mi_photon_pair_lpm {
+  double baseline_mass_density_g_per_cm3;
+  double molecular_density_per_cm3;
+  double sum_charge;
+  double e_lpm_mev;
+  double classical_electron_radius_cm;
+  double fine_structure_constant;
+  uint32_t component_count;
+  uint32_t reserved;
+  c8mi_photon_pair_lpm_component components[16];
+} c8mi_photon_pair_lpm;
+
+/* BremsLpmSnapshot (corsika/kokkos/em/models/BremsLpm.hpp:21-51). */
+typedef struct c8mi_brems_lpm_component {
+  uint64_t component_hash;
+  double nuclear_charge;
+  double atomic_mass_number;
+  double radiation_log_constant;
+} c8mi_brems_lpm_component;
+
+typedef struct c8mi_brems_lpm {
+  double baseline_mass_density_g_per_cm3;
+  double molecular_density_per_cm3;
+  double sum_charge;
+  double e_lpm_mev;
+  double lepton_mass_mev;
+  double electron_mass_mev;
+  double muon_mass_mev;
+  double classical_electron_radius_cm;
+  double fine_structure_constant;
+  uint32_t component_count;
+  uint32_t reserved;
+  c8mi_brems_lpm_component components[16];
+  double muon_baseline_mass_density_g_per_cm3;
+  double muon_molecular_density_per_cm3;
+  double muon_sum_charge;
+  double muon_e_lpm_mev;
+} c8mi_brems_lpm;
+
+/* EmThinningConfig (corsika/kokkos/em/algorithms/EmThinning.hpp:25-30). */
+typedef struct c8mi_em_thinning {
+  double threshold_gev;
+  double maximum_weight;
+  uint32_t enabled;
+  uint32_t erase_zero_weight;
+} c8mi_em_thinning;
+
+/* ---- EM backend (IEmBackend, corsika/kokkos/em/execution/IEmBackend.hpp:25-88) -- */
+
+enum { C8MI_PROFILE_CROSSING_BOTH = 0, C8MI_PROFILE_CROSSING_FORWARD = 1 };
+
+/* BackendConfig + ShowerConfig subset of the device path. The projection table
+ * (ProfileProjectionView) maps a position to grammage along the shower axis:
+ * axis_grammage[k] is the grammage at axis_start + k * axis_step * axis_direction. */
+typedef struct c8mi_em_config {
+  uint64_t random_seed;
+  double em_transport_cut_mev;   /* e+- kinetic and photon energy cut */
+  double muon_transport_cut_mev; /* muon kinetic cut (muons fall back to the host) */
+  uint64_t queue_capacity;       /* particles per species and buffer */
+  uint64_t observation_capacity; /* stored observation-surface particles */
+  uint64_t fallback_capacity;    /* stored host fallback requests */
+  c8mi_photon_pair_lpm photon_pair_lpm;
+  c8mi_brems_lpm brems_lpm;
+  c8mi_em_thinning thinning;
+  uint32_t profile_bins;
+  uint32_t profile_crossing_mode;
+  double profile_bin_width_g_per_cm2;
+  double profile_energy_loss_threshold_g_per_cm2;
+  double profile_weight_scale;     /* fixed-point units per unit weight */
+  double profile_energy_scale;     /* fixed-point units per GeV */
+  double axis_start_m[3];
+  double axis_direction[3];
+  double axis_step_m;
+  const double* axis_grammage_g_per_cm2;
+  uint64_t axis_support_count;
+} c8mi_em_config;
+
+/* Per-call statistics (PhotonCascadeResult / LeptonCascadeResult counters). */
+typedef struct c8mi_em_run_result {
+  uint64_t photon_fronts;
+  uint64_t lepton_fronts;
+  uint64_t photon_steps;    /* transport records, as PhotonCascade.inl:499 counts them */
+  uint64_t lepton_steps;
+  uint64_t photon_interactions;
+  uint64_t lepton_interactions;
+  uint64_t secondaries;
+  uint64_t observations;
+  uint64_t fallbacks;
+  uint64_t pending_photons;
+  uint64_t pending_leptons;
+  uint64_t next_history_id;
+  uint64_t max_front;       /* largest single front processed */
+} c8mi_em_run_result;
+
+/* Energy ledger of one shower (ProfileCounters / ProfileResult ledger terms,
+ * corsika/kokkos/output/ProfileCounters.hpp; closure ShowerReport.inl:1236-1276).
+ * Weighted energies in GeV, decoded from the device's int64 fixed point. */
+typedef struct c8mi_em_ledger {
+  double injected_energy;               /* weighted total energy enqueued */
+  double continuous_deposit;            /* dE/dX along lepton steps */
+  double cut_deposit;                   /* kinetic (photon: total) energy at particle cuts */
+  double cut_rest_mass;                 /* lepton rest mass at particle cuts */
+  double observed_energy;               /* total energy reaching the observation surface */
+  double escaped_energy;                /* total energy leaving the environment */
+  double observation_cut_overlap;       /* energy counted both as observed and cut */
+  double mass_convention_correction;    /* transport vs native mass of kept leptons */
+  double medium_rest_mass_input;        /* atomic electrons set in motion */
+  double photoelectric_binding;         /* photoelectric binding energy */
+  double fallback_energy;               /* handed to the host in fallback requests */
+  double thinning_discarded_energy;     /* weight-energy removed by thinning */
+  double pending_energy;                /* still queued on the device */
+  uint64_t steps;
+  uint64_t photon_cuts;
+  uint64_t lepton_limits[8];
+  uint64_t thinning_hillas_vertices;
+  uint64_t thinning_statistical_vertices;
+  uint64_t thinning_particles_discarded;
+  uint64_t deposited_steps;
+  uint64_t fixed_point_overflows;
+  uint64_t invalid_records;
+  uint64_t fallback_reasons[32];
+} c8mi_em_ledger;
+
+/* FallbackRequest subset: the particle at the failing stage and the reason
+ * (FallbackReason, corsika/kokkos/em/FallbackRequest.hpp:22-65). */
+typedef struct c8mi_em_fallback {
+  c8mi_particle_state particle;
+  int32_t reason;
+  int32_t process_id;
+  uint32_t stage; /* 1 selection, 2 transport, 3 vertex, 4 final state */
+  uint32_t pad_;
+} c8mi_em_fallback;
+
+typedef struct c8mi_em c8mi_em;
+
+/* EmBackend::initialize (corsika/kokkos/em/execution/EmBackend.hpp:29-31):
+ * copies tables, environment and configuration to the device. */
+int c8mi_em_create(const c8mi_em_config* config, const c8mi_native_table_set* tables,
+                   const c8mi_environment* environment, void* stream, c8mi_em** out);
+int c8mi_em_destroy(c8mi_em* em);
+
+/* IEmBackend::beginShower: clears queues, profile, ledger, observations and
+ * fallbacks; sets the Philox shower identity and the first secondary history ID. */
+int c8mi_em_begin_shower(c8mi_em* em, uint64_t shower_id, uint64_t first_history_id);
+
+/* IEmBackend::canTransport (photon or e+-, finite, inside the energy domain). */
+int c8mi_em_can_transport(const c8mi_em* em, const c8mi_particle_state* particle, int* out);
+
+/* Adds host particles to the device queues (photons and e+- split by PID). */
+int c8mi_em_enqueue(c8mi_em* em, const c8mi_particle_state* particles, uint64_t count);
+
+/* IEmBackend::runPhotonWavefront / runLeptonWavefront: enqueue `count`
+ * particles, then run at most `maximum_fronts` fronts of that species;
+ * secondaries of the other species stay queued on the device. */
+int c8mi_em_run_photon_wavefront(c8mi_em* em, const c8mi_particle_state* particles,
+                                 uint64_t count, uint64_t maximum_fronts,
+                                 c8mi_em_run_result* out);
+int c8mi_em_run_lepton_wavefront(c8mi_em* em, const c8mi_particle_state* particles,
+                                 uint64_t count, uint64_t maximum_fronts,
+                                 c8mi_em_run_result* out);
+
+/* Runs photon and lepton fronts alternately (photon front, then lepton front)
+ * until both queues are empty or `maximum_fronts` pairs have run. */
+int c8mi_em_run(c8mi_em* em, uint64_t maximum_fronts, c8mi_em_run_result* out);
+
+/* IEmBackend::pendingPhotonCount / pendingLeptonCount. */
+int c8mi_em_pending_counts(const c8mi_em* em, uint64_t* photons, uint64_t* leptons);
+
+/* IEmBackend::downloadProfile: five longitudinal histograms of profile_bins
+ * entries each (photons, electrons, positrons, e+- energy loss [GeV], and
+ * weighted particle counts are decoded) and the ledger. Any output may be NULL. */
+int c8mi_em_download_profile(c8mi_em* em, double* photons, double* electrons,
+                             double* positrons, double* energy_loss, c8mi_em_ledger* ledger);
+
+/* Observation-surface particles (ObservationRecord) and host fallback requests
+ * of the current shower; *count receives the total (which may exceed capacity). */
+int c8mi_em_download_observations(c8mi_em* em, c8mi_particle_state* out, uint64_t capacity,
+                                  uint64_t* count);
+int c8mi_em_download_fallbacks(c8mi_em* em, c8mi_em_fallback* out, uint64_t capacity,
+                               uint64_t* count);
+
+/* Per-kernel device time (HIP events) since enable: ms[0] photon step,
+ * ms[1] lepton step, ms[2] scans, ms[3] scatter. */
+int c8mi_em_enable_kernel_timing(c8mi_em* em, int enable);
+int c8mi_em_kernel_time(const c8mi_em* em, double ms[4], uint64_t launches[4]);
+
 #ifdef __cplusplus
 }
 #endif

@@ -16,6 +16,7 @@
 #include <stdint.h>
 
 #include "c8mi/c8mi_tables.h"
+#include "c8mi/c8mi_em.h"
 #include "c8mi/c8mi_types.h"
 
 #ifdef __cplusplus
@@ -102,6 +103,72 @@ double oracle_native_physical_loss(const c8mi_native_table_set* table, uint32_t 
                                    double energy_mev, double transformed_loss);
 double oracle_native_transformed_loss(const c8mi_native_table_set* table, uint32_t column,
                                       double energy_mev, double physical_loss);
+
+/* ---------------- EM transport (em_oracle.c) ---------------- */
+
+/* Fixed-point counter layout shared with the device ledger: energy terms
+ * (int64, profile_energy_scale units per GeV) first, then plain counts. */
+enum {
+  OR_EMC_INJECTED = 0,
+  OR_EMC_CONTINUOUS = 1,
+  OR_EMC_CUT_DEPOSIT = 2,
+  OR_EMC_CUT_REST_MASS = 3,
+  OR_EMC_OBSERVED = 4,
+  OR_EMC_ESCAPED = 5,
+  OR_EMC_OVERLAP = 6,
+  OR_EMC_MASS_CORRECTION = 7,
+  OR_EMC_MEDIUM_REST_MASS = 8,
+  OR_EMC_PHOTOELECTRIC_BINDING = 9,
+  OR_EMC_FALLBACK_ENERGY = 10,
+  OR_EMC_THINNING_ENERGY = 11,
+  OR_EMC_PENDING = 12,
+  OR_EMC_STEPS = 13,
+  OR_EMC_PHOTON_CUTS = 14,
+  OR_EMC_LEPTON_LIMIT0 = 15, /* 8 entries */
+  OR_EMC_THIN_HILLAS = 23,
+  OR_EMC_THIN_STATISTICAL = 24,
+  OR_EMC_THIN_DISCARDED = 25,
+  OR_EMC_DEPOSITED_STEPS = 26,
+  OR_EMC_OVERFLOWS = 27,
+  OR_EMC_INVALID = 28,
+  OR_EMC_FALLBACK_REASON0 = 29, /* 32 entries */
+  OR_EM_COUNTERS = 64
+};
+
+typedef c8mi_em_run_result or_em_stats;
+
+/* Runs one shower to completion (or maximum_fronts photon+lepton front pairs)
+ * with the serial front schedule of c8mi_em_run. Histograms hold profile_bins
+ * int64 each; counters OR_EM_COUNTERS; observations / fallbacks up to the
+ * configured capacities (may be NULL). */
+int oracle_em_run_shower(const c8mi_em_config* config, const c8mi_native_table_set* tables,
+                         const c8mi_environment* environment, uint64_t shower_id,
+                         uint64_t first_history_id, const c8mi_particle_state* primaries,
+                         uint64_t primary_count, uint64_t maximum_fronts, int64_t* photons,
+                         int64_t* electrons, int64_t* positrons, int64_t* energy_loss,
+                         int64_t* counters, c8mi_particle_state* observations,
+                         c8mi_em_fallback* fallbacks, or_em_stats* stats);
+
+int oracle_em_environment_query(const c8mi_environment* e, const double position[3],
+                                const double direction[3], int32_t node, int32_t* out_node,
+                                double* out_density);
+int oracle_em_boundary(const c8mi_environment* e, const double position[3],
+                       const double direction[3], int32_t node, double* distance,
+                       int32_t* next_node);
+int oracle_em_grammage(const c8mi_environment* e, int32_t node, const double position[3],
+                       const double direction[3], double distance, double* out);
+int oracle_em_arclength(const c8mi_environment* e, int32_t node, const double position[3],
+                        const double direction[3], double grammage, double* out);
+int oracle_em_pair_lpm(const c8mi_photon_pair_lpm* s, uint64_t comp, double e_mev, double x,
+                       double rho, double* out);
+int oracle_em_brems_lpm(const c8mi_brems_lpm* s, uint64_t comp, double e_mev, double v,
+                        double rho, double* out);
+int oracle_em_thinning(const c8mi_em_thinning* c, double pe, double pw, double e1, double e2,
+                       double u1, double u2, uint32_t* keep, double* w1, double* w2);
+int oracle_em_pair_split(double e_gev, double x, double* normalized, double* decoded);
+double oracle_em_sauter_cosine(double e, double u);
+int oracle_em_compton_cosines(double e, double v, double* pc, double* ec);
+int oracle_em_annihilation_rho(double e, double m, double u, double* rho);
 
 #ifdef __cplusplus
 }

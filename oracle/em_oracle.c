@@ -1,0 +1,2497 @@
+/*
+ * c8mi oracle — TEST INFRASTRUCTURE ONLY (see c8_oracle.h).
+ *
+ * Plain-C FP64 restatement of the EM transport step of CORSIKA8-kokkos and of
+ * a serial wavefront driver over it:
+ *   environment   corsika/detail/kokkos/environment/{EnvironmentQuery,ModelEvaluation,
+ *                 ModelGrammage,StraightIntersections,EnvironmentSearch,ObservationPlane,
+ *                 EnvironmentLeptonBoundary,UniformMagneticField}.inl,
+ *                 corsika/detail/framework/geometry/NextEnvironmentBoundaryBody.inl
+ *   selection     corsika/detail/kokkos/em/algorithms/InteractionSelection.inl:12-215,
+ *                 corsika/detail/kokkos/em/tables/{DiscretePhysicsQueries,
+ *                 ContinuousPhysicsQueries,TableQueryResult}.inl
+ *   photon        PhotonTransportStep.inl:57-261, PhotonFinalStateStep.inl:61-825,
+ *                 models/PhotonPairLpm.inl:58-157, models/PhotonPairKinematics.inl:29-72
+ *   lepton        LeptonContinuousStep.inl:43-159, LeptonTransportStep.inl:27-513
+ *                 (zero field), LeptonVertexSelection.inl:41-199,
+ *                 LeptonFinalStateStep.inl:131-1036, models/BremsLpm.inl:58-164
+ *   thinning      EmThinning.inl:28-104
+ *   fronts        corsika/detail/kokkos/em/execution/{PhotonFront,LeptonFront}.inl
+ *                 (stable source order, history IDs from the child prefix sum)
+ *   profile       corsika/detail/kokkos/output/algorithms/ProfileAccumulation.inl,
+ *                 ProfileProjection.inl, LongitudinalCrossings.inl
+ * Built without floating-point contraction (oracle/Makefile).
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "c8_oracle.h"
+#include "c8mi/c8mi_em.h"
+#include "native_internal.h"
+
+/* ---- constants (ScalarPhysicalConstants.hpp:19-22, TransportMass.hpp:26-27,
+ *      PhotonPairKinematics.hpp:15, ParticleId.hpp:24, ContinuousPhysicsQueries.hpp:29,
+ *      LeptonContinuousStep.hpp:18, LeptonFinalStateStep.hpp:79-81) ---- */
+#define C_LIGHT 299792458.
+#define ELEMENTARY_CHARGE 1.6021766208e-19
+#define MAGNETIC_RIGIDITY 0.2997925146834389
+#define ELECTRON_MASS_GEV 0.0005109989461
+#define TRANSPORT_ELECTRON_MASS_GEV 5.109989e-4
+#define TRANSPORT_MUON_MASS_GEV 1.056584e-1
+#define PARTICLE_CUT_TIME_S 10.e-3
+#define CONTINUOUS_CUT_SAFETY 0.9999
+#define MAX_CONTINUOUS_RELATIVE_LOSS 0.10
+#define PHOTON_TWO_PI 6.283185307179586476925286766559
+#define PHOTON_PI 3.1415926535897932384626433832795
+#define LEPTON_PI 3.141592653589793238462643383279502884
+#define LEPTON_TWO_PI 6.283185307179586476925286766559005768
+#define PROPOSAL_HALF_PRECISION 1.e-5
+
+/* random domains (RandomDomains.hpp:16-31, PhotonFinalStateRandomDomains.hpp:16-27,
+ * LeptonFinalStateRandomDomains.hpp:15-25, EmThinning.hpp:18-19) */
+#define RP_DISTANCE 0x454d0001U
+#define RP_COLUMN 0x454d0002U
+#define RP_SELECTION 0x454d0004U
+#define D_PAIR_SPLIT 1ULL
+#define D_PAIR_AZIMUTH 2ULL
+#define D_PAIR_E_POLAR 3ULL
+#define D_PAIR_P_POLAR 4ULL
+#define D_PAIR_LPM 5ULL
+#define D_COMPTON_AZIMUTH 1ULL
+#define D_BREMS_AZIMUTH 1ULL
+#define D_BREMS_LPM 2ULL
+#define D_ANNIHILATION_RHO 1ULL
+#define D_ANNIHILATION_AZIMUTH 2ULL
+#define D_IONIZATION_AZIMUTH 1ULL
+#define D_THIN_FIRST 0x100ULL
+#define D_THIN_SECOND 0x101ULL
+
+enum { P_BREMS = 1000000002, P_IONIZATION = 1000000003, P_EPAIR = 1000000004,
+       P_COMPTON = 1000000010, P_ANNIHILATION = 1000000012, P_PHOTON_PAIR = 1000000013,
+       P_PHOTON_MU_PAIR = 1000000015, P_PHOTOELECTRIC = 1000000016,
+       P_PHOTON_PAIR_FS = 2000000013 };
+
+/* InteractionStatus (InteractionRecord.hpp:17-45) */
+enum { IS_SELECTED = 0, IS_NO_DISCRETE = 1, IS_RESELECT = 2, IS_DISTANCE_SAMPLED = 3,
+       IS_PARTICLE_CUT = 4 };
+/* TableLookupStatus (TableQueryResult.hpp:18-33) */
+enum { TS_SUCCESS = 0, TS_PARTICLE_NOT_FOUND = 1, TS_COLUMN_NOT_FOUND = 2,
+       TS_RATE_ENERGY_OOR = 3, TS_INVERSE_CDF = 4, TS_LOSS_ENERGY_OOR = 5,
+       TS_LOSS_QUANTILE_OOR = 6, TS_NON_FINITE = 7, TS_INVALID_KIND = 8, TS_INVALID_VIEW = 9,
+       TS_CONT_PARTICLE_NOT_FOUND = 10, TS_CONT_ENERGY_OOR = 11, TS_CONT_RANGE_OOR = 12,
+       TS_TRANSPORT_CUT = 13 };
+/* FallbackReason (FallbackRequest.hpp:22-65) */
+enum { FR_UNSUPPORTED_PARTICLE = 0, FR_UNSUPPORTED_GEOMETRY = 2, FR_PARTICLE_TABLE_MISSING = 3,
+       FR_COMPONENT_MISSING = 4, FR_RATE_ENERGY_OOR = 5, FR_INVERSE_CDF = 6,
+       FR_LOSS_ENERGY_OOR = 7, FR_LOSS_QUANTILE_OOR = 8, FR_INVALID_TABLE_QUERY = 9,
+       FR_ZERO_TOTAL_RATE = 11, FR_CPU_ONLY = 12, FR_NOT_IMPLEMENTED = 13,
+       FR_INVALID_FINAL_STATE = 14, FR_LPM_UNAVAILABLE = 15, FR_INVALID_DENSITY = 16,
+       FR_CONT_ENERGY_OOR = 17, FR_CONT_RANGE_OOR = 18, FR_TRANSPORT_CUT = 19,
+       FR_MAGNETIC_TRANSPORT = 22, FR_MAGNETIC_BOUNDARY = 23, FR_GRAMMAGE = 24,
+       FR_INVERSE_GRAMMAGE = 25, FR_VERTEX_LOOKUP = 26, FR_SELECTION_REPLAY = 28 };
+/* EnvironmentStatus (EnvironmentQuery.hpp:24-32) */
+enum { ES_SUCCESS = 0, ES_INVALID = 1, ES_NON_FINITE = 2, ES_OUTSIDE = 3, ES_NO_FORWARD = 5,
+       ES_GRAMMAGE_OOR = 6 };
+/* Photon / lepton transport limits (PhotonTransportRecord.hpp:16-23,
+ * LeptonTransportRecord.hpp:16-26) */
+enum { PL_INTERACTION = 0, PL_LAYER = 1, PL_OBSERVATION = 2, PL_ESCAPED = 3, PL_CUT = 4 };
+enum { LL_INTERACTION = 0, LL_CONTINUOUS = 1, LL_CUT = 2, LL_LAYER = 3, LL_OBSERVATION = 4,
+       LL_ESCAPED = 5, LL_MAGNETIC = 6, LL_DECAY = 7 };
+/* fallback stages */
+enum { STAGE_SELECTION = 1, STAGE_TRANSPORT = 2, STAGE_VERTEX = 3, STAGE_FINAL = 4 };
+
+typedef c8mi_particle_state ps_t;
+typedef c8mi_native_table_set view_t;
+
+static int finite(double v) { return isfinite(v); }
+static double dot3(const double a[3], const double b[3]) {
+  return a[0] * b[0] + a[1] * b[1] + a[2] * b[2];
+}
+static int sgn(double v) { return (0. < v) - (v < 0.); }
+
+/* ========================== environment ========================== */
+
+typedef struct {
+  const c8mi_environment* e;
+} env_t;
+
+static const c8mi_env_model* get_model(const c8mi_environment* e, int32_t node) {
+  if (node < 0 || (uint32_t)node >= e->node_count) return NULL;
+  int32_t m = e->nodes[node].model;
+  if (m < 0 || (uint32_t)m >= e->model_count) return NULL;
+  return &e->models[m];
+}
+
+/* valid_environment (EnvironmentQuery.inl:53-65) */
+static int valid_environment(const c8mi_environment* e) {
+  return e->node_count > 0 && e->nodes && e->model_count > 0 && e->models &&
+         finite(e->maximum_magnetic_deflection_rad) && e->maximum_magnetic_deflection_rad > 0. &&
+         e->maximum_magnetic_deflection_rad < 0.5 * 3.141592653589793;
+}
+
+static int normalized(const double d[3]) {
+  double n = dot3(d, d);
+  return finite(n) && fabs(n - 1.) <= 1.e-12;
+}
+
+/* contains (EnvironmentSearch.inl:48-59), spheres and the universe */
+static int contains(const c8mi_env_node* s, const double p[3]) {
+  if (s->shape_kind == C8MI_SHAPE_UNIVERSE) return 1;
+  double d[3] = {p[0] - s->center_m[0], p[1] - s->center_m[1], p[2] - s->center_m[2]};
+  return dot3(d, d) < s->radius_m * s->radius_m;
+}
+
+/* locate_node (EnvironmentSearch.inl:62-100) from the root */
+static int32_t locate_node(const c8mi_environment* e, const double p[3]) {
+  int32_t current = 0;
+  for (uint32_t hop = 0; hop < e->node_count; ++hop) {
+    if (current < 0 || (uint32_t)current >= e->node_count) return -2;
+    if (!contains(&e->nodes[current], p)) return -1;
+    int32_t next = -1;
+    const c8mi_env_node* n = &e->nodes[current];
+    for (uint32_t c = 0; c < n->child_count; ++c) {
+      int32_t child = e->children[n->child_begin + c];
+      if (contains(&e->nodes[child], p)) {
+        next = child;
+        break;
+      }
+    }
+    if (next == -1) return current;
+    current = next;
+  }
+  return -2;
+}
+
+/* straight_intersect (StraightIntersections.inl:72-116), spheres and the universe */
+typedef struct {
+  double entry, exit;
+  int has;
+} isect_t;
+
+static isect_t straight_intersect(const c8mi_env_node* s, const double p[3], const double v[3]) {
+  isect_t r = {0., 0., 0};
+  if (s->shape_kind == C8MI_SHAPE_UNIVERSE) {
+    r.entry = -INFINITY;
+    r.exit = INFINITY;
+    r.has = 1;
+    return r;
+  }
+  double d[3] = {p[0] - s->center_m[0], p[1] - s->center_m[1], p[2] - s->center_m[2]};
+  double vv = dot3(v, v), vd = dot3(v, d);
+  double disc = vd * vd - vv * (dot3(d, d) - s->radius_m * s->radius_m);
+  if (!(disc > 0.)) return r;
+  double root = sqrt(disc);
+  r.entry = (-vd - root) / vv;
+  r.exit = (-vd + root) / vv;
+  r.has = 1;
+  return r;
+}
+
+/* next_environment_boundary (NextEnvironmentBoundaryBody.inl) */
+static void next_boundary(const c8mi_environment* e, const double p[3], const double v[3],
+                          int32_t current, double limit, double* parameter, int32_t* node) {
+  double selected = limit;
+  int32_t selected_node = current;
+  const c8mi_env_node* n = &e->nodes[current];
+  isect_t own = straight_intersect(n, p, v);
+  if (own.has && own.exit <= selected) {
+    selected = own.exit;
+    selected_node = n->parent;
+  }
+  for (uint32_t c = 0; c < n->child_count; ++c) {
+    int32_t child = e->children[n->child_begin + c];
+    isect_t x = straight_intersect(&e->nodes[child], p, v);
+    if (!x.has) continue;
+    if (x.exit > 0. && x.entry <= selected && -x.entry < x.exit) {
+      selected = x.entry;
+      selected_node = child;
+    }
+  }
+  if (selected < 0.) selected = 0.;
+  *parameter = selected;
+  *node = selected_node;
+}
+
+/* density (ModelEvaluation.inl:61-77) */
+static double density(const c8mi_env_model* m, const double p[3]) {
+  if (m->density_kind == C8MI_DENSITY_HOMOGENEOUS) return m->reference_density_g_per_cm3;
+  double d[3] = {p[0] - m->anchor_m[0], p[1] - m->anchor_m[1], p[2] - m->anchor_m[2]};
+  double h = m->density_kind == C8MI_DENSITY_SLIDING_PLANAR_EXPONENTIAL ? sqrt(dot3(d, d))
+                                                                         : dot3(d, m->axis);
+  return m->reference_density_g_per_cm3 *
+         exp(m->inverse_scale_length_per_m * (h - m->reference_height_m));
+}
+
+/* projected_axis (ModelGrammage.inl:16-26) */
+static double projected_axis(const c8mi_env_model* m, const double p[3], const double dir[3]) {
+  if (m->density_kind == C8MI_DENSITY_FLAT_EXPONENTIAL) return dot3(dir, m->axis);
+  double d[3] = {p[0] - m->anchor_m[0], p[1] - m->anchor_m[1], p[2] - m->anchor_m[2]};
+  return dot3(d, dir) / sqrt(dot3(d, d));
+}
+
+/* grammage / arclength closed forms (ModelGrammage.inl:39-84) */
+static double model_grammage(const c8mi_env_model* m, const double p[3], const double dir[3],
+                             double length) {
+  if (length == 0.) return 0.;
+  double rho = density(m, p);
+  if (m->density_kind == C8MI_DENSITY_HOMOGENEOUS) return length * rho * 100.;
+  double proj = projected_axis(m, p, dir);
+  if (proj == 0.) return length * rho * 100.;
+  return rho * (m->scale_length_m / proj) * expm1(proj * length * m->inverse_scale_length_per_m) *
+         100.;
+}
+
+static double model_arclength(const c8mi_env_model* m, const double p[3], const double dir[3],
+                              double grammage) {
+  double rho = density(m, p);
+  if (m->density_kind == C8MI_DENSITY_HOMOGENEOUS) return grammage / (rho * 100.);
+  double proj = projected_axis(m, p, dir);
+  if (proj == 0.) return grammage / (rho * 100.);
+  double arg = grammage * .01 * m->inverse_scale_length_per_m * proj / rho;
+  return arg > -1. ? m->scale_length_m / proj * log1p(arg) : INFINITY;
+}
+
+typedef struct {
+  int status;
+  int32_t node;
+  double density;
+} layer_t;
+
+/* query_environment (EnvironmentQuery.inl:73-102) */
+static layer_t query_environment(const c8mi_environment* e, const double p[3], const double* dir,
+                                 int32_t logical) {
+  layer_t r = {ES_INVALID, -1, 0.};
+  if (!valid_environment(e)) return r;
+  for (int a = 0; a < 3; ++a)
+    if (!finite(p[a]) || (dir && !finite(dir[a]))) {
+      r.status = ES_NON_FINITE;
+      return r;
+    }
+  if (dir && !normalized(dir)) {
+    r.status = ES_NON_FINITE;
+    return r;
+  }
+  int32_t node = logical == -3 ? locate_node(e, p) : logical;
+  const c8mi_env_model* m = get_model(e, node);
+  r.node = node;
+  if (!m) {
+    r.status = ES_OUTSIDE;
+    return r;
+  }
+  double rho = density(m, p);
+  r.density = rho;
+  r.status = finite(rho) && rho > 0. ? ES_SUCCESS : ES_INVALID;
+  return r;
+}
+
+typedef struct {
+  int status;
+  double distance;
+  int32_t next_node;
+} boundary_t;
+
+/* distance_to_environment_boundary (EnvironmentQuery.inl:104-123) */
+static boundary_t distance_to_boundary(const c8mi_environment* e, const double p[3],
+                                       const double dir[3], int32_t logical) {
+  boundary_t r = {ES_INVALID, 0., -3};
+  layer_t l = query_environment(e, p, dir, logical);
+  if (l.status != ES_SUCCESS) {
+    r.status = l.status;
+    return r;
+  }
+  double param;
+  int32_t node;
+  next_boundary(e, p, dir, l.node, INFINITY, &param, &node);
+  r.next_node = node;
+  if (!finite(param)) {
+    r.status = ES_NO_FORWARD;
+    return r;
+  }
+  r.status = ES_SUCCESS;
+  r.distance = param;
+  return r;
+}
+
+typedef struct {
+  int status;
+  double value;
+} gq_t;
+
+/* environment_grammage / environment_arclength / geometry_competition_grammage
+ * (EnvironmentQuery.inl:125-186) */
+static gq_t environment_grammage(const c8mi_environment* e, int32_t node, const double p[3],
+                                 const double dir[3], double distance) {
+  gq_t r = {ES_NON_FINITE, 0.};
+  const c8mi_env_model* m = get_model(e, node);
+  if (!valid_environment(e) || !m || !finite(distance) || distance < 0. || !normalized(dir))
+    return r;
+  double x = model_grammage(m, p, dir, distance);
+  r.status = finite(x) && x >= 0. ? ES_SUCCESS : ES_GRAMMAGE_OOR;
+  r.value = x;
+  return r;
+}
+
+static gq_t environment_arclength(const c8mi_environment* e, int32_t node, const double p[3],
+                                  const double dir[3], double grammage) {
+  gq_t r = {ES_NON_FINITE, 0.};
+  const c8mi_env_model* m = get_model(e, node);
+  if (!valid_environment(e) || !m || !finite(grammage) || grammage < 0. || !normalized(dir))
+    return r;
+  double l = model_arclength(m, p, dir, grammage);
+  r.status = finite(l) && l >= 0. ? ES_SUCCESS : ES_GRAMMAGE_OOR;
+  r.value = l;
+  return r;
+}
+
+static gq_t competition_grammage(const c8mi_environment* e, int32_t node, const double p[3],
+                                 const double dir[3], double distance) {
+  gq_t r = environment_grammage(e, node, p, dir, distance);
+  if (r.status == ES_GRAMMAGE_OOR && r.value == HUGE_VAL && finite(distance) && distance > 0.) {
+    const c8mi_env_model* m = get_model(e, node);
+    if (m && (m->density_kind == C8MI_DENSITY_SLIDING_PLANAR_EXPONENTIAL ||
+              m->density_kind == C8MI_DENSITY_FLAT_EXPONENTIAL)) {
+      double rho = density(m, p);
+      double growth = projected_axis(m, p, dir) * m->inverse_scale_length_per_m;
+      if (finite(rho) && rho > 0. && finite(growth) && growth > 0. && finite(m->scale_length_m) &&
+          m->scale_length_m * m->inverse_scale_length_per_m > 0.)
+        r.status = ES_SUCCESS;
+    }
+  }
+  return r;
+}
+
+/* intersect_observation_plane_straight (ObservationPlane.inl:61-90); returns the
+ * distance or -1 when there is no forward intersection. */
+static double observation_distance(const c8mi_environment* e, const double p[3],
+                                   const double dir[3]) {
+  if (!e->observation_enabled) return -1.;
+  double ns = 0.;
+  for (int a = 0; a < 3; ++a) {
+    if (!finite(e->observation_point_m[a]) || !finite(e->observation_normal[a])) return -1.;
+    ns += e->observation_normal[a] * e->observation_normal[a];
+  }
+  if (!finite(ns) || fabs(ns - 1.) > 1.e-12) return -1.;
+  double disp[3];
+  for (int a = 0; a < 3; ++a) {
+    if (!finite(p[a]) || !finite(dir[a])) return -1.;
+    disp[a] = e->observation_point_m[a] - p[a];
+  }
+  double den = dot3(e->observation_normal, dir);
+  if (den == 0.) return -1.;
+  double d = dot3(e->observation_normal, disp) / den;
+  if (!finite(d) || !(d > 0.)) return -1.;
+  return d;
+}
+
+/* Straight-line lepton observation distance with zero field: the separation-plane
+ * branch of CurvedBoundaryAccess::intersectShape (CurvedBoundaryAccess.inl:50-66)
+ * reduces to the linear root -c/b of quadratic_roots (PolynomialRoots.inl:98-116),
+ * scaled by 1/speed and back (EnvironmentLeptonBoundary.inl:62-79). */
+static double lepton_observation_distance(const c8mi_environment* e, const ps_t* s, double speed,
+                                          double momentum_si) {
+  double delta[3] = {s->position_m[0] - e->observation_point_m[0],
+                     s->position_m[1] - e->observation_point_m[1],
+                     s->position_m[2] - e->observation_point_m[2]};
+  double b = 2 * momentum_si * dot3(s->direction, e->observation_normal);
+  double c = 2 * momentum_si * dot3(e->observation_normal, delta);
+  double first = INFINITY;
+  if (b != 0.) {
+    double root = -c / b;
+    if (root >= 0. && root < first) first = root;
+  }
+  double entry = first / speed;
+  return entry > 0. ? entry * speed : INFINITY;
+}
+
+/* ========================== table facades ========================== */
+
+static int from_native(uint32_t status, int continuous) {
+  switch (status) {
+    case C8MI_QUERY_SUCCESS: return TS_SUCCESS;
+    case C8MI_QUERY_PARTICLE_NOT_FOUND: return continuous ? TS_CONT_PARTICLE_NOT_FOUND
+                                                          : TS_PARTICLE_NOT_FOUND;
+    case C8MI_QUERY_COLUMN_NOT_FOUND: return TS_COLUMN_NOT_FOUND;
+    case C8MI_QUERY_ENERGY_OUT_OF_RANGE: return continuous ? TS_CONT_ENERGY_OOR
+                                                           : TS_RATE_ENERGY_OOR;
+    case C8MI_QUERY_QUANTILE_OUT_OF_RANGE: return TS_LOSS_QUANTILE_OOR;
+    case C8MI_QUERY_UNSUPPORTED_KINEMATICS:
+    case C8MI_QUERY_ROOT_NOT_CONVERGED: return TS_INVERSE_CDF;
+    case C8MI_QUERY_NON_FINITE_INPUT: return TS_NON_FINITE;
+    default: return TS_INVALID_VIEW;
+  }
+}
+
+typedef struct {
+  int status;
+  double value;
+} tq_t;
+
+/* proposal_fallback_reason (FallbackRequest.inl:14-44) */
+static int fallback_reason(int status) {
+  switch (status) {
+    case TS_PARTICLE_NOT_FOUND: return FR_PARTICLE_TABLE_MISSING;
+    case TS_COLUMN_NOT_FOUND: return FR_COMPONENT_MISSING;
+    case TS_RATE_ENERGY_OOR: return FR_RATE_ENERGY_OOR;
+    case TS_INVERSE_CDF: return FR_INVERSE_CDF;
+    case TS_LOSS_ENERGY_OOR: return FR_LOSS_ENERGY_OOR;
+    case TS_LOSS_QUANTILE_OOR: return FR_LOSS_QUANTILE_OOR;
+    case TS_CONT_PARTICLE_NOT_FOUND: return FR_PARTICLE_TABLE_MISSING;
+    case TS_CONT_ENERGY_OOR: return FR_CONT_ENERGY_OOR;
+    case TS_CONT_RANGE_OOR: return FR_CONT_RANGE_OOR;
+    case TS_TRANSPORT_CUT: return FR_TRANSPORT_CUT;
+    default: return FR_INVALID_TABLE_QUERY;
+  }
+}
+
+static int requires_selected_loss(int reason) {
+  return reason == FR_SELECTION_REPLAY || reason == FR_INVERSE_CDF ||
+         reason == FR_LOSS_ENERGY_OOR || reason == FR_LOSS_QUANTILE_OOR;
+}
+
+typedef struct {
+  const view_t* t;
+  double em_cut_mev, muon_cut_mev;
+} cont_t;
+
+static double transport_mass_gev(int32_t pid) {
+  if (pid == 11 || pid == -11) return TRANSPORT_ELECTRON_MASS_GEV;
+  if (pid == 13 || pid == -13) return TRANSPORT_MUON_MASS_GEV;
+  return 0.;
+}
+static double secondary_transport_energy(int32_t pid, double native_total, double native_mass) {
+  return (native_total - native_mass) + transport_mass_gev(pid);
+}
+static double mass_correction(int32_t pid, double native_mass, double weight) {
+  return weight * (transport_mass_gev(pid) - native_mass);
+}
+static int is_ee(int32_t pid) { return pid == 11 || pid == -11; }
+static int is_muon(int32_t pid) { return pid == 13 || pid == -13; }
+static int is_charged_lepton(int32_t pid) { return is_ee(pid) || is_muon(pid); }
+
+/* query_continuous_* (ContinuousPhysicsQueries.inl:42-130) */
+static tq_t cq_mass(const cont_t* c, int32_t pid) {
+  tq_t r = {TS_CONT_PARTICLE_NOT_FOUND, 0.};
+  const c8mi_native_utility_column* u = oracle_nq_find_utility(c->t, pid);
+  if (!u) return r;
+  if (!finite(u->particle_mass_mev) || !(u->particle_mass_mev > 0.)) {
+    r.status = TS_INVALID_VIEW;
+    return r;
+  }
+  r.status = TS_SUCCESS;
+  r.value = u->particle_mass_mev;
+  return r;
+}
+static tq_t cq_transport_mass(const cont_t* c, int32_t pid) {
+  tq_t m = cq_mass(c, pid);
+  if (m.status != TS_SUCCESS) return m;
+  double tm = 1000. * transport_mass_gev(pid);
+  tq_t r = {TS_CONT_PARTICLE_NOT_FOUND, 0.};
+  if (!(tm > 0.)) return r;
+  r.status = TS_SUCCESS;
+  r.value = tm;
+  return r;
+}
+static tq_t cq_transport_cut(const cont_t* c, int32_t pid) {
+  double cut = (pid == 13 || pid == -13) ? c->muon_cut_mev : c->em_cut_mev;
+  tq_t r = {TS_INVALID_VIEW, 0.};
+  if (!finite(cut) || !(cut > 0.)) return r;
+  r.status = TS_SUCCESS;
+  r.value = cut;
+  return r;
+}
+static tq_t cq_minimum_energy(const cont_t* c, int32_t pid) {
+  tq_t r = {TS_CONT_PARTICLE_NOT_FOUND, 0.};
+  const c8mi_native_utility_column* u = oracle_nq_find_utility(c->t, pid);
+  if (!u) return r;
+  tq_t cut = cq_transport_cut(c, pid);
+  if (cut.status != TS_SUCCESS) return cut;
+  double minimum = (1000. * transport_mass_gev(pid) + cut.value) * CONTINUOUS_CUT_SAFETY;
+  if (!finite(u->lower_energy_limit_mev) || u->lower_energy_limit_mev < u->particle_mass_mev ||
+      !finite(minimum) || !(minimum > u->particle_mass_mev) ||
+      minimum < u->lower_energy_limit_mev || minimum > u->spline.axis.high) {
+    r.status = TS_INVALID_VIEW;
+    return r;
+  }
+  r.status = TS_SUCCESS;
+  r.value = minimum;
+  return r;
+}
+static tq_t cq_range(const cont_t* c, int32_t pid, double e) {
+  c8mi_native_result n = oracle_nq_range(c->t, pid, e);
+  tq_t r = {from_native(n.status, 1), n.status == C8MI_QUERY_SUCCESS ? n.value : 0.};
+  return r;
+}
+static tq_t cq_energy_after_loss(const cont_t* c, int32_t pid, double e0, double x) {
+  tq_t r = {TS_NON_FINITE, 0.};
+  if (!finite(x) || x < 0.) return r;
+  tq_t initial = cq_range(c, pid, e0);
+  if (initial.status != TS_SUCCESS) return initial;
+  if (x >= initial.value && x != 0.) {
+    r.status = TS_TRANSPORT_CUT;
+    return r;
+  }
+  c8mi_native_result n = oracle_nq_energy_after_loss(c->t, pid, e0, x);
+  r.status = from_native(n.status, 1);
+  r.value = n.status == C8MI_QUERY_SUCCESS ? n.value : 0.;
+  return r;
+}
+static tq_t dq_total_rate(const cont_t* c, int32_t pid, double e) {
+  c8mi_native_result n = oracle_nq_total_rate(c->t, pid, e);
+  tq_t r = {from_native(n.status, 0), n.status == C8MI_QUERY_SUCCESS ? n.value : 0.};
+  return r;
+}
+
+/* encode / decode of the normalized pair split (PhotonPairKinematics.inl:29-72) */
+static int pk_finite(double v) {
+  const double maximum = 1.7976931348623157e308;
+  return v == v && v <= maximum && v >= -maximum;
+}
+static int decode_pair_split(double e_gev, double s, double* rho) {
+  if (!pk_finite(e_gev) || !pk_finite(s) || !(e_gev > 2. * ELECTRON_MASS_GEV)) return 0;
+  const double tol = 64. * 2.2204460492503131e-16;
+  if (s < -tol || s > 1. + tol) return 0;
+  double b = s < 0. ? 0. : (s > 1. ? 1. : s);
+  double lower = ELECTRON_MASS_GEV / e_gev;
+  double width = 1. - 2. * lower;
+  *rho = lower + width * b;
+  return pk_finite(*rho) && *rho >= lower && *rho <= 1. - lower;
+}
+static int encode_pair_split(double e_gev, double rho, double* s) {
+  if (!pk_finite(e_gev) || !pk_finite(rho) || !(e_gev > 2. * ELECTRON_MASS_GEV)) return 0;
+  double lower = ELECTRON_MASS_GEV / e_gev;
+  double width = 1. - 2. * lower;
+  double upper = 1. - lower;
+  const double tol = 64. * 2.2204460492503131e-16;
+  if (!(width > 0.) || rho < lower - tol || rho > upper + tol) return 0;
+  double b = rho < lower ? lower : (rho > upper ? upper : rho);
+  *s = (b - lower) / width;
+  return pk_finite(*s) && *s >= 0. && *s <= 1.;
+}
+
+/* query_discrete_loss_fraction (DiscretePhysicsQueries.inl:69-96) */
+static tq_t dq_loss_fraction(const cont_t* c, int32_t pid, int32_t process, uint64_t comp,
+                             double e, double q) {
+  tq_t r;
+  if (process == P_PHOTON_PAIR_FS && pid == 22) {
+    c8mi_native_result n = oracle_nq_loss_fraction(c->t, pid, P_PHOTON_PAIR, comp, e, q, 1);
+    r.status = from_native(n.status, 0);
+    r.value = n.status == C8MI_QUERY_SUCCESS ? n.value : 0.;
+    if (r.status != TS_SUCCESS) return r;
+    double s = 0.;
+    if (!encode_pair_split(e / 1000., r.value, &s)) {
+      r.status = TS_INVERSE_CDF;
+      r.value = 0.;
+      return r;
+    }
+    r.value = s;
+    return r;
+  }
+  c8mi_native_result n = oracle_nq_loss_fraction(c->t, pid, process, comp, e, q, 0);
+  r.status = from_native(n.status, 0);
+  r.value = n.status == C8MI_QUERY_SUCCESS ? n.value : 0.;
+  return r;
+}
+
+/* select_rate_column_by_uniform (DiscretePhysicsQueries.inl:24-67) */
+typedef struct {
+  int status;
+  int32_t process;
+  uint64_t component;
+  double quantile;
+  int selected;
+} sel_t;
+
+static sel_t select_column(const cont_t* c, int32_t pid, double e, double u) {
+  sel_t r = {TS_NON_FINITE, 0, 0, 0., 0};
+  if (!finite(u) || u < 0. || u >= 1.) return r;
+  c8mi_native_result tot = oracle_nq_selection_total_rate(c->t, pid, e);
+  int ts = from_native(tot.status, 0);
+  if (ts != TS_SUCCESS) {
+    r.status = ts;
+    return r;
+  }
+  double total = tot.value;
+  r.status = TS_SUCCESS;
+  if (!(total > 0.)) return r;
+  double sampled = u * total;
+  uint32_t begin, end;
+  oracle_nq_dndx_range(c->t, pid, &begin, &end);
+  for (uint32_t pos = begin; pos < end; ++pos) {
+    const c8mi_native_dndx_column* col = oracle_nq_selection_dndx(c->t, pos);
+    if (!col || col->pdg_id != pid) {
+      r.status = TS_INVALID_VIEW;
+      return r;
+    }
+    c8mi_native_result rate = {C8MI_QUERY_SUCCESS, 0, 0, 0, 0.};
+    if (!oracle_nq_rate_below_threshold(col, e)) rate = oracle_nq_rate_for_column(c->t, col, e);
+    int rs = from_native(rate.status, 0);
+    double rv = rate.status == C8MI_QUERY_SUCCESS ? rate.value : 0.;
+    if (rs != TS_SUCCESS) {
+      r.status = rs;
+      r.process = col->process_id;
+      r.component = col->component_hash;
+      return r;
+    }
+    sampled -= rv;
+    if (sampled < 0.) {
+      double quantile = (-sampled) / rv;
+      r.process = col->process_id;
+      r.component = col->component_hash;
+      if (!finite(quantile) || quantile < 0. || quantile > 1.) {
+        r.status = TS_INVALID_VIEW;
+        return r;
+      }
+      r.quantile = quantile;
+      r.selected = 1;
+      return r;
+    }
+  }
+  return r;
+}
+
+/* process_capability (ProcessCapabilities.inl:45-96) reduced to the classes used */
+enum { CAP_PHOTON_PAIR, CAP_PHOTON_MU_PAIR, CAP_BREMS, CAP_COMPTON, CAP_PHOTOELECTRIC,
+       CAP_ANNIHILATION, CAP_IONIZATION, CAP_CPU_ONLY, CAP_NOT_IMPLEMENTED };
+static int capability(int32_t pid, int32_t process) {
+  if (pid == 22 && process == P_PHOTON_MU_PAIR) return CAP_PHOTON_MU_PAIR;
+  if (pid == 22 && process == P_PHOTON_PAIR) return CAP_PHOTON_PAIR;
+  if (is_ee(pid) && process == P_BREMS) return CAP_BREMS;
+  if (pid == 22 && process == P_COMPTON) return CAP_COMPTON;
+  if (pid == 22 && process == P_PHOTOELECTRIC) return CAP_PHOTOELECTRIC;
+  if (pid == -11 && process == P_ANNIHILATION) return CAP_ANNIHILATION;
+  if (is_charged_lepton(pid) && process == P_IONIZATION) return CAP_IONIZATION;
+  if (is_charged_lepton(pid) && process == P_EPAIR) return CAP_CPU_ONLY;
+  if (is_muon(pid) && process == P_BREMS) return CAP_BREMS;
+  switch (process) {
+    case 1000000005: case 1000000006: case 1000000007: case 1000000008: case 1000000011:
+    case 1000000014:
+      return CAP_CPU_ONLY;
+    default: return CAP_NOT_IMPLEMENTED;
+  }
+}
+static int requires_replay(int32_t pid, int32_t process, double quantile) {
+  int cap = capability(pid, process);
+  if (cap == CAP_CPU_ONLY) return 1;
+  if (cap == CAP_PHOTON_MU_PAIR || cap == CAP_NOT_IMPLEMENTED) return 0;
+  return quantile < 0. || quantile > 1.; /* NativeSelectionReplayEndpointWidth = 0 */
+}
+
+/* ========================== records ========================== */
+
+typedef struct {
+  ps_t particle;
+  int32_t deferred_reason; /* -1: none */
+  int32_t status;
+  int32_t process_id;
+  uint32_t vertex_reached;
+  uint64_t component_hash;
+  double total_rate;
+  double interaction_grammage;
+  double mass_density;
+  double particle_mass_gev;
+  double energy_fraction;
+  double process_uniform;
+  double loss_quantile;
+} inter_t;
+
+typedef struct {
+  int flag;
+  int reason;
+  ps_t particle;
+  int32_t process_id;
+} fb_t;
+
+static void set_fallback(fb_t* f, const inter_t* in, int reason) {
+  f->flag = 1;
+  f->reason = reason;
+  f->particle = in->particle;
+  f->process_id = in->process_id;
+}
+
+static double uniform(uint64_t seed, uint64_t shower, const ps_t* p, uint32_t process,
+                      uint64_t draw) {
+  return oracle_uniform_open01(seed, shower, p->history_id, p->step_id, process, draw, 0);
+}
+
+/* ========================== interaction selection ========================== */
+
+typedef struct {
+  const view_t* t;
+  cont_t cont;
+  const c8mi_environment* env;
+  const c8mi_em_config* cfg;
+  uint64_t seed, shower;
+} ctx_t;
+
+/* select_discrete_interaction (InteractionSelection.inl:12-215); muons are not
+ * transported by this path and never reach it. */
+static void select_interaction(const ctx_t* x, const ps_t* p, inter_t* in, fb_t* fb) {
+  memset(in, 0, sizeof *in);
+  in->particle = *p;
+  in->deferred_reason = -1;
+  double e_mev = p->energy_gev * 1000.;
+  if (p->time_s > PARTICLE_CUT_TIME_S) {
+    in->status = IS_PARTICLE_CUT;
+    in->interaction_grammage = INFINITY;
+    return;
+  }
+  int charged = is_charged_lepton(p->pid);
+  int photon = p->pid == 22;
+  if (photon && e_mev < x->cont.em_cut_mev) {
+    in->status = IS_PARTICLE_CUT;
+    in->interaction_grammage = INFINITY;
+    return;
+  }
+  if (charged) {
+    tq_t mass = cq_mass(&x->cont, p->pid);
+    tq_t tmass = cq_transport_mass(&x->cont, p->pid);
+    if (mass.status == TS_SUCCESS) in->particle_mass_gev = mass.value / 1000.;
+    tq_t minimum = cq_minimum_energy(&x->cont, p->pid);
+    tq_t cut = cq_transport_cut(&x->cont, p->pid);
+    int below = tmass.status == TS_SUCCESS && cut.status == TS_SUCCESS &&
+                e_mev - tmass.value < cut.value;
+    if (below || (minimum.status == TS_SUCCESS && e_mev <= minimum.value)) {
+      in->status = IS_PARTICLE_CUT;
+      in->interaction_grammage = INFINITY;
+      return;
+    }
+  }
+  tq_t total = dq_total_rate(&x->cont, p->pid, e_mev);
+  if (total.status != TS_SUCCESS) {
+    set_fallback(fb, in, fallback_reason(total.status));
+    return;
+  }
+  in->total_rate = total.value;
+  if (!(total.value > 0.)) {
+    in->status = IS_NO_DISCRETE;
+    in->interaction_grammage = INFINITY;
+    return;
+  }
+  double ud = uniform(x->seed, x->shower, p, RP_DISTANCE, 0);
+  in->interaction_grammage = -log(ud) / total.value;
+  in->process_uniform = uniform(x->seed, x->shower, p, RP_COLUMN, 0);
+  if (charged) {
+    in->status = IS_DISTANCE_SAMPLED;
+    return;
+  }
+  double us = uniform(x->seed, x->shower, p, RP_SELECTION, 0);
+  sel_t s = select_column(&x->cont, p->pid, e_mev, us);
+  if (s.status != TS_SUCCESS) {
+    set_fallback(fb, in, fallback_reason(s.status));
+    return;
+  }
+  if (!s.selected) {
+    set_fallback(fb, in, FR_ZERO_TOTAL_RATE);
+    return;
+  }
+  in->process_id = s.process;
+  in->component_hash = s.component;
+  in->loss_quantile = s.quantile;
+  if (requires_replay(p->pid, s.process, s.quantile)) {
+    in->deferred_reason = FR_SELECTION_REPLAY;
+    return; /* status stays Selected (0) */
+  }
+  tq_t loss = dq_loss_fraction(&x->cont, p->pid, s.process, s.component, e_mev, s.quantile);
+  if (loss.status != TS_SUCCESS) {
+    int reason = fallback_reason(loss.status);
+    if (requires_selected_loss(reason)) {
+      in->deferred_reason = reason;
+      in->energy_fraction = loss.value;
+      return;
+    }
+    set_fallback(fb, in, reason);
+    return;
+  }
+  in->status = IS_SELECTED;
+  in->energy_fraction = loss.value;
+}
+
+/* ========================== profile / ledger ========================== */
+
+typedef struct {
+  int64_t* photons;
+  int64_t* electrons;
+  int64_t* positrons;
+  int64_t* energy_loss;
+  int64_t counters[OR_EM_COUNTERS];
+} prof_t;
+
+static long long round_ll(double v) { return (long long)rint(v); }
+
+static void add_fixed(prof_t* pr, int64_t* address, double contribution, double scale) {
+  double scaled = contribution * scale;
+  if (!finite(scaled) || fabs(scaled) >= 9.2233720368547748e18) {
+    pr->counters[OR_EMC_OVERFLOWS]++;
+    return;
+  }
+  long long inc = round_ll(scaled);
+  long long prev = *address;
+  if ((inc > 0 && prev > INT64_MAX - inc) || (inc < 0 && prev < INT64_MIN - inc)) {
+    pr->counters[OR_EMC_OVERFLOWS]++;
+    return;
+  }
+  *address = prev + inc;
+}
+
+/* project_profile_grammage (ProfileProjection.inl:12-29) */
+static double project_grammage(const c8mi_em_config* c, const double p[3]) {
+  double proj = 0.;
+  for (int a = 0; a < 3; ++a) proj += (p[a] - c->axis_start_m[a]) * c->axis_direction[a];
+  double fb = proj / c->axis_step_m;
+  if (fb < 0.) return c->axis_grammage_g_per_cm2[0];
+  size_t lower = (size_t)fb;
+  size_t upper = lower + 1;
+  if (upper >= c->axis_support_count) return c->axis_grammage_g_per_cm2[c->axis_support_count - 1];
+  double f = fb - (double)lower;
+  return c->axis_grammage_g_per_cm2[upper] * f + c->axis_grammage_g_per_cm2[lower] * (1. - f);
+}
+
+/* longitudinal_profile_bins (LongitudinalCrossings.inl:12-57) for the Both and
+ * Forward modes */
+static void crossing_bins(double start, double end, size_t count, uint32_t mode, size_t* b,
+                          size_t* e) {
+  *b = *e = 0;
+  if (mode == C8MI_PROFILE_CROSSING_FORWARD && !(end > start)) return;
+  if (count == 0 || start == end || !(start >= -1.7976931348623157e308 && start <= 1.7976931348623157e308) ||
+      !(end >= -1.7976931348623157e308 && end <= 1.7976931348623157e308))
+    return;
+  double low = start < end ? start : end, high = start < end ? end : start;
+  double first = floor(low) + 1., stop = floor(high) + 1.;
+  double limit = (double)count;
+  first = first < 0. ? 0. : first;
+  stop = stop > limit ? limit : stop;
+  if (!(first < stop)) return;
+  *b = (size_t)first;
+  *e = (size_t)stop;
+}
+
+static void particle_profile(prof_t* pr, const c8mi_em_config* c, int32_t pid, double start,
+                             double end, double weight) {
+  size_t b, e;
+  crossing_bins(start / c->profile_bin_width_g_per_cm2, end / c->profile_bin_width_g_per_cm2,
+                c->profile_bins, c->profile_crossing_mode, &b, &e);
+  if (b == e) return;
+  int64_t* h = pid == 22 ? pr->photons : pid == 11 ? pr->electrons : pid == -11 ? pr->positrons
+                                                                                : NULL;
+  if (!h) {
+    pr->counters[OR_EMC_INVALID]++;
+    return;
+  }
+  for (size_t k = b; k < e; ++k) add_fixed(pr, h + k, weight, c->profile_weight_scale);
+}
+
+/* accumulate_energy_profile (ProfileAccumulation.inl:110-155) */
+static void energy_profile(prof_t* pr, const c8mi_em_config* c, double start, double end,
+                           double deposit) {
+  if (!(deposit > 0.) || c->profile_bins == 0) return;
+  pr->counters[OR_EMC_DEPOSITED_STEPS]++;
+  if (start > end) {
+    double t = start;
+    start = end;
+    end = t;
+  }
+  double delta = end - start;
+  int maximum_bin = (int)(c->profile_bins - 1);
+  double w = c->profile_bin_width_g_per_cm2;
+  int first = (int)(start / w);
+  first = first < 0 ? 0 : first;
+  first = first > maximum_bin ? maximum_bin : first;
+  if (delta < c->profile_energy_loss_threshold_g_per_cm2) {
+    add_fixed(pr, pr->energy_loss + first, deposit, c->profile_energy_scale);
+    return;
+  }
+  int last = (int)(end / w);
+  last = last < 0 ? 0 : last;
+  last = last > maximum_bin ? maximum_bin : last;
+  double dens = deposit / delta;
+  if (first == last) {
+    add_fixed(pr, pr->energy_loss + first, dens * delta, c->profile_energy_scale);
+    return;
+  }
+  add_fixed(pr, pr->energy_loss + first, dens * (1. + (double)first) * w - dens * start,
+            c->profile_energy_scale);
+  add_fixed(pr, pr->energy_loss + last, dens * end - dens * (double)last * w,
+            c->profile_energy_scale);
+  for (int bin = first + 1; bin < last; ++bin)
+    add_fixed(pr, pr->energy_loss + bin, dens * w, c->profile_energy_scale);
+}
+
+/* accumulate_point_energy_profile (ProfileAccumulation.inl:157-175) */
+static void point_energy_profile(prof_t* pr, const c8mi_em_config* c, double x, double deposit) {
+  if (!(deposit > 0.) || c->profile_bins == 0) return;
+  int maximum_bin = (int)(c->profile_bins - 1);
+  int bin = (int)(x / c->profile_bin_width_g_per_cm2);
+  bin = bin < 0 ? 0 : bin;
+  bin = bin > maximum_bin ? maximum_bin : bin;
+  pr->counters[OR_EMC_DEPOSITED_STEPS]++;
+  add_fixed(pr, pr->energy_loss + bin, deposit, c->profile_energy_scale);
+}
+
+static void ledger_add(prof_t* pr, const c8mi_em_config* c, int term, double value) {
+  add_fixed(pr, &pr->counters[term], value, c->profile_energy_scale);
+}
+
+/* ========================== photon transport ========================== */
+
+typedef struct {
+  inter_t in; /* updated interaction (vertex state when reached) */
+  ps_t start, end;
+  int limit;
+  double cut_deposit;
+  int observed_before_cut;
+} ptrans_t;
+
+static void advance_straight(const ps_t* s, double d, ps_t* e) {
+  *e = *s;
+  for (int a = 0; a < 3; ++a) e->position_m[a] = s->position_m[a] + d * s->direction[a];
+  e->time_s = s->time_s + d / C_LIGHT;
+}
+
+/* transport_photon (PhotonTransportStep.inl:57-261) */
+static void transport_photon(const ctx_t* x, const inter_t* in, ptrans_t* out, fb_t* fb) {
+  const c8mi_environment* env = x->env;
+  ps_t start = in->particle;
+  memset(out, 0, sizeof *out);
+  if (start.pid != 22) {
+    set_fallback(fb, in, FR_UNSUPPORTED_PARTICLE);
+    return;
+  }
+  if (in->status != IS_SELECTED && in->status != IS_NO_DISCRETE && in->status != IS_PARTICLE_CUT) {
+    set_fallback(fb, in, FR_INVALID_FINAL_STATE);
+    return;
+  }
+  layer_t layer = query_environment(env, start.position_m, start.direction, start.environment_node);
+  if (layer.status != ES_SUCCESS) {
+    set_fallback(fb, in, FR_UNSUPPORTED_GEOMETRY);
+    return;
+  }
+  start.environment_node = layer.node;
+  if (in->status == IS_PARTICLE_CUT) {
+    out->in = *in;
+    out->in.vertex_reached = 0;
+    out->start = start;
+    out->end = start;
+    out->end.step_id++;
+    out->limit = PL_CUT;
+    out->cut_deposit = start.energy_gev;
+    return;
+  }
+  boundary_t bnd = distance_to_boundary(env, start.position_m, start.direction, start.environment_node);
+  double obs = observation_distance(env, start.position_m, start.direction);
+  int has_b = bnd.status == ES_SUCCESS, has_o = obs > 0.;
+  if (!has_b && !has_o) {
+    set_fallback(fb, in, FR_UNSUPPORTED_GEOMETRY);
+    return;
+  }
+  int obs_wins = has_o && (!has_b || obs < bnd.distance);
+  double bd = obs_wins ? obs : bnd.distance;
+  gq_t bg = competition_grammage(env, layer.node, start.position_m, start.direction, bd);
+  if (bg.status != ES_SUCCESS) {
+    set_fallback(fb, in, FR_GRAMMAGE);
+    return;
+  }
+  out->in = *in;
+  out->in.vertex_reached = 0;
+  out->start = start;
+  int reaches = in->status == IS_SELECTED && finite(in->interaction_grammage) &&
+                in->interaction_grammage >= 0. && in->interaction_grammage <= bg.value;
+  if (reaches) {
+    gq_t dist = environment_arclength(env, layer.node, start.position_m, start.direction,
+                                      in->interaction_grammage);
+    if (dist.status != ES_SUCCESS || dist.value > bd * (1. + 1.e-12)) {
+      set_fallback(fb, in, FR_INVERSE_GRAMMAGE);
+      return;
+    }
+    advance_straight(&start, dist.value, &out->end);
+    layer_t v = query_environment(env, out->end.position_m, out->end.direction, start.environment_node);
+    if (v.status != ES_SUCCESS || !finite(v.density) || !(v.density > 0.)) {
+      set_fallback(fb, in, FR_VERTEX_LOOKUP);
+      return;
+    }
+    out->limit = PL_INTERACTION;
+    out->in.particle = out->end;
+    out->in.mass_density = v.density;
+    if (out->end.time_s > PARTICLE_CUT_TIME_S) {
+      out->end.step_id++;
+      out->limit = PL_CUT;
+      out->cut_deposit = start.energy_gev;
+    } else {
+      out->in.vertex_reached = 1;
+    }
+    return;
+  }
+  if (!finite(bg.value)) {
+    set_fallback(fb, in, FR_GRAMMAGE);
+    return;
+  }
+  advance_straight(&start, bd, &out->end);
+  out->end.step_id++;
+  if (obs_wins) {
+    out->limit = PL_OBSERVATION;
+  } else {
+    int32_t node = bnd.next_node;
+    out->end.environment_node = node;
+    const c8mi_env_model* m = get_model(env, node);
+    if (!m) {
+      out->limit = PL_ESCAPED;
+    } else {
+      layer_t next = query_environment(env, out->end.position_m, out->end.direction, node);
+      if (next.status != ES_SUCCESS || node == layer.node) {
+        set_fallback(fb, in, FR_UNSUPPORTED_GEOMETRY);
+        return;
+      }
+      out->limit = PL_LAYER;
+      out->end.medium_id = m->medium_id;
+    }
+  }
+  if (out->end.time_s > PARTICLE_CUT_TIME_S) {
+    out->observed_before_cut = out->limit == PL_OBSERVATION;
+    out->limit = PL_CUT;
+    out->cut_deposit = start.energy_gev;
+  }
+}
+
+/* ========================== final-state helpers ========================== */
+
+static double bounded_cosine(double v) {
+  double m = v < 1. ? v : 1.;
+  return -1. > m ? -1. : m;
+}
+
+/* sauter_cosine (PhotonFinalStateStep.inl:61-71) */
+static double sauter_cosine(double e, double u) {
+  double m2 = ELECTRON_MASS_GEV * ELECTRON_MASS_GEV;
+  double t = e * e - m2;
+  double p = sqrt(0. > t ? 0. : t);
+  double coord = 2. * u - 1.;
+  double emp = m2 / (e + p);
+  double omc = emp * (1. - coord) / (p * coord + e);
+  return bounded_cosine(1. - omc);
+}
+
+/* compton_cosines (PhotonFinalStateStep.inl:72-96) */
+static int compton_cosines(double e, double v, double* pc, double* ec) {
+  double scattered = e * (1. - v);
+  double pe2 = 2. * v * e * ELECTRON_MASS_GEV + v * v * e * e;
+  if (!finite(e) || !(e > 0.) || !finite(v) || !(v > 0.) || !(v < 1.) || !(scattered > 0.) ||
+      !(pe2 > 0.))
+    return 0;
+  *pc = 1. - (v * ELECTRON_MASS_GEV) / scattered;
+  *ec = v * (e + ELECTRON_MASS_GEV) / sqrt(pe2);
+  const double tol = 1.e-12;
+  if (!finite(*pc) || !finite(*ec) || *pc < -1. - tol || *pc > 1. + tol || *ec < -1. - tol ||
+      *ec > 1. + tol)
+    return 0;
+  *pc = bounded_cosine(*pc);
+  *ec = bounded_cosine(*ec);
+  return 1;
+}
+
+/* photoelectric_energy (PhotonFinalStateStep.inl:97-133) */
+static int photoelectric_energy(const c8mi_photon_pair_lpm* s, uint64_t comp, double e,
+                                double* total, double* fraction) {
+  if (!finite(e) || !(e > 0.) || s->component_count == 0 || s->component_count > 16 ||
+      !finite(s->fine_structure_constant) || !(s->fine_structure_constant > 0.))
+    return 0;
+  double z = 0.;
+  for (uint32_t i = 0; i < s->component_count; ++i)
+    if (s->components[i].component_hash == comp) {
+      z = s->components[i].nuclear_charge;
+      break;
+    }
+  if (!finite(z) || !(z > 0.)) return 0;
+  double za = z * s->fine_structure_constant;
+  double binding = za * za * ELECTRON_MASS_GEV / 2.;
+  double kinetic = e - binding;
+  if (!finite(binding) || !finite(kinetic) || !(kinetic >= 0.)) return 0;
+  *total = ELECTRON_MASS_GEV + kinetic;
+  *fraction = kinetic / e;
+  return finite(*total) && *total >= ELECTRON_MASS_GEV && finite(*fraction) && *fraction >= 0. &&
+         *fraction <= 1.;
+}
+
+/* deflect_photon_child (PhotonFinalStateStep.inl:134-160) and deflect_lepton_child
+ * (LeptonFinalStateStep.inl:86-116; the lepton form bounds the cosine first) */
+static void deflect(const double in[3], double cosine, double azimuth, double out[3], int bound) {
+  double t = in[0] * in[0] + in[1] * in[1];
+  double tr = sqrt(0. > t ? 0. : t);
+  double cp = 1., sp = 0.;
+  if (tr > 0.) {
+    cp = in[0] / tr;
+    sp = in[1] / tr;
+  }
+  double ct = in[2], st = tr;
+  double rx[3] = {ct * cp, ct * sp, -st};
+  double ry[3] = {-sp, cp, 0.};
+  double c = cosine;
+  if (bound) {
+    double m = 1. < c ? 1. : c;
+    c = -1. > m ? -1. : m;
+  }
+  double s2 = (1. - c) * (1. + c);
+  double s = sqrt(0. > s2 ? 0. : s2);
+  double lx = s * cos(azimuth), ly = s * sin(azimuth);
+  double z2 = 1. - lx * lx - ly * ly;
+  double lz = sqrt(0. > z2 ? 0. : z2);
+  if (c < 0.) lz = -lz;
+  for (int a = 0; a < 3; ++a) out[a] = lz * in[a] + lx * rx[a] + ly * ry[a];
+}
+
+/* apply_em_thinning (EmThinning.inl:28-104) */
+typedef struct {
+  int status; /* 0 not applied, 1 Hillas, 2 statistical, 3 invalid */
+  uint32_t keep;
+  double w1, w2;
+} thin_t;
+
+static thin_t em_thinning(const c8mi_em_thinning* c, double pe, double pw, double e1, double e2,
+                          double u1, double u2) {
+  thin_t r = {3, 0, 0., 0.};
+  if (!finite(pe) || !(pe > 0.) || !finite(pw) || pw < 0. || !finite(e1) || !(e1 > 0.) ||
+      !finite(e2) || !(e2 > 0.) || !finite(u1) || !(u1 > 0.) || !(u1 < 1.) || !finite(u2) ||
+      !(u2 > 0.) || !(u2 < 1.))
+    return r;
+  thin_t unchanged = {0, 3U, pw, pw};
+  if (c->enabled == 0) return unchanged;
+  if (!finite(c->threshold_gev) || !(c->threshold_gev > 0.) || !finite(c->maximum_weight) ||
+      !(c->maximum_weight > 0.))
+    return r;
+  if (pw >= c->maximum_weight || pe > c->threshold_gev) return unchanged;
+  double sum = e1 + e2;
+  if (!finite(sum) || !(sum > 0.)) return r;
+  double p1 = e1 / sum, p2 = e2 / sum;
+  double f1 = 1. / p1, f2 = 1. / p2, fm = c->maximum_weight / pw;
+  thin_t o = {0, 0U, 0., 0.};
+  if (f1 <= fm && f2 <= fm) {
+    o.status = 1;
+    if (u1 <= p1) {
+      o.keep = 1U;
+      o.w1 = pw * f1;
+    } else {
+      o.keep = 2U;
+      o.w2 = pw * f2;
+    }
+  } else {
+    o.status = 2;
+    double l1 = fmin(f1, fm), l2 = fmin(f2, fm);
+    if (u1 * l1 <= 1.) {
+      o.keep |= 1U;
+      o.w1 = pw * l1;
+    }
+    if (u2 * l2 <= 1.) {
+      o.keep |= 2U;
+      o.w2 = pw * l2;
+    }
+  }
+  if (c->erase_zero_weight == 0) {
+    o.keep = 3U;
+  } else {
+    if (o.w1 == 0.) o.keep &= ~1U;
+    if (o.w2 == 0.) o.keep &= ~2U;
+  }
+  return o;
+}
+
+static int two_child_thinning(const ctx_t* x, const ps_t* parent, int32_t process, double e1,
+                              double e2, int lepton, thin_t* t) {
+  double u1 = uniform(x->seed, x->shower, parent, (uint32_t)process, D_THIN_FIRST);
+  double u2 = uniform(x->seed, x->shower, parent, (uint32_t)process, D_THIN_SECOND);
+  c8mi_em_thinning cfg = x->cfg->thinning;
+  if (lepton && !is_ee(parent->pid)) cfg.enabled = 0;
+  *t = em_thinning(&cfg, parent->energy_gev, parent->weight, e1, e2, u1, u2);
+  return t->status != 3;
+}
+
+/* photon_pair_lpm_suppression_factor (PhotonPairLpm.inl:58-155); status 0 on success */
+static int pair_lpm(const c8mi_photon_pair_lpm* s, uint64_t comp, double e, double xx,
+                    double rho, double* prob) {
+  if (s->component_count == 0 || s->component_count > 16 ||
+      !finite(s->baseline_mass_density_g_per_cm3) || !(s->baseline_mass_density_g_per_cm3 > 0.) ||
+      !finite(s->molecular_density_per_cm3) || !(s->molecular_density_per_cm3 > 0.) ||
+      !finite(s->sum_charge) || !(s->sum_charge > 0.) || !finite(s->e_lpm_mev) ||
+      !(s->e_lpm_mev > 0.) || !finite(s->classical_electron_radius_cm) ||
+      !(s->classical_electron_radius_cm > 0.) || !finite(s->fine_structure_constant) ||
+      !(s->fine_structure_constant > 0.))
+    return 2;
+  if (!finite(e) || !(e > 0.) || !finite(xx) || !(xx > 0.) || !(xx < 1.) || !finite(rho) ||
+      !(rho > 0.))
+    return 3;
+  const c8mi_photon_pair_lpm_component* c = NULL;
+  for (uint32_t i = 0; i < s->component_count; ++i)
+    if (s->components[i].component_hash == comp) {
+      c = &s->components[i];
+      break;
+    }
+  if (!c) return 1;
+  if (!finite(c->nuclear_charge) || !(c->nuclear_charge > 0.) ||
+      !finite(c->radiation_log_constant) || !(c->radiation_log_constant > 0.))
+    return 2;
+  const double Pi = 3.141592653589793238462643383279502884;
+  const double SqrtTwo = 1.414213562373095048801688724209698079;
+  double dc = rho / s->baseline_mass_density_g_per_cm3;
+  double zm = pow(c->nuclear_charge, -1. / 3.);
+  double s1 = 1. / (zm * c->radiation_log_constant);
+  s1 = s1 * s1 * SqrtTwo;
+  double sp = 0.125 * sqrt(s->e_lpm_mev / (dc * e * xx * (1. - xx)));
+  double h = log(sp) / log(s1);
+  double xi = 1.;
+  if (sp < s1) xi = 2.;
+  else if (sp < 1.) xi = 1. + h - 0.08 * (1. - h) * (1. - (1. - h) * (1. - h)) / log(s1);
+  double g0 = s->classical_electron_radius_cm / (s->fine_structure_constant * xx);
+  double gamma = 1. + 4. * Pi * s->sum_charge * s->classical_electron_radius_cm * g0 * g0 *
+                          s->molecular_density_per_cm3 * dc;
+  double ss = sp / sqrt(xi) * gamma;
+  double s2 = ss * ss;
+  double fi, g;
+  if (ss < 1.54954)
+    fi = 1. - exp(-6. * ss * (1. + (3. - Pi) * ss) + s2 * ss / (0.623 + 0.796 * ss + 0.658 * s2));
+  else
+    fi = 1. - 0.012 / (s2 * s2);
+  if (ss < 0.710390) {
+    double psi = 1. - exp(-4. * ss - 8. * s2 / (1. + 3.936 * ss + 4.97 * s2 - 0.05 * s2 * ss +
+                                                7.50 * s2 * s2));
+    g = 3. * psi - 2. * fi;
+  } else if (ss < 0.904912) {
+    g = 36. * s2 / (36. * s2 + 1.);
+  } else {
+    g = 1. - 0.022 / (s2 * s2);
+  }
+  double p = ((xi / 3.) * (g / (gamma * gamma) + 2. * (xx * xx + (1. - xx) * (1. - xx)) * fi / gamma)) /
+             (1. - (4. / 3.) * xx * (1. - xx));
+  if (!finite(p) || p < 0.) return 4;
+  *prob = p;
+  return 0;
+}
+
+/* brems_lpm_suppression_factor (BremsLpm.inl:58-162); status 0 on success */
+static int brems_lpm(const c8mi_brems_lpm* s, uint64_t comp, double e, double v, double rho,
+                     double* prob) {
+  if (s->component_count == 0 || s->component_count > 16 ||
+      !finite(s->baseline_mass_density_g_per_cm3) || !(s->baseline_mass_density_g_per_cm3 > 0.) ||
+      !finite(s->molecular_density_per_cm3) || !(s->molecular_density_per_cm3 > 0.) ||
+      !finite(s->sum_charge) || !(s->sum_charge > 0.) || !finite(s->e_lpm_mev) ||
+      !(s->e_lpm_mev > 0.) || !finite(s->lepton_mass_mev) || !(s->lepton_mass_mev > 0.) ||
+      !finite(s->electron_mass_mev) || !(s->electron_mass_mev > 0.) ||
+      !finite(s->muon_mass_mev) || !(s->muon_mass_mev > 0.) ||
+      !finite(s->classical_electron_radius_cm) || !(s->classical_electron_radius_cm > 0.) ||
+      !finite(s->fine_structure_constant) || !(s->fine_structure_constant > 0.))
+    return 2;
+  if (!finite(e) || !(e > s->lepton_mass_mev) || !finite(v) || !(v > 0.) || !(v < 1.) ||
+      !finite(rho) || !(rho > 0.))
+    return 3;
+  const c8mi_brems_lpm_component* c = NULL;
+  for (uint32_t i = 0; i < s->component_count; ++i)
+    if (s->components[i].component_hash == comp) {
+      c = &s->components[i];
+      break;
+    }
+  if (!c) return 1;
+  if (!finite(c->nuclear_charge) || !(c->nuclear_charge > 0.) || !finite(c->atomic_mass_number) ||
+      !(c->atomic_mass_number > 0.) || !finite(c->radiation_log_constant) ||
+      !(c->radiation_log_constant > 0.))
+    return 2;
+  const double Pi = 3.141592653589793238462643383279502884;
+  const double SqrtTwo = 1.414213562373095048801688724209698079;
+  double dc = rho / s->baseline_mass_density_g_per_cm3;
+  double zm = pow(c->nuclear_charge, -1. / 3.);
+  double dn = 1.54 * pow(c->atomic_mass_number, 0.27);
+  double na = s->lepton_mass_mev / s->muon_mass_mev * dn;
+  double s1 = s->electron_mass_mev / (s->lepton_mass_mev * zm * c->radiation_log_constant);
+  s1 = s1 * s1 * (1. + na * na) * SqrtTwo;
+  double sp = 0.125 * sqrt(s->e_lpm_mev * v / (dc * e * (1. - v)));
+  double h = log(sp) / log(s1);
+  double xi = 1.;
+  if (sp < s1) xi = 2.;
+  else if (sp < 1.) xi = 1. + h - 0.08 * (1. - h) * (1. - (1. - h) * (1. - h)) / log(s1);
+  double g0 = s->classical_electron_radius_cm * s->electron_mass_mev /
+              (s->fine_structure_constant * s->lepton_mass_mev * v);
+  double gamma = 1. + 4. * Pi * s->sum_charge * s->classical_electron_radius_cm * g0 * g0 *
+                          s->molecular_density_per_cm3 * dc;
+  double ss = sp / sqrt(xi) * gamma;
+  double s2 = ss * ss;
+  double fi, g;
+  if (ss < 1.54954)
+    fi = 1. - exp(-6. * ss * (1. + (3. - Pi) * ss) + s2 * ss / (0.623 + 0.796 * ss + 0.658 * s2));
+  else
+    fi = 1. - 0.012 / (s2 * s2);
+  if (ss < 0.710390) {
+    double psi = 1. - exp(-4. * ss - 8. * s2 / (1. + 3.936 * ss + 4.97 * s2 - 0.05 * s2 * ss +
+                                                7.50 * s2 * s2));
+    g = 3. * psi - 2. * fi;
+  } else if (ss < 0.904912) {
+    g = 36. * s2 / (36. * s2 + 1.);
+  } else {
+    g = 1. - 0.022 / (s2 * s2);
+  }
+  double p = ((xi / 3.) * (v * v * g / (gamma * gamma) + 2. * (1. + (1. - v) * (1. - v)) * fi / gamma)) /
+             ((4. / 3.) * (1. - v) + v * v);
+  if (!finite(p) || p < 0.) return 4;
+  *prob = p;
+  return 0;
+}
+
+/* ========================== shower state ========================== */
+
+typedef struct {
+  ps_t* items;
+  uint64_t count, capacity;
+} queue_t;
+
+typedef struct {
+  ctx_t x;
+  queue_t photons, leptons;
+  uint64_t next_history;
+  prof_t prof;
+  ps_t* observations;
+  uint64_t observation_count;
+  c8mi_em_fallback* fallbacks;
+  uint64_t fallback_count;
+  or_em_stats stats;
+} shower_t;
+
+static void push(queue_t* q, const ps_t* p) {
+  if (q->count == q->capacity) {
+    q->capacity = q->capacity ? 2 * q->capacity : 1024;
+    q->items = (ps_t*)realloc(q->items, q->capacity * sizeof(ps_t));
+  }
+  q->items[q->count++] = *p;
+}
+
+static void record_observation(shower_t* s, const ps_t* p) {
+  if (s->observation_count < s->x.cfg->observation_capacity && s->observations)
+    s->observations[s->observation_count] = *p;
+  s->observation_count++;
+}
+
+static void record_fallback(shower_t* s, const fb_t* f, uint32_t stage) {
+  s->prof.counters[OR_EMC_FALLBACK_REASON0 + (f->reason & 31)]++;
+  ledger_add(&s->prof, s->x.cfg, OR_EMC_FALLBACK_ENERGY, f->particle.energy_gev * f->particle.weight);
+  if (s->fallback_count < s->x.cfg->fallback_capacity && s->fallbacks) {
+    c8mi_em_fallback* o = &s->fallbacks[s->fallback_count];
+    memset(o, 0, sizeof *o);
+    o->particle = f->particle;
+    o->reason = f->reason;
+    o->process_id = f->process_id;
+    o->stage = stage;
+  }
+  s->fallback_count++;
+  s->stats.fallbacks++;
+}
+
+static void terminal_energy(shower_t* s, double e, int observed) {
+  if (!(e >= 0.) || !finite(e)) {
+    s->prof.counters[OR_EMC_INVALID]++;
+    return;
+  }
+  ledger_add(&s->prof, s->x.cfg, observed ? OR_EMC_OBSERVED : OR_EMC_ESCAPED, e);
+}
+
+static void thinning_ledger(shower_t* s, int status, uint32_t multiplicity, uint32_t kept,
+                            double parent_energy_weight, double kept_energy_weight) {
+  if (status == 1) s->prof.counters[OR_EMC_THIN_HILLAS]++;
+  else if (status == 2) s->prof.counters[OR_EMC_THIN_STATISTICAL]++;
+  s->prof.counters[OR_EMC_THIN_DISCARDED] += multiplicity - kept;
+  ledger_add(&s->prof, s->x.cfg, OR_EMC_THINNING_ENERGY, parent_energy_weight - kept_energy_weight);
+}
+
+/* ========================== photon front ========================== */
+
+/* One photon: selection, transport, final state, profile and outputs in the
+ * order of the reference's materialisation (PhotonFront.inl:137-397). Outputs:
+ * continuing photon (next), charged children and history IDs. */
+typedef struct {
+  ps_t next[2];
+  uint32_t next_count;
+  ps_t charged[2];
+  uint32_t charged_count;
+  uint32_t child_count;
+} pout_t;
+
+static void photon_source(shower_t* s, const ps_t* p, pout_t* o, uint64_t child_offset,
+                          uint64_t first_history) {
+  const ctx_t* x = &s->x;
+  const c8mi_em_config* cfg = x->cfg;
+  memset(o, 0, sizeof *o);
+  inter_t in;
+  fb_t fb = {0, 0, {0}, 0};
+  select_interaction(x, p, &in, &fb);
+  if (fb.flag) {
+    record_fallback(s, &fb, STAGE_SELECTION);
+    return;
+  }
+  ptrans_t tr;
+  transport_photon(x, &in, &tr, &fb);
+  if (fb.flag) {
+    record_fallback(s, &fb, STAGE_TRANSPORT);
+    return;
+  }
+  s->stats.photon_steps++;
+  /* accumulate_photon_profile_step (ProfileAccumulation.inl:228-257) */
+  double w = tr.start.weight;
+  double gs = project_grammage(cfg, tr.start.position_m);
+  double ge = project_grammage(cfg, tr.end.position_m);
+  particle_profile(&s->prof, cfg, tr.start.pid, gs, ge, w);
+  point_energy_profile(&s->prof, cfg, ge, tr.cut_deposit * w);
+  s->prof.counters[OR_EMC_STEPS]++;
+  if (tr.limit == PL_CUT) {
+    s->prof.counters[OR_EMC_PHOTON_CUTS]++;
+    ledger_add(&s->prof, cfg, OR_EMC_CUT_DEPOSIT, tr.cut_deposit * w);
+    if (tr.observed_before_cut) {
+      ledger_add(&s->prof, cfg, OR_EMC_OVERLAP, tr.cut_deposit * w);
+      terminal_energy(s, tr.end.energy_gev * w, 1);
+      record_observation(s, &tr.end);
+    }
+    return;
+  }
+  if (tr.limit == PL_OBSERVATION) {
+    terminal_energy(s, tr.end.energy_gev * w, 1);
+    record_observation(s, &tr.end);
+    return;
+  }
+  if (tr.limit == PL_ESCAPED) {
+    terminal_energy(s, tr.end.energy_gev * w, 0);
+    record_observation(s, &tr.end);
+    return;
+  }
+  if (tr.limit == PL_LAYER) {
+    o->next[o->next_count++] = tr.end;
+    return;
+  }
+  /* interaction vertex: classify_photon_final_state (PhotonFinalStateStep.inl:273-558) */
+  s->stats.photon_interactions++;
+  const inter_t* vi = &tr.in;
+  const ps_t* parent = &vi->particle;
+  if (vi->deferred_reason >= 0) {
+    fb_t f;
+    if (vi->vertex_reached != 1 || !requires_selected_loss(vi->deferred_reason))
+      set_fallback(&f, vi, FR_INVALID_FINAL_STATE);
+    else
+      set_fallback(&f, vi, vi->deferred_reason);
+    record_fallback(s, &f, STAGE_FINAL);
+    return;
+  }
+  int cap = capability(parent->pid, vi->process_id);
+  fb_t f;
+  if (cap != CAP_PHOTON_PAIR && cap != CAP_COMPTON && cap != CAP_PHOTOELECTRIC) {
+    /* photon muon pairs are left to the host in this build */
+    set_fallback(&f, vi, cap == CAP_CPU_ONLY || cap == CAP_PHOTON_MU_PAIR ? FR_CPU_ONLY
+                                                                          : FR_NOT_IMPLEMENTED);
+    record_fallback(s, &f, STAGE_FINAL);
+    return;
+  }
+  const c8mi_photon_pair_lpm* lpm = &cfg->photon_pair_lpm;
+  if (cap == CAP_COMPTON) {
+    double pc, ec;
+    if (parent->generation == 0xffffffffU || parent->step_id == 0xffffffffffffffffULL ||
+        !compton_cosines(parent->energy_gev, vi->energy_fraction, &pc, &ec)) {
+      set_fallback(&f, vi, FR_INVALID_FINAL_STATE);
+      record_fallback(s, &f, STAGE_FINAL);
+      return;
+    }
+    double v = vi->energy_fraction;
+    double az_u = uniform(x->seed, x->shower, parent, P_COMPTON, D_COMPTON_AZIMUTH);
+    thin_t th;
+    double e1 = parent->energy_gev * (1. - v);
+    double e2 = secondary_transport_energy(11, ELECTRON_MASS_GEV + parent->energy_gev * v,
+                                           ELECTRON_MASS_GEV);
+    if (!two_child_thinning(x, parent, P_COMPTON, e1, e2, 0, &th)) {
+      set_fallback(&f, vi, FR_INVALID_FINAL_STATE);
+      record_fallback(s, &f, STAGE_FINAL);
+      return;
+    }
+    double az = az_u * PHOTON_TWO_PI;
+    ps_t ph = *parent, el = *parent;
+    ph.energy_gev = e1;
+    ph.parent_history_id = parent->history_id;
+    ph.generation = parent->generation + 1;
+    ph.step_id = 0;
+    ph.reserved = 0;
+    ph.weight = th.w1;
+    deflect(parent->direction, pc, az, ph.direction, 0);
+    el.pid = 11;
+    el.energy_gev = e2;
+    el.parent_history_id = parent->history_id;
+    el.generation = parent->generation + 1;
+    el.step_id = 0;
+    el.reserved = 0;
+    el.weight = th.w2;
+    deflect(parent->direction, ec, fmod(az + PHOTON_PI, PHOTON_TWO_PI), el.direction, 0);
+    uint32_t k = 0;
+    if (th.keep & 1U) {
+      ph.history_id = first_history + child_offset + k++;
+      o->next[o->next_count++] = ph;
+    }
+    if (th.keep & 2U) {
+      el.history_id = first_history + child_offset + k++;
+      o->charged[o->charged_count++] = el;
+    }
+    o->child_count = k;
+    s->stats.secondaries += k;
+    double kept_lepton_w = (th.keep & 2U) ? th.w2 : 0.;
+    ledger_add(&s->prof, cfg, OR_EMC_MASS_CORRECTION, mass_correction(11, ELECTRON_MASS_GEV, kept_lepton_w));
+    ledger_add(&s->prof, cfg, OR_EMC_MEDIUM_REST_MASS, ELECTRON_MASS_GEV * w);
+    thinning_ledger(s, th.status, 2, k, (e1 + e2) * w,
+                    ((th.keep & 1U) ? e1 * th.w1 : 0.) + ((th.keep & 2U) ? e2 * th.w2 : 0.));
+    return;
+  }
+  if (cap == CAP_PHOTOELECTRIC) {
+    double etot, frac;
+    if (parent->generation == 0xffffffffU || !finite(vi->energy_fraction) ||
+        fabs(vi->energy_fraction - 1.) > 1.e-12 ||
+        !photoelectric_energy(lpm, vi->component_hash, parent->energy_gev, &etot, &frac)) {
+      set_fallback(&f, vi, FR_INVALID_FINAL_STATE);
+      record_fallback(s, &f, STAGE_FINAL);
+      return;
+    }
+    ps_t el = *parent;
+    el.pid = 11;
+    el.energy_gev = secondary_transport_energy(11, ELECTRON_MASS_GEV + parent->energy_gev * frac,
+                                               ELECTRON_MASS_GEV);
+    el.parent_history_id = parent->history_id;
+    el.history_id = first_history + child_offset;
+    el.generation = parent->generation + 1;
+    el.step_id = 0;
+    el.reserved = 0;
+    o->charged[o->charged_count++] = el;
+    o->child_count = 1;
+    s->stats.secondaries += 1;
+    ledger_add(&s->prof, cfg, OR_EMC_MASS_CORRECTION, mass_correction(11, ELECTRON_MASS_GEV, parent->weight));
+    ledger_add(&s->prof, cfg, OR_EMC_MEDIUM_REST_MASS, ELECTRON_MASS_GEV * w);
+    ledger_add(&s->prof, cfg, OR_EMC_PHOTOELECTRIC_BINDING, tr.end.energy_gev * (1. - frac) * w);
+    thinning_ledger(s, 0, 1, 1, 0., 0.);
+    return;
+  }
+  /* photon pair */
+  if (!finite(parent->energy_gev) || parent->energy_gev < 2. * ELECTRON_MASS_GEV ||
+      parent->generation == 0xffffffffU || parent->step_id == 0xffffffffffffffffULL) {
+    set_fallback(&f, vi, FR_INVALID_FINAL_STATE);
+    record_fallback(s, &f, STAGE_FINAL);
+    return;
+  }
+  double su = uniform(x->seed, x->shower, parent, P_PHOTON_PAIR, D_PAIR_SPLIT);
+  tq_t split = dq_loss_fraction(&x->cont, parent->pid, P_PHOTON_PAIR_FS, vi->component_hash,
+                                parent->energy_gev * 1000., su);
+  double rho = 0.;
+  if (split.status != TS_SUCCESS) {
+    set_fallback(&f, vi, fallback_reason(split.status));
+    record_fallback(s, &f, STAGE_FINAL);
+    return;
+  }
+  if (!decode_pair_split(parent->energy_gev, split.value, &rho)) {
+    set_fallback(&f, vi, FR_INVALID_FINAL_STATE);
+    record_fallback(s, &f, STAGE_FINAL);
+    return;
+  }
+  double ee = parent->energy_gev * rho;
+  double pe = parent->energy_gev - ee;
+  if (!finite(rho) || !(rho > 0.) || !(rho < 1.) || ee < ELECTRON_MASS_GEV || pe < ELECTRON_MASS_GEV) {
+    set_fallback(&f, vi, FR_INVALID_FINAL_STATE);
+    record_fallback(s, &f, STAGE_FINAL);
+    return;
+  }
+  double az_u = uniform(x->seed, x->shower, parent, P_PHOTON_PAIR, D_PAIR_AZIMUTH);
+  double eu = uniform(x->seed, x->shower, parent, P_PHOTON_PAIR, D_PAIR_E_POLAR);
+  double pu = uniform(x->seed, x->shower, parent, P_PHOTON_PAIR, D_PAIR_P_POLAR);
+  double survival = 0.;
+  int ls = pair_lpm(lpm, vi->component_hash, parent->energy_gev * 1000., rho, vi->mass_density,
+                    &survival);
+  if (ls != 0) {
+    set_fallback(&f, vi, ls == 3 ? FR_INVALID_DENSITY : FR_LPM_UNAVAILABLE);
+    record_fallback(s, &f, STAGE_FINAL);
+    return;
+  }
+  double lu = uniform(x->seed, x->shower, parent, P_PHOTON_PAIR, D_PAIR_LPM);
+  if (lu > survival) {
+    ps_t cont = *parent;
+    cont.step_id++;
+    o->next[o->next_count++] = cont;
+    return;
+  }
+  thin_t th;
+  double e1 = secondary_transport_energy(11, ee, ELECTRON_MASS_GEV);
+  double e2 = secondary_transport_energy(-11, pe, ELECTRON_MASS_GEV);
+  if (!two_child_thinning(x, parent, P_PHOTON_PAIR, e1, e2, 0, &th)) {
+    set_fallback(&f, vi, FR_INVALID_FINAL_STATE);
+    record_fallback(s, &f, STAGE_FINAL);
+    return;
+  }
+  double az = az_u * PHOTON_TWO_PI;
+  double ec = sauter_cosine(ee, eu), pcos = sauter_cosine(pe, pu);
+  ps_t el = *parent, po = *parent;
+  el.pid = 11;
+  el.energy_gev = e1;
+  el.parent_history_id = parent->history_id;
+  el.generation = parent->generation + 1;
+  el.step_id = 0;
+  el.reserved = 0;
+  el.weight = th.w1;
+  deflect(parent->direction, ec, az, el.direction, 0);
+  po.pid = -11;
+  po.energy_gev = e2;
+  po.parent_history_id = parent->history_id;
+  po.generation = parent->generation + 1;
+  po.step_id = 0;
+  po.reserved = 0;
+  po.weight = th.w2;
+  deflect(parent->direction, pcos, fmod(az + PHOTON_PI, PHOTON_TWO_PI), po.direction, 0);
+  uint32_t k = 0;
+  if (th.keep & 1U) {
+    el.history_id = first_history + child_offset + k++;
+    o->charged[o->charged_count++] = el;
+  }
+  if (th.keep & 2U) {
+    po.history_id = first_history + child_offset + k++;
+    o->charged[o->charged_count++] = po;
+  }
+  o->child_count = k;
+  s->stats.secondaries += k;
+  double kept = ((th.keep & 1U) ? th.w1 : 0.) + ((th.keep & 2U) ? th.w2 : 0.);
+  ledger_add(&s->prof, cfg, OR_EMC_MASS_CORRECTION, mass_correction(11, ELECTRON_MASS_GEV, kept));
+  thinning_ledger(s, th.status, 2, k, (e1 + e2) * w,
+                  ((th.keep & 1U) ? e1 * th.w1 : 0.) + ((th.keep & 2U) ? e2 * th.w2 : 0.));
+}
+
+/* ========================== lepton step ========================== */
+
+typedef struct {
+  inter_t in;
+  ps_t start, end;
+  int limit;
+  double continuous_deposit;
+  double cut_deposit;
+  int observed_before_cut;
+} ltrans_t;
+
+/* prepare_lepton_continuous_step + transport_lepton (LeptonContinuousStep.inl:43-120,
+ * LeptonTransportStep.inl:27-513) in a field-free environment without Moliere
+ * deflection. */
+static void transport_lepton(const ctx_t* x, const inter_t* in, ltrans_t* out, fb_t* fb) {
+  const c8mi_environment* env = x->env;
+  const cont_t* c = &x->cont;
+  ps_t start = in->particle;
+  memset(out, 0, sizeof *out);
+  if (!is_ee(start.pid)) {
+    set_fallback(fb, in, FR_UNSUPPORTED_PARTICLE);
+    return;
+  }
+  if (in->status != IS_SELECTED && in->status != IS_DISTANCE_SAMPLED &&
+      in->status != IS_NO_DISCRETE && in->status != IS_PARTICLE_CUT) {
+    set_fallback(fb, in, FR_INVALID_FINAL_STATE);
+    return;
+  }
+  if ((in->status == IS_SELECTED || in->status == IS_DISTANCE_SAMPLED) &&
+      (!finite(in->interaction_grammage) || in->interaction_grammage < 0.)) {
+    set_fallback(fb, in, FR_INVALID_FINAL_STATE);
+    return;
+  }
+  /* prepare_lepton_continuous_step */
+  tq_t mass = cq_transport_mass(c, start.pid);
+  tq_t minimum = cq_minimum_energy(c, start.pid);
+  if (mass.status != TS_SUCCESS) {
+    set_fallback(fb, in, fallback_reason(mass.status));
+    return;
+  }
+  if (minimum.status != TS_SUCCESS) {
+    set_fallback(fb, in, fallback_reason(minimum.status));
+    return;
+  }
+  double e0 = start.energy_gev * 1000.;
+  if (!(e0 > mass.value)) {
+    set_fallback(fb, in, FR_INVALID_FINAL_STATE);
+    return;
+  }
+  tq_t tcut = cq_transport_cut(c, start.pid);
+  if (tcut.status != TS_SUCCESS || !finite(tcut.value) || !(tcut.value > 0.)) {
+    set_fallback(fb, in, FR_INVALID_TABLE_QUERY);
+    return;
+  }
+  int particle_cut = in->status == IS_PARTICLE_CUT || e0 - mass.value < tcut.value;
+  double max_grammage = 0.;
+  if (!particle_cut) {
+    tq_t r0 = cq_range(c, start.pid, e0);
+    if (r0.status != TS_SUCCESS) {
+      set_fallback(fb, in, fallback_reason(r0.status));
+      return;
+    }
+    double rl = (1. - MAX_CONTINUOUS_RELATIVE_LOSS) * e0;
+    double target = rl > minimum.value ? rl : minimum.value;
+    tq_t r1 = cq_range(c, start.pid, target);
+    if (r1.status != TS_SUCCESS) {
+      set_fallback(fb, in, fallback_reason(r1.status));
+      return;
+    }
+    max_grammage = r0.value - r1.value;
+    double scale = r0.value > 1. ? r0.value : 1.;
+    if (max_grammage < 0. && max_grammage > -64. * 2.22044604925031308085e-16 * scale)
+      max_grammage = 0.;
+    if (!finite(max_grammage) || max_grammage < 0.) {
+      set_fallback(fb, in, FR_INVALID_TABLE_QUERY);
+      return;
+    }
+  }
+  tq_t native_mass = cq_mass(c, start.pid);
+  if (particle_cut) {
+    layer_t layer = query_environment(env, start.position_m, start.direction, start.environment_node);
+    if (layer.status != ES_SUCCESS) {
+      set_fallback(fb, in, FR_UNSUPPORTED_GEOMETRY);
+      return;
+    }
+    start.environment_node = layer.node;
+    out->in = *in;
+    out->in.vertex_reached = 0;
+    out->in.particle_mass_gev = native_mass.value / 1000.;
+    out->start = start;
+    out->end = start;
+    out->end.step_id++;
+    out->limit = LL_CUT;
+    out->cut_deposit = (e0 - mass.value) / 1000.;
+    return;
+  }
+  layer_t layer = query_environment(env, start.position_m, start.direction, start.environment_node);
+  if (layer.status != ES_SUCCESS) {
+    set_fallback(fb, in, FR_UNSUPPORTED_GEOMETRY);
+    return;
+  }
+  start.environment_node = layer.node;
+  const c8mi_env_model* model = get_model(env, layer.node);
+  if (model->magnetic_field_t[0] != 0. || model->magnetic_field_t[1] != 0. ||
+      model->magnetic_field_t[2] != 0.) {
+    set_fallback(fb, in, FR_MAGNETIC_TRANSPORT);
+    return;
+  }
+  double m_gev = mass.value / 1000.;
+  /* maximum_uniform_magnetic_step: zero field gives Linear and an infinite limit
+   * (UniformMagneticField.inl:101-113); its particle check */
+  if (!finite(start.energy_gev) || !(start.energy_gev > m_gev) || !normalized(start.direction)) {
+    set_fallback(fb, in, FR_MAGNETIC_TRANSPORT);
+    return;
+  }
+  double p_gev = sqrt((start.energy_gev - m_gev) * (start.energy_gev + m_gev));
+  double speed = C_LIGHT * p_gev / start.energy_gev;
+  double momentum_si = p_gev * ELEMENTARY_CHARGE / MAGNETIC_RIGIDITY;
+  double vel[3] = {speed * start.direction[0], speed * start.direction[1], speed * start.direction[2]};
+  double param;
+  int32_t next_node;
+  next_boundary(env, start.position_m, vel, start.environment_node, INFINITY / speed, &param, &next_node);
+  double geometry = param * speed;
+  int magnetic_wins = next_node == start.environment_node;
+  int obs_wins = 0;
+  if (env->observation_enabled) {
+    double o = lepton_observation_distance(env, &start, speed, momentum_si);
+    if (o < geometry) {
+      geometry = o;
+      obs_wins = 1;
+    }
+    if (obs_wins) magnetic_wins = 0;
+  }
+  gq_t gg = competition_grammage(env, layer.node, start.position_m, start.direction, geometry);
+  if (gg.status != ES_SUCCESS) {
+    set_fallback(fb, in, FR_GRAMMAGE);
+    return;
+  }
+  int has_interaction = in->status == IS_SELECTED || in->status == IS_DISTANCE_SAMPLED;
+  double ig = has_interaction ? in->interaction_grammage : HUGE_VAL;
+  int continuous_wins = max_grammage < gg.value && max_grammage < ig;
+  int interaction_wins = !continuous_wins && has_interaction && ig <= gg.value;
+  out->in = *in;
+  out->in.vertex_reached = 0;
+  out->in.particle_mass_gev = native_mass.value / 1000.;
+  out->start = start;
+  double traversed = gg.value;
+  double distance = geometry;
+  if (continuous_wins) traversed = max_grammage;
+  else if (interaction_wins) traversed = ig;
+  if (continuous_wins || interaction_wins) {
+    gq_t d = environment_arclength(env, layer.node, start.position_m, start.direction, traversed);
+    if (d.status != ES_SUCCESS || d.value > geometry * (1. + 1.e-12)) {
+      set_fallback(fb, in, FR_INVERSE_GRAMMAGE);
+      return;
+    }
+    distance = d.value;
+  }
+  int observation_reached = obs_wins && !continuous_wins && !interaction_wins && !magnetic_wins;
+  /* advance_uniform_magnetic_field with zero field (UniformMagneticField.inl:146-195) */
+  out->end = start;
+  if (distance != 0.) {
+    double disp[3], kicked[3];
+    for (int a = 0; a < 3; ++a) {
+      kicked[a] = start.direction[a] + 0.;
+      disp[a] = distance * (start.direction[a] + 0.5 * 0.);
+      out->end.position_m[a] = start.position_m[a] + disp[a];
+    }
+    double kn = sqrt(dot3(kicked, kicked));
+    double chord = sqrt(dot3(disp, disp));
+    double beta = p_gev / start.energy_gev;
+    if (!finite(kn) || !(kn > 0.) || !finite(chord) || !finite(beta) || !(beta > 0.)) {
+      set_fallback(fb, in, FR_MAGNETIC_TRANSPORT);
+      return;
+    }
+    for (int a = 0; a < 3; ++a) out->end.direction[a] = kicked[a] / kn;
+    out->end.time_s = start.time_s + distance / (beta * C_LIGHT);
+  }
+  /* chord grammage (LeptonTransportStep.inl:301-331) */
+  double chord_dir[3], chord2 = 0.;
+  for (int a = 0; a < 3; ++a) {
+    double dd = out->end.position_m[a] - start.position_m[a];
+    chord_dir[a] = dd;
+    chord2 += dd * dd;
+  }
+  double chord = sqrt(chord2);
+  if (chord > 0.) {
+    for (int a = 0; a < 3; ++a) chord_dir[a] /= chord;
+  } else {
+    for (int a = 0; a < 3; ++a) chord_dir[a] = start.direction[a];
+  }
+  gq_t cg = environment_grammage(env, layer.node, start.position_m, chord_dir, chord);
+  if (cg.status != ES_SUCCESS) {
+    set_fallback(fb, in, FR_GRAMMAGE);
+    return;
+  }
+  traversed = cg.value;
+  /* evaluate_lepton_continuous_loss (LeptonContinuousStep.inl:122-159) */
+  tq_t fe = cq_energy_after_loss(c, start.pid, e0, traversed);
+  double final_mev;
+  int cut_reached = 0;
+  if (fe.status == TS_SUCCESS) {
+    final_mev = fe.value;
+  } else if (fe.status == TS_TRANSPORT_CUT) {
+    final_mev = minimum.value;
+    cut_reached = 1;
+  } else {
+    set_fallback(fb, in, fallback_reason(fe.status));
+    return;
+  }
+  double loss = e0 - final_mev;
+  if (continuous_wins && !(loss > 0.)) {
+    set_fallback(fb, in, FR_INVALID_TABLE_QUERY);
+    return;
+  }
+  out->end.energy_gev = final_mev / 1000.;
+  out->continuous_deposit = loss / 1000.;
+  if (out->end.time_s > PARTICLE_CUT_TIME_S) {
+    out->end.step_id++;
+    out->limit = LL_CUT;
+    out->cut_deposit = (final_mev - mass.value) / 1000.;
+    out->observed_before_cut = observation_reached;
+    return;
+  }
+  int reaches_cut = cut_reached || final_mev - mass.value < tcut.value;
+  if (reaches_cut) {
+    out->end.step_id++;
+    out->limit = LL_CUT;
+    if (!observation_reached) {
+      layer_t el = query_environment(env, out->end.position_m, out->end.direction, start.environment_node);
+      if (el.status != ES_SUCCESS) {
+        set_fallback(fb, in, FR_UNSUPPORTED_GEOMETRY);
+        return;
+      }
+    }
+    out->cut_deposit = (final_mev - mass.value) / 1000.;
+    out->observed_before_cut = observation_reached;
+    return;
+  }
+  if (interaction_wins) {
+    layer_t v = query_environment(env, out->end.position_m, out->end.direction, start.environment_node);
+    if (v.status != ES_SUCCESS || !finite(v.density) || !(v.density > 0.)) {
+      set_fallback(fb, in, FR_VERTEX_LOOKUP);
+      return;
+    }
+    out->limit = LL_INTERACTION;
+    out->in.particle = out->end;
+    out->in.status = IS_RESELECT;
+    out->in.vertex_reached = 1;
+    out->in.process_id = 0;
+    out->in.component_hash = 0;
+    out->in.energy_fraction = 0.;
+    out->in.loss_quantile = 0.;
+    out->in.mass_density = v.density;
+    return;
+  }
+  out->end.step_id++;
+  if (continuous_wins) {
+    layer_t el = query_environment(env, out->end.position_m, out->end.direction, start.environment_node);
+    if (el.status != ES_SUCCESS) {
+      set_fallback(fb, in, FR_UNSUPPORTED_GEOMETRY);
+      return;
+    }
+    out->limit = LL_CONTINUOUS;
+    return;
+  }
+  if (magnetic_wins) {
+    layer_t el = query_environment(env, out->end.position_m, out->end.direction, start.environment_node);
+    if (el.status != ES_SUCCESS) {
+      set_fallback(fb, in, FR_MAGNETIC_BOUNDARY);
+      return;
+    }
+    out->limit = LL_MAGNETIC;
+    return;
+  }
+  if (observation_reached) {
+    out->limit = LL_OBSERVATION;
+    return;
+  }
+  out->end.environment_node = next_node;
+  const c8mi_env_model* nm = get_model(env, next_node);
+  if (!nm) {
+    out->limit = LL_ESCAPED;
+    return;
+  }
+  layer_t next = query_environment(env, out->end.position_m, out->end.direction, next_node);
+  if (next.status != ES_SUCCESS || next_node == start.environment_node) {
+    set_fallback(fb, in, FR_UNSUPPORTED_GEOMETRY);
+    return;
+  }
+  out->limit = LL_LAYER;
+  out->end.medium_id = nm->medium_id;
+}
+
+/* sample_annihilation_rho (LeptonFinalStateStep.inl:150-192) */
+static double ann_term(double a1, double a2, double rho) { return a1 * log(rho) + a2 / rho - rho; }
+static double ann_eq(double v, double a1, double a2, double lt, double ut, double u) {
+  return ann_term(a1, a2, v) - lt - u * (ut - lt);
+}
+static int annihilation_rho(double e, double m, double u, double* rho) {
+  double gamma = e / m;
+  if (!finite(gamma) || !(gamma > 1.) || !finite(u) || !(u > 0.) || !(u < 1.)) return 0;
+  double aux = sqrt((gamma - 1.) / (gamma + 1.));
+  double lower = 0.5 * (1. - aux), upper = 0.5 * (1. + aux);
+  double a2 = 1. / ((gamma + 1.) * (gamma + 1.));
+  double a1 = 1. + 2. * gamma * a2;
+  double lt = ann_term(a1, a2, lower), ut = ann_term(a1, a2, upper);
+  double lv = ann_eq(lower, a1, a2, lt, ut, u), uv = ann_eq(upper, a1, a2, lt, ut, u);
+  if (!finite(lv) || !finite(uv) || lv * uv > 0.) return 0;
+  double precision = lower * (e * 1000.) * PROPOSAL_HALF_PRECISION;
+  if (!finite(precision) || !(precision > 0.)) return 0;
+  for (int it = 0; it <= 100; ++it) {
+    double center = (lower + upper) / 2.;
+    double cv = ann_eq(center, a1, a2, lt, ut, u);
+    if (sgn(cv) == sgn(lv)) {
+      lower = center;
+      lv = cv;
+    } else {
+      upper = center;
+    }
+    if (fabs(upper - lower) < precision) {
+      *rho = lower;
+      return finite(*rho) && *rho > 0. && *rho < 1.;
+    }
+  }
+  *rho = lower;
+  return finite(*rho) && *rho > 0. && *rho < 1.;
+}
+
+typedef struct {
+  ps_t next[2];
+  uint32_t next_count;
+  ps_t photons[2];
+  uint32_t photon_count;
+  uint32_t child_count;
+} lout_t;
+
+static void lepton_source(shower_t* s, const ps_t* p, lout_t* o, uint64_t child_offset,
+                          uint64_t first_history) {
+  const ctx_t* x = &s->x;
+  const c8mi_em_config* cfg = x->cfg;
+  memset(o, 0, sizeof *o);
+  inter_t in;
+  fb_t fb = {0, 0, {0}, 0};
+  select_interaction(x, p, &in, &fb);
+  if (fb.flag) {
+    record_fallback(s, &fb, STAGE_SELECTION);
+    return;
+  }
+  ltrans_t tr;
+  transport_lepton(x, &in, &tr, &fb);
+  if (fb.flag) {
+    record_fallback(s, &fb, STAGE_TRANSPORT);
+    return;
+  }
+  s->stats.lepton_steps++;
+  /* accumulate_lepton_profile_step (ProfileAccumulation.inl:300-375) */
+  double w = tr.start.weight;
+  double gs = project_grammage(cfg, tr.start.position_m);
+  double ge = project_grammage(cfg, tr.end.position_m);
+  particle_profile(&s->prof, cfg, tr.start.pid, gs, ge, w);
+  energy_profile(&s->prof, cfg, gs, ge, tr.continuous_deposit * w);
+  point_energy_profile(&s->prof, cfg, ge, tr.cut_deposit * w);
+  ledger_add(&s->prof, cfg, OR_EMC_CONTINUOUS, tr.continuous_deposit * w);
+  s->prof.counters[OR_EMC_STEPS]++;
+  s->prof.counters[OR_EMC_LEPTON_LIMIT0 + tr.limit]++;
+  double tm = transport_mass_gev(tr.start.pid);
+  if (tr.limit == LL_CUT) {
+    ledger_add(&s->prof, cfg, OR_EMC_CUT_DEPOSIT, tr.cut_deposit * w);
+    ledger_add(&s->prof, cfg, OR_EMC_CUT_REST_MASS, tm * w);
+    if (tr.observed_before_cut) {
+      ledger_add(&s->prof, cfg, OR_EMC_OVERLAP, (tr.cut_deposit + tm) * w);
+      terminal_energy(s, tr.end.energy_gev * w, 1);
+      record_observation(s, &tr.end);
+    }
+    return;
+  }
+  if (tr.limit == LL_OBSERVATION || tr.limit == LL_ESCAPED) {
+    terminal_energy(s, tr.end.energy_gev * w, tr.limit == LL_OBSERVATION);
+    record_observation(s, &tr.end);
+    return;
+  }
+  if (tr.limit != LL_INTERACTION) {
+    o->next[o->next_count++] = tr.end;
+    return;
+  }
+  /* select_lepton_vertex (LeptonVertexSelection.inl:41-197) */
+  s->stats.lepton_interactions++;
+  inter_t r = tr.in;
+  const ps_t* vp = &r.particle;
+  fb_t f;
+  if (r.status != IS_RESELECT || !finite(r.total_rate) || !(r.total_rate > 0.) ||
+      !finite(r.process_uniform) || !(r.process_uniform > 0.) || !(r.process_uniform < 1.)) {
+    set_fallback(&f, &r, FR_INVALID_FINAL_STATE);
+    record_fallback(s, &f, STAGE_VERTEX);
+    return;
+  }
+  double e_mev = vp->energy_gev * 1000.;
+  double outer = r.process_uniform * r.total_rate;
+  tq_t vt = dq_total_rate(&x->cont, vp->pid, e_mev);
+  if (vt.status != TS_SUCCESS) {
+    set_fallback(&f, &r, fallback_reason(vt.status));
+    record_fallback(s, &f, STAGE_VERTEX);
+    return;
+  }
+  if (!(vt.value > 0.) || outer >= vt.value) {
+    if (vp->step_id == 0xffffffffffffffffULL) {
+      set_fallback(&f, &r, FR_INVALID_FINAL_STATE);
+      record_fallback(s, &f, STAGE_VERTEX);
+      return;
+    }
+    ps_t cont = *vp;
+    cont.step_id++;
+    o->next[o->next_count++] = cont;
+    return;
+  }
+  double us = uniform(x->seed, x->shower, vp, RP_SELECTION, 0);
+  sel_t sel = select_column(&x->cont, vp->pid, e_mev, us);
+  if (sel.status != TS_SUCCESS) {
+    r.process_id = sel.process;
+    set_fallback(&f, &r, fallback_reason(sel.status));
+    record_fallback(s, &f, STAGE_VERTEX);
+    return;
+  }
+  if (!sel.selected) {
+    set_fallback(&f, &r, FR_ZERO_TOTAL_RATE);
+    record_fallback(s, &f, STAGE_VERTEX);
+    return;
+  }
+  r.process_id = sel.process;
+  r.component_hash = sel.component;
+  r.loss_quantile = sel.quantile;
+  if (requires_replay(vp->pid, sel.process, sel.quantile)) {
+    set_fallback(&f, &r, FR_SELECTION_REPLAY);
+    record_fallback(s, &f, STAGE_VERTEX);
+    return;
+  }
+  tq_t loss = dq_loss_fraction(&x->cont, vp->pid, sel.process, sel.component, e_mev, sel.quantile);
+  if (loss.status != TS_SUCCESS) {
+    set_fallback(&f, &r, fallback_reason(loss.status));
+    record_fallback(s, &f, STAGE_VERTEX);
+    return;
+  }
+  r.status = IS_SELECTED;
+  r.energy_fraction = loss.value;
+
+  /* classify_lepton_final_state (LeptonFinalStateStep.inl:322-498), e+- only */
+  const ps_t* parent = &r.particle;
+  int cap = capability(parent->pid, r.process_id);
+  if (cap != CAP_BREMS && cap != CAP_ANNIHILATION && cap != CAP_IONIZATION) {
+    set_fallback(&f, &r, cap == CAP_CPU_ONLY ? FR_CPU_ONLY : FR_NOT_IMPLEMENTED);
+    record_fallback(s, &f, STAGE_FINAL);
+    return;
+  }
+  const c8mi_brems_lpm* lpm = &cfg->brems_lpm;
+  double v = r.energy_fraction;
+  double me = lpm->electron_mass_mev / 1000.;
+  double lm = r.particle_mass_gev > 0. ? r.particle_mass_gev : lpm->lepton_mass_mev / 1000.;
+  double fs_me = lpm->lepton_mass_mev / 1000.; /* materialisation electron mass */
+  if (!finite(parent->energy_gev) || !(parent->energy_gev > lm) || parent->generation == 0xffffffffU ||
+      parent->step_id == 0xffffffffffffffffULL) {
+    set_fallback(&f, &r, FR_INVALID_FINAL_STATE);
+    record_fallback(s, &f, STAGE_FINAL);
+    return;
+  }
+  if (cap == CAP_ANNIHILATION) {
+    if (!finite(v) || fabs(v - 1.) > 1.e-12) {
+      set_fallback(&f, &r, FR_INVALID_FINAL_STATE);
+      record_fallback(s, &f, STAGE_FINAL);
+      return;
+    }
+    double ru = uniform(x->seed, x->shower, parent, P_ANNIHILATION, D_ANNIHILATION_RHO);
+    double au = uniform(x->seed, x->shower, parent, P_ANNIHILATION, D_ANNIHILATION_AZIMUTH);
+    double rho;
+    if (!annihilation_rho(parent->energy_gev, lm, ru, &rho)) {
+      set_fallback(&f, &r, FR_INVALID_FINAL_STATE);
+      record_fallback(s, &f, STAGE_FINAL);
+      return;
+    }
+    double total = parent->energy_gev + lm;
+    thin_t th;
+    if (!two_child_thinning(x, parent, P_ANNIHILATION, total * (1. - rho), total * rho, 1, &th)) {
+      set_fallback(&f, &r, FR_INVALID_FINAL_STATE);
+      record_fallback(s, &f, STAGE_FINAL);
+      return;
+    }
+    /* materialize (LeptonFinalStateStep.inl:700-790), in flight */
+    double e1 = total * (1. - rho), e2 = total * rho;
+    double mom = sqrt((parent->energy_gev + lm) * (parent->energy_gev - lm));
+    double c1 = (total * (1. - rho) - lm) / ((1. - rho) * mom);
+    double c2 = (total * rho - lm) / (rho * mom);
+    if (!finite(e1) || !finite(e2) || !(e1 > 0.) || !(e2 > 0.) || !finite(c1) || !finite(c2)) {
+      s->prof.counters[OR_EMC_INVALID]++;
+      return;
+    }
+    double az = au * LEPTON_TWO_PI;
+    ps_t g1 = *parent;
+    g1.pid = 22;
+    g1.energy_gev = e1;
+    g1.parent_history_id = parent->history_id;
+    g1.generation = parent->generation + 1;
+    g1.step_id = 0;
+    g1.reserved = 0;
+    g1.weight = th.w1;
+    ps_t g2 = g1;
+    g2.energy_gev = e2;
+    g2.weight = th.w2;
+    deflect(parent->direction, c1, az, g1.direction, 1);
+    deflect(parent->direction, c2, fmod(az + LEPTON_PI, LEPTON_TWO_PI), g2.direction, 1);
+    uint32_t k = 0;
+    if (th.keep & 1U) {
+      g1.history_id = first_history + child_offset + k++;
+      o->photons[o->photon_count++] = g1;
+    }
+    if (th.keep & 2U) {
+      g2.history_id = first_history + child_offset + k++;
+      o->photons[o->photon_count++] = g2;
+    }
+    o->child_count = k;
+    s->stats.secondaries += k;
+    ledger_add(&s->prof, cfg, OR_EMC_MEDIUM_REST_MASS, ELECTRON_MASS_GEV * w);
+    thinning_ledger(s, th.status, 2, k, (e1 + e2) * w,
+                    ((th.keep & 1U) ? e1 * th.w1 : 0.) + ((th.keep & 2U) ? e2 * th.w2 : 0.));
+    return;
+  }
+  if (cap == CAP_IONIZATION) {
+    double ev = (v * (parent->energy_gev * 1000.)) / (parent->energy_gev * 1000.);
+    double outgoing = parent->energy_gev * (1. - ev);
+    double delta = parent->energy_gev * ev + me;
+    if (!finite(ev) || !(ev > 0.) || !(ev < 1.) || !finite(outgoing) || outgoing < lm ||
+        !finite(delta) || !(delta > me)) {
+      set_fallback(&f, &r, FR_INVALID_FINAL_STATE);
+      record_fallback(s, &f, STAGE_FINAL);
+      return;
+    }
+    double au = uniform(x->seed, x->shower, parent, P_IONIZATION, D_IONIZATION_AZIMUTH);
+    thin_t th;
+    double e1 = secondary_transport_energy(parent->pid, outgoing, lm);
+    double e2 = secondary_transport_energy(11, delta, me);
+    if (!two_child_thinning(x, parent, P_IONIZATION, e1, e2, 1, &th)) {
+      set_fallback(&f, &r, FR_INVALID_FINAL_STATE);
+      record_fallback(s, &f, STAGE_FINAL);
+      return;
+    }
+    /* materialize (LeptonFinalStateStep.inl:796-876) */
+    double mv = (r.energy_fraction * (parent->energy_gev * 1000.)) / (parent->energy_gev * 1000.);
+    double oe = parent->energy_gev * (1. - mv);
+    double de = parent->energy_gev * mv + fs_me;
+    double im = sqrt((parent->energy_gev + lm) * (parent->energy_gev - lm));
+    double om = sqrt((oe + lm) * (oe - lm));
+    double dm = sqrt((de + fs_me) * (de - fs_me));
+    double oc = ((parent->energy_gev + fs_me) * oe - parent->energy_gev * fs_me - lm * lm) / (im * om);
+    double dc = ((parent->energy_gev + fs_me) * de - parent->energy_gev * fs_me - fs_me * fs_me) / (im * dm);
+    if (!finite(oc) || !finite(dc)) {
+      s->prof.counters[OR_EMC_INVALID]++;
+      return;
+    }
+    double az = au * LEPTON_TWO_PI;
+    ps_t out = *parent;
+    out.energy_gev = secondary_transport_energy(out.pid, oe, lm);
+    out.parent_history_id = parent->history_id;
+    out.generation = parent->generation + 1;
+    out.step_id = 0;
+    out.reserved = 0;
+    out.weight = th.w1;
+    ps_t dl = out;
+    dl.pid = 11;
+    dl.energy_gev = secondary_transport_energy(11, de, fs_me);
+    dl.weight = th.w2;
+    deflect(parent->direction, oc, az, out.direction, 1);
+    deflect(parent->direction, dc, fmod(az + LEPTON_PI, LEPTON_TWO_PI), dl.direction, 1);
+    uint32_t k = 0;
+    if (th.keep & 1U) {
+      out.history_id = first_history + child_offset + k++;
+      o->next[o->next_count++] = out;
+    }
+    if (th.keep & 2U) {
+      dl.history_id = first_history + child_offset + k++;
+      o->next[o->next_count++] = dl;
+    }
+    o->child_count = k;
+    s->stats.secondaries += k;
+    ledger_add(&s->prof, cfg, OR_EMC_MASS_CORRECTION,
+               mass_correction(parent->pid, lm, (th.keep & 1U) ? th.w1 : 0.) +
+                   mass_correction(11, fs_me, (th.keep & 2U) ? th.w2 : 0.));
+    ledger_add(&s->prof, cfg, OR_EMC_MEDIUM_REST_MASS, ELECTRON_MASS_GEV * w);
+    thinning_ledger(s, th.status, 2, k, (out.energy_gev + dl.energy_gev) * w,
+                    ((th.keep & 1U) ? out.energy_gev * th.w1 : 0.) +
+                        ((th.keep & 2U) ? dl.energy_gev * th.w2 : 0.));
+    return;
+  }
+  /* bremsstrahlung */
+  double pe = parent->energy_gev * v;
+  double le = parent->energy_gev - pe;
+  if (!finite(v) || !(v > 0.) || !(v < 1.) || !finite(pe) || !(pe > 0.) || !finite(le) || le < lm) {
+    set_fallback(&f, &r, FR_INVALID_FINAL_STATE);
+    record_fallback(s, &f, STAGE_FINAL);
+    return;
+  }
+  double au = uniform(x->seed, x->shower, parent, P_BREMS, D_BREMS_AZIMUTH);
+  double survival = 0.;
+  int ls = brems_lpm(lpm, r.component_hash, parent->energy_gev * 1000., v, r.mass_density, &survival);
+  if (ls != 0) {
+    set_fallback(&f, &r, ls == 3 ? FR_INVALID_DENSITY : FR_LPM_UNAVAILABLE);
+    record_fallback(s, &f, STAGE_FINAL);
+    return;
+  }
+  double lu = uniform(x->seed, x->shower, parent, P_BREMS, D_BREMS_LPM);
+  if (lu > survival) {
+    ps_t cont = *parent;
+    cont.step_id++;
+    o->next[o->next_count++] = cont;
+    return;
+  }
+  thin_t th;
+  double e1 = secondary_transport_energy(parent->pid, le, lm);
+  if (!two_child_thinning(x, parent, P_BREMS, e1, pe, 1, &th)) {
+    set_fallback(&f, &r, FR_INVALID_FINAL_STATE);
+    record_fallback(s, &f, STAGE_FINAL);
+    return;
+  }
+  /* materialize (LeptonFinalStateStep.inl:957-1025) */
+  double az = au * LEPTON_TWO_PI;
+  double pcos = cos(lm / parent->energy_gev);
+  double pdir[3];
+  deflect(parent->direction, pcos, az, pdir, 1);
+  ps_t lep = *parent;
+  lep.energy_gev = e1;
+  lep.parent_history_id = parent->history_id;
+  lep.generation = parent->generation + 1;
+  lep.step_id = 0;
+  lep.reserved = 0;
+  lep.weight = th.w1;
+  double pm2 = (parent->energy_gev + lm) * (parent->energy_gev - lm);
+  double pmom = sqrt(0. > pm2 ? 0. : pm2);
+  for (int a = 0; a < 3; ++a) lep.direction[a] = parent->direction[a] * pmom - pdir[a] * pe;
+  double n2 = dot3(lep.direction, lep.direction);
+  if (!finite(n2) || !(n2 > 0.)) {
+    s->prof.counters[OR_EMC_INVALID]++;
+    return;
+  }
+  double inv = 1. / sqrt(n2);
+  for (int a = 0; a < 3; ++a) lep.direction[a] *= inv;
+  if (!finite(lep.direction[0]) || !finite(lep.direction[1]) || !finite(lep.direction[2])) {
+    s->prof.counters[OR_EMC_INVALID]++;
+    return;
+  }
+  ps_t ph = *parent;
+  ph.pid = 22;
+  ph.energy_gev = pe;
+  ph.parent_history_id = parent->history_id;
+  ph.generation = parent->generation + 1;
+  ph.step_id = 0;
+  ph.reserved = 0;
+  ph.weight = th.w2;
+  for (int a = 0; a < 3; ++a) ph.direction[a] = pdir[a];
+  uint32_t k = 0;
+  if (th.keep & 1U) {
+    lep.history_id = first_history + child_offset + k++;
+    o->next[o->next_count++] = lep;
+  }
+  if (th.keep & 2U) {
+    ph.history_id = first_history + child_offset + k++;
+    o->photons[o->photon_count++] = ph;
+  }
+  o->child_count = k;
+  s->stats.secondaries += k;
+  ledger_add(&s->prof, cfg, OR_EMC_MASS_CORRECTION,
+             mass_correction(parent->pid, lm, (th.keep & 1U) ? th.w1 : 0.));
+  thinning_ledger(s, th.status, 2, k, (e1 + pe) * w,
+                  ((th.keep & 1U) ? e1 * th.w1 : 0.) + ((th.keep & 2U) ? pe * th.w2 : 0.));
+}
+
+/* ========================== driver ========================== */
+
+/* One front of one species: every source in order; continuing particles of the
+ * species go to `next`, the other species is appended to its queue. History IDs
+ * come from the running child count (the stable prefix sum of the device). */
+static void photon_front(shower_t* s) {
+  queue_t in = s->photons;
+  queue_t next = {NULL, 0, 0};
+  uint64_t child = 0;
+  uint64_t first = s->next_history;
+  for (uint64_t i = 0; i < in.count; ++i) {
+    pout_t o;
+    photon_source(s, &in.items[i], &o, child, first);
+    for (uint32_t k = 0; k < o.next_count; ++k) push(&next, &o.next[k]);
+    for (uint32_t k = 0; k < o.charged_count; ++k) push(&s->leptons, &o.charged[k]);
+    child += o.child_count;
+  }
+  s->next_history = first + child;
+  free(in.items);
+  s->photons = next;
+  s->stats.photon_fronts++;
+  if (in.count > s->stats.max_front) s->stats.max_front = in.count;
+}
+
+static void lepton_front(shower_t* s) {
+  queue_t in = s->leptons;
+  queue_t next = {NULL, 0, 0};
+  uint64_t child = 0;
+  uint64_t first = s->next_history;
+  for (uint64_t i = 0; i < in.count; ++i) {
+    lout_t o;
+    lepton_source(s, &in.items[i], &o, child, first);
+    for (uint32_t k = 0; k < o.next_count; ++k) push(&next, &o.next[k]);
+    for (uint32_t k = 0; k < o.photon_count; ++k) push(&s->photons, &o.photons[k]);
+    child += o.child_count;
+  }
+  s->next_history = first + child;
+  free(in.items);
+  s->leptons = next;
+  s->stats.lepton_fronts++;
+  if (in.count > s->stats.max_front) s->stats.max_front = in.count;
+}
+
+int oracle_em_run_shower(const c8mi_em_config* config, const c8mi_native_table_set* tables,
+                         const c8mi_environment* environment, uint64_t shower_id,
+                         uint64_t first_history_id, const c8mi_particle_state* primaries,
+                         uint64_t primary_count, uint64_t maximum_fronts, int64_t* photons,
+                         int64_t* electrons, int64_t* positrons, int64_t* energy_loss,
+                         int64_t* counters, c8mi_particle_state* observations,
+                         c8mi_em_fallback* fallbacks, or_em_stats* stats) {
+  shower_t s;
+  memset(&s, 0, sizeof s);
+  s.x.t = tables;
+  s.x.cont.t = tables;
+  s.x.cont.em_cut_mev = config->em_transport_cut_mev;
+  s.x.cont.muon_cut_mev = config->muon_transport_cut_mev;
+  s.x.env = environment;
+  s.x.cfg = config;
+  s.x.seed = config->random_seed;
+  s.x.shower = shower_id;
+  s.next_history = first_history_id;
+  s.prof.photons = photons;
+  s.prof.electrons = electrons;
+  s.prof.positrons = positrons;
+  s.prof.energy_loss = energy_loss;
+  s.observations = observations;
+  s.fallbacks = fallbacks;
+  for (uint64_t i = 0; i < primary_count; ++i) {
+    const ps_t* p = &primaries[i];
+    ledger_add(&s.prof, config, OR_EMC_INJECTED, p->energy_gev * p->weight);
+    if (p->pid == 22) push(&s.photons, p);
+    else if (is_ee(p->pid)) push(&s.leptons, p);
+    else {
+      fb_t f = {1, FR_UNSUPPORTED_PARTICLE, *p, 0};
+      record_fallback(&s, &f, STAGE_SELECTION);
+    }
+  }
+  uint64_t rounds = 0;
+  while ((s.photons.count || s.leptons.count) && rounds < maximum_fronts) {
+    if (s.photons.count) photon_front(&s);
+    if (s.leptons.count) lepton_front(&s);
+    ++rounds;
+  }
+  double pending = 0.;
+  for (uint64_t i = 0; i < s.photons.count; ++i)
+    pending += s.photons.items[i].energy_gev * s.photons.items[i].weight;
+  for (uint64_t i = 0; i < s.leptons.count; ++i)
+    pending += s.leptons.items[i].energy_gev * s.leptons.items[i].weight;
+  ledger_add(&s.prof, config, OR_EMC_PENDING, pending);
+  s.stats.pending_photons = s.photons.count;
+  s.stats.pending_leptons = s.leptons.count;
+  s.stats.next_history_id = s.next_history;
+  s.stats.observations = s.observation_count;
+  memcpy(counters, s.prof.counters, sizeof s.prof.counters);
+  *stats = s.stats;
+  free(s.photons.items);
+  free(s.leptons.items);
+  return 0;
+}
+
+/* ---- single-function probes for the tests ---- */
+
+int oracle_em_environment_query(const c8mi_environment* e, const double position[3],
+                                const double direction[3], int32_t node, int32_t* out_node,
+                                double* out_density) {
+  layer_t l = query_environment(e, position, direction, node);
+  *out_node = l.node;
+  *out_density = l.density;
+  return l.status;
+}
+
+int oracle_em_boundary(const c8mi_environment* e, const double position[3],
+                       const double direction[3], int32_t node, double* distance,
+                       int32_t* next_node) {
+  boundary_t b = distance_to_boundary(e, position, direction, node);
+  *distance = b.distance;
+  *next_node = b.next_node;
+  return b.status;
+}
+
+int oracle_em_grammage(const c8mi_environment* e, int32_t node, const double position[3],
+                       const double direction[3], double distance, double* out) {
+  gq_t g = environment_grammage(e, node, position, direction, distance);
+  *out = g.value;
+  return g.status;
+}
+
+int oracle_em_arclength(const c8mi_environment* e, int32_t node, const double position[3],
+                        const double direction[3], double grammage, double* out) {
+  gq_t g = environment_arclength(e, node, position, direction, grammage);
+  *out = g.value;
+  return g.status;
+}
+
+int oracle_em_pair_lpm(const c8mi_photon_pair_lpm* s, uint64_t comp, double e_mev, double x,
+                       double rho, double* out) {
+  *out = 0.;
+  return pair_lpm(s, comp, e_mev, x, rho, out);
+}
+
+int oracle_em_brems_lpm(const c8mi_brems_lpm* s, uint64_t comp, double e_mev, double v,
+                        double rho, double* out) {
+  *out = 0.;
+  return brems_lpm(s, comp, e_mev, v, rho, out);
+}
+
+int oracle_em_thinning(const c8mi_em_thinning* c, double pe, double pw, double e1, double e2,
+                       double u1, double u2, uint32_t* keep, double* w1, double* w2) {
+  thin_t t = em_thinning(c, pe, pw, e1, e2, u1, u2);
+  *keep = t.keep;
+  *w1 = t.w1;
+  *w2 = t.w2;
+  return t.status;
+}
+
+int oracle_em_pair_split(double e_gev, double x, double* normalized, double* decoded) {
+  if (!encode_pair_split(e_gev, x, normalized)) return 0;
+  return decode_pair_split(e_gev, *normalized, decoded) ? 1 : 0;
+}
+
+double oracle_em_sauter_cosine(double e, double u) { return sauter_cosine(e, u); }
+
+int oracle_em_compton_cosines(double e, double v, double* pc, double* ec) {
+  return compton_cosines(e, v, pc, ec);
+}
+
+int oracle_em_annihilation_rho(double e, double m, double u, double* rho) {
+  return annihilation_rho(e, m, u, rho);
+}

@@ -1031,3 +1031,36 @@ double oracle_native_transformed_loss(const c8mi_native_table_set* v, uint32_t c
                                       double x) {
   return transformed_loss(v, &v->dndx_columns[column], e, x);
 }
+
+/* ---------------- entry points for the EM step restatement (em_oracle.c) ---------------- */
+#include "native_internal.h"
+
+result_t oracle_nq_total_rate(const view_t* v, int32_t pid, double e) {
+  return total_rate(v, pid, e);
+}
+result_t oracle_nq_selection_total_rate(const view_t* v, int32_t pid, double e) {
+  return selection_total_rate(v, pid, e);
+}
+result_t oracle_nq_rate_for_column(const view_t* v, const c8mi_native_dndx_column* c, double e) {
+  return rate_for_column(v, c, e);
+}
+int oracle_nq_rate_below_threshold(const c8mi_native_dndx_column* c, double e) {
+  return rate_below_threshold(c, e);
+}
+result_t oracle_nq_loss_fraction(const view_t* v, int32_t pid, int32_t process, uint64_t component,
+                                 double e, double q, int photon_split) {
+  return loss_fraction(v, pid, process, component, e, q, photon_split);
+}
+result_t oracle_nq_range(const view_t* v, int32_t pid, double e) { return range(v, pid, e); }
+result_t oracle_nq_energy_after_loss(const view_t* v, int32_t pid, double e0, double grammage) {
+  return energy_after_loss(v, pid, e0, grammage);
+}
+const c8mi_native_utility_column* oracle_nq_find_utility(const view_t* v, int32_t pid) {
+  return find_utility(v, pid);
+}
+void oracle_nq_dndx_range(const view_t* v, int32_t pid, uint32_t* begin, uint32_t* end) {
+  dndx_range(v, pid, begin, end);
+}
+const c8mi_native_dndx_column* oracle_nq_selection_dndx(const view_t* v, uint32_t position) {
+  return selection_dndx(v, position);
+}

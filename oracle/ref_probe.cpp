@@ -4,12 +4,22 @@
 // restatement (and the HIP kernels) against the reference code itself on the
 // same inputs. Only headers that compile unmodified are included; nothing of
 // the reference is copied here.
+#include <corsika/kokkos/em/algorithms/EmThinning.hpp>
+#include <corsika/kokkos/em/models/BremsLpm.hpp>
+#include <corsika/kokkos/em/models/PhotonPairKinematics.hpp>
+#include <corsika/kokkos/em/models/PhotonPairLpm.hpp>
 #include <corsika/kokkos/em/random/Philox.hpp>
 #include <corsika/kokkos/em/tables/ProposalNativeQueries.hpp>
+#include <corsika/kokkos/environment/EnvironmentBorrowedView.hpp>
+#include <corsika/kokkos/environment/EnvironmentQuery.hpp>
+#include <corsika/kokkos/environment/ObservationPlane.hpp>
 
 #include <cstddef>
 #include <cstdint>
+#include <cstring>
+#include <vector>
 
+#include "c8mi/c8mi_em.h"
 #include "c8mi/c8mi_tables.h"
 
 namespace ck = corsika::kokkos;
@@ -36,6 +46,10 @@ static_assert(offsetof(c8mi_native_utility_column, reverse) ==
               offsetof(ck::NativeUtilityColumn, reverse_));
 static_assert(sizeof(c8mi_native_constants) == sizeof(ck::NativePhysicalConstants));
 static_assert(sizeof(c8mi_native_result) == sizeof(ck::NativeQueryResult));
+static_assert(sizeof(c8mi_photon_pair_lpm) == sizeof(ck::PhotonPairLpmSnapshot));
+static_assert(sizeof(c8mi_brems_lpm) == sizeof(ck::BremsLpmSnapshot));
+static_assert(offsetof(c8mi_brems_lpm, muon_e_lpm_mev) == offsetof(ck::BremsLpmSnapshot, muonELpmMeV_));
+static_assert(sizeof(c8mi_em_thinning) == sizeof(ck::EmThinningConfig));
 
 namespace {
 
@@ -172,6 +186,149 @@ void ref_native_queries(c8mi_native_table_set const* table, c8mi_native_query co
     }
     out[i] = convert(r);
   }
+}
+
+
+// ---- environment: the c8mi tree converted to the reference's data view ----
+
+namespace {
+
+struct RefEnvironment {
+  std::vector<ck::EnvironmentNode> nodes;
+  std::vector<ck::EnvironmentShape> shapes;
+  std::vector<std::int32_t> children;
+  std::vector<ck::EnvironmentModel> models;
+  ck::EnvironmentBorrowedView view;
+  ck::EnvironmentQueryConfig config;
+
+  explicit RefEnvironment(c8mi_environment const& e) {
+    for (std::uint32_t i = 0; i < e.node_count; ++i) {
+      auto const& n = e.nodes[i];
+      ck::EnvironmentNode rn{};
+      rn.parentIndex_ = n.parent;
+      rn.modelIndex_ = n.model;
+      rn.childOffset_ = n.child_begin;
+      rn.childCount_ = n.child_count;
+      nodes.push_back(rn);
+      ck::EnvironmentShape sh{};
+      sh.kind_ = n.shape_kind == C8MI_SHAPE_UNIVERSE ? ck::EnvironmentShape::Kind::Universe
+                                                     : ck::EnvironmentShape::Kind::Sphere;
+      for (int a = 0; a < 3; ++a) sh.centerM_[a] = n.center_m[a];
+      sh.radiusM_ = n.radius_m;
+      shapes.push_back(sh);
+    }
+    children.assign(e.children, e.children + e.child_index_count);
+    for (std::uint32_t i = 0; i < e.model_count; ++i) {
+      auto const& m = e.models[i];
+      ck::EnvironmentModel rm{};
+      rm.densityKind_ = m.density_kind == C8MI_DENSITY_HOMOGENEOUS ? ck::EnvironmentModel::Density::Homogeneous
+                        : m.density_kind == C8MI_DENSITY_SLIDING_PLANAR_EXPONENTIAL
+                            ? ck::EnvironmentModel::Density::SlidingPlanarExponential
+                            : ck::EnvironmentModel::Density::FlatExponential;
+      rm.referenceDensityGPerCm3_ = m.reference_density_g_per_cm3;
+      rm.scaleLengthM_ = m.scale_length_m;
+      rm.inverseScaleLengthPerM_ = m.inverse_scale_length_per_m;
+      rm.referenceHeightM_ = m.reference_height_m;
+      for (int a = 0; a < 3; ++a) {
+        rm.anchorM_[a] = m.anchor_m[a];
+        rm.axis_[a] = m.axis[a];
+        rm.magneticFieldT_[a] = m.magnetic_field_t[a];
+      }
+      rm.mediumId_ = m.medium_id;
+      models.push_back(rm);
+    }
+    ck::EnvironmentDataView d{};
+    d.nodes_ = nodes.data();
+    d.shapes_ = shapes.data();
+    d.children_ = children.data();
+    d.models_ = models.data();
+    d.nodeCount_ = static_cast<std::uint32_t>(nodes.size());
+    d.childCount_ = static_cast<std::uint32_t>(children.size());
+    d.modelCount_ = static_cast<std::uint32_t>(models.size());
+    view = ck::EnvironmentBorrowedView(d);
+    config.maximumMagneticDeflectionRad_ = e.maximum_magnetic_deflection_rad;
+  }
+};
+
+}  // namespace
+
+int ref_em_environment_query(c8mi_environment const* e, double const position[3],
+                             double const direction[3], std::int32_t node, std::int32_t* out_node,
+                             double* out_density) {
+  RefEnvironment env(*e);
+  double const p[3] = {position[0], position[1], position[2]};
+  auto const r = ck::query_environment(env.view, env.config, p, direction, node);
+  *out_node = r.nodeIndex_;
+  *out_density = r.densityGPerCm3_;
+  return static_cast<int>(r.status_);
+}
+
+int ref_em_boundary(c8mi_environment const* e, double const position[3], double const direction[3],
+                    std::int32_t node, double* distance, std::int32_t* next_node) {
+  RefEnvironment env(*e);
+  double const p[3] = {position[0], position[1], position[2]};
+  double const d[3] = {direction[0], direction[1], direction[2]};
+  auto const r = ck::distance_to_environment_boundary(env.view, env.config, p, d, node);
+  *distance = r.distanceM_;
+  *next_node = r.nextNode_;
+  return static_cast<int>(r.status_);
+}
+
+int ref_em_grammage(c8mi_environment const* e, std::int32_t node, double const position[3],
+                    double const direction[3], double distance, double* out) {
+  RefEnvironment env(*e);
+  double const p[3] = {position[0], position[1], position[2]};
+  double const d[3] = {direction[0], direction[1], direction[2]};
+  auto const r = ck::environment_grammage(env.view, env.config, node, p, d, distance);
+  *out = r.value_;
+  return static_cast<int>(r.status_);
+}
+
+int ref_em_arclength(c8mi_environment const* e, std::int32_t node, double const position[3],
+                     double const direction[3], double grammage, double* out) {
+  RefEnvironment env(*e);
+  double const p[3] = {position[0], position[1], position[2]};
+  double const d[3] = {direction[0], direction[1], direction[2]};
+  auto const r = ck::environment_arclength(env.view, env.config, node, p, d, grammage);
+  *out = r.value_;
+  return static_cast<int>(r.status_);
+}
+
+int ref_em_pair_lpm(c8mi_photon_pair_lpm const* s, std::uint64_t comp, double e_mev, double x,
+                    double rho, double* out) {
+  ck::PhotonPairLpmSnapshot snap{};
+  std::memcpy(&snap, s, sizeof snap);
+  auto const r = ck::photon_pair_lpm_suppression_factor(snap, comp, e_mev, x, rho);
+  *out = r.survivalProbability_;
+  return static_cast<int>(r.status_);
+}
+
+int ref_em_brems_lpm(c8mi_brems_lpm const* s, std::uint64_t comp, double e_mev, double v, double rho,
+                     double* out) {
+  ck::BremsLpmSnapshot snap{};
+  std::memcpy(&snap, s, sizeof snap);
+  auto const r = ck::brems_lpm_suppression_factor(snap, comp, e_mev, v, rho);
+  *out = r.survivalProbability_;
+  return static_cast<int>(r.status_);
+}
+
+int ref_em_thinning(c8mi_em_thinning const* c, double pe, double pw, double e1, double e2, double u1,
+                    double u2, std::uint32_t* keep, double* w1, double* w2) {
+  ck::EmThinningConfig cfg{};
+  cfg.thresholdGeV_ = c->threshold_gev;
+  cfg.maximumWeight_ = c->maximum_weight;
+  cfg.enabled_ = c->enabled;
+  cfg.eraseZeroWeight_ = c->erase_zero_weight;
+  auto const r = ck::apply_em_thinning(cfg, pe, pw, e1, e2, u1, u2);
+  *keep = r.keepMask_;
+  *w1 = r.firstWeight_;
+  *w2 = r.secondWeight_;
+  return static_cast<int>(r.status_);
+}
+
+int ref_em_pair_split(double e_gev, double x, double* normalized, double* decoded) {
+  if (!ck::encode_photon_pair_normalized_split(e_gev, x, *normalized)) return 0;
+  return ck::decode_photon_pair_normalized_split(e_gev, *normalized, *decoded) ? 1 : 0;
 }
 
 }  // extern "C"

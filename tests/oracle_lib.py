@@ -178,3 +178,104 @@ def native_queries(tables, queries, reference=False):
     fn(byref(tables.struct), queries.ctypes.data_as(c_void_p), c_uint64(queries.size),
        out.ctypes.data_as(c_void_p))
     return out
+
+
+# ---- EM transport (em_oracle.c) -----------------------------------------------
+
+def em_run_shower(config, tables, environment, primaries, shower_id=1, first_history_id=2,
+                  maximum_fronts=1 << 40):
+    """Runs the oracle's serial front schedule; returns histograms (int64),
+    raw counters, stats and the stored observations / fallbacks."""
+    from c8mi import PARTICLE_STATE_DTYPE
+    from c8mi.em import COUNTERS, FALLBACK_DTYPE, RunResult
+
+    lib = load()
+    cfg = config.struct if hasattr(config, "struct") else config
+    bins = cfg.profile_bins
+    hist = {k: np.zeros(bins, dtype=np.int64) for k in ("photons", "electrons", "positrons", "energy_loss")}
+    counters = np.zeros(COUNTERS, dtype=np.int64)
+    obs = np.zeros(cfg.observation_capacity, dtype=PARTICLE_STATE_DTYPE)
+    fbs = np.zeros(cfg.fallback_capacity, dtype=FALLBACK_DTYPE)
+    prim = np.ascontiguousarray(primaries, dtype=PARTICLE_STATE_DTYPE).reshape(-1)
+    stats = RunResult()
+    env = environment.struct if hasattr(environment, "struct") else environment
+    p = lambda a: a.ctypes.data_as(c_void_p)
+    lib.oracle_em_run_shower(byref(cfg), byref(tables.struct), byref(env), c_uint64(shower_id),
+                             c_uint64(first_history_id), p(prim), c_uint64(prim.size),
+                             c_uint64(maximum_fronts), p(hist["photons"]), p(hist["electrons"]),
+                             p(hist["positrons"]), p(hist["energy_loss"]), p(counters), p(obs),
+                             p(fbs), byref(stats))
+    st = stats.as_dict()
+    n_obs = min(st["observations"], obs.size)
+    n_fb = min(st["fallbacks"], fbs.size)
+    return dict(hist=hist, counters=counters, stats=st, observations=obs[:n_obs], fallbacks=fbs[:n_fb])
+
+
+def _env_ptr(environment):
+    return byref(environment.struct if hasattr(environment, "struct") else environment)
+
+
+def em_env_query(environment, position, direction, node, reference=False):
+    lib = load_ref() if reference else load()
+    fn = lib.ref_em_environment_query if reference else lib.oracle_em_environment_query
+    out_node, dens = ctypes.c_int32(), c_double()
+    p = (c_double * 3)(*position)
+    d = None if direction is None else (c_double * 3)(*direction)
+    st = fn(_env_ptr(environment), p, d, ctypes.c_int32(node), byref(out_node), byref(dens))
+    return st, out_node.value, dens.value
+
+
+def em_boundary(environment, position, direction, node, reference=False):
+    lib = load_ref() if reference else load()
+    fn = lib.ref_em_boundary if reference else lib.oracle_em_boundary
+    dist, nxt = c_double(), ctypes.c_int32()
+    st = fn(_env_ptr(environment), (c_double * 3)(*position), (c_double * 3)(*direction),
+            ctypes.c_int32(node), byref(dist), byref(nxt))
+    return st, dist.value, nxt.value
+
+
+def em_grammage(environment, node, position, direction, value, arclength=False, reference=False):
+    lib = load_ref() if reference else load()
+    name = ("ref_em_" if reference else "oracle_em_") + ("arclength" if arclength else "grammage")
+    out = c_double()
+    st = getattr(lib, name)(_env_ptr(environment), ctypes.c_int32(node), (c_double * 3)(*position),
+                            (c_double * 3)(*direction), c_double(value), byref(out))
+    return st, out.value
+
+
+def em_lpm(snapshot, comp, e_mev, x, rho, brems=False, reference=False):
+    lib = load_ref() if reference else load()
+    name = ("ref_em_" if reference else "oracle_em_") + ("brems_lpm" if brems else "pair_lpm")
+    out = c_double()
+    st = getattr(lib, name)(byref(snapshot), c_uint64(comp), c_double(e_mev), c_double(x), c_double(rho),
+                            byref(out))
+    return st, out.value
+
+
+def em_thinning(cfg, pe, pw, e1, e2, u1, u2, reference=False):
+    lib = load_ref() if reference else load()
+    fn = lib.ref_em_thinning if reference else lib.oracle_em_thinning
+    keep, w1, w2 = c_uint32(), c_double(), c_double()
+    st = fn(byref(cfg), c_double(pe), c_double(pw), c_double(e1), c_double(e2), c_double(u1), c_double(u2),
+            byref(keep), byref(w1), byref(w2))
+    return st, keep.value, w1.value, w2.value
+
+
+def em_pair_split(e_gev, x, reference=False):
+    lib = load_ref() if reference else load()
+    fn = lib.ref_em_pair_split if reference else lib.oracle_em_pair_split
+    s, d = c_double(), c_double()
+    ok = fn(c_double(e_gev), c_double(x), byref(s), byref(d))
+    return ok, s.value, d.value
+
+
+def em_scalar(name, *args):
+    """Calls a double-returning or status-returning oracle helper."""
+    lib = load()
+    fn = getattr(lib, name)
+    if name == "oracle_em_sauter_cosine":
+        fn.restype = c_double
+        return fn(*[c_double(a) for a in args])
+    outs = [c_double(), c_double()] if name == "oracle_em_compton_cosines" else [c_double()]
+    st = fn(*[c_double(a) for a in args], *[byref(o) for o in outs])
+    return (st, *[o.value for o in outs])

@@ -1,0 +1,145 @@
+"""GPU parity of the EM transport fronts (c8mi_em.h) against the oracle's
+serial restatement (oracle/em_oracle.c) on the same tables, atmosphere,
+primaries and Philox keys.
+
+Both run the same front schedule (photon front, then lepton front) with the
+stable source order of the reference's scans, so history IDs, every Philox
+draw and therefore whole showers coincide. The device evaluates log/exp/pow
+with the GPU math library, whose results may differ from the host's in the
+last ulp; the bar is:
+  * identical integer statistics (fronts, steps, interactions, secondaries,
+    history IDs, observations, fallbacks, ledger counts);
+  * ledger energies and profile bins within 1e-9 of the primary energy;
+  * observation-surface particles identical up to 1e-9 relative.
+At 100 GeV (the benchmark shower) the checks are size-independent: energy
+closure of the device ledger to 1e-9 of E0 and agreement of the shower totals
+with the oracle.
+"""
+import numpy as np
+import pytest
+
+import c8mi
+import oracle_lib
+from c8mi import airtables, em
+
+pytestmark = pytest.mark.gpu
+
+STAT_KEYS = ("photon_fronts", "lepton_fronts", "photon_steps", "lepton_steps", "photon_interactions",
+             "lepton_interactions", "secondaries", "observations", "fallbacks", "pending_photons",
+             "pending_leptons", "next_history_id", "max_front")
+
+
+@pytest.fixture(scope="module")
+def tables():
+    return airtables.build_air_tables()
+
+
+def run_pair(tables, pid, energy, seed, shower_id=1, thinning=None, start=(0.0, 0.0, 112750.0),
+             direction=(0.0, 0.0, -1.0), primaries=None):
+    atm = em.Atmosphere()
+    cfg = em.make_config(airtables, seed=seed, atmosphere=atm, thinning=thinning, start=start,
+                         direction=direction)
+    prim = primaries if primaries is not None else em.particle(pid, energy, start, direction, history_id=1)
+    want = oracle_lib.em_run_shower(cfg, tables, atm, prim, shower_id=shower_id, first_history_id=2)
+    dev = em.EmBackend(cfg, tables, atm)
+    dev.begin_shower(shower_id, 2)
+    dev.enqueue(prim)
+    got = dev.run()
+    hist, ledger = dev.download_profile()
+    obs, nobs = dev.observations()
+    fbs, nfb = dev.fallbacks()
+    dev.close()
+    return cfg, want, dict(stats=got, hist=hist, ledger=ledger, observations=obs, fallbacks=fbs)
+
+
+def compare(cfg, want, got, energy):
+    for k in STAT_KEYS:
+        assert got["stats"][k] == want["stats"][k], (k, got["stats"][k], want["stats"][k])
+    wl = em.decode_counters(want["counters"], cfg.struct.profile_energy_scale)
+    gl = got["ledger"]
+    for k in ("steps", "photon_cuts", "lepton_limits", "thinning_hillas_vertices",
+              "thinning_statistical_vertices", "thinning_particles_discarded", "deposited_steps",
+              "fixed_point_overflows", "invalid_records", "fallback_reasons"):
+        assert gl[k] == wl[k], (k, gl[k], wl[k])
+    for k in em.LEDGER_ENERGY_TERMS:
+        assert abs(gl[k] - wl[k]) <= 1e-9 * energy, (k, gl[k], wl[k])
+    ws, es = cfg.struct.profile_weight_scale, cfg.struct.profile_energy_scale
+    for k in ("photons", "electrons", "positrons"):
+        assert np.max(np.abs(got["hist"][k] - want["hist"][k] / ws)) <= 1e-6, k
+    assert np.max(np.abs(got["hist"]["energy_loss"] - want["hist"]["energy_loss"] / es)) <= 1e-9 * energy
+    # observation-surface particles, as sets
+    key = lambda a: np.lexsort((a["step_id"], a["history_id"]))
+    go, wo = got["observations"], want["observations"]
+    assert go.size == wo.size
+    go, wo = go[key(go)], wo[key(wo)]
+    assert np.array_equal(go["history_id"], wo["history_id"])
+    assert np.array_equal(go["pid"], wo["pid"])
+    for f in ("energy_gev", "time_s"):
+        np.testing.assert_allclose(go[f], wo[f], rtol=1e-9, atol=0)
+    np.testing.assert_allclose(go["position_m"], wo["position_m"], rtol=1e-9, atol=1e-6)
+    assert abs(em.ledger_residual(gl)) <= 1e-9 * energy
+
+
+@pytest.mark.parametrize("pid,energy,seed", [(22, 1.0, 1), (22, 1.0, 2), (22, 10.0, 3), (11, 5.0, 4),
+                                             (-11, 3.0, 5)])
+def test_shower_matches_oracle(tables, pid, energy, seed):
+    cfg, want, got = run_pair(tables, pid, energy, seed)
+    compare(cfg, want, got, energy)
+    assert got["stats"]["photon_steps"] + got["stats"]["lepton_steps"] > 100
+
+
+def test_thinned_shower_matches_oracle(tables):
+    cfg, want, got = run_pair(tables, 22, 10.0, 6, thinning=(1.0, 100.0))
+    compare(cfg, want, got, 10.0)
+    assert got["ledger"]["thinning_hillas_vertices"] + got["ledger"]["thinning_statistical_vertices"] > 0
+
+
+def test_inclined_shower_reaches_ground_and_matches(tables):
+    # a 60 degree shower started low enough that particles reach the observation level
+    d = (np.sin(np.radians(60.0)), 0.0, -np.cos(np.radians(60.0)))
+    start = (-2000.0 * d[0] / d[2] * 0, 0.0, 3000.0)
+    cfg, want, got = run_pair(tables, 22, 20.0, 7, start=start, direction=d)
+    compare(cfg, want, got, 20.0)
+    assert got["stats"]["observations"] > 0
+
+
+def test_mixed_primary_batch(tables):
+    prim = np.concatenate([np.atleast_1d(em.particle(22, 2.0, (0, 0, 30000.0), (0, 0, -1), history_id=1)),
+                           np.atleast_1d(em.particle(11, 1.5, (10, 0, 30000.0), (0, 0, -1), history_id=2)),
+                           np.atleast_1d(em.particle(-11, 0.8, (0, 10, 30000.0), (0, 0, -1), history_id=3))])
+    cfg, want, got = run_pair(tables, 0, 0, 8, primaries=prim, start=(0.0, 0.0, 30000.0))
+    compare(cfg, want, got, 4.3)
+
+
+def test_benchmark_shower_closes_energy(tables):
+    """100 GeV vertical photon from 112.75 km: device ledger closes to 1e-9 E0
+    and the totals agree with the oracle's shower."""
+    cfg, want, got = run_pair(tables, 22, 100.0, 1)
+    gl = got["ledger"]
+    assert abs(em.ledger_residual(gl)) <= 1e-9 * 100.0
+    assert gl["invalid_records"] == 0 and gl["fixed_point_overflows"] == 0
+    total = got["stats"]["photon_steps"] + got["stats"]["lepton_steps"]
+    want_total = want["stats"]["photon_steps"] + want["stats"]["lepton_steps"]
+    assert abs(total - want_total) <= 1e-3 * want_total
+    wl = em.decode_counters(want["counters"], cfg.struct.profile_energy_scale)
+    for k in ("continuous_deposit", "cut_deposit", "observed_energy"):
+        assert abs(gl[k] - wl[k]) <= 1e-3 * 100.0, k
+
+
+def test_backend_contract(tables):
+    atm = em.Atmosphere()
+    cfg = em.make_config(airtables, seed=3, atmosphere=atm)
+    dev = em.EmBackend(cfg, tables, atm)
+    dev.begin_shower(1, 2)
+    assert dev.can_transport(em.particle(22, 1.0, (0, 0, 1e4), (0, 0, -1)))
+    assert not dev.can_transport(em.particle(13, 1.0, (0, 0, 1e4), (0, 0, -1)))
+    with pytest.raises(c8mi.C8miError) as err:
+        dev.enqueue(em.particle(2212, 1.0, (0, 0, 1e4), (0, 0, -1)))
+    assert err.value.status == 1
+    r = dev.run_photon_wavefront(em.particle(22, 1.0, (0, 0, 1e4), (0, 0, -1), history_id=1), 1)
+    assert r["photon_fronts"] == 1 and r["lepton_fronts"] == 0
+    photons, leptons = dev.pending()
+    assert (photons, leptons) == (r["pending_photons"], r["pending_leptons"])
+    r2 = dev.run_lepton_wavefront(np.zeros(0, dtype=c8mi.PARTICLE_STATE_DTYPE), 4)
+    assert r2["photon_fronts"] == 0
+    dev.close()
