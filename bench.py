@@ -53,6 +53,9 @@ def parse():
                    help="run CoREAS and ZHS as two launches instead of one fused launch")
     p.add_argument("--exact", action="store_true",
                    help="reference FP64 operation order (bit-identical to the FP64 oracle)")
+    p.add_argument("--em-showers", type=int, default=10,
+                   help="timed 100 GeV showers of the EM section (0 disables it)")
+    p.add_argument("--em-only", action="store_true", help="run only the EM section")
     return p.parse_args()
 
 
@@ -122,6 +125,123 @@ def field_error(tracks, observers, propagation, mode, limit, exact):
     return errors
 
 
+EM_BYTES_PER_STEP = 232  # 116 B SoA state read + 116 B written per particle-step
+
+
+def em_cpu_baseline(tables, atmosphere, energy, target_seconds):
+    """Oracle EM showers (the serial C restatement, one shower per host thread,
+    OpenMP over showers on all host cores) for about `target_seconds`."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib
+    from c8mi import airtables, em
+
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count() or 1
+    threads = max(1, min(threads, os.cpu_count() or 1))
+    cfg = em.make_config(airtables, atmosphere=atmosphere)
+    prim = em.particle(22, energy, (0.0, 0.0, 112750.0), (0.0, 0.0, -1.0), history_id=1)
+    count = threads
+    first = 1000
+    while True:
+        t0 = time.perf_counter()
+        stats = oracle_lib.em_run_batch(cfg, tables, atmosphere, np.stack([prim] * count),
+                                        first_shower=first, threads=threads)
+        dt = time.perf_counter() - t0
+        if dt >= 0.5 * target_seconds:
+            break
+        first += count
+        count = int(count * max(2.0, 1.2 * target_seconds / max(dt, 1e-3)))
+    steps = sum(st["photon_steps"] + st["lepton_steps"] for st in stats)
+    return {"value": steps / dt, "unit": "particle-steps/s", "cores": threads, "kind": "port",
+            "sample": f"{count} x {energy:g} GeV vertical photon showers, one per thread, on {threads} "
+                      f"threads (oracle, serial C per shower, OpenMP over showers; {steps} steps, "
+                      f"{dt:.1f} s wall)"}
+
+
+def em_measured_traffic(kernel):
+    path = os.path.join(ROOT, "profiles", "em_traffic.json")
+    try:
+        with open(path) as f:
+            data = json.load(f)
+    except (OSError, ValueError):
+        return None
+    return data.get("traffic_bytes_per_step") if data.get("kernel") == kernel else None
+
+
+def em_section(showers, warmup, cpu_seconds, no_cpu_baseline, energy=100.0):
+    """EM particle-steps/s on the plain 100 GeV vertical photon shower in the
+    USStdBK atmosphere (1 MeV cut, no thinning, FP64), the whole cascade on the
+    device from the primary on; one step = one shower with a fresh seed."""
+    from c8mi import airtables, em
+
+    tables = airtables.build_air_tables()
+    atm = em.Atmosphere()
+    cfg = em.make_config(airtables, atmosphere=atm)
+    dev = em.EmBackend(cfg, tables, atm)
+
+    def shower(seed):
+        cfg.struct.random_seed = seed
+        prim = em.particle(22, energy, (0.0, 0.0, 112750.0), (0.0, 0.0, -1.0), history_id=1)
+        dev.begin_shower(seed, 2)
+        dev.enqueue(prim)
+        return dev.run(1 << 20)
+
+    for k in range(warmup):
+        shower(1 + k)
+    dev.enable_kernel_timing(True)
+    steps = 0
+    latencies = []
+    t0 = time.perf_counter()
+    for k in range(showers):
+        ts = time.perf_counter()
+        r = shower(100 + k)
+        latencies.append(time.perf_counter() - ts)
+        steps += r["photon_steps"] + r["lepton_steps"]
+    elapsed = time.perf_counter() - t0
+    timing = dev.kernel_time()
+    _, ledger = dev.download_profile()
+    dev.close()
+    kind = max(("photon_step", "lepton_step"), key=lambda k: timing[k][0])
+    step_ms, launches = timing[kind]
+    total_kernel_ms = sum(v[0] for v in timing.values())
+    # algorithmic bytes of all particle-steps over the summed step-kernel time
+    step_kernel_ms = timing["photon_step"][0] + timing["lepton_step"][0]
+    achieved = steps * EM_BYTES_PER_STEP / (step_kernel_ms * 1e-3) / 1e9
+    kernel_name = "step_kernel<photon>" if kind == "photon_step" else "step_kernel<lepton>"
+    out = {
+        "metric": "EM particle-steps/s",
+        "value": steps / elapsed,
+        "unit": "particle-steps/s",
+        "showers": showers,
+        "warmup": warmup,
+        "particle_steps": steps,
+        "ms_per_shower": elapsed * 1e3 / showers,
+        "latency_ms": {"p50": float(np.percentile(latencies, 50) * 1e3),
+                       "p99": float(np.percentile(latencies, 99) * 1e3)},
+        "dtype": "f64",
+        "config": {"workload": f"EM plain: {energy:g} GeV vertical photon at 112.75 km, USStdBK 5-layer "
+                               "spherical air, observation level 0 m, 1 MeV cut, no thinning, no radio, "
+                               "closed-form v7 air tables, whole cascade on the device"},
+        "roofline": {
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": em_measured_traffic(kernel_name),
+            "kernel": kernel_name,
+            "bytes_per_step": EM_BYTES_PER_STEP,
+            "kernel_ms_per_shower": {k: v[0] / showers for k, v in timing.items()},
+            "device_busy_fraction": total_kernel_ms * 1e-3 / elapsed,
+        },
+        "ledger_residual_gev": em.ledger_residual(ledger),
+    }
+    if not no_cpu_baseline:
+        base = em_cpu_baseline(tables, atm, energy, cpu_seconds)
+        out["cpu_baseline"] = base
+        out["speedup_vs_cpu_baseline"] = out["value"] / base["value"]
+    return out
+
+
 def measured_traffic(kernel):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 PMC summary
     (FETCH_SIZE + WRITE_SIZE, copy-calibrated), or None."""
@@ -151,6 +271,13 @@ def main():
     if c8mi.device_count() < 1:
         print("bench.py: no HIP device visible", file=sys.stderr)
         return 1
+    if args.em_only:
+        em_result = em_section(max(args.em_showers, 1), args.warmup, args.cpu_seconds, args.no_cpu_baseline)
+        em_result.update({"n_gpus": 1, "steps": em_result["showers"], "higher_is_better": True,
+                          "scaling": "weak", "vs_baseline": None, "data": "synthetic",
+                          "ms_per_step": em_result["ms_per_shower"]})
+        print(json.dumps(em_result), flush=True)
+        return 0
     tracks, observers, refractivity, (lo, hi) = workloads.config2_setup(count=args.tracks)
     propagation = c8mi.Propagation(refractivity=refractivity, minimum_height_m=lo,
                                    maximum_height_m=hi, step_m=1.0)
@@ -242,6 +369,8 @@ def main():
         base = cpu_baseline(tracks, observers, propagation, args.mode, args.cpu_seconds)
         result["cpu_baseline"] = base
         result["speedup_vs_cpu_baseline"] = value / base["value"]
+    if args.em_showers > 0:
+        result["em"] = em_section(args.em_showers, 2, args.cpu_seconds, args.no_cpu_baseline)
     print(json.dumps(result), flush=True)
     return 0
 

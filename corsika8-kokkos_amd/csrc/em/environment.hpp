@@ -46,7 +46,7 @@ __device__ __forceinline__ bool contains(c8mi_env_node const& s, double const p[
   return dot3(d, d) < s.radius_m * s.radius_m;
 }
 
-__device__ inline std::int32_t locate(EmContext const& c, double const p[3]) {
+__device__ __forceinline__ std::int32_t locate(EmContext const& c, double const p[3]) {
   std::int32_t current = 0;
   for (std::uint32_t hop = 0; hop < c.node_count; ++hop) {
     if (current < 0 || static_cast<std::uint32_t>(current) >= c.node_count) return -2;
@@ -83,7 +83,7 @@ __device__ __forceinline__ Intersections intersect(c8mi_env_node const& s, doubl
 }
 
 // next_environment_boundary: own exit or nearest child entry.
-__device__ inline void next_boundary(EmContext const& c, double const p[3], double const v[3],
+__device__ __forceinline__ void next_boundary(EmContext const& c, double const p[3], double const v[3],
                                      std::int32_t current, double limit, double& parameter,
                                      std::int32_t& node) {
   double selected = limit;
@@ -152,7 +152,7 @@ struct Layer {
   double density;
 };
 
-__device__ inline Layer query(EmContext const& c, double const p[3], double const* dir,
+__device__ __forceinline__ Layer query(EmContext const& c, double const p[3], double const* dir,
                               std::int32_t logical) {
   if (!valid_environment(c)) return {kInvalid, -1, 0.};
   for (int a = 0; a < 3; ++a)
@@ -171,7 +171,7 @@ struct Boundary {
   std::int32_t next_node;
 };
 
-__device__ inline Boundary distance_to_boundary(EmContext const& c, double const p[3],
+__device__ __forceinline__ Boundary distance_to_boundary(EmContext const& c, double const p[3],
                                                 double const dir[3], std::int32_t logical) {
   Layer const l = query(c, p, dir, logical);
   if (l.status != kSuccess) return {l.status, 0., -3};
@@ -187,7 +187,7 @@ struct Grammage {
   double value;
 };
 
-__device__ inline Grammage grammage(EmContext const& c, std::int32_t node, double const p[3],
+__device__ __forceinline__ Grammage grammage(EmContext const& c, std::int32_t node, double const p[3],
                                     double const dir[3], double distance) {
   c8mi_env_model const* m = model_of(c, node);
   if (!valid_environment(c) || !m || !isfinite(distance) || distance < 0. || !normalized(dir))
@@ -196,7 +196,7 @@ __device__ inline Grammage grammage(EmContext const& c, std::int32_t node, doubl
   return {isfinite(x) && x >= 0. ? kSuccess : kGrammageOutOfRange, x};
 }
 
-__device__ inline Grammage arclength(EmContext const& c, std::int32_t node, double const p[3],
+__device__ __forceinline__ Grammage arclength(EmContext const& c, std::int32_t node, double const p[3],
                                      double const dir[3], double x) {
   c8mi_env_model const* m = model_of(c, node);
   if (!valid_environment(c) || !m || !isfinite(x) || x < 0. || !normalized(dir))
@@ -205,7 +205,7 @@ __device__ inline Grammage arclength(EmContext const& c, std::int32_t node, doub
   return {isfinite(l) && l >= 0. ? kSuccess : kGrammageOutOfRange, l};
 }
 
-__device__ inline Grammage competition_grammage(EmContext const& c, std::int32_t node,
+__device__ __forceinline__ Grammage competition_grammage(EmContext const& c, std::int32_t node,
                                                 double const p[3], double const dir[3],
                                                 double distance) {
   Grammage r = grammage(c, node, p, dir, distance);
@@ -225,7 +225,7 @@ __device__ inline Grammage competition_grammage(EmContext const& c, std::int32_t
 }
 
 // Straight intersection with the observation plane; -1 when there is none.
-__device__ inline double observation_distance(EmContext const& c, double const p[3],
+__device__ __forceinline__ double observation_distance(EmContext const& c, double const p[3],
                                               double const dir[3]) {
   if (!c.observation_enabled) return -1.;
   double ns = 0.;
@@ -248,7 +248,7 @@ __device__ inline double observation_distance(EmContext const& c, double const p
 
 // Field-free lepton distance to the observation plane: the linear root of the
 // separation-plane quadratic in CurvedBoundaryAccess, in time units and back.
-__device__ inline double lepton_observation_distance(EmContext const& c, double const p[3],
+__device__ __forceinline__ double lepton_observation_distance(EmContext const& c, double const p[3],
                                                      double const dir[3], double speed,
                                                      double momentum_si) {
   double const delta[3] = {p[0] - c.observation_point[0], p[1] - c.observation_point[1],

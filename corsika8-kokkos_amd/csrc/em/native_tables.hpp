@@ -28,7 +28,7 @@ struct QueryResult {
   double value;
 };
 
-__device__ inline QueryResult qr(std::uint32_t status, double value = 0., std::uint32_t newton = 0,
+__device__ __forceinline__ QueryResult qr(std::uint32_t status, double value = 0., std::uint32_t newton = 0,
                                  std::uint32_t bisection = 0) {
   return {status, newton, bisection, value};
 }
@@ -37,7 +37,7 @@ namespace detail {
 
 constexpr double kLn2 = 0.693147180559945309417232121458176568;
 
-__device__ inline double axis_transform(c8mi_native_axis const& a, double x) {
+__device__ __forceinline__ double axis_transform(c8mi_native_axis const& a, double x) {
   switch (a.type) {
     case C8MI_AXIS_LINEAR: return (x - a.low) / a.step;
     case C8MI_AXIS_EXPONENTIAL: return log(x / a.low) / a.step;
@@ -46,7 +46,7 @@ __device__ inline double axis_transform(c8mi_native_axis const& a, double x) {
   return __builtin_nan("");
 }
 
-__device__ inline double axis_back_transform(c8mi_native_axis const& a, double c) {
+__device__ __forceinline__ double axis_back_transform(c8mi_native_axis const& a, double c) {
   switch (a.type) {
     case C8MI_AXIS_LINEAR: return c * a.step + a.low;
     case C8MI_AXIS_EXPONENTIAL: return a.low * exp(c * a.step);
@@ -55,7 +55,7 @@ __device__ inline double axis_back_transform(c8mi_native_axis const& a, double c
   return __builtin_nan("");
 }
 
-__device__ inline double axis_derivative(c8mi_native_axis const& a, double x) {
+__device__ __forceinline__ double axis_derivative(c8mi_native_axis const& a, double x) {
   switch (a.type) {
     case C8MI_AXIS_LINEAR: return 1. / a.step;
     case C8MI_AXIS_EXPONENTIAL: return 1. / (x * a.step);
@@ -64,7 +64,7 @@ __device__ inline double axis_derivative(c8mi_native_axis const& a, double x) {
   return __builtin_nan("");
 }
 
-__device__ inline double axis_back_derivative(c8mi_native_axis const& a, double c) {
+__device__ __forceinline__ double axis_back_derivative(c8mi_native_axis const& a, double c) {
   switch (a.type) {
     case C8MI_AXIS_LINEAR: return a.step;
     case C8MI_AXIS_EXPONENTIAL: return a.low * a.step * exp(c * a.step);
@@ -77,13 +77,13 @@ __device__ inline double axis_back_derivative(c8mi_native_axis const& a, double 
 // maps to 0 (the host form indexes with (uint32)-1 there); the fraction is NaN
 // either way, so the interpolated value is NaN as on the host, and no load
 // leaves the spline's coefficients.
-__device__ inline std::uint32_t interval_node(double coordinate, std::uint32_t nodes) {
+__device__ __forceinline__ std::uint32_t interval_node(double coordinate, std::uint32_t nodes) {
   if (!(coordinate >= 0.)) return 0;
   if (coordinate >= static_cast<double>(nodes - 2u)) return nodes - 2u;
   return static_cast<std::uint32_t>(coordinate);
 }
 
-__device__ inline void hermite(double x, double b[4]) {
+__device__ __forceinline__ void hermite(double x, double b[4]) {
   double const x2 = x * x, x3 = x2 * x;
   b[0] = 1. - 3. * x2 + 2. * x3;
   b[1] = 3. * x2 - 2. * x3;
@@ -91,7 +91,7 @@ __device__ inline void hermite(double x, double b[4]) {
   b[3] = -x2 + x3;
 }
 
-__device__ inline void hermite_derivative(double x, double b[4]) {
+__device__ __forceinline__ void hermite_derivative(double x, double b[4]) {
   double const x2 = x * x;
   b[0] = -6. * x + 6. * x2;
   b[1] = 6. * x - 6. * x2;
@@ -101,7 +101,7 @@ __device__ inline void hermite_derivative(double x, double b[4]) {
 
 // Four coefficient planes, column-major per spline: (row, column) at
 // offset + row + rows * column.
-__device__ inline double bicubic_sum(Tables const& t, c8mi_native_bicubic const& s, double e,
+__device__ __forceinline__ double bicubic_sum(Tables const& t, c8mi_native_bicubic const& s, double e,
                                      double loss, bool derivative) {
   double const x0 = axis_transform(s.energy_axis, e);
   double const x1 = axis_transform(s.loss_axis, loss);
@@ -129,14 +129,14 @@ __device__ inline double bicubic_sum(Tables const& t, c8mi_native_bicubic const&
   return derivative ? r / s.loss_axis.step : r;
 }
 
-__device__ inline double dot4(double a0, double a1, double a2, double a3, double b0, double b1,
+__device__ __forceinline__ double dot4(double a0, double a1, double a2, double a3, double b0, double b1,
                               double b2, double b3) {
   double const p0 = a0 * b0, p1 = a1 * b1, p2 = a2 * b2, p3 = a3 * b3;
   return ((p0 + p1) + p2) + p3;
 }
 
 // Host-precomputed 16-coefficient cell polynomial.
-__device__ inline double bicubic_precomputed(Tables const& t, c8mi_native_bicubic const& s,
+__device__ __forceinline__ double bicubic_precomputed(Tables const& t, c8mi_native_bicubic const& s,
                                              double e, double loss) {
   if (s.polynomial_coefficient_offset == C8MI_NO_POLYNOMIAL_COEFFICIENTS ||
       t.bicubic_polynomial_coefficients == nullptr)
@@ -162,7 +162,7 @@ __device__ inline double bicubic_precomputed(Tables const& t, c8mi_native_bicubi
   return dot4(v0[0], v0[1], v0[2], v0[3], p[0], p[1], p[2], p[3]);
 }
 
-__device__ inline double bicubic76(Tables const& t, c8mi_native_bicubic const& s, double e,
+__device__ __forceinline__ double bicubic76(Tables const& t, c8mi_native_bicubic const& s, double e,
                                    double loss) {
   if (s.polynomial_coefficient_offset != C8MI_NO_POLYNOMIAL_COEFFICIENTS &&
       t.bicubic_polynomial_coefficients != nullptr)
@@ -171,7 +171,7 @@ __device__ inline double bicubic76(Tables const& t, c8mi_native_bicubic const& s
 }
 
 // CubicInterpolation 0.1.5's prime(): Boost's sixth-order finite difference.
-__device__ inline double bicubic_loss_derivative76(Tables const& t, c8mi_native_bicubic const& s,
+__device__ __forceinline__ double bicubic_loss_derivative76(Tables const& t, c8mi_native_bicubic const& s,
                                                    double e, double loss) {
   constexpr double epsilon = 2.220446049250313080847263336181640625e-16;
   double h = pow(epsilon / 168., 1. / 7.);
@@ -186,7 +186,7 @@ __device__ inline double bicubic_loss_derivative76(Tables const& t, c8mi_native_
   return cd * axis_derivative(s.loss_axis, loss);
 }
 
-__device__ inline double cubic(Tables const& t, c8mi_native_cubic const& s, double x) {
+__device__ __forceinline__ double cubic(Tables const& t, c8mi_native_cubic const& s, double x) {
   double const coordinate = axis_transform(s.axis, x);
   std::uint32_t const node = interval_node(coordinate, s.coefficient_count);
   double const fraction = coordinate - node;
@@ -199,7 +199,7 @@ __device__ inline double cubic(Tables const& t, c8mi_native_cubic const& s, doub
   return value;
 }
 
-__device__ inline double cubic_derivative76(Tables const& t, c8mi_native_cubic const& s,
+__device__ __forceinline__ double cubic_derivative76(Tables const& t, c8mi_native_cubic const& s,
                                             double x) {
   double const coordinate = axis_transform(s.axis, x);
   std::uint32_t const node = interval_node(coordinate, s.coefficient_count);
@@ -218,7 +218,7 @@ __device__ inline double cubic_derivative76(Tables const& t, c8mi_native_cubic c
   return derivative * axis_derivative(s.axis, x);
 }
 
-__device__ inline int sgn(double x) { return (0. < x ? 1 : 0) - (x < 0. ? 1 : 0); }
+__device__ __forceinline__ int sgn(double x) { return (0. < x ? 1 : 0) - (x < 0. ? 1 : 0); }
 
 struct NewtonResult {
   double value;
@@ -228,7 +228,7 @@ struct NewtonResult {
 
 // Boost 1.79 newton_raphson_iterate(f, guess, lower, upper, digits = 20).
 template <class F, class D>
-__device__ inline NewtonResult newton76(F const& f, D const& df, double initial, double lower,
+__device__ __forceinline__ NewtonResult newton76(F const& f, D const& df, double initial, double lower,
                                         double upper) {
   double residual = 0., last_residual = 0., result = initial, guess = initial;
   double delta = 1.7976931348623157e308, delta1 = delta, delta2 = delta;
@@ -293,7 +293,7 @@ __device__ inline NewtonResult newton76(F const& f, D const& df, double initial,
   return {result, it, it == 256};
 }
 
-__device__ inline bool valid_view(Tables const& t) {
+__device__ __forceinline__ bool valid_view(Tables const& t) {
   return t.dndx_column_count > 0 && t.dndx_columns != nullptr && t.total_rate_column_count > 0 &&
          t.total_rate_columns != nullptr && t.selection_column_indices != nullptr &&
          (t.dedx_column_count == 0 ||
@@ -303,7 +303,7 @@ __device__ inline bool valid_view(Tables const& t) {
          t.bicubic_coefficient_count > 0;
 }
 
-__device__ inline c8mi_native_dndx_column const* find_dndx(Tables const& t, std::int32_t pid,
+__device__ __forceinline__ c8mi_native_dndx_column const* find_dndx(Tables const& t, std::int32_t pid,
                                                            std::int32_t process,
                                                            std::uint64_t component) {
   std::uint32_t lo = 0, hi = t.dndx_column_count;
@@ -324,7 +324,7 @@ __device__ inline c8mi_native_dndx_column const* find_dndx(Tables const& t, std:
   return nullptr;
 }
 
-__device__ inline void dndx_range(Tables const& t, std::int32_t pid, std::uint32_t& begin,
+__device__ __forceinline__ void dndx_range(Tables const& t, std::int32_t pid, std::uint32_t& begin,
                                   std::uint32_t& end) {
   std::uint32_t lo = 0, hi = t.dndx_column_count;
   while (lo < hi) {
@@ -342,7 +342,7 @@ __device__ inline void dndx_range(Tables const& t, std::int32_t pid, std::uint32
   end = lo;
 }
 
-__device__ inline c8mi_native_dndx_column const* selection_dndx(Tables const& t,
+__device__ __forceinline__ c8mi_native_dndx_column const* selection_dndx(Tables const& t,
                                                                 std::uint32_t position) {
   if (position >= t.dndx_column_count) return nullptr;
   std::uint32_t const index = t.selection_column_indices[position];
@@ -350,7 +350,7 @@ __device__ inline c8mi_native_dndx_column const* selection_dndx(Tables const& t,
   return &t.dndx_columns[index];
 }
 
-__device__ inline c8mi_native_total_rate_column const* find_total_rate(Tables const& t,
+__device__ __forceinline__ c8mi_native_total_rate_column const* find_total_rate(Tables const& t,
                                                                       std::int32_t pid) {
   std::uint32_t lo = 0, hi = t.total_rate_column_count;
   while (lo < hi) {
@@ -363,7 +363,7 @@ __device__ inline c8mi_native_total_rate_column const* find_total_rate(Tables co
   return nullptr;
 }
 
-__device__ inline c8mi_native_utility_column const* find_utility(Tables const& t,
+__device__ __forceinline__ c8mi_native_utility_column const* find_utility(Tables const& t,
                                                                  std::int32_t pid) {
   for (std::uint32_t i = 0; i < t.utility_column_count; ++i)
     if (t.utility_columns[i].pdg_id == pid) return &t.utility_columns[i];
@@ -375,7 +375,7 @@ struct Limits {
   bool valid;
 };
 
-__device__ inline Limits kinematic_limits(Tables const& t, c8mi_native_dndx_column const& c,
+__device__ __forceinline__ Limits kinematic_limits(Tables const& t, c8mi_native_dndx_column const& c,
                                           double e) {
   if (!(e > 0.)) return {0., 0., false};
   double const mass = c.particle_mass_mev;
@@ -456,7 +456,7 @@ __device__ inline Limits kinematic_limits(Tables const& t, c8mi_native_dndx_colu
   return {cut, maximum, true};
 }
 
-__device__ inline double physical_loss(Tables const& t, c8mi_native_dndx_column const& c,
+__device__ __forceinline__ double physical_loss(Tables const& t, c8mi_native_dndx_column const& c,
                                        double e, double loss) {
   Limits const l = kinematic_limits(t, c, e);
   if (!l.valid) return __builtin_nan("");
@@ -479,14 +479,14 @@ __device__ inline double physical_loss(Tables const& t, c8mi_native_dndx_column 
   return __builtin_nan("");
 }
 
-__device__ inline bool rate_below_threshold(c8mi_native_dndx_column const& c, double e) {
+__device__ __forceinline__ bool rate_below_threshold(c8mi_native_dndx_column const& c, double e) {
   return c.rate_model == C8MI_RATE_BICUBIC &&
          (e < c.lower_energy_limit_mev || e < c.spline.energy_axis.low);
 }
 
 }  // namespace detail
 
-__device__ inline QueryResult cumulative_rate(Tables const& t, c8mi_native_dndx_column const& c,
+__device__ __forceinline__ QueryResult cumulative_rate(Tables const& t, c8mi_native_dndx_column const& c,
                                               double e, double loss) {
   using namespace detail;
   if (!valid_view(t)) return qr(C8MI_QUERY_INVALID_VIEW);
@@ -501,7 +501,7 @@ __device__ inline QueryResult cumulative_rate(Tables const& t, c8mi_native_dndx_
   return qr(C8MI_QUERY_SUCCESS, value);
 }
 
-__device__ inline QueryResult rate_for_column(Tables const& t, c8mi_native_dndx_column const& c,
+__device__ __forceinline__ QueryResult rate_for_column(Tables const& t, c8mi_native_dndx_column const& c,
                                               double e) {
   using namespace detail;
   if (!valid_view(t)) return qr(C8MI_QUERY_INVALID_VIEW);
@@ -575,7 +575,7 @@ __device__ inline QueryResult rate_for_column(Tables const& t, c8mi_native_dndx_
   return cumulative_rate(t, c, e, 1.);
 }
 
-__device__ inline QueryResult query_rate(Tables const& t, std::int32_t pid, std::int32_t process,
+__device__ __forceinline__ QueryResult query_rate(Tables const& t, std::int32_t pid, std::int32_t process,
                                          std::uint64_t component, double e) {
   using namespace detail;
   if (!valid_view(t)) return qr(C8MI_QUERY_INVALID_VIEW);
@@ -585,7 +585,7 @@ __device__ inline QueryResult query_rate(Tables const& t, std::int32_t pid, std:
   return rate_for_column(t, *c, e);
 }
 
-__device__ inline QueryResult total_rate(Tables const& t, std::int32_t pid, double e) {
+__device__ __forceinline__ QueryResult total_rate(Tables const& t, std::int32_t pid, double e) {
   using namespace detail;
   if (!valid_view(t)) return qr(C8MI_QUERY_INVALID_VIEW);
   if (!isfinite(e)) return qr(C8MI_QUERY_NON_FINITE_INPUT);
@@ -598,7 +598,7 @@ __device__ inline QueryResult total_rate(Tables const& t, std::int32_t pid, doub
   return qr(C8MI_QUERY_SUCCESS, value < 0. ? 0. : value);
 }
 
-__device__ inline QueryResult selection_total_rate(Tables const& t, std::int32_t pid, double e) {
+__device__ __forceinline__ QueryResult selection_total_rate(Tables const& t, std::int32_t pid, double e) {
   using namespace detail;
   if (!valid_view(t)) return qr(C8MI_QUERY_INVALID_VIEW);
   double total = 0.;
@@ -616,7 +616,7 @@ __device__ inline QueryResult selection_total_rate(Tables const& t, std::int32_t
   return qr(C8MI_QUERY_SUCCESS, total);
 }
 
-__device__ inline QueryResult dedx(Tables const& t, std::int32_t pid, double e) {
+__device__ __forceinline__ QueryResult dedx(Tables const& t, std::int32_t pid, double e) {
   using namespace detail;
   if (!isfinite(e)) return qr(C8MI_QUERY_NON_FINITE_INPUT);
   double total = 0.;
@@ -635,7 +635,7 @@ __device__ inline QueryResult dedx(Tables const& t, std::int32_t pid, double e) 
   return qr(C8MI_QUERY_SUCCESS, total);
 }
 
-__device__ inline QueryResult loss_fraction(Tables const& t, std::int32_t pid, std::int32_t process,
+__device__ __forceinline__ QueryResult loss_fraction(Tables const& t, std::int32_t pid, std::int32_t process,
                                             std::uint64_t component, double e, double q,
                                             bool photon_split) {
   using namespace detail;
@@ -832,7 +832,7 @@ __device__ inline QueryResult loss_fraction(Tables const& t, std::int32_t pid, s
   return qr(C8MI_QUERY_SUCCESS, physical, newton, bisection);
 }
 
-__device__ inline QueryResult range(Tables const& t, std::int32_t pid, double e) {
+__device__ __forceinline__ QueryResult range(Tables const& t, std::int32_t pid, double e) {
   using namespace detail;
   auto const* c = find_utility(t, pid);
   if (!c) return qr(C8MI_QUERY_PARTICLE_NOT_FOUND);
@@ -846,7 +846,7 @@ __device__ inline QueryResult range(Tables const& t, std::int32_t pid, double e)
   return qr(C8MI_QUERY_SUCCESS, value);
 }
 
-__device__ inline QueryResult energy_after_loss(Tables const& t, std::int32_t pid, double e0,
+__device__ __forceinline__ QueryResult energy_after_loss(Tables const& t, std::int32_t pid, double e0,
                                                 double grammage) {
   using namespace detail;
   auto const* c = find_utility(t, pid);
@@ -894,7 +894,7 @@ __device__ inline QueryResult energy_after_loss(Tables const& t, std::int32_t pi
   return qr(C8MI_QUERY_SUCCESS, lower, n.iterations, bisection);
 }
 
-__device__ inline QueryResult energy_of_range(Tables const& t, std::int32_t pid, double r) {
+__device__ __forceinline__ QueryResult energy_of_range(Tables const& t, std::int32_t pid, double r) {
   using namespace detail;
   auto const* c = find_utility(t, pid);
   if (!c) return qr(C8MI_QUERY_PARTICLE_NOT_FOUND);
@@ -917,7 +917,7 @@ __device__ inline QueryResult energy_of_range(Tables const& t, std::int32_t pid,
   return qr(C8MI_QUERY_SUCCESS, 0.5 * (lower + upper), 0, it);
 }
 
-__device__ inline QueryResult run_query(Tables const& t, c8mi_native_query const& q) {
+__device__ __forceinline__ QueryResult run_query(Tables const& t, c8mi_native_query const& q) {
   switch (q.kind) {
     case C8MI_NATIVE_TOTAL_RATE: return total_rate(t, q.pdg_id, q.energy_mev);
     case C8MI_NATIVE_SELECTION_TOTAL_RATE: return selection_total_rate(t, q.pdg_id, q.energy_mev);

@@ -128,7 +128,7 @@ __device__ __forceinline__ void ledger(Accumulator const& a, int term, double va
   add_fixed(a, &a.counters[term], value, a.ctx->energy_scale);
 }
 
-__device__ inline double project_grammage(EmContext const& c, double const p[3]) {
+__device__ __forceinline__ double project_grammage(EmContext const& c, double const p[3]) {
   double proj = 0.;
   for (int k = 0; k < 3; ++k) proj += (p[k] - c.axis_start[k]) * c.axis_direction[k];
   double const fb = proj / c.axis_step;
@@ -141,7 +141,7 @@ __device__ inline double project_grammage(EmContext const& c, double const p[3])
   return c.axis_grammage[upper] * f + c.axis_grammage[lower] * (1. - f);
 }
 
-__device__ inline void particle_profile(Accumulator const& a, std::int32_t pid, double start,
+__device__ __forceinline__ void particle_profile(Accumulator const& a, std::int32_t pid, double start,
                                         double end, double weight) {
   EmContext const& c = *a.ctx;
   double const s = start / c.bin_width, e = end / c.bin_width;
@@ -165,7 +165,7 @@ __device__ inline void particle_profile(Accumulator const& a, std::int32_t pid, 
     add_fixed(a, hist + k, weight, c.weight_scale);
 }
 
-__device__ inline void energy_profile(Accumulator const& a, double start, double end, double deposit) {
+__device__ __forceinline__ void energy_profile(Accumulator const& a, double start, double end, double deposit) {
   EmContext const& c = *a.ctx;
   if (!(deposit > 0.) || c.profile_bins == 0) return;
   count(a, kDepositedSteps);
@@ -198,7 +198,7 @@ __device__ inline void energy_profile(Accumulator const& a, double start, double
   for (int bin = first + 1; bin < last; ++bin) add_fixed(a, hist + bin, dens * w, c.energy_scale);
 }
 
-__device__ inline void point_energy_profile(Accumulator const& a, double x, double deposit) {
+__device__ __forceinline__ void point_energy_profile(Accumulator const& a, double x, double deposit) {
   EmContext const& c = *a.ctx;
   if (!(deposit > 0.) || c.profile_bins == 0) return;
   int const maximum_bin = static_cast<int>(c.profile_bins - 1);
@@ -210,7 +210,7 @@ __device__ inline void point_energy_profile(Accumulator const& a, double x, doub
             c.energy_scale);
 }
 
-__device__ inline void terminal_energy(Accumulator const& a, double e, bool observed) {
+__device__ __forceinline__ void terminal_energy(Accumulator const& a, double e, bool observed) {
   if (!(e >= 0.) || !isfinite(e)) {
     count(a, kInvalid);
     return;
@@ -218,7 +218,7 @@ __device__ inline void terminal_energy(Accumulator const& a, double e, bool obse
   ledger(a, observed ? kObserved : kEscaped, e);
 }
 
-__device__ inline void thinning_ledger(Accumulator const& a, int status, std::uint32_t multiplicity,
+__device__ __forceinline__ void thinning_ledger(Accumulator const& a, int status, std::uint32_t multiplicity,
                                        std::uint32_t kept, double parent_energy_weight,
                                        double kept_energy_weight) {
   if (status == 1) count(a, kThinHillas);
@@ -228,12 +228,12 @@ __device__ inline void thinning_ledger(Accumulator const& a, int status, std::ui
 }
 
 // Observation / fallback records: appended with one atomic per record (rare).
-__device__ inline void record_observation(EmContext const& c, Particle const& p) {
+__device__ __forceinline__ void record_observation(EmContext const& c, Particle const& p) {
   unsigned long long const slot = atomicAdd(&c.record_counts[0], 1ULL);
   if (slot < c.observation_capacity) c.observations[slot] = p;
 }
 
-__device__ inline void record_fallback(Accumulator const& a, Particle const& p, int reason,
+__device__ __forceinline__ void record_fallback(Accumulator const& a, Particle const& p, int reason,
                                        std::int32_t process, std::uint32_t stage) {
   EmContext const& c = *a.ctx;
   count(a, kFallbackReason0 + (reason & 31));
@@ -296,20 +296,20 @@ __device__ __forceinline__ Tq native(QueryResult const& r, bool continuous) {
   return {from_native(r.status, continuous), r.status == C8MI_QUERY_SUCCESS ? r.value : 0.};
 }
 
-__device__ inline Tq cq_mass(EmContext const& c, std::int32_t pid) {
+__device__ __forceinline__ Tq cq_mass(EmContext const& c, std::int32_t pid) {
   auto const* u = detail::find_utility(c.tables, pid);
   if (!u) return {kTsContParticle, 0.};
   if (!isfinite(u->particle_mass_mev) || !(u->particle_mass_mev > 0.)) return {kTsInvalidView, 0.};
   return {kTsSuccess, u->particle_mass_mev};
 }
 
-__device__ inline Tq cq_transport_cut(EmContext const& c, std::int32_t pid) {
+__device__ __forceinline__ Tq cq_transport_cut(EmContext const& c, std::int32_t pid) {
   double const cut = (pid == 13 || pid == -13) ? c.muon_cut_mev : c.em_cut_mev;
   if (!isfinite(cut) || !(cut > 0.)) return {kTsInvalidView, 0.};
   return {kTsSuccess, cut};
 }
 
-__device__ inline Tq cq_minimum_energy(EmContext const& c, std::int32_t pid) {
+__device__ __forceinline__ Tq cq_minimum_energy(EmContext const& c, std::int32_t pid) {
   auto const* u = detail::find_utility(c.tables, pid);
   if (!u) return {kTsContParticle, 0.};
   Tq const cut = cq_transport_cut(c, pid);
@@ -322,7 +322,7 @@ __device__ inline Tq cq_minimum_energy(EmContext const& c, std::int32_t pid) {
   return {kTsSuccess, minimum};
 }
 
-__device__ inline Tq cq_energy_after_loss(EmContext const& c, std::int32_t pid, double e0, double x) {
+__device__ __forceinline__ Tq cq_energy_after_loss(EmContext const& c, std::int32_t pid, double e0, double x) {
   if (!isfinite(x) || x < 0.) return {kTsNonFinite, 0.};
   Tq const initial = native(range(c.tables, pid, e0), true);
   if (initial.status != kTsSuccess) return initial;
@@ -334,7 +334,7 @@ __device__ __forceinline__ bool pk_finite(double v) {
   return v == v && v <= 1.7976931348623157e308 && v >= -1.7976931348623157e308;
 }
 
-__device__ inline bool decode_pair_split(double e, double s, double& rho) {
+__device__ __forceinline__ bool decode_pair_split(double e, double s, double& rho) {
   if (!pk_finite(e) || !pk_finite(s) || !(e > 2. * kElectronMassGeV)) return false;
   double const tol = 64. * 2.2204460492503131e-16;
   if (s < -tol || s > 1. + tol) return false;
@@ -345,7 +345,7 @@ __device__ inline bool decode_pair_split(double e, double s, double& rho) {
   return pk_finite(rho) && rho >= lower && rho <= 1. - lower;
 }
 
-__device__ inline bool encode_pair_split(double e, double rho, double& s) {
+__device__ __forceinline__ bool encode_pair_split(double e, double rho, double& s) {
   if (!pk_finite(e) || !pk_finite(rho) || !(e > 2. * kElectronMassGeV)) return false;
   double const lower = kElectronMassGeV / e;
   double const width = 1. - 2. * lower;
@@ -357,7 +357,7 @@ __device__ inline bool encode_pair_split(double e, double rho, double& s) {
   return pk_finite(s) && s >= 0. && s <= 1.;
 }
 
-__device__ inline Tq dq_loss_fraction(EmContext const& c, std::int32_t pid, std::int32_t process,
+__device__ __forceinline__ Tq dq_loss_fraction(EmContext const& c, std::int32_t pid, std::int32_t process,
                                       std::uint64_t comp, double e, double q) {
   if (process == kPhotonPairFinalState && pid == 22) {
     Tq r = native(loss_fraction(c.tables, pid, kPhotonPair, comp, e, q, true), false);
@@ -377,7 +377,7 @@ struct Selection {
   bool selected;
 };
 
-__device__ inline Selection select_column(EmContext const& c, std::int32_t pid, double e, double u) {
+__device__ __forceinline__ Selection select_column(EmContext const& c, std::int32_t pid, double e, double u) {
   Selection r{kTsNonFinite, 0, 0, 0., false};
   if (!isfinite(u) || u < 0. || u >= 1.) return r;
   Tq const total = native(selection_total_rate(c.tables, pid, e), false);
@@ -473,7 +473,7 @@ struct Fallback {
   int reason;
 };
 
-__device__ inline void select_interaction(EmContext const& c, Particle const& p, Interaction& in,
+__device__ __forceinline__ void select_interaction(EmContext const& c, Particle const& p, Interaction& in,
                                           Fallback& fb) {
   in.particle = p;
   in.deferred_reason = -1;
@@ -570,7 +570,7 @@ __device__ __forceinline__ double bounded_cosine(double v) {
   return -1. > m ? -1. : m;
 }
 
-__device__ inline double sauter_cosine(double e, double u) {
+__device__ __forceinline__ double sauter_cosine(double e, double u) {
   double const m2 = kElectronMassGeV * kElectronMassGeV;
   double const t = e * e - m2;
   double const p = sqrt(0. > t ? 0. : t);
@@ -580,7 +580,7 @@ __device__ inline double sauter_cosine(double e, double u) {
   return bounded_cosine(1. - omc);
 }
 
-__device__ inline bool compton_cosines(double e, double v, double& pc, double& ec) {
+__device__ __forceinline__ bool compton_cosines(double e, double v, double& pc, double& ec) {
   double const scattered = e * (1. - v);
   double const pe2 = 2. * v * e * kElectronMassGeV + v * v * e * e;
   if (!isfinite(e) || !(e > 0.) || !isfinite(v) || !(v > 0.) || !(v < 1.) || !(scattered > 0.) ||
@@ -596,7 +596,7 @@ __device__ inline bool compton_cosines(double e, double v, double& pc, double& e
   return true;
 }
 
-__device__ inline bool photoelectric_energy(c8mi_photon_pair_lpm const& s, std::uint64_t comp,
+__device__ __forceinline__ bool photoelectric_energy(c8mi_photon_pair_lpm const& s, std::uint64_t comp,
                                             double e, double& total, double& fraction) {
   if (!isfinite(e) || !(e > 0.) || s.component_count == 0 || s.component_count > 16 ||
       !isfinite(s.fine_structure_constant) || !(s.fine_structure_constant > 0.))
@@ -618,7 +618,7 @@ __device__ inline bool photoelectric_energy(c8mi_photon_pair_lpm const& s, std::
          fraction <= 1.;
 }
 
-__device__ inline void deflect(double const in[3], double cosine, double azimuth, double out[3],
+__device__ __forceinline__ void deflect(double const in[3], double cosine, double azimuth, double out[3],
                                bool bound) {
   double const t = in[0] * in[0] + in[1] * in[1];
   double const tr = sqrt(0. > t ? 0. : t);
@@ -650,7 +650,7 @@ struct Thin {
   double w1, w2;
 };
 
-__device__ inline Thin em_thinning(c8mi_em_thinning const& c, bool enabled, double pe, double pw,
+__device__ __forceinline__ Thin em_thinning(c8mi_em_thinning const& c, bool enabled, double pe, double pw,
                                    double e1, double e2, double u1, double u2) {
   if (!isfinite(pe) || !(pe > 0.) || !isfinite(pw) || pw < 0. || !isfinite(e1) || !(e1 > 0.) ||
       !isfinite(e2) || !(e2 > 0.) || !isfinite(u1) || !(u1 > 0.) || !(u1 < 1.) || !isfinite(u2) ||
@@ -696,7 +696,7 @@ __device__ inline Thin em_thinning(c8mi_em_thinning const& c, bool enabled, doub
   return o;
 }
 
-__device__ inline bool two_child_thinning(EmContext const& c, Particle const& parent,
+__device__ __forceinline__ bool two_child_thinning(EmContext const& c, Particle const& parent,
                                           std::int32_t process, double e1, double e2, bool lepton,
                                           Thin& t) {
   double const u1 = uniform(c, parent, static_cast<std::uint32_t>(process), 0x100);
@@ -707,7 +707,7 @@ __device__ inline bool two_child_thinning(EmContext const& c, Particle const& pa
 }
 
 // LPM tails shared by the photon-pair and bremsstrahlung suppressions.
-__device__ inline void lpm_functions(double sp, double s1, double gamma, double& xi, double& fi,
+__device__ __forceinline__ void lpm_functions(double sp, double s1, double gamma, double& xi, double& fi,
                                      double& g) {
   constexpr double Pi = 3.141592653589793238462643383279502884;
   double const h = log(sp) / log(s1);
@@ -731,7 +731,7 @@ __device__ inline void lpm_functions(double sp, double s1, double gamma, double&
   }
 }
 
-__device__ inline int pair_lpm(c8mi_photon_pair_lpm const& s, std::uint64_t comp, double e, double x,
+__device__ __forceinline__ int pair_lpm(c8mi_photon_pair_lpm const& s, std::uint64_t comp, double e, double x,
                                double rho, double& prob) {
   if (s.component_count == 0 || s.component_count > 16 || !isfinite(s.baseline_mass_density_g_per_cm3) ||
       !(s.baseline_mass_density_g_per_cm3 > 0.) || !isfinite(s.molecular_density_per_cm3) ||
@@ -770,7 +770,7 @@ __device__ inline int pair_lpm(c8mi_photon_pair_lpm const& s, std::uint64_t comp
   return 0;
 }
 
-__device__ inline int brems_lpm(c8mi_brems_lpm const& s, std::uint64_t comp, double e, double v,
+__device__ __forceinline__ int brems_lpm(c8mi_brems_lpm const& s, std::uint64_t comp, double e, double v,
                                 double rho, double& prob) {
   if (s.component_count == 0 || s.component_count > 16 || !isfinite(s.baseline_mass_density_g_per_cm3) ||
       !(s.baseline_mass_density_g_per_cm3 > 0.) || !isfinite(s.molecular_density_per_cm3) ||
@@ -818,11 +818,11 @@ __device__ inline int brems_lpm(c8mi_brems_lpm const& s, std::uint64_t comp, dou
   return 0;
 }
 
-__device__ inline double annihilation_term(double a1, double a2, double rho) {
+__device__ __forceinline__ double annihilation_term(double a1, double a2, double rho) {
   return a1 * log(rho) + a2 / rho - rho;
 }
 
-__device__ inline bool annihilation_rho(double e, double m, double u, double& rho) {
+__device__ __forceinline__ bool annihilation_rho(double e, double m, double u, double& rho) {
   double const gamma = e / m;
   if (!isfinite(gamma) || !(gamma > 1.) || !isfinite(u) || !(u > 0.) || !(u < 1.)) return false;
   double const aux = sqrt((gamma - 1.) / (gamma + 1.));

@@ -13,7 +13,7 @@ namespace c8mi::em::phys {
 
 // ---------------------------------------------------------------- photon ----
 
-__device__ inline void photon_step(EmContext const& c, Accumulator const& acc, Particle const& p,
+__device__ __forceinline__ void photon_step(EmContext const& c, Accumulator const& acc, Particle const& p,
                                    Outputs& o) {
   Interaction in;
   Fallback fb{false, 0};
@@ -244,7 +244,7 @@ __device__ inline void photon_step(EmContext const& c, Accumulator const& acc, P
 
 // ---------------------------------------------------------------- lepton ----
 
-__device__ inline void lepton_step(EmContext const& c, Accumulator const& acc, Particle const& p,
+__device__ __forceinline__ void lepton_step(EmContext const& c, Accumulator const& acc, Particle const& p,
                                    Outputs& o) {
   Interaction in;
   Fallback fb{false, 0};

@@ -149,6 +149,14 @@ int oracle_em_run_shower(const c8mi_em_config* config, const c8mi_native_table_s
                          int64_t* counters, c8mi_particle_state* observations,
                          c8mi_em_fallback* fallbacks, or_em_stats* stats);
 
+/* oracle_em_run_shower over shower_count single-primary showers on `threads`
+ * host threads (shower k: primaries[k], shower_id first_shower + k, first
+ * history 2); per-shower statistics only. */
+int oracle_em_run_batch(const c8mi_em_config* config, const c8mi_native_table_set* tables,
+                        const c8mi_environment* environment, uint64_t first_shower,
+                        const c8mi_particle_state* primaries, uint64_t shower_count,
+                        uint64_t maximum_fronts, int threads, or_em_stats* stats);
+
 int oracle_em_environment_query(const c8mi_environment* e, const double position[3],
                                 const double direction[3], int32_t node, int32_t* out_node,
                                 double* out_density);

@@ -2426,6 +2426,43 @@ int oracle_em_run_shower(const c8mi_em_config* config, const c8mi_native_table_s
   return 0;
 }
 
+/* Independent showers in parallel over host threads (the CPU baseline of the
+ * benchmark): shower k runs primaries[k] alone with shower_id first_shower + k,
+ * into thread-private profile and record buffers; only the statistics are
+ * returned. Each shower is the serial oracle_em_run_shower above. */
+int oracle_em_run_batch(const c8mi_em_config* config, const c8mi_native_table_set* tables,
+                        const c8mi_environment* environment, uint64_t first_shower,
+                        const c8mi_particle_state* primaries, uint64_t shower_count,
+                        uint64_t maximum_fronts, int threads, or_em_stats* stats) {
+  int failed = 0;
+  uint64_t const bins = config->profile_bins ? config->profile_bins : 1;
+#pragma omp parallel num_threads(threads > 0 ? threads : 1) reduction(| : failed)
+  {
+    int64_t* hist = calloc(4 * bins, sizeof(int64_t));
+    int64_t counters[OR_EM_COUNTERS];
+    c8mi_particle_state* obs = config->observation_capacity
+                                   ? malloc(config->observation_capacity * sizeof *obs) : NULL;
+    c8mi_em_fallback* fb = config->fallback_capacity
+                               ? malloc(config->fallback_capacity * sizeof *fb) : NULL;
+    if (!hist || (config->observation_capacity && !obs) || (config->fallback_capacity && !fb)) {
+      failed = 1;
+    } else {
+#pragma omp for schedule(dynamic, 1)
+      for (uint64_t k = 0; k < shower_count; ++k) {
+        memset(hist, 0, 4 * bins * sizeof(int64_t));
+        failed |= oracle_em_run_shower(config, tables, environment, first_shower + k, 2,
+                                       &primaries[k], 1, maximum_fronts, hist, hist + bins,
+                                       hist + 2 * bins, hist + 3 * bins, counters, obs, fb,
+                                       &stats[k]) != 0;
+      }
+    }
+    free(hist);
+    free(obs);
+    free(fb);
+  }
+  return failed ? -1 : 0;
+}
+
 /* ---- single-function probes for the tests ---- */
 
 int oracle_em_environment_query(const c8mi_environment* e, const double position[3],

@@ -211,6 +211,26 @@ def em_run_shower(config, tables, environment, primaries, shower_id=1, first_his
     return dict(hist=hist, counters=counters, stats=st, observations=obs[:n_obs], fallbacks=fbs[:n_fb])
 
 
+def em_run_batch(config, tables, environment, primaries, first_shower=1, threads=1,
+                 maximum_fronts=1 << 40):
+    """Independent single-primary showers on `threads` host threads; returns
+    the list of per-shower statistics dicts."""
+    from c8mi import PARTICLE_STATE_DTYPE
+    from c8mi.em import RunResult
+
+    lib = load()
+    cfg = config.struct if hasattr(config, "struct") else config
+    prim = np.ascontiguousarray(primaries, dtype=PARTICLE_STATE_DTYPE).reshape(-1)
+    stats = (RunResult * prim.size)()
+    env = environment.struct if hasattr(environment, "struct") else environment
+    rc = lib.oracle_em_run_batch(byref(cfg), byref(tables.struct), byref(env), c_uint64(first_shower),
+                                 prim.ctypes.data_as(c_void_p), c_uint64(prim.size),
+                                 c_uint64(maximum_fronts), ctypes.c_int(threads), stats)
+    if rc != 0:
+        raise RuntimeError("oracle_em_run_batch failed")
+    return [s.as_dict() for s in stats]
+
+
 def _env_ptr(environment):
     return byref(environment.struct if hasattr(environment, "struct") else environment)
 
