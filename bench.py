@@ -53,8 +53,11 @@ def parse():
                    help="run CoREAS and ZHS as two launches instead of one fused launch")
     p.add_argument("--exact", action="store_true",
                    help="reference FP64 operation order (bit-identical to the FP64 oracle)")
-    p.add_argument("--em-showers", type=int, default=10,
-                   help="timed 100 GeV showers of the EM section (0 disables it)")
+    p.add_argument("--em-showers", type=int, default=2048,
+                   help="timed 100 GeV showers of the EM pool (0 disables the EM section)")
+    p.add_argument("--em-plain-showers", type=int, default=20,
+                   help="timed 100 GeV showers of the one-at-a-time EM form")
+    p.add_argument("--em-slots", type=int, default=512, help="concurrent showers of the EM pool")
     p.add_argument("--em-only", action="store_true", help="run only the EM section")
     return p.parse_args()
 
@@ -125,7 +128,7 @@ def field_error(tracks, observers, propagation, mode, limit, exact):
     return errors
 
 
-EM_BYTES_PER_STEP = 232  # 116 B SoA state read + 116 B written per particle-step
+EM_BYTES_PER_STEP = 240  # 120 B SoA state (116 B particle + 4 B pool slot) read + written per step
 
 
 def em_cpu_baseline(tables, atmosphere, energy, target_seconds):
@@ -167,44 +170,56 @@ def em_measured_traffic(kernel):
     return data.get("traffic_bytes_per_step") if data.get("kernel") == kernel else None
 
 
-def em_section(showers, warmup, cpu_seconds, no_cpu_baseline, energy=100.0):
+def em_showers(energy, count, first_shower, atm=None, pid=22):
+    from c8mi import em
+
+    prim = em.particle(pid, energy, (0.0, 0.0, 112750.0), (0.0, 0.0, -1.0), history_id=1)
+    return [dict(primary=prim, shower_id=first_shower + k, arrival_s=0.0,
+                 footprint=em.footprint_estimate(energy)) for k in range(count)]
+
+
+def em_section(showers, plain_showers, slots, cpu_seconds, no_cpu_baseline, energy=100.0):
     """EM particle-steps/s on the plain 100 GeV vertical photon shower in the
     USStdBK atmosphere (1 MeV cut, no thinning, FP64), the whole cascade on the
-    device from the primary on; one step = one shower with a fresh seed."""
+    device from the primary on. Two forms: the pool (`showers` showers served
+    through `slots` concurrent slots, continuous batching; the throughput
+    figure) and one shower at a time (`plain_showers` showers, per-shower
+    latency)."""
     from c8mi import airtables, em
 
     tables = airtables.build_air_tables()
     atm = em.Atmosphere()
-    cfg = em.make_config(airtables, atmosphere=atm)
+    capacity = 1 << 23
+    cfg = em.make_config(airtables, atmosphere=atm, slots=slots, queue_capacity=capacity,
+                         observation_capacity=1 << 12, fallback_capacity=1 << 8)
     dev = em.EmBackend(cfg, tables, atm)
-
-    def shower(seed):
-        cfg.struct.random_seed = seed
-        prim = em.particle(22, energy, (0.0, 0.0, 112750.0), (0.0, 0.0, -1.0), history_id=1)
-        dev.begin_shower(seed, 2)
-        dev.enqueue(prim)
-        return dev.run(1 << 20)
-
-    for k in range(warmup):
-        shower(1 + k)
+    budget = 0.4 * capacity
+    dev.serve(em_showers(energy, min(slots, 64), 1), budget, realtime=False)  # warm-up
     dev.enable_kernel_timing(True)
-    steps = 0
-    latencies = []
     t0 = time.perf_counter()
-    for k in range(showers):
-        ts = time.perf_counter()
-        r = shower(100 + k)
-        latencies.append(time.perf_counter() - ts)
-        steps += r["photon_steps"] + r["lepton_steps"]
+    results, totals = dev.serve(em_showers(energy, showers, 1000), budget, realtime=False)
     elapsed = time.perf_counter() - t0
     timing = dev.kernel_time()
-    _, ledger = dev.download_profile()
     dev.close()
+    steps = totals["photon_steps"] + totals["lepton_steps"]
+    residual = max(abs(em.ledger_residual(r["ledger"])) for r in results)
+    pool_latency = [r["finished_s"] - r["admitted_s"] for r in results]
+
+    plain_cfg = em.make_config(airtables, atmosphere=atm, slots=1, queue_capacity=1 << 20,
+                               observation_capacity=1 << 14, fallback_capacity=1 << 10)
+    plain = em.EmBackend(plain_cfg, tables, atm)
+    plain.serve(em_showers(energy, 2, 1), 1e18, realtime=False)  # warm-up
+    t1 = time.perf_counter()
+    plain_results, plain_totals = plain.serve(em_showers(energy, plain_showers, 5000), 1e18, realtime=False)
+    plain_elapsed = time.perf_counter() - t1
+    plain.close()
+    plain_steps = plain_totals["photon_steps"] + plain_totals["lepton_steps"]
+    plain_latency = [r["finished_s"] - r["admitted_s"] for r in plain_results]
+
     kind = max(("photon_step", "lepton_step"), key=lambda k: timing[k][0])
-    step_ms, launches = timing[kind]
+    step_kernel_ms = timing["photon_step"][0] + timing["lepton_step"][0]
     total_kernel_ms = sum(v[0] for v in timing.values())
     # algorithmic bytes of all particle-steps over the summed step-kernel time
-    step_kernel_ms = timing["photon_step"][0] + timing["lepton_step"][0]
     achieved = steps * EM_BYTES_PER_STEP / (step_kernel_ms * 1e-3) / 1e9
     kernel_name = "step_kernel<photon>" if kind == "photon_step" else "step_kernel<lepton>"
     out = {
@@ -212,15 +227,21 @@ def em_section(showers, warmup, cpu_seconds, no_cpu_baseline, energy=100.0):
         "value": steps / elapsed,
         "unit": "particle-steps/s",
         "showers": showers,
-        "warmup": warmup,
+        "slots": slots,
         "particle_steps": steps,
         "ms_per_shower": elapsed * 1e3 / showers,
-        "latency_ms": {"p50": float(np.percentile(latencies, 50) * 1e3),
-                       "p99": float(np.percentile(latencies, 99) * 1e3)},
+        "latency_ms": {"p50": float(np.percentile(pool_latency, 50) * 1e3),
+                       "p99": float(np.percentile(pool_latency, 99) * 1e3)},
+        "fronts": totals["photon_fronts"] + totals["lepton_fronts"],
+        "max_front": totals["max_front"],
+        "one_at_a_time": {"value": plain_steps / plain_elapsed, "showers": plain_showers,
+                          "latency_ms": {"p50": float(np.percentile(plain_latency, 50) * 1e3),
+                                         "p99": float(np.percentile(plain_latency, 99) * 1e3)}},
         "dtype": "f64",
         "config": {"workload": f"EM plain: {energy:g} GeV vertical photon at 112.75 km, USStdBK 5-layer "
                                "spherical air, observation level 0 m, 1 MeV cut, no thinning, no radio, "
-                               "closed-form v7 air tables, whole cascade on the device"},
+                               "closed-form v7 air tables, whole cascade on the device; "
+                               f"{showers} showers through {slots} pool slots"},
         "roofline": {
             "bound": "hbm",
             "achieved": achieved,
@@ -230,10 +251,11 @@ def em_section(showers, warmup, cpu_seconds, no_cpu_baseline, energy=100.0):
             "traffic": em_measured_traffic(kernel_name),
             "kernel": kernel_name,
             "bytes_per_step": EM_BYTES_PER_STEP,
-            "kernel_ms_per_shower": {k: v[0] / showers for k, v in timing.items()},
+            "kernel_ms": {k: v[0] for k, v in timing.items()},
+            "launches": {k: v[1] for k, v in timing.items()},
             "device_busy_fraction": total_kernel_ms * 1e-3 / elapsed,
         },
-        "ledger_residual_gev": em.ledger_residual(ledger),
+        "ledger_residual_gev_max": residual,
     }
     if not no_cpu_baseline:
         base = em_cpu_baseline(tables, atm, energy, cpu_seconds)
@@ -272,7 +294,8 @@ def main():
         print("bench.py: no HIP device visible", file=sys.stderr)
         return 1
     if args.em_only:
-        em_result = em_section(max(args.em_showers, 1), args.warmup, args.cpu_seconds, args.no_cpu_baseline)
+        em_result = em_section(max(args.em_showers, 1), args.em_plain_showers, args.em_slots, args.cpu_seconds,
+                               args.no_cpu_baseline)
         em_result.update({"n_gpus": 1, "steps": em_result["showers"], "higher_is_better": True,
                           "scaling": "weak", "vs_baseline": None, "data": "synthetic",
                           "ms_per_step": em_result["ms_per_shower"]})
@@ -370,7 +393,8 @@ def main():
         result["cpu_baseline"] = base
         result["speedup_vs_cpu_baseline"] = value / base["value"]
     if args.em_showers > 0:
-        result["em"] = em_section(args.em_showers, 2, args.cpu_seconds, args.no_cpu_baseline)
+        result["em"] = em_section(args.em_showers, args.em_plain_showers, args.em_slots, args.cpu_seconds,
+                                  args.no_cpu_baseline)
     print(json.dumps(result), flush=True)
     return 0
 

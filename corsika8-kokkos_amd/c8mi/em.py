@@ -97,7 +97,37 @@ class EmConfig(ctypes.Structure):
                 ("profile_weight_scale", c_double), ("profile_energy_scale", c_double),
                 ("axis_start_m", c_double * 3), ("axis_direction", c_double * 3),
                 ("axis_step_m", c_double), ("axis_grammage_g_per_cm2", POINTER(c_double)),
-                ("axis_support_count", c_uint64)]
+                ("axis_support_count", c_uint64), ("shower_slots", c_uint32), ("reserved_", c_uint32)]
+
+
+class ShowerAxis(ctypes.Structure):
+    _fields_ = [("start_m", c_double * 3), ("direction", c_double * 3), ("step_m", c_double),
+                ("grammage_g_per_cm2", POINTER(c_double)), ("support_count", c_uint64)]
+
+
+class SlotState(ctypes.Structure):
+    _fields_ = [(k, c_uint64) for k in (
+        "shower_id", "pending_photons", "pending_leptons", "next_history_id", "photon_fronts",
+        "lepton_fronts", "photon_steps", "lepton_steps", "photon_interactions", "lepton_interactions",
+        "secondaries", "observations", "fallbacks")] + [("active", c_uint32), ("pad_", c_uint32)]
+
+    def as_dict(self):
+        return {k: int(getattr(self, k)) for k, _ in self._fields_ if k != "pad_"}
+
+
+class _ParticleState(ctypes.Structure):
+    # c8mi_particle_state as an opaque 128-byte, 16-aligned record
+    _fields_ = [("words", c_uint64 * 16)]
+
+
+class Request(ctypes.Structure):
+    _fields_ = [("primary", _ParticleState), ("shower_id", c_uint64), ("first_history_id", c_uint64),
+                ("arrival_s", c_double), ("footprint_particles", c_double), ("axis", ShowerAxis),
+                ("align_", c_uint64)]  # the C struct is 16-byte aligned (240 bytes)
+
+
+class ServeOptions(ctypes.Structure):
+    _fields_ = [("footprint_budget_particles", c_double), ("realtime", c_uint32), ("pad_", c_uint32)]
 
 
 class RunResult(ctypes.Structure):
@@ -130,6 +160,18 @@ class Ledger(ctypes.Structure):
         for k, _ in self._fields_:
             v = getattr(self, k)
             out[k] = list(v) if hasattr(v, "__len__") else v
+        return out
+
+
+class RequestResult(ctypes.Structure):
+    _fields_ = [("admitted_s", c_double), ("finished_s", c_double)] + [(k, c_uint64) for k in (
+        "photon_steps", "lepton_steps", "photon_interactions", "lepton_interactions", "secondaries",
+        "next_history_id", "observations", "fallbacks")] + [("slot", c_uint32), ("pad_", c_uint32),
+                                                             ("ledger", Ledger)]
+
+    def as_dict(self):
+        out = {k: getattr(self, k) for k, _ in self._fields_ if k not in ("pad_", "ledger")}
+        out["ledger"] = self.ledger.as_dict()
         return out
 
 
@@ -262,9 +304,10 @@ def particle(pid, energy_gev, position, direction, history_id=1, weight=1.0, nod
 def make_config(tables_module, seed=1, cut_mev=1.0, start=(0.0, 0.0, 112750.0),
                 direction=(0.0, 0.0, -1.0), atmosphere=None, bins=128, bin_width=10.0,
                 queue_capacity=1 << 22, observation_capacity=1 << 20, fallback_capacity=1 << 16,
-                thinning=None, crossing_mode=0):
+                thinning=None, crossing_mode=0, slots=1):
     """EmConfig + the arrays it points to (kept alive on the returned object)."""
     cfg = EmConfig()
+    cfg.shower_slots = slots
     cfg.random_seed = seed
     cfg.em_transport_cut_mev = cut_mev
     cfg.muon_transport_cut_mev = cut_mev
@@ -316,6 +359,30 @@ def make_config(tables_module, seed=1, cut_mev=1.0, start=(0.0, 0.0, 112750.0),
     h = Holder()
     h.struct, h.axis_table, h.atmosphere = cfg, table, atmosphere
     return h
+
+
+def footprint_estimate(energy_gev):
+    """Admission estimate of a shower's peak front (particles of one species in
+    flight) at a 1 MeV cut: about 1.5x the largest front of oracle showers
+    (52, 305, 2282, 17147 at 1, 10, 100, 1000 GeV)."""
+    return 60.0 * float(energy_gev) ** 0.875
+
+
+def shower_axis(atmosphere, start, direction, step_m=10.0):
+    """ShowerAxis of a straight shower axis (the projection table is kept alive
+    on the returned object as .table)."""
+    d = np.asarray(direction, dtype=np.float64)
+    d = d / np.linalg.norm(d)
+    table = atmosphere.axis_table(start, d, step_m=step_m)
+    ax = ShowerAxis()
+    for a in range(3):
+        ax.start_m[a] = start[a]
+        ax.direction[a] = d[a]
+    ax.step_m = step_m
+    ax.grammage_g_per_cm2 = table.ctypes.data_as(POINTER(c_double))
+    ax.support_count = table.size
+    ax.table = table
+    return ax
 
 
 def decode_counters(counters, energy_scale):
@@ -420,6 +487,84 @@ class EmBackend:
         _check(library().c8mi_em_download_fallbacks(self.handle, out.ctypes.data_as(c_void_p),
                                                     c_uint64(out.size), byref(n)))
         return out[:min(n.value, out.size)], n.value
+
+    # ---- shower pool (c8mi_em_slot_*) ----
+
+    def slot_begin(self, slot, shower_id, first_history_id=2, axis=None):
+        _check(library().c8mi_em_slot_begin(self.handle, c_uint32(slot), c_uint64(shower_id),
+                                            c_uint64(first_history_id), byref(axis) if axis is not None else None))
+
+    def slot_enqueue(self, slot, particles):
+        p = np.ascontiguousarray(particles, dtype=PARTICLE_STATE_DTYPE).reshape(-1)
+        _check(library().c8mi_em_slot_enqueue(self.handle, c_uint32(slot), p.ctypes.data_as(c_void_p),
+                                              c_uint64(p.size)))
+
+    def run_rounds(self, maximum_rounds=1 << 40) -> dict:
+        r = RunResult()
+        _check(library().c8mi_em_run_rounds(self.handle, c_uint64(maximum_rounds), byref(r)))
+        return r.as_dict()
+
+    def slot_state(self, slot) -> dict:
+        st = SlotState()
+        _check(library().c8mi_em_slot_state_get(self.handle, c_uint32(slot), byref(st)))
+        return st.as_dict()
+
+    def slot_download_profile(self, slot):
+        hist = {k: np.zeros(self.bins) for k in ("photons", "electrons", "positrons", "energy_loss")}
+        led = Ledger()
+        dp = lambda a: a.ctypes.data_as(POINTER(c_double))
+        _check(library().c8mi_em_slot_download_profile(self.handle, c_uint32(slot), dp(hist["photons"]),
+                                                       dp(hist["electrons"]), dp(hist["positrons"]),
+                                                       dp(hist["energy_loss"]), byref(led)))
+        return hist, led.as_dict()
+
+    def slot_observations(self, slot):
+        out = np.zeros(self.observation_capacity, dtype=PARTICLE_STATE_DTYPE)
+        n = c_uint64()
+        _check(library().c8mi_em_slot_download_observations(self.handle, c_uint32(slot),
+                                                            out.ctypes.data_as(c_void_p), c_uint64(out.size),
+                                                            byref(n)))
+        return out[:min(n.value, out.size)], n.value
+
+    def slot_fallbacks(self, slot):
+        out = np.zeros(self.fallback_capacity, dtype=FALLBACK_DTYPE)
+        n = c_uint64()
+        _check(library().c8mi_em_slot_download_fallbacks(self.handle, c_uint32(slot),
+                                                         out.ctypes.data_as(c_void_p), c_uint64(out.size),
+                                                         byref(n)))
+        return out[:min(n.value, out.size)], n.value
+
+    def slot_pending(self, slot):
+        a, b = c_uint64(), c_uint64()
+        _check(library().c8mi_em_slot_pending(self.handle, c_uint32(slot), byref(a), byref(b)))
+        return a.value, b.value
+
+    def serve(self, requests, footprint_budget, realtime=True):
+        """requests: list of dicts with primary (particle), shower_id, arrival_s,
+        footprint, axis (ShowerAxis or None); returns (per-request dicts, totals)."""
+        n = len(requests)
+        arr = (Request * n)()
+        keep = []
+        for k, r in enumerate(requests):
+            p = np.ascontiguousarray(r["primary"], dtype=PARTICLE_STATE_DTYPE).reshape(-1)[:1]
+            ctypes.memmove(ctypes.addressof(arr[k].primary), p.ctypes.data, 128)
+            arr[k].shower_id = r["shower_id"]
+            arr[k].first_history_id = r.get("first_history_id", 2)
+            arr[k].arrival_s = r.get("arrival_s", 0.0)
+            arr[k].footprint_particles = r.get("footprint", 0.0)
+            if r.get("axis") is not None:
+                arr[k].axis = r["axis"]
+                keep.append(r["axis"])
+        opts = ServeOptions()
+        opts.footprint_budget_particles = footprint_budget
+        opts.realtime = 1 if realtime else 0
+        results = (RequestResult * n)()
+        totals = RunResult()
+        _check(library().c8mi_em_serve(self.handle, arr, c_uint64(n), byref(opts), results, byref(totals)))
+        return [results[k].as_dict() for k in range(n)], totals.as_dict()
+
+    def slot_end(self, slot):
+        _check(library().c8mi_em_slot_end(self.handle, c_uint32(slot)))
 
     def enable_kernel_timing(self, enable=True):
         _check(library().c8mi_em_enable_kernel_timing(self.handle, c_int(1 if enable else 0)))

@@ -183,6 +183,91 @@ int c8mi_em_download_fallbacks(c8mi_em* em, c8mi_em_fallback* out, uint64_t capa
   });
 }
 
+int c8mi_em_slot_begin(c8mi_em* em, uint32_t slot, uint64_t shower_id, uint64_t first_history_id,
+                       c8mi_em_shower_axis const* axis) {
+  return guarded([&] {
+    require_handle(em);
+    em->backend.slot_begin(slot, shower_id, first_history_id, axis);
+  });
+}
+
+int c8mi_em_slot_enqueue(c8mi_em* em, uint32_t slot, c8mi_particle_state const* particles, uint64_t count) {
+  return guarded([&] {
+    require_handle(em);
+    em->backend.slot_enqueue(slot, particles, count);
+  });
+}
+
+int c8mi_em_run_rounds(c8mi_em* em, uint64_t maximum_rounds, c8mi_em_run_result* out) {
+  return guarded([&] {
+    require_handle(em);
+    if (!out) throw std::invalid_argument("null result");
+    em->backend.run(0, maximum_rounds, *out);
+  });
+}
+
+int c8mi_em_slot_state_get(c8mi_em const* em, uint32_t slot, c8mi_em_slot_state* out) {
+  return guarded([&] {
+    require_handle(em);
+    if (!out) throw std::invalid_argument("null result");
+    em->backend.slot_state(slot, *out);
+  });
+}
+
+int c8mi_em_slot_download_profile(c8mi_em* em, uint32_t slot, double* photons, double* electrons,
+                                  double* positrons, double* energy_loss, c8mi_em_ledger* ledger) {
+  return guarded([&] {
+    require_handle(em);
+    em->backend.slot_download_profile(slot, photons, electrons, positrons, energy_loss, ledger);
+  });
+}
+
+int c8mi_em_slot_download_observations(c8mi_em* em, uint32_t slot, c8mi_particle_state* out,
+                                       uint64_t capacity, uint64_t* count) {
+  return guarded([&] {
+    require_handle(em);
+    uint64_t const n = em->backend.slot_download_records(slot, 0, out, sizeof(c8mi_particle_state), capacity);
+    if (count) *count = n;
+  });
+}
+
+int c8mi_em_slot_download_fallbacks(c8mi_em* em, uint32_t slot, c8mi_em_fallback* out, uint64_t capacity,
+                                    uint64_t* count) {
+  return guarded([&] {
+    require_handle(em);
+    uint64_t const n = em->backend.slot_download_records(slot, 1, out, sizeof(c8mi_em_fallback), capacity);
+    if (count) *count = n;
+  });
+}
+
+int c8mi_em_slot_end(c8mi_em* em, uint32_t slot) {
+  return guarded([&] {
+    require_handle(em);
+    em->backend.slot_end(slot);
+  });
+}
+
+int c8mi_em_slot_pending(c8mi_em const* em, uint32_t slot, uint64_t* photons, uint64_t* leptons) {
+  return guarded([&] {
+    require_handle(em);
+    if (!photons || !leptons) throw std::invalid_argument("null argument");
+    uint64_t p = 0, l = 0;
+    em->backend.slot_pending(slot, p, l);
+    *photons = p;
+    *leptons = l;
+  });
+}
+
+int c8mi_em_serve(c8mi_em* em, c8mi_em_request const* requests, uint64_t count,
+                  c8mi_em_serve_options const* options, c8mi_em_request_result* results,
+                  c8mi_em_run_result* totals) {
+  return guarded([&] {
+    require_handle(em);
+    if (!options || !totals) throw std::invalid_argument("null argument");
+    em->backend.serve(requests, count, *options, results, *totals);
+  });
+}
+
 int c8mi_em_enable_kernel_timing(c8mi_em* em, int enable) {
   return guarded([&] {
     require_handle(em);

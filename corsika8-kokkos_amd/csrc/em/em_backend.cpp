@@ -1,6 +1,9 @@
 #include "em/em_backend.hpp"
 
+#include <algorithm>
+#include <chrono>
 #include <cmath>
+#include <thread>
 #include <cstring>
 #include <stdexcept>
 #include <string>
@@ -73,7 +76,17 @@ void validate_config(c8mi_em_config const& c) {
   require(c.axis_grammage_g_per_cm2 != nullptr && c.axis_support_count > 0, "profile projection table");
   require(std::isfinite(c.axis_step_m) && c.axis_step_m > 0., "projection step");
   require(c.profile_crossing_mode <= C8MI_PROFILE_CROSSING_FORWARD, "profile crossing mode");
+  require(c.shower_slots <= kMaxShowerSlots, "shower slots above the supported maximum (1024)");
 }
+
+void validate_axis(c8mi_em_shower_axis const& a) {
+  require(a.grammage_g_per_cm2 != nullptr && a.support_count > 0, "shower axis projection table");
+  require(std::isfinite(a.step_m) && a.step_m > 0., "shower axis projection step");
+  for (int k = 0; k < 3; ++k)
+    require(std::isfinite(a.start_m[k]) && std::isfinite(a.direction[k]), "shower axis start / direction");
+}
+
+std::uint64_t nonzero(std::uint64_t n) { return n ? n : 1; }
 
 }  // namespace
 
@@ -82,7 +95,7 @@ QueueStorage::QueueStorage(std::uint64_t capacity) : capacity_(capacity) {
   std::size_t const n = capacity;
   auto round = [](std::size_t b) { return (b + 255u) & ~std::size_t{255u}; };
   std::size_t const b4 = round(4 * n), b8 = round(8 * n);
-  storage_.allocate(5 * b4 + 12 * b8);
+  storage_.allocate(6 * b4 + 12 * b8);
   unsigned char* p = storage_.data();
   auto take4 = [&](auto*& field) {
     field = reinterpret_cast<std::remove_reference_t<decltype(field)>>(p);
@@ -97,6 +110,7 @@ QueueStorage::QueueStorage(std::uint64_t capacity) : capacity_(capacity) {
   take4(view_.generation);
   take4(view_.reserved);
   take4(view_.node);
+  take4(view_.shower);
   take8(view_.energy);
   take8(view_.x);
   take8(view_.y);
@@ -113,9 +127,11 @@ QueueStorage::QueueStorage(std::uint64_t capacity) : capacity_(capacity) {
 
 EmBackend::EmBackend(c8mi_em_config const& config, c8mi_native_table_set const& tables,
                      c8mi_environment const& environment, hipStream_t stream)
-    : config_(config), tables_(tables), host_totals_(1) {
+    : config_(config), tables_(tables) {
   validate_config(config);
   validate_environment(environment);
+  slots_ = config.shower_slots ? config.shower_slots : 1u;
+  config_.shower_slots = slots_;
   if (stream) {
     stream_ = stream;
   } else {
@@ -126,13 +142,37 @@ EmBackend::EmBackend(c8mi_em_config const& config, c8mi_native_table_set const& 
   upload(children_, environment.children, environment.child_index_count);
   upload(models_, environment.models, environment.model_count);
   upload(axis_, config.axis_grammage_g_per_cm2, config.axis_support_count);
-  counters_.allocate(kCounters);
-  profile_.allocate(4u * (config.profile_bins ? config.profile_bins : 1u));
-  observations_.allocate(config.observation_capacity ? config.observation_capacity : 1);
-  fallbacks_.allocate(config.fallback_capacity ? config.fallback_capacity : 1);
-  record_counts_.allocate(2);
+  std::uint64_t const bins = nonzero(config.profile_bins);
+  counters_.allocate(slots_ * static_cast<std::uint64_t>(kCounters));
+  profile_.allocate(slots_ * 4u * bins);
+  observations_.allocate(slots_ * nonzero(config.observation_capacity));
+  fallbacks_.allocate(slots_ * nonzero(config.fallback_capacity));
+  record_counts_.allocate(2u * slots_);
+  history_.allocate(slots_);
+  stats_.allocate(slots_ * static_cast<std::uint64_t>(kSlotStats));
+  shower_totals_.allocate(static_cast<std::size_t>(kMaxShowerSlots) * kMaxRuns * kShowerFields);
+  adjust_.allocate(slots_ * static_cast<std::uint64_t>(kMaxRuns) * 3u);
+  report_.allocate(kReportHeader + 2u * slots_);
+  host_report_ = PinnedBuffer<unsigned long long>(kReportHeader + 2u * slots_);
+  for (auto* b : {&history_, &stats_, &shower_totals_, &record_counts_})
+    C8MI_HIP(hipMemset(b->data(), 0, b->bytes()));
+  C8MI_HIP(hipMemset(counters_.data(), 0, counters_.bytes()));
+  C8MI_HIP(hipMemset(profile_.data(), 0, profile_.bytes()));
   for (auto& species : queues_)
     for (auto& q : species) q = std::make_unique<QueueStorage>(config.queue_capacity);
+
+  ShowerSlot base{};
+  for (int a = 0; a < 3; ++a) {
+    base.axis_start[a] = config.axis_start_m[a];
+    base.axis_direction[a] = config.axis_direction[a];
+  }
+  base.axis_step = config.axis_step_m;
+  base.axis_grammage = axis_.data();
+  base.axis_support_count = config.axis_support_count;
+  host_showers_.assign(slots_, base);
+  showers_.allocate(slots_);
+  C8MI_HIP(hipMemcpy(showers_.data(), host_showers_.data(), slots_ * sizeof(ShowerSlot), hipMemcpyHostToDevice));
+  slot_host_ = std::vector<SlotHost>(slots_);
 
   EmContext& h = host_context_;
   h.tables = tables_.view();
@@ -145,12 +185,9 @@ EmBackend::EmBackend(c8mi_em_config const& config, c8mi_native_table_set const& 
   for (int a = 0; a < 3; ++a) {
     h.observation_point[a] = environment.observation_point_m[a];
     h.observation_normal[a] = environment.observation_normal[a];
-    h.axis_start[a] = config.axis_start_m[a];
-    h.axis_direction[a] = config.axis_direction[a];
   }
   h.maximum_magnetic_deflection = environment.maximum_magnetic_deflection_rad;
   h.seed = config.random_seed;
-  h.shower = 0;
   h.em_cut_mev = config.em_transport_cut_mev;
   h.muon_cut_mev = config.muon_transport_cut_mev;
   h.pair_lpm = config.photon_pair_lpm;
@@ -162,9 +199,8 @@ EmBackend::EmBackend(c8mi_em_config const& config, c8mi_native_table_set const& 
   h.energy_loss_threshold = config.profile_energy_loss_threshold_g_per_cm2;
   h.weight_scale = config.profile_weight_scale;
   h.energy_scale = config.profile_energy_scale;
-  h.axis_step = config.axis_step_m;
-  h.axis_grammage = axis_.data();
-  h.axis_support_count = config.axis_support_count;
+  h.showers = showers_.data();
+  h.shower_slots = slots_;
   h.observations = observations_.data();
   h.observation_capacity = config.observation_capacity;
   h.fallbacks = fallbacks_.data();
@@ -175,7 +211,7 @@ EmBackend::EmBackend(c8mi_em_config const& config, c8mi_native_table_set const& 
   context_.allocate(1);
   C8MI_HIP(hipMemcpy(context_.data(), &h, sizeof(EmContext), hipMemcpyHostToDevice));
   for (auto& e : events_) C8MI_HIP(hipEventCreate(&e));
-  begin_shower(0, 1);
+  reset_pool();
 }
 
 EmBackend::~EmBackend() {
@@ -185,17 +221,92 @@ EmBackend::~EmBackend() {
   if (own_stream_ && stream_) (void)hipStreamDestroy(stream_);
 }
 
+void EmBackend::check_slot(std::uint32_t slot) const {
+  if (slot >= slots_) throw std::invalid_argument("em: slot " + std::to_string(slot) + " outside the pool");
+}
+
+void EmBackend::update_active_range() {
+  active_range_ = 0;
+  for (std::uint32_t s = 0; s < slots_; ++s)
+    if (slot_host_[s].active) active_range_ = s + 1;
+}
+
 void EmBackend::begin_shower(std::uint64_t shower_id, std::uint64_t first_history_id) {
-  host_context_.shower = shower_id;
-  C8MI_HIP(hipMemcpyAsync(context_.data(), &host_context_, sizeof(EmContext), hipMemcpyHostToDevice, stream_));
-  C8MI_HIP(hipMemsetAsync(counters_.data(), 0, counters_.bytes(), stream_));
-  C8MI_HIP(hipMemsetAsync(profile_.data(), 0, profile_.bytes(), stream_));
-  C8MI_HIP(hipMemsetAsync(record_counts_.data(), 0, record_counts_.bytes(), stream_));
+  reset_pool();
+  slot_begin(0, shower_id, first_history_id, nullptr);
+}
+
+// Every slot idle, every queue empty.
+void EmBackend::reset_pool() {
   C8MI_HIP(hipStreamSynchronize(stream_));
-  count_[0] = count_[1] = 0;
-  current_[0] = current_[1] = 0;
-  next_history_ = first_history_id;
-  injected_ = 0;
+  for (auto& h : slot_host_) {
+    h.active = false;
+    h.pending[0] = h.pending[1] = 0;
+    h.staged[0] = h.staged[1] = 0;
+  }
+  for (int s = 0; s < 2; ++s) {
+    count_[s] = 0;
+    current_[s] = 0;
+    runs_[s].clear();
+    staged_[s].clear();
+  }
+  update_active_range();
+}
+
+void EmBackend::slot_begin(std::uint32_t slot, std::uint64_t shower_id, std::uint64_t first_history_id,
+                           c8mi_em_shower_axis const* axis) {
+  check_slot(slot);
+  SlotHost& h = slot_host_[slot];
+  if (h.pending[0] || h.pending[1] || h.staged[0] || h.staged[1])
+    throw std::logic_error("em: slot " + std::to_string(slot) + " still has pending particles");
+  ShowerSlot s = host_showers_[slot];
+  if (axis) {
+    validate_axis(*axis);
+    upload(h.axis, axis->grammage_g_per_cm2, axis->support_count);
+    for (int a = 0; a < 3; ++a) {
+      s.axis_start[a] = axis->start_m[a];
+      s.axis_direction[a] = axis->direction[a];
+    }
+    s.axis_step = axis->step_m;
+    s.axis_grammage = h.axis.data();
+    s.axis_support_count = axis->support_count;
+  } else {
+    for (int a = 0; a < 3; ++a) {
+      s.axis_start[a] = config_.axis_start_m[a];
+      s.axis_direction[a] = config_.axis_direction[a];
+    }
+    s.axis_step = config_.axis_step_m;
+    s.axis_grammage = axis_.data();
+    s.axis_support_count = config_.axis_support_count;
+  }
+  s.key = shower_id;
+  host_showers_[slot] = s;
+  std::uint64_t const bins = nonzero(config_.profile_bins);
+  C8MI_HIP(hipMemcpyAsync(showers_.data() + slot, &host_showers_[slot], sizeof(ShowerSlot),
+                          hipMemcpyHostToDevice, stream_));
+  C8MI_HIP(hipMemsetAsync(profile_.data() + slot * 4u * bins, 0, 4u * bins * sizeof(long long), stream_));
+  C8MI_HIP(hipMemsetAsync(counters_.data() + static_cast<std::uint64_t>(slot) * kCounters, 0,
+                          kCounters * sizeof(long long), stream_));
+  C8MI_HIP(hipMemsetAsync(record_counts_.data() + 2u * slot, 0, 2 * sizeof(unsigned long long), stream_));
+  C8MI_HIP(hipMemsetAsync(stats_.data() + static_cast<std::uint64_t>(slot) * kSlotStats, 0,
+                          kSlotStats * sizeof(unsigned long long), stream_));
+  unsigned long long const first = first_history_id;
+  C8MI_HIP(hipMemcpyAsync(history_.data() + slot, &first, sizeof first, hipMemcpyHostToDevice, stream_));
+  C8MI_HIP(hipStreamSynchronize(stream_));
+  h.active = true;
+  h.shower_id = shower_id;
+  h.fronts[0] = h.fronts[1] = 0;
+  h.injected = 0;
+  update_active_range();
+}
+
+void EmBackend::slot_end(std::uint32_t slot) {
+  check_slot(slot);
+  SlotHost& h = slot_host_[slot];
+  if (h.pending[0] || h.pending[1] || h.staged[0] || h.staged[1])
+    throw std::logic_error("em: slot " + std::to_string(slot) + " still has pending particles");
+  h.active = false;
+  update_active_range();
 }
 
 bool EmBackend::can_transport(c8mi_particle_state const& p) const {
@@ -207,77 +318,108 @@ bool EmBackend::can_transport(c8mi_particle_state const& p) const {
   return true;
 }
 
-void EmBackend::enqueue(c8mi_particle_state const* particles, std::uint64_t count) {
+void EmBackend::slot_enqueue(std::uint32_t slot, c8mi_particle_state const* particles, std::uint64_t count) {
+  check_slot(slot);
   if (count == 0) return;
   if (!particles) throw std::invalid_argument("em: null particle array");
-  std::vector<c8mi_particle_state> split[2];
+  SlotHost& h = slot_host_[slot];
+  if (!h.active) throw std::logic_error("em: slot " + std::to_string(slot) + " has no shower begun");
+  for (std::uint64_t i = 0; i < count; ++i)
+    if (!can_transport(particles[i]))
+      throw std::invalid_argument("em: particle " + std::to_string(i) + " (pid " +
+                                  std::to_string(particles[i].pid) +
+                                  ") cannot be transported by the device path");
   for (std::uint64_t i = 0; i < count; ++i) {
     c8mi_particle_state const& p = particles[i];
-    if (!can_transport(p))
-      throw std::invalid_argument("em: particle " + std::to_string(i) + " (pid " + std::to_string(p.pid) +
-                                  ") cannot be transported by the device path");
-    split[p.pid == 22 ? 0 : 1].push_back(p);
+    int const s = p.pid == 22 ? 0 : 1;
+    staged_[s].push_back(Staged{p, slot});
+    ++h.staged[s];
+    h.injected += static_cast<long double>(std::llrint(p.energy_gev * p.weight * config_.profile_energy_scale));
   }
-  for (int s = 0; s < 2; ++s) {
-    auto const& v = split[s];
-    if (v.empty()) continue;
-    if (count_[s] + v.size() > config_.queue_capacity)
-      throw std::length_error("em: enqueue exceeds the device queue capacity");
-    std::size_t const n = v.size();
-    std::vector<std::int32_t> i32[3];
-    std::vector<std::uint32_t> u32[2];
-    std::vector<double> f64[9];
-    std::vector<std::uint64_t> u64[3];
-    for (auto& a : i32) a.resize(n);
-    for (auto& a : u32) a.resize(n);
-    for (auto& a : f64) a.resize(n);
-    for (auto& a : u64) a.resize(n);
-    for (std::size_t k = 0; k < n; ++k) {
-      auto const& p = v[k];
-      i32[0][k] = p.pid;
-      i32[1][k] = p.medium_id;
-      i32[2][k] = p.environment_node;
-      u32[0][k] = p.generation;
-      u32[1][k] = p.reserved;
-      f64[0][k] = p.energy_gev;
-      f64[1][k] = p.position_m[0];
-      f64[2][k] = p.position_m[1];
-      f64[3][k] = p.position_m[2];
-      f64[4][k] = p.direction[0];
-      f64[5][k] = p.direction[1];
-      f64[6][k] = p.direction[2];
-      f64[7][k] = p.time_s;
-      f64[8][k] = p.weight;
-      u64[0][k] = p.history_id;
-      u64[1][k] = p.parent_history_id;
-      u64[2][k] = p.step_id;
-      double const scaled = p.energy_gev * p.weight * config_.profile_energy_scale;
-      injected_ += static_cast<long double>(std::llrint(scaled));
-    }
-    ParticleQueue const q = queues_[s][current_[s]]->view();
-    std::uint64_t const at = count_[s];
-    auto put = [&](void* dst, void const* src, std::size_t elem) {
-      C8MI_HIP(hipMemcpy(static_cast<unsigned char*>(dst) + at * elem, src, n * elem, hipMemcpyHostToDevice));
-    };
-    put(q.pid, i32[0].data(), 4);
-    put(q.medium, i32[1].data(), 4);
-    put(q.node, i32[2].data(), 4);
-    put(q.generation, u32[0].data(), 4);
-    put(q.reserved, u32[1].data(), 4);
-    put(q.energy, f64[0].data(), 8);
-    put(q.x, f64[1].data(), 8);
-    put(q.y, f64[2].data(), 8);
-    put(q.z, f64[3].data(), 8);
-    put(q.dx, f64[4].data(), 8);
-    put(q.dy, f64[5].data(), 8);
-    put(q.dz, f64[6].data(), 8);
-    put(q.time, f64[7].data(), 8);
-    put(q.weight, f64[8].data(), 8);
-    put(q.history, u64[0].data(), 8);
-    put(q.parent, u64[1].data(), 8);
-    put(q.step, u64[2].data(), 8);
-    count_[s] += n;
+}
+
+void EmBackend::append_run(int species, std::uint64_t end) {
+  auto& r = runs_[species];
+  if (!r.empty() && r.back() == end) return;
+  std::uint32_t active = 0;
+  for (auto const& h : slot_host_) active += h.active ? 1u : 0u;
+  if (active <= 1) {  // one shower: its queue is a single run in any order of appends
+    r.assign(1, end);
+    return;
   }
+  if (r.size() >= static_cast<std::size_t>(kMaxRuns))
+    throw std::logic_error("em: too many queued runs for a pool of showers; alternate photon and lepton fronts");
+  r.push_back(end);
+}
+
+// Uploads the host-staged particles of one species as one run, sorted by slot
+// (stable, so each slot keeps its enqueue order).
+void EmBackend::flush_staged(int species) {
+  auto& v = staged_[species];
+  if (v.empty()) return;
+  std::stable_sort(v.begin(), v.end(), [](Staged const& a, Staged const& b) { return a.slot < b.slot; });
+  std::size_t const n = v.size();
+  if (count_[species] + n > config_.queue_capacity)
+    throw std::length_error("em: enqueue exceeds the device queue capacity");
+  std::vector<std::int32_t> i32[3];
+  std::vector<std::uint32_t> u32[3];
+  std::vector<double> f64[9];
+  std::vector<std::uint64_t> u64[3];
+  for (auto& a : i32) a.resize(n);
+  for (auto& a : u32) a.resize(n);
+  for (auto& a : f64) a.resize(n);
+  for (auto& a : u64) a.resize(n);
+  for (std::size_t k = 0; k < n; ++k) {
+    auto const& p = v[k].particle;
+    i32[0][k] = p.pid;
+    i32[1][k] = p.medium_id;
+    i32[2][k] = p.environment_node;
+    u32[0][k] = p.generation;
+    u32[1][k] = p.reserved;
+    u32[2][k] = v[k].slot;
+    f64[0][k] = p.energy_gev;
+    f64[1][k] = p.position_m[0];
+    f64[2][k] = p.position_m[1];
+    f64[3][k] = p.position_m[2];
+    f64[4][k] = p.direction[0];
+    f64[5][k] = p.direction[1];
+    f64[6][k] = p.direction[2];
+    f64[7][k] = p.time_s;
+    f64[8][k] = p.weight;
+    u64[0][k] = p.history_id;
+    u64[1][k] = p.parent_history_id;
+    u64[2][k] = p.step_id;
+  }
+  ParticleQueue const q = queues_[species][current_[species]]->view();
+  std::uint64_t const at = count_[species];
+  auto put = [&](void* dst, void const* src, std::size_t elem) {
+    C8MI_HIP(hipMemcpy(static_cast<unsigned char*>(dst) + at * elem, src, n * elem, hipMemcpyHostToDevice));
+  };
+  put(q.pid, i32[0].data(), 4);
+  put(q.medium, i32[1].data(), 4);
+  put(q.node, i32[2].data(), 4);
+  put(q.generation, u32[0].data(), 4);
+  put(q.reserved, u32[1].data(), 4);
+  put(q.shower, u32[2].data(), 4);
+  put(q.energy, f64[0].data(), 8);
+  put(q.x, f64[1].data(), 8);
+  put(q.y, f64[2].data(), 8);
+  put(q.z, f64[3].data(), 8);
+  put(q.dx, f64[4].data(), 8);
+  put(q.dy, f64[5].data(), 8);
+  put(q.dz, f64[6].data(), 8);
+  put(q.time, f64[7].data(), 8);
+  put(q.weight, f64[8].data(), 8);
+  put(q.history, u64[0].data(), 8);
+  put(q.parent, u64[1].data(), 8);
+  put(q.step, u64[2].data(), 8);
+  for (auto const& e : v) {
+    --slot_host_[e.slot].staged[species];
+    ++slot_host_[e.slot].pending[species];
+  }
+  count_[species] += n;
+  append_run(species, count_[species]);
+  v.clear();
 }
 
 void EmBackend::resize_scratch(std::uint64_t n) {
@@ -288,33 +430,53 @@ void EmBackend::resize_scratch(std::uint64_t n) {
   packed_.allocate(cap);
   block_counts_.allocate(block_count_bytes(cap));
   offsets_.allocate(offset_bytes(cap) / sizeof(unsigned long long));
-  if (!totals_.data()) totals_.allocate(1);
   scratch_capacity_ = cap;
 }
 
 void EmBackend::front(bool photon, c8mi_em_run_result& out) {
   int const s = photon ? 0 : 1, o = 1 - s;
+  // staged particles of both species join their queues now, ahead of the
+  // secondaries this front appends (the order of a serial run)
+  flush_staged(0);
+  flush_staged(1);
   std::uint64_t const n = count_[s];
   if (n == 0) return;
   std::uint64_t const cap = config_.queue_capacity;
   if (2 * n > cap || count_[o] + 2 * n > cap)
     throw std::length_error("em: a front's outputs could exceed the device queue capacity");
   resize_scratch(n);
-  ParticleQueue const in = queues_[s][current_[s]]->view();
+  FrontArgs f{};
+  f.in = queues_[s][current_[s]]->view();
+  f.count = n;
+  f.runs = static_cast<std::uint32_t>(runs_[s].size());
+  for (std::uint32_t r = 0; r < f.runs; ++r) f.run_end[r] = runs_[s][r];
+  f.slots = active_range_;
+  f.staging = staging_.data();
+  f.packed = packed_.data();
+  f.block_counts = block_counts_.data();
+  f.offsets = offsets_.data();
+  f.shower_totals = shower_totals_.data();
+  f.adjust = adjust_.data();
+  f.history = history_.data();
+  f.stats = stats_.data();
+  f.report = report_.data();
   ParticleQueue const next = queues_[s][1 - current_[s]]->view();
   ParticleQueue const other = queues_[o][current_[o]]->view();
+  for (std::uint32_t k = 0; k < active_range_; ++k)
+    if (slot_host_[k].pending[s]) ++slot_host_[k].fronts[s];
   if (timing_) C8MI_HIP(hipEventRecord(events_[0], stream_));
-  launch_step(photon, context_.data(), config_.profile_bins, in, n, staging_.data(), packed_.data(),
-              block_counts_.data(), stream_);
+  launch_step(photon, context_.data(), config_.profile_bins, f, stream_);
   if (timing_) C8MI_HIP(hipEventRecord(events_[1], stream_));
-  launch_scan(block_counts_.data(), n, offsets_.data(), totals_.data(), stream_);
+  launch_scan(photon, f, stream_);
   if (timing_) C8MI_HIP(hipEventRecord(events_[2], stream_));
-  launch_scatter(packed_.data(), n, staging_.data(), offsets_.data(), next, other, count_[o], next_history_,
-                 stream_);
+  launch_scatter(f, next, other, count_[o], stream_);
   if (timing_) C8MI_HIP(hipEventRecord(events_[3], stream_));
-  C8MI_HIP(hipMemcpyAsync(host_totals_.data(), totals_.data(), sizeof(FrontTotals), hipMemcpyDeviceToHost, stream_));
+  C8MI_HIP(hipMemcpyAsync(host_report_.data(), report_.data(),
+                          (kReportHeader + 2u * active_range_) * sizeof(unsigned long long),
+                          hipMemcpyDeviceToHost, stream_));
   C8MI_HIP(hipStreamSynchronize(stream_));
-  FrontTotals const t = *host_totals_.data();
+  FrontTotals t;
+  std::memcpy(&t, host_report_.data(), sizeof t);
   if (timing_) {
     float a = 0.f, b = 0.f, c = 0.f;
     C8MI_HIP(hipEventElapsedTime(&a, events_[0], events_[1]));
@@ -327,10 +489,17 @@ void EmBackend::front(bool photon, c8mi_em_run_result& out) {
     ms_[3] += c;
     launches_[3] += 1;
   }
+  unsigned long long const* per_slot = host_report_.data() + kReportHeader;
+  for (std::uint32_t k = 0; k < active_range_; ++k) {
+    slot_host_[k].pending[s] = per_slot[2 * k];
+    slot_host_[k].pending[o] += per_slot[2 * k + 1];
+  }
   count_[s] = t.next;
   current_[s] ^= 1;
+  runs_[s].clear();
+  if (t.next) runs_[s].push_back(t.next);
   count_[o] += t.other;
-  next_history_ += t.children;
+  if (t.other) append_run(o, count_[o]);
   if (photon) {
     out.photon_fronts += 1;
     out.photon_steps += t.steps;
@@ -346,37 +515,93 @@ void EmBackend::front(bool photon, c8mi_em_run_result& out) {
 
 void EmBackend::run(int species, std::uint64_t maximum_fronts, c8mi_em_run_result& out) {
   std::memset(&out, 0, sizeof out);
+  auto has = [&](int s) { return count_[s] + staged_[s].size() > 0; };
   for (std::uint64_t round = 0; round < maximum_fronts; ++round) {
     if (species == 0) {
-      if (count_[0] == 0 && count_[1] == 0) break;
-      if (count_[0]) front(true, out);
-      if (count_[1]) front(false, out);
+      if (!has(0) && !has(1)) break;
+      if (has(0)) front(true, out);
+      if (has(1)) front(false, out);
     } else {
       int const s = species == 1 ? 0 : 1;
-      if (count_[s] == 0) break;
+      if (!has(s)) break;
       front(s == 0, out);
     }
   }
-  unsigned long long records[2];
-  C8MI_HIP(hipMemcpy(records, record_counts_.data(), sizeof records, hipMemcpyDeviceToHost));
-  out.observations = records[0];
-  out.fallbacks = records[1];
-  out.pending_photons = count_[0];
-  out.pending_leptons = count_[1];
-  out.next_history_id = next_history_;
+  std::vector<unsigned long long> records(2u * slots_);
+  C8MI_HIP(hipMemcpy(records.data(), record_counts_.data(), records.size() * sizeof(unsigned long long),
+                     hipMemcpyDeviceToHost));
+  for (std::uint32_t k = 0; k < active_range_; ++k) {
+    out.observations += records[2 * k];
+    out.fallbacks += records[2 * k + 1];
+  }
+  pending(out.pending_photons, out.pending_leptons);
+  unsigned long long next_history = 0;
+  C8MI_HIP(hipMemcpy(&next_history, history_.data(), sizeof next_history, hipMemcpyDeviceToHost));
+  out.next_history_id = next_history;
 }
 
 void EmBackend::pending(std::uint64_t& photons, std::uint64_t& leptons) const {
-  photons = count_[0];
-  leptons = count_[1];
+  photons = count_[0] + staged_[0].size();
+  leptons = count_[1] + staged_[1].size();
 }
 
-void EmBackend::download_profile(double* photons, double* electrons, double* positrons, double* energy_loss,
-                                 c8mi_em_ledger* ledger) {
+void EmBackend::slot_state(std::uint32_t slot, c8mi_em_slot_state& out) const {
+  check_slot(slot);
+  std::memset(&out, 0, sizeof out);
+  SlotHost const& h = slot_host_[slot];
+  unsigned long long st[kSlotStats], records[2], history = 0;
+  C8MI_HIP(hipStreamSynchronize(stream_));
+  C8MI_HIP(hipMemcpy(st, stats_.data() + static_cast<std::uint64_t>(slot) * kSlotStats, sizeof st,
+                     hipMemcpyDeviceToHost));
+  C8MI_HIP(hipMemcpy(records, record_counts_.data() + 2u * slot, sizeof records, hipMemcpyDeviceToHost));
+  C8MI_HIP(hipMemcpy(&history, history_.data() + slot, sizeof history, hipMemcpyDeviceToHost));
+  out.shower_id = h.shower_id;
+  out.pending_photons = h.pending[0] + h.staged[0];
+  out.pending_leptons = h.pending[1] + h.staged[1];
+  out.next_history_id = history;
+  out.photon_fronts = h.fronts[0];
+  out.lepton_fronts = h.fronts[1];
+  out.photon_steps = st[0];
+  out.lepton_steps = st[1];
+  out.photon_interactions = st[2];
+  out.lepton_interactions = st[3];
+  out.secondaries = st[4];
+  out.observations = records[0];
+  out.fallbacks = records[1];
+  out.active = h.active ? 1u : 0u;
+}
+
+// Weighted energy of a slot still queued: device queues (photons first, then
+// leptons, in queue order), then host-staged particles.
+long long EmBackend::pending_energy(std::uint32_t slot) {
+  double pending = 0.;
+  for (int s = 0; s < 2; ++s) {
+    std::uint64_t const n = count_[s];
+    if (n == 0 || slot_host_[slot].pending[s] == 0) continue;
+    ParticleQueue const q = queues_[s][current_[s]]->view();
+    std::vector<double> e(n), w(n);
+    std::vector<std::uint32_t> owner(n);
+    C8MI_HIP(hipMemcpy(e.data(), q.energy, n * 8, hipMemcpyDeviceToHost));
+    C8MI_HIP(hipMemcpy(w.data(), q.weight, n * 8, hipMemcpyDeviceToHost));
+    C8MI_HIP(hipMemcpy(owner.data(), q.shower, n * 4, hipMemcpyDeviceToHost));
+    for (std::uint64_t k = 0; k < n; ++k)
+      if (owner[k] == slot) pending += e[k] * w[k];
+  }
+  for (int s = 0; s < 2; ++s)
+    for (auto const& p : staged_[s])
+      if (p.slot == slot) pending += p.particle.energy_gev * p.particle.weight;
+  return std::llrint(pending * config_.profile_energy_scale);
+}
+
+void EmBackend::slot_download_profile(std::uint32_t slot, double* photons, double* electrons, double* positrons,
+                                      double* energy_loss, c8mi_em_ledger* ledger) {
+  check_slot(slot);
   C8MI_HIP(hipStreamSynchronize(stream_));
   std::uint32_t const bins = config_.profile_bins;
-  std::vector<long long> hist(4u * (bins ? bins : 1u));
-  C8MI_HIP(hipMemcpy(hist.data(), profile_.data(), hist.size() * sizeof(long long), hipMemcpyDeviceToHost));
+  std::uint64_t const stride = 4u * nonzero(bins);
+  std::vector<long long> hist(stride);
+  C8MI_HIP(hipMemcpy(hist.data(), profile_.data() + slot * stride, stride * sizeof(long long),
+                     hipMemcpyDeviceToHost));
   double* outs[4] = {photons, electrons, positrons, energy_loss};
   for (int h = 0; h < 4; ++h) {
     if (!outs[h]) continue;
@@ -385,9 +610,10 @@ void EmBackend::download_profile(double* photons, double* electrons, double* pos
   }
   if (!ledger) return;
   std::vector<long long> c(kCounters);
-  C8MI_HIP(hipMemcpy(c.data(), counters_.data(), kCounters * sizeof(long long), hipMemcpyDeviceToHost));
-  c[kInjected] += static_cast<long long>(injected_);
-  sync_pending_energy_to(c.data());
+  C8MI_HIP(hipMemcpy(c.data(), counters_.data() + static_cast<std::uint64_t>(slot) * kCounters,
+                     kCounters * sizeof(long long), hipMemcpyDeviceToHost));
+  c[kInjected] += static_cast<long long>(slot_host_[slot].injected);
+  c[kPending] += pending_energy(slot);
   double const es = config_.profile_energy_scale;
   double* energy = &ledger->injected_energy;
   for (int k = 0; k < kEnergyTerms; ++k) energy[k] = static_cast<double>(c[k]) / es;
@@ -403,41 +629,118 @@ void EmBackend::download_profile(double* photons, double* electrons, double* pos
   for (int k = 0; k < 32; ++k) ledger->fallback_reasons[k] = static_cast<std::uint64_t>(c[kFallbackReason0 + k]);
 }
 
-// Weighted energy still queued (photons first, then leptons, in queue order).
-void EmBackend::sync_pending_energy_to(long long* counters) {
-  double pending = 0.;
-  for (int s = 0; s < 2; ++s) {
-    std::uint64_t const n = count_[s];
-    if (n == 0) continue;
-    ParticleQueue const q = queues_[s][current_[s]]->view();
-    std::vector<double> e(n), w(n);
-    C8MI_HIP(hipMemcpy(e.data(), q.energy, n * 8, hipMemcpyDeviceToHost));
-    C8MI_HIP(hipMemcpy(w.data(), q.weight, n * 8, hipMemcpyDeviceToHost));
-    for (std::uint64_t k = 0; k < n; ++k) pending += e[k] * w[k];
+std::uint64_t EmBackend::slot_download_records(std::uint32_t slot, int kind, void* out, std::size_t record_bytes,
+                                               std::uint64_t capacity) {
+  check_slot(slot);
+  C8MI_HIP(hipStreamSynchronize(stream_));
+  unsigned long long n = 0;
+  C8MI_HIP(hipMemcpy(&n, record_counts_.data() + 2u * slot + kind, sizeof n, hipMemcpyDeviceToHost));
+  std::uint64_t const region = kind == 0 ? config_.observation_capacity : config_.fallback_capacity;
+  std::uint64_t const stored = n < region ? n : region;
+  std::uint64_t const copy = stored < capacity ? stored : capacity;
+  if (out && copy) {
+    unsigned char const* base = kind == 0 ? reinterpret_cast<unsigned char const*>(observations_.data())
+                                          : reinterpret_cast<unsigned char const*>(fallbacks_.data());
+    C8MI_HIP(hipMemcpy(out, base + slot * region * record_bytes, copy * record_bytes, hipMemcpyDeviceToHost));
   }
-  counters[kPending] += std::llrint(pending * config_.profile_energy_scale);
-}
-
-std::uint64_t EmBackend::download_observations(c8mi_particle_state* out, std::uint64_t capacity) {
-  C8MI_HIP(hipStreamSynchronize(stream_));
-  unsigned long long n = 0;
-  C8MI_HIP(hipMemcpy(&n, record_counts_.data(), sizeof n, hipMemcpyDeviceToHost));
-  std::uint64_t const stored = n < config_.observation_capacity ? n : config_.observation_capacity;
-  std::uint64_t const copy = stored < capacity ? stored : capacity;
-  if (out && copy)
-    C8MI_HIP(hipMemcpy(out, observations_.data(), copy * sizeof(c8mi_particle_state), hipMemcpyDeviceToHost));
   return n;
 }
 
-std::uint64_t EmBackend::download_fallbacks(c8mi_em_fallback* out, std::uint64_t capacity) {
-  C8MI_HIP(hipStreamSynchronize(stream_));
-  unsigned long long n = 0;
-  C8MI_HIP(hipMemcpy(&n, record_counts_.data() + 1, sizeof n, hipMemcpyDeviceToHost));
-  std::uint64_t const stored = n < config_.fallback_capacity ? n : config_.fallback_capacity;
-  std::uint64_t const copy = stored < capacity ? stored : capacity;
-  if (out && copy)
-    C8MI_HIP(hipMemcpy(out, fallbacks_.data(), copy * sizeof(c8mi_em_fallback), hipMemcpyDeviceToHost));
-  return n;
+void EmBackend::slot_pending(std::uint32_t slot, std::uint64_t& photons, std::uint64_t& leptons) const {
+  check_slot(slot);
+  SlotHost const& h = slot_host_[slot];
+  photons = h.pending[0] + h.staged[0];
+  leptons = h.pending[1] + h.staged[1];
+}
+
+void EmBackend::serve(c8mi_em_request const* requests, std::uint64_t count, c8mi_em_serve_options const& options,
+                      c8mi_em_request_result* results, c8mi_em_run_result& totals) {
+  std::memset(&totals, 0, sizeof totals);
+  if (count == 0) return;
+  if (!requests || !results) throw std::invalid_argument("em: null request or result array");
+  for (std::uint32_t slot = 0; slot < slots_; ++slot) {
+    SlotHost const& h = slot_host_[slot];
+    if (h.pending[0] || h.pending[1] || h.staged[0] || h.staged[1])
+      throw std::logic_error("em: serve needs a pool without pending particles");
+  }
+  for (std::uint32_t slot = 0; slot < slots_; ++slot) slot_host_[slot].active = false;
+  update_active_range();
+  for (std::uint64_t k = 0; k < count; ++k) {
+    c8mi_em_request const& r = requests[k];
+    if (!std::isfinite(r.arrival_s) || (k && r.arrival_s < requests[k - 1].arrival_s))
+      throw std::invalid_argument("em: request arrival times must be finite and non-decreasing");
+    if (!std::isfinite(r.footprint_particles) || r.footprint_particles < 0.)
+      throw std::invalid_argument("em: request footprint");
+    if (!can_transport(r.primary))
+      throw std::invalid_argument("em: request " + std::to_string(k) + " has a primary the device cannot transport");
+    if (r.axis.grammage_g_per_cm2) validate_axis(r.axis);
+    std::memset(&results[k], 0, sizeof results[k]);
+  }
+  using clock = std::chrono::steady_clock;
+  auto const t0 = clock::now();
+  auto elapsed = [&] { return std::chrono::duration<double>(clock::now() - t0).count(); };
+  std::vector<std::uint32_t> free_slots;
+  for (std::uint32_t s = slots_; s-- > 0;) free_slots.push_back(s);
+  std::vector<std::int64_t> owner(slots_, -1);
+  std::uint64_t next = 0, active = 0;
+  double footprint = 0.;
+  while (next < count || active) {
+    double now = elapsed();
+    while (next < count && !free_slots.empty() && (!options.realtime || requests[next].arrival_s <= now) &&
+           (active == 0 || footprint + requests[next].footprint_particles <= options.footprint_budget_particles)) {
+      c8mi_em_request const& r = requests[next];
+      std::uint32_t const slot = free_slots.back();
+      free_slots.pop_back();
+      slot_begin(slot, r.shower_id, r.first_history_id, r.axis.grammage_g_per_cm2 ? &r.axis : nullptr);
+      slot_enqueue(slot, &r.primary, 1);
+      owner[slot] = static_cast<std::int64_t>(next);
+      results[next].admitted_s = elapsed();
+      results[next].slot = slot;
+      footprint += r.footprint_particles;
+      ++active;
+      ++next;
+    }
+    if (active == 0) {  // idle until the next arrival
+      double const wait = requests[next].arrival_s - elapsed();
+      if (wait > 0.) std::this_thread::sleep_for(std::chrono::duration<double>(wait));
+      continue;
+    }
+    c8mi_em_run_result round{};
+    run(0, 1, round);
+    totals.photon_fronts += round.photon_fronts;
+    totals.lepton_fronts += round.lepton_fronts;
+    totals.photon_steps += round.photon_steps;
+    totals.lepton_steps += round.lepton_steps;
+    totals.photon_interactions += round.photon_interactions;
+    totals.lepton_interactions += round.lepton_interactions;
+    totals.secondaries += round.secondaries;
+    totals.max_front = std::max(totals.max_front, round.max_front);
+    for (std::uint32_t slot = 0; slot < slots_; ++slot) {
+      if (owner[slot] < 0) continue;
+      SlotHost const& h = slot_host_[slot];
+      if (h.pending[0] || h.pending[1] || h.staged[0] || h.staged[1]) continue;
+      c8mi_em_request_result& out = results[owner[slot]];
+      out.finished_s = elapsed();
+      c8mi_em_slot_state st;
+      slot_state(slot, st);
+      out.photon_steps = st.photon_steps;
+      out.lepton_steps = st.lepton_steps;
+      out.photon_interactions = st.photon_interactions;
+      out.lepton_interactions = st.lepton_interactions;
+      out.secondaries = st.secondaries;
+      out.next_history_id = st.next_history_id;
+      out.observations = st.observations;
+      out.fallbacks = st.fallbacks;
+      slot_download_profile(slot, nullptr, nullptr, nullptr, nullptr, &out.ledger);
+      totals.observations += st.observations;
+      totals.fallbacks += st.fallbacks;
+      slot_end(slot);
+      footprint -= requests[owner[slot]].footprint_particles;
+      owner[slot] = -1;
+      free_slots.push_back(slot);
+      --active;
+    }
+  }
 }
 
 void EmBackend::enable_timing(bool enable) {

@@ -1,9 +1,12 @@
 // Host side of the EM backend: device storage of tables, environment and
-// context; per-species double-buffered particle queues; the front loop.
+// context; per-species double-buffered particle queues shared by a pool of
+// shower slots; the front loop.
 // Mirrors the reference's IEmBackend implementation
 // (corsika/kokkos/em/execution/IEmBackend.hpp:25-88,
 // corsika/detail/kokkos/em/execution/BackendInstanceImplementation.inl) with
-// one HIP stream per backend and one small device-to-host read per front.
+// one HIP stream per backend and one small device-to-host read per front. The
+// IEmBackend-shaped calls (begin_shower, enqueue, run, downloads) act on slot 0
+// of the pool; the slot_* calls keep several showers in the same fronts.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -16,6 +19,7 @@
 #include "c8mi/c8mi_em.h"
 #include "em/device_tables.hpp"
 #include "em/em_device.hpp"
+#include "em/em_front.hpp"
 #include "runtime.hpp"
 
 namespace c8mi::em {
@@ -40,25 +44,70 @@ class EmBackend {
   EmBackend(EmBackend const&) = delete;
   EmBackend& operator=(EmBackend const&) = delete;
 
+  // IEmBackend-shaped single-shower calls: the whole pool is reset and the
+  // shower runs in slot 0.
   void begin_shower(std::uint64_t shower_id, std::uint64_t first_history_id);
   bool can_transport(c8mi_particle_state const& p) const;
-  void enqueue(c8mi_particle_state const* particles, std::uint64_t count);
+  void enqueue(c8mi_particle_state const* particles, std::uint64_t count) { slot_enqueue(0, particles, count); }
   // species: 0 both (alternating), 1 photons only, 2 leptons only
   void run(int species, std::uint64_t maximum_fronts, c8mi_em_run_result& out);
   void pending(std::uint64_t& photons, std::uint64_t& leptons) const;
   void download_profile(double* photons, double* electrons, double* positrons, double* energy_loss,
-                        c8mi_em_ledger* ledger);
-  std::uint64_t download_observations(c8mi_particle_state* out, std::uint64_t capacity);
-  std::uint64_t download_fallbacks(c8mi_em_fallback* out, std::uint64_t capacity);
+                        c8mi_em_ledger* ledger) {
+    slot_download_profile(0, photons, electrons, positrons, energy_loss, ledger);
+  }
+  std::uint64_t download_observations(c8mi_particle_state* out, std::uint64_t capacity) {
+    return slot_download_records(0, 0, out, sizeof(c8mi_particle_state), capacity);
+  }
+  std::uint64_t download_fallbacks(c8mi_em_fallback* out, std::uint64_t capacity) {
+    return slot_download_records(0, 1, out, sizeof(c8mi_em_fallback), capacity);
+  }
+
+  // pool
+  std::uint32_t slots() const noexcept { return slots_; }
+  void slot_begin(std::uint32_t slot, std::uint64_t shower_id, std::uint64_t first_history_id,
+                  c8mi_em_shower_axis const* axis);
+  void slot_enqueue(std::uint32_t slot, c8mi_particle_state const* particles, std::uint64_t count);
+  void slot_state(std::uint32_t slot, c8mi_em_slot_state& out) const;
+  void slot_download_profile(std::uint32_t slot, double* photons, double* electrons, double* positrons,
+                             double* energy_loss, c8mi_em_ledger* ledger);
+  std::uint64_t slot_download_records(std::uint32_t slot, int kind, void* out, std::size_t record_bytes,
+                                      std::uint64_t capacity);
+  void slot_end(std::uint32_t slot);
+  void slot_pending(std::uint32_t slot, std::uint64_t& photons, std::uint64_t& leptons) const;
+  void serve(c8mi_em_request const* requests, std::uint64_t count, c8mi_em_serve_options const& options,
+             c8mi_em_request_result* results, c8mi_em_run_result& totals);
+
   void enable_timing(bool enable);
   void kernel_time(double ms[4], std::uint64_t launches[4]) const;
 
  private:
+  struct Staged {
+    c8mi_particle_state particle;
+    std::uint32_t slot;
+  };
+  struct SlotHost {
+    bool active = false;
+    std::uint64_t shower_id = 0;
+    std::uint64_t pending[2] = {0, 0};  // on the device
+    std::uint64_t staged[2] = {0, 0};   // on the host
+    std::uint64_t fronts[2] = {0, 0};
+    long double injected = 0;
+    DeviceBuffer<double> axis;          // own projection table, if any
+  };
+
+  void reset_pool();
   void front(bool photon, c8mi_em_run_result& out);
+  void flush_staged(int species);
+  void append_run(int species, std::uint64_t end);
   void resize_scratch(std::uint64_t n);
-  void sync_pending_energy_to(long long* counters);
+  void check_slot(std::uint32_t slot) const;
+  void update_active_range();
+  long long pending_energy(std::uint32_t slot);
 
   c8mi_em_config config_;
+  std::uint32_t slots_ = 1;
+  std::uint32_t active_range_ = 0;  // 1 + highest active slot
   hipStream_t stream_ = nullptr;
   bool own_stream_ = false;
   DeviceTables tables_;
@@ -68,24 +117,31 @@ class EmBackend {
   DeviceBuffer<double> axis_;
   DeviceBuffer<EmContext> context_;
   EmContext host_context_{};
+  DeviceBuffer<ShowerSlot> showers_;
+  std::vector<ShowerSlot> host_showers_;
+  std::vector<SlotHost> slot_host_;
   DeviceBuffer<long long> counters_;
   DeviceBuffer<long long> profile_;
   DeviceBuffer<c8mi_particle_state> observations_;
   DeviceBuffer<c8mi_em_fallback> fallbacks_;
   DeviceBuffer<unsigned long long> record_counts_;
+  DeviceBuffer<unsigned long long> history_;
+  DeviceBuffer<unsigned long long> stats_;
+  DeviceBuffer<unsigned long long> shower_totals_;
+  DeviceBuffer<unsigned long long> adjust_;
+  DeviceBuffer<unsigned long long> report_;
+  PinnedBuffer<unsigned long long> host_report_;
   // queues: [species][buffer]; species 0 photons, 1 leptons
   std::unique_ptr<QueueStorage> queues_[2][2];
   int current_[2] = {0, 0};
   std::uint64_t count_[2] = {0, 0};
-  std::uint64_t next_history_ = 0;
-  long double injected_ = 0;
+  std::vector<std::uint64_t> runs_[2];  // exclusive run ends in the current buffer
+  std::vector<Staged> staged_[2];
   // front scratch
   DeviceBuffer<c8mi_particle_state> staging_;
   DeviceBuffer<std::uint32_t> packed_;
   DeviceBuffer<unsigned char> block_counts_;
   DeviceBuffer<unsigned long long> offsets_;
-  DeviceBuffer<FrontTotals> totals_;
-  PinnedBuffer<FrontTotals> host_totals_;
   std::uint64_t scratch_capacity_ = 0;
   // timing
   bool timing_ = false;

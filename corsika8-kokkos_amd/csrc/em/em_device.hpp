@@ -48,6 +48,16 @@ inline constexpr std::uint32_t kMaxProfileBins = 1024;
 // Histogram order in the profile buffer.
 enum Histogram : int { kPhotonsHist = 0, kElectronsHist = 1, kPositronsHist = 2, kEnergyLossHist = 3 };
 
+// Per-slot shower parameters (one entry per pool slot, device memory).
+struct ShowerSlot {
+  std::uint64_t key;  // Philox showerId
+  std::uint64_t axis_support_count;
+  double axis_start[3];
+  double axis_direction[3];
+  double axis_step;
+  double const* axis_grammage;
+};
+
 struct EmContext {
   c8mi_native_table_set tables;  // device pointers
   // environment (device pointers)
@@ -63,7 +73,6 @@ struct EmContext {
   double maximum_magnetic_deflection;
   // physics
   std::uint64_t seed;
-  std::uint64_t shower;
   double em_cut_mev;
   double muon_cut_mev;
   c8mi_photon_pair_lpm pair_lpm;
@@ -76,23 +85,22 @@ struct EmContext {
   double energy_loss_threshold;
   double weight_scale;
   double energy_scale;
-  double axis_start[3];
-  double axis_direction[3];
-  double axis_step;
-  double const* axis_grammage;
-  std::uint64_t axis_support_count;
-  // outputs of records that are kept
-  c8mi_particle_state* observations;
+  // pool slots and their outputs
+  ShowerSlot const* showers;          // shower_slots entries
+  std::uint32_t shower_slots;
+  std::uint32_t pad1;
+  c8mi_particle_state* observations;  // observation_capacity per slot
   std::uint64_t observation_capacity;
-  c8mi_em_fallback* fallbacks;
+  c8mi_em_fallback* fallbacks;        // fallback_capacity per slot
   std::uint64_t fallback_capacity;
-  unsigned long long* record_counts;  // [0] observations, [1] fallbacks
-  long long* counters;                // kCounters
-  long long* profile;                 // 4 * profile_bins
+  unsigned long long* record_counts;  // per slot: [0] observations, [1] fallbacks
+  long long* counters;                // per slot: kCounters
+  long long* profile;                 // per slot: 4 * profile_bins
 };
 
 // Struct-of-arrays particle queue (ParticleSoA, corsika/kokkos/data/ParticleSoA.hpp:27-82):
-// 13 fields, 116 bytes per particle.
+// the 13 state fields (116 bytes) plus the pool slot of the particle's shower
+// (4 bytes); a queue holds up to a few runs, each sorted by slot.
 struct ParticleQueue {
   std::int32_t* pid;
   std::int32_t* medium;
@@ -111,6 +119,7 @@ struct ParticleQueue {
   std::uint64_t* parent;
   std::uint64_t* step;
   std::int32_t* node;
+  std::uint32_t* shower;
 };
 
 __device__ __forceinline__ c8mi_particle_state load_particle(ParticleQueue const& q,
@@ -165,11 +174,12 @@ __device__ __forceinline__ void store_particle(ParticleQueue const& q, std::uint
 enum SlotKind : std::uint32_t { kSlotNone = 0, kSlotNextContinue = 1, kSlotNextChild = 2, kSlotOtherChild = 3 };
 
 // Packed per-source counts: bits 0-1 next, 2-3 other, 4-5 children, 6-7 slot0
-// kind, 8-9 slot1 kind.
+// kind, 8-9 slot1 kind, 12-31 pool slot of the source's shower.
+inline constexpr std::uint32_t kMaxShowerSlots = 1024;
 __host__ __device__ __forceinline__ std::uint32_t pack_counts(std::uint32_t next, std::uint32_t other,
                                                               std::uint32_t child, std::uint32_t k0,
-                                                              std::uint32_t k1) {
-  return next | (other << 2) | (child << 4) | (k0 << 6) | (k1 << 8);
+                                                              std::uint32_t k1, std::uint32_t shower) {
+  return next | (other << 2) | (child << 4) | (k0 << 6) | (k1 << 8) | (shower << 12);
 }
 
 // Front totals written by the scan kernel and read back by the host.

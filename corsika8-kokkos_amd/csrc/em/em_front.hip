@@ -4,17 +4,25 @@
 //               transport, vertex, final state) of PhotonFront.inl /
 //               LeptonFront.inl fused into one kernel; up to two output
 //               particles go to a per-source staging pair; profile and ledger
-//               contributions accumulate in block-private LDS copies that are
-//               added to the device totals once per block; per-block output
-//               counts are reduced for the scan.
-//   2. scan     one workgroup: exclusive scan of the per-block counts (next,
-//               other species, children) and the front totals.
-//   3. scatter  stable compaction: block-local prefix of the per-source counts
-//               plus the block offset gives every output its queue slot, so the
-//               order (and the history IDs first + child offset + k) follows
-//               the source order exactly, as the reference's stable scans do.
+//               contributions accumulate in block-private LDS copies (one for
+//               the shower of the block's first particle, one for that of its
+//               last) that are added to the shower's device totals once per
+//               block; per-block output counts and per-(slot, run) output
+//               totals are reduced for the scan.
+//   2. scan     one workgroup: exclusive scan of the per-block counts, and per
+//               slot the position and history offsets of every run's outputs.
+//   3. scatter  stable compaction: the block prefix of the per-source counts
+//               plus the block offset gives the source's global output rank;
+//               the (slot, run) offset turns it into its rank inside its own
+//               shower, so each shower's queue segment, order and history IDs
+//               (first + child rank + k) are those of the shower run alone, as
+//               the reference's stable scans give them.
+// Queues are sorted by slot within each run (the previous front's output is
+// one run, the other species' secondaries and host-staged particles each
+// append one).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdint>
 
 #include "em/em_device.hpp"
@@ -28,100 +36,135 @@ namespace {
 
 constexpr int kStepBlock = 256;
 constexpr int kScanBlock = 1024;
+constexpr std::uint32_t kNoKey = 0xffffffffu;
+constexpr int kRunShift = 20;
 
 struct BlockCounts {
   std::uint32_t next, other, children, steps, interactions, pad[3];
 };
 
-__device__ __forceinline__ std::uint32_t wave_sum(std::uint32_t v) {
+__device__ __forceinline__ std::uint32_t run_of(FrontArgs const& f, std::uint64_t i) {
+  std::uint32_t r = 0;
+  while (r + 1 < f.runs && i >= f.run_end[r]) ++r;
+  return r;
+}
+
+template <class T>
+__device__ __forceinline__ T wave_inclusive(T x, std::uint32_t lane) {
 #pragma unroll
-  for (int offset = 32; offset > 0; offset >>= 1) v += __shfl_xor(v, offset, 64);
-  return v;
+  for (int d = 1; d < 64; d <<= 1) {
+    T const y = __shfl_up(x, d, 64);
+    if (static_cast<int>(lane) >= d) x += y;
+  }
+  return x;
 }
 
 template <bool Photon>
-__global__ __launch_bounds__(kStepBlock) void step_kernel(EmContext const* __restrict__ context,
-                                                          ParticleQueue in, std::uint64_t count,
-                                                          c8mi_particle_state* __restrict__ staging,
-                                                          std::uint32_t* __restrict__ packed,
-                                                          BlockCounts* __restrict__ block_counts) {
+__global__ __launch_bounds__(kStepBlock) void step_kernel(EmContext const* __restrict__ context, FrontArgs f) {
   extern __shared__ long long lds[];
+  __shared__ std::uint32_t keys[kStepBlock];
+  __shared__ std::uint32_t wave_sums[kStepBlock / 64][kShowerFields];
   EmContext const& c = *context;
   std::uint32_t const bins = c.profile_bins;
-  long long* hist = lds;
-  long long* counters = lds + 4u * bins;
-  for (std::uint32_t k = threadIdx.x; k < 4u * bins + kCounters; k += kStepBlock) lds[k] = 0;
+  std::uint32_t const stride = 4u * bins + kCounters;
+  std::uint64_t const first = static_cast<std::uint64_t>(blockIdx.x) * kStepBlock;
+  std::uint64_t const last = (first + kStepBlock < f.count ? first + kStepBlock : f.count) - 1;
+  std::uint32_t const sh0 = f.in.shower[first], sh1 = f.in.shower[last];
+  std::uint32_t const copies = sh0 == sh1 ? 1u : 2u;
+  for (std::uint32_t k = threadIdx.x; k < copies * stride; k += kStepBlock) lds[k] = 0;
   __syncthreads();
-  phys::Accumulator const acc{hist, counters, context};
 
-  std::uint64_t const i = static_cast<std::uint64_t>(blockIdx.x) * kStepBlock + threadIdx.x;
-  std::uint32_t word = 0;
-  std::uint32_t next = 0, other = 0, children = 0, steps = 0, interactions = 0;
-  if (i < count) {
-    phys::Particle const p = load_particle(in, i);
+  std::uint64_t const i = first + threadIdx.x;
+  std::uint32_t v[kShowerFields] = {0, 0, 0, 0, 0};
+  std::uint32_t key = kNoKey;
+  if (i < f.count) {
+    std::uint32_t const s = f.in.shower[i];
+    long long* hist;
+    if (s == sh0) hist = lds;
+    else if (s == sh1) hist = lds + stride;
+    else hist = nullptr;
+    phys::Accumulator const acc{
+        hist ? hist : c.profile + static_cast<std::uint64_t>(s) * 4u * bins,
+        hist ? hist + 4u * bins : c.counters + static_cast<std::uint64_t>(s) * kCounters, context,
+        &c.showers[s], s};
+    phys::Particle p;
+    static_cast<c8mi_particle_state&>(p) = load_particle(f.in, i);
+    p.shower_key = c.showers[s].key;
     phys::Outputs o;
     o.kind[0] = o.kind[1] = kSlotNone;
     o.next = o.other = o.children = 0;
     o.step = o.interaction = false;
     if constexpr (Photon) phys::photon_step(c, acc, p, o);
     else phys::lepton_step(c, acc, p, o);
-    if (o.kind[0] != kSlotNone) staging[2 * i] = o.slot[0];
-    if (o.kind[1] != kSlotNone) staging[2 * i + 1] = o.slot[1];
-    word = pack_counts(o.next, o.other, o.children, o.kind[0], o.kind[1]);
-    packed[i] = word;
-    next = o.next;
-    other = o.other;
-    children = o.children;
-    steps = o.step ? 1u : 0u;
-    interactions = o.interaction ? 1u : 0u;
+    if (o.kind[0] != kSlotNone) f.staging[2 * i] = o.slot[0];
+    if (o.kind[1] != kSlotNone) f.staging[2 * i + 1] = o.slot[1];
+    f.packed[i] = pack_counts(o.next, o.other, o.children, o.kind[0], o.kind[1], s);
+    v[0] = o.next;
+    v[1] = o.other;
+    v[2] = o.children;
+    v[3] = o.step ? 1u : 0u;
+    v[4] = o.interaction ? 1u : 0u;
+    key = (run_of(f, i) << kRunShift) | s;
   }
-  // per-block totals
-  __shared__ std::uint32_t partial[kStepBlock / 64][5];
+  keys[threadIdx.x] = key;
   std::uint32_t const wave = threadIdx.x / 64, lane = threadIdx.x % 64;
-  std::uint32_t const s0 = wave_sum(next), s1 = wave_sum(other), s2 = wave_sum(children);
-  std::uint32_t const s3 = wave_sum(steps), s4 = wave_sum(interactions);
-  if (lane == 0) {
-    partial[wave][0] = s0;
-    partial[wave][1] = s1;
-    partial[wave][2] = s2;
-    partial[wave][3] = s3;
-    partial[wave][4] = s4;
-  }
+  std::uint32_t inc[kShowerFields];
+#pragma unroll
+  for (int k = 0; k < kShowerFields; ++k) inc[k] = wave_inclusive(v[k], lane);
+  if (lane == 63)
+    for (int k = 0; k < kShowerFields; ++k) wave_sums[wave][k] = inc[k];
   __syncthreads();
+
   if (threadIdx.x == 0) {
-    BlockCounts b{0, 0, 0, 0, 0, {0, 0, 0}};
-    for (int w = 0; w < kStepBlock / 64; ++w) {
-      b.next += partial[w][0];
-      b.other += partial[w][1];
-      b.children += partial[w][2];
-      b.steps += partial[w][3];
-      b.interactions += partial[w][4];
+    std::uint32_t t[kShowerFields] = {0, 0, 0, 0, 0};
+    for (int w = 0; w < kStepBlock / 64; ++w)
+      for (int k = 0; k < kShowerFields; ++k) t[k] += wave_sums[w][k];
+    static_cast<BlockCounts*>(f.block_counts)[blockIdx.x] = BlockCounts{t[0], t[1], t[2], t[3], t[4], {0, 0, 0}};
+  }
+  // Per-(slot, run) totals: within the block the keys are non-decreasing
+  // segments; the first source of a segment subtracts its exclusive block
+  // prefix and the last adds its inclusive prefix (arithmetic modulo 2^64).
+  if (key != kNoKey) {
+    std::uint32_t const s = key & ((1u << kRunShift) - 1u), r = key >> kRunShift;
+    unsigned long long* t =
+        f.shower_totals + (static_cast<std::uint64_t>(s) * kMaxRuns + r) * kShowerFields;
+    bool const start = threadIdx.x == 0 || keys[threadIdx.x - 1] != key;
+    bool const end = threadIdx.x == kStepBlock - 1 || keys[threadIdx.x + 1] != key;
+#pragma unroll
+    for (int k = 0; k < kShowerFields; ++k) {
+      std::uint32_t before = 0;
+      for (std::uint32_t w = 0; w < wave; ++w) before += wave_sums[w][k];
+      std::uint32_t const incl = before + inc[k], excl = incl - v[k];
+      if (start && excl != 0) atomicAdd(&t[k], 0ULL - excl);
+      if (end && incl != 0) atomicAdd(&t[k], static_cast<unsigned long long>(incl));
     }
-    block_counts[blockIdx.x] = b;
   }
-  // flush the block's profile and ledger contributions
-  for (std::uint32_t k = threadIdx.x; k < 4u * bins; k += kStepBlock) {
-    long long const v = hist[k];
-    if (v != 0) atomicAdd(reinterpret_cast<unsigned long long*>(&c.profile[k]), static_cast<unsigned long long>(v));
-  }
-  for (std::uint32_t k = threadIdx.x; k < kCounters; k += kStepBlock) {
-    long long const v = counters[k];
-    if (v != 0) atomicAdd(reinterpret_cast<unsigned long long*>(&c.counters[k]), static_cast<unsigned long long>(v));
+  // flush the block's profile and ledger contributions to the showers' totals
+  for (std::uint32_t copy = 0; copy < copies; ++copy) {
+    std::uint32_t const s = copy ? sh1 : sh0;
+    long long const* src = lds + copy * stride;
+    long long* hist = c.profile + static_cast<std::uint64_t>(s) * 4u * bins;
+    long long* counters = c.counters + static_cast<std::uint64_t>(s) * kCounters;
+    for (std::uint32_t k = threadIdx.x; k < stride; k += kStepBlock) {
+      long long const value = src[k];
+      if (value == 0) continue;
+      long long* dst = k < 4u * bins ? hist + k : counters + (k - 4u * bins);
+      atomicAdd(reinterpret_cast<unsigned long long*>(dst), static_cast<unsigned long long>(value));
+    }
   }
 }
 
-__global__ __launch_bounds__(kScanBlock) void scan_kernel(BlockCounts const* __restrict__ counts,
-                                                          std::uint32_t blocks,
-                                                          unsigned long long* __restrict__ offsets,
-                                                          FrontTotals* __restrict__ totals) {
-  __shared__ unsigned long long carry[5];
-  __shared__ unsigned long long wave_sums[kScanBlock / 64][5];
-  if (threadIdx.x < 5) carry[threadIdx.x] = 0;
+__global__ __launch_bounds__(kScanBlock) void scan_kernel(FrontArgs f, std::uint32_t blocks, int photon) {
+  // ---- per-block offsets of the three output streams and the front totals ----
+  __shared__ unsigned long long carry[kShowerFields];
+  __shared__ unsigned long long wave_sums[kScanBlock / 64][kShowerFields];
+  if (threadIdx.x < kShowerFields) carry[threadIdx.x] = 0;
   __syncthreads();
   std::uint32_t const wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+  auto const* counts = static_cast<BlockCounts const*>(f.block_counts);
   for (std::uint32_t base = 0; base < blocks; base += kScanBlock) {
     std::uint32_t const b = base + threadIdx.x;
-    unsigned long long v[5] = {0, 0, 0, 0, 0};
+    unsigned long long v[kShowerFields] = {0, 0, 0, 0, 0};
     if (b < blocks) {
       BlockCounts const bc = counts[b];
       v[0] = bc.next;
@@ -130,82 +173,133 @@ __global__ __launch_bounds__(kScanBlock) void scan_kernel(BlockCounts const* __r
       v[3] = bc.steps;
       v[4] = bc.interactions;
     }
-    // inclusive wave scan
-    unsigned long long inc[5];
+    unsigned long long inc[kShowerFields];
 #pragma unroll
-    for (int f = 0; f < 5; ++f) {
-      unsigned long long x = v[f];
-#pragma unroll
-      for (int d = 1; d < 64; d <<= 1) {
-        unsigned long long const y = __shfl_up(x, d, 64);
-        if (static_cast<int>(lane) >= d) x += y;
-      }
-      inc[f] = x;
-    }
+    for (int k = 0; k < kShowerFields; ++k) inc[k] = wave_inclusive(v[k], lane);
     if (lane == 63)
-      for (int f = 0; f < 5; ++f) wave_sums[wave][f] = inc[f];
+      for (int k = 0; k < kShowerFields; ++k) wave_sums[wave][k] = inc[k];
     __syncthreads();
-    unsigned long long before[5] = {0, 0, 0, 0, 0};
+    unsigned long long before[kShowerFields] = {0, 0, 0, 0, 0};
     for (std::uint32_t w = 0; w < wave; ++w)
-      for (int f = 0; f < 5; ++f) before[f] += wave_sums[w][f];
+      for (int k = 0; k < kShowerFields; ++k) before[k] += wave_sums[w][k];
     if (b < blocks)
-      for (int f = 0; f < 3; ++f) offsets[3 * static_cast<std::uint64_t>(b) + f] = carry[f] + before[f] + inc[f] - v[f];
+      for (int k = 0; k < 3; ++k)
+        f.offsets[3 * static_cast<std::uint64_t>(b) + k] = carry[k] + before[k] + inc[k] - v[k];
     __syncthreads();
     if (threadIdx.x == kScanBlock - 1)
-      for (int f = 0; f < 5; ++f) carry[f] += before[f] + inc[f];
+      for (int k = 0; k < kShowerFields; ++k) carry[k] += before[k] + inc[k];
     __syncthreads();
   }
+
+  // ---- per slot: offsets that turn a global output rank into the rank inside
+  // the shower, new queue counts, history counters and statistics. One
+  // exclusive scan over the slots per (run, stream). ----
+  __shared__ unsigned long long slot_sums[2][kScanBlock / 64];
+  std::uint32_t const s = threadIdx.x;
+  bool const mine = s < f.slots;
+  unsigned long long* row = f.shower_totals + static_cast<std::uint64_t>(s) * kMaxRuns * kShowerFields;
+  // partial[r][k] = this slot's outputs in earlier runs - the rank offset of
+  // the slot inside run r - the outputs of all slots in earlier runs
+  unsigned long long partial[kMaxRuns][3];
+  unsigned long long base[3] = {0, 0, 0}, within[3] = {0, 0, 0}, run_start[3] = {0, 0, 0};
+  int parity = 0;
+#pragma unroll
+  for (int r = 0; r < kMaxRuns; ++r) {
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      partial[r][k] = 0;
+      if (r >= static_cast<int>(f.runs)) continue;  // uniform
+      unsigned long long const x = mine ? row[r * kShowerFields + k] : 0ULL;
+      unsigned long long const inc = wave_inclusive(x, lane);
+      if (lane == 63) slot_sums[parity][wave] = inc;
+      __syncthreads();
+      unsigned long long before = 0, all = 0;
+      for (std::uint32_t w = 0; w < kScanBlock / 64; ++w) {
+        unsigned long long const y = slot_sums[parity][w];
+        before += w < wave ? y : 0ULL;
+        all += y;
+      }
+      parity ^= 1;  // the next scan writes the other buffer; the one after is past a barrier
+      unsigned long long const excl = before + inc - x;
+      partial[r][k] = within[k] - excl - run_start[k];
+      base[k] += excl;
+      within[k] += x;
+      run_start[k] += all;
+    }
+  }
+  if (mine) {
+    unsigned long long const history = f.history[s];
+    base[2] = history;
+#pragma unroll
+    for (int r = 0; r < kMaxRuns; ++r)
+#pragma unroll
+      for (int k = 0; k < 3; ++k)
+        if (r < static_cast<int>(f.runs))
+          f.adjust[(static_cast<std::uint64_t>(s) * kMaxRuns + r) * 3 + k] = base[k] + partial[r][k];
+    unsigned long long steps = 0, interactions = 0;
+#pragma unroll
+    for (int r = 0; r < kMaxRuns; ++r) {
+      steps += row[r * kShowerFields + 3];
+      interactions += row[r * kShowerFields + 4];
+#pragma unroll
+      for (int k = 0; k < kShowerFields; ++k) row[r * kShowerFields + k] = 0;  // ready for the next front
+    }
+    f.history[s] = history + within[2];
+    unsigned long long* st = f.stats + static_cast<std::uint64_t>(s) * kSlotStats;
+    st[photon ? 0 : 1] += steps;
+    st[photon ? 2 : 3] += interactions;
+    st[4] += within[2];
+    f.report[kReportHeader + 2 * static_cast<std::uint64_t>(s)] = within[0];
+    f.report[kReportHeader + 2 * static_cast<std::uint64_t>(s) + 1] = within[1];
+  }
   if (threadIdx.x == 0) {
-    FrontTotals t{};
-    t.next = carry[0];
-    t.other = carry[1];
-    t.children = carry[2];
-    t.steps = carry[3];
-    t.interactions = carry[4];
-    *totals = t;
+    FrontTotals out{};
+    out.next = carry[0];
+    out.other = carry[1];
+    out.children = carry[2];
+    out.steps = carry[3];
+    out.interactions = carry[4];
+    *reinterpret_cast<FrontTotals*>(f.report) = out;
   }
 }
 
-__global__ __launch_bounds__(kStepBlock) void scatter_kernel(
-    std::uint32_t const* __restrict__ packed, std::uint64_t count,
-    c8mi_particle_state const* __restrict__ staging, unsigned long long const* __restrict__ offsets,
-    ParticleQueue next_queue, ParticleQueue other_queue, std::uint64_t other_base,
-    std::uint64_t first_history) {
+__global__ __launch_bounds__(kStepBlock) void scatter_kernel(FrontArgs f, ParticleQueue next_queue,
+                                                             ParticleQueue other_queue,
+                                                             std::uint64_t other_base) {
   __shared__ std::uint32_t wave_sums[kStepBlock / 64][3];
   std::uint64_t const i = static_cast<std::uint64_t>(blockIdx.x) * kStepBlock + threadIdx.x;
-  std::uint32_t const word = i < count ? packed[i] : 0u;
+  std::uint32_t const word = i < f.count ? f.packed[i] : 0u;
   std::uint32_t const v[3] = {word & 3u, (word >> 2) & 3u, (word >> 4) & 3u};
   std::uint32_t const wave = threadIdx.x / 64, lane = threadIdx.x % 64;
   std::uint32_t inc[3];
 #pragma unroll
-  for (int f = 0; f < 3; ++f) {
-    std::uint32_t x = v[f];
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-      std::uint32_t const y = __shfl_up(x, d, 64);
-      if (static_cast<int>(lane) >= d) x += y;
-    }
-    inc[f] = x;
-  }
+  for (int k = 0; k < 3; ++k) inc[k] = wave_inclusive(v[k], lane);
   if (lane == 63)
-    for (int f = 0; f < 3; ++f) wave_sums[wave][f] = inc[f];
+    for (int k = 0; k < 3; ++k) wave_sums[wave][k] = inc[k];
   __syncthreads();
-  if (i >= count) return;
+  if (i >= f.count) return;
+  std::uint32_t const s = word >> 12;
+  unsigned long long const* adj = f.adjust + (static_cast<std::uint64_t>(s) * kMaxRuns + run_of(f, i)) * 3;
   std::uint64_t pos[3];
-  for (int f = 0; f < 3; ++f) {
+  for (int k = 0; k < 3; ++k) {
     std::uint32_t before = 0;
-    for (std::uint32_t w = 0; w < wave; ++w) before += wave_sums[w][f];
-    pos[f] = offsets[3 * static_cast<std::uint64_t>(blockIdx.x) + f] + before + inc[f] - v[f];
+    for (std::uint32_t w = 0; w < wave; ++w) before += wave_sums[w][k];
+    pos[k] = f.offsets[3 * static_cast<std::uint64_t>(blockIdx.x) + k] + before + inc[k] - v[k] + adj[k];
   }
   std::uint64_t next_pos = pos[0], other_pos = other_base + pos[1];
-  std::uint64_t const child_base = first_history + pos[2];
-  for (int s = 0; s < 2; ++s) {
-    std::uint32_t const kind = (word >> (6 + 2 * s)) & 3u;
+  std::uint64_t const child_base = pos[2];
+  for (int k = 0; k < 2; ++k) {
+    std::uint32_t const kind = (word >> (6 + 2 * k)) & 3u;
     if (kind == kSlotNone) continue;
-    c8mi_particle_state p = staging[2 * i + s];
+    c8mi_particle_state p = f.staging[2 * i + k];
     if (kind != kSlotNextContinue) p.history_id = child_base + p.history_id;
-    if (kind == kSlotOtherChild) store_particle(other_queue, other_pos++, p);
-    else store_particle(next_queue, next_pos++, p);
+    if (kind == kSlotOtherChild) {
+      store_particle(other_queue, other_pos, p);
+      other_queue.shower[other_pos++] = s;
+    } else {
+      store_particle(next_queue, next_pos, p);
+      next_queue.shower[next_pos++] = s;
+    }
   }
 }
 
@@ -218,42 +312,36 @@ unsigned grid_for(std::uint64_t n, int block) {
 }  // namespace
 
 std::size_t step_lds_bytes(std::uint32_t bins) {
-  return (4u * static_cast<std::size_t>(bins) + kCounters) * sizeof(long long);
+  return 2 * (4u * static_cast<std::size_t>(bins) + kCounters) * sizeof(long long);
 }
 
 std::size_t block_count_bytes(std::uint64_t n) { return grid_for(n, kStepBlock) * sizeof(BlockCounts); }
 
 std::size_t offset_bytes(std::uint64_t n) { return grid_for(n, kStepBlock) * 3 * sizeof(unsigned long long); }
 
-void launch_step(bool photon, EmContext const* context, std::uint32_t bins, ParticleQueue const& in,
-                 std::uint64_t count, c8mi_particle_state* staging, std::uint32_t* packed,
-                 void* block_counts, hipStream_t stream) {
-  if (count == 0) return;
-  unsigned const grid = grid_for(count, kStepBlock);
+void launch_step(bool photon, EmContext const* context, std::uint32_t bins, FrontArgs const& f,
+                 hipStream_t stream) {
+  if (f.count == 0) return;
+  unsigned const grid = grid_for(f.count, kStepBlock);
   std::size_t const lds = step_lds_bytes(bins);
-  auto* bc = static_cast<BlockCounts*>(block_counts);
-  if (photon)
-    step_kernel<true><<<grid, kStepBlock, lds, stream>>>(context, in, count, staging, packed, bc);
-  else
-    step_kernel<false><<<grid, kStepBlock, lds, stream>>>(context, in, count, staging, packed, bc);
+  if (photon) step_kernel<true><<<grid, kStepBlock, lds, stream>>>(context, f);
+  else step_kernel<false><<<grid, kStepBlock, lds, stream>>>(context, f);
   C8MI_HIP(hipGetLastError());
 }
 
-void launch_scan(void const* block_counts, std::uint64_t count, unsigned long long* offsets,
-                 FrontTotals* totals, hipStream_t stream) {
-  std::uint32_t const blocks = count == 0 ? 0u : grid_for(count, kStepBlock);
-  scan_kernel<<<1, kScanBlock, 0, stream>>>(static_cast<BlockCounts const*>(block_counts), blocks,
-                                             offsets, totals);
+void launch_scan(bool photon, FrontArgs const& f, hipStream_t stream) {
+  if (f.slots > static_cast<std::uint32_t>(kScanBlock) || f.slots > kMaxShowerSlots)
+    throw std::length_error("em: more pool slots than one scan workgroup covers");
+  std::uint32_t const blocks = f.count == 0 ? 0u : grid_for(f.count, kStepBlock);
+  scan_kernel<<<1, kScanBlock, 0, stream>>>(f, blocks, photon ? 1 : 0);
   C8MI_HIP(hipGetLastError());
 }
 
-void launch_scatter(std::uint32_t const* packed, std::uint64_t count, c8mi_particle_state const* staging,
-                    unsigned long long const* offsets, ParticleQueue const& next_queue,
-                    ParticleQueue const& other_queue, std::uint64_t other_base,
-                    std::uint64_t first_history, hipStream_t stream) {
-  if (count == 0) return;
-  scatter_kernel<<<grid_for(count, kStepBlock), kStepBlock, 0, stream>>>(
-      packed, count, staging, offsets, next_queue, other_queue, other_base, first_history);
+void launch_scatter(FrontArgs const& f, ParticleQueue const& next_queue, ParticleQueue const& other_queue,
+                    std::uint64_t other_base, hipStream_t stream) {
+  if (f.count == 0) return;
+  scatter_kernel<<<grid_for(f.count, kStepBlock), kStepBlock, 0, stream>>>(f, next_queue, other_queue,
+                                                                           other_base);
   C8MI_HIP(hipGetLastError());
 }
 

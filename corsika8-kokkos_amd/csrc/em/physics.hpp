@@ -26,7 +26,11 @@
 
 namespace c8mi::em::phys {
 
-using Particle = c8mi_particle_state;
+// Working copy of a particle inside a step: the ABI state plus the Philox
+// shower key of the shower it belongs to (children inherit it).
+struct Particle : c8mi_particle_state {
+  std::uint64_t shower_key;
+};
 
 // ---- constants (ScalarPhysicalConstants.hpp, TransportMass.hpp, PhotonPairKinematics.hpp,
 //      ParticleId.hpp, ContinuousPhysicsQueries.hpp, LeptonContinuousStep.hpp) ----
@@ -82,7 +86,7 @@ __device__ __forceinline__ int sgn(double v) { return (0. < v) - (v < 0.); }
 
 __device__ __forceinline__ double uniform(EmContext const& c, Particle const& p,
                                           std::uint32_t process, std::uint64_t draw) {
-  return uniform_open01(RandomKey{c.seed, c.shower, p.history_id, p.step_id, process, draw});
+  return uniform_open01(RandomKey{c.seed, p.shower_key, p.history_id, p.step_id, process, draw});
 }
 
 __device__ __forceinline__ double transport_mass(std::int32_t pid) {
@@ -101,12 +105,17 @@ __device__ __forceinline__ bool is_charged_lepton(std::int32_t pid) {
   return pid == 11 || pid == -11 || pid == 13 || pid == -13;
 }
 
-// ---- accumulation: block-private LDS copies of the histograms and counters ----
+// ---- accumulation: the histograms and counters of the particle's shower ----
+// hist / counters point at the block-private LDS copy of the shower (kept for
+// the showers of the block's first and last particle) or at the shower's
+// device totals.
 
 struct Accumulator {
-  long long* hist;      // LDS, 4 * bins
-  long long* counters;  // LDS, kCounters
+  long long* hist;      // 4 * bins
+  long long* counters;  // kCounters
   EmContext const* ctx;
+  ShowerSlot const* shower;
+  std::uint32_t slot;
 };
 
 __device__ __forceinline__ void add_fixed(Accumulator const& a, long long* address, double value,
@@ -128,7 +137,8 @@ __device__ __forceinline__ void ledger(Accumulator const& a, int term, double va
   add_fixed(a, &a.counters[term], value, a.ctx->energy_scale);
 }
 
-__device__ __forceinline__ double project_grammage(EmContext const& c, double const p[3]) {
+__device__ __forceinline__ double project_grammage(Accumulator const& a, double const p[3]) {
+  ShowerSlot const& c = *a.shower;
   double proj = 0.;
   for (int k = 0; k < 3; ++k) proj += (p[k] - c.axis_start[k]) * c.axis_direction[k];
   double const fb = proj / c.axis_step;
@@ -227,10 +237,12 @@ __device__ __forceinline__ void thinning_ledger(Accumulator const& a, int status
   ledger(a, kThinningEnergy, parent_energy_weight - kept_energy_weight);
 }
 
-// Observation / fallback records: appended with one atomic per record (rare).
-__device__ __forceinline__ void record_observation(EmContext const& c, Particle const& p) {
-  unsigned long long const slot = atomicAdd(&c.record_counts[0], 1ULL);
-  if (slot < c.observation_capacity) c.observations[slot] = p;
+// Observation / fallback records: appended to the shower's own region with one
+// atomic per record (rare).
+__device__ __forceinline__ void record_observation(Accumulator const& a, Particle const& p) {
+  EmContext const& c = *a.ctx;
+  unsigned long long const n = atomicAdd(&c.record_counts[2u * a.slot], 1ULL);
+  if (n < c.observation_capacity) c.observations[a.slot * c.observation_capacity + n] = p;
 }
 
 __device__ __forceinline__ void record_fallback(Accumulator const& a, Particle const& p, int reason,
@@ -238,15 +250,15 @@ __device__ __forceinline__ void record_fallback(Accumulator const& a, Particle c
   EmContext const& c = *a.ctx;
   count(a, kFallbackReason0 + (reason & 31));
   ledger(a, kFallbackEnergy, p.energy_gev * p.weight);
-  unsigned long long const slot = atomicAdd(&c.record_counts[1], 1ULL);
-  if (slot < c.fallback_capacity) {
+  unsigned long long const n = atomicAdd(&c.record_counts[2u * a.slot + 1u], 1ULL);
+  if (n < c.fallback_capacity) {
     c8mi_em_fallback f;
     f.particle = p;
     f.reason = reason;
     f.process_id = process;
     f.stage = stage;
     f.pad_ = 0;
-    c.fallbacks[slot] = f;
+    c.fallbacks[a.slot * c.fallback_capacity + n] = f;
   }
 }
 

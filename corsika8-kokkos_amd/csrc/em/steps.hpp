@@ -104,8 +104,8 @@ __device__ __forceinline__ void photon_step(EmContext const& c, Accumulator cons
   o.step = true;
   // accumulate_photon_profile_step
   double const w = start.weight;
-  double const gs = project_grammage(c, start.position_m);
-  double const ge = project_grammage(c, end.position_m);
+  double const gs = project_grammage(acc, start.position_m);
+  double const ge = project_grammage(acc, end.position_m);
   particle_profile(acc, start.pid, gs, ge, w);
   point_energy_profile(acc, ge, cut_deposit * w);
   count(acc, kSteps);
@@ -115,13 +115,13 @@ __device__ __forceinline__ void photon_step(EmContext const& c, Accumulator cons
     if (observed_before_cut) {
       ledger(acc, kOverlap, cut_deposit * w);
       terminal_energy(acc, end.energy_gev * w, true);
-      record_observation(c, end);
+      record_observation(acc, end);
     }
     return;
   }
   if (limit == kPlObservation || limit == kPlEscaped) {
     terminal_energy(acc, end.energy_gev * w, limit == kPlObservation);
-    record_observation(c, end);
+    record_observation(acc, end);
     return;
   }
   if (limit == kPlLayer) {
@@ -438,8 +438,8 @@ __device__ __forceinline__ void lepton_step(EmContext const& c, Accumulator cons
   o.step = true;
   // accumulate_lepton_profile_step
   double const w = start.weight;
-  double const gs = project_grammage(c, start.position_m);
-  double const ge = project_grammage(c, end.position_m);
+  double const gs = project_grammage(acc, start.position_m);
+  double const ge = project_grammage(acc, end.position_m);
   particle_profile(acc, start.pid, gs, ge, w);
   energy_profile(acc, gs, ge, continuous_deposit * w);
   point_energy_profile(acc, ge, cut_deposit * w);
@@ -453,13 +453,13 @@ __device__ __forceinline__ void lepton_step(EmContext const& c, Accumulator cons
     if (observed_before_cut) {
       ledger(acc, kOverlap, (cut_deposit + tmg) * w);
       terminal_energy(acc, end.energy_gev * w, true);
-      record_observation(c, end);
+      record_observation(acc, end);
     }
     return;
   }
   if (limit == kLlObservation || limit == kLlEscaped) {
     terminal_energy(acc, end.energy_gev * w, limit == kLlObservation);
-    record_observation(c, end);
+    record_observation(acc, end);
     return;
   }
   if (limit != kLlInteraction) {

@@ -196,7 +196,42 @@ typedef struct c8mi_em_config {
   double axis_step_m;
   const double* axis_grammage_g_per_cm2;
   uint64_t axis_support_count;
+  /* Concurrent showers (pool slots, 1..1024; 0 means 1). Each slot has its
+   * own Philox shower key, history counter, projection axis, profile, ledger,
+   * observation_capacity observations and fallback_capacity fallbacks; the
+   * particle queues (queue_capacity per species and buffer) are shared. */
+  uint32_t shower_slots;
+  uint32_t reserved_;
 } c8mi_em_config;
+
+/* Shower axis of one pool slot (the ProfileProjectionView of its ShowerConfig):
+ * grammage[k] at start_m + k * step_m * direction. */
+typedef struct c8mi_em_shower_axis {
+  double start_m[3];
+  double direction[3];
+  double step_m;
+  const double* grammage_g_per_cm2;
+  uint64_t support_count;
+} c8mi_em_shower_axis;
+
+/* Progress of one pool slot. */
+typedef struct c8mi_em_slot_state {
+  uint64_t shower_id;
+  uint64_t pending_photons; /* queued on the device or staged on the host */
+  uint64_t pending_leptons;
+  uint64_t next_history_id;
+  uint64_t photon_fronts;   /* fronts in which the shower had photons */
+  uint64_t lepton_fronts;
+  uint64_t photon_steps;
+  uint64_t lepton_steps;
+  uint64_t photon_interactions;
+  uint64_t lepton_interactions;
+  uint64_t secondaries;
+  uint64_t observations;
+  uint64_t fallbacks;
+  uint32_t active;          /* begun and not ended */
+  uint32_t pad_;
+} c8mi_em_slot_state;
 
 /* Per-call statistics (PhotonCascadeResult / LeptonCascadeResult counters). */
 typedef struct c8mi_em_run_result {
@@ -301,6 +336,90 @@ int c8mi_em_download_observations(c8mi_em* em, c8mi_particle_state* out, uint64_
                                   uint64_t* count);
 int c8mi_em_download_fallbacks(c8mi_em* em, c8mi_em_fallback* out, uint64_t capacity,
                                uint64_t* count);
+
+/* ---- Shower pool: several showers in the same device fronts ----
+ * No counterpart in IEmBackend, which holds one shower per backend; this is the
+ * device side of a serving loop that keeps many showers resident. Particles of
+ * all active slots share the queues, sorted by slot; compaction is stable
+ * within each slot and history IDs are drawn per slot, so every shower's
+ * Philox draws, records, profile and ledger are those of the same shower run
+ * alone. The single-shower entry points above act on slot 0. */
+
+/* Starts a shower in an idle slot (no pending particles): clears its profile,
+ * ledger and records, sets its Philox shower key and first secondary history
+ * ID and its projection axis (NULL: the axis of the configuration). */
+int c8mi_em_slot_begin(c8mi_em* em, uint32_t slot, uint64_t shower_id,
+                       uint64_t first_history_id, const c8mi_em_shower_axis* axis);
+
+/* Stages particles of one slot; they join the device queues, sorted by slot,
+ * at the start of the next front (of either species). */
+int c8mi_em_slot_enqueue(c8mi_em* em, uint32_t slot, const c8mi_particle_state* particles,
+                         uint64_t count);
+
+/* Runs up to `maximum_rounds` rounds (a photon front, then a lepton front, each
+ * over all slots) or until every queue is empty; *out sums over the slots. */
+int c8mi_em_run_rounds(c8mi_em* em, uint64_t maximum_rounds, c8mi_em_run_result* out);
+
+int c8mi_em_slot_state_get(const c8mi_em* em, uint32_t slot, c8mi_em_slot_state* out);
+
+/* Per-slot forms of the downloads above. */
+int c8mi_em_slot_download_profile(c8mi_em* em, uint32_t slot, double* photons, double* electrons,
+                                  double* positrons, double* energy_loss, c8mi_em_ledger* ledger);
+int c8mi_em_slot_download_observations(c8mi_em* em, uint32_t slot, c8mi_particle_state* out,
+                                       uint64_t capacity, uint64_t* count);
+int c8mi_em_slot_download_fallbacks(c8mi_em* em, uint32_t slot, c8mi_em_fallback* out,
+                                    uint64_t capacity, uint64_t* count);
+
+/* Marks a slot idle. A slot that still has pending particles is refused
+ * (logic error): run it to completion first. */
+int c8mi_em_slot_end(c8mi_em* em, uint32_t slot);
+
+/* Host-side pending count of a slot (no device call): particles queued on the
+ * device plus particles staged on the host. */
+int c8mi_em_slot_pending(const c8mi_em* em, uint32_t slot, uint64_t* photons, uint64_t* leptons);
+
+/* ---- Serving loop over the pool ----
+ * A stream of shower requests is admitted into free slots as they arrive and
+ * as the admitted showers' footprints (estimated peak queue occupancy) fit the
+ * budget; every round steps all admitted showers together (continuous
+ * batching). A pool of one slot is the one-shower-at-a-time form. */
+typedef struct c8mi_em_request {
+  c8mi_particle_state primary;
+  uint64_t shower_id;
+  uint64_t first_history_id;
+  double arrival_s;            /* offset from the start of the serve call, non-decreasing */
+  double footprint_particles;  /* admission estimate of the shower's peak queue occupancy */
+  c8mi_em_shower_axis axis;    /* grammage_g_per_cm2 NULL: the configuration's axis */
+} c8mi_em_request;
+
+typedef struct c8mi_em_serve_options {
+  double footprint_budget_particles; /* admitted footprints stay below this (one shower always fits) */
+  uint32_t realtime;                 /* 1: admit at the wall-clock arrival; 0: ignore arrival times */
+  uint32_t pad_;
+} c8mi_em_serve_options;
+
+typedef struct c8mi_em_request_result {
+  double admitted_s;  /* offsets from the start of the serve call */
+  double finished_s;
+  uint64_t photon_steps;
+  uint64_t lepton_steps;
+  uint64_t photon_interactions;
+  uint64_t lepton_interactions;
+  uint64_t secondaries;
+  uint64_t next_history_id;
+  uint64_t observations;
+  uint64_t fallbacks;
+  uint32_t slot;
+  uint32_t pad_;
+  c8mi_em_ledger ledger;
+} c8mi_em_request_result;
+
+/* Runs the whole request stream to completion; *totals sums the fronts. The
+ * pool must hold no pending particles on entry (finished showers are ended);
+ * it is idle on return. */
+int c8mi_em_serve(c8mi_em* em, const c8mi_em_request* requests, uint64_t count,
+                  const c8mi_em_serve_options* options, c8mi_em_request_result* results,
+                  c8mi_em_run_result* totals);
 
 /* Per-kernel device time (HIP events) since enable: ms[0] photon step,
  * ms[1] lepton step, ms[2] scans, ms[3] scatter. */

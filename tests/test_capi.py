@@ -156,3 +156,28 @@ def test_random_key_dtype_matches_c_layout():
 
     assert c8mi.RANDOM_KEY_DTYPE.itemsize == 48
     assert c8mi.RANDOM_KEY_DTYPE.fields["draw_id"][1] == 40
+
+
+def test_em_binding_structs_match_c_layout(tmp_path):
+    """The ctypes mirrors of the EM ABI structs have the C sizes (gcc on the
+    public header)."""
+    from c8mi import em
+
+    names = {"c8mi_em_config": em.EmConfig, "c8mi_em_run_result": em.RunResult,
+             "c8mi_em_ledger": em.Ledger, "c8mi_em_shower_axis": em.ShowerAxis,
+             "c8mi_em_slot_state": em.SlotState, "c8mi_em_request": em.Request,
+             "c8mi_em_serve_options": em.ServeOptions, "c8mi_em_request_result": em.RequestResult,
+             "c8mi_environment": em.Environment, "c8mi_env_node": em.EnvNode,
+             "c8mi_env_model": em.EnvModel}
+    src = tmp_path / "sizes.c"
+    src.write_text('#include <stdio.h>\n#include "c8mi/c8mi_em.h"\nint main(void){\n' +
+                   "".join(f'printf("%zu\\n", sizeof({n}));\n' for n in names) + "return 0;}\n")
+    exe = tmp_path / "sizes"
+    import subprocess
+
+    subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    sizes = [int(x) for x in subprocess.run([str(exe)], check=True, capture_output=True,
+                                            text=True).stdout.split()]
+    for (name, cls), size in zip(names.items(), sizes):
+        assert ctypes.sizeof(cls) == size, name
+    assert em.FALLBACK_DTYPE.itemsize == 144

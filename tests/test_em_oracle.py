@@ -206,7 +206,15 @@ def test_oracle_shower_closes_energy(tables, atmosphere, pid, energy, seed, thin
     prim = em.particle(pid, energy, (0.0, 0.0, 112750.0), (0.0, 0.0, -1.0), history_id=1)
     r = oracle_lib.em_run_shower(cfg, tables, atmosphere, prim)
     led = em.decode_counters(r["counters"], cfg.struct.profile_energy_scale)
-    assert abs(em.ledger_residual(led)) <= 1e-9 * energy
+    # without thinning the closure is exact to 1e-9 E0; a thinned two-child
+    # vertex books the transport / native electron mass difference for both
+    # children at the parent's weight but corrects it with the kept weights only
+    # (the reference's bookkeeping), bounded by 2 dm w_max per thinned vertex
+    bound = 1e-9 * energy
+    if thinning:
+        vertices = led["thinning_hillas_vertices"] + led["thinning_statistical_vertices"]
+        bound += 2 * 4.61e-11 * thinning[1] * vertices
+    assert abs(em.ledger_residual(led)) <= bound
     assert led["invalid_records"] == 0 and led["fixed_point_overflows"] == 0
     st = r["stats"]
     assert st["pending_photons"] == 0 and st["pending_leptons"] == 0
