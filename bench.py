@@ -216,12 +216,14 @@ def em_section(showers, plain_showers, slots, cpu_seconds, no_cpu_baseline, ener
     plain_steps = plain_totals["photon_steps"] + plain_totals["lepton_steps"]
     plain_latency = [r["finished_s"] - r["admitted_s"] for r in plain_results]
 
-    kind = max(("photon_step", "lepton_step"), key=lambda k: timing[k][0])
-    step_kernel_ms = timing["photon_step"][0] + timing["lepton_step"][0]
+    step_kinds = ("photon_step", "lepton_transport", "lepton_vertex")
+    kind = max(step_kinds, key=lambda k: timing[k][0])
+    step_kernel_ms = sum(timing[k][0] for k in step_kinds)
     total_kernel_ms = sum(v[0] for v in timing.values())
     # algorithmic bytes of all particle-steps over the summed step-kernel time
     achieved = steps * EM_BYTES_PER_STEP / (step_kernel_ms * 1e-3) / 1e9
-    kernel_name = "step_kernel<photon>" if kind == "photon_step" else "step_kernel<lepton>"
+    kernel_name = {"photon_step": "photon_step_kernel", "lepton_transport": "lepton_transport_kernel",
+                   "lepton_vertex": "lepton_vertex_kernel"}[kind]
     out = {
         "metric": "EM particle-steps/s",
         "value": steps / elapsed,
@@ -232,6 +234,8 @@ def em_section(showers, plain_showers, slots, cpu_seconds, no_cpu_baseline, ener
         "ms_per_shower": elapsed * 1e3 / showers,
         "latency_ms": {"p50": float(np.percentile(pool_latency, 50) * 1e3),
                        "p99": float(np.percentile(pool_latency, 99) * 1e3)},
+        "photon_steps": totals["photon_steps"],
+        "lepton_steps": totals["lepton_steps"],
         "fronts": totals["photon_fronts"] + totals["lepton_fronts"],
         "max_front": totals["max_front"],
         "one_at_a_time": {"value": plain_steps / plain_elapsed, "showers": plain_showers,

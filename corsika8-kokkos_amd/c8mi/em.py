@@ -570,6 +570,7 @@ class EmBackend:
         _check(library().c8mi_em_enable_kernel_timing(self.handle, c_int(1 if enable else 0)))
 
     def kernel_time(self):
-        ms, n = (c_double * 4)(), (c_uint64 * 4)()
+        ms, n = (c_double * 6)(), (c_uint64 * 6)()
         _check(library().c8mi_em_kernel_time(self.handle, ms, n))
-        return {k: (ms[i], n[i]) for i, k in enumerate(("photon_step", "lepton_step", "scan", "scatter"))}
+        return {k: (ms[i], n[i]) for i, k in enumerate(("photon_step", "lepton_transport", "lepton_vertex",
+                                                        "count", "scan", "scatter"))}

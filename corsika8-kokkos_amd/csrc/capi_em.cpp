@@ -275,7 +275,7 @@ int c8mi_em_enable_kernel_timing(c8mi_em* em, int enable) {
   });
 }
 
-int c8mi_em_kernel_time(c8mi_em const* em, double ms[4], uint64_t launches[4]) {
+int c8mi_em_kernel_time(c8mi_em const* em, double ms[6], uint64_t launches[6]) {
   return guarded([&] {
     require_handle(em);
     if (!ms || !launches) throw std::invalid_argument("null argument");

@@ -210,14 +210,57 @@ EmBackend::EmBackend(c8mi_em_config const& config, c8mi_native_table_set const& 
   h.profile = profile_.data();
   context_.allocate(1);
   C8MI_HIP(hipMemcpy(context_.data(), &h, sizeof(EmContext), hipMemcpyHostToDevice));
+  vertex_count_.allocate(1);
+  C8MI_HIP(hipMemset(vertex_count_.data(), 0, vertex_count_.bytes()));
+  init_dev_.allocate(slots_);
+  init_host_ = PinnedBuffer<SlotInit>(slots_);
+  gather_slots_dev_.allocate(slots_);
+  gather_slots_host_ = PinnedBuffer<std::uint32_t>(slots_);
+  gather_dev_.allocate(static_cast<std::size_t>(slots_) * kGatherWords);
+  gather_host_ = PinnedBuffer<unsigned long long>(static_cast<std::size_t>(slots_) * kGatherWords);
   for (auto& e : events_) C8MI_HIP(hipEventCreate(&e));
+  C8MI_HIP(hipEventCreateWithFlags(&init_copied_, hipEventDisableTiming));
+  C8MI_HIP(hipEventCreateWithFlags(&unstage_copied_, hipEventDisableTiming));
   reset_pool();
+}
+
+PoolArrays EmBackend::pool_arrays() const {
+  PoolArrays p;
+  p.showers = showers_.data();
+  p.profile = profile_.data();
+  p.counters = counters_.data();
+  p.record_counts = record_counts_.data();
+  p.stats = stats_.data();
+  p.history = history_.data();
+  p.bins = static_cast<std::uint32_t>(nonzero(config_.profile_bins));
+  return p;
+}
+
+void EmBackend::start_slots(SlotInit const* inits, std::uint32_t count) {
+  if (count == 0) return;
+  C8MI_HIP(hipEventSynchronize(init_copied_));  // the pinned staging of the previous batch is free
+  std::memcpy(init_host_.data(), inits, count * sizeof(SlotInit));
+  C8MI_HIP(hipMemcpyAsync(init_dev_.data(), init_host_.data(), count * sizeof(SlotInit), hipMemcpyHostToDevice,
+                          stream_));
+  C8MI_HIP(hipEventRecord(init_copied_, stream_));
+  launch_slot_init(pool_arrays(), init_dev_.data(), count, stream_);
+  for (std::uint32_t k = 0; k < count; ++k) host_showers_[inits[k].slot] = inits[k].shower;
+}
+
+void EmBackend::activate(std::uint32_t slot, std::uint64_t shower_id) {
+  SlotHost& h = slot_host_[slot];
+  h.active = true;
+  h.shower_id = shower_id;
+  h.fronts[0] = h.fronts[1] = 0;
+  h.injected = 0;
 }
 
 EmBackend::~EmBackend() {
   if (stream_) (void)hipStreamSynchronize(stream_);
   for (auto& e : events_)
     if (e) (void)hipEventDestroy(e);
+  if (init_copied_) (void)hipEventDestroy(init_copied_);
+  if (unstage_copied_) (void)hipEventDestroy(unstage_copied_);
   if (own_stream_ && stream_) (void)hipStreamDestroy(stream_);
 }
 
@@ -259,9 +302,11 @@ void EmBackend::slot_begin(std::uint32_t slot, std::uint64_t shower_id, std::uin
   SlotHost& h = slot_host_[slot];
   if (h.pending[0] || h.pending[1] || h.staged[0] || h.staged[1])
     throw std::logic_error("em: slot " + std::to_string(slot) + " still has pending particles");
-  ShowerSlot s = host_showers_[slot];
+  SlotInit init{};
+  ShowerSlot& s = init.shower;
   if (axis) {
     validate_axis(*axis);
+    C8MI_HIP(hipStreamSynchronize(stream_));  // the slot's previous table may still be read
     upload(h.axis, axis->grammage_g_per_cm2, axis->support_count);
     for (int a = 0; a < 3; ++a) {
       s.axis_start[a] = axis->start_m[a];
@@ -280,23 +325,11 @@ void EmBackend::slot_begin(std::uint32_t slot, std::uint64_t shower_id, std::uin
     s.axis_support_count = config_.axis_support_count;
   }
   s.key = shower_id;
-  host_showers_[slot] = s;
-  std::uint64_t const bins = nonzero(config_.profile_bins);
-  C8MI_HIP(hipMemcpyAsync(showers_.data() + slot, &host_showers_[slot], sizeof(ShowerSlot),
-                          hipMemcpyHostToDevice, stream_));
-  C8MI_HIP(hipMemsetAsync(profile_.data() + slot * 4u * bins, 0, 4u * bins * sizeof(long long), stream_));
-  C8MI_HIP(hipMemsetAsync(counters_.data() + static_cast<std::uint64_t>(slot) * kCounters, 0,
-                          kCounters * sizeof(long long), stream_));
-  C8MI_HIP(hipMemsetAsync(record_counts_.data() + 2u * slot, 0, 2 * sizeof(unsigned long long), stream_));
-  C8MI_HIP(hipMemsetAsync(stats_.data() + static_cast<std::uint64_t>(slot) * kSlotStats, 0,
-                          kSlotStats * sizeof(unsigned long long), stream_));
-  unsigned long long const first = first_history_id;
-  C8MI_HIP(hipMemcpyAsync(history_.data() + slot, &first, sizeof first, hipMemcpyHostToDevice, stream_));
+  init.first_history = first_history_id;
+  init.slot = slot;
+  start_slots(&init, 1);
   C8MI_HIP(hipStreamSynchronize(stream_));
-  h.active = true;
-  h.shower_id = shower_id;
-  h.fronts[0] = h.fronts[1] = 0;
-  h.injected = 0;
+  activate(slot, shower_id);
   update_active_range();
 }
 
@@ -332,7 +365,10 @@ void EmBackend::slot_enqueue(std::uint32_t slot, c8mi_particle_state const* part
   for (std::uint64_t i = 0; i < count; ++i) {
     c8mi_particle_state const& p = particles[i];
     int const s = p.pid == 22 ? 0 : 1;
-    staged_[s].push_back(Staged{p, slot});
+    Staged e{};
+    e.particle = p;
+    e.slot = slot;
+    staged_[s].push_back(e);
     ++h.staged[s];
     h.injected += static_cast<long double>(std::llrint(p.energy_gev * p.weight * config_.profile_energy_scale));
   }
@@ -353,7 +389,8 @@ void EmBackend::append_run(int species, std::uint64_t end) {
 }
 
 // Uploads the host-staged particles of one species as one run, sorted by slot
-// (stable, so each slot keeps its enqueue order).
+// (stable, so each slot keeps its enqueue order): one copy from pinned memory
+// and one kernel that writes the queue's fields.
 void EmBackend::flush_staged(int species) {
   auto& v = staged_[species];
   if (v.empty()) return;
@@ -361,58 +398,18 @@ void EmBackend::flush_staged(int species) {
   std::size_t const n = v.size();
   if (count_[species] + n > config_.queue_capacity)
     throw std::length_error("em: enqueue exceeds the device queue capacity");
-  std::vector<std::int32_t> i32[3];
-  std::vector<std::uint32_t> u32[3];
-  std::vector<double> f64[9];
-  std::vector<std::uint64_t> u64[3];
-  for (auto& a : i32) a.resize(n);
-  for (auto& a : u32) a.resize(n);
-  for (auto& a : f64) a.resize(n);
-  for (auto& a : u64) a.resize(n);
-  for (std::size_t k = 0; k < n; ++k) {
-    auto const& p = v[k].particle;
-    i32[0][k] = p.pid;
-    i32[1][k] = p.medium_id;
-    i32[2][k] = p.environment_node;
-    u32[0][k] = p.generation;
-    u32[1][k] = p.reserved;
-    u32[2][k] = v[k].slot;
-    f64[0][k] = p.energy_gev;
-    f64[1][k] = p.position_m[0];
-    f64[2][k] = p.position_m[1];
-    f64[3][k] = p.position_m[2];
-    f64[4][k] = p.direction[0];
-    f64[5][k] = p.direction[1];
-    f64[6][k] = p.direction[2];
-    f64[7][k] = p.time_s;
-    f64[8][k] = p.weight;
-    u64[0][k] = p.history_id;
-    u64[1][k] = p.parent_history_id;
-    u64[2][k] = p.step_id;
+  C8MI_HIP(hipEventSynchronize(unstage_copied_));  // the previous batch has left the pinned buffer
+  if (unstage_host_.size() < n) {
+    std::size_t cap = unstage_host_.size() ? unstage_host_.size() : 1024;
+    while (cap < n) cap *= 2;
+    unstage_host_ = PinnedBuffer<StagedParticle>(cap);
+    unstage_dev_.allocate(cap);
   }
-  ParticleQueue const q = queues_[species][current_[species]]->view();
-  std::uint64_t const at = count_[species];
-  auto put = [&](void* dst, void const* src, std::size_t elem) {
-    C8MI_HIP(hipMemcpy(static_cast<unsigned char*>(dst) + at * elem, src, n * elem, hipMemcpyHostToDevice));
-  };
-  put(q.pid, i32[0].data(), 4);
-  put(q.medium, i32[1].data(), 4);
-  put(q.node, i32[2].data(), 4);
-  put(q.generation, u32[0].data(), 4);
-  put(q.reserved, u32[1].data(), 4);
-  put(q.shower, u32[2].data(), 4);
-  put(q.energy, f64[0].data(), 8);
-  put(q.x, f64[1].data(), 8);
-  put(q.y, f64[2].data(), 8);
-  put(q.z, f64[3].data(), 8);
-  put(q.dx, f64[4].data(), 8);
-  put(q.dy, f64[5].data(), 8);
-  put(q.dz, f64[6].data(), 8);
-  put(q.time, f64[7].data(), 8);
-  put(q.weight, f64[8].data(), 8);
-  put(q.history, u64[0].data(), 8);
-  put(q.parent, u64[1].data(), 8);
-  put(q.step, u64[2].data(), 8);
+  std::memcpy(unstage_host_.data(), v.data(), n * sizeof(StagedParticle));
+  C8MI_HIP(hipMemcpyAsync(unstage_dev_.data(), unstage_host_.data(), n * sizeof(StagedParticle),
+                          hipMemcpyHostToDevice, stream_));
+  C8MI_HIP(hipEventRecord(unstage_copied_, stream_));
+  launch_unstage(unstage_dev_.data(), n, queues_[species][current_[species]]->view(), count_[species], stream_);
   for (auto const& e : v) {
     --slot_host_[e.slot].staged[species];
     ++slot_host_[e.slot].pending[species];
@@ -430,6 +427,7 @@ void EmBackend::resize_scratch(std::uint64_t n) {
   packed_.allocate(cap);
   block_counts_.allocate(block_count_bytes(cap));
   offsets_.allocate(offset_bytes(cap) / sizeof(unsigned long long));
+  vertices_.allocate(cap);
   scratch_capacity_ = cap;
 }
 
@@ -464,30 +462,43 @@ void EmBackend::front(bool photon, c8mi_em_run_result& out) {
   ParticleQueue const other = queues_[o][current_[o]]->view();
   for (std::uint32_t k = 0; k < active_range_; ++k)
     if (slot_host_[k].pending[s]) ++slot_host_[k].fronts[s];
+  f.vertices = vertices_.data();
+  f.vertex_count = vertex_count_.data();
+  // the launches of this front, each followed by an event when timing
+  int kinds[kFrontKernels];
+  int launched = 0;
+  auto mark = [&](int kind) {
+    if (!timing_) return;
+    kinds[launched] = kind;
+    C8MI_HIP(hipEventRecord(events_[++launched], stream_));
+  };
   if (timing_) C8MI_HIP(hipEventRecord(events_[0], stream_));
-  launch_step(photon, context_.data(), config_.profile_bins, f, stream_);
-  if (timing_) C8MI_HIP(hipEventRecord(events_[1], stream_));
+  if (photon) {
+    launch_photon_step(context_.data(), config_.profile_bins, f, stream_);
+    mark(kKernPhotonStep);
+  } else {
+    launch_lepton_transport(context_.data(), config_.profile_bins, f, stream_);
+    mark(kKernLeptonTransport);
+    launch_lepton_vertex(context_.data(), config_.profile_bins, f, stream_);
+    mark(kKernLeptonVertex);
+  }
+  launch_count(f, stream_);
+  mark(kKernCount);
   launch_scan(photon, f, stream_);
-  if (timing_) C8MI_HIP(hipEventRecord(events_[2], stream_));
+  mark(kKernScan);
   launch_scatter(f, next, other, count_[o], stream_);
-  if (timing_) C8MI_HIP(hipEventRecord(events_[3], stream_));
+  mark(kKernScatter);
   C8MI_HIP(hipMemcpyAsync(host_report_.data(), report_.data(),
                           (kReportHeader + 2u * active_range_) * sizeof(unsigned long long),
                           hipMemcpyDeviceToHost, stream_));
   C8MI_HIP(hipStreamSynchronize(stream_));
   FrontTotals t;
   std::memcpy(&t, host_report_.data(), sizeof t);
-  if (timing_) {
-    float a = 0.f, b = 0.f, c = 0.f;
-    C8MI_HIP(hipEventElapsedTime(&a, events_[0], events_[1]));
-    C8MI_HIP(hipEventElapsedTime(&b, events_[1], events_[2]));
-    C8MI_HIP(hipEventElapsedTime(&c, events_[2], events_[3]));
-    ms_[photon ? 0 : 1] += a;
-    launches_[photon ? 0 : 1] += 1;
-    ms_[2] += b;
-    launches_[2] += 1;
-    ms_[3] += c;
-    launches_[3] += 1;
+  for (int k = 0; k < launched; ++k) {
+    float ms = 0.f;
+    C8MI_HIP(hipEventElapsedTime(&ms, events_[k], events_[k + 1]));
+    ms_[kinds[k]] += ms;
+    launches_[kinds[k]] += 1;
   }
   unsigned long long const* per_slot = host_report_.data() + kReportHeader;
   for (std::uint32_t k = 0; k < active_range_; ++k) {
@@ -515,6 +526,22 @@ void EmBackend::front(bool photon, c8mi_em_run_result& out) {
 
 void EmBackend::run(int species, std::uint64_t maximum_fronts, c8mi_em_run_result& out) {
   std::memset(&out, 0, sizeof out);
+  run_fronts(species, maximum_fronts, out);
+  std::vector<unsigned long long> records(2u * slots_);
+  C8MI_HIP(hipMemcpy(records.data(), record_counts_.data(), records.size() * sizeof(unsigned long long),
+                     hipMemcpyDeviceToHost));
+  for (std::uint32_t k = 0; k < active_range_; ++k) {
+    out.observations += records[2 * k];
+    out.fallbacks += records[2 * k + 1];
+  }
+  pending(out.pending_photons, out.pending_leptons);
+  unsigned long long next_history = 0;
+  C8MI_HIP(hipMemcpy(&next_history, history_.data(), sizeof next_history, hipMemcpyDeviceToHost));
+  out.next_history_id = next_history;
+}
+
+// The front loop alone: no record or history read-back.
+void EmBackend::run_fronts(int species, std::uint64_t maximum_fronts, c8mi_em_run_result& out) {
   auto has = [&](int s) { return count_[s] + staged_[s].size() > 0; };
   for (std::uint64_t round = 0; round < maximum_fronts; ++round) {
     if (species == 0) {
@@ -527,17 +554,6 @@ void EmBackend::run(int species, std::uint64_t maximum_fronts, c8mi_em_run_resul
       front(s == 0, out);
     }
   }
-  std::vector<unsigned long long> records(2u * slots_);
-  C8MI_HIP(hipMemcpy(records.data(), record_counts_.data(), records.size() * sizeof(unsigned long long),
-                     hipMemcpyDeviceToHost));
-  for (std::uint32_t k = 0; k < active_range_; ++k) {
-    out.observations += records[2 * k];
-    out.fallbacks += records[2 * k + 1];
-  }
-  pending(out.pending_photons, out.pending_leptons);
-  unsigned long long next_history = 0;
-  C8MI_HIP(hipMemcpy(&next_history, history_.data(), sizeof next_history, hipMemcpyDeviceToHost));
-  out.next_history_id = next_history;
 }
 
 void EmBackend::pending(std::uint64_t& photons, std::uint64_t& leptons) const {
@@ -612,8 +628,15 @@ void EmBackend::slot_download_profile(std::uint32_t slot, double* photons, doubl
   std::vector<long long> c(kCounters);
   C8MI_HIP(hipMemcpy(c.data(), counters_.data() + static_cast<std::uint64_t>(slot) * kCounters,
                      kCounters * sizeof(long long), hipMemcpyDeviceToHost));
-  c[kInjected] += static_cast<long long>(slot_host_[slot].injected);
   c[kPending] += pending_energy(slot);
+  decode_ledger(c.data(), slot, *ledger);
+}
+
+// Ledger of a slot from its device counters (the host adds the injected energy
+// it accumulated at enqueue).
+void EmBackend::decode_ledger(long long* c, std::uint32_t slot, c8mi_em_ledger& out) const {
+  c8mi_em_ledger* ledger = &out;
+  c[kInjected] += static_cast<long long>(slot_host_[slot].injected);
   double const es = config_.profile_energy_scale;
   double* energy = &ledger->injected_energy;
   for (int k = 0; k < kEnergyTerms; ++k) energy[k] = static_cast<double>(c[k]) / es;
@@ -663,8 +686,7 @@ void EmBackend::serve(c8mi_em_request const* requests, std::uint64_t count, c8mi
     if (h.pending[0] || h.pending[1] || h.staged[0] || h.staged[1])
       throw std::logic_error("em: serve needs a pool without pending particles");
   }
-  for (std::uint32_t slot = 0; slot < slots_; ++slot) slot_host_[slot].active = false;
-  update_active_range();
+  std::uint64_t axis_words = 0;
   for (std::uint64_t k = 0; k < count; ++k) {
     c8mi_em_request const& r = requests[k];
     if (!std::isfinite(r.arrival_s) || (k && r.arrival_s < requests[k - 1].arrival_s))
@@ -673,32 +695,76 @@ void EmBackend::serve(c8mi_em_request const* requests, std::uint64_t count, c8mi
       throw std::invalid_argument("em: request footprint");
     if (!can_transport(r.primary))
       throw std::invalid_argument("em: request " + std::to_string(k) + " has a primary the device cannot transport");
-    if (r.axis.grammage_g_per_cm2) validate_axis(r.axis);
+    if (r.axis.grammage_g_per_cm2) {
+      validate_axis(r.axis);
+      axis_words += r.axis.support_count;
+    }
     std::memset(&results[k], 0, sizeof results[k]);
   }
+  for (std::uint32_t slot = 0; slot < slots_; ++slot) slot_host_[slot].active = false;
+  update_active_range();
+  // every request's projection table, uploaded once
+  C8MI_HIP(hipStreamSynchronize(stream_));
+  DeviceBuffer<double> axes(axis_words ? axis_words : 1);
+  std::vector<double const*> axis_of(count, nullptr);
+  {
+    std::vector<double> host(axis_words);
+    std::uint64_t at = 0;
+    for (std::uint64_t k = 0; k < count; ++k) {
+      c8mi_em_shower_axis const& a = requests[k].axis;
+      if (!a.grammage_g_per_cm2) continue;
+      std::memcpy(host.data() + at, a.grammage_g_per_cm2, a.support_count * sizeof(double));
+      axis_of[k] = axes.data() + at;
+      at += a.support_count;
+    }
+    if (axis_words)
+      C8MI_HIP(hipMemcpy(axes.data(), host.data(), axis_words * sizeof(double), hipMemcpyHostToDevice));
+  }
+
   using clock = std::chrono::steady_clock;
   auto const t0 = clock::now();
   auto elapsed = [&] { return std::chrono::duration<double>(clock::now() - t0).count(); };
   std::vector<std::uint32_t> free_slots;
   for (std::uint32_t s = slots_; s-- > 0;) free_slots.push_back(s);
   std::vector<std::int64_t> owner(slots_, -1);
+  std::vector<SlotInit> inits;
+  std::vector<std::uint32_t> finished;
   std::uint64_t next = 0, active = 0;
   double footprint = 0.;
   while (next < count || active) {
-    double now = elapsed();
+    double const now = elapsed();
+    inits.clear();
     while (next < count && !free_slots.empty() && (!options.realtime || requests[next].arrival_s <= now) &&
            (active == 0 || footprint + requests[next].footprint_particles <= options.footprint_budget_particles)) {
       c8mi_em_request const& r = requests[next];
       std::uint32_t const slot = free_slots.back();
       free_slots.pop_back();
-      slot_begin(slot, r.shower_id, r.first_history_id, r.axis.grammage_g_per_cm2 ? &r.axis : nullptr);
+      SlotInit init{};
+      ShowerSlot& sh = init.shower;
+      c8mi_em_shower_axis const* a = r.axis.grammage_g_per_cm2 ? &r.axis : nullptr;
+      for (int d = 0; d < 3; ++d) {
+        sh.axis_start[d] = a ? a->start_m[d] : config_.axis_start_m[d];
+        sh.axis_direction[d] = a ? a->direction[d] : config_.axis_direction[d];
+      }
+      sh.axis_step = a ? a->step_m : config_.axis_step_m;
+      sh.axis_grammage = a ? axis_of[next] : axis_.data();
+      sh.axis_support_count = a ? a->support_count : config_.axis_support_count;
+      sh.key = r.shower_id;
+      init.first_history = r.first_history_id;
+      init.slot = slot;
+      inits.push_back(init);
+      activate(slot, r.shower_id);
       slot_enqueue(slot, &r.primary, 1);
       owner[slot] = static_cast<std::int64_t>(next);
-      results[next].admitted_s = elapsed();
+      results[next].admitted_s = now;
       results[next].slot = slot;
       footprint += r.footprint_particles;
       ++active;
       ++next;
+    }
+    if (!inits.empty()) {
+      start_slots(inits.data(), static_cast<std::uint32_t>(inits.size()));
+      update_active_range();
     }
     if (active == 0) {  // idle until the next arrival
       double const wait = requests[next].arrival_s - elapsed();
@@ -706,7 +772,7 @@ void EmBackend::serve(c8mi_em_request const* requests, std::uint64_t count, c8mi
       continue;
     }
     c8mi_em_run_result round{};
-    run(0, 1, round);
+    run_fronts(0, 1, round);
     totals.photon_fronts += round.photon_fronts;
     totals.lepton_fronts += round.lepton_fronts;
     totals.photon_steps += round.photon_steps;
@@ -715,32 +781,51 @@ void EmBackend::serve(c8mi_em_request const* requests, std::uint64_t count, c8mi
     totals.lepton_interactions += round.lepton_interactions;
     totals.secondaries += round.secondaries;
     totals.max_front = std::max(totals.max_front, round.max_front);
+    finished.clear();
     for (std::uint32_t slot = 0; slot < slots_; ++slot) {
       if (owner[slot] < 0) continue;
       SlotHost const& h = slot_host_[slot];
-      if (h.pending[0] || h.pending[1] || h.staged[0] || h.staged[1]) continue;
+      if (!(h.pending[0] || h.pending[1] || h.staged[0] || h.staged[1])) finished.push_back(slot);
+    }
+    if (finished.empty()) continue;
+    // one gather and one read-back for every shower that ended this round
+    std::uint32_t const n = static_cast<std::uint32_t>(finished.size());
+    std::memcpy(gather_slots_host_.data(), finished.data(), n * sizeof(std::uint32_t));
+    C8MI_HIP(hipMemcpyAsync(gather_slots_dev_.data(), gather_slots_host_.data(), n * sizeof(std::uint32_t),
+                            hipMemcpyHostToDevice, stream_));
+    launch_slot_gather(pool_arrays(), gather_slots_dev_.data(), n, gather_dev_.data(), stream_);
+    C8MI_HIP(hipMemcpyAsync(gather_host_.data(), gather_dev_.data(),
+                            static_cast<std::size_t>(n) * kGatherWords * sizeof(unsigned long long),
+                            hipMemcpyDeviceToHost, stream_));
+    C8MI_HIP(hipStreamSynchronize(stream_));
+    double const done = elapsed();
+    for (std::uint32_t k = 0; k < n; ++k) {
+      std::uint32_t const slot = finished[k];
+      unsigned long long const* g = gather_host_.data() + static_cast<std::size_t>(k) * kGatherWords;
       c8mi_em_request_result& out = results[owner[slot]];
-      out.finished_s = elapsed();
-      c8mi_em_slot_state st;
-      slot_state(slot, st);
-      out.photon_steps = st.photon_steps;
-      out.lepton_steps = st.lepton_steps;
-      out.photon_interactions = st.photon_interactions;
-      out.lepton_interactions = st.lepton_interactions;
-      out.secondaries = st.secondaries;
-      out.next_history_id = st.next_history_id;
-      out.observations = st.observations;
-      out.fallbacks = st.fallbacks;
-      slot_download_profile(slot, nullptr, nullptr, nullptr, nullptr, &out.ledger);
-      totals.observations += st.observations;
-      totals.fallbacks += st.fallbacks;
-      slot_end(slot);
+      out.finished_s = done;
+      out.photon_steps = g[kGatherStats + 0];
+      out.lepton_steps = g[kGatherStats + 1];
+      out.photon_interactions = g[kGatherStats + 2];
+      out.lepton_interactions = g[kGatherStats + 3];
+      out.secondaries = g[kGatherStats + 4];
+      out.next_history_id = g[kGatherHistory];
+      out.observations = g[kGatherRecords];
+      out.fallbacks = g[kGatherRecords + 1];
+      long long c[kCounters];
+      for (int j = 0; j < kCounters; ++j) c[j] = static_cast<long long>(g[kGatherCounters + j]);
+      decode_ledger(c, slot, out.ledger);
+      totals.observations += out.observations;
+      totals.fallbacks += out.fallbacks;
+      slot_host_[slot].active = false;
       footprint -= requests[owner[slot]].footprint_particles;
       owner[slot] = -1;
       free_slots.push_back(slot);
       --active;
     }
+    update_active_range();
   }
+  C8MI_HIP(hipStreamSynchronize(stream_));  // nothing may read `axes` after it is freed
 }
 
 void EmBackend::enable_timing(bool enable) {
@@ -749,8 +834,8 @@ void EmBackend::enable_timing(bool enable) {
   launches_.fill(0);
 }
 
-void EmBackend::kernel_time(double ms[4], std::uint64_t launches[4]) const {
-  for (int k = 0; k < 4; ++k) {
+void EmBackend::kernel_time(double ms[kFrontKernels], std::uint64_t launches[kFrontKernels]) const {
+  for (int k = 0; k < kFrontKernels; ++k) {
     ms[k] = ms_[k];
     launches[k] = launches_[k];
   }

@@ -79,13 +79,10 @@ class EmBackend {
              c8mi_em_request_result* results, c8mi_em_run_result& totals);
 
   void enable_timing(bool enable);
-  void kernel_time(double ms[4], std::uint64_t launches[4]) const;
+  void kernel_time(double ms[kFrontKernels], std::uint64_t launches[kFrontKernels]) const;
 
  private:
-  struct Staged {
-    c8mi_particle_state particle;
-    std::uint32_t slot;
-  };
+  using Staged = StagedParticle;
   struct SlotHost {
     bool active = false;
     std::uint64_t shower_id = 0;
@@ -97,7 +94,13 @@ class EmBackend {
   };
 
   void reset_pool();
+  PoolArrays pool_arrays() const;
+  // Enqueues the start of the given slots (no host synchronisation).
+  void start_slots(SlotInit const* inits, std::uint32_t count);
+  void activate(std::uint32_t slot, std::uint64_t shower_id);
+  void decode_ledger(long long* counters, std::uint32_t slot, c8mi_em_ledger& out) const;
   void front(bool photon, c8mi_em_run_result& out);
+  void run_fronts(int species, std::uint64_t maximum_fronts, c8mi_em_run_result& out);
   void flush_staged(int species);
   void append_run(int species, std::uint64_t end);
   void resize_scratch(std::uint64_t n);
@@ -131,6 +134,17 @@ class EmBackend {
   DeviceBuffer<unsigned long long> adjust_;
   DeviceBuffer<unsigned long long> report_;
   PinnedBuffer<unsigned long long> host_report_;
+  // batched slot starts and result gathers
+  DeviceBuffer<SlotInit> init_dev_;
+  PinnedBuffer<SlotInit> init_host_;
+  DeviceBuffer<std::uint32_t> gather_slots_dev_;
+  PinnedBuffer<std::uint32_t> gather_slots_host_;
+  DeviceBuffer<unsigned long long> gather_dev_;
+  PinnedBuffer<unsigned long long> gather_host_;
+  hipEvent_t init_copied_ = nullptr;
+  DeviceBuffer<StagedParticle> unstage_dev_;
+  PinnedBuffer<StagedParticle> unstage_host_;
+  hipEvent_t unstage_copied_ = nullptr;
   // queues: [species][buffer]; species 0 photons, 1 leptons
   std::unique_ptr<QueueStorage> queues_[2][2];
   int current_[2] = {0, 0};
@@ -142,12 +156,14 @@ class EmBackend {
   DeviceBuffer<std::uint32_t> packed_;
   DeviceBuffer<unsigned char> block_counts_;
   DeviceBuffer<unsigned long long> offsets_;
+  DeviceBuffer<VertexRecord> vertices_;
+  DeviceBuffer<unsigned long long> vertex_count_;
   std::uint64_t scratch_capacity_ = 0;
   // timing
   bool timing_ = false;
-  std::array<double, 4> ms_{};
-  std::array<std::uint64_t, 4> launches_{};
-  hipEvent_t events_[4] = {nullptr, nullptr, nullptr, nullptr};
+  std::array<double, kFrontKernels> ms_{};
+  std::array<std::uint64_t, kFrontKernels> launches_{};
+  hipEvent_t events_[kFrontKernels + 1] = {};
 };
 
 }  // namespace c8mi::em

@@ -174,12 +174,16 @@ __device__ __forceinline__ void store_particle(ParticleQueue const& q, std::uint
 enum SlotKind : std::uint32_t { kSlotNone = 0, kSlotNextContinue = 1, kSlotNextChild = 2, kSlotOtherChild = 3 };
 
 // Packed per-source counts: bits 0-1 next, 2-3 other, 4-5 children, 6-7 slot0
-// kind, 8-9 slot1 kind, 12-31 pool slot of the source's shower.
+// kind, 8-9 slot1 kind, 10 a transport step was made, 11 an interaction
+// vertex was reached, 12-31 pool slot of the source's shower.
 inline constexpr std::uint32_t kMaxShowerSlots = 1024;
+inline constexpr int kPackedSlotShift = 12;
 __host__ __device__ __forceinline__ std::uint32_t pack_counts(std::uint32_t next, std::uint32_t other,
                                                               std::uint32_t child, std::uint32_t k0,
-                                                              std::uint32_t k1, std::uint32_t shower) {
-  return next | (other << 2) | (child << 4) | (k0 << 6) | (k1 << 8) | (shower << 12);
+                                                              std::uint32_t k1, bool step, bool interaction,
+                                                              std::uint32_t shower) {
+  return next | (other << 2) | (child << 4) | (k0 << 6) | (k1 << 8) | (step ? 1u << 10 : 0u) |
+         (interaction ? 1u << 11 : 0u) | (shower << kPackedSlotShift);
 }
 
 // Front totals written by the scan kernel and read back by the host.

@@ -1,22 +1,31 @@
-// Wavefront kernels of the EM backend. One front of one species is three
-// launches on the backend's stream:
-//   1. step     one thread per queued particle: the whole step (selection,
-//               transport, vertex, final state) of PhotonFront.inl /
-//               LeptonFront.inl fused into one kernel; up to two output
-//               particles go to a per-source staging pair; profile and ledger
-//               contributions accumulate in block-private LDS copies (one for
-//               the shower of the block's first particle, one for that of its
-//               last) that are added to the shower's device totals once per
-//               block; per-block output counts and per-(slot, run) output
-//               totals are reduced for the scan.
-//   2. scan     one workgroup: exclusive scan of the per-block counts, and per
-//               slot the position and history offsets of every run's outputs.
-//   3. scatter  stable compaction: the block prefix of the per-source counts
-//               plus the block offset gives the source's global output rank;
-//               the (slot, run) offset turns it into its rank inside its own
-//               shower, so each shower's queue segment, order and history IDs
-//               (first + child rank + k) are those of the shower run alone, as
-//               the reference's stable scans give them.
+// Wavefront kernels of the EM backend. One front of one species is a short
+// chain of launches on the backend's stream:
+//   step       one thread per queued particle. Photons: the whole step
+//              (selection, transport, vertex, final state) of PhotonFront.inl
+//              fused into one kernel. e+-: the transport part (selection,
+//              continuous step, transport, profile) of LeptonFront.inl; a step
+//              that ends at an interaction vertex is compacted into a vertex
+//              list (wave ballot, one atomic per workgroup), and
+//   vertex     one thread per listed vertex runs the vertex reselection and the
+//              final state, so the ~7% of e+- steps that interact run in full
+//              wavefronts instead of holding every wavefront of the transport
+//              kernel.
+//              Up to two outputs per source go to a per-source staging pair
+//              and a packed count word; profile and ledger contributions
+//              accumulate in workgroup-private LDS copies (one for the shower
+//              of the workgroup's first source, one for its last) that are
+//              added to the showers' device totals once per workgroup.
+//   count      per-workgroup output counts and per-(slot, run) output totals
+//              from the packed words.
+//   scan       one workgroup: exclusive scan of the per-workgroup counts, and
+//              per slot the position and history offsets of every run's
+//              outputs.
+//   scatter    stable compaction: the block prefix of the per-source counts
+//              plus the block offset gives the source's global output rank;
+//              the (slot, run) offset turns it into its rank inside its own
+//              shower, so each shower's queue segment, order and history IDs
+//              (first + child rank + k) are those of the shower run alone, as
+//              the reference's stable scans give them.
 // Queues are sorted by slot within each run (the previous front's output is
 // one run, the other species' secondaries and host-staged particles each
 // append one).
@@ -38,6 +47,9 @@ constexpr int kStepBlock = 256;
 constexpr int kScanBlock = 1024;
 constexpr std::uint32_t kNoKey = 0xffffffffu;
 constexpr int kRunShift = 20;
+// Vertex kernel grid: one workgroup per 2048 front particles (a chunk of 256
+// vertices per workgroup and pass), more vertices are covered by the stride loop.
+constexpr std::uint64_t kSourcesPerVertexBlock = 2048;
 
 struct BlockCounts {
   std::uint32_t next, other, children, steps, interactions, pad[3];
@@ -59,53 +71,186 @@ __device__ __forceinline__ T wave_inclusive(T x, std::uint32_t lane) {
   return x;
 }
 
-template <bool Photon>
-__global__ __launch_bounds__(kStepBlock) void step_kernel(EmContext const* __restrict__ context, FrontArgs f) {
-  extern __shared__ long long lds[];
-  __shared__ std::uint32_t keys[kStepBlock];
-  __shared__ std::uint32_t wave_sums[kStepBlock / 64][kShowerFields];
+// Workgroup-private LDS copies of the profile and ledger of up to two showers.
+struct LdsShowers {
+  long long* base;
+  std::uint32_t stride;  // 4 * bins + kCounters
+  std::uint32_t sh0, sh1;
+  std::uint32_t copies;
+};
+
+__device__ __forceinline__ LdsShowers lds_showers(long long* lds, std::uint32_t bins, std::uint32_t sh0,
+                                                  std::uint32_t sh1) {
+  LdsShowers l{lds, 4u * bins + kCounters, sh0, sh1, sh0 == sh1 ? 1u : 2u};
+  for (std::uint32_t k = threadIdx.x; k < l.copies * l.stride; k += blockDim.x) lds[k] = 0;
+  return l;
+}
+
+__device__ __forceinline__ phys::Accumulator accumulator(EmContext const* context, LdsShowers const& l,
+                                                         std::uint32_t s) {
   EmContext const& c = *context;
   std::uint32_t const bins = c.profile_bins;
-  std::uint32_t const stride = 4u * bins + kCounters;
+  long long* hist = s == l.sh0 ? l.base : s == l.sh1 ? l.base + l.stride : nullptr;
+  return phys::Accumulator{hist ? hist : c.profile + static_cast<std::uint64_t>(s) * 4u * bins,
+                           hist ? hist + 4u * bins : c.counters + static_cast<std::uint64_t>(s) * kCounters,
+                           context, &c.showers[s], s};
+}
+
+// Adds the workgroup's LDS copies to the showers' device totals (after a barrier).
+__device__ __forceinline__ void flush(EmContext const& c, LdsShowers const& l) {
+  std::uint32_t const bins = c.profile_bins;
+  for (std::uint32_t copy = 0; copy < l.copies; ++copy) {
+    std::uint32_t const s = copy ? l.sh1 : l.sh0;
+    long long const* src = l.base + copy * l.stride;
+    long long* hist = c.profile + static_cast<std::uint64_t>(s) * 4u * bins;
+    long long* counters = c.counters + static_cast<std::uint64_t>(s) * kCounters;
+    for (std::uint32_t k = threadIdx.x; k < l.stride; k += blockDim.x) {
+      long long const value = src[k];
+      if (value == 0) continue;
+      long long* dst = k < 4u * bins ? hist + k : counters + (k - 4u * bins);
+      atomicAdd(reinterpret_cast<unsigned long long*>(dst), static_cast<unsigned long long>(value));
+    }
+  }
+}
+
+__device__ __forceinline__ void write_outputs(FrontArgs const& f, std::uint64_t i, phys::Outputs const& o,
+                                              std::uint32_t s) {
+  f.packed[i] = pack_counts(o.next, o.other, o.children, o.kind[0], o.kind[1], o.step, o.interaction, s);
+}
+
+__device__ __forceinline__ phys::Outputs no_outputs(FrontArgs const& f, std::uint64_t i) {
+  phys::Outputs o;
+  o.staging = f.staging + 2 * i;
+  o.kind[0] = o.kind[1] = kSlotNone;
+  o.next = o.other = o.children = 0;
+  o.step = o.interaction = false;
+  return o;
+}
+
+__global__ __launch_bounds__(kStepBlock) void photon_step_kernel(EmContext const* __restrict__ context,
+                                                                 FrontArgs f) {
+  extern __shared__ long long lds[];
+  EmContext const& c = *context;
   std::uint64_t const first = static_cast<std::uint64_t>(blockIdx.x) * kStepBlock;
   std::uint64_t const last = (first + kStepBlock < f.count ? first + kStepBlock : f.count) - 1;
-  std::uint32_t const sh0 = f.in.shower[first], sh1 = f.in.shower[last];
-  std::uint32_t const copies = sh0 == sh1 ? 1u : 2u;
-  for (std::uint32_t k = threadIdx.x; k < copies * stride; k += kStepBlock) lds[k] = 0;
+  LdsShowers const l = lds_showers(lds, c.profile_bins, f.in.shower[first], f.in.shower[last]);
   __syncthreads();
-
   std::uint64_t const i = first + threadIdx.x;
-  std::uint32_t v[kShowerFields] = {0, 0, 0, 0, 0};
-  std::uint32_t key = kNoKey;
   if (i < f.count) {
     std::uint32_t const s = f.in.shower[i];
-    long long* hist;
-    if (s == sh0) hist = lds;
-    else if (s == sh1) hist = lds + stride;
-    else hist = nullptr;
-    phys::Accumulator const acc{
-        hist ? hist : c.profile + static_cast<std::uint64_t>(s) * 4u * bins,
-        hist ? hist + 4u * bins : c.counters + static_cast<std::uint64_t>(s) * kCounters, context,
-        &c.showers[s], s};
+    phys::Accumulator const acc = accumulator(context, l, s);
     phys::Particle p;
     static_cast<c8mi_particle_state&>(p) = load_particle(f.in, i);
     p.shower_key = c.showers[s].key;
-    phys::Outputs o;
-    o.kind[0] = o.kind[1] = kSlotNone;
-    o.next = o.other = o.children = 0;
-    o.step = o.interaction = false;
-    if constexpr (Photon) phys::photon_step(c, acc, p, o);
-    else phys::lepton_step(c, acc, p, o);
-    if (o.kind[0] != kSlotNone) f.staging[2 * i] = o.slot[0];
-    if (o.kind[1] != kSlotNone) f.staging[2 * i + 1] = o.slot[1];
-    f.packed[i] = pack_counts(o.next, o.other, o.children, o.kind[0], o.kind[1], s);
-    v[0] = o.next;
-    v[1] = o.other;
-    v[2] = o.children;
-    v[3] = o.step ? 1u : 0u;
-    v[4] = o.interaction ? 1u : 0u;
-    key = (run_of(f, i) << kRunShift) | s;
+    phys::Outputs o = no_outputs(f, i);
+    phys::photon_step(c, acc, p, o);
+    write_outputs(f, i, o, s);
   }
+  __syncthreads();
+  flush(c, l);
+}
+
+// Three waves per SIMD (168 VGPRs, a little scratch) measured faster than two
+// for the transport part; the photon and vertex kernels keep two and one.
+__global__ __launch_bounds__(kStepBlock, 3) void lepton_transport_kernel(EmContext const* __restrict__ context,
+                                                                      FrontArgs f) {
+  extern __shared__ long long lds[];
+  __shared__ std::uint32_t wave_vertices[kStepBlock / 64];
+  __shared__ unsigned long long block_base;
+  EmContext const& c = *context;
+  std::uint64_t const first = static_cast<std::uint64_t>(blockIdx.x) * kStepBlock;
+  std::uint64_t const last = (first + kStepBlock < f.count ? first + kStepBlock : f.count) - 1;
+  LdsShowers const l = lds_showers(lds, c.profile_bins, f.in.shower[first], f.in.shower[last]);
+  __syncthreads();
+  std::uint64_t const i = first + threadIdx.x;
+  bool deferred = false;
+  std::uint32_t s = 0;
+  phys::LeptonVertex vx;
+  if (i < f.count) {
+    s = f.in.shower[i];
+    phys::Accumulator const acc = accumulator(context, l, s);
+    phys::Particle p;
+    static_cast<c8mi_particle_state&>(p) = load_particle(f.in, i);
+    p.shower_key = c.showers[s].key;
+    phys::Outputs o = no_outputs(f, i);
+    deferred = phys::lepton_transport(c, acc, p, o, vx);
+    write_outputs(f, i, o, s);  // a deferred source's word is rewritten by the vertex kernel
+  }
+  // compact the vertices of the workgroup, in source order
+  std::uint32_t const wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+  unsigned long long const mask = __ballot(deferred);
+  if (lane == 0) wave_vertices[wave] = static_cast<std::uint32_t>(__popcll(mask));
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    std::uint32_t total = 0;
+    for (int w = 0; w < kStepBlock / 64; ++w) total += wave_vertices[w];
+    block_base = total ? atomicAdd(f.vertex_count, static_cast<unsigned long long>(total)) : 0ULL;
+  }
+  flush(c, l);
+  __syncthreads();
+  if (deferred) {
+    std::uint32_t before = 0;
+    for (std::uint32_t w = 0; w < wave; ++w) before += wave_vertices[w];
+    unsigned long long const below = lane ? (mask & (~0ULL >> (64 - lane))) : 0ULL;
+    VertexRecord r;
+    r.parent = vx.parent;
+    r.shower_key = vx.parent.shower_key;
+    r.source = i;
+    r.total_rate = vx.total_rate;
+    r.process_uniform = vx.process_uniform;
+    r.particle_mass_gev = vx.particle_mass_gev;
+    r.vertex_density = vx.vertex_density;
+    r.slot = s;
+    r.pad[0] = r.pad[1] = r.pad[2] = 0;
+    f.vertices[block_base + before + static_cast<std::uint32_t>(__popcll(below))] = r;
+  }
+}
+
+__global__ __launch_bounds__(kStepBlock) void lepton_vertex_kernel(EmContext const* __restrict__ context,
+                                                                   FrontArgs f) {
+  extern __shared__ long long lds[];
+  EmContext const& c = *context;
+  std::uint64_t const vertices = *f.vertex_count;
+  for (std::uint64_t first = static_cast<std::uint64_t>(blockIdx.x) * kStepBlock; first < vertices;
+       first += static_cast<std::uint64_t>(gridDim.x) * kStepBlock) {
+    std::uint64_t const last = (first + kStepBlock < vertices ? first + kStepBlock : vertices) - 1;
+    LdsShowers const l = lds_showers(lds, c.profile_bins, f.vertices[first].slot, f.vertices[last].slot);
+    __syncthreads();
+    std::uint64_t const j = first + threadIdx.x;
+    if (j < vertices) {
+      VertexRecord const& r = f.vertices[j];
+      std::uint32_t const s = r.slot;
+      phys::Accumulator const acc = accumulator(context, l, s);
+      phys::LeptonVertex vx;
+      static_cast<c8mi_particle_state&>(vx.parent) = r.parent;
+      vx.parent.shower_key = r.shower_key;
+      vx.total_rate = r.total_rate;
+      vx.process_uniform = r.process_uniform;
+      vx.particle_mass_gev = r.particle_mass_gev;
+      vx.vertex_density = r.vertex_density;
+      phys::Outputs o = no_outputs(f, r.source);
+      o.step = o.interaction = true;
+      phys::lepton_vertex(c, acc, vx, o);
+      write_outputs(f, r.source, o, s);
+    }
+    __syncthreads();
+    flush(c, l);
+    __syncthreads();  // the LDS copies are cleared for the next chunk
+  }
+}
+
+// Per-workgroup totals and per-(slot, run) totals of the packed counts. Within
+// a workgroup the keys are non-decreasing segments; the first source of a
+// segment subtracts its exclusive block prefix and the last adds its
+// inclusive prefix (arithmetic modulo 2^64).
+__global__ __launch_bounds__(kStepBlock) void count_kernel(FrontArgs f) {
+  __shared__ std::uint32_t keys[kStepBlock];
+  __shared__ std::uint32_t wave_sums[kStepBlock / 64][kShowerFields];
+  std::uint64_t const i = static_cast<std::uint64_t>(blockIdx.x) * kStepBlock + threadIdx.x;
+  std::uint32_t const word = i < f.count ? f.packed[i] : 0u;
+  std::uint32_t const v[kShowerFields] = {word & 3u, (word >> 2) & 3u, (word >> 4) & 3u, (word >> 10) & 1u,
+                                          (word >> 11) & 1u};
+  std::uint32_t const key = i < f.count ? (run_of(f, i) << kRunShift) | (word >> kPackedSlotShift) : kNoKey;
   keys[threadIdx.x] = key;
   std::uint32_t const wave = threadIdx.x / 64, lane = threadIdx.x % 64;
   std::uint32_t inc[kShowerFields];
@@ -114,43 +259,26 @@ __global__ __launch_bounds__(kStepBlock) void step_kernel(EmContext const* __res
   if (lane == 63)
     for (int k = 0; k < kShowerFields; ++k) wave_sums[wave][k] = inc[k];
   __syncthreads();
-
   if (threadIdx.x == 0) {
     std::uint32_t t[kShowerFields] = {0, 0, 0, 0, 0};
     for (int w = 0; w < kStepBlock / 64; ++w)
       for (int k = 0; k < kShowerFields; ++k) t[k] += wave_sums[w][k];
     static_cast<BlockCounts*>(f.block_counts)[blockIdx.x] = BlockCounts{t[0], t[1], t[2], t[3], t[4], {0, 0, 0}};
+    if (blockIdx.x == 0) *f.vertex_count = 0;  // the vertex kernel of this front has finished
   }
-  // Per-(slot, run) totals: within the block the keys are non-decreasing
-  // segments; the first source of a segment subtracts its exclusive block
-  // prefix and the last adds its inclusive prefix (arithmetic modulo 2^64).
-  if (key != kNoKey) {
-    std::uint32_t const s = key & ((1u << kRunShift) - 1u), r = key >> kRunShift;
-    unsigned long long* t =
-        f.shower_totals + (static_cast<std::uint64_t>(s) * kMaxRuns + r) * kShowerFields;
-    bool const start = threadIdx.x == 0 || keys[threadIdx.x - 1] != key;
-    bool const end = threadIdx.x == kStepBlock - 1 || keys[threadIdx.x + 1] != key;
+  if (key == kNoKey) return;
+  std::uint32_t const s = key & ((1u << kRunShift) - 1u), r = key >> kRunShift;
+  unsigned long long* t = f.shower_totals + (static_cast<std::uint64_t>(s) * kMaxRuns + r) * kShowerFields;
+  bool const start = threadIdx.x == 0 || keys[threadIdx.x - 1] != key;
+  bool const end = threadIdx.x == kStepBlock - 1 || keys[threadIdx.x + 1] != key;
+  if (!start && !end) return;
 #pragma unroll
-    for (int k = 0; k < kShowerFields; ++k) {
-      std::uint32_t before = 0;
-      for (std::uint32_t w = 0; w < wave; ++w) before += wave_sums[w][k];
-      std::uint32_t const incl = before + inc[k], excl = incl - v[k];
-      if (start && excl != 0) atomicAdd(&t[k], 0ULL - excl);
-      if (end && incl != 0) atomicAdd(&t[k], static_cast<unsigned long long>(incl));
-    }
-  }
-  // flush the block's profile and ledger contributions to the showers' totals
-  for (std::uint32_t copy = 0; copy < copies; ++copy) {
-    std::uint32_t const s = copy ? sh1 : sh0;
-    long long const* src = lds + copy * stride;
-    long long* hist = c.profile + static_cast<std::uint64_t>(s) * 4u * bins;
-    long long* counters = c.counters + static_cast<std::uint64_t>(s) * kCounters;
-    for (std::uint32_t k = threadIdx.x; k < stride; k += kStepBlock) {
-      long long const value = src[k];
-      if (value == 0) continue;
-      long long* dst = k < 4u * bins ? hist + k : counters + (k - 4u * bins);
-      atomicAdd(reinterpret_cast<unsigned long long*>(dst), static_cast<unsigned long long>(value));
-    }
+  for (int k = 0; k < kShowerFields; ++k) {
+    std::uint32_t before = 0;
+    for (std::uint32_t w = 0; w < wave; ++w) before += wave_sums[w][k];
+    std::uint32_t const incl = before + inc[k], excl = incl - v[k];
+    if (start && excl != 0) atomicAdd(&t[k], 0ULL - excl);
+    if (end && incl != 0) atomicAdd(&t[k], static_cast<unsigned long long>(incl));
   }
 }
 
@@ -278,7 +406,7 @@ __global__ __launch_bounds__(kStepBlock) void scatter_kernel(FrontArgs f, Partic
     for (int k = 0; k < 3; ++k) wave_sums[wave][k] = inc[k];
   __syncthreads();
   if (i >= f.count) return;
-  std::uint32_t const s = word >> 12;
+  std::uint32_t const s = word >> kPackedSlotShift;
   unsigned long long const* adj = f.adjust + (static_cast<std::uint64_t>(s) * kMaxRuns + run_of(f, i)) * 3;
   std::uint64_t pos[3];
   for (int k = 0; k < 3; ++k) {
@@ -303,6 +431,40 @@ __global__ __launch_bounds__(kStepBlock) void scatter_kernel(FrontArgs f, Partic
   }
 }
 
+__global__ __launch_bounds__(256) void unstage_kernel(StagedParticle const* __restrict__ staged, std::uint64_t n,
+                                                      ParticleQueue queue, std::uint64_t at) {
+  std::uint64_t const i = static_cast<std::uint64_t>(blockIdx.x) * 256 + threadIdx.x;
+  if (i >= n) return;
+  store_particle(queue, at + i, staged[i].particle);
+  queue.shower[at + i] = staged[i].slot;
+}
+
+__global__ __launch_bounds__(256) void slot_init_kernel(PoolArrays pool, SlotInit const* __restrict__ inits) {
+  SlotInit const init = inits[blockIdx.x];
+  std::uint64_t const slot = init.slot;
+  long long* hist = pool.profile + slot * 4u * pool.bins;
+  for (std::uint32_t k = threadIdx.x; k < 4u * pool.bins; k += blockDim.x) hist[k] = 0;
+  if (threadIdx.x < kCounters) pool.counters[slot * kCounters + threadIdx.x] = 0;
+  if (threadIdx.x < kSlotStats) pool.stats[slot * kSlotStats + threadIdx.x] = 0;
+  if (threadIdx.x < 2) pool.record_counts[2 * slot + threadIdx.x] = 0;
+  if (threadIdx.x == 0) {
+    pool.history[slot] = init.first_history;
+    pool.showers[slot] = init.shower;
+  }
+}
+
+__global__ __launch_bounds__(128) void slot_gather_kernel(PoolArrays pool, std::uint32_t const* __restrict__ slots,
+                                                          unsigned long long* __restrict__ out) {
+  std::uint64_t const slot = slots[blockIdx.x];
+  unsigned long long* o = out + static_cast<std::uint64_t>(blockIdx.x) * kGatherWords;
+  std::uint32_t const k = threadIdx.x;
+  if (k < kCounters) o[kGatherCounters + k] = static_cast<unsigned long long>(pool.counters[slot * kCounters + k]);
+  else if (k < kCounters + kSlotStats) o[k] = pool.stats[slot * kSlotStats + (k - kCounters)];
+  else if (k == kGatherHistory) o[k] = pool.history[slot];
+  else if (k < kGatherRecords + 2) o[k] = pool.record_counts[2 * slot + (k - kGatherRecords)];
+  else if (k < kGatherWords) o[k] = 0;
+}
+
 unsigned grid_for(std::uint64_t n, int block) {
   std::uint64_t const g = (n + block - 1) / block;
   if (g > 0x7fffffffULL) throw std::length_error("front exceeds the launch grid");
@@ -319,13 +481,28 @@ std::size_t block_count_bytes(std::uint64_t n) { return grid_for(n, kStepBlock) 
 
 std::size_t offset_bytes(std::uint64_t n) { return grid_for(n, kStepBlock) * 3 * sizeof(unsigned long long); }
 
-void launch_step(bool photon, EmContext const* context, std::uint32_t bins, FrontArgs const& f,
-                 hipStream_t stream) {
+void launch_photon_step(EmContext const* context, std::uint32_t bins, FrontArgs const& f, hipStream_t stream) {
   if (f.count == 0) return;
-  unsigned const grid = grid_for(f.count, kStepBlock);
-  std::size_t const lds = step_lds_bytes(bins);
-  if (photon) step_kernel<true><<<grid, kStepBlock, lds, stream>>>(context, f);
-  else step_kernel<false><<<grid, kStepBlock, lds, stream>>>(context, f);
+  photon_step_kernel<<<grid_for(f.count, kStepBlock), kStepBlock, step_lds_bytes(bins), stream>>>(context, f);
+  C8MI_HIP(hipGetLastError());
+}
+
+void launch_lepton_transport(EmContext const* context, std::uint32_t bins, FrontArgs const& f, hipStream_t stream) {
+  if (f.count == 0) return;
+  lepton_transport_kernel<<<grid_for(f.count, kStepBlock), kStepBlock, step_lds_bytes(bins), stream>>>(context, f);
+  C8MI_HIP(hipGetLastError());
+}
+
+void launch_lepton_vertex(EmContext const* context, std::uint32_t bins, FrontArgs const& f, hipStream_t stream) {
+  if (f.count == 0) return;
+  unsigned const grid = grid_for((f.count + kSourcesPerVertexBlock - 1) / kSourcesPerVertexBlock, 1);
+  lepton_vertex_kernel<<<grid, kStepBlock, step_lds_bytes(bins), stream>>>(context, f);
+  C8MI_HIP(hipGetLastError());
+}
+
+void launch_count(FrontArgs const& f, hipStream_t stream) {
+  if (f.count == 0) return;
+  count_kernel<<<grid_for(f.count, kStepBlock), kStepBlock, 0, stream>>>(f);
   C8MI_HIP(hipGetLastError());
 }
 
@@ -342,6 +519,27 @@ void launch_scatter(FrontArgs const& f, ParticleQueue const& next_queue, Particl
   if (f.count == 0) return;
   scatter_kernel<<<grid_for(f.count, kStepBlock), kStepBlock, 0, stream>>>(f, next_queue, other_queue,
                                                                            other_base);
+  C8MI_HIP(hipGetLastError());
+}
+
+void launch_unstage(StagedParticle const* staged, std::uint64_t n, ParticleQueue const& queue, std::uint64_t at,
+                    hipStream_t stream) {
+  if (n == 0) return;
+  unstage_kernel<<<grid_for(n, 256), 256, 0, stream>>>(staged, n, queue, at);
+  C8MI_HIP(hipGetLastError());
+}
+
+void launch_slot_init(PoolArrays const& pool, SlotInit const* inits, std::uint32_t count, hipStream_t stream) {
+  if (count == 0) return;
+  slot_init_kernel<<<count, 256, 0, stream>>>(pool, inits);
+  C8MI_HIP(hipGetLastError());
+}
+
+void launch_slot_gather(PoolArrays const& pool, std::uint32_t const* slots, std::uint32_t count,
+                        unsigned long long* out, hipStream_t stream) {
+  if (count == 0) return;
+  static_assert(kGatherRecords + 2 <= kGatherWords && kGatherWords <= 128, "gather layout");
+  slot_gather_kernel<<<count, 128, 0, stream>>>(pool, slots, out);
   C8MI_HIP(hipGetLastError());
 }
 

@@ -24,6 +24,20 @@ inline constexpr int kSlotStats = 8;
 // new count of the stepped species and the count appended to the other one.
 inline constexpr int kReportHeader = 8;
 
+// An e+- step that reached its interaction vertex, handed from the transport
+// kernel to the vertex kernel (compacted, in source order within each block).
+struct VertexRecord {
+  c8mi_particle_state parent;
+  std::uint64_t shower_key;
+  std::uint64_t source;
+  double total_rate;
+  double process_uniform;
+  double particle_mass_gev;
+  double vertex_density;
+  std::uint32_t slot;
+  std::uint32_t pad[3];
+};
+
 struct FrontArgs {
   ParticleQueue in;
   std::uint64_t count;
@@ -39,16 +53,66 @@ struct FrontArgs {
   unsigned long long* history;        // [slots] next secondary history ID
   unsigned long long* stats;          // [slots][kSlotStats]
   unsigned long long* report;         // kReportHeader + 2 * slots
+  VertexRecord* vertices;             // lepton fronts: up to `count` records
+  unsigned long long* vertex_count;   // zero between fronts
+};
+
+// Start of a shower in a slot: its parameters and first secondary history ID.
+struct SlotInit {
+  ShowerSlot shower;
+  unsigned long long first_history;
+  std::uint32_t slot;
+  std::uint32_t pad;
+};
+
+// Words gathered per finished slot: ledger counters, statistics, next history
+// ID, record counts.
+inline constexpr int kGatherCounters = 0;
+inline constexpr int kGatherStats = kCounters;
+inline constexpr int kGatherHistory = kCounters + kSlotStats;
+inline constexpr int kGatherRecords = kGatherHistory + 1;
+inline constexpr int kGatherWords = 80;
+
+struct PoolArrays {
+  ShowerSlot* showers;
+  long long* profile;
+  long long* counters;
+  unsigned long long* record_counts;
+  unsigned long long* stats;
+  unsigned long long* history;
+  std::uint32_t bins;
 };
 
 std::size_t step_lds_bytes(std::uint32_t bins);
 std::size_t block_count_bytes(std::uint64_t n);
 std::size_t offset_bytes(std::uint64_t n);
 
-void launch_step(bool photon, EmContext const* context, std::uint32_t bins, FrontArgs const& f,
-                 hipStream_t stream);
+// Kernel time categories of a front.
+enum FrontKernel : int { kKernPhotonStep = 0, kKernLeptonTransport = 1, kKernLeptonVertex = 2, kKernCount = 3,
+                         kKernScan = 4, kKernScatter = 5, kFrontKernels = 6 };
+
+void launch_photon_step(EmContext const* context, std::uint32_t bins, FrontArgs const& f, hipStream_t stream);
+void launch_lepton_transport(EmContext const* context, std::uint32_t bins, FrontArgs const& f, hipStream_t stream);
+void launch_lepton_vertex(EmContext const* context, std::uint32_t bins, FrontArgs const& f, hipStream_t stream);
+void launch_count(FrontArgs const& f, hipStream_t stream);
 void launch_scan(bool photon, FrontArgs const& f, hipStream_t stream);
 void launch_scatter(FrontArgs const& f, ParticleQueue const& next_queue, ParticleQueue const& other_queue,
                     std::uint64_t other_base, hipStream_t stream);
+// Host-staged particle with the pool slot of its shower.
+struct StagedParticle {
+  c8mi_particle_state particle;
+  std::uint32_t slot;
+  std::uint32_t pad[3];
+};
+
+// Writes n staged particles into queue entries [at, at + n).
+void launch_unstage(StagedParticle const* staged, std::uint64_t n, ParticleQueue const& queue, std::uint64_t at,
+                    hipStream_t stream);
+// Clears the profile, ledger, records and statistics of each started slot and
+// sets its parameters and history counter (one workgroup per slot).
+void launch_slot_init(PoolArrays const& pool, SlotInit const* inits, std::uint32_t count, hipStream_t stream);
+// Copies kGatherWords per listed slot into `out` (one workgroup per slot).
+void launch_slot_gather(PoolArrays const& pool, std::uint32_t const* slots, std::uint32_t count,
+                        unsigned long long* out, hipStream_t stream);
 
 }  // namespace c8mi::em

@@ -99,16 +99,31 @@ __device__ __forceinline__ void hermite_derivative(double x, double b[4]) {
   b[3] = -2. * x + 3. * x2;
 }
 
+// Energy-axis part of a bicubic evaluation. Within one query the energy is
+// fixed while the loss varies (root finding, finite differences), so the
+// energy coordinate, interval and fraction are computed once per query; the
+// values are the ones every evaluation would compute.
+struct EnergyRow {
+  double x0;
+  std::uint32_t n0;
+  double f0;
+};
+
+__device__ __forceinline__ EnergyRow energy_row(c8mi_native_bicubic const& s, double e) {
+  double const x0 = axis_transform(s.energy_axis, e);
+  std::uint32_t const n0 = interval_node(x0, s.rows);
+  return {x0, n0, x0 - n0};
+}
+
 // Four coefficient planes, column-major per spline: (row, column) at
 // offset + row + rows * column.
-__device__ __forceinline__ double bicubic_sum(Tables const& t, c8mi_native_bicubic const& s, double e,
-                                     double loss, bool derivative) {
-  double const x0 = axis_transform(s.energy_axis, e);
+__device__ __forceinline__ double bicubic_sum(Tables const& t, c8mi_native_bicubic const& s, EnergyRow const& er,
+                                              double loss, bool derivative) {
   double const x1 = axis_transform(s.loss_axis, loss);
-  std::uint32_t const n0 = interval_node(x0, s.rows);
+  std::uint32_t const n0 = er.n0;
   std::uint32_t const n1 = interval_node(x1, s.columns);
   double h0[4], h1[4];
-  hermite(x0 - n0, h0);
+  hermite(er.f0, h0);
   if (derivative) hermite_derivative(x1 - n1, h1);
   else hermite(x1 - n1, h1);
   double r = 0.;
@@ -137,21 +152,19 @@ __device__ __forceinline__ double dot4(double a0, double a1, double a2, double a
 
 // Host-precomputed 16-coefficient cell polynomial.
 __device__ __forceinline__ double bicubic_precomputed(Tables const& t, c8mi_native_bicubic const& s,
-                                             double e, double loss) {
+                                                      EnergyRow const& er, double loss) {
   if (s.polynomial_coefficient_offset == C8MI_NO_POLYNOMIAL_COEFFICIENTS ||
       t.bicubic_polynomial_coefficients == nullptr)
     return __builtin_nan("");
-  double const x0 = axis_transform(s.energy_axis, e);
   double const x1 = axis_transform(s.loss_axis, loss);
-  std::uint32_t const n0 = interval_node(x0, s.rows);
   std::uint32_t const n1 = interval_node(x1, s.columns);
   std::uint64_t const cell =
-      static_cast<std::uint64_t>(n0) + static_cast<std::uint64_t>(s.rows - 1u) * n1;
+      static_cast<std::uint64_t>(er.n0) + static_cast<std::uint64_t>(s.rows - 1u) * n1;
   std::uint64_t const offset = s.polynomial_coefficient_offset + 16u * cell;
   std::uint64_t const size = t.bicubic_polynomial_coefficient_count;
   if (offset > size || 16u > size - offset) return __builtin_nan("");
   double const* c = t.bicubic_polynomial_coefficients + offset;
-  double const f0 = x0 - n0, f1 = x1 - n1;
+  double const f0 = er.f0, f1 = x1 - n1;
   double const v0[4] = {1., f0, f0 * f0, f0 * f0 * f0};
   double const v1[4] = {1., f1, f1 * f1, f1 * f1 * f1};
   double p[4];
@@ -162,23 +175,28 @@ __device__ __forceinline__ double bicubic_precomputed(Tables const& t, c8mi_nati
   return dot4(v0[0], v0[1], v0[2], v0[3], p[0], p[1], p[2], p[3]);
 }
 
-__device__ __forceinline__ double bicubic76(Tables const& t, c8mi_native_bicubic const& s, double e,
-                                   double loss) {
+__device__ __forceinline__ double bicubic76(Tables const& t, c8mi_native_bicubic const& s, EnergyRow const& er,
+                                            double loss) {
   if (s.polynomial_coefficient_offset != C8MI_NO_POLYNOMIAL_COEFFICIENTS &&
       t.bicubic_polynomial_coefficients != nullptr)
-    return bicubic_precomputed(t, s, e, loss);
-  return bicubic_sum(t, s, e, loss, false);
+    return bicubic_precomputed(t, s, er, loss);
+  return bicubic_sum(t, s, er, loss, false);
+}
+
+__device__ __forceinline__ double bicubic76(Tables const& t, c8mi_native_bicubic const& s, double e,
+                                            double loss) {
+  return bicubic76(t, s, energy_row(s, e), loss);
 }
 
 // CubicInterpolation 0.1.5's prime(): Boost's sixth-order finite difference.
+// The step is (epsilon / 168)^(1/7) rounded to double, 0x1.6deb70960cf79p-9.
 __device__ __forceinline__ double bicubic_loss_derivative76(Tables const& t, c8mi_native_bicubic const& s,
-                                                   double e, double loss) {
-  constexpr double epsilon = 2.220446049250313080847263336181640625e-16;
-  double h = pow(epsilon / 168., 1. / 7.);
+                                                            EnergyRow const& er, double loss) {
+  double h = 0x1.6deb70960cf79p-9;
   double const coordinate = axis_transform(s.loss_axis, loss);
   double const representable = coordinate + h;
   h = representable - coordinate;
-  auto at = [&](double c) { return bicubic76(t, s, e, axis_back_transform(s.loss_axis, c)); };
+  auto at = [&](double c) { return bicubic76(t, s, er, axis_back_transform(s.loss_axis, c)); };
   double const y1 = at(coordinate + h) - at(coordinate - h);
   double const y2 = at(coordinate - 2. * h) - at(coordinate + 2. * h);
   double const y3 = at(coordinate + 3. * h) - at(coordinate - 3. * h);
@@ -487,7 +505,7 @@ __device__ __forceinline__ bool rate_below_threshold(c8mi_native_dndx_column con
 }  // namespace detail
 
 __device__ __forceinline__ QueryResult cumulative_rate(Tables const& t, c8mi_native_dndx_column const& c,
-                                              double e, double loss) {
+                                                       double e, double loss) {
   using namespace detail;
   if (!valid_view(t)) return qr(C8MI_QUERY_INVALID_VIEW);
   if (!isfinite(e) || !isfinite(loss)) return qr(C8MI_QUERY_NON_FINITE_INPUT);
@@ -502,7 +520,7 @@ __device__ __forceinline__ QueryResult cumulative_rate(Tables const& t, c8mi_nat
 }
 
 __device__ __forceinline__ QueryResult rate_for_column(Tables const& t, c8mi_native_dndx_column const& c,
-                                              double e) {
+                                                       double e) {
   using namespace detail;
   if (!valid_view(t)) return qr(C8MI_QUERY_INVALID_VIEW);
   if (rate_below_threshold(c, e)) return qr(C8MI_QUERY_SUCCESS);
@@ -567,7 +585,7 @@ __device__ __forceinline__ QueryResult rate_for_column(Tables const& t, c8mi_nat
   if (c.rate_model == C8MI_RATE_BICUBIC && (c.pdg_id == 13 || c.pdg_id == -13)) {
     if (!isfinite(e)) return qr(C8MI_QUERY_NON_FINITE_INPUT);
     if (e > c.spline.energy_axis.high) return qr(C8MI_QUERY_ENERGY_OUT_OF_RANGE);
-    double value = bicubic_precomputed(t, c.spline, e, 1.) / c.component_weight;
+    double value = bicubic_precomputed(t, c.spline, energy_row(c.spline, e), 1.) / c.component_weight;
     if (value < 0.) value = 0.;
     if (!isfinite(value)) return qr(C8MI_QUERY_INVALID_VIEW);
     return qr(C8MI_QUERY_SUCCESS, value);
@@ -682,7 +700,8 @@ __device__ __forceinline__ QueryResult loss_fraction(Tables const& t, std::int32
   double const target = photon_split ? q * total.value : (q * total.value) * c.component_weight;
   if (q == 0.) return qr(C8MI_QUERY_SUCCESS, lim.minimum);
   if (q == 1.) return qr(C8MI_QUERY_SUCCESS, lim.maximum);
-  auto residual_at = [&](double loss) { return bicubic76(t, c.spline, e, loss) - target; };
+  EnergyRow const row = energy_row(c.spline, e);
+  auto residual_at = [&](double loss) { return bicubic76(t, c.spline, row, loss) - target; };
 
   double lower = c.spline.loss_axis.low, upper = c.spline.loss_axis.high;
   double lower_residual = residual_at(lower);
@@ -747,7 +766,7 @@ __device__ __forceinline__ QueryResult loss_fraction(Tables const& t, std::int32
       exact = true;
       break;
     }
-    double const derivative = bicubic_loss_derivative76(t, c.spline, e, current);
+    double const derivative = bicubic_loss_derivative76(t, c.spline, row, current);
     if (!isfinite(derivative)) {
       need_bisection = true;
       break;

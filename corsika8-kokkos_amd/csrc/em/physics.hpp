@@ -877,8 +877,11 @@ __device__ __forceinline__ Particle child_of(Particle const& parent, std::int32_
 
 // ---- step outputs ----
 
+// Outputs of one source: up to two particles, stored as soon as they are made
+// into the source's two staging entries (so they do not stay live in
+// registers through the rest of the step).
 struct Outputs {
-  Particle slot[2];
+  c8mi_particle_state* staging;
   std::uint32_t kind[2];  // SlotKind
   std::uint32_t next, other, children;
   bool step, interaction;
@@ -886,7 +889,7 @@ struct Outputs {
 
 __device__ __forceinline__ void emit(Outputs& o, Particle const& p, std::uint32_t kind) {
   std::uint32_t const k = (o.kind[0] == kSlotNone) ? 0u : 1u;
-  o.slot[k] = p;
+  o.staging[k] = p;
   o.kind[k] = kind;
   if (kind == kSlotOtherChild) ++o.other;
   else ++o.next;

@@ -244,17 +244,30 @@ __device__ __forceinline__ void photon_step(EmContext const& c, Accumulator cons
 
 // ---------------------------------------------------------------- lepton ----
 
-__device__ __forceinline__ void lepton_step(EmContext const& c, Accumulator const& acc, Particle const& p,
-                                   Outputs& o) {
+// What the vertex part of an e+- step takes over from its transport part.
+struct LeptonVertex {
+  Particle parent;  // the particle at the vertex
+  double total_rate;
+  double process_uniform;
+  double particle_mass_gev;
+  double vertex_density;
+};
+
+// The step of one e+- up to its interaction vertex (selection, continuous
+// step, transport, profile and ledger). Returns true when the step ends at an
+// interaction vertex: `vx` then holds what lepton_vertex needs.
+__device__ __forceinline__ bool lepton_transport(EmContext const& c, Accumulator const& acc, Particle const& p,
+                                                 Outputs& o, LeptonVertex& vx) {
   Interaction in;
   Fallback fb{false, 0};
   select_interaction(c, p, in, fb);
   if (fb.flag) {
     record_fallback(acc, p, fb.reason, in.process_id, kStageSelection);
-    return;
+    return false;
   }
   auto transport_fallback = [&](int reason) {
     record_fallback(acc, in.particle, reason, in.process_id, kStageTransport);
+    return false;
   };
   Particle start = in.particle;
   if (!is_ee(start.pid)) return transport_fallback(kFrUnsupportedParticle);
@@ -455,29 +468,42 @@ __device__ __forceinline__ void lepton_step(EmContext const& c, Accumulator cons
       terminal_energy(acc, end.energy_gev * w, true);
       record_observation(acc, end);
     }
-    return;
+    return false;
   }
   if (limit == kLlObservation || limit == kLlEscaped) {
     terminal_energy(acc, end.energy_gev * w, limit == kLlObservation);
     record_observation(acc, end);
-    return;
+    return false;
   }
   if (limit != kLlInteraction) {
     emit(o, end, kSlotNextContinue);
-    return;
+    return false;
   }
-  // select_lepton_vertex (LeptonVertexSelection.inl:41-197)
   o.interaction = true;
-  Particle parent = end;
+  vx.parent = end;
+  vx.total_rate = in.total_rate;
+  vx.process_uniform = in.process_uniform;
+  vx.particle_mass_gev = in.particle_mass_gev;
+  vx.vertex_density = vertex_density;
+  return true;
+}
+
+// Interaction vertex of an e+- step: vertex reselection at the post-loss
+// energy, then the final state.
+__device__ __forceinline__ void lepton_vertex(EmContext const& c, Accumulator const& acc, LeptonVertex const& vx,
+                                              Outputs& o) {
+  // select_lepton_vertex (LeptonVertexSelection.inl:41-197)
+  Particle parent = vx.parent;
+  double const w = parent.weight;
   std::int32_t process = 0;
   auto vertex_fallback = [&](int reason) {
     record_fallback(acc, parent, reason, process, kStageVertex);
   };
-  if (!isfinite(in.total_rate) || !(in.total_rate > 0.) || !isfinite(in.process_uniform) ||
-      !(in.process_uniform > 0.) || !(in.process_uniform < 1.))
+  if (!isfinite(vx.total_rate) || !(vx.total_rate > 0.) || !isfinite(vx.process_uniform) ||
+      !(vx.process_uniform > 0.) || !(vx.process_uniform < 1.))
     return vertex_fallback(kFrInvalidFinalState);
   double const e_mev = parent.energy_gev * 1000.;
-  double const outer = in.process_uniform * in.total_rate;
+  double const outer = vx.process_uniform * vx.total_rate;
   Tq const vt = native(total_rate(c.tables, parent.pid, e_mev), false);
   if (vt.status != kTsSuccess) return vertex_fallback(fallback_reason(vt.status));
   if (!(vt.value > 0.) || outer >= vt.value) {
@@ -508,10 +534,8 @@ __device__ __forceinline__ void lepton_step(EmContext const& c, Accumulator cons
     return final_fallback(cap == kCapCpuOnly ? kFrCpuOnly : kFrNotImplemented);
   c8mi_brems_lpm const& lpm = c.brems_lpm;
   double const me = lpm.electron_mass_mev / 1000.;
-  double const lm = in.particle_mass_gev > 0. ? in.particle_mass_gev : lpm.lepton_mass_mev / 1000.;
+  double const lm = vx.particle_mass_gev > 0. ? vx.particle_mass_gev : lpm.lepton_mass_mev / 1000.;
   double const fs_me = lpm.lepton_mass_mev / 1000.;
-  double const lmass = native_mass.value / 1000.;
-  (void)lmass;
   if (!isfinite(parent.energy_gev) || !(parent.energy_gev > lm) || parent.generation == 0xffffffffU ||
       parent.step_id == 0xffffffffffffffffULL)
     return final_fallback(kFrInvalidFinalState);
@@ -606,7 +630,7 @@ __device__ __forceinline__ void lepton_step(EmContext const& c, Accumulator cons
   double const au = uniform(c, parent, kBrems, 1);
   double survival = 0.;
   // the vertex density comes from the transport step
-  int const ls = brems_lpm(lpm, sel.component, parent.energy_gev * 1000., v, vertex_density, survival);
+  int const ls = brems_lpm(lpm, sel.component, parent.energy_gev * 1000., v, vx.vertex_density, survival);
   if (ls != 0) return final_fallback(ls == 3 ? kFrInvalidDensity : kFrLpmUnavailable);
   double const lu = uniform(c, parent, kBrems, 2);
   if (lu > survival) {
@@ -650,6 +674,13 @@ __device__ __forceinline__ void lepton_step(EmContext const& c, Accumulator cons
   ledger(acc, kMassCorrection, mass_correction(parent.pid, lm, (th.keep & 1U) ? th.w1 : 0.));
   thinning_ledger(acc, th.status, 2, k, (e1 + pe) * w,
                   ((th.keep & 1U) ? e1 * th.w1 : 0.) + ((th.keep & 2U) ? pe * th.w2 : 0.));
+}
+
+// The whole e+- step (transport, then the vertex when one is reached).
+__device__ __forceinline__ void lepton_step(EmContext const& c, Accumulator const& acc, Particle const& p,
+                                            Outputs& o) {
+  LeptonVertex vx;
+  if (lepton_transport(c, acc, p, o, vx)) lepton_vertex(c, acc, vx, o);
 }
 
 }  // namespace c8mi::em::phys

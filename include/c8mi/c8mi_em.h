@@ -422,9 +422,10 @@ int c8mi_em_serve(c8mi_em* em, const c8mi_em_request* requests, uint64_t count,
                   c8mi_em_run_result* totals);
 
 /* Per-kernel device time (HIP events) since enable: ms[0] photon step,
- * ms[1] lepton step, ms[2] scans, ms[3] scatter. */
+ * ms[1] lepton transport, ms[2] lepton vertex, ms[3] count, ms[4] scan,
+ * ms[5] scatter. */
 int c8mi_em_enable_kernel_timing(c8mi_em* em, int enable);
-int c8mi_em_kernel_time(const c8mi_em* em, double ms[4], uint64_t launches[4]);
+int c8mi_em_kernel_time(const c8mi_em* em, double ms[6], uint64_t launches[6]);
 
 #ifdef __cplusplus
 }
