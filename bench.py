@@ -58,6 +58,7 @@ def parse():
     p.add_argument("--em-plain-showers", type=int, default=20,
                    help="timed 100 GeV showers of the one-at-a-time EM form")
     p.add_argument("--em-slots", type=int, default=512, help="concurrent showers of the EM pool")
+    p.add_argument("--em-warmup-showers", type=int, default=64, help="untimed showers before the EM pool run")
     p.add_argument("--em-only", action="store_true", help="run only the EM section")
     return p.parse_args()
 
@@ -161,13 +162,16 @@ def em_cpu_baseline(tables, atmosphere, energy, target_seconds):
 
 
 def em_measured_traffic(kernel):
+    """HBM bytes per particle-step of `kernel` from the committed rocprofv3 PMC
+    summary (FETCH_SIZE + WRITE_SIZE, copy-calibrated), or None."""
     path = os.path.join(ROOT, "profiles", "em_traffic.json")
     try:
         with open(path) as f:
             data = json.load(f)
     except (OSError, ValueError):
         return None
-    return data.get("traffic_bytes_per_step") if data.get("kernel") == kernel else None
+    entry = data.get("kernels", {}).get(kernel)
+    return entry.get("traffic_bytes_per_step") if entry else None
 
 
 def em_showers(energy, count, first_shower, atm=None, pid=22):
@@ -178,7 +182,7 @@ def em_showers(energy, count, first_shower, atm=None, pid=22):
                  footprint=em.footprint_estimate(energy)) for k in range(count)]
 
 
-def em_section(showers, plain_showers, slots, cpu_seconds, no_cpu_baseline, energy=100.0):
+def em_section(showers, plain_showers, slots, cpu_seconds, no_cpu_baseline, energy=100.0, warmup_showers=64):
     """EM particle-steps/s on the plain 100 GeV vertical photon shower in the
     USStdBK atmosphere (1 MeV cut, no thinning, FP64), the whole cascade on the
     device from the primary on. Two forms: the pool (`showers` showers served
@@ -194,7 +198,10 @@ def em_section(showers, plain_showers, slots, cpu_seconds, no_cpu_baseline, ener
                          observation_capacity=1 << 12, fallback_capacity=1 << 8)
     dev = em.EmBackend(cfg, tables, atm)
     budget = 0.4 * capacity
-    dev.serve(em_showers(energy, min(slots, 64), 1), budget, realtime=False)  # warm-up
+    zero = {"photon_steps": 0, "lepton_steps": 0, "lepton_interactions": 0}
+    warm_totals = zero
+    if warmup_showers:
+        _, warm_totals = dev.serve(em_showers(energy, warmup_showers, 1), budget, realtime=False)
     dev.enable_kernel_timing(True)
     t0 = time.perf_counter()
     results, totals = dev.serve(em_showers(energy, showers, 1000), budget, realtime=False)
@@ -205,16 +212,23 @@ def em_section(showers, plain_showers, slots, cpu_seconds, no_cpu_baseline, ener
     residual = max(abs(em.ledger_residual(r["ledger"])) for r in results)
     pool_latency = [r["finished_s"] - r["admitted_s"] for r in results]
 
-    plain_cfg = em.make_config(airtables, atmosphere=atm, slots=1, queue_capacity=1 << 20,
-                               observation_capacity=1 << 14, fallback_capacity=1 << 10)
-    plain = em.EmBackend(plain_cfg, tables, atm)
-    plain.serve(em_showers(energy, 2, 1), 1e18, realtime=False)  # warm-up
-    t1 = time.perf_counter()
-    plain_results, plain_totals = plain.serve(em_showers(energy, plain_showers, 5000), 1e18, realtime=False)
-    plain_elapsed = time.perf_counter() - t1
-    plain.close()
-    plain_steps = plain_totals["photon_steps"] + plain_totals["lepton_steps"]
-    plain_latency = [r["finished_s"] - r["admitted_s"] for r in plain_results]
+    plain_warm_totals = plain_totals = zero
+    one_at_a_time = None
+    if plain_showers:
+        plain_cfg = em.make_config(airtables, atmosphere=atm, slots=1, queue_capacity=1 << 20,
+                                   observation_capacity=1 << 14, fallback_capacity=1 << 10)
+        plain = em.EmBackend(plain_cfg, tables, atm)
+        _, plain_warm_totals = plain.serve(em_showers(energy, 2, 1), 1e18, realtime=False)  # warm-up
+        t1 = time.perf_counter()
+        plain_results, plain_totals = plain.serve(em_showers(energy, plain_showers, 5000), 1e18,
+                                                  realtime=False)
+        plain_elapsed = time.perf_counter() - t1
+        plain.close()
+        plain_steps = plain_totals["photon_steps"] + plain_totals["lepton_steps"]
+        plain_latency = [r["finished_s"] - r["admitted_s"] for r in plain_results]
+        one_at_a_time = {"value": plain_steps / plain_elapsed, "showers": plain_showers,
+                         "latency_ms": {"p50": float(np.percentile(plain_latency, 50) * 1e3),
+                                        "p99": float(np.percentile(plain_latency, 99) * 1e3)}}
 
     step_kinds = ("photon_step", "lepton_transport", "lepton_vertex")
     kind = max(step_kinds, key=lambda k: timing[k][0])
@@ -236,11 +250,13 @@ def em_section(showers, plain_showers, slots, cpu_seconds, no_cpu_baseline, ener
                        "p99": float(np.percentile(pool_latency, 99) * 1e3)},
         "photon_steps": totals["photon_steps"],
         "lepton_steps": totals["lepton_steps"],
+        # every device step of the section (warm-up and one-at-a-time runs included),
+        # the denominator of per-step counter totals from a profiler run
+        "all_device_steps": {k: sum(t[k] for t in (warm_totals, totals, plain_warm_totals, plain_totals))
+                             for k in ("photon_steps", "lepton_steps", "lepton_interactions")},
         "fronts": totals["photon_fronts"] + totals["lepton_fronts"],
         "max_front": totals["max_front"],
-        "one_at_a_time": {"value": plain_steps / plain_elapsed, "showers": plain_showers,
-                          "latency_ms": {"p50": float(np.percentile(plain_latency, 50) * 1e3),
-                                         "p99": float(np.percentile(plain_latency, 99) * 1e3)}},
+        "one_at_a_time": one_at_a_time,
         "dtype": "f64",
         "config": {"workload": f"EM plain: {energy:g} GeV vertical photon at 112.75 km, USStdBK 5-layer "
                                "spherical air, observation level 0 m, 1 MeV cut, no thinning, no radio, "
@@ -299,7 +315,7 @@ def main():
         return 1
     if args.em_only:
         em_result = em_section(max(args.em_showers, 1), args.em_plain_showers, args.em_slots, args.cpu_seconds,
-                               args.no_cpu_baseline)
+                               args.no_cpu_baseline, warmup_showers=args.em_warmup_showers)
         em_result.update({"n_gpus": 1, "steps": em_result["showers"], "higher_is_better": True,
                           "scaling": "weak", "vs_baseline": None, "data": "synthetic",
                           "ms_per_step": em_result["ms_per_shower"]})
@@ -398,7 +414,7 @@ def main():
         result["speedup_vs_cpu_baseline"] = value / base["value"]
     if args.em_showers > 0:
         result["em"] = em_section(args.em_showers, args.em_plain_showers, args.em_slots, args.cpu_seconds,
-                                  args.no_cpu_baseline)
+                                  args.no_cpu_baseline, warmup_showers=args.em_warmup_showers)
     print(json.dumps(result), flush=True)
     return 0
 
