@@ -59,6 +59,8 @@ def parse():
                    help="timed 100 GeV showers of the one-at-a-time EM form")
     p.add_argument("--em-slots", type=int, default=512, help="concurrent showers of the EM pool")
     p.add_argument("--em-warmup-showers", type=int, default=64, help="untimed showers before the EM pool run")
+    p.add_argument("--em-no-moliere", action="store_true",
+                   help="EM without Moliere multiple scattering (on by default, as in the reference's production)")
     p.add_argument("--em-only", action="store_true", help="run only the EM section")
     return p.parse_args()
 
@@ -132,7 +134,7 @@ def field_error(tracks, observers, propagation, mode, limit, exact):
 EM_BYTES_PER_STEP = 240  # 120 B SoA state (116 B particle + 4 B pool slot) read + written per step
 
 
-def em_cpu_baseline(tables, atmosphere, energy, target_seconds):
+def em_cpu_baseline(tables, atmosphere, energy, target_seconds, moliere=True):
     """Oracle EM showers (the serial C restatement, one shower per host thread,
     OpenMP over showers on all host cores) for about `target_seconds`."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
@@ -141,7 +143,7 @@ def em_cpu_baseline(tables, atmosphere, energy, target_seconds):
 
     threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count() or 1
     threads = max(1, min(threads, os.cpu_count() or 1))
-    cfg = em.make_config(airtables, atmosphere=atmosphere)
+    cfg = em.make_config(airtables, atmosphere=atmosphere, moliere=moliere)
     prim = em.particle(22, energy, (0.0, 0.0, 112750.0), (0.0, 0.0, -1.0), history_id=1)
     count = threads
     first = 1000
@@ -182,7 +184,8 @@ def em_showers(energy, count, first_shower, atm=None, pid=22):
                  footprint=em.footprint_estimate(energy)) for k in range(count)]
 
 
-def em_section(showers, plain_showers, slots, cpu_seconds, no_cpu_baseline, energy=100.0, warmup_showers=64):
+def em_section(showers, plain_showers, slots, cpu_seconds, no_cpu_baseline, energy=100.0, warmup_showers=64,
+               moliere=True):
     """EM particle-steps/s on the plain 100 GeV vertical photon shower in the
     USStdBK atmosphere (1 MeV cut, no thinning, FP64), the whole cascade on the
     device from the primary on. Two forms: the pool (`showers` showers served
@@ -195,7 +198,7 @@ def em_section(showers, plain_showers, slots, cpu_seconds, no_cpu_baseline, ener
     atm = em.Atmosphere()
     capacity = 1 << 23
     cfg = em.make_config(airtables, atmosphere=atm, slots=slots, queue_capacity=capacity,
-                         observation_capacity=1 << 12, fallback_capacity=1 << 8)
+                         observation_capacity=1 << 12, fallback_capacity=1 << 8, moliere=moliere)
     dev = em.EmBackend(cfg, tables, atm)
     budget = 0.4 * capacity
     zero = {"photon_steps": 0, "lepton_steps": 0, "lepton_interactions": 0}
@@ -216,7 +219,7 @@ def em_section(showers, plain_showers, slots, cpu_seconds, no_cpu_baseline, ener
     one_at_a_time = None
     if plain_showers:
         plain_cfg = em.make_config(airtables, atmosphere=atm, slots=1, queue_capacity=1 << 20,
-                                   observation_capacity=1 << 14, fallback_capacity=1 << 10)
+                                   observation_capacity=1 << 14, fallback_capacity=1 << 10, moliere=moliere)
         plain = em.EmBackend(plain_cfg, tables, atm)
         _, plain_warm_totals = plain.serve(em_showers(energy, 2, 1), 1e18, realtime=False)  # warm-up
         t1 = time.perf_counter()
@@ -260,6 +263,7 @@ def em_section(showers, plain_showers, slots, cpu_seconds, no_cpu_baseline, ener
         "dtype": "f64",
         "config": {"workload": f"EM plain: {energy:g} GeV vertical photon at 112.75 km, USStdBK 5-layer "
                                "spherical air, observation level 0 m, 1 MeV cut, no thinning, no radio, "
+                               f"Moliere scattering {'on' if moliere else 'off'}, "
                                "closed-form v7 air tables, whole cascade on the device; "
                                f"{showers} showers through {slots} pool slots"},
         "roofline": {
@@ -278,7 +282,7 @@ def em_section(showers, plain_showers, slots, cpu_seconds, no_cpu_baseline, ener
         "ledger_residual_gev_max": residual,
     }
     if not no_cpu_baseline:
-        base = em_cpu_baseline(tables, atm, energy, cpu_seconds)
+        base = em_cpu_baseline(tables, atm, energy, cpu_seconds, moliere=moliere)
         out["cpu_baseline"] = base
         out["speedup_vs_cpu_baseline"] = out["value"] / base["value"]
     return out
@@ -315,7 +319,8 @@ def main():
         return 1
     if args.em_only:
         em_result = em_section(max(args.em_showers, 1), args.em_plain_showers, args.em_slots, args.cpu_seconds,
-                               args.no_cpu_baseline, warmup_showers=args.em_warmup_showers)
+                               args.no_cpu_baseline, warmup_showers=args.em_warmup_showers,
+                               moliere=not args.em_no_moliere)
         em_result.update({"n_gpus": 1, "steps": em_result["showers"], "higher_is_better": True,
                           "scaling": "weak", "vs_baseline": None, "data": "synthetic",
                           "ms_per_step": em_result["ms_per_shower"]})
@@ -414,7 +419,8 @@ def main():
         result["speedup_vs_cpu_baseline"] = value / base["value"]
     if args.em_showers > 0:
         result["em"] = em_section(args.em_showers, args.em_plain_showers, args.em_slots, args.cpu_seconds,
-                                  args.no_cpu_baseline, warmup_showers=args.em_warmup_showers)
+                                  args.no_cpu_baseline, warmup_showers=args.em_warmup_showers,
+                               moliere=not args.em_no_moliere)
     print(json.dumps(result), flush=True)
     return 0
 

@@ -23,6 +23,7 @@ dE/dX and ranges are cubic splines over exponential energy axes.
 """
 from __future__ import annotations
 
+import functools
 import math
 
 import numpy as np
@@ -314,3 +315,74 @@ def brems_lpm_snapshot():
                 electron_mass_mev=0.5109989461, muon_mass_mev=105.6583745,
                 classical_electron_radius_cm=RE_CM, fine_structure_constant=ALPHA,
                 components=[(h, z, a, 183.0) for _, z, a, _, _, h in AIR])
+
+
+# ---- Molière multiple scattering (the MoliereSnapshot of the air medium) ----
+
+HBAR_MEV_S = 6.582119569e-22
+SPEED_OF_LIGHT_CM_PER_S = 2.99792458e10
+EULER_MASCHERONI = 0.57721566490153286061
+
+
+@functools.lru_cache(maxsize=None)
+def moliere_coefficients():
+    """Coefficients of Molière's projected-angle correction functions in
+    x = theta^2 / (chi_c^2 B), with f_n(x) = sqrt(pi) g_n(sqrt(x)) and
+    g_n(t) = 1/(n! pi) int_0^inf cos(t y) exp(-y^2/4) ((y^2/4) ln(y^2/4))^n dy.
+
+    Small x (power series): c1[k] = (-4)^k Gamma(k+3/2) psi(k+3/2) / (sqrt(pi) (2k)!),
+    c2[k] = (-4)^k Gamma(k+5/2) (psi(k+5/2)^2 + psi'(k+5/2)) / (2 sqrt(pi) (2k)!).
+    Large x (from the |y|^a ln|y| singularities of the integrand at y = 0):
+    f2 ~ sum_i c2l[i] (ln(x)/2 + s2l[i]) x^-(i+1/2), c2l[i] = sqrt(pi) (2i)! / (8 4^(i-2) (i-2)!),
+    s2l[i] = ln 2 - psi(2i+1), i = 2..12; the integral of f1 from sqrt(x) to infinity
+    gives int_0^sqrt(x) f1 = sum_i c1il[i] x^-i, c1il[i] = -sqrt(pi) (2i-1)! / (4^i (i-1)!).
+    Computed at 60 digits and rounded to double."""
+    import mpmath as mp
+
+    with mp.workdps(60):
+        sq = mp.sqrt(mp.pi)
+        c1 = [float((-4) ** k * mp.gamma(k + mp.mpf(3) / 2) * mp.digamma(k + mp.mpf(3) / 2) /
+                    (sq * mp.factorial(2 * k))) for k in range(70)]
+        c2 = []
+        for k in range(70):
+            a = k + mp.mpf(5) / 2
+            c2.append(float((-4) ** k * mp.gamma(a) * (mp.digamma(a) ** 2 + mp.polygamma(1, a)) /
+                            (2 * sq * mp.factorial(2 * k))))
+        c2l = [0.0, 0.0] + [float(sq * mp.factorial(2 * i) / (8 * mp.mpf(4) ** (i - 2) * mp.factorial(i - 2)))
+                            for i in range(2, 13)]
+        s2l = [0.0, 0.0] + [float(mp.log(2) - mp.digamma(2 * i + 1)) for i in range(2, 13)]
+        c1il = [0.0] + [float(-sq * mp.factorial(2 * i - 1) / (mp.mpf(4) ** i * mp.factorial(i - 1)))
+                        for i in range(1, 15)]
+    return tuple(c1), tuple(c2), tuple(c2l), tuple(s2l), tuple(c1il)
+
+
+def moliere_snapshot(particle_mass_mev=ELECTRON_MASS_MEV):
+    """Electron Molière snapshot of the air medium. Component fields are derived
+    as the reference's auxiliary export derives them from a medium
+    (ProposalNativeAux.cpp:99-135): mass weights, weight * Z^2, the mean
+    Z(Z+1)/A for electrons, chi_0 = m_e alpha (128 Z / (9 pi^2))^(1/3) and the
+    Coulomb correction 3.76 (alpha Z)^2."""
+    c1, c2, c2l, s2l, c1il = moliere_coefficients()
+    components = []
+    mean_a = 0.0
+    mean_zz = 0.0
+    weight_zz_sum = 0.0
+    weights = []
+    for _, z, a, w, _, _ in AIR:
+        weights.append(w)
+        mean_a += w * a
+        wzz = w * z * z
+        weight_zz_sum += wzz
+        mean_zz += w * z * (z + 1.0) if particle_mass_mev == ELECTRON_MASS_MEV else wzz
+        chi0 = ELECTRON_MASS_MEV * ALPHA * (z * 128.0 / (9.0 * math.pi * math.pi)) ** (1.0 / 3.0)
+        components.append((float(z), wzz, chi0 * chi0, 3.76 * ALPHA * ALPHA * z * z))
+    maximum = 0
+    for k in range(len(weights) - 1):
+        if weights[k + 1] > weights[k]:
+            maximum = k + 1
+    return dict(particle_mass_mev=particle_mass_mev, electron_mass_mev=ELECTRON_MASS_MEV,
+                fine_structure_constant=ALPHA, avogadro_per_mol=AVOGADRO, hbar_mev_s=HBAR_MEV_S,
+                speed_of_light_cm_per_s=SPEED_OF_LIGHT_CM_PER_S, z_squared_over_a_average=mean_zz / mean_a,
+                inverse_weight_zz_sum=1.0 / weight_zz_sum, euler_mascheroni=EULER_MASCHERONI,
+                maximum_weight_index=maximum, components=components, c1=c1, c2=c2, c2_large=c2l,
+                s2_large=s2l, c1_integral_large=c1il)

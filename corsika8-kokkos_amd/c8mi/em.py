@@ -86,11 +86,44 @@ class Thinning(ctypes.Structure):
                 ("erase_zero_weight", c_uint32)]
 
 
+class MoliereComponent(ctypes.Structure):
+    _fields_ = [("nuclear_charge", c_double), ("weight_zz", c_double), ("chi0_squared_mev2", c_double),
+                ("coulomb_correction", c_double)]
+
+
+class MoliereSnapshot(ctypes.Structure):
+    _fields_ = [(k, c_double) for k in ("particle_mass_mev", "electron_mass_mev", "fine_structure_constant",
+                                        "avogadro_per_mol", "hbar_mev_s", "speed_of_light_cm_per_s",
+                                        "z_squared_over_a_average", "inverse_weight_zz_sum",
+                                        "euler_mascheroni")] + [
+        ("component_count", c_uint32), ("maximum_weight_index", c_uint32),
+        ("components", MoliereComponent * 16), ("c1", c_double * 70), ("c2", c_double * 70),
+        ("c2_large", c_double * 13), ("s2_large", c_double * 13), ("c1_integral_large", c_double * 15)]
+
+    @classmethod
+    def from_dict(cls, d):
+        m = cls()
+        for k in ("particle_mass_mev", "electron_mass_mev", "fine_structure_constant", "avogadro_per_mol",
+                  "hbar_mev_s", "speed_of_light_cm_per_s", "z_squared_over_a_average", "inverse_weight_zz_sum",
+                  "euler_mascheroni", "maximum_weight_index"):
+            setattr(m, k, d[k])
+        m.component_count = len(d["components"])
+        for i, comp in enumerate(d["components"]):
+            m.components[i] = MoliereComponent(*comp)
+        for k in ("c1", "c2", "c2_large", "s2_large", "c1_integral_large"):
+            arr = getattr(m, k)
+            for i, v in enumerate(d[k]):
+                arr[i] = v
+        return m
+
+
 class EmConfig(ctypes.Structure):
     _fields_ = [("random_seed", c_uint64), ("em_transport_cut_mev", c_double),
                 ("muon_transport_cut_mev", c_double), ("queue_capacity", c_uint64),
                 ("observation_capacity", c_uint64), ("fallback_capacity", c_uint64),
                 ("photon_pair_lpm", PairLpm), ("brems_lpm", BremsLpm), ("thinning", Thinning),
+                ("electron_moliere", MoliereSnapshot), ("moliere_enabled", c_uint32),
+                ("moliere_reserved_", c_uint32),
                 ("profile_bins", c_uint32), ("profile_crossing_mode", c_uint32),
                 ("profile_bin_width_g_per_cm2", c_double),
                 ("profile_energy_loss_threshold_g_per_cm2", c_double),
@@ -304,10 +337,15 @@ def particle(pid, energy_gev, position, direction, history_id=1, weight=1.0, nod
 def make_config(tables_module, seed=1, cut_mev=1.0, start=(0.0, 0.0, 112750.0),
                 direction=(0.0, 0.0, -1.0), atmosphere=None, bins=128, bin_width=10.0,
                 queue_capacity=1 << 22, observation_capacity=1 << 20, fallback_capacity=1 << 16,
-                thinning=None, crossing_mode=0, slots=1):
-    """EmConfig + the arrays it points to (kept alive on the returned object)."""
+                thinning=None, crossing_mode=0, slots=1, moliere=False):
+    """EmConfig + the arrays it points to (kept alive on the returned object).
+    moliere: apply Molière multiple scattering to e+- steps (the reference's
+    production setting); off, e+- directions change only at vertices."""
     cfg = EmConfig()
     cfg.shower_slots = slots
+    if moliere:
+        cfg.electron_moliere = MoliereSnapshot.from_dict(tables_module.moliere_snapshot())
+        cfg.moliere_enabled = 1
     cfg.random_seed = seed
     cfg.em_transport_cut_mev = cut_mev
     cfg.muon_transport_cut_mev = cut_mev

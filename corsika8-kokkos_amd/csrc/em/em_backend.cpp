@@ -65,6 +65,30 @@ void validate_environment(c8mi_environment const& e) {
   }
 }
 
+// valid_moliere_snapshot (MoliereSnapshot.inl:26-66): finite, positive constants
+// and components, finite coefficients.
+bool valid_moliere_snapshot(c8mi_moliere_snapshot const& m) {
+  auto pos = [](double v) { return std::isfinite(v) && v > 0.; };
+  if (m.component_count == 0 || m.component_count > C8MI_MOLIERE_MAX_COMPONENTS ||
+      m.maximum_weight_index >= m.component_count || !pos(m.particle_mass_mev) || !pos(m.electron_mass_mev) ||
+      !pos(m.fine_structure_constant) || !pos(m.avogadro_per_mol) || !pos(m.hbar_mev_s) ||
+      !pos(m.speed_of_light_cm_per_s) || !pos(m.z_squared_over_a_average) || !pos(m.inverse_weight_zz_sum) ||
+      !std::isfinite(m.euler_mascheroni))
+    return false;
+  for (std::uint32_t k = 0; k < m.component_count; ++k) {
+    c8mi_moliere_component const& c = m.components[k];
+    if (!pos(c.nuclear_charge) || !pos(c.weight_zz) || !pos(c.chi0_squared_mev2) || !pos(c.coulomb_correction))
+      return false;
+  }
+  for (int k = 0; k < 70; ++k)
+    if (!std::isfinite(m.c1[k]) || !std::isfinite(m.c2[k])) return false;
+  for (int k = 0; k < 13; ++k)
+    if (!std::isfinite(m.c2_large[k]) || !std::isfinite(m.s2_large[k])) return false;
+  for (int k = 0; k < 15; ++k)
+    if (!std::isfinite(m.c1_integral_large[k])) return false;
+  return true;
+}
+
 void validate_config(c8mi_em_config const& c) {
   require(c.queue_capacity > 0, "queue capacity");
   require(std::isfinite(c.em_transport_cut_mev) && c.em_transport_cut_mev > 0., "EM transport cut");
@@ -77,6 +101,11 @@ void validate_config(c8mi_em_config const& c) {
   require(std::isfinite(c.axis_step_m) && c.axis_step_m > 0., "projection step");
   require(c.profile_crossing_mode <= C8MI_PROFILE_CROSSING_FORWARD, "profile crossing mode");
   require(c.shower_slots <= kMaxShowerSlots, "shower slots above the supported maximum (1024)");
+  require(c.moliere_enabled <= 1, "moliere_enabled must be 0 or 1");
+  if (c.moliere_enabled) {
+    require(valid_moliere_snapshot(c.electron_moliere), "invalid Moliere snapshot");
+    require(c.electron_moliere.component_count <= 4, "Moliere snapshots of more than 4 components are not supported");
+  }
 }
 
 void validate_axis(c8mi_em_shower_axis const& a) {
@@ -193,6 +222,16 @@ EmBackend::EmBackend(c8mi_em_config const& config, c8mi_native_table_set const& 
   h.pair_lpm = config.photon_pair_lpm;
   h.brems_lpm = config.brems_lpm;
   h.thinning = config.thinning;
+  h.moliere = config.electron_moliere;
+  h.moliere_enabled = config.moliere_enabled;
+  for (int i = 0; i < 70; ++i) {
+    h.moliere_i1[i] = config.electron_moliere.c1[i] / (2. * i + 1.);
+    h.moliere_i2[i] = config.electron_moliere.c2[i] / (2. * i + 1.);
+  }
+  for (int i = 0; i < 13; ++i) {
+    h.moliere_i2_large[i] = i ? -0.5 * config.electron_moliere.c2_large[i] / i : 0.;
+    h.moliere_half_over[i] = i ? 0.5 / i : 0.;
+  }
   h.profile_bins = config.profile_bins;
   h.crossing_mode = config.profile_crossing_mode;
   h.bin_width = config.profile_bin_width_g_per_cm2;
@@ -428,6 +467,10 @@ void EmBackend::resize_scratch(std::uint64_t n) {
   block_counts_.allocate(block_count_bytes(cap));
   offsets_.allocate(offset_bytes(cap) / sizeof(unsigned long long));
   vertices_.allocate(cap);
+  if (config_.moliere_enabled) {
+    moves_.allocate(cap * lepton_move_bytes());
+    moved_.allocate(cap);
+  }
   scratch_capacity_ = cap;
 }
 
@@ -464,8 +507,10 @@ void EmBackend::front(bool photon, c8mi_em_run_result& out) {
     if (slot_host_[k].pending[s]) ++slot_host_[k].fronts[s];
   f.vertices = vertices_.data();
   f.vertex_count = vertex_count_.data();
+  f.moves = moves_.data();
+  f.moved = moved_.data();
   // the launches of this front, each followed by an event when timing
-  int kinds[kFrontKernels];
+  int kinds[kFrontKernels + 1];
   int launched = 0;
   auto mark = [&](int kind) {
     if (!timing_) return;
@@ -476,6 +521,14 @@ void EmBackend::front(bool photon, c8mi_em_run_result& out) {
   if (photon) {
     launch_photon_step(context_.data(), config_.profile_bins, f, stream_);
     mark(kKernPhotonStep);
+  } else if (config_.moliere_enabled) {
+    // Molière scattering runs in the finish kernel, with its own register budget
+    launch_lepton_move(context_.data(), config_.profile_bins, f, stream_);
+    mark(kKernLeptonTransport);
+    launch_lepton_finish(context_.data(), config_.profile_bins, f, stream_);
+    mark(kKernLeptonTransport);
+    launch_lepton_vertex(context_.data(), config_.profile_bins, f, stream_);
+    mark(kKernLeptonVertex);
   } else {
     launch_lepton_transport(context_.data(), config_.profile_bins, f, stream_);
     mark(kKernLeptonTransport);

@@ -157,13 +157,15 @@ class EmBackend {
   DeviceBuffer<unsigned char> block_counts_;
   DeviceBuffer<unsigned long long> offsets_;
   DeviceBuffer<VertexRecord> vertices_;
+  DeviceBuffer<unsigned char> moves_;
+  DeviceBuffer<std::uint32_t> moved_;
   DeviceBuffer<unsigned long long> vertex_count_;
   std::uint64_t scratch_capacity_ = 0;
   // timing
   bool timing_ = false;
   std::array<double, kFrontKernels> ms_{};
   std::array<std::uint64_t, kFrontKernels> launches_{};
-  hipEvent_t events_[kFrontKernels + 1] = {};
+  hipEvent_t events_[kFrontKernels + 2] = {};
 };
 
 }  // namespace c8mi::em

@@ -78,6 +78,15 @@ struct EmContext {
   c8mi_photon_pair_lpm pair_lpm;
   c8mi_brems_lpm brems_lpm;
   c8mi_em_thinning thinning;
+  c8mi_moliere_snapshot moliere;
+  // quotients of the Molière integral series, divided once on the host
+  // (IEEE division: the values each term of the series would compute)
+  double moliere_i1[70];        // c1[i] / (2 i + 1)
+  double moliere_i2[70];        // c2[i] / (2 i + 1)
+  double moliere_i2_large[13];  // -0.5 c2_large[i] / i
+  double moliere_half_over[13]; // 0.5 / i
+  std::uint32_t moliere_enabled;
+  std::uint32_t pad2;
   // profile
   std::uint32_t profile_bins;
   std::uint32_t crossing_mode;

@@ -71,6 +71,19 @@ __device__ __forceinline__ T wave_inclusive(T x, std::uint32_t lane) {
   return x;
 }
 
+// The context (tables, environment and physics parameters, Molière coefficients)
+// is read by every lane at uniform addresses throughout a step; a workgroup
+// copies it into the front of its LDS once, so those reads are LDS reads
+// instead of per-lane global loads.
+inline constexpr std::uint32_t kContextWords = (sizeof(EmContext) + 7) / 8;
+static_assert(alignof(EmContext) <= 8, "context words");
+
+__device__ __forceinline__ EmContext const* stage_context(EmContext const* __restrict__ context, long long* lds) {
+  auto const* src = reinterpret_cast<long long const*>(context);
+  for (std::uint32_t k = threadIdx.x; k < kContextWords; k += blockDim.x) lds[k] = src[k];
+  return reinterpret_cast<EmContext const*>(lds);
+}
+
 // Workgroup-private LDS copies of the profile and ledger of up to two showers.
 struct LdsShowers {
   long long* base;
@@ -130,15 +143,16 @@ __device__ __forceinline__ phys::Outputs no_outputs(FrontArgs const& f, std::uin
 __global__ __launch_bounds__(kStepBlock) void photon_step_kernel(EmContext const* __restrict__ context,
                                                                  FrontArgs f) {
   extern __shared__ long long lds[];
-  EmContext const& c = *context;
+  EmContext const* const local = stage_context(context, lds);
   std::uint64_t const first = static_cast<std::uint64_t>(blockIdx.x) * kStepBlock;
   std::uint64_t const last = (first + kStepBlock < f.count ? first + kStepBlock : f.count) - 1;
-  LdsShowers const l = lds_showers(lds, c.profile_bins, f.in.shower[first], f.in.shower[last]);
+  LdsShowers const l = lds_showers(lds + kContextWords, context->profile_bins, f.in.shower[first], f.in.shower[last]);
   __syncthreads();
+  EmContext const& c = *local;
   std::uint64_t const i = first + threadIdx.x;
   if (i < f.count) {
     std::uint32_t const s = f.in.shower[i];
-    phys::Accumulator const acc = accumulator(context, l, s);
+    phys::Accumulator const acc = accumulator(local, l, s);
     phys::Particle p;
     static_cast<c8mi_particle_state&>(p) = load_particle(f.in, i);
     p.shower_key = c.showers[s].key;
@@ -157,18 +171,19 @@ __global__ __launch_bounds__(kStepBlock, 3) void lepton_transport_kernel(EmConte
   extern __shared__ long long lds[];
   __shared__ std::uint32_t wave_vertices[kStepBlock / 64];
   __shared__ unsigned long long block_base;
-  EmContext const& c = *context;
+  EmContext const* const local = stage_context(context, lds);
   std::uint64_t const first = static_cast<std::uint64_t>(blockIdx.x) * kStepBlock;
   std::uint64_t const last = (first + kStepBlock < f.count ? first + kStepBlock : f.count) - 1;
-  LdsShowers const l = lds_showers(lds, c.profile_bins, f.in.shower[first], f.in.shower[last]);
+  LdsShowers const l = lds_showers(lds + kContextWords, context->profile_bins, f.in.shower[first], f.in.shower[last]);
   __syncthreads();
+  EmContext const& c = *local;
   std::uint64_t const i = first + threadIdx.x;
   bool deferred = false;
   std::uint32_t s = 0;
   phys::LeptonVertex vx;
   if (i < f.count) {
     s = f.in.shower[i];
-    phys::Accumulator const acc = accumulator(context, l, s);
+    phys::Accumulator const acc = accumulator(local, l, s);
     phys::Particle p;
     static_cast<c8mi_particle_state&>(p) = load_particle(f.in, i);
     p.shower_key = c.showers[s].key;
@@ -206,21 +221,108 @@ __global__ __launch_bounds__(kStepBlock, 3) void lepton_transport_kernel(EmConte
   }
 }
 
+__global__ __launch_bounds__(kStepBlock, 3) void lepton_move_kernel(EmContext const* __restrict__ context,
+                                                                    FrontArgs f) {
+  extern __shared__ long long lds[];
+  EmContext const* const local = stage_context(context, lds);
+  std::uint64_t const first = static_cast<std::uint64_t>(blockIdx.x) * kStepBlock;
+  std::uint64_t const last = (first + kStepBlock < f.count ? first + kStepBlock : f.count) - 1;
+  LdsShowers const l = lds_showers(lds + kContextWords, context->profile_bins, f.in.shower[first], f.in.shower[last]);
+  __syncthreads();
+  EmContext const& c = *local;
+  std::uint64_t const i = first + threadIdx.x;
+  if (i < f.count) {
+    std::uint32_t const s = f.in.shower[i];
+    phys::Accumulator const acc = accumulator(local, l, s);
+    phys::Particle p;
+    static_cast<c8mi_particle_state&>(p) = load_particle(f.in, i);
+    p.shower_key = c.showers[s].key;
+    phys::LeptonMove mv;
+    bool const moved = phys::lepton_move(c, acc, p, mv);
+    if (moved) static_cast<phys::LeptonMove*>(f.moves)[i] = mv;
+    else f.packed[i] = pack_counts(0, 0, 0, kSlotNone, kSlotNone, false, false, s);
+    f.moved[i] = moved ? 1u : 0u;
+  }
+  __syncthreads();
+  flush(c, l);
+}
+
+// Two waves per SIMD (a little scratch) measured faster than one for the finish.
+__global__ __launch_bounds__(kStepBlock, 2) void lepton_finish_kernel(EmContext const* __restrict__ context,
+                                                                   FrontArgs f) {
+  extern __shared__ long long lds[];
+  __shared__ std::uint32_t wave_vertices[kStepBlock / 64];
+  __shared__ unsigned long long block_base;
+  EmContext const* const local = stage_context(context, lds);
+  std::uint64_t const first = static_cast<std::uint64_t>(blockIdx.x) * kStepBlock;
+  std::uint64_t const last = (first + kStepBlock < f.count ? first + kStepBlock : f.count) - 1;
+  LdsShowers const l = lds_showers(lds + kContextWords, context->profile_bins, f.in.shower[first], f.in.shower[last]);
+  __syncthreads();
+  EmContext const& c = *local;
+  std::uint64_t const i = first + threadIdx.x;
+  bool deferred = false;
+  std::uint32_t s = 0;
+  phys::LeptonVertex vx;
+  if (i < f.count && f.moved[i]) {
+    s = f.in.shower[i];
+    phys::Accumulator const acc = accumulator(local, l, s);
+    phys::LeptonMove const mv = static_cast<phys::LeptonMove const*>(f.moves)[i];
+    phys::Outputs o = no_outputs(f, i);
+    auto const input = [&] {
+      phys::Particle p;
+      static_cast<c8mi_particle_state&>(p) = load_particle(f.in, i);
+      p.shower_key = mv.start.shower_key;
+      return p;
+    };
+    deferred = phys::lepton_finish(c, acc, input, mv, o, vx);
+    write_outputs(f, i, o, s);
+  }
+  std::uint32_t const wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+  unsigned long long const mask = __ballot(deferred);
+  if (lane == 0) wave_vertices[wave] = static_cast<std::uint32_t>(__popcll(mask));
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    std::uint32_t total = 0;
+    for (int w = 0; w < kStepBlock / 64; ++w) total += wave_vertices[w];
+    block_base = total ? atomicAdd(f.vertex_count, static_cast<unsigned long long>(total)) : 0ULL;
+  }
+  flush(c, l);
+  __syncthreads();
+  if (deferred) {
+    std::uint32_t before = 0;
+    for (std::uint32_t w = 0; w < wave; ++w) before += wave_vertices[w];
+    unsigned long long const below = lane ? (mask & (~0ULL >> (64 - lane))) : 0ULL;
+    VertexRecord r;
+    r.parent = vx.parent;
+    r.shower_key = vx.parent.shower_key;
+    r.source = i;
+    r.total_rate = vx.total_rate;
+    r.process_uniform = vx.process_uniform;
+    r.particle_mass_gev = vx.particle_mass_gev;
+    r.vertex_density = vx.vertex_density;
+    r.slot = s;
+    r.pad[0] = r.pad[1] = r.pad[2] = 0;
+    f.vertices[block_base + before + static_cast<std::uint32_t>(__popcll(below))] = r;
+  }
+}
+
 __global__ __launch_bounds__(kStepBlock) void lepton_vertex_kernel(EmContext const* __restrict__ context,
                                                                    FrontArgs f) {
   extern __shared__ long long lds[];
-  EmContext const& c = *context;
+  EmContext const* const local = stage_context(context, lds);
+  EmContext const& c = *local;
   std::uint64_t const vertices = *f.vertex_count;
   for (std::uint64_t first = static_cast<std::uint64_t>(blockIdx.x) * kStepBlock; first < vertices;
        first += static_cast<std::uint64_t>(gridDim.x) * kStepBlock) {
     std::uint64_t const last = (first + kStepBlock < vertices ? first + kStepBlock : vertices) - 1;
-    LdsShowers const l = lds_showers(lds, c.profile_bins, f.vertices[first].slot, f.vertices[last].slot);
+    LdsShowers const l =
+        lds_showers(lds + kContextWords, context->profile_bins, f.vertices[first].slot, f.vertices[last].slot);
     __syncthreads();
     std::uint64_t const j = first + threadIdx.x;
     if (j < vertices) {
       VertexRecord const& r = f.vertices[j];
       std::uint32_t const s = r.slot;
-      phys::Accumulator const acc = accumulator(context, l, s);
+      phys::Accumulator const acc = accumulator(local, l, s);
       phys::LeptonVertex vx;
       static_cast<c8mi_particle_state&>(vx.parent) = r.parent;
       vx.parent.shower_key = r.shower_key;
@@ -474,7 +576,7 @@ unsigned grid_for(std::uint64_t n, int block) {
 }  // namespace
 
 std::size_t step_lds_bytes(std::uint32_t bins) {
-  return 2 * (4u * static_cast<std::size_t>(bins) + kCounters) * sizeof(long long);
+  return (kContextWords + 2 * (4u * static_cast<std::size_t>(bins) + kCounters)) * sizeof(long long);
 }
 
 std::size_t block_count_bytes(std::uint64_t n) { return grid_for(n, kStepBlock) * sizeof(BlockCounts); }
@@ -490,6 +592,20 @@ void launch_photon_step(EmContext const* context, std::uint32_t bins, FrontArgs 
 void launch_lepton_transport(EmContext const* context, std::uint32_t bins, FrontArgs const& f, hipStream_t stream) {
   if (f.count == 0) return;
   lepton_transport_kernel<<<grid_for(f.count, kStepBlock), kStepBlock, step_lds_bytes(bins), stream>>>(context, f);
+  C8MI_HIP(hipGetLastError());
+}
+
+std::size_t lepton_move_bytes() { return sizeof(phys::LeptonMove); }
+
+void launch_lepton_move(EmContext const* context, std::uint32_t bins, FrontArgs const& f, hipStream_t stream) {
+  if (f.count == 0) return;
+  lepton_move_kernel<<<grid_for(f.count, kStepBlock), kStepBlock, step_lds_bytes(bins), stream>>>(context, f);
+  C8MI_HIP(hipGetLastError());
+}
+
+void launch_lepton_finish(EmContext const* context, std::uint32_t bins, FrontArgs const& f, hipStream_t stream) {
+  if (f.count == 0) return;
+  lepton_finish_kernel<<<grid_for(f.count, kStepBlock), kStepBlock, step_lds_bytes(bins), stream>>>(context, f);
   C8MI_HIP(hipGetLastError());
 }
 

@@ -55,6 +55,8 @@ struct FrontArgs {
   unsigned long long* report;         // kReportHeader + 2 * slots
   VertexRecord* vertices;             // lepton fronts: up to `count` records
   unsigned long long* vertex_count;   // zero between fronts
+  void* moves;                        // split lepton fronts: one LeptonMove per source
+  std::uint32_t* moved;               // 1 where the move kernel produced a move
 };
 
 // Start of a shower in a slot: its parameters and first secondary history ID.
@@ -88,11 +90,17 @@ std::size_t block_count_bytes(std::uint64_t n);
 std::size_t offset_bytes(std::uint64_t n);
 
 // Kernel time categories of a front.
+// The split e+- form books its move and finish kernels as the transport.
 enum FrontKernel : int { kKernPhotonStep = 0, kKernLeptonTransport = 1, kKernLeptonVertex = 2, kKernCount = 3,
                          kKernScan = 4, kKernScatter = 5, kFrontKernels = 6 };
 
 void launch_photon_step(EmContext const* context, std::uint32_t bins, FrontArgs const& f, hipStream_t stream);
 void launch_lepton_transport(EmContext const* context, std::uint32_t bins, FrontArgs const& f, hipStream_t stream);
+// Split form of the e+- transport (when Molière scattering is on): the move,
+// then the finish (scattering, profile, outputs, vertex compaction).
+std::size_t lepton_move_bytes();
+void launch_lepton_move(EmContext const* context, std::uint32_t bins, FrontArgs const& f, hipStream_t stream);
+void launch_lepton_finish(EmContext const* context, std::uint32_t bins, FrontArgs const& f, hipStream_t stream);
 void launch_lepton_vertex(EmContext const* context, std::uint32_t bins, FrontArgs const& f, hipStream_t stream);
 void launch_count(FrontArgs const& f, hipStream_t stream);
 void launch_scan(bool photon, FrontArgs const& f, hipStream_t stream);

@@ -72,7 +72,7 @@ enum Reason : int {
   kFrRateEnergy = 5, kFrInverseCdf = 6, kFrLossEnergy = 7, kFrLossQuantile = 8, kFrInvalidQuery = 9,
   kFrZeroRate = 11, kFrCpuOnly = 12, kFrNotImplemented = 13, kFrInvalidFinalState = 14,
   kFrLpmUnavailable = 15, kFrInvalidDensity = 16, kFrContEnergy = 17, kFrContRange = 18,
-  kFrTransportCut = 19, kFrMagneticTransport = 22, kFrMagneticBoundary = 23, kFrGrammage = 24,
+  kFrTransportCut = 19, kFrMoliereUnavailable = 20, kFrMoliereFailed = 21, kFrMagneticTransport = 22, kFrMagneticBoundary = 23, kFrGrammage = 24,
   kFrInverseGrammage = 25, kFrVertexLookup = 26, kFrSelectionReplay = 28
 };
 enum PhotonLimit : int { kPlInteraction = 0, kPlLayer = 1, kPlObservation = 2, kPlEscaped = 3, kPlCut = 4 };

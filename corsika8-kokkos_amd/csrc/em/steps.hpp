@@ -7,6 +7,7 @@
 // scatter kernel replaces it with first_history + child offset + k.
 #pragma once
 
+#include "em/moliere.hpp"
 #include "em/physics.hpp"
 
 namespace c8mi::em::phys {
@@ -253,11 +254,27 @@ struct LeptonVertex {
   double vertex_density;
 };
 
-// The step of one e+- up to its interaction vertex (selection, continuous
-// step, transport, profile and ledger). Returns true when the step ends at an
-// interaction vertex: `vx` then holds what lepton_vertex needs.
-__device__ __forceinline__ bool lepton_transport(EmContext const& c, Accumulator const& acc, Particle const& p,
-                                                 Outputs& o, LeptonVertex& vx) {
+// The move of an e+- step as the finish part needs it.
+struct LeptonMove {
+  Particle start;              // at the step start, layer resolved
+  c8mi_particle_state end;     // before Molière scattering
+  double continuous_deposit;
+  double cut_deposit;
+  double step_grammage;        // chord grammage of the step
+  double vertex_density;
+  double total_rate;
+  double process_uniform;
+  double particle_mass_gev;
+  std::int32_t limit;
+  std::int32_t process_id;
+  std::uint32_t particle_cut;  // cut at the start: no scattering
+  std::uint32_t observed_before_cut;
+};
+
+// The move of one e+- step (selection, continuous step, transport): its end
+// state and limit. Returns false after recording a fallback.
+__device__ __forceinline__ bool lepton_move(EmContext const& c, Accumulator const& acc, Particle const& p,
+                                            LeptonMove& mv) {
   Interaction in;
   Fallback fb{false, 0};
   select_interaction(c, p, in, fb);
@@ -310,6 +327,7 @@ __device__ __forceinline__ bool lepton_transport(EmContext const& c, Accumulator
   double continuous_deposit = 0., cut_deposit = 0.;
   bool observed_before_cut = false;
   double vertex_density = 0.;
+  double step_grammage = 0.;
   if (particle_cut) {
     end.step_id++;
     limit = kLlCut;
@@ -386,6 +404,7 @@ __device__ __forceinline__ bool lepton_transport(EmContext const& c, Accumulator
     env::Grammage const cg = env::grammage(c, layer.node, start.position_m, chord_dir, chord);
     if (cg.status != env::kSuccess) return transport_fallback(kFrGrammage);
     traversed = cg.value;
+    step_grammage = traversed;
     Tq const fe = cq_energy_after_loss(c, start.pid, e0, traversed);
     double final_mev;
     bool cut_reached = false;
@@ -448,6 +467,60 @@ __device__ __forceinline__ bool lepton_transport(EmContext const& c, Accumulator
       }
     }
   }
+  mv.start = start;
+  mv.end = end;
+  mv.continuous_deposit = continuous_deposit;
+  mv.cut_deposit = cut_deposit;
+  mv.step_grammage = step_grammage;
+  mv.vertex_density = vertex_density;
+  mv.total_rate = in.total_rate;
+  mv.process_uniform = in.process_uniform;
+  mv.particle_mass_gev = in.particle_mass_gev;
+  mv.limit = limit;
+  mv.process_id = in.process_id;
+  mv.particle_cut = particle_cut ? 1u : 0u;
+  mv.observed_before_cut = observed_before_cut ? 1u : 0u;
+  return true;
+}
+
+// The rest of the step: Molière scattering of the end direction, profile and
+// ledger, records and outputs. `input()` gives the step's input particle (the
+// fallback state of a failed scattering, as the transport record's interaction
+// state; fetched only then). Returns true when the step ends at an interaction
+// vertex: `vx` then holds what lepton_vertex needs.
+template <class Input>
+__device__ __forceinline__ bool lepton_finish(EmContext const& c, Accumulator const& acc, Input const& input,
+                                              LeptonMove const& mv, Outputs& o, LeptonVertex& vx) {
+  Particle const& start = mv.start;
+  Particle end;
+  static_cast<c8mi_particle_state&>(end) = mv.end;
+  end.shower_key = start.shower_key;
+  int const limit = mv.limit;
+  bool const particle_cut = mv.particle_cut != 0;
+  double const step_grammage = mv.step_grammage;
+  double const continuous_deposit = mv.continuous_deposit, cut_deposit = mv.cut_deposit;
+  bool const observed_before_cut = mv.observed_before_cut != 0;
+
+  // apply_moliere_scattering_stage (MoliereStep.inl:74-99): every finite step;
+  // draws keyed by the start state
+  if (c.moliere_enabled && !particle_cut) {
+    double const u1 = uniform(c, start, kScatteringProcess, 0);
+    double const u2 = uniform(c, start, kScatteringProcess, 1);
+    double const ua = uniform(c, start, kScatteringProcess, 2);
+    double angle;
+    moliere::Quotients const q{c.moliere_i1, c.moliere_i2, c.moliere_i2_large, c.moliere_half_over};
+    int const ms = moliere::angle_2d(c.moliere, q, step_grammage, start.energy_gev * 1000., u1, u2, angle);
+    if (ms != moliere::kNoDeflection) {
+      double dir[3];
+      if (ms != moliere::kSuccess || moliere::deflect_end(start.direction, end.direction, angle, ua, dir) != 0) {
+        // the transport record's interaction state: the vertex for an interaction candidate
+        if (limit == kLlInteraction) record_fallback(acc, end, kFrMoliereFailed, 0, kStageTransport);
+        else record_fallback(acc, input(), kFrMoliereFailed, mv.process_id, kStageTransport);
+        return false;
+      }
+      for (int a = 0; a < 3; ++a) end.direction[a] = dir[a];
+    }
+  }
   o.step = true;
   // accumulate_lepton_profile_step
   double const w = start.weight;
@@ -481,11 +554,19 @@ __device__ __forceinline__ bool lepton_transport(EmContext const& c, Accumulator
   }
   o.interaction = true;
   vx.parent = end;
-  vx.total_rate = in.total_rate;
-  vx.process_uniform = in.process_uniform;
-  vx.particle_mass_gev = in.particle_mass_gev;
-  vx.vertex_density = vertex_density;
+  vx.total_rate = mv.total_rate;
+  vx.process_uniform = mv.process_uniform;
+  vx.particle_mass_gev = mv.particle_mass_gev;
+  vx.vertex_density = mv.vertex_density;
   return true;
+}
+
+// The step of one e+- up to its interaction vertex (move, then finish).
+__device__ __forceinline__ bool lepton_transport(EmContext const& c, Accumulator const& acc, Particle const& p,
+                                                 Outputs& o, LeptonVertex& vx) {
+  LeptonMove mv;
+  if (!lepton_move(c, acc, p, mv)) return false;
+  return lepton_finish(c, acc, [&] { return p; }, mv, o, vx);
 }
 
 // Interaction vertex of an e+- step: vertex reselection at the post-loss

@@ -160,6 +160,40 @@ typedef struct c8mi_brems_lpm {
   double muon_e_lpm_mev;
 } c8mi_brems_lpm;
 
+/* MoliereSnapshot (corsika/kokkos/em/models/MoliereSnapshot.hpp:22-55): the
+ * per-medium state of Molière multiple scattering, byte-compatible with the
+ * reference's record. Components: charge, weight * Z^2, chi_0^2 [MeV^2] and the
+ * Coulomb correction 3.76 (alpha Z)^2; c1/c2 the power series of the projected
+ * correction functions f1, f2 in x = theta^2 / (chi_c^2 B) for small x,
+ * c2_large/s2_large and c1_integral_large their large-x expansions. */
+#define C8MI_MOLIERE_MAX_COMPONENTS 16
+typedef struct c8mi_moliere_component {
+  double nuclear_charge;
+  double weight_zz;
+  double chi0_squared_mev2;
+  double coulomb_correction;
+} c8mi_moliere_component;
+
+typedef struct c8mi_moliere_snapshot {
+  double particle_mass_mev;
+  double electron_mass_mev;
+  double fine_structure_constant;
+  double avogadro_per_mol;
+  double hbar_mev_s;
+  double speed_of_light_cm_per_s;
+  double z_squared_over_a_average;
+  double inverse_weight_zz_sum;
+  double euler_mascheroni;
+  uint32_t component_count;
+  uint32_t maximum_weight_index;
+  c8mi_moliere_component components[C8MI_MOLIERE_MAX_COMPONENTS];
+  double c1[70];
+  double c2[70];
+  double c2_large[13];
+  double s2_large[13];
+  double c1_integral_large[15];
+} c8mi_moliere_snapshot;
+
 /* EmThinningConfig (corsika/kokkos/em/algorithms/EmThinning.hpp:25-30). */
 typedef struct c8mi_em_thinning {
   double threshold_gev;
@@ -185,6 +219,12 @@ typedef struct c8mi_em_config {
   c8mi_photon_pair_lpm photon_pair_lpm;
   c8mi_brems_lpm brems_lpm;
   c8mi_em_thinning thinning;
+  /* Molière multiple scattering of e+- steps (MoliereStep.inl:74-101) with the
+   * analytical correction functions (no interpolation table): enabled when
+   * moliere_enabled is 1; the snapshot is the electron one of the medium. */
+  c8mi_moliere_snapshot electron_moliere;
+  uint32_t moliere_enabled;
+  uint32_t moliere_reserved_;
   uint32_t profile_bins;
   uint32_t profile_crossing_mode;
   double profile_bin_width_g_per_cm2;

@@ -157,6 +157,20 @@ int oracle_em_run_batch(const c8mi_em_config* config, const c8mi_native_table_se
                         const c8mi_particle_state* primaries, uint64_t shower_count,
                         uint64_t maximum_fronts, int threads, or_em_stats* stats);
 
+/* Molière probes (analytical correction functions). */
+double oracle_moliere_f1(const c8mi_moliere_snapshot* m, double x);
+double oracle_moliere_f2(const c8mi_moliere_snapshot* m, double x);
+double oracle_moliere_integral_f1(const c8mi_moliere_snapshot* m, double x);
+double oracle_moliere_integral_f2(const c8mi_moliere_snapshot* m, double x);
+double oracle_moliere_solve_b(double rhs);
+double oracle_moliere_normal_ppf(double p);
+int oracle_moliere_evaluate(const c8mi_moliere_snapshot* m, double grammage, double e_mev, double theta,
+                            double* density, double* cumulative);
+int oracle_moliere_angle(const c8mi_moliere_snapshot* m, double grammage, double e_mev, double u1, double u2,
+                         uint32_t* iterations, double* angle);
+int oracle_moliere_direction(const double start[3], const double tracked[3], double angle, double au,
+                             double out[3]);
+
 int oracle_em_environment_query(const c8mi_environment* e, const double position[3],
                                 const double direction[3], int32_t node, int32_t* out_node,
                                 double* out_density);

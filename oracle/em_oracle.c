@@ -54,6 +54,7 @@
 #define RP_DISTANCE 0x454d0001U
 #define RP_COLUMN 0x454d0002U
 #define RP_SELECTION 0x454d0004U
+#define RP_SCATTERING 0x454d0003U
 #define D_PAIR_SPLIT 1ULL
 #define D_PAIR_AZIMUTH 2ULL
 #define D_PAIR_E_POLAR 3ULL
@@ -90,6 +91,7 @@ enum { FR_UNSUPPORTED_PARTICLE = 0, FR_UNSUPPORTED_GEOMETRY = 2, FR_PARTICLE_TAB
        FR_INVALID_FINAL_STATE = 14, FR_LPM_UNAVAILABLE = 15, FR_INVALID_DENSITY = 16,
        FR_CONT_ENERGY_OOR = 17, FR_CONT_RANGE_OOR = 18, FR_TRANSPORT_CUT = 19,
        FR_MAGNETIC_TRANSPORT = 22, FR_MAGNETIC_BOUNDARY = 23, FR_GRAMMAGE = 24,
+       FR_MOLIERE_UNAVAILABLE = 20, FR_MOLIERE_FAILED = 21,
        FR_INVERSE_GRAMMAGE = 25, FR_VERTEX_LOOKUP = 26, FR_SELECTION_REPLAY = 28 };
 /* EnvironmentStatus (EnvironmentQuery.hpp:24-32) */
 enum { ES_SUCCESS = 0, ES_INVALID = 1, ES_NON_FINITE = 2, ES_OUTSIDE = 3, ES_NO_FORWARD = 5,
@@ -1653,6 +1655,307 @@ static void photon_source(shower_t* s, const ps_t* p, pout_t* o, uint64_t child_
                   ((th.keep & 1U) ? e1 * th.w1 : 0.) + ((th.keep & 2U) ? e2 * th.w2 : 0.));
 }
 
+/* ========================== Molière multiple scattering ==========================
+ * sample_moliere_scattering_angle_2d with an empty interpolation view (the
+ * analytical correction functions), apply_moliere_direction and its
+ * step-compatible form (MoliereScattering.inl:219-688), and the scattering stage
+ * of a completed e+- step (MoliereStep.inl:11-101). */
+
+enum { MS_SUCCESS = 0, MS_NO_DEFLECTION = 1, MS_INVALID_SNAPSHOT = 2, MS_INVALID_INPUT = 3,
+       MS_NON_CONVERGENT = 4, MS_NON_FINITE = 5 };
+#define MOLIERE_PI 3.141592653589793238462643383279502884
+#define MOLIERE_SQRT2 1.414213562373095048801688724209698079
+#define MOLIERE_MIN_B 4.5
+#define MOLIERE_MIN_B_RHS 2.9959226032237258
+
+typedef struct {
+  double inverse_scale[C8MI_MOLIERE_MAX_COMPONENTS];
+  double inverse_b[C8MI_MOLIERE_MAX_COMPONENTS];
+  double density_weight[C8MI_MOLIERE_MAX_COMPONENTS];
+  double prefactor;
+  double beta_squared;
+} mol_state_t;
+
+static int moliere_valid(const c8mi_moliere_snapshot* m) {
+#define POS(v) (finite(v) && (v) > 0.)
+  if (m->component_count == 0 || m->component_count > C8MI_MOLIERE_MAX_COMPONENTS ||
+      m->maximum_weight_index >= m->component_count || !POS(m->particle_mass_mev) ||
+      !POS(m->electron_mass_mev) || !POS(m->fine_structure_constant) || !POS(m->avogadro_per_mol) ||
+      !POS(m->hbar_mev_s) || !POS(m->speed_of_light_cm_per_s) || !POS(m->z_squared_over_a_average) ||
+      !POS(m->inverse_weight_zz_sum) || !finite(m->euler_mascheroni))
+    return 0;
+  for (uint32_t i = 0; i < m->component_count; ++i) {
+    const c8mi_moliere_component* c = &m->components[i];
+    if (!POS(c->nuclear_charge) || !POS(c->weight_zz) || !POS(c->chi0_squared_mev2) ||
+        !POS(c->coulomb_correction))
+      return 0;
+  }
+#undef POS
+  for (int i = 0; i < 70; ++i)
+    if (!finite(m->c1[i]) || !finite(m->c2[i])) return 0;
+  for (int i = 0; i < 13; ++i)
+    if (!finite(m->c2_large[i]) || !finite(m->s2_large[i])) return 0;
+  for (int i = 0; i < 15; ++i)
+    if (!finite(m->c1_integral_large[i])) return 0;
+  return 1;
+}
+
+/* Acklam's rational approximation of the normal quantile (the Newton seed). */
+double oracle_moliere_normal_ppf(double p) {
+  static const double a[] = {-3.969683028665376e+01, 2.209460984245205e+02, -2.759285104469687e+02,
+                             1.383577518672690e+02, -3.066479806614716e+01, 2.506628277459239e+00};
+  static const double b[] = {-5.447609879822406e+01, 1.615858368580409e+02, -1.556989798598866e+02,
+                             6.680131188771972e+01, -1.328068155288572e+01};
+  static const double c[] = {-7.784894002430293e-03, -3.223964580411365e-01, -2.400758277161838e+00,
+                             -2.549732539343734e+00, 4.374664141464968e+00, 2.938163982698783e+00};
+  static const double d[] = {7.784695709041462e-03, 3.224671290700398e-01, 2.445134137142996e+00,
+                             3.754408661907416e+00};
+  const double lo = 0.02425, hi = 1. - lo;
+  if (p < lo) {
+    double q = sqrt(-2. * log(p));
+    return (((((c[0] * q + c[1]) * q + c[2]) * q + c[3]) * q + c[4]) * q + c[5]) /
+           ((((d[0] * q + d[1]) * q + d[2]) * q + d[3]) * q + 1.);
+  }
+  if (p <= hi) {
+    double q = p - 0.5, r = q * q;
+    return (((((a[0] * r + a[1]) * r + a[2]) * r + a[3]) * r + a[4]) * r + a[5]) * q /
+           (((((b[0] * r + b[1]) * r + b[2]) * r + b[3]) * r + b[4]) * r + 1.);
+  }
+  double q = sqrt(-2. * log(1. - p));
+  return -(((((c[0] * q + c[1]) * q + c[2]) * q + c[3]) * q + c[4]) * q + c[5]) /
+         ((((d[0] * q + d[1]) * q + d[2]) * q + d[3]) * q + 1.);
+}
+
+/* B - ln B = rhs on the B > 1 branch: fourth-order asymptotic seed, one Halley step. */
+double oracle_moliere_solve_b(double rhs) {
+  if (!finite(rhs) || rhs < MOLIERE_MIN_B_RHS) return 0.;
+  double l = log(rhs), ir = 1. / rhs, ir2 = ir * ir, l2 = l * l, l3 = l2 * l;
+  double b = rhs + l + l * ir + l * (2. - l) * 0.5 * ir2 + l * (6. - 9. * l + 2. * l2) / 6. * ir2 * ir +
+             l * (12. - 36. * l + 22. * l2 - 3. * l3) / 12. * ir2 * ir2;
+  if (!finite(b) || !(b > 1.)) return 0.;
+  double f = b - log(b) - rhs, ib = 1. / b, df = 1. - ib;
+  double den = 2. * df * df - f * ib * ib;
+  if (!finite(den) || den == 0.) return 0.;
+  b -= 2. * f * df / den;
+  return finite(b) && b >= MOLIERE_MIN_B ? b : 0.;
+}
+
+double oracle_moliere_f1(const c8mi_moliere_snapshot* m, double x) {
+  if (x > 12.) return 0.5 * sqrt(MOLIERE_PI) / (pow(x, 1.5) * pow(1. - 4.5 / x, 2. / 3.));
+  double sum = m->c1[69];
+  for (int i = 68; i >= 0; --i) sum = sum * x + m->c1[i];
+  return sum;
+}
+
+double oracle_moliere_f2(const c8mi_moliere_snapshot* m, double x) {
+  if (x > 4.25 * 4.25) {
+    if (x <= 6.5 * 6.5)
+      return 0.00013567765224589459194769192063035 * x - 0.0022635502525409950842771866774683 * sqrt(x) +
+             0.0098037758070269476889935233998585;
+    double sum = 0.;
+    for (int i = 2; i < 13; ++i) sum += m->c2_large[i] * (0.5 * log(x) + m->s2_large[i]) * pow(x, -(i + 0.5));
+    return sum;
+  }
+  double sum = m->c2[69];
+  for (int i = 68; i >= 0; --i) sum = sum * x + m->c2[i];
+  return sum;
+}
+
+double oracle_moliere_integral_f1(const c8mi_moliere_snapshot* m, double x) {
+  if (x > 12.) {
+    double sum = m->c1_integral_large[14];
+    for (int i = 13; i >= 0; --i) sum = m->c1_integral_large[i] + sum / x;
+    return sum;
+  }
+  double sum = m->c1[69] / (2. * 69. + 1.);
+  for (int i = 68; i >= 0; --i) sum = sum * x + m->c1[i] / (2. * i + 1.);
+  return sum * sqrt(x > 0. ? x : 0.);
+}
+
+double oracle_moliere_integral_f2(const c8mi_moliere_snapshot* m, double x) {
+  if (x > 4.25 * 4.25) {
+    if (x <= 6.5 * 6.5)
+      return -0.00026360133958801203364619158975302 * x + 0.0039965027465457608410459577896745 * sqrt(x) -
+             0.016305842044996649714549974419242;
+    double sum = 0.;
+    for (int i = 2; i < 13; ++i)
+      sum += -0.5 * m->c2_large[i] / i * (0.5 / i + 0.5 * log(x) + m->s2_large[i]) * pow(x, -i);
+    return sum;
+  }
+  double sum = m->c2[69] / (2. * 69. + 1.);
+  for (int i = 68; i >= 0; --i) sum = sum * x + m->c2[i] / (2. * i + 1.);
+  return sum * sqrt(x > 0. ? x : 0.);
+}
+
+static int moliere_build(const c8mi_moliere_snapshot* m, double grammage, double e_mev, mol_state_t* st) {
+  if (m->component_count == 0 || m->component_count > C8MI_MOLIERE_MAX_COMPONENTS) return MS_INVALID_SNAPSHOT;
+  if (!moliere_valid(m)) return MS_INVALID_SNAPSHOT;
+  if (!finite(grammage) || grammage < 0. || !finite(e_mev) || !(e_mev > m->particle_mass_mev))
+    return MS_INVALID_INPUT;
+  if (grammage == 0.) return MS_NO_DEFLECTION;
+  double mass = m->particle_mass_mev;
+  double p2 = (e_mev - mass) * (e_mev + mass);
+  double beta2 = 1. / (1. + mass * mass / p2);
+  st->beta_squared = beta2;
+  double bp2 = p2 / e_mev;
+  bp2 *= bp2;
+  double chi_c2 = 4. * MOLIERE_PI * m->avogadro_per_mol * m->fine_structure_constant *
+                  m->fine_structure_constant * m->hbar_mev_s * m->hbar_mev_s * m->speed_of_light_cm_per_s *
+                  m->speed_of_light_cm_per_s * grammage / bp2 * m->z_squared_over_a_average;
+  if (!finite(chi_c2) || !(chi_c2 > 0.)) return MS_NON_FINITE;
+  double b_max = 0.;
+  for (uint32_t i = 0; i < m->component_count; ++i) {
+    double chi_a2 = m->components[i].chi0_squared_mev2 / p2 * (1.13 + m->components[i].coulomb_correction / beta2);
+    double b = oracle_moliere_solve_b(log(chi_c2 / chi_a2) + 1. - 2. * m->euler_mascheroni);
+    if (!(b > 0.)) return MS_NO_DEFLECTION;
+    double scale = chi_c2 * b;
+    if (!finite(scale) || !(scale > 0.)) return MS_NON_FINITE;
+    st->inverse_scale[i] = 1. / scale;
+    st->inverse_b[i] = 1. / b;
+    st->density_weight[i] = m->components[i].weight_zz / sqrt(scale * MOLIERE_PI);
+    if (!finite(st->inverse_scale[i]) || !(st->inverse_scale[i] > 0.) || !finite(st->inverse_b[i]) ||
+        !(st->inverse_b[i] > 0.) || !finite(st->density_weight[i]) || !(st->density_weight[i] > 0.))
+      return MS_NON_FINITE;
+    if (i == m->maximum_weight_index) b_max = b;
+  }
+  st->prefactor = sqrt(chi_c2 * b_max);
+  return finite(st->prefactor) && st->prefactor > 0. ? MS_SUCCESS : MS_NON_FINITE;
+}
+
+static void moliere_evaluate(const c8mi_moliere_snapshot* m, const mol_state_t* st, double theta,
+                             double* density, double* cumulative) {
+  double d = 0., cdf = 0., t2 = theta * theta;
+  for (uint32_t i = 0; i < m->component_count; ++i) {
+    double x = t2 * st->inverse_scale[i], ib = st->inverse_b[i], ib2 = ib * ib;
+    d += st->density_weight[i] * (exp(-x) + oracle_moliere_f1(m, x) * ib + oracle_moliere_f2(m, x) * ib2);
+    cdf += m->components[i].weight_zz *
+           (0.5 * erf(sqrt(x > 0. ? x : 0.)) +
+            sqrt(1. / MOLIERE_PI) * (oracle_moliere_integral_f1(m, x) * ib + oracle_moliere_integral_f2(m, x) * ib2));
+  }
+  *density = d * m->inverse_weight_zz_sum;
+  cdf *= m->inverse_weight_zz_sum;
+  *cumulative = theta < 0. ? -cdf : cdf;
+}
+
+static int moliere_one(const c8mi_moliere_snapshot* m, const mol_state_t* st, double u, uint32_t* iterations,
+                       double* angle) {
+  *iterations = 0;
+  *angle = 0.;
+  if (!finite(u) || !(u > 0.) || !(u < 1.)) return MS_INVALID_INPUT;
+  if (u == 0.5) return MS_SUCCESS;
+  double current = st->prefactor * (oracle_moliere_normal_ppf(u) / MOLIERE_SQRT2);
+  double target = u - 0.5;
+  for (uint32_t it = 1; it <= 100; ++it) {
+    double previous = current, density, cdf;
+    moliere_evaluate(m, st, previous, &density, &cdf);
+    *iterations = it;
+    if (!finite(density) || density == 0.) {
+      *angle = previous;
+      return MS_NON_FINITE;
+    }
+    current = previous - (cdf - target) / density;
+    *angle = current;
+    if (!finite(current)) return MS_NON_FINITE;
+    if (fabs((previous - current) / current) <= 5.e-5) return MS_SUCCESS;
+  }
+  *iterations = 100;
+  return MS_NON_CONVERGENT;
+}
+
+/* Density and CDF of one projected angle for a step (test probe). */
+int oracle_moliere_evaluate(const c8mi_moliere_snapshot* m, double grammage, double e_mev, double theta,
+                            double* density, double* cumulative) {
+  mol_state_t st;
+  memset(&st, 0, sizeof st);
+  int status = moliere_build(m, grammage, e_mev, &st);
+  if (status != MS_SUCCESS) return status;
+  moliere_evaluate(m, &st, theta, density, cumulative);
+  return MS_SUCCESS;
+}
+
+/* Status, iteration count and the space angle of two projected draws. */
+int oracle_moliere_angle(const c8mi_moliere_snapshot* m, double grammage, double e_mev, double u1, double u2,
+                         uint32_t* iterations, double* angle) {
+  mol_state_t st;
+  memset(&st, 0, sizeof st);
+  *iterations = 0;
+  *angle = 0.;
+  int status = moliere_build(m, grammage, e_mev, &st);
+  if (status != MS_SUCCESS) return status;
+  uint32_t i1, i2;
+  double a1, a2;
+  status = moliere_one(m, &st, u1, &i1, &a1);
+  if (status != MS_SUCCESS) {
+    *iterations = i1;
+    *angle = a1;
+    return status;
+  }
+  status = moliere_one(m, &st, u2, &i2, &a2);
+  if (status != MS_SUCCESS) {
+    *iterations = i2;
+    *angle = a2;
+    return status;
+  }
+  *iterations = i1 + i2;
+  *angle = sqrt(a1 * a1 + a2 * a2);
+  return finite(*angle) ? MS_SUCCESS : MS_NON_FINITE;
+}
+
+static int moliere_rotate(const double d[3], double angle, double au, double out[3]) {
+  if (!finite(angle) || angle < 0. || !finite(au) || !(au > 0.) || !(au < 1.)) return MS_INVALID_INPUT;
+  double n2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+  const double lo = (1. - 1.e-12) * (1. - 1.e-12), hi = (1. + 1.e-12) * (1. + 1.e-12);
+  if (!finite(n2) || n2 < lo || n2 > hi) return MS_INVALID_INPUT;
+  double a1[3] = {0., 0., 0.};
+  if (d[0] > 0.1) {
+    a1[0] = -d[1];
+    a1[1] = d[0];
+  } else {
+    a1[1] = -d[2];
+    a1[2] = d[1];
+  }
+  double an = sqrt(a1[0] * a1[0] + a1[1] * a1[1] + a1[2] * a1[2]);
+  if (!finite(an) || !(an > 0.)) return MS_INVALID_INPUT;
+  for (int k = 0; k < 3; ++k) a1[k] /= an;
+  double ax[3] = {a1[1] * d[2] - a1[2] * d[1], a1[2] * d[0] - a1[0] * d[2], a1[0] * d[1] - a1[1] * d[0]};
+  double ct = cos(angle), mst = -sin(angle);
+  double z[3];
+  for (int k = 0; k < 3; ++k) z[k] = d[k] * ct + ax[k] * mst;
+  double az = au * 2. * MOLIERE_PI;
+  double ca = cos(az), msa = -sin(az);
+  double dz[3] = {d[1] * z[2] - d[2] * z[1], d[2] * z[0] - d[0] * z[2], d[0] * z[1] - d[1] * z[0]};
+  double par = d[0] * z[0] + d[1] * z[1] + d[2] * z[2];
+  for (int k = 0; k < 3; ++k) out[k] = z[k] * ca + dz[k] * msa + d[k] * par * (1. - ca);
+  double on = sqrt(out[0] * out[0] + out[1] * out[1] + out[2] * out[2]);
+  if (!finite(on) || !(on > 0.)) return MS_NON_FINITE;
+  for (int k = 0; k < 3; ++k) out[k] /= on;
+  return MS_SUCCESS;
+}
+
+/* normalize(tracked + rotate(start) - start). */
+int oracle_moliere_direction(const double start[3], const double tracked[3], double angle, double au,
+                             double out[3]) {
+  double scattered[3];
+  int status = moliere_rotate(start, angle, au, scattered);
+  if (status != MS_SUCCESS) {
+    if (status == MS_NON_FINITE)
+      for (int k = 0; k < 3; ++k) out[k] = scattered[k];
+    return status;
+  }
+  double n2 = tracked[0] * tracked[0] + tracked[1] * tracked[1] + tracked[2] * tracked[2];
+  const double lo = (1. - 1.e-12) * (1. - 1.e-12), hi = (1. + 1.e-12) * (1. + 1.e-12);
+  if (!finite(n2) || n2 < lo || n2 > hi) return MS_INVALID_INPUT;
+  double norm2 = 0.;
+  for (int k = 0; k < 3; ++k) {
+    out[k] = tracked[k] + scattered[k] - start[k];
+    norm2 += out[k] * out[k];
+  }
+  double norm = sqrt(norm2);
+  if (!finite(norm) || !(norm > 0.)) return MS_NON_FINITE;
+  for (int k = 0; k < 3; ++k) out[k] /= norm;
+  return MS_SUCCESS;
+}
+
 /* ========================== lepton step ========================== */
 
 typedef struct {
@@ -1662,11 +1965,13 @@ typedef struct {
   double continuous_deposit;
   double cut_deposit;
   int observed_before_cut;
+  double traversed;   /* chord grammage of the step (g/cm^2) */
+  int zero_length;    /* particle cut at the start: no scattering stage */
 } ltrans_t;
 
 /* prepare_lepton_continuous_step + transport_lepton (LeptonContinuousStep.inl:43-120,
- * LeptonTransportStep.inl:27-513) in a field-free environment without Moliere
- * deflection. */
+ * LeptonTransportStep.inl:27-513) in a field-free environment; the Molière stage
+ * follows in lepton_source. */
 static void transport_lepton(const ctx_t* x, const inter_t* in, ltrans_t* out, fb_t* fb) {
   const c8mi_environment* env = x->env;
   const cont_t* c = &x->cont;
@@ -1744,6 +2049,7 @@ static void transport_lepton(const ctx_t* x, const inter_t* in, ltrans_t* out, f
     out->in.particle_mass_gev = native_mass.value / 1000.;
     out->start = start;
     out->end = start;
+    out->zero_length = 1;
     out->end.step_id++;
     out->limit = LL_CUT;
     out->cut_deposit = (e0 - mass.value) / 1000.;
@@ -1850,6 +2156,7 @@ static void transport_lepton(const ctx_t* x, const inter_t* in, ltrans_t* out, f
     return;
   }
   traversed = cg.value;
+  out->traversed = traversed;
   /* evaluate_lepton_continuous_loss (LeptonContinuousStep.inl:122-159) */
   tq_t fe = cq_energy_after_loss(c, start.pid, e0, traversed);
   double final_mev;
@@ -2007,6 +2314,29 @@ static void lepton_source(shower_t* s, const ps_t* p, lout_t* o, uint64_t child_
   if (fb.flag) {
     record_fallback(s, &fb, STAGE_TRANSPORT);
     return;
+  }
+  /* apply_moliere_scattering_stage (MoliereStep.inl:74-99): draws keyed by the
+   * start state; the end direction becomes normalize(tracked + scattered - start) */
+  if (cfg->moliere_enabled && !tr.zero_length) {
+    const ps_t* st = &tr.start;
+    double u1 = uniform(x->seed, x->shower, st, RP_SCATTERING, 0);
+    double u2 = uniform(x->seed, x->shower, st, RP_SCATTERING, 1);
+    double ua = uniform(x->seed, x->shower, st, RP_SCATTERING, 2);
+    uint32_t its;
+    double angle;
+    int ms = oracle_moliere_angle(&cfg->electron_moliere, tr.traversed, st->energy_gev * 1000., u1, u2, &its,
+                                  &angle);
+    if (ms != MS_NO_DEFLECTION) {
+      double dir[3];
+      if (ms != MS_SUCCESS ||
+          oracle_moliere_direction(tr.start.direction, tr.end.direction, angle, ua, dir) != MS_SUCCESS) {
+        set_fallback(&fb, &tr.in, FR_MOLIERE_FAILED);
+        record_fallback(s, &fb, STAGE_TRANSPORT);
+        return;
+      }
+      for (int k = 0; k < 3; ++k) tr.end.direction[k] = dir[k];
+    }
+    if (tr.limit == LL_INTERACTION) tr.in.particle = tr.end;
   }
   s->stats.lepton_steps++;
   /* accumulate_lepton_profile_step (ProfileAccumulation.inl:300-375) */

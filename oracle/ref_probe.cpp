@@ -6,6 +6,7 @@
 // the reference is copied here.
 #include <corsika/kokkos/em/algorithms/EmThinning.hpp>
 #include <corsika/kokkos/em/models/BremsLpm.hpp>
+#include <corsika/kokkos/em/models/MoliereScattering.hpp>
 #include <corsika/kokkos/em/models/PhotonPairKinematics.hpp>
 #include <corsika/kokkos/em/models/PhotonPairLpm.hpp>
 #include <corsika/kokkos/em/random/Philox.hpp>
@@ -109,7 +110,51 @@ c8mi_native_result convert(ck::NativeQueryResult const& r) {
 
 }  // namespace
 
+static_assert(sizeof(c8mi_moliere_snapshot) == sizeof(ck::MoliereSnapshot));
+static_assert(offsetof(c8mi_moliere_snapshot, components) == offsetof(ck::MoliereSnapshot, components_));
+static_assert(offsetof(c8mi_moliere_snapshot, c1) == offsetof(ck::MoliereSnapshot, c1_));
+static_assert(offsetof(c8mi_moliere_snapshot, c2_large) == offsetof(ck::MoliereSnapshot, c2Large_));
+static_assert(offsetof(c8mi_moliere_snapshot, c1_integral_large) ==
+              offsetof(ck::MoliereSnapshot, c1IntegralLarge_));
+
+static ck::MoliereSnapshot as_ref(c8mi_moliere_snapshot const* s) {
+  ck::MoliereSnapshot r;
+  std::memcpy(&r, s, sizeof r);
+  return r;
+}
+
 extern "C" {
+
+// f1, f2 and their integrals (analytical functions, no interpolation view).
+void ref_moliere_functions(c8mi_moliere_snapshot const* s, double x, double out[4]) {
+  auto const r = as_ref(s);
+  out[0] = ck::moliere_detail::f1_m(r, x);
+  out[1] = ck::moliere_detail::f2_m(r, x);
+  out[2] = ck::moliere_detail::integral_f1_m(r, x);
+  out[3] = ck::moliere_detail::integral_f2_m(r, x);
+}
+
+double ref_moliere_solve_b(double rhs) { return ck::moliere_detail::solve_moliere_b(rhs); }
+
+double ref_moliere_normal_ppf(double p) { return ck::moliere_detail::normal_ppf(p); }
+
+int ref_moliere_angle(c8mi_moliere_snapshot const* s, double grammage, double e_mev, double u1, double u2,
+                      uint32_t* iterations, double* angle) {
+  auto const r = as_ref(s);
+  auto const a = ck::sample_moliere_scattering_angle_2d(r, grammage, e_mev, 0.9 * e_mev, u1, u2, true);
+  *iterations = a.iterations_;
+  *angle = a.angleRad_;
+  return static_cast<int>(a.status_);
+}
+
+int ref_moliere_direction(double const start[3], double const tracked[3], double angle, double azimuth_uniform,
+                          double out[3]) {
+  double s3[3] = {start[0], start[1], start[2]}, t3[3] = {tracked[0], tracked[1], tracked[2]};
+  auto const d = ck::apply_moliere_direction_cpu_step_compatible(s3, t3, angle, azimuth_uniform);
+  for (int k = 0; k < 3; ++k) out[k] = d.direction_[k];
+  return static_cast<int>(d.status_);
+}
+
 
 void ref_philox4x32_10(std::uint32_t const counter[4], std::uint32_t const key[2],
                        std::uint32_t out[4]) {

@@ -299,3 +299,46 @@ def em_scalar(name, *args):
     outs = [c_double(), c_double()] if name == "oracle_em_compton_cosines" else [c_double()]
     st = fn(*[c_double(a) for a in args], *[byref(o) for o in outs])
     return (st, *[o.value for o in outs])
+
+
+# ---- Molière (oracle/em_oracle.c; reference MoliereScattering.hpp via oracle/_ref) ----
+
+def moliere_functions(snapshot, x, reference=False):
+    """f1, f2 and their integrals at x: oracle (4 separate functions) or the
+    reference header."""
+    if reference:
+        out = (c_double * 4)()
+        load_ref().ref_moliere_functions(byref(snapshot), c_double(x), out)
+        return tuple(out)
+    lib = load()
+    vals = []
+    for name in ("oracle_moliere_f1", "oracle_moliere_f2", "oracle_moliere_integral_f1",
+                 "oracle_moliere_integral_f2"):
+        fn = getattr(lib, name)
+        fn.restype = c_double
+        vals.append(fn(byref(snapshot), c_double(x)))
+    return tuple(vals)
+
+
+def moliere_scalar(name, value, reference=False):
+    lib = load_ref() if reference else load()
+    fn = getattr(lib, ("ref_moliere_" if reference else "oracle_moliere_") + name)
+    fn.restype = c_double
+    return fn(c_double(value))
+
+
+def moliere_angle(snapshot, grammage, e_mev, u1, u2, reference=False):
+    lib = load_ref() if reference else load()
+    fn = lib.ref_moliere_angle if reference else lib.oracle_moliere_angle
+    its, angle = ctypes.c_uint32(), c_double()
+    st = fn(byref(snapshot), c_double(grammage), c_double(e_mev), c_double(u1), c_double(u2), byref(its),
+            byref(angle))
+    return st, its.value, angle.value
+
+
+def moliere_direction(start, tracked, angle, azimuth_uniform, reference=False):
+    lib = load_ref() if reference else load()
+    fn = lib.ref_moliere_direction if reference else lib.oracle_moliere_direction
+    out = (c_double * 3)()
+    st = fn((c_double * 3)(*start), (c_double * 3)(*tracked), c_double(angle), c_double(azimuth_uniform), out)
+    return st, tuple(out)

@@ -35,10 +35,10 @@ def tables():
 
 
 def run_pair(tables, pid, energy, seed, shower_id=1, thinning=None, start=(0.0, 0.0, 112750.0),
-             direction=(0.0, 0.0, -1.0), primaries=None):
+             direction=(0.0, 0.0, -1.0), primaries=None, moliere=False):
     atm = em.Atmosphere()
     cfg = em.make_config(airtables, seed=seed, atmosphere=atm, thinning=thinning, start=start,
-                         direction=direction)
+                         direction=direction, moliere=moliere)
     prim = primaries if primaries is not None else em.particle(pid, energy, start, direction, history_id=1)
     want = oracle_lib.em_run_shower(cfg, tables, atm, prim, shower_id=shower_id, first_history_id=2)
     dev = em.EmBackend(cfg, tables, atm)
@@ -113,6 +113,16 @@ def test_inclined_shower_reaches_ground_and_matches(tables):
     cfg, want, got = run_pair(tables, 22, 20.0, 7, start=start, direction=d)
     compare(cfg, want, got, 20.0)
     assert got["stats"]["observations"] > 0
+
+
+@pytest.mark.parametrize("pid,energy,seed,start", [(11, 5.0, 13, 1000.0), (22, 10.0, 14, 112750.0),
+                                                   (-11, 3.0, 15, 4000.0)])
+def test_moliere_shower_matches_oracle(tables, pid, energy, seed, start):
+    """Molière multiple scattering on every finite e+- step (the reference's
+    production setting): device and oracle agree shower for shower."""
+    cfg, want, got = run_pair(tables, pid, energy, seed, start=(0.0, 0.0, start), moliere=True)
+    compare(cfg, want, got, energy)
+    assert got["stats"]["lepton_steps"] > 100
 
 
 def test_mixed_primary_batch(tables):
