@@ -460,12 +460,19 @@ class EmBackend:
         if self.handle:
             _check(library().c8mi_em_destroy(self.handle))
             self.handle = c_void_p()
+        self._radio = None
 
     def __del__(self):
         try:
             self.close()
         except Exception:
             pass
+
+    def attach_radio(self, radio):
+        """Project every e+- step of the later lepton fronts onto `radio` (a
+        c8mi.RadioAccumulator; None detaches). Keep it open while attached."""
+        _check(library().c8mi_em_attach_radio(self.handle, radio.handle if radio is not None else None))
+        self._radio = radio
 
     def begin_shower(self, shower_id=1, first_history_id=2):
         _check(library().c8mi_em_begin_shower(self.handle, c_uint64(shower_id), c_uint64(first_history_id)))

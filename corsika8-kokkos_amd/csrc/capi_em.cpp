@@ -6,6 +6,7 @@
 
 #include "c8mi/c8mi_em.h"
 #include "capi_guard.hpp"
+#include "capi_handles.hpp"
 #include "em/device_tables.hpp"
 #include "em/em_backend.hpp"
 #include "em/em_launch.hpp"
@@ -265,6 +266,13 @@ int c8mi_em_serve(c8mi_em* em, c8mi_em_request const* requests, uint64_t count,
     require_handle(em);
     if (!options || !totals) throw std::invalid_argument("null argument");
     em->backend.serve(requests, count, *options, results, *totals);
+  });
+}
+
+int c8mi_em_attach_radio(c8mi_em* em, c8mi_radio* radio) {
+  return guarded([&] {
+    require_handle(em);
+    em->backend.attach_radio(radio ? &radio->accumulator : nullptr);
   });
 }
 

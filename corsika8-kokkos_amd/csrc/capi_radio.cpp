@@ -11,12 +11,8 @@
 #include "c8mi/c8mi_radio.h"
 #include "radio/radio_accumulator.hpp"
 #include "capi_guard.hpp"
+#include "capi_handles.hpp"
 #include "runtime.hpp"
-
-struct c8mi_radio {
-  c8mi::radio::RadioAccumulator accumulator;
-  c8mi_radio(c8mi_radio_config const& config, hipStream_t stream) : accumulator(config, stream) {}
-};
 
 namespace c8mi {
 

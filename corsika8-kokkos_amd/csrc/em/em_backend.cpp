@@ -296,6 +296,11 @@ void EmBackend::activate(std::uint32_t slot, std::uint64_t shower_id) {
 
 EmBackend::~EmBackend() {
   if (stream_) (void)hipStreamSynchronize(stream_);
+  for (int b = 0; b < 2; ++b) {
+    if (track_busy_[b]) (void)hipEventSynchronize(track_done_[b]);
+    if (track_ready_[b]) (void)hipEventDestroy(track_ready_[b]);
+    if (track_done_[b]) (void)hipEventDestroy(track_done_[b]);
+  }
   for (auto& e : events_)
     if (e) (void)hipEventDestroy(e);
   if (init_copied_) (void)hipEventDestroy(init_copied_);
@@ -474,6 +479,24 @@ void EmBackend::resize_scratch(std::uint64_t n) {
   scratch_capacity_ = cap;
 }
 
+void EmBackend::wait_radio_tracks() {
+  for (int b = 0; b < 2; ++b)
+    if (track_busy_[b]) {
+      C8MI_HIP(hipEventSynchronize(track_done_[b]));
+      track_busy_[b] = false;
+    }
+}
+
+void EmBackend::attach_radio(radio::RadioAccumulator* radio) {
+  wait_radio_tracks();
+  if (radio && !track_ready_[0])
+    for (int b = 0; b < 2; ++b) {
+      C8MI_HIP(hipEventCreateWithFlags(&track_ready_[b], hipEventDisableTiming));
+      C8MI_HIP(hipEventCreateWithFlags(&track_done_[b], hipEventDisableTiming));
+    }
+  radio_ = radio;
+}
+
 void EmBackend::front(bool photon, c8mi_em_run_result& out) {
   int const s = photon ? 0 : 1, o = 1 - s;
   // staged particles of both species join their queues now, ahead of the
@@ -509,6 +532,30 @@ void EmBackend::front(bool photon, c8mi_em_run_result& out) {
   f.vertex_count = vertex_count_.data();
   f.moves = moves_.data();
   f.moved = moved_.data();
+  int const track_buffer = track_parity_;
+  if (radio_ && !photon) {
+    // tracks [9][n]; the buffer's previous projection must be done first
+    std::uint64_t& capacity = track_capacity_[track_buffer];
+    if (capacity < n) {
+      if (track_busy_[track_buffer]) C8MI_HIP(hipEventSynchronize(track_done_[track_buffer]));
+      track_busy_[track_buffer] = false;
+      capacity = std::max<std::uint64_t>(n, 2 * capacity);
+      tracks_[track_buffer].allocate(C8MI_RADIO_TRACK_FIELDS * capacity);
+    }
+    if (track_busy_[track_buffer]) C8MI_HIP(hipStreamWaitEvent(stream_, track_done_[track_buffer], 0));
+    f.tracks = tracks_[track_buffer].data();
+    f.track_stride = n;
+  }
+  // Projects this front's tracks on the accumulator's stream once the step kernel is done.
+  auto project_tracks = [&] {
+    if (!f.tracks) return;
+    C8MI_HIP(hipEventRecord(track_ready_[track_buffer], stream_));
+    C8MI_HIP(hipStreamWaitEvent(radio_->stream(), track_ready_[track_buffer], 0));
+    radio_->accumulate_tracks(f.tracks, n, n);
+    C8MI_HIP(hipEventRecord(track_done_[track_buffer], radio_->stream()));
+    track_busy_[track_buffer] = true;
+    track_parity_ ^= 1;
+  };
   // the launches of this front, each followed by an event when timing
   int kinds[kFrontKernels + 1];
   int launched = 0;
@@ -527,11 +574,13 @@ void EmBackend::front(bool photon, c8mi_em_run_result& out) {
     mark(kKernLeptonTransport);
     launch_lepton_finish(context_.data(), config_.profile_bins, f, stream_);
     mark(kKernLeptonTransport);
+    project_tracks();
     launch_lepton_vertex(context_.data(), config_.profile_bins, f, stream_);
     mark(kKernLeptonVertex);
   } else {
     launch_lepton_transport(context_.data(), config_.profile_bins, f, stream_);
     mark(kKernLeptonTransport);
+    project_tracks();
     launch_lepton_vertex(context_.data(), config_.profile_bins, f, stream_);
     mark(kKernLeptonVertex);
   }

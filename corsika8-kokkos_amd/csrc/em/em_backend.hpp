@@ -20,6 +20,7 @@
 #include "em/device_tables.hpp"
 #include "em/em_device.hpp"
 #include "em/em_front.hpp"
+#include "radio/radio_accumulator.hpp"
 #include "runtime.hpp"
 
 namespace c8mi::em {
@@ -78,6 +79,14 @@ class EmBackend {
   void serve(c8mi_em_request const* requests, std::uint64_t count, c8mi_em_serve_options const& options,
              c8mi_em_request_result* results, c8mi_em_run_result& totals);
 
+  // Radio (IEmBackend::gpuRadioEnabled / downloadRadioWaveforms with the
+  // backend's RadioConfig): every e+- step of every lepton front becomes a
+  // compact track projected onto `radio` (null detaches). The projection runs on
+  // the accumulator's own stream, overlapping the following fronts; two track
+  // buffers alternate between fronts. The accumulator must outlive the
+  // attachment.
+  void attach_radio(radio::RadioAccumulator* radio);
+
   void enable_timing(bool enable);
   void kernel_time(double ms[kFrontKernels], std::uint64_t launches[kFrontKernels]) const;
 
@@ -107,6 +116,7 @@ class EmBackend {
   void check_slot(std::uint32_t slot) const;
   void update_active_range();
   long long pending_energy(std::uint32_t slot);
+  void wait_radio_tracks();
 
   c8mi_em_config config_;
   std::uint32_t slots_ = 1;
@@ -161,6 +171,15 @@ class EmBackend {
   DeviceBuffer<std::uint32_t> moved_;
   DeviceBuffer<unsigned long long> vertex_count_;
   std::uint64_t scratch_capacity_ = 0;
+  // radio: tracks of the last two lepton fronts; `track_done_` marks the end
+  // of their projection on the accumulator's stream
+  radio::RadioAccumulator* radio_ = nullptr;
+  DeviceBuffer<double> tracks_[2];
+  std::uint64_t track_capacity_[2] = {0, 0};
+  hipEvent_t track_ready_[2] = {nullptr, nullptr};
+  hipEvent_t track_done_[2] = {nullptr, nullptr};
+  bool track_busy_[2] = {false, false};
+  int track_parity_ = 0;
   // timing
   bool timing_ = false;
   std::array<double, kFrontKernels> ms_{};

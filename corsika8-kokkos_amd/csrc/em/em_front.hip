@@ -134,6 +134,8 @@ __device__ __forceinline__ void write_outputs(FrontArgs const& f, std::uint64_t 
 __device__ __forceinline__ phys::Outputs no_outputs(FrontArgs const& f, std::uint64_t i) {
   phys::Outputs o;
   o.staging = f.staging + 2 * i;
+  o.track = f.tracks ? f.tracks + i : nullptr;
+  o.track_stride = f.track_stride;
   o.kind[0] = o.kind[1] = kSlotNone;
   o.next = o.other = o.children = 0;
   o.step = o.interaction = false;
@@ -190,6 +192,7 @@ __global__ __launch_bounds__(kStepBlock, 3) void lepton_transport_kernel(EmConte
     phys::Outputs o = no_outputs(f, i);
     deferred = phys::lepton_transport(c, acc, p, o, vx);
     write_outputs(f, i, o, s);  // a deferred source's word is rewritten by the vertex kernel
+    if (o.track && !o.step) phys::write_no_track(o.track, o.track_stride);
   }
   // compact the vertices of the workgroup, in source order
   std::uint32_t const wave = threadIdx.x / 64, lane = threadIdx.x % 64;
@@ -263,6 +266,7 @@ __global__ __launch_bounds__(kStepBlock, 2) void lepton_finish_kernel(EmContext 
   bool deferred = false;
   std::uint32_t s = 0;
   phys::LeptonVertex vx;
+  bool stepped = false;
   if (i < f.count && f.moved[i]) {
     s = f.in.shower[i];
     phys::Accumulator const acc = accumulator(local, l, s);
@@ -276,7 +280,9 @@ __global__ __launch_bounds__(kStepBlock, 2) void lepton_finish_kernel(EmContext 
     };
     deferred = phys::lepton_finish(c, acc, input, mv, o, vx);
     write_outputs(f, i, o, s);
+    stepped = o.step;
   }
+  if (f.tracks && i < f.count && !stepped) phys::write_no_track(f.tracks + i, f.track_stride);
   std::uint32_t const wave = threadIdx.x / 64, lane = threadIdx.x % 64;
   unsigned long long const mask = __ballot(deferred);
   if (lane == 0) wave_vertices[wave] = static_cast<std::uint32_t>(__popcll(mask));

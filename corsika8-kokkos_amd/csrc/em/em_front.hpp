@@ -57,6 +57,8 @@ struct FrontArgs {
   unsigned long long* vertex_count;   // zero between fronts
   void* moves;                        // split lepton fronts: one LeptonMove per source
   std::uint32_t* moved;               // 1 where the move kernel produced a move
+  double* tracks;                     // lepton fronts with radio: [9][track_stride] compact tracks
+  std::uint64_t track_stride;
 };
 
 // Start of a shower in a slot: its parameters and first secondary history ID.

@@ -882,10 +882,33 @@ __device__ __forceinline__ Particle child_of(Particle const& parent, std::int32_
 // registers through the rest of the step).
 struct Outputs {
   c8mi_particle_state* staging;
+  double* track;               // radio track of the step: [9] rows `track_stride` apart; null: none
+  std::uint64_t track_stride;
   std::uint32_t kind[2];  // SlotKind
   std::uint32_t next, other, children;
   bool step, interaction;
 };
+
+// Compact radio track of one e+- step (the make_radio_track_kinematics input,
+// RadioProjection.inl:145-168): start and end positions, start and end times,
+// weight signed by charge (negative for e-).
+__device__ __forceinline__ void write_track(double* t, std::uint64_t stride, c8mi_particle_state const& start,
+                                            c8mi_particle_state const& end) {
+  t[0] = start.position_m[0];
+  t[stride] = start.position_m[1];
+  t[2 * stride] = start.position_m[2];
+  t[3 * stride] = end.position_m[0];
+  t[4 * stride] = end.position_m[1];
+  t[5 * stride] = end.position_m[2];
+  t[6 * stride] = start.time_s;
+  t[7 * stride] = end.time_s;
+  t[8 * stride] = start.pid == 11 ? -start.weight : start.weight;
+}
+
+// A source without an e+- step record: a zero-length track, which the projection rejects.
+__device__ __forceinline__ void write_no_track(double* t, std::uint64_t stride) {
+  for (int k = 0; k < 9; ++k) t[k * stride] = 0.;
+}
 
 __device__ __forceinline__ void emit(Outputs& o, Particle const& p, std::uint32_t kind) {
   std::uint32_t const k = (o.kind[0] == kSlotNone) ? 0u : 1u;

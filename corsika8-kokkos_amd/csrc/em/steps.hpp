@@ -522,6 +522,7 @@ __device__ __forceinline__ bool lepton_finish(EmContext const& c, Accumulator co
     }
   }
   o.step = true;
+  if (o.track) write_track(o.track, o.track_stride, start, end);  // e+- only reach here
   // accumulate_lepton_profile_step
   double const w = start.weight;
   double const gs = project_grammage(acc, start.position_m);

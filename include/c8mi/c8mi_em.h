@@ -10,6 +10,7 @@
 #ifndef C8MI_EM_H
 #define C8MI_EM_H
 
+#include "c8mi_radio.h"
 #include "c8mi_tables.h"
 #include "c8mi_types.h"
 
@@ -460,6 +461,19 @@ typedef struct c8mi_em_request_result {
 int c8mi_em_serve(c8mi_em* em, const c8mi_em_request* requests, uint64_t count,
                   const c8mi_em_serve_options* options, c8mi_em_request_result* results,
                   c8mi_em_run_result* totals);
+
+/* ---- Radio from the lepton fronts ----
+ * IEmBackend::gpuRadioEnabled / downloadRadioWaveforms with BackendConfig::radio_
+ * (corsika/kokkos/em/execution/IEmBackend.hpp:78,86; BackendConfig.hpp:66) and
+ * the per-front RadioAccumulator::accumulateLeptonTracks call
+ * (corsika/detail/kokkos/em/execution/LeptonCascade.inl:81-82): once attached,
+ * every e+- step of every later lepton front (all pool slots) is projected onto
+ * `radio` as a compact track (start/end position and time, charge-signed
+ * weight). The projection runs asynchronously on the radio handle's stream and
+ * overlaps the following fronts; c8mi_radio_download waits for it. NULL
+ * detaches (after the projections in flight finish). The radio handle must not
+ * be destroyed while attached. */
+int c8mi_em_attach_radio(c8mi_em* em, c8mi_radio* radio);
 
 /* Per-kernel device time (HIP events) since enable: ms[0] photon step,
  * ms[1] lepton transport, ms[2] lepton vertex, ms[3] count, ms[4] scan,

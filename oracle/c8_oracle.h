@@ -149,6 +149,19 @@ int oracle_em_run_shower(const c8mi_em_config* config, const c8mi_native_table_s
                          int64_t* counters, c8mi_particle_state* observations,
                          c8mi_em_fallback* fallbacks, or_em_stats* stats);
 
+/* oracle_em_run_shower that also returns the compact radio track of every e+-
+ * step, row-major [track_count][9] (the fields of the [9][stride] form of
+ * c8mi_types.h), in a buffer the caller releases with oracle_free. */
+int oracle_em_run_shower_tracks(const c8mi_em_config* config, const c8mi_native_table_set* tables,
+                                const c8mi_environment* environment, uint64_t shower_id,
+                                uint64_t first_history_id, const c8mi_particle_state* primaries,
+                                uint64_t primary_count, uint64_t maximum_fronts, int64_t* photons,
+                                int64_t* electrons, int64_t* positrons, int64_t* energy_loss,
+                                int64_t* counters, c8mi_particle_state* observations,
+                                c8mi_em_fallback* fallbacks, or_em_stats* stats, double** tracks,
+                                uint64_t* track_count);
+void oracle_free(void* p);
+
 /* oracle_em_run_shower over shower_count single-primary showers on `threads`
  * host threads (shower k: primaries[k], shower_id first_shower + k, first
  * history 2); per-shower statistics only. */

@@ -1351,7 +1351,36 @@ typedef struct {
   c8mi_em_fallback* fallbacks;
   uint64_t fallback_count;
   or_em_stats stats;
+  /* optional radio tracks of the e+- steps, [track_count][9] */
+  double* tracks;
+  uint64_t track_count, track_capacity;
+  int record_tracks, track_overflow;
 } shower_t;
+
+/* Compact radio track of one e+- step (the make_radio_track_kinematics input,
+ * RadioProjection.inl:145-168): start and end positions and times, weight
+ * signed by charge (negative for e-). */
+static void record_track(shower_t* s, const ps_t* start, const ps_t* end) {
+  if (!s->record_tracks) return;
+  if (s->track_count == s->track_capacity) {
+    uint64_t cap = s->track_capacity ? 2 * s->track_capacity : 4096;
+    double* grown = realloc(s->tracks, cap * 9 * sizeof(double));
+    if (!grown) {
+      s->track_overflow = 1;
+      return;
+    }
+    s->tracks = grown;
+    s->track_capacity = cap;
+  }
+  double* t = s->tracks + 9 * s->track_count++;
+  for (int k = 0; k < 3; ++k) {
+    t[k] = start->position_m[k];
+    t[3 + k] = end->position_m[k];
+  }
+  t[6] = start->time_s;
+  t[7] = end->time_s;
+  t[8] = start->pid == 11 ? -start->weight : start->weight;
+}
 
 static void push(queue_t* q, const ps_t* p) {
   if (q->count == q->capacity) {
@@ -2339,6 +2368,9 @@ static void lepton_source(shower_t* s, const ps_t* p, lout_t* o, uint64_t child_
     if (tr.limit == LL_INTERACTION) tr.in.particle = tr.end;
   }
   s->stats.lepton_steps++;
+  /* the front's step record feeds the radio accumulator
+   * (LeptonCascade.inl:81-82) */
+  record_track(s, &tr.start, &tr.end);
   /* accumulate_lepton_profile_step (ProfileAccumulation.inl:300-375) */
   double w = tr.start.weight;
   double gs = project_grammage(cfg, tr.start.position_m);
@@ -2699,15 +2731,17 @@ static void lepton_front(shower_t* s) {
   if (in.count > s->stats.max_front) s->stats.max_front = in.count;
 }
 
-int oracle_em_run_shower(const c8mi_em_config* config, const c8mi_native_table_set* tables,
-                         const c8mi_environment* environment, uint64_t shower_id,
-                         uint64_t first_history_id, const c8mi_particle_state* primaries,
-                         uint64_t primary_count, uint64_t maximum_fronts, int64_t* photons,
-                         int64_t* electrons, int64_t* positrons, int64_t* energy_loss,
-                         int64_t* counters, c8mi_particle_state* observations,
-                         c8mi_em_fallback* fallbacks, or_em_stats* stats) {
+static int run_shower(const c8mi_em_config* config, const c8mi_native_table_set* tables,
+                      const c8mi_environment* environment, uint64_t shower_id,
+                      uint64_t first_history_id, const c8mi_particle_state* primaries,
+                      uint64_t primary_count, uint64_t maximum_fronts, int64_t* photons,
+                      int64_t* electrons, int64_t* positrons, int64_t* energy_loss,
+                      int64_t* counters, c8mi_particle_state* observations,
+                      c8mi_em_fallback* fallbacks, or_em_stats* stats, double** tracks,
+                      uint64_t* track_count) {
   shower_t s;
   memset(&s, 0, sizeof s);
+  s.record_tracks = tracks != NULL;
   s.x.t = tables;
   s.x.cont.t = tables;
   s.x.cont.em_cut_mev = config->em_transport_cut_mev;
@@ -2753,8 +2787,46 @@ int oracle_em_run_shower(const c8mi_em_config* config, const c8mi_native_table_s
   *stats = s.stats;
   free(s.photons.items);
   free(s.leptons.items);
+  if (tracks) {
+    if (s.track_overflow) {
+      free(s.tracks);
+      return -1;
+    }
+    *tracks = s.tracks;
+    *track_count = s.track_count;
+  }
   return 0;
 }
+
+int oracle_em_run_shower(const c8mi_em_config* config, const c8mi_native_table_set* tables,
+                         const c8mi_environment* environment, uint64_t shower_id,
+                         uint64_t first_history_id, const c8mi_particle_state* primaries,
+                         uint64_t primary_count, uint64_t maximum_fronts, int64_t* photons,
+                         int64_t* electrons, int64_t* positrons, int64_t* energy_loss,
+                         int64_t* counters, c8mi_particle_state* observations,
+                         c8mi_em_fallback* fallbacks, or_em_stats* stats) {
+  return run_shower(config, tables, environment, shower_id, first_history_id, primaries,
+                    primary_count, maximum_fronts, photons, electrons, positrons, energy_loss,
+                    counters, observations, fallbacks, stats, NULL, NULL);
+}
+
+int oracle_em_run_shower_tracks(const c8mi_em_config* config, const c8mi_native_table_set* tables,
+                                const c8mi_environment* environment, uint64_t shower_id,
+                                uint64_t first_history_id, const c8mi_particle_state* primaries,
+                                uint64_t primary_count, uint64_t maximum_fronts, int64_t* photons,
+                                int64_t* electrons, int64_t* positrons, int64_t* energy_loss,
+                                int64_t* counters, c8mi_particle_state* observations,
+                                c8mi_em_fallback* fallbacks, or_em_stats* stats, double** tracks,
+                                uint64_t* track_count) {
+  if (!tracks || !track_count) return -1;
+  *tracks = NULL;
+  *track_count = 0;
+  return run_shower(config, tables, environment, shower_id, first_history_id, primaries,
+                    primary_count, maximum_fronts, photons, electrons, positrons, energy_loss,
+                    counters, observations, fallbacks, stats, tracks, track_count);
+}
+
+void oracle_free(void* p) { free(p); }
 
 /* Independent showers in parallel over host threads (the CPU baseline of the
  * benchmark): shower k runs primaries[k] alone with shower_id first_shower + k,

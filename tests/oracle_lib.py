@@ -40,6 +40,7 @@ def load():
         lib.oracle_radio_nearest_bin.argtypes = [c_double]
         lib.oracle_radio_doppler.restype = c_double
         lib.oracle_radio_doppler.argtypes = [c_double, POINTER(c_double), POINTER(c_double), c_int]
+        lib.oracle_free.argtypes = [c_void_p]
         _LIB = lib
     return _LIB
 
@@ -183,9 +184,10 @@ def native_queries(tables, queries, reference=False):
 # ---- EM transport (em_oracle.c) -----------------------------------------------
 
 def em_run_shower(config, tables, environment, primaries, shower_id=1, first_history_id=2,
-                  maximum_fronts=1 << 40):
+                  maximum_fronts=1 << 40, tracks=False):
     """Runs the oracle's serial front schedule; returns histograms (int64),
-    raw counters, stats and the stored observations / fallbacks."""
+    raw counters, stats and the stored observations / fallbacks; with
+    tracks=True also the radio track of every e+- step as a [9][n] array."""
     from c8mi import PARTICLE_STATE_DTYPE
     from c8mi.em import COUNTERS, FALLBACK_DTYPE, RunResult
 
@@ -200,15 +202,31 @@ def em_run_shower(config, tables, environment, primaries, shower_id=1, first_his
     stats = RunResult()
     env = environment.struct if hasattr(environment, "struct") else environment
     p = lambda a: a.ctypes.data_as(c_void_p)
-    lib.oracle_em_run_shower(byref(cfg), byref(tables.struct), byref(env), c_uint64(shower_id),
-                             c_uint64(first_history_id), p(prim), c_uint64(prim.size),
-                             c_uint64(maximum_fronts), p(hist["photons"]), p(hist["electrons"]),
-                             p(hist["positrons"]), p(hist["energy_loss"]), p(counters), p(obs),
-                             p(fbs), byref(stats))
+    args = [byref(cfg), byref(tables.struct), byref(env), c_uint64(shower_id),
+            c_uint64(first_history_id), p(prim), c_uint64(prim.size),
+            c_uint64(maximum_fronts), p(hist["photons"]), p(hist["electrons"]),
+            p(hist["positrons"]), p(hist["energy_loss"]), p(counters), p(obs),
+            p(fbs), byref(stats)]
+    track_array = None
+    if tracks:
+        buf, n = c_void_p(), c_uint64()
+        if lib.oracle_em_run_shower_tracks(*args, byref(buf), byref(n)) != 0:
+            raise RuntimeError("oracle_em_run_shower_tracks failed")
+        try:
+            rows = np.ctypeslib.as_array(ctypes.cast(buf, ctypes.POINTER(ctypes.c_double)),
+                                         shape=(n.value * 9,)) if n.value else np.zeros(0)
+            track_array = np.ascontiguousarray(rows.reshape(-1, 9).T.copy())
+        finally:
+            lib.oracle_free(buf)
+    else:
+        lib.oracle_em_run_shower(*args)
     st = stats.as_dict()
     n_obs = min(st["observations"], obs.size)
     n_fb = min(st["fallbacks"], fbs.size)
-    return dict(hist=hist, counters=counters, stats=st, observations=obs[:n_obs], fallbacks=fbs[:n_fb])
+    out = dict(hist=hist, counters=counters, stats=st, observations=obs[:n_obs], fallbacks=fbs[:n_fb])
+    if tracks:
+        out["tracks"] = track_array
+    return out
 
 
 def em_run_batch(config, tables, environment, primaries, first_shower=1, threads=1,

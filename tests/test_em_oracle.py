@@ -354,3 +354,28 @@ def test_moliere_shower_closes_energy_and_spreads(tables, atmosphere):
         lateral[moliere] = float(np.mean(np.hypot(obs["position_m"][:, 0], obs["position_m"][:, 1])))
     # multiple scattering widens the shower at the observation level
     assert lateral[True] > 1.5 * lateral[False]
+
+
+@pytest.mark.parametrize("pid,sign", [(11, -1.0), (-11, 1.0)])
+def test_oracle_step_tracks_follow_the_steps(tables, atmosphere, pid, sign):
+    """Radio tracks of the e+- steps: one per step record, in step order. The
+    primary's first step runs from the injection point down to the 100 km layer
+    boundary at the speed of a 1 GeV lepton; every track is traversed at a
+    speed between 0.9 c (the 1 MeV cut) and c; the weight carries the charge."""
+    cfg = em.make_config(airtables, seed=4, atmosphere=atmosphere, moliere=True)
+    prim = em.particle(pid, 1.0, (0.0, 0.0, 112750.0), (0.0, 0.0, -1.0), history_id=1)
+    out = oracle_lib.em_run_shower(cfg, tables, atmosphere, prim, tracks=True)
+    tracks = out["tracks"]
+    assert tracks.shape == (9, out["stats"]["lepton_steps"])
+    first = tracks[:, 0]
+    assert np.array_equal(first[0:3], [0.0, 0.0, 112750.0])
+    assert first[3] == 0.0 and first[4] == 0.0 and abs(first[5] - 100000.0) < 1e-6
+    beta = math.sqrt(1.0 - (0.0005109989461 / 1.0) ** 2)  # transport electron mass
+    assert first[6] == 0.0
+    assert first[7] == pytest.approx(12750.0 / (beta * 299792458.0), rel=1e-9)
+    assert first[8] == sign
+    length = np.linalg.norm(tracks[3:6] - tracks[0:3], axis=0)
+    moving = length > 1e-6
+    speed = length[moving] / (tracks[7, moving] - tracks[6, moving]) / 299792458.0
+    assert speed.min() > 0.9 and speed.max() <= 1.0 + 1e-12
+    assert set(np.unique(tracks[8])) == {-1.0, 1.0}

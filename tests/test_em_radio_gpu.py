@@ -1,0 +1,196 @@
+"""Radio from the EM lepton fronts on the GPU: every e+- step of the device
+shower is projected onto an attached radio accumulator (the reference's
+per-front RadioAccumulator::accumulateLeptonTracks call,
+corsika/detail/kokkos/em/execution/LeptonCascade.inl:81-82).
+
+The device shower takes the oracle's discrete history (every step, vertex and
+secondary), but its positions and times agree with the oracle's only to the
+last bits (the device contracts products into FMAs and has its own exp/log). A
+track moved by 1e-15 s can put one contribution into the neighbouring time bin,
+so whole-shower waveforms are compared by what such shifts leave unchanged: the
+contribution counters exactly, the time-integrated field of every observer and
+component to 1e-9 of its magnitude sum, and the waveform of every observer to
+1e-5 in the L2 norm (measured: below 1e-7; the radio path's stated field
+tolerance is 1e-3). The projection itself is bit-exact for identical tracks
+(the radio tests)."""
+import numpy as np
+import pytest
+
+import c8mi
+import oracle_lib
+from c8mi import airtables, em, workloads
+from radio_cases import window_observers
+
+pytestmark = pytest.mark.gpu
+
+START = (0.0, 0.0, 112750.0)
+DOWN = (0.0, 0.0, -1.0)
+
+
+@pytest.fixture(scope="module")
+def tables():
+    return airtables.build_air_tables()
+
+
+def propagation():
+    refractivity = workloads.exponential_refractivity(-1000.0, 113000.0, 1.0)
+    return c8mi.Propagation(refractivity=refractivity, minimum_height_m=-1000.0,
+                            maximum_height_m=113000.0, step_m=1.0)
+
+
+def ground_antennas():
+    return [(r * np.cos(a), r * np.sin(a), 0.0) for r in (30.0, 120.0) for a in (0.0, 2.1, 4.2)]
+
+
+def oracle_tracks(cfg, tables, atm, primaries, shower_id):
+    want = oracle_lib.em_run_shower(cfg, tables, atm, primaries, shower_id=shower_id,
+                                    first_history_id=2, tracks=True)
+    assert want["tracks"].shape == (9, want["stats"]["lepton_steps"])
+    return want
+
+
+def assert_waveforms_close(got, want, observers):
+    """got / want: [3][total bins] int64 fixed point of the same observers."""
+    got = got.astype(np.float64)
+    want = want.astype(np.float64)
+    begin = 0
+    worst = 0.0
+    for n in observers["number_of_bins"]:
+        g, w = got[:, begin:begin + n], want[:, begin:begin + n]
+        begin += n
+        for k in range(3):
+            magnitude = np.abs(w[k]).sum()
+            assert abs(g[k].sum() - w[k].sum()) <= 1e-9 * magnitude + 8 * n, (k, g[k].sum(), w[k].sum())
+        norm = np.linalg.norm(w)
+        if norm > 0:
+            worst = max(worst, np.linalg.norm(g - w) / norm)
+            assert np.linalg.norm(g - w) <= 1e-5 * norm
+        else:
+            assert not g.any()
+    print(f"max relative L2 difference over observers: {worst:.3e}; "
+          f"differing samples {np.count_nonzero(got != want)} of {np.count_nonzero(want)} nonzero")
+
+
+def assert_same_radio(got, ref, observers):
+    coreas, zhs, counters = got
+    for k in ("valid_tracks", "coreas_contributions", "zhs_contributions", "zhs_subtracks"):
+        assert counters[k] == ref["counters"][k], k
+    assert counters["fixed_point_overflows"] == 0
+    assert_waveforms_close(coreas, ref["coreas_fixed"], observers)
+    assert_waveforms_close(zhs, ref["zhs_fixed"], observers)
+
+
+def device_radio(cfg, tables, atm, observers, prop, showers, exact=True):
+    """Runs `showers` [(shower_id, primaries)] one after another in slot 0 with
+    the radio attached; returns the fixed-point waveforms and run totals."""
+    dev = em.EmBackend(cfg, tables, atm)
+    radio = c8mi.RadioAccumulator(observers, prop, exact=exact)
+    dev.attach_radio(radio)
+    totals = []
+    for shower_id, prim in showers:
+        dev.begin_shower(shower_id, 2)
+        dev.enqueue(prim)
+        totals.append(dev.run())
+    out = radio.download_fixed()
+    dev.attach_radio(None)
+    dev.close()
+    radio.close()
+    return out, totals
+
+
+@pytest.mark.parametrize("pid,energy,seed,moliere", [(22, 1.0, 1, False), (11, 2.0, 2, True),
+                                                     (22, 3.0, 3, True)])
+def test_shower_radio_matches_oracle(tables, pid, energy, seed, moliere):
+    atm = em.Atmosphere()
+    cfg = em.make_config(airtables, seed=seed, atmosphere=atm, moliere=moliere)
+    prim = em.particle(pid, energy, START, DOWN, history_id=1)
+    want = oracle_tracks(cfg, tables, atm, prim, shower_id=1)
+    tracks = want["tracks"]
+    assert tracks.shape[1] > 100
+    observers = window_observers(tracks, ground_antennas(), 1.0e9, 512)
+    prop = propagation()
+    ref = oracle_lib.radio_project(prop, observers, tracks)
+    got, totals = device_radio(cfg, tables, atm, observers, prop, [(1, prim)])
+    assert totals[0]["lepton_steps"] == tracks.shape[1]
+    assert ref["counters"]["valid_tracks"] > 0 and ref["counters"]["coreas_contributions"] > 0
+    assert_same_radio(got, ref, observers)
+    assert np.any(got[0] != 0) and np.any(got[1] != 0)
+
+
+def test_consecutive_showers_accumulate(tables):
+    """Two showers, one after the other, into one accumulator: the sum of both
+    oracle projections (fixed point adds exactly). Many fronts alternate the
+    two track buffers."""
+    atm = em.Atmosphere()
+    cfg = em.make_config(airtables, seed=7, atmosphere=atm, moliere=True)
+    showers = [(11, em.particle(11, 2.0, START, DOWN, history_id=1)),
+               (12, em.particle(-11, 1.5, START, DOWN, history_id=1))]
+    tracks = np.concatenate([oracle_tracks(cfg, tables, atm, p, sid)["tracks"] for sid, p in showers],
+                            axis=1)
+    observers = window_observers(tracks, ground_antennas(), 1.0e9, 512)
+    prop = propagation()
+    ref = oracle_lib.radio_project(prop, observers, tracks)
+    got, totals = device_radio(cfg, tables, atm, observers, prop, showers)
+    assert sum(t["lepton_steps"] for t in totals) == tracks.shape[1]
+    assert sum(t["lepton_fronts"] for t in totals) > 4
+    assert_same_radio(got, ref, observers)
+
+
+def test_pool_slots_share_the_accumulator(tables):
+    """Two showers in two pool slots of the same fronts: the attached
+    accumulator receives both showers' steps."""
+    atm = em.Atmosphere()
+    cfg = em.make_config(airtables, seed=9, atmosphere=atm, slots=2, moliere=True)
+    cases = [(21, em.particle(22, 2.0, START, DOWN, history_id=1)),
+             (22, em.particle(11, 1.0, START, DOWN, history_id=1))]
+    tracks = np.concatenate([oracle_tracks(cfg, tables, atm, p, sid)["tracks"] for sid, p in cases],
+                            axis=1)
+    observers = window_observers(tracks, ground_antennas(), 1.0e9, 512)
+    prop = propagation()
+    ref = oracle_lib.radio_project(prop, observers, tracks)
+    dev = em.EmBackend(cfg, tables, atm)
+    radio = c8mi.RadioAccumulator(observers, prop, exact=True)
+    dev.attach_radio(radio)
+    for slot, (sid, prim) in enumerate(cases):
+        dev.slot_begin(slot, sid, 2)
+        dev.slot_enqueue(slot, prim)
+    totals = dev.run_rounds(1 << 20)
+    got = radio.download_fixed()
+    dev.close()
+    radio.close()
+    assert totals["lepton_steps"] == tracks.shape[1]
+    assert_same_radio(got, ref, observers)
+
+
+def test_fast_projection_within_tolerance(tables):
+    """The default (non-exact) projection in the coupled path meets the same bar."""
+    atm = em.Atmosphere()
+    cfg = em.make_config(airtables, seed=5, atmosphere=atm, moliere=True)
+    prim = em.particle(11, 2.0, START, DOWN, history_id=1)
+    tracks = oracle_tracks(cfg, tables, atm, prim, 1)["tracks"]
+    observers = window_observers(tracks, ground_antennas(), 1.0e9, 512)
+    prop = propagation()
+    ref = oracle_lib.radio_project(prop, observers, tracks)
+    got, _ = device_radio(cfg, tables, atm, observers, prop, [(1, prim)], exact=False)
+    assert_same_radio(got, ref, observers)
+
+
+def test_detached_backend_projects_nothing(tables):
+    atm = em.Atmosphere()
+    cfg = em.make_config(airtables, seed=3, atmosphere=atm)
+    prim = em.particle(11, 1.0, START, DOWN, history_id=1)
+    tracks = oracle_tracks(cfg, tables, atm, prim, 1)["tracks"]
+    observers = window_observers(tracks, ground_antennas(), 1.0e9, 256)
+    prop = propagation()
+    dev = em.EmBackend(cfg, tables, atm)
+    radio = c8mi.RadioAccumulator(observers, prop, exact=True)
+    dev.attach_radio(radio)
+    dev.attach_radio(None)
+    dev.begin_shower(1, 2)
+    dev.enqueue(prim)
+    dev.run()
+    coreas, zhs, counters = radio.download_fixed()
+    dev.close()
+    radio.close()
+    assert counters["valid_tracks"] == 0
+    assert not coreas.any() and not zhs.any()
