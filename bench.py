@@ -62,6 +62,10 @@ def parse():
     p.add_argument("--em-no-moliere", action="store_true",
                    help="EM without Moliere multiple scattering (on by default, as in the reference's production)")
     p.add_argument("--em-only", action="store_true", help="run only the EM section")
+    p.add_argument("--em-radio-showers", type=int, default=2,
+                   help="timed showers of the EM + radio section (0 skips it)")
+    p.add_argument("--em-radio-energy", type=float, default=1.0e4, help="primary electron energy (GeV)")
+    p.add_argument("--em-radio-only", action="store_true", help="run only the EM + radio section")
     return p.parse_args()
 
 
@@ -294,6 +298,136 @@ def em_section(showers, plain_showers, slots, cpu_seconds, no_cpu_baseline, ener
     return out
 
 
+def em_radio_cpu_baseline(tables, atmosphere, cfg, observers, propagation, target_seconds):
+    """Oracle EM showers with their step tracks (one 100 GeV electron shower per
+    host thread) followed by the FP64 radio oracle projecting all their tracks
+    on the same antennas (OpenMP over tracks), on all host cores."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from concurrent.futures import ThreadPoolExecutor
+
+    import oracle_lib
+    from c8mi import em
+
+    threads = max(1, min(int(os.environ.get("OMP_NUM_THREADS", "0")) or 16, os.cpu_count() or 1))
+    prim = em.particle(11, 100.0, (0.0, 0.0, 112750.0), (0.0, 0.0, -1.0), history_id=1)
+    count = threads
+    while True:
+        t0 = time.perf_counter()
+        with ThreadPoolExecutor(threads) as pool:  # ctypes calls release the GIL
+            runs = list(pool.map(lambda k: oracle_lib.em_run_shower(cfg, tables, atmosphere, prim,
+                                                                    shower_id=500 + k, tracks=True),
+                                 range(count)))
+        t_em = time.perf_counter() - t0
+        tracks = np.concatenate([r["tracks"] for r in runs], axis=1)
+        t1 = time.perf_counter()
+        oracle_lib.radio_project(propagation, observers, tracks, floating=False, threads=threads)
+        t_radio = time.perf_counter() - t1
+        if t_em + t_radio >= 0.5 * target_seconds or count >= 8 * threads:
+            break
+        count *= 2
+    steps = sum(r["stats"]["photon_steps"] + r["stats"]["lepton_steps"] for r in runs)
+    wall = t_em + t_radio
+    return {"value": steps / wall, "unit": "particle-steps/s", "cores": threads, "kind": "port",
+            "updates_per_s": tracks.shape[1] * observers.size / wall,
+            "sample": f"{count} x 100 GeV vertical electron showers with radio tracks, one per thread "
+                      f"(oracle, {t_em:.1f} s), then their {tracks.shape[1]} tracks x {observers.size} "
+                      f"antennas by the FP64 radio oracle, OpenMP over tracks ({t_radio:.1f} s); "
+                      f"{threads} threads"}
+
+
+def em_radio_section(showers, energy, cpu_seconds, no_cpu_baseline, moliere=True):
+    """EM transport with radio: a vertical electron of `energy` GeV at 112.75 km
+    in USStdBK air (1 MeV cut, no thinning, Molière on), the whole cascade on
+    the device, every e+- step projected by CoREAS+ZHS onto the 160-antenna
+    star (8 arms x 20 radii, 25-500 m) at 1024 bins @ 0.5 ns, n(h) = 1 +
+    3.25e-4 exp(-h / 8 km). Showers run one after another, each with its own
+    radio reset and fixed-point download; the projection of a front overlaps
+    the next fronts on its own stream."""
+    import c8mi
+    from c8mi import airtables, em, workloads
+
+    tables = airtables.build_air_tables()
+    atm = em.Atmosphere()
+    cfg = em.make_config(airtables, atmosphere=atm, queue_capacity=1 << 23, observation_capacity=1 << 16,
+                         fallback_capacity=1 << 10, moliere=moliere)
+    start, down = (0.0, 0.0, 112750.0), (0.0, 0.0, -1.0)
+    observers = workloads.star_observers(arms=8, radii=20, sample_rate_hz=2.0e9, number_of_bins=1024)
+    workloads.plane_front_windows(observers, start, down)
+    refractivity = workloads.exponential_refractivity(-1000.0, 113000.0, 1.0)
+    prop = c8mi.Propagation(refractivity=refractivity, minimum_height_m=-1000.0, maximum_height_m=113000.0,
+                            step_m=1.0)
+    dev = em.EmBackend(cfg, tables, atm)
+
+    def run(radio, shower_id, e):
+        radio.reset()
+        dev.begin_shower(shower_id, 2)
+        dev.enqueue(em.particle(11, e, start, down, history_id=1))
+        t0 = time.perf_counter()
+        totals = dev.run()
+        fixed = radio.download_fixed()  # waits for the last projections
+        return time.perf_counter() - t0, totals, fixed
+
+    radio = c8mi.RadioAccumulator(observers, prop)
+    dev.attach_radio(radio)
+    run(radio, 999, 100.0)  # warm-up
+    dev.enable_kernel_timing(True)
+    radio.enable_kernel_timing(True)
+    results = [run(radio, 1 + k, energy) for k in range(showers)]
+    em_timing = dev.kernel_time()
+    radio_timing = radio.kernel_time()
+    # field error of this (fast FP64) projection: the same first shower again
+    # with the exact projection, whose samples equal the FP64 oracle's
+    exact = c8mi.RadioAccumulator(observers, prop, exact=True)
+    dev.attach_radio(exact)
+    _, _, exact_fixed = run(exact, 1, energy)
+    dev.attach_radio(None)
+    dev.close()
+    radio.close()
+    exact.close()
+    errors = []
+    for got, want in zip(results[0][2][:2], exact_fixed[:2]):
+        g, w = got.astype(np.float64), want.astype(np.float64)
+        errors.append(float(np.linalg.norm(g - w) / max(np.linalg.norm(w), 1.0)))
+
+    elapsed = sum(r[0] for r in results)
+    steps = sum(r[1]["photon_steps"] + r[1]["lepton_steps"] for r in results)
+    tracks = sum(r[1]["lepton_steps"] for r in results)
+    updates = tracks * observers.size
+    em_ms = sum(v[0] for v in em_timing.values())
+    radio_ms = sum(v[0] for v in radio_timing.values())
+    latency = [r[0] for r in results]
+    out = {
+        "metric": "EM particle-steps/s with radio; radio track·antenna updates/s",
+        "value": steps / elapsed,
+        "unit": "particle-steps/s",
+        "updates_per_s": updates / elapsed,
+        "showers": showers,
+        "particle_steps": steps,
+        "radio_tracks": tracks,
+        "valid_tracks": int(results[0][2][2]["valid_tracks"]),
+        "latency_ms": {"p50": float(np.percentile(latency, 50) * 1e3),
+                       "p99": float(np.percentile(latency, 99) * 1e3)},
+        "dtype": "f64",
+        "precision": "FP64 transport and ledger; FP64 projection with hardware reciprocal/rsqrt + Newton",
+        "field_error_vs_fp64": {"coreas": errors[0], "zhs": errors[1]},
+        "config": {"workload": f"EM + radio: {energy:g} GeV vertical electron at 112.75 km, USStdBK, "
+                               f"1 MeV cut, Moliere {'on' if moliere else 'off'}, CoREAS+ZHS at "
+                               f"{observers.size} antennas, 1024 bins @ 0.5 ns, whole cascade on the device",
+                   "antennas": int(observers.size), "bins": 1024, "sample_rate_hz": 2.0e9},
+        "kernel_ms": {**{k: v[0] for k, v in em_timing.items()},
+                      **{"radio_" + k: v[0] for k, v in radio_timing.items() if v[1]}},
+        # device time of EM and radio kernels over wall time: above 1 when the
+        # projection overlaps the fronts
+        "device_busy_fraction": (em_ms + radio_ms) * 1e-3 / elapsed,
+    }
+    if not no_cpu_baseline:
+        base = em_radio_cpu_baseline(tables, atm, cfg, observers, prop, cpu_seconds)
+        out["cpu_baseline"] = base
+        out["speedup_vs_cpu_baseline"] = out["value"] / base["value"]
+        out["updates_speedup_vs_cpu_baseline"] = out["updates_per_s"] / base["updates_per_s"]
+    return out
+
+
 def measured_traffic(kernel):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 PMC summary
     (FETCH_SIZE + WRITE_SIZE, copy-calibrated), or None."""
@@ -323,6 +457,14 @@ def main():
     if c8mi.device_count() < 1:
         print("bench.py: no HIP device visible", file=sys.stderr)
         return 1
+    if args.em_radio_only:
+        out = em_radio_section(max(args.em_radio_showers, 1), args.em_radio_energy, args.cpu_seconds,
+                               args.no_cpu_baseline, moliere=not args.em_no_moliere)
+        out.update({"n_gpus": 1, "steps": out["showers"], "higher_is_better": True, "scaling": "weak",
+                    "vs_baseline": None, "data": "synthetic",
+                    "ms_per_step": out["latency_ms"]["p50"]})
+        print(json.dumps(out), flush=True)
+        return 0
     if args.em_only:
         em_result = em_section(max(args.em_showers, 1), args.em_plain_showers, args.em_slots, args.cpu_seconds,
                                args.no_cpu_baseline, warmup_showers=args.em_warmup_showers,
@@ -427,6 +569,9 @@ def main():
         result["em"] = em_section(args.em_showers, args.em_plain_showers, args.em_slots, args.cpu_seconds,
                                   args.no_cpu_baseline, warmup_showers=args.em_warmup_showers,
                                moliere=not args.em_no_moliere)
+    if args.em_radio_showers > 0:
+        result["em_radio"] = em_radio_section(args.em_radio_showers, args.em_radio_energy, args.cpu_seconds,
+                                              args.no_cpu_baseline, moliere=not args.em_no_moliere)
     print(json.dumps(result), flush=True)
     return 0
 

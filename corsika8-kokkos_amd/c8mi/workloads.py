@@ -79,6 +79,17 @@ def star_observers(arms: int = 8, radii: int = 20, radius_step_m: float = 25.0,
     return obs
 
 
+def plane_front_windows(observers: np.ndarray, injection_m, direction, pretrigger_s: float = 10.0e-9,
+                        start_time_s: float = 0.0) -> None:
+    """Sets each observer's window to open `pretrigger_s` before a plane front,
+    leaving `injection_m` at `start_time_s` along the unit `direction` at c,
+    passes it (the earliest arrival of an air shower's signal, up to the
+    refractive delay)."""
+    pos = np.stack([observers["position_m"][:, k] for k in range(3)], axis=1)
+    along = (pos - np.asarray(injection_m, dtype=np.float64)) @ np.asarray(direction, dtype=np.float64)
+    observers["start_time_s"] = start_time_s + along / SPEED_OF_LIGHT - pretrigger_s
+
+
 def shower_axis(zenith_deg: float, azimuth_deg: float = 0.0):
     th, ph = math.radians(zenith_deg), math.radians(azimuth_deg)
     up = np.array([math.sin(th) * math.cos(ph), math.sin(th) * math.sin(ph), math.cos(th)])
