@@ -61,6 +61,8 @@ def parse():
     p.add_argument("--em-warmup-showers", type=int, default=64, help="untimed showers before the EM pool run")
     p.add_argument("--em-no-moliere", action="store_true",
                    help="EM without Moliere multiple scattering (on by default, as in the reference's production)")
+    p.add_argument("--em-analytic-moliere", action="store_true",
+                   help="Moliere with the analytical correction functions instead of the interpolation table")
     p.add_argument("--em-only", action="store_true", help="run only the EM section")
     p.add_argument("--em-radio-showers", type=int, default=2,
                    help="timed showers of the EM + radio section (0 skips it)")
@@ -138,7 +140,7 @@ def field_error(tracks, observers, propagation, mode, limit, exact):
 EM_BYTES_PER_STEP = 240  # 120 B SoA state (116 B particle + 4 B pool slot) read + written per step
 
 
-def em_cpu_baseline(tables, atmosphere, energy, target_seconds, moliere=True):
+def em_cpu_baseline(tables, atmosphere, energy, target_seconds, moliere=True, moliere_table=True):
     """Oracle EM showers (the serial C restatement, one shower per host thread,
     OpenMP over showers on all host cores) for about `target_seconds`."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
@@ -147,7 +149,8 @@ def em_cpu_baseline(tables, atmosphere, energy, target_seconds, moliere=True):
 
     threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count() or 1
     threads = max(1, min(threads, os.cpu_count() or 1))
-    cfg = em.make_config(airtables, atmosphere=atmosphere, moliere=moliere)
+    cfg = em.make_config(airtables, atmosphere=atmosphere, moliere=moliere,
+                         moliere_table=True if moliere_table else None)
     prim = em.particle(22, energy, (0.0, 0.0, 112750.0), (0.0, 0.0, -1.0), history_id=1)
     count = threads
     first = 1000
@@ -189,7 +192,7 @@ def em_showers(energy, count, first_shower, atm=None, pid=22):
 
 
 def em_section(showers, plain_showers, slots, cpu_seconds, no_cpu_baseline, energy=100.0, warmup_showers=64,
-               moliere=True):
+               moliere=True, moliere_table=True):
     """EM particle-steps/s on the plain 100 GeV vertical photon shower in the
     USStdBK atmosphere (1 MeV cut, no thinning, FP64), the whole cascade on the
     device from the primary on. Two forms: the pool (`showers` showers served
@@ -201,8 +204,10 @@ def em_section(showers, plain_showers, slots, cpu_seconds, no_cpu_baseline, ener
     tables = airtables.build_air_tables()
     atm = em.Atmosphere()
     capacity = 1 << 23
+    table = True if moliere_table else None
     cfg = em.make_config(airtables, atmosphere=atm, slots=slots, queue_capacity=capacity,
-                         observation_capacity=1 << 12, fallback_capacity=1 << 8, moliere=moliere)
+                         observation_capacity=1 << 12, fallback_capacity=1 << 8, moliere=moliere,
+                         moliere_table=table)
     dev = em.EmBackend(cfg, tables, atm)
     budget = 0.4 * capacity
     zero = {"photon_steps": 0, "lepton_steps": 0, "lepton_interactions": 0}
@@ -228,7 +233,8 @@ def em_section(showers, plain_showers, slots, cpu_seconds, no_cpu_baseline, ener
     one_at_a_time = None
     if plain_showers:
         plain_cfg = em.make_config(airtables, atmosphere=atm, slots=1, queue_capacity=1 << 20,
-                                   observation_capacity=1 << 14, fallback_capacity=1 << 10, moliere=moliere)
+                                   observation_capacity=1 << 14, fallback_capacity=1 << 10, moliere=moliere,
+                                   moliere_table=table)
         plain = em.EmBackend(plain_cfg, tables, atm)
         _, plain_warm_totals = plain.serve(em_showers(energy, 2, 1), 1e18, realtime=False)  # warm-up
         t1 = time.perf_counter()
@@ -272,7 +278,7 @@ def em_section(showers, plain_showers, slots, cpu_seconds, no_cpu_baseline, ener
         "dtype": "f64",
         "config": {"workload": f"EM plain: {energy:g} GeV vertical photon at 112.75 km, USStdBK 5-layer "
                                "spherical air, observation level 0 m, 1 MeV cut, no thinning, no radio, "
-                               f"Moliere scattering {'on' if moliere else 'off'}, "
+                               f"Moliere scattering {('on (interpolation table)' if moliere_table else 'on (analytical functions)') if moliere else 'off'}, "
                                "closed-form v7 air tables, whole cascade on the device; "
                                f"{showers} showers through {slots} pool slots"},
         "roofline": {
@@ -292,7 +298,7 @@ def em_section(showers, plain_showers, slots, cpu_seconds, no_cpu_baseline, ener
         "showers_with_moliere_fallbacks": len(moliere_failed),
     }
     if not no_cpu_baseline:
-        base = em_cpu_baseline(tables, atm, energy, cpu_seconds, moliere=moliere)
+        base = em_cpu_baseline(tables, atm, energy, cpu_seconds, moliere=moliere, moliere_table=moliere_table)
         out["cpu_baseline"] = base
         out["speedup_vs_cpu_baseline"] = out["value"] / base["value"]
     return out
@@ -335,7 +341,7 @@ def em_radio_cpu_baseline(tables, atmosphere, cfg, observers, propagation, targe
                       f"{threads} threads"}
 
 
-def em_radio_section(showers, energy, cpu_seconds, no_cpu_baseline, moliere=True):
+def em_radio_section(showers, energy, cpu_seconds, no_cpu_baseline, moliere=True, moliere_table=True):
     """EM transport with radio: a vertical electron of `energy` GeV at 112.75 km
     in USStdBK air (1 MeV cut, no thinning, Molière on), the whole cascade on
     the device, every e+- step projected by CoREAS+ZHS onto the 160-antenna
@@ -349,7 +355,8 @@ def em_radio_section(showers, energy, cpu_seconds, no_cpu_baseline, moliere=True
     tables = airtables.build_air_tables()
     atm = em.Atmosphere()
     cfg = em.make_config(airtables, atmosphere=atm, queue_capacity=1 << 23, observation_capacity=1 << 16,
-                         fallback_capacity=1 << 10, moliere=moliere)
+                         fallback_capacity=1 << 10, moliere=moliere,
+                         moliere_table=True if moliere_table else None)
     start, down = (0.0, 0.0, 112750.0), (0.0, 0.0, -1.0)
     observers = workloads.star_observers(arms=8, radii=20, sample_rate_hz=2.0e9, number_of_bins=1024)
     workloads.plane_front_windows(observers, start, down)
@@ -411,7 +418,7 @@ def em_radio_section(showers, energy, cpu_seconds, no_cpu_baseline, moliere=True
         "precision": "FP64 transport and ledger; FP64 projection with hardware reciprocal/rsqrt + Newton",
         "field_error_vs_fp64": {"coreas": errors[0], "zhs": errors[1]},
         "config": {"workload": f"EM + radio: {energy:g} GeV vertical electron at 112.75 km, USStdBK, "
-                               f"1 MeV cut, Moliere {'on' if moliere else 'off'}, CoREAS+ZHS at "
+                               f"1 MeV cut, Moliere {('on (table)' if moliere_table else 'on (analytical)') if moliere else 'off'}, CoREAS+ZHS at "
                                f"{observers.size} antennas, 1024 bins @ 0.5 ns, whole cascade on the device",
                    "antennas": int(observers.size), "bins": 1024, "sample_rate_hz": 2.0e9},
         "kernel_ms": {**{k: v[0] for k, v in em_timing.items()},
@@ -459,7 +466,8 @@ def main():
         return 1
     if args.em_radio_only:
         out = em_radio_section(max(args.em_radio_showers, 1), args.em_radio_energy, args.cpu_seconds,
-                               args.no_cpu_baseline, moliere=not args.em_no_moliere)
+                               args.no_cpu_baseline, moliere=not args.em_no_moliere,
+                               moliere_table=not args.em_analytic_moliere)
         out.update({"n_gpus": 1, "steps": out["showers"], "higher_is_better": True, "scaling": "weak",
                     "vs_baseline": None, "data": "synthetic",
                     "ms_per_step": out["latency_ms"]["p50"]})
@@ -468,7 +476,8 @@ def main():
     if args.em_only:
         em_result = em_section(max(args.em_showers, 1), args.em_plain_showers, args.em_slots, args.cpu_seconds,
                                args.no_cpu_baseline, warmup_showers=args.em_warmup_showers,
-                               moliere=not args.em_no_moliere)
+                               moliere=not args.em_no_moliere,
+                               moliere_table=not args.em_analytic_moliere)
         em_result.update({"n_gpus": 1, "steps": em_result["showers"], "higher_is_better": True,
                           "scaling": "weak", "vs_baseline": None, "data": "synthetic",
                           "ms_per_step": em_result["ms_per_shower"]})
@@ -568,10 +577,12 @@ def main():
     if args.em_showers > 0:
         result["em"] = em_section(args.em_showers, args.em_plain_showers, args.em_slots, args.cpu_seconds,
                                   args.no_cpu_baseline, warmup_showers=args.em_warmup_showers,
-                               moliere=not args.em_no_moliere)
+                               moliere=not args.em_no_moliere,
+                               moliere_table=not args.em_analytic_moliere)
     if args.em_radio_showers > 0:
         result["em_radio"] = em_radio_section(args.em_radio_showers, args.em_radio_energy, args.cpu_seconds,
-                                              args.no_cpu_baseline, moliere=not args.em_no_moliere)
+                                              args.no_cpu_baseline, moliere=not args.em_no_moliere,
+                               moliere_table=not args.em_analytic_moliere)
     print(json.dumps(result), flush=True)
     return 0
 

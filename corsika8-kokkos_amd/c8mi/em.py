@@ -117,13 +117,63 @@ class MoliereSnapshot(ctypes.Structure):
         return m
 
 
+class MoliereCubic(ctypes.Structure):
+    _fields_ = [("c", c_double * 4)]
+
+
+MOLIERE_INTERVALS = 99
+MOLIERE_GUESS_SHAPE = (64, 8, 128)
+
+
+class MoliereInterpolation(ctypes.Structure):
+    """c8mi_moliere_interpolation (the reference's MoliereInterpolationTable)."""
+    _fields_ = [("polynomials", MoliereCubic * (4 * MOLIERE_INTERVALS)),
+                ("initial_guess_delta", c_double * (64 * 8 * 128))]
+
+    @classmethod
+    def build(cls, snapshot):
+        """The table of a MoliereSnapshot (c8mi_em_moliere_interpolation_build)."""
+        t = cls()
+        _check(library().c8mi_em_moliere_interpolation_build(byref(snapshot), byref(t)))
+        return t
+
+    @classmethod
+    def read(cls, path, snapshot):
+        t = cls()
+        _check(library().c8mi_em_moliere_interpolation_read(str(path).encode(), byref(snapshot), byref(t)))
+        return t
+
+    def write(self, path, snapshot):
+        _check(library().c8mi_em_moliere_interpolation_write(str(path).encode(), byref(snapshot), byref(self)))
+
+    def polynomial_array(self):
+        """[4][99][4] coefficients: f1, f2, integral f1, integral f2."""
+        return np.ctypeslib.as_array(self.polynomials).view(np.float64).reshape(4, MOLIERE_INTERVALS, 4).copy()
+
+    def guess_array(self):
+        return np.ctypeslib.as_array(self.initial_guess_delta).reshape(MOLIERE_GUESS_SHAPE).copy()
+
+
+_MOLIERE_TABLES = {}
+
+
+def air_moliere_interpolation(tables_module):
+    """The interpolation table of the air electron snapshot, built once."""
+    key = tables_module.__name__
+    if key not in _MOLIERE_TABLES:
+        snap = MoliereSnapshot.from_dict(tables_module.moliere_snapshot())
+        _MOLIERE_TABLES[key] = MoliereInterpolation.build(snap)
+    return _MOLIERE_TABLES[key]
+
+
 class EmConfig(ctypes.Structure):
     _fields_ = [("random_seed", c_uint64), ("em_transport_cut_mev", c_double),
                 ("muon_transport_cut_mev", c_double), ("queue_capacity", c_uint64),
                 ("observation_capacity", c_uint64), ("fallback_capacity", c_uint64),
                 ("photon_pair_lpm", PairLpm), ("brems_lpm", BremsLpm), ("thinning", Thinning),
                 ("electron_moliere", MoliereSnapshot), ("moliere_enabled", c_uint32),
-                ("moliere_reserved_", c_uint32),
+                ("moliere_fast_iterations", c_uint32),
+                ("electron_moliere_interpolation", POINTER(MoliereInterpolation)),
                 ("profile_bins", c_uint32), ("profile_crossing_mode", c_uint32),
                 ("profile_bin_width_g_per_cm2", c_double),
                 ("profile_energy_loss_threshold_g_per_cm2", c_double),
@@ -337,15 +387,29 @@ def particle(pid, energy_gev, position, direction, history_id=1, weight=1.0, nod
 def make_config(tables_module, seed=1, cut_mev=1.0, start=(0.0, 0.0, 112750.0),
                 direction=(0.0, 0.0, -1.0), atmosphere=None, bins=128, bin_width=10.0,
                 queue_capacity=1 << 22, observation_capacity=1 << 20, fallback_capacity=1 << 16,
-                thinning=None, crossing_mode=0, slots=1, moliere=False):
+                thinning=None, crossing_mode=0, slots=1, moliere=False, moliere_table=None,
+                moliere_fast_iterations=0):
     """EmConfig + the arrays it points to (kept alive on the returned object).
     moliere: apply Molière multiple scattering to e+- steps (the reference's
-    production setting); off, e+- directions change only at vertices."""
+    production setting); off, e+- directions change only at vertices.
+    moliere_table: None (analytical correction functions), True (the built
+    interpolation table of the snapshot, as the reference backend uses) or a
+    MoliereInterpolation. moliere_fast_iterations: Newton iterations per angle
+    in the step kernel before a sample moves to its own wavefront (0: 12;
+    results do not depend on it)."""
     cfg = EmConfig()
     cfg.shower_slots = slots
+    cfg.moliere_fast_iterations = moliere_fast_iterations
+    table_ref = None
     if moliere:
         cfg.electron_moliere = MoliereSnapshot.from_dict(tables_module.moliere_snapshot())
         cfg.moliere_enabled = 1
+        if moliere_table is True:
+            table_ref = air_moliere_interpolation(tables_module)
+        elif moliere_table is not None:
+            table_ref = moliere_table
+        if table_ref is not None:
+            cfg.electron_moliere_interpolation = ctypes.pointer(table_ref)
     cfg.random_seed = seed
     cfg.em_transport_cut_mev = cut_mev
     cfg.muon_transport_cut_mev = cut_mev
@@ -395,7 +459,7 @@ def make_config(tables_module, seed=1, cut_mev=1.0, start=(0.0, 0.0, 112750.0),
         pass
 
     h = Holder()
-    h.struct, h.axis_table, h.atmosphere = cfg, table, atmosphere
+    h.struct, h.axis_table, h.atmosphere, h.moliere_table = cfg, table, atmosphere, table_ref
     return h
 
 

@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <stdexcept>
+#include <string>
 
 #include "c8mi/c8mi_em.h"
 #include "capi_guard.hpp"
@@ -10,6 +11,7 @@
 #include "em/device_tables.hpp"
 #include "em/em_backend.hpp"
 #include "em/em_launch.hpp"
+#include "em/moliere_table.hpp"
 #include "runtime.hpp"
 
 struct c8mi_em_tables {
@@ -266,6 +268,32 @@ int c8mi_em_serve(c8mi_em* em, c8mi_em_request const* requests, uint64_t count,
     require_handle(em);
     if (!options || !totals) throw std::invalid_argument("null argument");
     em->backend.serve(requests, count, *options, results, *totals);
+  });
+}
+
+int c8mi_em_moliere_interpolation_build(c8mi_moliere_snapshot const* snapshot,
+                                        c8mi_moliere_interpolation* out) {
+  return guarded([&] {
+    if (!snapshot || !out) throw std::invalid_argument("null argument");
+    c8mi::em::build_moliere_table(*snapshot, *out);
+  });
+}
+
+int c8mi_em_moliere_interpolation_read(char const* path, c8mi_moliere_snapshot const* snapshot,
+                                       c8mi_moliere_interpolation* out) {
+  return guarded([&] {
+    if (!path || !snapshot || !out) throw std::invalid_argument("null argument");
+    if (!c8mi::em::read_moliere_table(path, *snapshot, *out))
+      throw std::invalid_argument(std::string("no valid Moliere cache for this snapshot in ") + path);
+  });
+}
+
+int c8mi_em_moliere_interpolation_write(char const* path, c8mi_moliere_snapshot const* snapshot,
+                                        c8mi_moliere_interpolation const* table) {
+  return guarded([&] {
+    if (!path || !snapshot || !table) throw std::invalid_argument("null argument");
+    if (!c8mi::em::valid_moliere_table(*table)) throw std::invalid_argument("Moliere table with non-finite entries");
+    c8mi::em::write_moliere_table(path, *snapshot, *table);
   });
 }
 

@@ -9,6 +9,7 @@
 #include <string>
 
 #include "em/em_front.hpp"
+#include "em/moliere_table.hpp"
 
 namespace c8mi::em {
 
@@ -65,30 +66,6 @@ void validate_environment(c8mi_environment const& e) {
   }
 }
 
-// valid_moliere_snapshot (MoliereSnapshot.inl:26-66): finite, positive constants
-// and components, finite coefficients.
-bool valid_moliere_snapshot(c8mi_moliere_snapshot const& m) {
-  auto pos = [](double v) { return std::isfinite(v) && v > 0.; };
-  if (m.component_count == 0 || m.component_count > C8MI_MOLIERE_MAX_COMPONENTS ||
-      m.maximum_weight_index >= m.component_count || !pos(m.particle_mass_mev) || !pos(m.electron_mass_mev) ||
-      !pos(m.fine_structure_constant) || !pos(m.avogadro_per_mol) || !pos(m.hbar_mev_s) ||
-      !pos(m.speed_of_light_cm_per_s) || !pos(m.z_squared_over_a_average) || !pos(m.inverse_weight_zz_sum) ||
-      !std::isfinite(m.euler_mascheroni))
-    return false;
-  for (std::uint32_t k = 0; k < m.component_count; ++k) {
-    c8mi_moliere_component const& c = m.components[k];
-    if (!pos(c.nuclear_charge) || !pos(c.weight_zz) || !pos(c.chi0_squared_mev2) || !pos(c.coulomb_correction))
-      return false;
-  }
-  for (int k = 0; k < 70; ++k)
-    if (!std::isfinite(m.c1[k]) || !std::isfinite(m.c2[k])) return false;
-  for (int k = 0; k < 13; ++k)
-    if (!std::isfinite(m.c2_large[k]) || !std::isfinite(m.s2_large[k])) return false;
-  for (int k = 0; k < 15; ++k)
-    if (!std::isfinite(m.c1_integral_large[k])) return false;
-  return true;
-}
-
 void validate_config(c8mi_em_config const& c) {
   require(c.queue_capacity > 0, "queue capacity");
   require(std::isfinite(c.em_transport_cut_mev) && c.em_transport_cut_mev > 0., "EM transport cut");
@@ -102,9 +79,12 @@ void validate_config(c8mi_em_config const& c) {
   require(c.profile_crossing_mode <= C8MI_PROFILE_CROSSING_FORWARD, "profile crossing mode");
   require(c.shower_slots <= kMaxShowerSlots, "shower slots above the supported maximum (1024)");
   require(c.moliere_enabled <= 1, "moliere_enabled must be 0 or 1");
+  require(c.moliere_fast_iterations <= 100, "moliere_fast_iterations above the Newton limit (100)");
   if (c.moliere_enabled) {
     require(valid_moliere_snapshot(c.electron_moliere), "invalid Moliere snapshot");
     require(c.electron_moliere.component_count <= 4, "Moliere snapshots of more than 4 components are not supported");
+    require(c.electron_moliere_interpolation == nullptr || valid_moliere_table(*c.electron_moliere_interpolation),
+            "Moliere interpolation table with non-finite entries");
   }
 }
 
@@ -224,6 +204,7 @@ EmBackend::EmBackend(c8mi_em_config const& config, c8mi_native_table_set const& 
   h.thinning = config.thinning;
   h.moliere = config.electron_moliere;
   h.moliere_enabled = config.moliere_enabled;
+  h.moliere_fast_iterations = config.moliere_fast_iterations ? config.moliere_fast_iterations : 12;
   for (int i = 0; i < 70; ++i) {
     h.moliere_i1[i] = config.electron_moliere.c1[i] / (2. * i + 1.);
     h.moliere_i2[i] = config.electron_moliere.c2[i] / (2. * i + 1.);
@@ -231,6 +212,18 @@ EmBackend::EmBackend(c8mi_em_config const& config, c8mi_native_table_set const& 
   for (int i = 0; i < 13; ++i) {
     h.moliere_i2_large[i] = i ? -0.5 * config.electron_moliere.c2_large[i] / i : 0.;
     h.moliere_half_over[i] = i ? 0.5 / i : 0.;
+  }
+  if (config.moliere_enabled && config.electron_moliere_interpolation) {
+    c8mi_moliere_interpolation const& t = *config.electron_moliere_interpolation;
+    constexpr std::size_t cubics = sizeof(t.polynomials) / sizeof(t.polynomials[0]);
+    constexpr std::size_t guesses = sizeof(t.initial_guess_delta) / sizeof(double);
+    moliere_polynomials_.allocate(cubics);
+    moliere_guess_.allocate(guesses);
+    C8MI_HIP(hipMemcpy(moliere_polynomials_.data(), t.polynomials, sizeof(t.polynomials), hipMemcpyHostToDevice));
+    C8MI_HIP(hipMemcpy(moliere_guess_.data(), t.initial_guess_delta, sizeof(t.initial_guess_delta),
+                       hipMemcpyHostToDevice));
+    h.moliere_polynomials = moliere_polynomials_.data();
+    h.moliere_guess = moliere_guess_.data();
   }
   h.profile_bins = config.profile_bins;
   h.crossing_mode = config.profile_crossing_mode;
@@ -475,6 +468,8 @@ void EmBackend::resize_scratch(std::uint64_t n) {
   if (config_.moliere_enabled) {
     moves_.allocate(cap * lepton_move_bytes());
     moved_.allocate(cap);
+    moliere_open_.allocate(cap * moliere_open_bytes());
+    if (!moliere_open_count_.data()) moliere_open_count_.allocate(1);
   }
   scratch_capacity_ = cap;
 }
@@ -532,6 +527,8 @@ void EmBackend::front(bool photon, c8mi_em_run_result& out) {
   f.vertex_count = vertex_count_.data();
   f.moves = moves_.data();
   f.moved = moved_.data();
+  f.moliere_open = moliere_open_.data();
+  f.moliere_open_count = moliere_open_count_.data();
   int const track_buffer = track_parity_;
   if (radio_ && !photon) {
     // tracks [9][n]; the buffer's previous projection must be done first
@@ -557,7 +554,7 @@ void EmBackend::front(bool photon, c8mi_em_run_result& out) {
     track_parity_ ^= 1;
   };
   // the launches of this front, each followed by an event when timing
-  int kinds[kFrontKernels + 1];
+  int kinds[kFrontKernels + 2];
   int launched = 0;
   auto mark = [&](int kind) {
     if (!timing_) return;
@@ -571,6 +568,8 @@ void EmBackend::front(bool photon, c8mi_em_run_result& out) {
   } else if (config_.moliere_enabled) {
     // Molière scattering runs in the finish kernel, with its own register budget
     launch_lepton_move(context_.data(), config_.profile_bins, f, stream_);
+    mark(kKernLeptonTransport);
+    launch_lepton_moliere(context_.data(), host_context_.moliere_polynomials != nullptr, f, stream_);
     mark(kKernLeptonTransport);
     launch_lepton_finish(context_.data(), config_.profile_bins, f, stream_);
     mark(kKernLeptonTransport);

@@ -128,6 +128,8 @@ class EmBackend {
   DeviceBuffer<std::int32_t> children_;
   DeviceBuffer<c8mi_env_model> models_;
   DeviceBuffer<double> axis_;
+  DeviceBuffer<c8mi_moliere_cubic> moliere_polynomials_;
+  DeviceBuffer<double> moliere_guess_;
   DeviceBuffer<EmContext> context_;
   EmContext host_context_{};
   DeviceBuffer<ShowerSlot> showers_;
@@ -169,6 +171,8 @@ class EmBackend {
   DeviceBuffer<VertexRecord> vertices_;
   DeviceBuffer<unsigned char> moves_;
   DeviceBuffer<std::uint32_t> moved_;
+  DeviceBuffer<unsigned char> moliere_open_;
+  DeviceBuffer<unsigned long long> moliere_open_count_;
   DeviceBuffer<unsigned long long> vertex_count_;
   std::uint64_t scratch_capacity_ = 0;
   // radio: tracks of the last two lepton fronts; `track_done_` marks the end
@@ -184,7 +188,7 @@ class EmBackend {
   bool timing_ = false;
   std::array<double, kFrontKernels> ms_{};
   std::array<std::uint64_t, kFrontKernels> launches_{};
-  hipEvent_t events_[kFrontKernels + 2] = {};
+  hipEvent_t events_[kFrontKernels + 3] = {};  // up to 8 launches per front, each followed by one
 };
 
 }  // namespace c8mi::em

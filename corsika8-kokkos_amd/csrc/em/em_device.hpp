@@ -86,7 +86,10 @@ struct EmContext {
   double moliere_i2_large[13];  // -0.5 c2_large[i] / i
   double moliere_half_over[13]; // 0.5 / i
   std::uint32_t moliere_enabled;
-  std::uint32_t pad2;
+  std::uint32_t moliere_fast_iterations;  // Newton iterations in the step kernel (1..100)
+  // interpolation view of the snapshot (null: analytical functions)
+  c8mi_moliere_cubic const* moliere_polynomials;  // 4 x 99 cubics
+  double const* moliere_guess;                    // 64 x 8 x 128 corrections
   // profile
   std::uint32_t profile_bins;
   std::uint32_t crossing_mode;

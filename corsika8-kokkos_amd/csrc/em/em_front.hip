@@ -241,13 +241,58 @@ __global__ __launch_bounds__(kStepBlock, 3) void lepton_move_kernel(EmContext co
     static_cast<c8mi_particle_state&>(p) = load_particle(f.in, i);
     p.shower_key = c.showers[s].key;
     phys::LeptonMove mv;
+    if (i == 0) *f.moliere_open_count = 0;  // the previous front's tail kernel has finished
     bool const moved = phys::lepton_move(c, acc, p, mv);
     if (moved) static_cast<phys::LeptonMove*>(f.moves)[i] = mv;
     else f.packed[i] = pack_counts(0, 0, 0, kSlotNone, kSlotNone, false, false, s);
-    f.moved[i] = moved ? 1u : 0u;
+    f.moved[i] = moved ? kMovedOk : kMovedNone;
   }
   __syncthreads();
   flush(c, l);
+}
+
+// Molière scattering of the moved sources: the move record's end direction is
+// deflected in place; a failed sampling marks the source (kMovedMoliereFailed)
+// for the finish kernel; a sample not converged after the first Newton
+// iterations is listed for the tail kernel. A kernel of its own: the sampler's
+// registers do not weigh on the rest of the step. The context is read from
+// global memory (its snapshot and table pointers are uniform, scalar loads).
+template <bool Table>
+__global__ __launch_bounds__(kStepBlock) void lepton_moliere_kernel(EmContext const* __restrict__ context,
+                                                                    FrontArgs f) {
+  std::uint64_t const i = static_cast<std::uint64_t>(blockIdx.x) * kStepBlock + threadIdx.x;
+  if (i >= f.count || f.moved[i] != kMovedOk) return;
+  auto* mv = static_cast<phys::LeptonMove*>(f.moves) + i;
+  phys::MoliereOpen open;
+  int const outcome = phys::moliere_scatter<Table>(*context, *mv, open);
+  if (outcome == phys::kMoliereFailed) {
+    f.moved[i] = kMovedMoliereFailed;
+  } else if (outcome == phys::kMoliereOpen) {
+    open.source = static_cast<std::uint32_t>(i);
+    unsigned long long const k = atomicAdd(f.moliere_open_count, 1ULL);
+    static_cast<phys::MoliereOpen*>(f.moliere_open)[k] = open;  // at most one per source
+  }
+}
+
+// The open samples, one wavefront each (grid-stride over the list).
+template <bool Table>
+__global__ __launch_bounds__(moliere::kTailWave) void lepton_moliere_tail_kernel(
+    EmContext const* __restrict__ context, FrontArgs f) {
+  __shared__ double jobs[moliere::kDeviceComponents * moliere::kJobSlots];
+  unsigned long long const n = *f.moliere_open_count;
+  for (unsigned long long w = blockIdx.x; w < n; w += gridDim.x) {
+    phys::MoliereOpen const open = static_cast<phys::MoliereOpen const*>(f.moliere_open)[w];
+    phys::LeptonMove mv = static_cast<phys::LeptonMove const*>(f.moves)[open.source];
+    int const outcome = phys::moliere_scatter_wave<Table>(*context, mv, open, jobs);
+    if (threadIdx.x == 0) {
+      if (outcome == phys::kMoliereFailed) {
+        f.moved[open.source] = kMovedMoliereFailed;
+      } else {
+        auto* out = static_cast<phys::LeptonMove*>(f.moves) + open.source;
+        for (int a = 0; a < 3; ++a) out->end.direction[a] = mv.end.direction[a];
+      }
+    }
+  }
 }
 
 // Two waves per SIMD (a little scratch) measured faster than one for the finish.
@@ -267,7 +312,8 @@ __global__ __launch_bounds__(kStepBlock, 2) void lepton_finish_kernel(EmContext 
   std::uint32_t s = 0;
   phys::LeptonVertex vx;
   bool stepped = false;
-  if (i < f.count && f.moved[i]) {
+  std::uint32_t const moved = i < f.count ? f.moved[i] : kMovedNone;
+  if (moved != kMovedNone) {
     s = f.in.shower[i];
     phys::Accumulator const acc = accumulator(local, l, s);
     phys::LeptonMove const mv = static_cast<phys::LeptonMove const*>(f.moves)[i];
@@ -278,7 +324,7 @@ __global__ __launch_bounds__(kStepBlock, 2) void lepton_finish_kernel(EmContext 
       p.shower_key = mv.start.shower_key;
       return p;
     };
-    deferred = phys::lepton_finish(c, acc, input, mv, o, vx);
+    deferred = phys::lepton_finish(c, acc, input, mv, moved == kMovedMoliereFailed, o, vx);
     write_outputs(f, i, o, s);
     stepped = o.step;
   }
@@ -602,10 +648,28 @@ void launch_lepton_transport(EmContext const* context, std::uint32_t bins, Front
 }
 
 std::size_t lepton_move_bytes() { return sizeof(phys::LeptonMove); }
+std::size_t moliere_open_bytes() { return sizeof(phys::MoliereOpen); }
 
 void launch_lepton_move(EmContext const* context, std::uint32_t bins, FrontArgs const& f, hipStream_t stream) {
   if (f.count == 0) return;
   lepton_move_kernel<<<grid_for(f.count, kStepBlock), kStepBlock, step_lds_bytes(bins), stream>>>(context, f);
+  C8MI_HIP(hipGetLastError());
+}
+
+void launch_lepton_moliere(EmContext const* context, bool table, FrontArgs const& f, hipStream_t stream) {
+  if (f.count == 0) return;
+  if (table)
+    lepton_moliere_kernel<true><<<grid_for(f.count, kStepBlock), kStepBlock, 0, stream>>>(context, f);
+  else
+    lepton_moliere_kernel<false><<<grid_for(f.count, kStepBlock), kStepBlock, 0, stream>>>(context, f);
+  C8MI_HIP(hipGetLastError());
+  // the open samples are rare (~1e-5 of the steps): a fixed grid of single
+  // wavefronts that leave at once when there are none
+  constexpr unsigned kTailGrid = 256;
+  if (table)
+    lepton_moliere_tail_kernel<true><<<kTailGrid, moliere::kTailWave, 0, stream>>>(context, f);
+  else
+    lepton_moliere_tail_kernel<false><<<kTailGrid, moliere::kTailWave, 0, stream>>>(context, f);
   C8MI_HIP(hipGetLastError());
 }
 

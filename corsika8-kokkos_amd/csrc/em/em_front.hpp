@@ -59,7 +59,14 @@ struct FrontArgs {
   std::uint32_t* moved;               // 1 where the move kernel produced a move
   double* tracks;                     // lepton fronts with radio: [9][track_stride] compact tracks
   std::uint64_t track_stride;
+  void* moliere_open;                 // split lepton fronts: up to `count` open Molière samples
+  unsigned long long* moliere_open_count;  // zeroed by the move kernel
 };
+
+// States of a source between the split lepton kernels (FrontArgs::moved).
+inline constexpr std::uint32_t kMovedNone = 0;           // no move: the move kernel recorded a fallback
+inline constexpr std::uint32_t kMovedOk = 1;             // moved (and scattered, when Molière is on)
+inline constexpr std::uint32_t kMovedMoliereFailed = 2;  // the scattering failed: a transport fallback
 
 // Start of a shower in a slot: its parameters and first secondary history ID.
 struct SlotInit {
@@ -101,7 +108,12 @@ void launch_lepton_transport(EmContext const* context, std::uint32_t bins, Front
 // Split form of the e+- transport (when Molière scattering is on): the move,
 // then the finish (scattering, profile, outputs, vertex compaction).
 std::size_t lepton_move_bytes();
+std::size_t moliere_open_bytes();
 void launch_lepton_move(EmContext const* context, std::uint32_t bins, FrontArgs const& f, hipStream_t stream);
+// Molière scattering of the moved sources between the move and finish kernels
+// (the step kernel, then the tail kernel for samples still open);
+// table: the interpolation-view sampler (else the analytical functions).
+void launch_lepton_moliere(EmContext const* context, bool table, FrontArgs const& f, hipStream_t stream);
 void launch_lepton_finish(EmContext const* context, std::uint32_t bins, FrontArgs const& f, hipStream_t stream);
 void launch_lepton_vertex(EmContext const* context, std::uint32_t bins, FrontArgs const& f, hipStream_t stream);
 void launch_count(FrontArgs const& f, hipStream_t stream);

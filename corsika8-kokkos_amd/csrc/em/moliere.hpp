@@ -2,10 +2,12 @@
 // distribution with the analytical correction functions, two Newton-solved
 // projected angles, and the deflection of the step's end direction.
 //
-// Reference: corsika/detail/kokkos/em/models/MoliereScattering.inl:219-688
-// (normal_ppf, solve_moliere_b, f1_m / f2_m and their integrals, build_distribution,
-// evaluate_distribution, sample_one_dimensional, apply_moliere_direction and its
-// step-compatible form) and corsika/detail/kokkos/em/algorithms/MoliereStep.inl:11-101.
+// Reference: corsika/detail/kokkos/em/models/MoliereScattering.inl:118-688
+// (evaluate_interpolation, improve_moliere_initial_guess, normal_ppf,
+// solve_moliere_b, f1_m / f2_m and their integrals with and without the
+// interpolation view, build_distribution, evaluate_distribution,
+// sample_one_dimensional, apply_moliere_direction and its step-compatible form)
+// and corsika/detail/kokkos/em/algorithms/MoliereStep.inl:11-101.
 // The snapshot (c8mi_moliere_snapshot) sits in the EmContext and is read through
 // a uniform pointer, so its coefficients are scalar loads shared by the wavefront.
 #pragma once
@@ -84,6 +86,17 @@ struct Quotients {
   double const* half_over;
 };
 
+// Interpolation view (MoliereInterpolationView): the four functions' cubics on
+// x in [0, 20] and the initial-guess corrections; `polynomials` null for the
+// analytical functions.
+struct Table {
+  c8mi_moliere_cubic const* polynomials;
+  double const* guess;
+};
+
+inline constexpr double kTableUpper = 20.;
+inline constexpr int kIntervals = C8MI_MOLIERE_INTERVALS;
+
 // The snapshot was validated when the backend was created.
 template <int Cap>
 __device__ __forceinline__ int build(c8mi_moliere_snapshot const& m, double grammage, double e_mev, State<Cap>& st) {
@@ -138,20 +151,33 @@ __device__ __noinline__ double integral_f1_large(c8mi_moliere_snapshot const& m,
   for (int i = 13; i >= 0; --i) sum = m.c1_integral_large[i] + sum / x;
   return sum;
 }
+// f2 and its integral between 4.25^2 and 6.5^2 (fitted), and the terms of
+// their expansions above.
+__device__ __forceinline__ double f2_fit(double x) {
+  return 0.00013567765224589459194769192063035 * x - 0.0022635502525409950842771866774683 * sqrt(x) +
+         0.0098037758070269476889935233998585;
+}
+__device__ __forceinline__ double integral_f2_fit(double x) {
+  return -0.00026360133958801203364619158975302 * x + 0.0039965027465457608410459577896745 * sqrt(x) -
+         0.016305842044996649714549974419242;
+}
+__device__ __forceinline__ double f2_term(c8mi_moliere_snapshot const& m, double x, int i) {
+  return m.c2_large[i] * (0.5 * log(x) + m.s2_large[i]) * pow(x, -(i + 0.5));
+}
+__device__ __forceinline__ double integral_f2_term(c8mi_moliere_snapshot const& m, Quotients const& q, double x,
+                                                   int i) {
+  return q.i2_large[i] * (q.half_over[i] + 0.5 * log(x) + m.s2_large[i]) * pow(x, -i);
+}
 __device__ __noinline__ double f2_large(c8mi_moliere_snapshot const& m, double x) {
-  if (x <= 6.5 * 6.5)
-    return 0.00013567765224589459194769192063035 * x - 0.0022635502525409950842771866774683 * sqrt(x) +
-           0.0098037758070269476889935233998585;
+  if (x <= 6.5 * 6.5) return f2_fit(x);
   double sum = 0.;
-  for (int i = 2; i < 13; ++i) sum += m.c2_large[i] * (0.5 * log(x) + m.s2_large[i]) * pow(x, -(i + 0.5));
+  for (int i = 2; i < 13; ++i) sum += f2_term(m, x, i);
   return sum;
 }
 __device__ __noinline__ double integral_f2_large(c8mi_moliere_snapshot const& m, Quotients const& q, double x) {
-  if (x <= 6.5 * 6.5)
-    return -0.00026360133958801203364619158975302 * x + 0.0039965027465457608410459577896745 * sqrt(x) -
-           0.016305842044996649714549974419242;
+  if (x <= 6.5 * 6.5) return integral_f2_fit(x);
   double sum = 0.;
-  for (int i = 2; i < 13; ++i) sum += q.i2_large[i] * (q.half_over[i] + 0.5 * log(x) + m.s2_large[i]) * pow(x, -i);
+  for (int i = 2; i < 13; ++i) sum += integral_f2_term(m, q, x, i);
   return sum;
 }
 
@@ -200,42 +226,271 @@ __device__ __forceinline__ void evaluate(c8mi_moliere_snapshot const& m, Quotien
   cumulative = theta < 0. ? -cdf : cdf;
 }
 
-template <int Cap>
-__device__ __forceinline__ int one_dimensional(c8mi_moliere_snapshot const& m, Quotients const& q,
-                                               State<Cap> const& st, double u, double& angle) {
-  angle = 0.;
+// evaluate_distribution with an interpolation view: the cubic of x's interval
+// for x <= 20 (the four functions share the interval), the large-x branches
+// above (x > 20 is past both branch points of the series).
+template <int N>
+__device__ __forceinline__ void evaluate_table(c8mi_moliere_snapshot const& m, Quotients const& q, Table const& t,
+                                               State<N> const& st, double theta, double& density,
+                                               double& cumulative) {
+  double const t2 = theta * theta;
+  double dens = 0., cdf = 0.;
+#pragma unroll 1
+  for (int k = 0; k < N; ++k) {
+    if (k >= static_cast<int>(m.component_count)) break;
+    double const x = t2 * st.inverse_scale[k], ib = st.inverse_b[k], ib2 = ib * ib;
+    double f1v, f2v, i1v, i2v;
+    if (x <= kTableUpper) {
+      // evaluate_interpolation (MoliereScattering.inl:118-134)
+      double coordinate = (x - 0.) * (static_cast<double>(kIntervals) / (kTableUpper - 0.));
+      if (coordinate < 0.) coordinate = 0.;
+      if (coordinate > static_cast<double>(kIntervals)) coordinate = static_cast<double>(kIntervals);
+      auto interval = static_cast<std::uint32_t>(coordinate);
+      if (interval >= static_cast<std::uint32_t>(kIntervals)) interval = kIntervals - 1;
+      double const u = coordinate - static_cast<double>(interval);
+      auto cubic = [&](int f) {
+        double const* c = t.polynomials[f * kIntervals + interval].c;
+        return c[0] + u * (c[1] + u * (c[2] + u * c[3]));
+      };
+      f1v = cubic(0);
+      f2v = cubic(1);
+      i1v = cubic(2);
+      i2v = cubic(3);
+    } else {
+      f1v = f1_large(x);
+      i1v = integral_f1_large(m, x);
+      f2v = f2_large(m, x);
+      i2v = integral_f2_large(m, q, x);
+    }
+    double const root = sqrt(x > 0. ? x : 0.);
+    dens += st.density_weight[k] * (exp(-x) + f1v * ib + f2v * ib2);
+    cdf += m.components[k].weight_zz * (0.5 * erf(root) + sqrt(1. / kPi) * (i1v * ib + i2v * ib2));
+  }
+  density = dens * m.inverse_weight_zz_sum;
+  cdf *= m.inverse_weight_zz_sum;
+  cumulative = theta < 0. ? -cdf : cdf;
+}
+
+// improve_moliere_initial_guess (MoliereScattering.inl:136-216): the Gaussian
+// coordinate plus the trilinear interpolation of the tabulated correction.
+__device__ __forceinline__ double improve_guess(double const* g, double reference_b, double beta2, double gaussian) {
+  constexpr double b_lo = 4.5, b_hi = 64., beta_lo = 0.25, beta_hi = 1., c_lo = 0., c_hi = 5.;
+  constexpr std::uint32_t nb = C8MI_MOLIERE_GUESS_B_NODES, nt = C8MI_MOLIERE_GUESS_BETA2_NODES,
+                          nc = C8MI_MOLIERE_GUESS_COORDINATE_NODES;
+  constexpr double inv_b = static_cast<double>(nb - 1) / (b_hi - b_lo);
+  constexpr double inv_beta = static_cast<double>(nt - 1) / (beta_hi - beta_lo);
+  constexpr double inv_c = static_cast<double>(nc - 1) / (c_hi - c_lo);
+  if (!isfinite(reference_b) || !isfinite(beta2) || !isfinite(gaussian)) return gaussian;
+  double const mag = fabs(gaussian);
+  if (reference_b < b_lo || reference_b > b_hi || beta2 < beta_lo || beta2 > beta_hi || mag < c_lo || mag > c_hi)
+    return gaussian;
+  double const bc = (reference_b - b_lo) * inv_b, cc = (mag - c_lo) * inv_c, tc = (beta2 - beta_lo) * inv_beta;
+  auto bi = static_cast<std::uint32_t>(bc), ti = static_cast<std::uint32_t>(tc), ci = static_cast<std::uint32_t>(cc);
+  if (bi + 1 >= nb) bi = nb - 2;
+  if (ti + 1 >= nt) ti = nt - 2;
+  if (ci + 1 >= nc) ci = nc - 2;
+  double const bf = bc - static_cast<double>(bi), tf = tc - static_cast<double>(ti), cf = cc - static_cast<double>(ci);
+  std::uint32_t const beta_stride = nt * nc;
+  std::uint32_t const lb = bi * beta_stride, ub = lb + beta_stride;
+  std::uint32_t const lt = ti * nc + ci, ut = lt + nc;
+  double const ll = g[lb + lt] * (1. - cf) + g[lb + lt + 1] * cf;
+  double const lu = g[lb + ut] * (1. - cf) + g[lb + ut + 1] * cf;
+  double const ul = g[ub + lt] * (1. - cf) + g[ub + lt + 1] * cf;
+  double const uu = g[ub + ut] * (1. - cf) + g[ub + ut + 1] * cf;
+  double const lower = ll * (1. - tf) + lu * tf;
+  double const upper = ul * (1. - tf) + uu * tf;
+  double const corrected = mag + lower * (1. - bf) + upper * bf;
+  if (!isfinite(corrected) || corrected < 0.) return gaussian;
+  return gaussian < 0. ? -corrected : corrected;
+}
+
+// Start of sample_one_dimensional: the invalid-input and median cases, else the
+// Newton start (the Gaussian quantile, corrected by the tabulated guess).
+inline constexpr int kStarted = -1;  // Newton iteration to follow
+template <bool Tab, int Cap>
+__device__ __forceinline__ int newton_start(c8mi_moliere_snapshot const& m, Table const& t, State<Cap> const& st,
+                                            double u, double& current) {
+  current = 0.;
   if (!isfinite(u) || !(u > 0.) || !(u < 1.)) return kInvalidInput;
   if (u == 0.5) return kSuccess;
-  double current = st.prefactor * (normal_ppf(u) / kSqrt2);
+  double const gaussian = normal_ppf(u) / kSqrt2;
+  if constexpr (Tab) {
+    double const reference_b = 1. / st.inverse_b[m.maximum_weight_index];
+    current = st.prefactor * improve_guess(t.guess, reference_b, st.beta_squared, gaussian);
+  } else {
+    current = st.prefactor * gaussian;
+  }
+  return kStarted;
+}
+
+// One Newton update of sample_one_dimensional: `current` becomes the next
+// iterate; kStarted while not converged.
+__device__ __forceinline__ int newton_update(double density, double cdf, double target, double& current) {
+  double const previous = current;
+  if (!isfinite(density) || density == 0.) return kNonFinite;  // current stays the previous iterate
+  current = previous - (cdf - target) / density;
+  if (!isfinite(current)) return kNonFinite;
+  if (fabs((previous - current) / current) <= 5.e-5) return kSuccess;
+  return kStarted;
+}
+
+inline constexpr int kMaximumIterations = 100;
+inline constexpr int kPending = -2;  // not converged within the iterations allowed here
+
+// Newton iterations `first` .. `last` (1-based, at most 100) from `current`:
+// kSuccess, kNonFinite, kNonConvergent after iteration 100, or kPending after
+// `last` < 100.
+template <bool Tab, int Cap>
+__device__ __forceinline__ int newton_iterate(c8mi_moliere_snapshot const& m, Quotients const& q, Table const& t,
+                                              State<Cap> const& st, double u, int first, int last,
+                                              double& current) {
   double const target = u - 0.5;
-  for (int it = 1; it <= 100; ++it) {
-    double const previous = current;
+  for (int it = first; it <= last; ++it) {
     double density, cdf;
-    evaluate(m, q, st, previous, density, cdf);
-    if (!isfinite(density) || density == 0.) {
-      angle = previous;
-      return kNonFinite;
+    if constexpr (Tab) evaluate_table(m, q, t, st, current, density, cdf);
+    else evaluate(m, q, st, current, density, cdf);
+    int const status = newton_update(density, cdf, target, current);
+    if (status != kStarted) return status;
+  }
+  return last >= kMaximumIterations ? kNonConvergent : kPending;
+}
+
+template <bool Tab, int Cap>
+__device__ __forceinline__ int one_dimensional(c8mi_moliere_snapshot const& m, Quotients const& q, Table const& t,
+                                               State<Cap> const& st, double u, double& angle) {
+  int const status = newton_start<Tab>(m, t, st, u, angle);
+  if (status != kStarted) return status;
+  return newton_iterate<Tab>(m, q, t, st, u, 1, kMaximumIterations, angle);
+}
+
+// ---- the slow tail: one wavefront per sample that did not converge early ----
+//
+// About one e+- step in 1e5 draws a far-tail quantile whose Newton iteration
+// runs to its limit of 100 through the large-x expansions (two dozen pow per
+// component and iteration). Serially that is ~1 ms of one lane, and the whole
+// launch waits for it. The step kernels therefore stop after a few iterations
+// (EmContext::moliere_fast_iterations); a sample still open continues here
+// with its iterate: each evaluation spreads
+// the exp, erf, large-x functions and expansion terms of every component over
+// the 64 lanes (LDS `jobs`), then every lane sums them in the serial order, so
+// density, CDF and iterates are bit-identical to the serial evaluation.
+
+inline constexpr int kJobSlots = 26;  // per component: exp, erf, f1, integral f1, 11 + 11 terms
+inline constexpr int kTailWave = 64;
+
+// Horner of a 70-coefficient power series (the order of each chain of evaluate).
+__device__ __forceinline__ double series70(double const* c, double x) {
+  double s = c[69];
+  for (int i = 68; i >= 0; --i) s = s * x + c[i];
+  return s;
+}
+
+template <bool Tab, int N>
+__device__ __forceinline__ void evaluate_wave(c8mi_moliere_snapshot const& m, Quotients const& q, Table const& t,
+                                              State<N> const& st, double theta, double* jobs, double& density,
+                                              double& cumulative) {
+  double const t2 = theta * theta;
+  int const nc = static_cast<int>(m.component_count);
+  double const f1_from = Tab ? kTableUpper : 12.;
+  __syncthreads();  // the previous evaluation has read `jobs`
+  for (int j = threadIdx.x; j < nc * kJobSlots; j += kTailWave) {
+    int const k = j / kJobSlots, slot = j % kJobSlots;
+    double scale = 0.;
+#pragma unroll
+    for (int c = 0; c < N; ++c)
+      if (c == k) scale = st.inverse_scale[c];
+    double const x = t2 * scale;
+    double v = 0.;
+    if (slot == 0) v = exp(-x);
+    else if (slot == 1) v = erf(sqrt(x > 0. ? x : 0.));
+    else if (slot == 2) v = x > f1_from ? f1_large(x) : 0.;
+    else if (slot == 3) v = x > f1_from ? integral_f1_large(m, x) : 0.;
+    else if (slot < 15) v = x > 6.5 * 6.5 ? f2_term(m, x, slot - 2) : 0.;
+    else v = x > 6.5 * 6.5 ? integral_f2_term(m, q, x, slot - 13) : 0.;
+    jobs[j] = v;
+  }
+  __syncthreads();
+  double dens = 0., cdf = 0.;
+#pragma unroll
+  for (int k = 0; k < N; ++k) {
+    if (k >= nc) break;
+    double const* J = jobs + k * kJobSlots;
+    double const x = t2 * st.inverse_scale[k], ib = st.inverse_b[k], ib2 = ib * ib;
+    double const root = sqrt(x > 0. ? x : 0.);
+    double f1v, f2v, i1v, i2v;
+    auto f2_above = [&] {
+      if (x <= 6.5 * 6.5) return f2_fit(x);
+      double sum = 0.;
+      for (int i = 2; i < 13; ++i) sum += J[2 + i];
+      return sum;
+    };
+    auto i2_above = [&] {
+      if (x <= 6.5 * 6.5) return integral_f2_fit(x);
+      double sum = 0.;
+      for (int i = 2; i < 13; ++i) sum += J[13 + i];
+      return sum;
+    };
+    if (Tab && x <= kTableUpper) {
+      double coordinate = (x - 0.) * (static_cast<double>(kIntervals) / (kTableUpper - 0.));
+      if (coordinate < 0.) coordinate = 0.;
+      if (coordinate > static_cast<double>(kIntervals)) coordinate = static_cast<double>(kIntervals);
+      auto interval = static_cast<std::uint32_t>(coordinate);
+      if (interval >= static_cast<std::uint32_t>(kIntervals)) interval = kIntervals - 1;
+      double const u = coordinate - static_cast<double>(interval);
+      auto cubic = [&](int f) {
+        double const* c = t.polynomials[f * kIntervals + interval].c;
+        return c[0] + u * (c[1] + u * (c[2] + u * c[3]));
+      };
+      f1v = cubic(0);
+      f2v = cubic(1);
+      i1v = cubic(2);
+      i2v = cubic(3);
+    } else if (Tab) {
+      f1v = J[2];
+      i1v = J[3];
+      f2v = f2_above();
+      i2v = i2_above();
+    } else {
+      f1v = x > 12. ? J[2] : series70(m.c1, x);
+      i1v = x > 12. ? J[3] : series70(q.i1, x) * root;
+      f2v = x > 4.25 * 4.25 ? f2_above() : series70(m.c2, x);
+      i2v = x > 4.25 * 4.25 ? i2_above() : series70(q.i2, x) * root;
     }
-    current = previous - (cdf - target) / density;
-    angle = current;
-    if (!isfinite(current)) return kNonFinite;
-    if (fabs((previous - current) / current) <= 5.e-5) return kSuccess;
+    dens += st.density_weight[k] * (J[0] + f1v * ib + f2v * ib2);
+    cdf += m.components[k].weight_zz * (0.5 * J[1] + sqrt(1. / kPi) * (i1v * ib + i2v * ib2));
+  }
+  density = dens * m.inverse_weight_zz_sum;
+  cdf *= m.inverse_weight_zz_sum;
+  cumulative = theta < 0. ? -cdf : cdf;
+}
+
+// newton_iterate with the wavefront evaluation (every lane the same iterates).
+template <bool Tab, int Cap>
+__device__ __forceinline__ int newton_iterate_wave(c8mi_moliere_snapshot const& m, Quotients const& q,
+                                                   Table const& t, State<Cap> const& st, double u, int first,
+                                                   double* jobs, double& current) {
+  double const target = u - 0.5;
+  for (int it = first; it <= kMaximumIterations; ++it) {
+    double density, cdf;
+    evaluate_wave<Tab>(m, q, t, st, current, jobs, density, cdf);
+    int const status = newton_update(density, cdf, target, current);
+    if (status != kStarted) return status;
   }
   return kNonConvergent;
 }
 
 // Space angle of two projected draws (sample_moliere_scattering_angle_2d).
-template <int Cap>
-__device__ __forceinline__ int angle_2d_for(c8mi_moliere_snapshot const& m, Quotients const& q, double grammage,
-                                            double e_mev, double u1, double u2, double& angle) {
+template <bool Tab, int Cap>
+__device__ __forceinline__ int angle_2d_for(c8mi_moliere_snapshot const& m, Quotients const& q, Table const& t,
+                                            double grammage, double e_mev, double u1, double u2, double& angle) {
   State<Cap> st;
   angle = 0.;
   int status = build(m, grammage, e_mev, st);
   if (status != kSuccess) return status;
   double a1, a2;
-  status = one_dimensional(m, q, st, u1, a1);
+  status = one_dimensional<Tab>(m, q, t, st, u1, a1);
   if (status != kSuccess) return status;
-  status = one_dimensional(m, q, st, u2, a2);
+  status = one_dimensional<Tab>(m, q, t, st, u2, a2);
   if (status != kSuccess) return status;
   angle = sqrt(a1 * a1 + a2 * a2);
   return isfinite(angle) ? kSuccess : kNonFinite;
@@ -244,10 +499,6 @@ __device__ __forceinline__ int angle_2d_for(c8mi_moliere_snapshot const& m, Quot
 // Media of up to kDeviceComponents components (air has three; the backend
 // rejects larger Molière snapshots at create).
 inline constexpr int kDeviceComponents = 4;
-__device__ __forceinline__ int angle_2d(c8mi_moliere_snapshot const& m, Quotients const& q, double grammage,
-                                     double e_mev, double u1, double u2, double& angle) {
-  return angle_2d_for<kDeviceComponents>(m, q, grammage, e_mev, u1, u2, angle);
-}
 
 // apply_moliere_direction_cpu_step_compatible: normalize(tracked + rotate(start) - start).
 __device__ __forceinline__ int deflect_end(double const start[3], double const tracked[3], double angle, double au,

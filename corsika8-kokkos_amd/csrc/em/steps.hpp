@@ -483,43 +483,125 @@ __device__ __forceinline__ bool lepton_move(EmContext const& c, Accumulator cons
   return true;
 }
 
-// The rest of the step: Molière scattering of the end direction, profile and
-// ledger, records and outputs. `input()` gives the step's input particle (the
-// fallback state of a failed scattering, as the transport record's interaction
-// state; fetched only then). Returns true when the step ends at an interaction
-// vertex: `vx` then holds what lepton_vertex needs.
+// apply_moliere_scattering_stage (MoliereStep.inl:74-99) on a move, with
+// sample_moliere_scattering_angle_2d (MoliereScattering.inl:513-540): draws
+// keyed by the start state, two projected angles, and on success the end
+// direction becomes normalize(tracked + scattered - start). A failed sampling
+// or deflection turns the step into a transport fallback. `Table`: the
+// interpolation view's sampler, else the analytical functions.
+enum MoliereOutcome : int { kMoliereDone = 0, kMoliereFailed = 1, kMoliereOpen = 2 };
+
+// A sample whose Newton iteration did not converge in the step kernel: where
+// to continue (stage 1: the first projected angle, 2: the second).
+struct MoliereOpen {
+  std::uint32_t source;
+  std::uint32_t stage;
+  std::int32_t iterations;  // done so far for this sample
+  std::uint32_t pad;
+  double current;           // the Newton iterate
+  double first_angle;       // stage 2: the first projected angle
+};
+
+struct MoliereDraws {
+  double u1, u2, ua;
+};
+
+__device__ __forceinline__ MoliereDraws moliere_draws(EmContext const& c, Particle const& start) {
+  return {uniform(c, start, kScatteringProcess, 0), uniform(c, start, kScatteringProcess, 1),
+          uniform(c, start, kScatteringProcess, 2)};
+}
+
+// The space angle of the two projected angles, then the deflection of the end direction.
+__device__ __forceinline__ int moliere_apply(LeptonMove& mv, double a1, double a2, double ua) {
+  double const angle = sqrt(a1 * a1 + a2 * a2);
+  double dir[3];
+  if (!isfinite(angle) || moliere::deflect_end(mv.start.direction, mv.end.direction, angle, ua, dir) != 0)
+    return kMoliereFailed;
+  for (int a = 0; a < 3; ++a) mv.end.direction[a] = dir[a];
+  return kMoliereDone;
+}
+
+// Up to the context's moliere_fast_iterations Newton iterations per sample; a
+// sample still open is described in `open` (kMoliereOpen) for
+// moliere_scatter_wave.
+template <bool Table>
+__device__ __forceinline__ int moliere_scatter(EmContext const& c, LeptonMove& mv, MoliereOpen& open) {
+  if (mv.particle_cut) return kMoliereDone;
+  MoliereDraws const d = moliere_draws(c, mv.start);
+  moliere::Quotients const q{c.moliere_i1, c.moliere_i2, c.moliere_i2_large, c.moliere_half_over};
+  moliere::Table const t{c.moliere_polynomials, c.moliere_guess};
+  moliere::State<moliere::kDeviceComponents> st;
+  int status = moliere::build(c.moliere, mv.step_grammage, mv.start.energy_gev * 1000., st);
+  if (status == moliere::kNoDeflection) return kMoliereDone;
+  if (status != moliere::kSuccess) return kMoliereFailed;
+  double a[2];
+  double const u[2] = {d.u1, d.u2};
+  for (int k = 0; k < 2; ++k) {
+    status = moliere::newton_start<Table>(c.moliere, t, st, u[k], a[k]);
+    int const fast = static_cast<int>(c.moliere_fast_iterations);
+    if (status == moliere::kStarted)
+      status = moliere::newton_iterate<Table>(c.moliere, q, t, st, u[k], 1, fast, a[k]);
+    if (status == moliere::kPending) {
+      open = {0u, static_cast<std::uint32_t>(k + 1), fast, 0u, a[k], k ? a[0] : 0.};
+      return kMoliereOpen;
+    }
+    if (status != moliere::kSuccess) return kMoliereFailed;
+  }
+  return moliere_apply(mv, a[0], a[1], d.ua);
+}
+
+// The rest of an open sample (and the second one after it) on one wavefront
+// of kTailWave lanes; every lane returns the same outcome and deflected move.
+template <bool Table>
+__device__ __forceinline__ int moliere_scatter_wave(EmContext const& c, LeptonMove& mv, MoliereOpen const& open,
+                                                    double* jobs) {
+  MoliereDraws const d = moliere_draws(c, mv.start);
+  moliere::Quotients const q{c.moliere_i1, c.moliere_i2, c.moliere_i2_large, c.moliere_half_over};
+  moliere::Table const t{c.moliere_polynomials, c.moliere_guess};
+  moliere::State<moliere::kDeviceComponents> st;
+  moliere::build(c.moliere, mv.step_grammage, mv.start.energy_gev * 1000., st);  // succeeded before
+  double a1, a2;
+  int status;
+  if (open.stage == 1) {
+    a1 = open.current;
+    status = moliere::newton_iterate_wave<Table>(c.moliere, q, t, st, d.u1, open.iterations + 1, jobs, a1);
+    if (status != moliere::kSuccess) return kMoliereFailed;
+    status = moliere::newton_start<Table>(c.moliere, t, st, d.u2, a2);
+    if (status == moliere::kStarted)
+      status = moliere::newton_iterate_wave<Table>(c.moliere, q, t, st, d.u2, 1, jobs, a2);
+  } else {
+    a1 = open.first_angle;
+    a2 = open.current;
+    status = moliere::newton_iterate_wave<Table>(c.moliere, q, t, st, d.u2, open.iterations + 1, jobs, a2);
+  }
+  if (status != moliere::kSuccess) return kMoliereFailed;
+  return moliere_apply(mv, a1, a2, d.ua);
+}
+
+// The rest of the step after Molière scattering (applied to `mv` beforehand, or
+// off): profile and ledger, records and outputs. `moliere_failed`: the
+// scattering failed, the step becomes a transport fallback whose state is the
+// transport record's interaction state (the vertex for an interaction
+// candidate, else the step's input particle, `input()`, fetched only then).
+// Returns true when the step ends at an interaction vertex: `vx` then holds
+// what lepton_vertex needs.
 template <class Input>
 __device__ __forceinline__ bool lepton_finish(EmContext const& c, Accumulator const& acc, Input const& input,
-                                              LeptonMove const& mv, Outputs& o, LeptonVertex& vx) {
+                                              LeptonMove const& mv, bool moliere_failed, Outputs& o,
+                                              LeptonVertex& vx) {
   Particle const& start = mv.start;
   Particle end;
   static_cast<c8mi_particle_state&>(end) = mv.end;
   end.shower_key = start.shower_key;
   int const limit = mv.limit;
-  bool const particle_cut = mv.particle_cut != 0;
-  double const step_grammage = mv.step_grammage;
   double const continuous_deposit = mv.continuous_deposit, cut_deposit = mv.cut_deposit;
   bool const observed_before_cut = mv.observed_before_cut != 0;
+  (void)c;
 
-  // apply_moliere_scattering_stage (MoliereStep.inl:74-99): every finite step;
-  // draws keyed by the start state
-  if (c.moliere_enabled && !particle_cut) {
-    double const u1 = uniform(c, start, kScatteringProcess, 0);
-    double const u2 = uniform(c, start, kScatteringProcess, 1);
-    double const ua = uniform(c, start, kScatteringProcess, 2);
-    double angle;
-    moliere::Quotients const q{c.moliere_i1, c.moliere_i2, c.moliere_i2_large, c.moliere_half_over};
-    int const ms = moliere::angle_2d(c.moliere, q, step_grammage, start.energy_gev * 1000., u1, u2, angle);
-    if (ms != moliere::kNoDeflection) {
-      double dir[3];
-      if (ms != moliere::kSuccess || moliere::deflect_end(start.direction, end.direction, angle, ua, dir) != 0) {
-        // the transport record's interaction state: the vertex for an interaction candidate
-        if (limit == kLlInteraction) record_fallback(acc, end, kFrMoliereFailed, 0, kStageTransport);
-        else record_fallback(acc, input(), kFrMoliereFailed, mv.process_id, kStageTransport);
-        return false;
-      }
-      for (int a = 0; a < 3; ++a) end.direction[a] = dir[a];
-    }
+  if (moliere_failed) {
+    if (limit == kLlInteraction) record_fallback(acc, end, kFrMoliereFailed, 0, kStageTransport);
+    else record_fallback(acc, input(), kFrMoliereFailed, mv.process_id, kStageTransport);
+    return false;
   }
   o.step = true;
   if (o.track) write_track(o.track, o.track_stride, start, end);  // e+- only reach here
@@ -562,12 +644,13 @@ __device__ __forceinline__ bool lepton_finish(EmContext const& c, Accumulator co
   return true;
 }
 
-// The step of one e+- up to its interaction vertex (move, then finish).
+// The step of one e+- up to its interaction vertex (move, then finish),
+// without Molière scattering.
 __device__ __forceinline__ bool lepton_transport(EmContext const& c, Accumulator const& acc, Particle const& p,
                                                  Outputs& o, LeptonVertex& vx) {
   LeptonMove mv;
   if (!lepton_move(c, acc, p, mv)) return false;
-  return lepton_finish(c, acc, [&] { return p; }, mv, o, vx);
+  return lepton_finish(c, acc, [&] { return p; }, mv, false, o, vx);
 }
 
 // Interaction vertex of an e+- step: vertex reselection at the post-loss
@@ -756,13 +839,6 @@ __device__ __forceinline__ void lepton_vertex(EmContext const& c, Accumulator co
   ledger(acc, kMassCorrection, mass_correction(parent.pid, lm, (th.keep & 1U) ? th.w1 : 0.));
   thinning_ledger(acc, th.status, 2, k, (e1 + pe) * w,
                   ((th.keep & 1U) ? e1 * th.w1 : 0.) + ((th.keep & 2U) ? pe * th.w2 : 0.));
-}
-
-// The whole e+- step (transport, then the vertex when one is reached).
-__device__ __forceinline__ void lepton_step(EmContext const& c, Accumulator const& acc, Particle const& p,
-                                            Outputs& o) {
-  LeptonVertex vx;
-  if (lepton_transport(c, acc, p, o, vx)) lepton_vertex(c, acc, vx, o);
 }
 
 }  // namespace c8mi::em::phys

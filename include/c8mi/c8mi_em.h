@@ -195,6 +195,47 @@ typedef struct c8mi_moliere_snapshot {
   double c1_integral_large[15];
 } c8mi_moliere_snapshot;
 
+/* MoliereInterpolationTable (corsika/kokkos/em/tables/MoliereInterpolation.hpp:19-56),
+ * byte-compatible with the reference's record and with the table part of its
+ * cache file: f1, f2 and the integrals of f1 and f2 (in that order) as cubic
+ * polynomials in q = (x - x_k) / h, ascending powers, on the 99 intervals of
+ * x in [0, 20]; then the Newton initial-guess corrections, tabulated in
+ * reference B (64 nodes on [4.5, 64]), beta^2 (8 on [0.25, 1]) and the
+ * Gaussian quantile coordinate (128 on [0, 5]), index (b * 8 + beta) * 128 + c. */
+#define C8MI_MOLIERE_INTERVALS 99
+#define C8MI_MOLIERE_GUESS_B_NODES 64
+#define C8MI_MOLIERE_GUESS_BETA2_NODES 8
+#define C8MI_MOLIERE_GUESS_COORDINATE_NODES 128
+typedef struct c8mi_moliere_cubic {
+  double c[4];
+} c8mi_moliere_cubic;
+
+typedef struct c8mi_moliere_interpolation {
+  c8mi_moliere_cubic polynomials[4 * C8MI_MOLIERE_INTERVALS];
+  double initial_guess_delta[C8MI_MOLIERE_GUESS_B_NODES * C8MI_MOLIERE_GUESS_BETA2_NODES *
+                             C8MI_MOLIERE_GUESS_COORDINATE_NODES];
+} c8mi_moliere_interpolation;
+
+/* make_moliere_interpolation_table (src/kokkos/em/tables/MoliereInterpolation.cpp:438-460)
+ * on the host: interpolating cubic splines with natural ends through f1, f2 and
+ * their integrals at the 100 nodes, written per interval in the Hermite form of
+ * the reference (:272-310), then the initial-guess corrections by bisection of
+ * the mixture CDF (:362-436). The reference builds its splines with the
+ * CubicInterpolation library (not part of this build); its cache file holds the
+ * exact production table and can be read with c8mi_em_moliere_interpolation_read. */
+int c8mi_em_moliere_interpolation_build(const c8mi_moliere_snapshot* snapshot,
+                                        c8mi_moliere_interpolation* out);
+
+/* read_interpolation_cache / write_interpolation_cache
+ * (MoliereInterpolation.cpp:204-270): the reference's "C8MOLI01" version-1
+ * cache file (header with the snapshot and an FNV-1a checksum of the table,
+ * then the table). Reading fails (status 1, invalid argument) unless the file
+ * is complete, its checksum and snapshot match and every entry is finite. */
+int c8mi_em_moliere_interpolation_read(const char* path, const c8mi_moliere_snapshot* snapshot,
+                                       c8mi_moliere_interpolation* out);
+int c8mi_em_moliere_interpolation_write(const char* path, const c8mi_moliere_snapshot* snapshot,
+                                        const c8mi_moliere_interpolation* table);
+
 /* EmThinningConfig (corsika/kokkos/em/algorithms/EmThinning.hpp:25-30). */
 typedef struct c8mi_em_thinning {
   double threshold_gev;
@@ -220,12 +261,20 @@ typedef struct c8mi_em_config {
   c8mi_photon_pair_lpm photon_pair_lpm;
   c8mi_brems_lpm brems_lpm;
   c8mi_em_thinning thinning;
-  /* Molière multiple scattering of e+- steps (MoliereStep.inl:74-101) with the
-   * analytical correction functions (no interpolation table): enabled when
-   * moliere_enabled is 1; the snapshot is the electron one of the medium. */
+  /* Molière multiple scattering of e+- steps (MoliereStep.inl:74-101): enabled
+   * when moliere_enabled is 1; the snapshot is the electron one of the medium. */
   c8mi_moliere_snapshot electron_moliere;
   uint32_t moliere_enabled;
-  uint32_t moliere_reserved_;
+  /* Newton iterations per projected angle in the step kernel (0: 12); a sample
+   * still unconverged continues on a wavefront of its own. Results do not
+   * depend on it (launch tuning only). */
+  uint32_t moliere_fast_iterations;
+  /* The interpolation table of that snapshot (the reference backend always
+   * builds one, BackendInstanceImplementation.inl:469): f1, f2 and their
+   * integrals from the table for x <= 20 and the tabulated Newton initial
+   * guess. NULL: the analytical functions everywhere. Host memory, copied at
+   * create. */
+  const c8mi_moliere_interpolation* electron_moliere_interpolation;
   uint32_t profile_bins;
   uint32_t profile_crossing_mode;
   double profile_bin_width_g_per_cm2;

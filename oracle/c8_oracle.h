@@ -181,6 +181,12 @@ int oracle_moliere_evaluate(const c8mi_moliere_snapshot* m, double grammage, dou
                             double* density, double* cumulative);
 int oracle_moliere_angle(const c8mi_moliere_snapshot* m, double grammage, double e_mev, double u1, double u2,
                          uint32_t* iterations, double* angle);
+/* The same with an interpolation table (NULL: analytical functions). */
+int oracle_moliere_angle_interpolated(const c8mi_moliere_snapshot* m, const c8mi_moliere_interpolation* t,
+                                      double grammage, double e_mev, double u1, double u2, uint32_t* iterations,
+                                      double* angle);
+void oracle_moliere_interpolated_functions(const c8mi_moliere_snapshot* m, const c8mi_moliere_interpolation* t,
+                                           double x, double out[4]);
 int oracle_moliere_direction(const double start[3], const double tracked[3], double angle, double au,
                              double out[3]);
 

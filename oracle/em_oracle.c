@@ -1816,6 +1816,82 @@ double oracle_moliere_integral_f2(const c8mi_moliere_snapshot* m, double x) {
   return sum * sqrt(x > 0. ? x : 0.);
 }
 
+/* ---- interpolation view (MoliereScattering.inl:118-216, 350-385) ---- */
+
+#define MOLIERE_TABLE_UPPER 20.
+
+/* evaluate_interpolation: cubic of the interval holding x on [0, 20]. */
+static double moliere_interpolate(const c8mi_moliere_cubic* p, double x) {
+  double inverse_step = (double)C8MI_MOLIERE_INTERVALS / (MOLIERE_TABLE_UPPER - 0.);
+  double coordinate = (x - 0.) * inverse_step;
+  if (coordinate < 0.) coordinate = 0.;
+  if (coordinate > (double)C8MI_MOLIERE_INTERVALS) coordinate = (double)C8MI_MOLIERE_INTERVALS;
+  uint32_t interval = (uint32_t)coordinate;
+  if (interval >= C8MI_MOLIERE_INTERVALS) interval = C8MI_MOLIERE_INTERVALS - 1;
+  double q = coordinate - (double)interval;
+  const double* c = p[interval].c;
+  return c[0] + q * (c[1] + q * (c[2] + q * c[3]));
+}
+
+/* f1_m / f2_m / integral_f1_m / integral_f2_m with a view: the table for x <= 20. */
+static double mol_f1(const c8mi_moliere_snapshot* m, const c8mi_moliere_interpolation* t, double x) {
+  return t && x <= MOLIERE_TABLE_UPPER ? moliere_interpolate(t->polynomials, x) : oracle_moliere_f1(m, x);
+}
+static double mol_f2(const c8mi_moliere_snapshot* m, const c8mi_moliere_interpolation* t, double x) {
+  return t && x <= MOLIERE_TABLE_UPPER ? moliere_interpolate(t->polynomials + C8MI_MOLIERE_INTERVALS, x)
+                                       : oracle_moliere_f2(m, x);
+}
+static double mol_if1(const c8mi_moliere_snapshot* m, const c8mi_moliere_interpolation* t, double x) {
+  return t && x <= MOLIERE_TABLE_UPPER ? moliere_interpolate(t->polynomials + 2 * C8MI_MOLIERE_INTERVALS, x)
+                                       : oracle_moliere_integral_f1(m, x);
+}
+static double mol_if2(const c8mi_moliere_snapshot* m, const c8mi_moliere_interpolation* t, double x) {
+  return t && x <= MOLIERE_TABLE_UPPER ? moliere_interpolate(t->polynomials + 3 * C8MI_MOLIERE_INTERVALS, x)
+                                       : oracle_moliere_integral_f2(m, x);
+}
+
+void oracle_moliere_interpolated_functions(const c8mi_moliere_snapshot* m, const c8mi_moliere_interpolation* t,
+                                           double x, double out[4]) {
+  out[0] = mol_f1(m, t, x);
+  out[1] = mol_f2(m, t, x);
+  out[2] = mol_if1(m, t, x);
+  out[3] = mol_if2(m, t, x);
+}
+
+/* improve_moliere_initial_guess: the Gaussian coordinate plus the trilinear
+ * interpolation of the tabulated correction in (reference B, beta^2, |coordinate|). */
+static double moliere_improve_guess(const c8mi_moliere_interpolation* t, double reference_b, double beta2,
+                                    double gaussian) {
+  const double b_lo = 4.5, b_hi = 64., beta_lo = 0.25, beta_hi = 1., c_lo = 0., c_hi = 5.;
+  const uint32_t nb = C8MI_MOLIERE_GUESS_B_NODES, nt = C8MI_MOLIERE_GUESS_BETA2_NODES,
+                 nc = C8MI_MOLIERE_GUESS_COORDINATE_NODES;
+  const double inv_b = (double)(nb - 1) / (b_hi - b_lo), inv_beta = (double)(nt - 1) / (beta_hi - beta_lo),
+               inv_c = (double)(nc - 1) / (c_hi - c_lo);
+  if (!t || !finite(reference_b) || !finite(beta2) || !finite(gaussian)) return gaussian;
+  double mag = fabs(gaussian);
+  if (reference_b < b_lo || reference_b > b_hi || beta2 < beta_lo || beta2 > beta_hi || mag < c_lo || mag > c_hi)
+    return gaussian;
+  double bc = (reference_b - b_lo) * inv_b, cc = (mag - c_lo) * inv_c, tc = (beta2 - beta_lo) * inv_beta;
+  uint32_t bi = (uint32_t)bc, ti = (uint32_t)tc, ci = (uint32_t)cc;
+  if (bi + 1 >= nb) bi = nb - 2;
+  if (ti + 1 >= nt) ti = nt - 2;
+  if (ci + 1 >= nc) ci = nc - 2;
+  double bf = bc - (double)bi, tf = tc - (double)ti, cf = cc - (double)ci;
+  const double* g = t->initial_guess_delta;
+  size_t beta_stride = (size_t)nt * nc;
+  size_t lb = (size_t)bi * beta_stride, ub = lb + beta_stride;
+  size_t lt = (size_t)ti * nc + ci, ut = lt + nc;
+  double ll = g[lb + lt] * (1. - cf) + g[lb + lt + 1] * cf;
+  double lu = g[lb + ut] * (1. - cf) + g[lb + ut + 1] * cf;
+  double ul = g[ub + lt] * (1. - cf) + g[ub + lt + 1] * cf;
+  double uu = g[ub + ut] * (1. - cf) + g[ub + ut + 1] * cf;
+  double lower = ll * (1. - tf) + lu * tf;
+  double upper = ul * (1. - tf) + uu * tf;
+  double corrected = mag + lower * (1. - bf) + upper * bf;
+  if (!finite(corrected) || corrected < 0.) return gaussian;
+  return gaussian < 0. ? -corrected : corrected;
+}
+
 static int moliere_build(const c8mi_moliere_snapshot* m, double grammage, double e_mev, mol_state_t* st) {
   if (m->component_count == 0 || m->component_count > C8MI_MOLIERE_MAX_COMPONENTS) return MS_INVALID_SNAPSHOT;
   if (!moliere_valid(m)) return MS_INVALID_SNAPSHOT;
@@ -1851,32 +1927,34 @@ static int moliere_build(const c8mi_moliere_snapshot* m, double grammage, double
   return finite(st->prefactor) && st->prefactor > 0. ? MS_SUCCESS : MS_NON_FINITE;
 }
 
-static void moliere_evaluate(const c8mi_moliere_snapshot* m, const mol_state_t* st, double theta,
-                             double* density, double* cumulative) {
+static void moliere_evaluate(const c8mi_moliere_snapshot* m, const c8mi_moliere_interpolation* t,
+                             const mol_state_t* st, double theta, double* density, double* cumulative) {
   double d = 0., cdf = 0., t2 = theta * theta;
   for (uint32_t i = 0; i < m->component_count; ++i) {
     double x = t2 * st->inverse_scale[i], ib = st->inverse_b[i], ib2 = ib * ib;
-    d += st->density_weight[i] * (exp(-x) + oracle_moliere_f1(m, x) * ib + oracle_moliere_f2(m, x) * ib2);
+    d += st->density_weight[i] * (exp(-x) + mol_f1(m, t, x) * ib + mol_f2(m, t, x) * ib2);
     cdf += m->components[i].weight_zz *
            (0.5 * erf(sqrt(x > 0. ? x : 0.)) +
-            sqrt(1. / MOLIERE_PI) * (oracle_moliere_integral_f1(m, x) * ib + oracle_moliere_integral_f2(m, x) * ib2));
+            sqrt(1. / MOLIERE_PI) * (mol_if1(m, t, x) * ib + mol_if2(m, t, x) * ib2));
   }
   *density = d * m->inverse_weight_zz_sum;
   cdf *= m->inverse_weight_zz_sum;
   *cumulative = theta < 0. ? -cdf : cdf;
 }
 
-static int moliere_one(const c8mi_moliere_snapshot* m, const mol_state_t* st, double u, uint32_t* iterations,
-                       double* angle) {
+static int moliere_one(const c8mi_moliere_snapshot* m, const c8mi_moliere_interpolation* t,
+                       const mol_state_t* st, double u, uint32_t* iterations, double* angle) {
   *iterations = 0;
   *angle = 0.;
   if (!finite(u) || !(u > 0.) || !(u < 1.)) return MS_INVALID_INPUT;
   if (u == 0.5) return MS_SUCCESS;
-  double current = st->prefactor * (oracle_moliere_normal_ppf(u) / MOLIERE_SQRT2);
+  double gaussian = oracle_moliere_normal_ppf(u) / MOLIERE_SQRT2;
+  double reference_b = 1. / st->inverse_b[m->maximum_weight_index];
+  double current = st->prefactor * moliere_improve_guess(t, reference_b, st->beta_squared, gaussian);
   double target = u - 0.5;
   for (uint32_t it = 1; it <= 100; ++it) {
     double previous = current, density, cdf;
-    moliere_evaluate(m, st, previous, &density, &cdf);
+    moliere_evaluate(m, t, st, previous, &density, &cdf);
     *iterations = it;
     if (!finite(density) || density == 0.) {
       *angle = previous;
@@ -1898,13 +1976,15 @@ int oracle_moliere_evaluate(const c8mi_moliere_snapshot* m, double grammage, dou
   memset(&st, 0, sizeof st);
   int status = moliere_build(m, grammage, e_mev, &st);
   if (status != MS_SUCCESS) return status;
-  moliere_evaluate(m, &st, theta, density, cumulative);
+  moliere_evaluate(m, NULL, &st, theta, density, cumulative);
   return MS_SUCCESS;
 }
 
-/* Status, iteration count and the space angle of two projected draws. */
-int oracle_moliere_angle(const c8mi_moliere_snapshot* m, double grammage, double e_mev, double u1, double u2,
-                         uint32_t* iterations, double* angle) {
+/* Status, iteration count and the space angle of two projected draws; with a
+ * table (may be NULL) as the reference's sampler with an interpolation view. */
+int oracle_moliere_angle_interpolated(const c8mi_moliere_snapshot* m, const c8mi_moliere_interpolation* t,
+                                      double grammage, double e_mev, double u1, double u2, uint32_t* iterations,
+                                      double* angle) {
   mol_state_t st;
   memset(&st, 0, sizeof st);
   *iterations = 0;
@@ -1913,13 +1993,13 @@ int oracle_moliere_angle(const c8mi_moliere_snapshot* m, double grammage, double
   if (status != MS_SUCCESS) return status;
   uint32_t i1, i2;
   double a1, a2;
-  status = moliere_one(m, &st, u1, &i1, &a1);
+  status = moliere_one(m, t, &st, u1, &i1, &a1);
   if (status != MS_SUCCESS) {
     *iterations = i1;
     *angle = a1;
     return status;
   }
-  status = moliere_one(m, &st, u2, &i2, &a2);
+  status = moliere_one(m, t, &st, u2, &i2, &a2);
   if (status != MS_SUCCESS) {
     *iterations = i2;
     *angle = a2;
@@ -1928,6 +2008,11 @@ int oracle_moliere_angle(const c8mi_moliere_snapshot* m, double grammage, double
   *iterations = i1 + i2;
   *angle = sqrt(a1 * a1 + a2 * a2);
   return finite(*angle) ? MS_SUCCESS : MS_NON_FINITE;
+}
+
+int oracle_moliere_angle(const c8mi_moliere_snapshot* m, double grammage, double e_mev, double u1, double u2,
+                         uint32_t* iterations, double* angle) {
+  return oracle_moliere_angle_interpolated(m, NULL, grammage, e_mev, u1, u2, iterations, angle);
 }
 
 static int moliere_rotate(const double d[3], double angle, double au, double out[3]) {
@@ -2353,8 +2438,8 @@ static void lepton_source(shower_t* s, const ps_t* p, lout_t* o, uint64_t child_
     double ua = uniform(x->seed, x->shower, st, RP_SCATTERING, 2);
     uint32_t its;
     double angle;
-    int ms = oracle_moliere_angle(&cfg->electron_moliere, tr.traversed, st->energy_gev * 1000., u1, u2, &its,
-                                  &angle);
+    int ms = oracle_moliere_angle_interpolated(&cfg->electron_moliere, cfg->electron_moliere_interpolation,
+                                               tr.traversed, st->energy_gev * 1000., u1, u2, &its, &angle);
     if (ms != MS_NO_DEFLECTION) {
       double dir[3];
       if (ms != MS_SUCCESS ||

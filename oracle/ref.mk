@@ -8,7 +8,7 @@ CXXFLAGS ?= -O2 -fPIC -std=c++17 -ffp-contract=off -w
 
 all: _ref/libref.so
 
-_ref/libref.so: ref_probe.cpp ../include/c8mi/c8mi_tables.h
+_ref/libref.so: ref_probe.cpp ../include/c8mi/c8mi_tables.h ../include/c8mi/c8mi_em.h
 	@mkdir -p _ref
 	$(CXX) $(CXXFLAGS) -include cstring -I$(REFERENCE) -I../include -shared ref_probe.cpp -o $@
 

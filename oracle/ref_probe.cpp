@@ -10,6 +10,7 @@
 #include <corsika/kokkos/em/models/PhotonPairKinematics.hpp>
 #include <corsika/kokkos/em/models/PhotonPairLpm.hpp>
 #include <corsika/kokkos/em/random/Philox.hpp>
+#include <corsika/kokkos/em/tables/MoliereInterpolation.hpp>
 #include <corsika/kokkos/em/tables/ProposalNativeQueries.hpp>
 #include <corsika/kokkos/environment/EnvironmentBorrowedView.hpp>
 #include <corsika/kokkos/environment/EnvironmentQuery.hpp>
@@ -123,6 +124,26 @@ static ck::MoliereSnapshot as_ref(c8mi_moliere_snapshot const* s) {
   return r;
 }
 
+static_assert(sizeof(c8mi_moliere_interpolation) == sizeof(ck::MoliereInterpolationTable));
+static_assert(offsetof(c8mi_moliere_interpolation, initial_guess_delta) ==
+              offsetof(ck::MoliereInterpolationTable, initialGuessDelta_));
+static_assert(sizeof(c8mi_moliere_cubic) == sizeof(ck::MoliereCubicPolynomial));
+
+// The reference's view (make_moliere_interpolation_view) over a copy of a c8mi
+// table in the reference's own record, kept while the same table is probed.
+static ck::MoliereInterpolationView as_view(c8mi_moliere_interpolation const* t) {
+  static std::vector<unsigned char> storage(sizeof(ck::MoliereInterpolationTable));
+  static bool filled = false;
+  auto* table = reinterpret_cast<ck::MoliereInterpolationTable*>(storage.data());
+  if (!filled || std::memcmp(storage.data(), t, sizeof *t) != 0) {
+    std::memcpy(storage.data(), t, sizeof *t);
+    filled = true;
+  }
+  return ck::make_moliere_interpolation_view(
+      ck::NativeArrayView<ck::MoliereCubicPolynomial>{table->polynomials_},
+      ck::NativeArrayView<double>{table->initialGuessDelta_});
+}
+
 extern "C" {
 
 // f1, f2 and their integrals (analytical functions, no interpolation view).
@@ -142,6 +163,27 @@ int ref_moliere_angle(c8mi_moliere_snapshot const* s, double grammage, double e_
                       uint32_t* iterations, double* angle) {
   auto const r = as_ref(s);
   auto const a = ck::sample_moliere_scattering_angle_2d(r, grammage, e_mev, 0.9 * e_mev, u1, u2, true);
+  *iterations = a.iterations_;
+  *angle = a.angleRad_;
+  return static_cast<int>(a.status_);
+}
+
+// The same with the interpolation view of a table.
+void ref_moliere_interpolated_functions(c8mi_moliere_snapshot const* s, c8mi_moliere_interpolation const* t,
+                                        double x, double out[4]) {
+  auto const r = as_ref(s);
+  auto const v = as_view(t);
+  out[0] = ck::moliere_detail::f1_m(r, v, x);
+  out[1] = ck::moliere_detail::f2_m(r, v, x);
+  out[2] = ck::moliere_detail::integral_f1_m(r, v, x);
+  out[3] = ck::moliere_detail::integral_f2_m(r, v, x);
+}
+
+int ref_moliere_angle_interpolated(c8mi_moliere_snapshot const* s, c8mi_moliere_interpolation const* t,
+                                   double grammage, double e_mev, double u1, double u2, uint32_t* iterations,
+                                   double* angle) {
+  auto const r = as_ref(s);
+  auto const a = ck::sample_moliere_scattering_angle_2d(r, as_view(t), grammage, e_mev, 0.9 * e_mev, u1, u2, true);
   *iterations = a.iterations_;
   *angle = a.angleRad_;
   return static_cast<int>(a.status_);

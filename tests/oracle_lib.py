@@ -338,6 +338,26 @@ def moliere_functions(snapshot, x, reference=False):
     return tuple(vals)
 
 
+def moliere_table_functions(snapshot, table, x, reference=False):
+    """f1, f2 and their integrals at x through an interpolation table: oracle or
+    the reference header with a view over the same table."""
+    out = (c_double * 4)()
+    if reference:
+        load_ref().ref_moliere_interpolated_functions(byref(snapshot), byref(table), c_double(x), out)
+    else:
+        load().oracle_moliere_interpolated_functions(byref(snapshot), byref(table), c_double(x), out)
+    return tuple(out)
+
+
+def moliere_table_angle(snapshot, table, grammage, e_mev, u1, u2, reference=False):
+    it, angle = ctypes.c_uint32(), c_double()
+    fn = (load_ref().ref_moliere_angle_interpolated if reference
+          else load().oracle_moliere_angle_interpolated)
+    st = fn(byref(snapshot), byref(table) if table is not None else None, c_double(grammage), c_double(e_mev),
+            c_double(u1), c_double(u2), byref(it), byref(angle))
+    return st, it.value, angle.value
+
+
 def moliere_scalar(name, value, reference=False):
     lib = load_ref() if reference else load()
     fn = getattr(lib, ("ref_moliere_" if reference else "oracle_moliere_") + name)

@@ -35,10 +35,11 @@ def tables():
 
 
 def run_pair(tables, pid, energy, seed, shower_id=1, thinning=None, start=(0.0, 0.0, 112750.0),
-             direction=(0.0, 0.0, -1.0), primaries=None, moliere=False):
+             direction=(0.0, 0.0, -1.0), primaries=None, moliere=False, moliere_table=None, fast_iterations=0):
     atm = em.Atmosphere()
     cfg = em.make_config(airtables, seed=seed, atmosphere=atm, thinning=thinning, start=start,
-                         direction=direction, moliere=moliere)
+                         direction=direction, moliere=moliere, moliere_table=moliere_table,
+                         moliere_fast_iterations=fast_iterations)
     prim = primaries if primaries is not None else em.particle(pid, energy, start, direction, history_id=1)
     want = oracle_lib.em_run_shower(cfg, tables, atm, prim, shower_id=shower_id, first_history_id=2)
     dev = em.EmBackend(cfg, tables, atm)
@@ -88,7 +89,11 @@ def compare(cfg, want, got, energy):
     for f in ("energy_gev", "time_s"):
         np.testing.assert_allclose(go[f], wo[f], rtol=1e-9, atol=0)
     np.testing.assert_allclose(go["position_m"], wo["position_m"], rtol=1e-9, atol=1e-6)
-    assert abs(em.ledger_residual(gl)) <= closure_bound(cfg, gl, energy)
+    # A Molière-failed step's transport fallback carries the vertex state, so its
+    # continuous deposit is not booked (MoliereStep.inl:60-64): no closure then,
+    # but the device books exactly what the oracle does.
+    if gl["fallback_reasons"][21] == 0:
+        assert abs(em.ledger_residual(gl)) <= closure_bound(cfg, gl, energy)
     assert abs(em.ledger_residual(gl) - em.ledger_residual(wl)) <= 1e-9 * energy
 
 
@@ -115,12 +120,29 @@ def test_inclined_shower_reaches_ground_and_matches(tables):
     assert got["stats"]["observations"] > 0
 
 
-@pytest.mark.parametrize("pid,energy,seed,start", [(11, 5.0, 13, 1000.0), (22, 10.0, 14, 112750.0),
-                                                   (-11, 3.0, 15, 4000.0)])
-def test_moliere_shower_matches_oracle(tables, pid, energy, seed, start):
+@pytest.mark.parametrize("pid,energy,seed,start,fast", [(11, 5.0, 13, 1000.0, 0), (22, 10.0, 14, 112750.0, 0),
+                                                        (-11, 3.0, 15, 4000.0, 0), (11, 5.0, 13, 1000.0, 1),
+                                                        (22, 30.0, 17, 112750.0, 2)])
+def test_moliere_shower_matches_oracle(tables, pid, energy, seed, start, fast):
     """Molière multiple scattering on every finite e+- step (the reference's
-    production setting): device and oracle agree shower for shower."""
-    cfg, want, got = run_pair(tables, pid, energy, seed, start=(0.0, 0.0, start), moliere=True)
+    production setting): device and oracle agree shower for shower. With one
+    or two Newton iterations in the step kernel nearly every sample continues
+    on its own wavefront (the tail kernel), with the same results."""
+    cfg, want, got = run_pair(tables, pid, energy, seed, start=(0.0, 0.0, start), moliere=True,
+                              fast_iterations=fast)
+    compare(cfg, want, got, energy)
+    assert got["stats"]["lepton_steps"] > 100
+
+
+@pytest.mark.parametrize("pid,energy,seed,start,fast", [(11, 5.0, 13, 1000.0, 0), (22, 10.0, 14, 112750.0, 0),
+                                                        (-11, 3.0, 16, 4000.0, 0), (22, 30.0, 17, 112750.0, 0),
+                                                        (22, 30.0, 17, 112750.0, 1), (11, 5.0, 13, 1000.0, 2)])
+def test_moliere_table_shower_matches_oracle(tables, pid, energy, seed, start, fast):
+    """Molière scattering through the interpolation table and the tabulated
+    Newton initial guess (the reference backend's production path); `fast` 1
+    or 2 sends nearly every sample through the tail kernel."""
+    cfg, want, got = run_pair(tables, pid, energy, seed, start=(0.0, 0.0, start), moliere=True,
+                              moliere_table=True, fast_iterations=fast)
     compare(cfg, want, got, energy)
     assert got["stats"]["lepton_steps"] > 100
 
@@ -180,9 +202,10 @@ def pool_case(case):
     return pid, energy, shower, start, d
 
 
-def solo_oracle(tables, atm, case, seed=21):
+def solo_oracle(tables, atm, case, seed=21, moliere_table=None):
     pid, energy, shower, start, d = pool_case(case)
-    cfg = em.make_config(airtables, seed=seed, atmosphere=atm, start=start, direction=d)
+    cfg = em.make_config(airtables, seed=seed, atmosphere=atm, start=start, direction=d,
+                         moliere=moliere_table is not None, moliere_table=moliere_table)
     prim = em.particle(pid, energy, start, d, history_id=1)
     return cfg, oracle_lib.em_run_shower(cfg, tables, atm, prim, shower_id=shower, first_history_id=2)
 
@@ -293,26 +316,29 @@ def serve_requests(atm):
     return reqs
 
 
-@pytest.mark.parametrize("slots,budget", [(3, 250.0), (1, 1e9)])
-def test_serving_loop_results_equal_solo_runs(tables, slots, budget):
+@pytest.mark.parametrize("slots,budget,moliere_table", [(3, 250.0, None), (1, 1e9, None), (4, 250.0, True),
+                                                        (1, 1e9, True)])
+def test_serving_loop_results_equal_solo_runs(tables, slots, budget, moliere_table):
     """The native serving loop (footprint-limited admission into a pool, or the
     one-at-a-time form with one slot): every request's statistics and ledger
-    equal the oracle's run of that shower alone."""
+    equal the oracle's run of that shower alone, with and without Molière
+    scattering (through the interpolation table)."""
     atm = em.Atmosphere()
     cfg = em.make_config(airtables, seed=21, atmosphere=atm, slots=slots, observation_capacity=1 << 14,
-                         fallback_capacity=1 << 10)
+                         fallback_capacity=1 << 10, moliere=moliere_table is not None, moliere_table=moliere_table)
     dev = em.EmBackend(cfg, tables, atm)
     results, totals = dev.serve(serve_requests(atm), budget, realtime=False)
     dev.close()
     steps = 0
     for case, res in zip(POOL_CASES, results):
-        ocfg, want = solo_oracle(tables, atm, case)
+        ocfg, want = solo_oracle(tables, atm, case, moliere_table=moliere_table)
         for k in ("photon_steps", "lepton_steps", "photon_interactions", "lepton_interactions", "secondaries",
                   "observations", "fallbacks", "next_history_id"):
             assert res[k] == want["stats"][k], (case, k)
         wl = em.decode_counters(want["counters"], ocfg.struct.profile_energy_scale)
         for k in em.LEDGER_ENERGY_TERMS:
             assert abs(res["ledger"][k] - wl[k]) <= 1e-9 * case[1], (case, k)
+        assert res["ledger"]["fallback_reasons"] == wl["fallback_reasons"], case
         assert res["finished_s"] >= res["admitted_s"] >= 0.0
         steps += res["photon_steps"] + res["lepton_steps"]
     assert totals["photon_steps"] + totals["lepton_steps"] == steps

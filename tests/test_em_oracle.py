@@ -20,6 +20,7 @@ import math
 import numpy as np
 import pytest
 
+import c8mi
 import oracle_lib
 from c8mi import airtables, em
 
@@ -379,3 +380,132 @@ def test_oracle_step_tracks_follow_the_steps(tables, atmosphere, pid, sign):
     speed = length[moving] / (tracks[7, moving] - tracks[6, moving]) / 299792458.0
     assert speed.min() > 0.9 and speed.max() <= 1.0 + 1e-12
     assert set(np.unique(tracks[8])) == {-1.0, 1.0}
+
+
+# ---- Molière interpolation table (the reference backend's production path) ----
+
+@pytest.fixture(scope="module")
+def moliere_table(moliere):
+    return em.MoliereInterpolation.build(moliere)
+
+
+@needs_ref
+def test_moliere_table_queries_match_reference(moliere, moliere_table):
+    """With the same table, f1, f2, their integrals and sampled angles are the
+    reference's own (its MoliereScattering.hpp with a view over the table):
+    interval selection, clamping at 0 and 20, the analytical branches above 20,
+    the tabulated initial guess and the Newton iteration counts."""
+    xs = [0.0, 1e-300, 0.2020202020202, 10.0, 19.999999, 20.0, np.nextafter(20.0, 21.0), 25.0, 1e3]
+    xs += list(np.random.default_rng(11).uniform(0.0, 30.0, 400))
+    xs += [k * 20.0 / 99.0 for k in range(100)]
+    for x in xs:
+        a = oracle_lib.moliere_table_functions(moliere, moliere_table, x)
+        b = oracle_lib.moliere_table_functions(moliere, moliere_table, x, reference=True)
+        assert a == b, (x, a, b)
+    rng = np.random.default_rng(12)
+    cases = [(0.0, 100.0, 0.3, 0.7), (1.0, 10.0, 0.5, 0.2), (-1.0, 10.0, 0.3, 0.3), (1.0, 0.51, 0.3, 0.3)]
+    for _ in range(3000):
+        cases.append((10 ** rng.uniform(-8, 2), 10 ** rng.uniform(-0.25, 6), rng.random(), rng.random()))
+    statuses = set()
+    table_iterations = analytic_iterations = 0
+    for g, e, u1, u2 in cases:
+        a = oracle_lib.moliere_table_angle(moliere, moliere_table, g, e, u1, u2)
+        b = oracle_lib.moliere_table_angle(moliere, moliere_table, g, e, u1, u2, reference=True)
+        assert a == b, (g, e, u1, u2, a, b)
+        statuses.add(a[0])
+        table_iterations += a[1]
+        analytic_iterations += oracle_lib.moliere_angle(moliere, g, e, u1, u2)[1]
+    assert {0, 1, 3}.issubset(statuses)
+    # the tabulated initial guess saves Newton iterations
+    assert table_iterations < 0.8 * analytic_iterations
+
+
+def test_moliere_table_is_a_natural_spline_of_the_functions(moliere, moliere_table):
+    """The builder's cubics interpolate the analytical functions at the 100
+    nodes (exactly: c0 is the function value), join with continuous value,
+    slope and curvature, have zero curvature at both ends (natural spline),
+    and follow the functions between the nodes away from the series' branch
+    points and the sqrt(x) start of the integrals."""
+    p = moliere_table.polynomial_array()  # [function][interval][power]
+    h = 20.0 / 99.0
+    for f in range(4):
+        for k in range(99):
+            assert p[f, k, 0] == oracle_lib.moliere_functions(moliere, k * h)[f]
+        end = p[f, :, 0] + p[f, :, 1] + p[f, :, 2] + p[f, :, 3]
+        np.testing.assert_allclose(end[:-1], p[f, 1:, 0], rtol=1e-11, atol=1e-13)
+        assert end[-1] == pytest.approx(oracle_lib.moliere_functions(moliere, 20.0)[f], rel=1e-11, abs=1e-13)
+        slope_end = p[f, :, 1] + 2 * p[f, :, 2] + 3 * p[f, :, 3]
+        np.testing.assert_allclose(slope_end[:-1], p[f, 1:, 1], rtol=1e-9, atol=1e-11)
+        curvature_end = 2 * p[f, :, 2] + 6 * p[f, :, 3]
+        np.testing.assert_allclose(curvature_end[:-1], 2 * p[f, 1:, 2], rtol=1e-8, atol=1e-10)
+        assert abs(p[f, 0, 2]) < 1e-12 and abs(curvature_end[-1]) < 1e-10
+    # Between nodes: within 5e-6 from x = 2 on (measured <= 3.2e-6), outside the
+    # jumps of the analytical branches at x = 12 (f1) and 4.25^2 (f2). Below
+    # x = 2 the free (natural) end and the sqrt(x) start of the integrals allow
+    # up to 0.04 in the integral of f2 near x = 0, the same interpolation the
+    # reference tabulates on this grid.
+    xs = np.concatenate([np.linspace(2.0, 11.5, 191), np.linspace(12.5, 17.5, 101), np.linspace(18.6, 20.0, 29)])
+    for x in xs:
+        got = oracle_lib.moliere_table_functions(moliere, moliere_table, x)
+        want = oracle_lib.moliere_functions(moliere, x)
+        assert np.allclose(got, want, rtol=0, atol=5e-6), (x, got, want)
+
+
+def test_moliere_initial_guess_inverts_the_mixture_cdf(moliere, moliere_table):
+    """Each correction moves the Gaussian quantile coordinate onto the inverse
+    of the tabulated multi-component CDF (build_initial_guess_table), to the
+    bisection's resolution: checked at random nodes of the (B, beta^2,
+    coordinate) grid against the mixture CDF recomputed here."""
+    g = moliere_table.guess_array()
+    comps = [(moliere.components[i].weight_zz, moliere.components[i].chi0_squared_mev2,
+              moliere.components[i].coulomb_correction) for i in range(moliere.component_count)]
+    ref = comps[moliere.maximum_weight_index]
+    rng = np.random.default_rng(13)
+    checked = 0
+    for _ in range(600):
+        bi, ti, ci = int(rng.integers(64)), int(rng.integers(8)), int(rng.integers(1, 128))
+        delta = g[bi, ti, ci]
+        if abs(delta) > 3.0:
+            continue
+        b_ref = 4.5 + bi * (64.0 - 4.5) / 63.0
+        beta2 = 0.25 + ti * 0.75 / 7.0
+        gauss = ci * 5.0 / 127.0
+        ref_screen = ref[1] * (1.13 + ref[2] / beta2)
+        bs = [oracle_lib.moliere_scalar("solve_b", b_ref - math.log(b_ref) +
+                                        math.log(ref_screen / (c[1] * (1.13 + c[2] / beta2)))) for c in comps]
+        if min(bs) <= 0.0:  # a component without a solution: the Gaussian guess stays
+            assert delta == 0.0
+            continue
+
+        def cdf(coord):
+            total = 0.0
+            for (w, _, _), b in zip(comps, bs):
+                x = coord * coord * b_ref / b
+                _, _, i1, i2 = oracle_lib.moliere_table_functions(moliere, moliere_table, x)
+                total += w * (0.5 * math.erf(math.sqrt(x)) + math.sqrt(1 / math.pi) * (i1 / b + i2 / b ** 2))
+            return total * moliere.inverse_weight_zz_sum
+
+        target = 0.5 * math.erf(gauss)
+        eps = 1e-6 * max(1.0, gauss + delta)
+        assert cdf(gauss + delta - eps) <= target <= cdf(gauss + delta + eps), (bi, ti, ci, delta)
+        checked += 1
+    assert checked > 200
+
+
+def test_moliere_cache_file_round_trip(tmp_path, moliere, moliere_table):
+    """The reference's C8MOLI01 cache format: written and read back unchanged;
+    a corrupted table or another snapshot is refused."""
+    path = tmp_path / "moliere.cache"
+    moliere_table.write(path, moliere)
+    back = em.MoliereInterpolation.read(path, moliere)
+    assert bytes(back) == bytes(moliere_table)
+    raw = bytearray(path.read_bytes())
+    raw[-8] ^= 1
+    bad = tmp_path / "bad.cache"
+    bad.write_bytes(bytes(raw))
+    with pytest.raises(c8mi.C8miError):
+        em.MoliereInterpolation.read(bad, moliere)
+    other = em.MoliereSnapshot.from_dict(airtables.moliere_snapshot())
+    other.c1[3] *= 1.0 + 1e-15
+    with pytest.raises(c8mi.C8miError):
+        em.MoliereInterpolation.read(path, other)
