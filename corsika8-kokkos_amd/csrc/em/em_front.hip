@@ -274,9 +274,9 @@ __global__ __launch_bounds__(kStepBlock) void lepton_moliere_kernel(EmContext co
   }
 }
 
-// The open samples, one wavefront each (grid-stride over the list).
+// The open samples, one workgroup each (grid-stride over the list).
 template <bool Table>
-__global__ __launch_bounds__(moliere::kTailWave) void lepton_moliere_tail_kernel(
+__global__ __launch_bounds__(moliere::kTailThreads) void lepton_moliere_tail_kernel(
     EmContext const* __restrict__ context, FrontArgs f) {
   __shared__ double jobs[moliere::kDeviceComponents * moliere::kJobSlots];
   unsigned long long const n = *f.moliere_open_count;
@@ -663,13 +663,13 @@ void launch_lepton_moliere(EmContext const* context, bool table, FrontArgs const
   else
     lepton_moliere_kernel<false><<<grid_for(f.count, kStepBlock), kStepBlock, 0, stream>>>(context, f);
   C8MI_HIP(hipGetLastError());
-  // the open samples are rare (~1e-5 of the steps): a fixed grid of single
-  // wavefronts that leave at once when there are none
+  // the open samples are rare (~1e-5 of the steps): a fixed grid of small
+  // workgroups that leave at once when there are none
   constexpr unsigned kTailGrid = 256;
   if (table)
-    lepton_moliere_tail_kernel<true><<<kTailGrid, moliere::kTailWave, 0, stream>>>(context, f);
+    lepton_moliere_tail_kernel<true><<<kTailGrid, moliere::kTailThreads, 0, stream>>>(context, f);
   else
-    lepton_moliere_tail_kernel<false><<<kTailGrid, moliere::kTailWave, 0, stream>>>(context, f);
+    lepton_moliere_tail_kernel<false><<<kTailGrid, moliere::kTailThreads, 0, stream>>>(context, f);
   C8MI_HIP(hipGetLastError());
 }
 

@@ -372,11 +372,13 @@ __device__ __forceinline__ int one_dimensional(c8mi_moliere_snapshot const& m, Q
 // (EmContext::moliere_fast_iterations); a sample still open continues here
 // with its iterate: each evaluation spreads
 // the exp, erf, large-x functions and expansion terms of every component over
-// the 64 lanes (LDS `jobs`), then every lane sums them in the serial order, so
-// density, CDF and iterates are bit-identical to the serial evaluation.
+// a workgroup of kTailThreads (LDS `jobs`), then every thread sums them in the
+// serial order, so density, CDF and iterates are bit-identical to the serial
+// evaluation.
 
 inline constexpr int kJobSlots = 26;  // per component: exp, erf, f1, integral f1, 11 + 11 terms
-inline constexpr int kTailWave = 64;
+// threads per open sample: one job each for the 4 x 26 jobs of a 4-component medium
+inline constexpr int kTailThreads = 128;
 
 // Horner of a 70-coefficient power series (the order of each chain of evaluate).
 __device__ __forceinline__ double series70(double const* c, double x) {
@@ -393,7 +395,7 @@ __device__ __forceinline__ void evaluate_wave(c8mi_moliere_snapshot const& m, Qu
   int const nc = static_cast<int>(m.component_count);
   double const f1_from = Tab ? kTableUpper : 12.;
   __syncthreads();  // the previous evaluation has read `jobs`
-  for (int j = threadIdx.x; j < nc * kJobSlots; j += kTailWave) {
+  for (int j = threadIdx.x; j < nc * kJobSlots; j += kTailThreads) {
     int const k = j / kJobSlots, slot = j % kJobSlots;
     double scale = 0.;
 #pragma unroll
@@ -464,7 +466,7 @@ __device__ __forceinline__ void evaluate_wave(c8mi_moliere_snapshot const& m, Qu
   cumulative = theta < 0. ? -cdf : cdf;
 }
 
-// newton_iterate with the wavefront evaluation (every lane the same iterates).
+// newton_iterate with the workgroup evaluation (every thread the same iterates).
 template <bool Tab, int Cap>
 __device__ __forceinline__ int newton_iterate_wave(c8mi_moliere_snapshot const& m, Quotients const& q,
                                                    Table const& t, State<Cap> const& st, double u, int first,

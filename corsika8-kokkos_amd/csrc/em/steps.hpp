@@ -550,8 +550,9 @@ __device__ __forceinline__ int moliere_scatter(EmContext const& c, LeptonMove& m
   return moliere_apply(mv, a[0], a[1], d.ua);
 }
 
-// The rest of an open sample (and the second one after it) on one wavefront
-// of kTailWave lanes; every lane returns the same outcome and deflected move.
+// The rest of an open sample (and the second one after it) on one workgroup of
+// moliere::kTailThreads threads; every thread returns the same outcome and
+// deflected move.
 template <bool Table>
 __device__ __forceinline__ int moliere_scatter_wave(EmContext const& c, LeptonMove& mv, MoliereOpen const& open,
                                                     double* jobs) {

@@ -266,7 +266,7 @@ typedef struct c8mi_em_config {
   c8mi_moliere_snapshot electron_moliere;
   uint32_t moliere_enabled;
   /* Newton iterations per projected angle in the step kernel (0: 12); a sample
-   * still unconverged continues on a wavefront of its own. Results do not
+   * still unconverged continues on a workgroup of its own. Results do not
    * depend on it (launch tuning only). */
   uint32_t moliere_fast_iterations;
   /* The interpolation table of that snapshot (the reference backend always
