@@ -13,7 +13,7 @@ from ctypes import POINTER, byref, c_double, c_int, c_int32, c_uint32, c_uint64,
 
 import numpy as np
 
-from . import (PARTICLE_STATE_DTYPE, _check, library)
+from . import (PARTICLE_STATE_DTYPE, RadioCounters as _RadioCounters, _check, library)
 
 EARTH_RADIUS_M = 6371000.0
 C_LIGHT = 299792458.0
@@ -206,7 +206,9 @@ class _ParticleState(ctypes.Structure):
 class Request(ctypes.Structure):
     _fields_ = [("primary", _ParticleState), ("shower_id", c_uint64), ("first_history_id", c_uint64),
                 ("arrival_s", c_double), ("footprint_particles", c_double), ("axis", ShowerAxis),
-                ("align_", c_uint64)]  # the C struct is 16-byte aligned (240 bytes)
+                ("radio_observers", c_void_p), ("radio_observer_count", c_uint64),
+                ("radio_coreas", c_void_p), ("radio_zhs", c_void_p),
+                ("align_", c_uint64)]  # the C struct is 16-byte aligned
 
 
 class ServeOptions(ctypes.Structure):
@@ -250,11 +252,13 @@ class RequestResult(ctypes.Structure):
     _fields_ = [("admitted_s", c_double), ("finished_s", c_double)] + [(k, c_uint64) for k in (
         "photon_steps", "lepton_steps", "photon_interactions", "lepton_interactions", "secondaries",
         "next_history_id", "observations", "fallbacks")] + [("slot", c_uint32), ("pad_", c_uint32),
-                                                             ("ledger", Ledger)]
+                                                             ("ledger", Ledger)] + [
+        ("radio_counters", _RadioCounters)]
 
     def as_dict(self):
-        out = {k: getattr(self, k) for k, _ in self._fields_ if k not in ("pad_", "ledger")}
+        out = {k: getattr(self, k) for k, _ in self._fields_ if k not in ("pad_", "ledger", "radio_counters")}
         out["ledger"] = self.ledger.as_dict()
+        out["radio_counters"] = self.radio_counters.as_dict()
         return out
 
 
@@ -650,10 +654,16 @@ class EmBackend:
 
     def serve(self, requests, footprint_budget, realtime=True):
         """requests: list of dicts with primary (particle), shower_id, arrival_s,
-        footprint, axis (ShowerAxis or None); returns (per-request dicts, totals)."""
+        footprint, axis (ShowerAxis or None) and, with per-shower radio, radio
+        observers (an OBSERVER_DTYPE array); returns (per-request dicts,
+        totals). A request with observers gets its raw fixed-point waveforms
+        as result["radio"] = (coreas [3][bins], zhs [3][bins])."""
+        from . import observers_array
+
         n = len(requests)
         arr = (Request * n)()
         keep = []
+        radio_out = {}
         for k, r in enumerate(requests):
             p = np.ascontiguousarray(r["primary"], dtype=PARTICLE_STATE_DTYPE).reshape(-1)[:1]
             ctypes.memmove(ctypes.addressof(arr[k].primary), p.ctypes.data, 128)
@@ -664,13 +674,72 @@ class EmBackend:
             if r.get("axis") is not None:
                 arr[k].axis = r["axis"]
                 keep.append(r["axis"])
+            if r.get("radio_observers") is not None:
+                obs = observers_array(r["radio_observers"])
+                bins = int(obs["number_of_bins"].sum())
+                coreas = np.zeros((3, bins), dtype=np.int64)
+                zhs = np.zeros((3, bins), dtype=np.int64)
+                keep += [obs, coreas, zhs]
+                arr[k].radio_observers = obs.ctypes.data
+                arr[k].radio_observer_count = obs.size
+                arr[k].radio_coreas = coreas.ctypes.data
+                arr[k].radio_zhs = zhs.ctypes.data
+                radio_out[k] = (coreas, zhs)
         opts = ServeOptions()
         opts.footprint_budget_particles = footprint_budget
         opts.realtime = 1 if realtime else 0
         results = (RequestResult * n)()
         totals = RunResult()
         _check(library().c8mi_em_serve(self.handle, arr, c_uint64(n), byref(opts), results, byref(totals)))
-        return [results[k].as_dict() for k in range(n)], totals.as_dict()
+        out = [results[k].as_dict() for k in range(n)]
+        for k, waves in radio_out.items():
+            out[k]["radio"] = waves
+        return out, totals.as_dict()
+
+    def enable_slot_radio(self, observers_per_slot, bins_per_observer, propagation, coreas=True, zhs=True,
+                          exact=False, fixed_point_limit=1.0, zhs_refinement=1):
+        """Per-shower radio of the pool (c8mi_em_enable_slot_radio)."""
+        from . import RadioConfig
+
+        cfg = RadioConfig()
+        cfg.coreas_enabled = 1 if coreas else 0
+        cfg.zhs_enabled = 1 if zhs else 0
+        cfg.deterministic = 1
+        cfg.zhs_subtrack_refinement = zhs_refinement
+        cfg.exact_fp64 = 1 if exact else 0
+        cfg.fixed_point_field_limit_v_per_m = fixed_point_limit
+        cfg.propagation = propagation.struct
+        _check(library().c8mi_em_enable_slot_radio(self.handle, byref(cfg), c_uint32(observers_per_slot),
+                                                   c_uint32(bins_per_observer)))
+        self._slot_radio_propagation = propagation
+
+    def slot_set_radio(self, slot, observers):
+        from . import observers_array
+
+        obs = observers_array(observers) if observers is not None else None
+        _check(library().c8mi_em_slot_set_radio(self.handle, c_uint32(slot),
+                                                obs.ctypes.data_as(c_void_p) if obs is not None else None,
+                                                c_uint64(obs.size if obs is not None else 0)))
+        if not hasattr(self, "_slot_bins"):
+            self._slot_bins = {}
+        self._slot_bins[slot] = int(obs["number_of_bins"].sum()) if obs is not None else 0
+
+    def slot_download_radio(self, slot):
+        """(coreas [3][bins], zhs [3][bins], counters dict) of a slot."""
+        from . import RadioCounters
+
+        bins = self._slot_bins.get(slot, 0)
+        coreas = np.zeros((3, bins), dtype=np.int64)
+        zhs = np.zeros((3, bins), dtype=np.int64)
+        counters = RadioCounters()
+        _check(library().c8mi_em_slot_download_radio(self.handle, c_uint32(slot), coreas.ctypes.data_as(c_void_p),
+                                                     zhs.ctypes.data_as(c_void_p), byref(counters)))
+        return coreas, zhs, counters.as_dict()
+
+    def slot_radio_kernel_time(self):
+        ms, n = c_double(), c_uint64()
+        _check(library().c8mi_em_slot_radio_kernel_time(self.handle, byref(ms), byref(n)))
+        return ms.value, n.value
 
     def slot_end(self, slot):
         _check(library().c8mi_em_slot_end(self.handle, c_uint32(slot)))

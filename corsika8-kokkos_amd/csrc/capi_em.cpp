@@ -304,6 +304,38 @@ int c8mi_em_attach_radio(c8mi_em* em, c8mi_radio* radio) {
   });
 }
 
+int c8mi_em_enable_slot_radio(c8mi_em* em, c8mi_radio_config const* config, uint32_t observers_per_slot,
+                              uint32_t bins_per_observer) {
+  return guarded([&] {
+    require_handle(em);
+    if (!config) throw std::invalid_argument("null radio configuration");
+    em->backend.enable_slot_radio(*config, observers_per_slot, bins_per_observer);
+  });
+}
+
+int c8mi_em_slot_set_radio(c8mi_em* em, uint32_t slot, c8mi_radio_observer const* observers, uint64_t count) {
+  return guarded([&] {
+    require_handle(em);
+    em->backend.slot_set_radio(slot, observers, count);
+  });
+}
+
+int c8mi_em_slot_download_radio(c8mi_em* em, uint32_t slot, int64_t* coreas, int64_t* zhs,
+                                c8mi_radio_counters* counters) {
+  return guarded([&] {
+    require_handle(em);
+    em->backend.slot_download_radio(slot, coreas, zhs, counters);
+  });
+}
+
+int c8mi_em_slot_radio_kernel_time(c8mi_em* em, double* ms, uint64_t* launches) {
+  return guarded([&] {
+    require_handle(em);
+    if (!ms || !launches) throw std::invalid_argument("null argument");
+    em->backend.slot_radio_time(*ms, *launches);
+  });
+}
+
 int c8mi_em_enable_kernel_timing(c8mi_em* em, int enable) {
   return guarded([&] {
     require_handle(em);

@@ -291,9 +291,12 @@ EmBackend::~EmBackend() {
   if (stream_) (void)hipStreamSynchronize(stream_);
   for (int b = 0; b < 2; ++b) {
     if (track_busy_[b]) (void)hipEventSynchronize(track_done_[b]);
+    if (slot_busy_[b]) (void)hipEventSynchronize(slot_done_[b]);
     if (track_ready_[b]) (void)hipEventDestroy(track_ready_[b]);
     if (track_done_[b]) (void)hipEventDestroy(track_done_[b]);
+    if (slot_done_[b]) (void)hipEventDestroy(slot_done_[b]);
   }
+  slot_radio_.reset();
   for (auto& e : events_)
     if (e) (void)hipEventDestroy(e);
   if (init_copied_) (void)hipEventDestroy(init_copied_);
@@ -339,6 +342,7 @@ void EmBackend::slot_begin(std::uint32_t slot, std::uint64_t shower_id, std::uin
   SlotHost& h = slot_host_[slot];
   if (h.pending[0] || h.pending[1] || h.staged[0] || h.staged[1])
     throw std::logic_error("em: slot " + std::to_string(slot) + " still has pending particles");
+  if (slot_radio_) slot_radio_->set_slot(slot, nullptr, 0);  // no radio until the shower's observers are set
   SlotInit init{};
   ShowerSlot& s = init.shower;
   if (axis) {
@@ -475,21 +479,66 @@ void EmBackend::resize_scratch(std::uint64_t n) {
 }
 
 void EmBackend::wait_radio_tracks() {
-  for (int b = 0; b < 2; ++b)
+  for (int b = 0; b < 2; ++b) {
     if (track_busy_[b]) {
       C8MI_HIP(hipEventSynchronize(track_done_[b]));
       track_busy_[b] = false;
     }
+    if (slot_busy_[b]) {
+      C8MI_HIP(hipEventSynchronize(slot_done_[b]));
+      slot_busy_[b] = false;
+    }
+  }
+}
+
+void EmBackend::create_track_events() {
+  if (track_ready_[0]) return;
+  for (int b = 0; b < 2; ++b) {
+    C8MI_HIP(hipEventCreateWithFlags(&track_ready_[b], hipEventDisableTiming));
+    C8MI_HIP(hipEventCreateWithFlags(&track_done_[b], hipEventDisableTiming));
+    C8MI_HIP(hipEventCreateWithFlags(&slot_done_[b], hipEventDisableTiming));
+  }
 }
 
 void EmBackend::attach_radio(radio::RadioAccumulator* radio) {
   wait_radio_tracks();
-  if (radio && !track_ready_[0])
-    for (int b = 0; b < 2; ++b) {
-      C8MI_HIP(hipEventCreateWithFlags(&track_ready_[b], hipEventDisableTiming));
-      C8MI_HIP(hipEventCreateWithFlags(&track_done_[b], hipEventDisableTiming));
-    }
+  if (radio) create_track_events();
   radio_ = radio;
+}
+
+void EmBackend::enable_slot_radio(c8mi_radio_config const& config, std::uint32_t observers_per_slot,
+                                  std::uint32_t bins_per_observer) {
+  wait_radio_tracks();
+  slot_radio_.reset();
+  slot_radio_ = std::make_unique<radio::SlotRadio>(config, slots_, observers_per_slot, bins_per_observer, nullptr);
+  create_track_events();
+  std::size_t const keys = static_cast<std::size_t>(slots_) * kMaxRuns;
+  for (int b = 0; b < 2; ++b) {
+    segment_keys_[b].allocate(2 * keys);
+    C8MI_HIP(hipMemset(segment_keys_[b].data(), 0, segment_keys_[b].bytes()));
+    segments_[b].allocate(keys);
+    segment_count_[b].allocate(1);
+    item_count_[b].allocate(1);
+  }
+}
+
+void EmBackend::slot_set_radio(std::uint32_t slot, c8mi_radio_observer const* observers, std::uint64_t count) {
+  check_slot(slot);
+  if (!slot_radio_) throw std::logic_error("em: per-shower radio is not enabled");
+  slot_radio_->set_slot(slot, observers, count);
+}
+
+void EmBackend::slot_download_radio(std::uint32_t slot, std::int64_t* coreas, std::int64_t* zhs,
+                                    c8mi_radio_counters* counters) {
+  check_slot(slot);
+  if (!slot_radio_) throw std::logic_error("em: per-shower radio is not enabled");
+  slot_radio_->download(slot, coreas, zhs, counters);
+}
+
+void EmBackend::slot_radio_time(double& ms, std::uint64_t& launches) {
+  ms = 0.;
+  launches = 0;
+  if (slot_radio_) slot_radio_->kernel_time_ms(ms, launches);
 }
 
 void EmBackend::front(bool photon, c8mi_em_run_result& out) {
@@ -530,27 +579,45 @@ void EmBackend::front(bool photon, c8mi_em_run_result& out) {
   f.moliere_open = moliere_open_.data();
   f.moliere_open_count = moliere_open_count_.data();
   int const track_buffer = track_parity_;
-  if (radio_ && !photon) {
-    // tracks [9][n]; the buffer's previous projection must be done first
+  if ((radio_ || slot_radio_) && !photon) {
+    // tracks [9][n]; the buffer's previous projections must be done first
     std::uint64_t& capacity = track_capacity_[track_buffer];
     if (capacity < n) {
       if (track_busy_[track_buffer]) C8MI_HIP(hipEventSynchronize(track_done_[track_buffer]));
-      track_busy_[track_buffer] = false;
+      if (slot_busy_[track_buffer]) C8MI_HIP(hipEventSynchronize(slot_done_[track_buffer]));
+      track_busy_[track_buffer] = slot_busy_[track_buffer] = false;
       capacity = std::max<std::uint64_t>(n, 2 * capacity);
       tracks_[track_buffer].allocate(C8MI_RADIO_TRACK_FIELDS * capacity);
     }
     if (track_busy_[track_buffer]) C8MI_HIP(hipStreamWaitEvent(stream_, track_done_[track_buffer], 0));
+    if (slot_busy_[track_buffer]) C8MI_HIP(hipStreamWaitEvent(stream_, slot_done_[track_buffer], 0));
     f.tracks = tracks_[track_buffer].data();
     f.track_stride = n;
   }
-  // Projects this front's tracks on the accumulator's stream once the step kernel is done.
+  // Projects this front's tracks on the accumulators' streams once the step kernel is done.
   auto project_tracks = [&] {
     if (!f.tracks) return;
+    if (slot_radio_) {
+      std::size_t const keys = static_cast<std::size_t>(slots_) * kMaxRuns;
+      unsigned long long* key_begin = segment_keys_[track_buffer].data();
+      launch_radio_segments(f, active_range_, key_begin, key_begin + keys, slot_radio_->observer_counts(),
+                            slot_radio_->chunk_size(), slot_radio_->bin_tiles(), segments_[track_buffer].data(),
+                            segment_count_[track_buffer].data(), item_count_[track_buffer].data(), stream_);
+    }
     C8MI_HIP(hipEventRecord(track_ready_[track_buffer], stream_));
-    C8MI_HIP(hipStreamWaitEvent(radio_->stream(), track_ready_[track_buffer], 0));
-    radio_->accumulate_tracks(f.tracks, n, n);
-    C8MI_HIP(hipEventRecord(track_done_[track_buffer], radio_->stream()));
-    track_busy_[track_buffer] = true;
+    if (radio_) {
+      C8MI_HIP(hipStreamWaitEvent(radio_->stream(), track_ready_[track_buffer], 0));
+      radio_->accumulate_tracks(f.tracks, n, n);
+      C8MI_HIP(hipEventRecord(track_done_[track_buffer], radio_->stream()));
+      track_busy_[track_buffer] = true;
+    }
+    if (slot_radio_) {
+      C8MI_HIP(hipStreamWaitEvent(slot_radio_->stream(), track_ready_[track_buffer], 0));
+      slot_radio_->project(f.tracks, n, n, segments_[track_buffer].data(), segment_count_[track_buffer].data(),
+                           item_count_[track_buffer].data(), active_range_ * kMaxRuns);
+      C8MI_HIP(hipEventRecord(slot_done_[track_buffer], slot_radio_->stream()));
+      slot_busy_[track_buffer] = true;
+    }
     track_parity_ ^= 1;
   };
   // the launches of this front, each followed by an event when timing
@@ -800,6 +867,8 @@ void EmBackend::serve(c8mi_em_request const* requests, std::uint64_t count, c8mi
       validate_axis(r.axis);
       axis_words += r.axis.support_count;
     }
+    if (r.radio_observer_count && !slot_radio_)
+      throw std::invalid_argument("em: request " + std::to_string(k) + " has radio observers, per-shower radio is off");
     std::memset(&results[k], 0, sizeof results[k]);
   }
   for (std::uint32_t slot = 0; slot < slots_; ++slot) slot_host_[slot].active = false;
@@ -855,6 +924,7 @@ void EmBackend::serve(c8mi_em_request const* requests, std::uint64_t count, c8mi
       init.slot = slot;
       inits.push_back(init);
       activate(slot, r.shower_id);
+      if (slot_radio_) slot_radio_->set_slot(slot, r.radio_observers, r.radio_observer_count);
       slot_enqueue(slot, &r.primary, 1);
       owner[slot] = static_cast<std::int64_t>(next);
       results[next].admitted_s = now;
@@ -916,6 +986,9 @@ void EmBackend::serve(c8mi_em_request const* requests, std::uint64_t count, c8mi
       long long c[kCounters];
       for (int j = 0; j < kCounters; ++j) c[j] = static_cast<long long>(g[kGatherCounters + j]);
       decode_ledger(c, slot, out.ledger);
+      c8mi_em_request const& req = requests[owner[slot]];
+      if (slot_radio_ && req.radio_observer_count)
+        slot_radio_->download(slot, req.radio_coreas, req.radio_zhs, &out.radio_counters);
       totals.observations += out.observations;
       totals.fallbacks += out.fallbacks;
       slot_host_[slot].active = false;
@@ -930,6 +1003,7 @@ void EmBackend::serve(c8mi_em_request const* requests, std::uint64_t count, c8mi
 }
 
 void EmBackend::enable_timing(bool enable) {
+  if (slot_radio_) slot_radio_->enable_kernel_timing(enable);
   timing_ = enable;
   ms_.fill(0.);
   launches_.fill(0);

@@ -21,6 +21,7 @@
 #include "em/em_device.hpp"
 #include "em/em_front.hpp"
 #include "radio/radio_accumulator.hpp"
+#include "radio/slot_radio.hpp"
 #include "runtime.hpp"
 
 namespace c8mi::em {
@@ -87,6 +88,16 @@ class EmBackend {
   // attachment.
   void attach_radio(radio::RadioAccumulator* radio);
 
+  // Per-shower radio of the pool (IEmBackend's radio per shower, here for
+  // every slot at once): each slot's e+- steps are projected onto that slot's
+  // own observers, waveforms and counters, resident on the device.
+  void enable_slot_radio(c8mi_radio_config const& config, std::uint32_t observers_per_slot,
+                         std::uint32_t bins_per_observer);
+  void slot_set_radio(std::uint32_t slot, c8mi_radio_observer const* observers, std::uint64_t count);
+  void slot_download_radio(std::uint32_t slot, std::int64_t* coreas, std::int64_t* zhs,
+                           c8mi_radio_counters* counters);
+  void slot_radio_time(double& ms, std::uint64_t& launches);
+
   void enable_timing(bool enable);
   void kernel_time(double ms[kFrontKernels], std::uint64_t launches[kFrontKernels]) const;
 
@@ -117,6 +128,7 @@ class EmBackend {
   void update_active_range();
   long long pending_energy(std::uint32_t slot);
   void wait_radio_tracks();
+  void create_track_events();
 
   c8mi_em_config config_;
   std::uint32_t slots_ = 1;
@@ -178,12 +190,20 @@ class EmBackend {
   // radio: tracks of the last two lepton fronts; `track_done_` marks the end
   // of their projection on the accumulator's stream
   radio::RadioAccumulator* radio_ = nullptr;
+  std::unique_ptr<radio::SlotRadio> slot_radio_;
   DeviceBuffer<double> tracks_[2];
   std::uint64_t track_capacity_[2] = {0, 0};
   hipEvent_t track_ready_[2] = {nullptr, nullptr};
-  hipEvent_t track_done_[2] = {nullptr, nullptr};
+  hipEvent_t track_done_[2] = {nullptr, nullptr};  // attached accumulator
+  hipEvent_t slot_done_[2] = {nullptr, nullptr};   // per-shower radio
   bool track_busy_[2] = {false, false};
+  bool slot_busy_[2] = {false, false};
   int track_parity_ = 0;
+  // per-shower radio: (slot, run) key arrays and the segment list of a front
+  DeviceBuffer<unsigned long long> segment_keys_[2];  // begin [slots * kMaxRuns], then end
+  DeviceBuffer<radio::SegmentDev> segments_[2];
+  DeviceBuffer<std::uint32_t> segment_count_[2];
+  DeviceBuffer<unsigned long long> item_count_[2];
   // timing
   bool timing_ = false;
   std::array<double, kFrontKernels> ms_{};

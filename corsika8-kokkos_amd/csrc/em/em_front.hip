@@ -585,6 +585,82 @@ __global__ __launch_bounds__(kStepBlock) void scatter_kernel(FrontArgs f, Partic
   }
 }
 
+// Per-shower radio: the (slot, run) segments of a lepton front's sources. A
+// run holds every slot at most once, contiguous, so each (slot, run) key has
+// one first and one last source. The arrays are zero between uses; `end` > 0
+// marks a present segment.
+__global__ __launch_bounds__(kStepBlock) void radio_segments_kernel(FrontArgs f, unsigned long long* begin,
+                                                                     unsigned long long* end) {
+  std::uint64_t const i = static_cast<std::uint64_t>(blockIdx.x) * kStepBlock + threadIdx.x;
+  if (i >= f.count) return;
+  std::uint32_t const s = f.in.shower[i], r = run_of(f, i);
+  std::uint64_t const key = static_cast<std::uint64_t>(s) * kMaxRuns + r;
+  if (i == 0 || f.in.shower[i - 1] != s || run_of(f, i - 1) != r) begin[key] = i;
+  if (i + 1 == f.count || f.in.shower[i + 1] != s || run_of(f, i + 1) != r) end[key] = i + 1;
+}
+
+// The segment list of the projection (one workgroup): every present segment
+// of a slot with observers, in key order, with the offset of its first item
+// (chunks x the slot's observers x window tiles); clears the key arrays.
+__global__ __launch_bounds__(kScanBlock) void radio_items_kernel(unsigned long long* begin, unsigned long long* end,
+                                                                  std::uint32_t keys,
+                                                                  std::uint32_t const* observer_counts,
+                                                                  std::uint64_t chunk, std::uint32_t tiles,
+                                                                  radio::SegmentDev* segments,
+                                                                  std::uint32_t* segment_count,
+                                                                  unsigned long long* item_count) {
+  __shared__ unsigned long long wave_sums[kScanBlock / 64][2];
+  __shared__ unsigned long long carry[2];
+  if (threadIdx.x < 2) carry[threadIdx.x] = 0;
+  __syncthreads();
+  std::uint32_t const wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+  for (std::uint32_t base = 0; base < keys; base += kScanBlock) {
+    std::uint32_t const e = base + threadIdx.x;
+    unsigned long long b = 0, t = 0, items = 0;
+    std::uint32_t slot = 0;
+    if (e < keys) {
+      t = end[e];
+      if (t) {
+        b = begin[e];
+        slot = e / kMaxRuns;
+        items = (t - b + chunk - 1) / chunk * observer_counts[slot] * tiles;
+        begin[e] = end[e] = 0;
+      }
+    }
+    unsigned long long const v[2] = {items, items ? 1ULL : 0ULL};
+    unsigned long long inc[2];
+    for (int k = 0; k < 2; ++k) inc[k] = wave_inclusive(v[k], lane);
+    if (lane == 63)
+      for (int k = 0; k < 2; ++k) wave_sums[wave][k] = inc[k];
+    __syncthreads();
+    unsigned long long before[2] = {0, 0}, all[2] = {0, 0};
+    for (std::uint32_t w = 0; w < kScanBlock / 64; ++w)
+      for (int k = 0; k < 2; ++k) {
+        before[k] += w < wave ? wave_sums[w][k] : 0ULL;
+        all[k] += wave_sums[w][k];
+      }
+    if (items) {
+      radio::SegmentDev d;
+      d.begin = b;
+      d.end = t;
+      d.item_begin = carry[0] + before[0] + inc[0] - v[0];
+      d.slot = slot;
+      d.pad = 0;
+      segments[carry[1] + before[1] + inc[1] - v[1]] = d;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      carry[0] += all[0];
+      carry[1] += all[1];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    *item_count = carry[0];
+    *segment_count = static_cast<std::uint32_t>(carry[1]);
+  }
+}
+
 __global__ __launch_bounds__(256) void unstage_kernel(StagedParticle const* __restrict__ staged, std::uint64_t n,
                                                       ParticleQueue queue, std::uint64_t at) {
   std::uint64_t const i = static_cast<std::uint64_t>(blockIdx.x) * 256 + threadIdx.x;
@@ -670,6 +746,17 @@ void launch_lepton_moliere(EmContext const* context, bool table, FrontArgs const
     lepton_moliere_tail_kernel<true><<<kTailGrid, moliere::kTailThreads, 0, stream>>>(context, f);
   else
     lepton_moliere_tail_kernel<false><<<kTailGrid, moliere::kTailThreads, 0, stream>>>(context, f);
+  C8MI_HIP(hipGetLastError());
+}
+
+void launch_radio_segments(FrontArgs const& f, std::uint32_t slots, unsigned long long* begin,
+                           unsigned long long* end, std::uint32_t const* observer_counts, std::uint64_t chunk,
+                           std::uint32_t tiles, radio::SegmentDev* segments, std::uint32_t* segment_count,
+                           unsigned long long* item_count, hipStream_t stream) {
+  if (f.count)
+    radio_segments_kernel<<<grid_for(f.count, kStepBlock), kStepBlock, 0, stream>>>(f, begin, end);
+  radio_items_kernel<<<1, kScanBlock, 0, stream>>>(begin, end, slots * kMaxRuns, observer_counts, chunk, tiles,
+                                                   segments, segment_count, item_count);
   C8MI_HIP(hipGetLastError());
 }
 

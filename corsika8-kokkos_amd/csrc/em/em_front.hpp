@@ -7,6 +7,7 @@
 #include <cstdint>
 
 #include "em/em_device.hpp"
+#include "radio/radio_device.hpp"
 
 namespace c8mi::em {
 
@@ -110,6 +111,13 @@ void launch_lepton_transport(EmContext const* context, std::uint32_t bins, Front
 std::size_t lepton_move_bytes();
 std::size_t moliere_open_bytes();
 void launch_lepton_move(EmContext const* context, std::uint32_t bins, FrontArgs const& f, hipStream_t stream);
+// Per-shower radio: the segment list of a lepton front's tracks for
+// radio::SlotRadio::project (key arrays [slots][kMaxRuns], zero between uses).
+void launch_radio_segments(FrontArgs const& f, std::uint32_t slots, unsigned long long* begin,
+                           unsigned long long* end, std::uint32_t const* observer_counts, std::uint64_t chunk,
+                           std::uint32_t tiles, radio::SegmentDev* segments, std::uint32_t* segment_count,
+                           unsigned long long* item_count, hipStream_t stream);
+
 // Molière scattering of the moved sources between the move and finish kernels
 // (the step kernel, then the tail kernel for samples still open);
 // table: the interpolation-view sampler (else the analytical functions).

@@ -9,98 +9,10 @@
 #include <string>
 
 #include "radio_launch.hpp"
+#include "radio_setup.hpp"
 
 namespace c8mi::radio {
 namespace {
-
-constexpr double kHeadroom = 4611686018427387904.;  // 2^62, RadioProjectionData.hpp:34
-
-// RadioAccumulator::validate (corsika/detail/kokkos/radio/RadioAccumulator.inl:934-975).
-void validate(c8mi_radio_config const& config) {
-  auto const& p = config.propagation;
-  if (!config.coreas_enabled && !config.zhs_enabled)
-    throw std::invalid_argument("enabled radio has no algorithm");
-  if (!std::isfinite(config.fixed_point_field_limit_v_per_m) ||
-      !(config.fixed_point_field_limit_v_per_m > 0.))
-    throw std::invalid_argument("invalid radio fixed-point limit");
-  if (config.zhs_subtrack_refinement < 1 || config.zhs_subtrack_refinement > 64)
-    throw std::invalid_argument("ZHS subtrack refinement must be in [1,64]");
-  if (!std::isfinite(p.homogeneous_refractive_index) || p.homogeneous_refractive_index < 0.)
-    throw std::invalid_argument("invalid homogeneous radio refractive index");
-  bool const homogeneous = p.homogeneous_refractive_index > 0.;
-  if (!homogeneous &&
-      (p.table_size < 11 || p.refractivity == nullptr || p.integrated_refractivity == nullptr ||
-       !std::isfinite(p.minimum_height_m) || !std::isfinite(p.maximum_height_m) ||
-       !(p.maximum_height_m > p.minimum_height_m) || !std::isfinite(p.step_m) ||
-       !(p.step_m > 0.) || !std::isfinite(p.inverse_step_per_m) ||
-       !(p.inverse_step_per_m > 0.)))
-    throw std::invalid_argument("invalid flat-atmosphere radio snapshot");
-  if (!homogeneous && p.table_size >= (std::uint64_t{1} << 32))
-    throw std::length_error("radio propagation table above 2^32 entries");
-  auto check = [&](c8mi_radio_observer const* observers, std::uint64_t count) {
-    if (count != 0 && observers == nullptr)
-      throw std::invalid_argument("null radio observer list");
-    for (std::uint64_t i = 0; i < count; ++i) {
-      auto const& o = observers[i];
-      if (!std::isfinite(o.start_time_s) || !std::isfinite(o.duration_s) ||
-          !(o.duration_s >= 0.) || !std::isfinite(o.sample_rate_hz) ||
-          !(o.sample_rate_hz > 0.) || o.number_of_bins == 0)
-        throw std::invalid_argument("invalid radio observer");
-      if (o.number_of_bins >= (std::uint64_t{1} << 31))
-        throw std::length_error("radio observer with 2^31 or more bins");
-      double const height = (o.position_m[2] - p.minimum_height_m) * p.inverse_step_per_m;
-      if (!homogeneous && (!std::isfinite(height) || height < 0. ||
-                           height + 0.5 >= static_cast<double>(p.table_size)))
-        throw std::invalid_argument("radio observer outside propagation table");
-      for (double coordinate : o.position_m)
-        if (!std::isfinite(coordinate))
-          throw std::invalid_argument("radio observer position is not finite");
-    }
-  };
-  check(config.coreas_observers, config.coreas_enabled ? config.coreas_observer_count : 0);
-  check(config.zhs_observers, config.zhs_enabled ? config.zhs_observer_count : 0);
-}
-
-// RadioAccumulator::makeObservers (RadioAccumulator.inl:591-617).
-std::vector<ObserverDev> make_observers(c8mi_radio_observer const* input, std::uint64_t count,
-                                        double limit, bool zhs_potential,
-                                        std::uint64_t& total_bins, std::uint64_t& max_bins) {
-  std::vector<ObserverDev> result;
-  result.reserve(count);
-  total_bins = 0;
-  max_bins = 0;
-  for (std::uint64_t i = 0; i < count; ++i) {
-    auto const& o = input[i];
-    ObserverDev d{};
-    for (int a = 0; a < 3; ++a) d.position[a] = o.position_m[a];
-    d.start_time = o.start_time_s;
-    d.duration = o.duration_s;
-    d.sample_rate = o.sample_rate_hz;
-    double const potential_factor = zhs_potential ? o.sample_rate_hz : 1.;
-    d.fixed_scale = kHeadroom * potential_factor / limit;
-    d.number_of_bins = o.number_of_bins;
-    d.waveform_offset = total_bins;
-    if (o.number_of_bins > std::numeric_limits<std::uint64_t>::max() - total_bins)
-      throw std::overflow_error("radio waveform bin overflow");
-    total_bins += o.number_of_bins;
-    max_bins = std::max<std::uint64_t>(max_bins, o.number_of_bins);
-    result.push_back(d);
-  }
-  return result;
-}
-
-// observersCanShareTrackKinematics (RadioAccumulator.inl:619-632).
-bool can_fuse(std::vector<ObserverDev> const& a, std::vector<ObserverDev> const& b) {
-  if (a.size() != b.size() || a.empty()) return false;
-  for (std::size_t i = 0; i < a.size(); ++i) {
-    if (a[i].start_time != b[i].start_time || a[i].duration != b[i].duration ||
-        a[i].sample_rate != b[i].sample_rate || a[i].number_of_bins != b[i].number_of_bins ||
-        a[i].position[0] != b[i].position[0] || a[i].position[1] != b[i].position[1] ||
-        a[i].position[2] != b[i].position[2])
-      return false;
-  }
-  return true;
-}
 
 template <class T>
 void upload(DeviceBuffer<T>& device, T const* host, std::size_t count, hipStream_t stream) {
@@ -114,7 +26,7 @@ void upload(DeviceBuffer<T>& device, T const* host, std::size_t count, hipStream
 
 RadioAccumulator::RadioAccumulator(c8mi_radio_config const& config, hipStream_t stream)
     : config_(config) {
-  validate(config);
+  validate_radio_config(config);
   if (stream == nullptr) {
     C8MI_HIP(hipStreamCreate(&stream_));
     owns_stream_ = true;
@@ -142,22 +54,11 @@ RadioAccumulator::RadioAccumulator(c8mi_radio_config const& config, hipStream_t 
     upload(coreas_observers_, coreas_host_.data(), coreas_host_.size(), stream_);
     upload(zhs_observers_, zhs_host_.data(), zhs_host_.size(), stream_);
     auto const& p = config.propagation;
-    propagation_.homogeneous_index = p.homogeneous_refractive_index;
-    propagation_.minimum_height = p.minimum_height_m;
-    propagation_.maximum_height = p.maximum_height_m;
-    propagation_.step = p.step_m;
-    propagation_.inverse_step = p.inverse_step_per_m;
-    propagation_.slope_n_lower = p.slope_refractivity_lower;
-    propagation_.slope_i_lower = p.slope_integrated_refractivity_lower;
-    propagation_.slope_n_upper = p.slope_refractivity_upper;
-    propagation_.slope_i_upper = p.slope_integrated_refractivity_upper;
     if (!(p.homogeneous_refractive_index > 0.)) {
       upload(refractivity_, p.refractivity, p.table_size, stream_);
       upload(integrated_, p.integrated_refractivity, p.table_size, stream_);
-      propagation_.table_size = p.table_size;
     }
-    propagation_.refractivity = refractivity_.data();
-    propagation_.integrated = integrated_.data();
+    propagation_ = propagation_view(p, refractivity_.data(), integrated_.data());
     config_.propagation.refractivity = nullptr;
     config_.propagation.integrated_refractivity = nullptr;
     if (config.deterministic) {

@@ -48,6 +48,16 @@ struct CountersDev {
   unsigned long long pad[3];
 };
 
+// Tracks [begin, end) of one pool slot in a batch of several showers' tracks;
+// its items (chunks x the slot's observers x window tiles) start at item_begin.
+struct SegmentDev {
+  std::uint64_t begin;
+  std::uint64_t end;
+  std::uint64_t item_begin;
+  std::uint32_t slot;
+  std::uint32_t pad;
+};
+
 struct ProjectionLaunch {
   double const* tracks;  // [9][stride] compact tracks
   std::uint64_t stride;
@@ -68,6 +78,15 @@ struct ProjectionLaunch {
   std::uint32_t tile_bins;   // bins held in LDS per mode
   std::uint32_t count_valid; // tally valid tracks in this launch (once per batch)
   std::uint32_t pad;
+  // Pooled showers (segments non-null): workgroup b < *item_count is an item of
+  // the segment holding it; the segment's slot selects the observers
+  // [slot * slot_observers, + slot_observer_counts[slot]) and counters[slot].
+  SegmentDev const* segments;
+  std::uint32_t const* segment_count;
+  unsigned long long const* item_count;
+  std::uint32_t const* slot_observer_counts;
+  std::uint32_t slot_observers;
+  std::uint32_t pad2;
 };
 
 }  // namespace c8mi::radio

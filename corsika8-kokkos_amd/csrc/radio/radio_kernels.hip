@@ -696,16 +696,49 @@ __global__ void __launch_bounds__(Block, 4) radio_project_kernel(ProjectionLaunc
   __shared__ unsigned long long block_counts[5];
   __shared__ double block_abs[2];
 
-  // XCD-aware decode of blockIdx.x into (chunk, observer, window tile).
-  std::uint32_t const linear = blockIdx.x;
-  std::uint32_t const xcd = linear % kXcds;
-  std::uint32_t const sequence = linear / kXcds;
-  std::uint32_t const per_chunk = L.observer_count * L.bin_tiles;
-  std::uint32_t const chunk = (sequence / per_chunk) * kXcds + xcd;
-  if (chunk >= L.chunks) return;
-  std::uint32_t const within = sequence % per_chunk;
-  std::uint32_t const observer = within / L.bin_tiles;
-  std::uint32_t const bin_tile = within % L.bin_tiles;
+  std::uint32_t observer, bin_tile;
+  std::uint64_t chunk_begin, chunk_end;
+  CountersDev* counters = L.counters;
+  bool first_observer_tile;
+  if (L.segments == nullptr) {
+    // XCD-aware decode of blockIdx.x into (chunk, observer, window tile).
+    std::uint32_t const linear = blockIdx.x;
+    std::uint32_t const xcd = linear % kXcds;
+    std::uint32_t const sequence = linear / kXcds;
+    std::uint32_t const per_chunk = L.observer_count * L.bin_tiles;
+    std::uint32_t const chunk = (sequence / per_chunk) * kXcds + xcd;
+    if (chunk >= L.chunks) return;
+    std::uint32_t const within = sequence % per_chunk;
+    observer = within / L.bin_tiles;
+    bin_tile = within % L.bin_tiles;
+    chunk_begin = static_cast<std::uint64_t>(chunk) * L.chunk_size;
+    chunk_end = chunk_begin + L.chunk_size < L.track_count ? chunk_begin + L.chunk_size : L.track_count;
+    first_observer_tile = L.count_valid && observer == 0 && bin_tile == 0;
+  } else {
+    // Pooled showers: the segment whose items hold blockIdx.x (binary search
+    // over item_begin), then (chunk, observer of the slot, window tile).
+    unsigned long long const item = blockIdx.x;
+    if (item >= *L.item_count) return;
+    std::uint32_t lo = 0, hi = *L.segment_count;  // segments[lo].item_begin <= item
+    while (hi - lo > 1) {
+      std::uint32_t const mid = (lo + hi) / 2;
+      if (L.segments[mid].item_begin <= item) lo = mid;
+      else hi = mid;
+    }
+    SegmentDev const seg = L.segments[lo];
+    std::uint64_t const local = item - seg.item_begin;
+    std::uint64_t const per_chunk =
+        static_cast<std::uint64_t>(L.slot_observer_counts[seg.slot]) * L.bin_tiles;
+    std::uint64_t const chunk = local / per_chunk;
+    std::uint32_t const within = static_cast<std::uint32_t>(local % per_chunk);
+    std::uint32_t const slot_observer = within / L.bin_tiles;
+    observer = seg.slot * L.slot_observers + slot_observer;
+    bin_tile = within % L.bin_tiles;
+    chunk_begin = seg.begin + chunk * L.chunk_size;
+    chunk_end = chunk_begin + L.chunk_size < seg.end ? chunk_begin + L.chunk_size : seg.end;
+    counters = L.counters + seg.slot;
+    first_observer_tile = slot_observer == 0 && bin_tile == 0;
+  }
 
   std::uint32_t const W = L.tile_bins;
   Acc* const coreas_lds = reinterpret_cast<Acc*>(lds_raw);
@@ -732,11 +765,7 @@ __global__ void __launch_bounds__(Block, 4) radio_project_kernel(ProjectionLaunc
                           static_cast<std::int32_t>(tile_bins)};
   Tile<Fixed> const ztile{zhs_lds, W + 1, static_cast<std::int32_t>(tile_begin),
                           static_cast<std::int32_t>(tile_bins)};
-  bool const first_observer_tile = L.count_valid && observer == 0 && bin_tile == 0;
 
-  std::uint64_t const chunk_begin = static_cast<std::uint64_t>(chunk) * L.chunk_size;
-  std::uint64_t const chunk_end =
-      chunk_begin + L.chunk_size < L.track_count ? chunk_begin + L.chunk_size : L.track_count;
   Tally ctally, ztally;
   for (std::uint64_t i = chunk_begin + threadIdx.x; i < chunk_end; i += blockDim.x) {
     Track t;
@@ -802,11 +831,11 @@ __global__ void __launch_bounds__(Block, 4) radio_project_kernel(ProjectionLaunc
     if constexpr (Fixed) {
       if (block_abs[0] >= kIntLimit || block_abs[1] >= kIntLimit) ++overflow;
     }
-    if (block_counts[0]) atomicAdd(&L.counters->coreas_contributions, block_counts[0]);
-    if (block_counts[1]) atomicAdd(&L.counters->zhs_contributions, block_counts[1]);
-    if (block_counts[2]) atomicAdd(&L.counters->zhs_subtracks, block_counts[2]);
-    if (block_counts[3]) atomicAdd(&L.counters->valid_tracks, block_counts[3]);
-    if (overflow) atomicAdd(&L.counters->fixed_point_overflows, overflow);
+    if (block_counts[0]) atomicAdd(&counters->coreas_contributions, block_counts[0]);
+    if (block_counts[1]) atomicAdd(&counters->zhs_contributions, block_counts[1]);
+    if (block_counts[2]) atomicAdd(&counters->zhs_subtracks, block_counts[2]);
+    if (block_counts[3]) atomicAdd(&counters->valid_tracks, block_counts[3]);
+    if (overflow) atomicAdd(&counters->fixed_point_overflows, overflow);
   }
 }
 
@@ -891,6 +920,15 @@ void launch_projection(ProjectionLaunch const& launch, bool coreas, bool zhs, bo
   if (coreas && zhs) launch_modes<true, true>(launch, fixed, exact, block, stream, lds, grid);
   else if (coreas) launch_modes<true, false>(launch, fixed, exact, block, stream, lds, grid);
   else if (zhs) launch_modes<false, true>(launch, fixed, exact, block, stream, lds, grid);
+}
+
+void launch_pooled_projection(ProjectionLaunch const& launch, bool coreas, bool zhs, bool exact,
+                              std::uint32_t block, std::uint32_t grid, hipStream_t stream) {
+  if (launch.track_count == 0 || grid == 0) return;
+  std::size_t const lds = projection_lds_bytes(coreas, zhs, launch.tile_bins);
+  if (coreas && zhs) launch_modes<true, true>(launch, true, exact, block, stream, lds, grid);
+  else if (coreas) launch_modes<true, false>(launch, true, exact, block, stream, lds, grid);
+  else if (zhs) launch_modes<false, true>(launch, true, exact, block, stream, lds, grid);
 }
 
 void launch_records_to_tracks(c8mi_lepton_transport_record const* records, std::uint64_t count,

@@ -27,6 +27,11 @@ std::uint32_t projection_grid(ProjectionLaunch const& launch);
 void launch_projection(ProjectionLaunch const& launch, bool coreas, bool zhs, bool fixed,
                        bool exact, std::uint32_t block, hipStream_t stream);
 
+// Pooled showers (launch.segments set): `grid` workgroups, an upper bound on
+// the items (*launch.item_count); int64 fixed point.
+void launch_pooled_projection(ProjectionLaunch const& launch, bool coreas, bool zhs, bool exact,
+                              std::uint32_t block, std::uint32_t grid, hipStream_t stream);
+
 void launch_records_to_tracks(c8mi_lepton_transport_record const* records,
                               std::uint64_t count, double* tracks, std::uint64_t stride,
                               hipStream_t stream);

@@ -480,6 +480,13 @@ typedef struct c8mi_em_request {
   double arrival_s;            /* offset from the start of the serve call, non-decreasing */
   double footprint_particles;  /* admission estimate of the shower's peak queue occupancy */
   c8mi_em_shower_axis axis;    /* grammage_g_per_cm2 NULL: the configuration's axis */
+  /* Per-shower radio (c8mi_em_enable_slot_radio): the shower's observers
+   * (none: no radio) and where its raw fixed-point sums go when it ends,
+   * [3][sum of the observers' bins] each (either may be NULL). */
+  const c8mi_radio_observer* radio_observers;
+  uint64_t radio_observer_count;
+  int64_t* radio_coreas;
+  int64_t* radio_zhs;
 } c8mi_em_request;
 
 typedef struct c8mi_em_serve_options {
@@ -502,6 +509,7 @@ typedef struct c8mi_em_request_result {
   uint32_t slot;
   uint32_t pad_;
   c8mi_em_ledger ledger;
+  c8mi_radio_counters radio_counters; /* per-shower radio only */
 } c8mi_em_request_result;
 
 /* Runs the whole request stream to completion; *totals sums the fronts. The
@@ -523,6 +531,31 @@ int c8mi_em_serve(c8mi_em* em, const c8mi_em_request* requests, uint64_t count,
  * detaches (after the projections in flight finish). The radio handle must not
  * be destroyed while attached. */
 int c8mi_em_attach_radio(c8mi_em* em, c8mi_radio* radio);
+
+/* ---- Per-shower radio of the pool ----
+ * The reference gives every shower its own RadioAccumulator (BackendConfig::radio_,
+ * corsika/kokkos/em/execution/BackendConfig.hpp:66) fed by each lepton front
+ * (LeptonCascade.inl:81-82). Here every pool slot keeps its own observers,
+ * int64 fixed-point waveforms and counters resident on the device, and one
+ * projection per lepton front covers all slots, each slot's e+- steps on its
+ * own observers. `config` gives the algorithms, refinement, exactness,
+ * fixed-point limit and propagation (observer lists ignored; deterministic
+ * only); each slot holds up to observers_per_slot (<= 256) observers of up
+ * to bins_per_observer bins. Enabling (again) clears every slot's observers. */
+int c8mi_em_enable_slot_radio(c8mi_em* em, const c8mi_radio_config* config, uint32_t observers_per_slot,
+                              uint32_t bins_per_observer);
+/* Sets a slot's observers (count 0: none) and clears its waveforms and
+ * counters; applies to the lepton fronts that follow. c8mi_em_slot_begin (and
+ * c8mi_em_begin_shower for slot 0) starts a shower without observers;
+ * c8mi_em_serve sets every admitted request's own. */
+int c8mi_em_slot_set_radio(c8mi_em* em, uint32_t slot, const c8mi_radio_observer* observers, uint64_t count);
+/* RadioAccumulator::downloadFixed for one slot: [3][sum of its observers'
+ * bins] per algorithm (either may be NULL) and its counters; waits. */
+int c8mi_em_slot_download_radio(c8mi_em* em, uint32_t slot, int64_t* coreas, int64_t* zhs,
+                                c8mi_radio_counters* counters);
+/* Per-shower projection time (HIP events on its stream) and launches since
+ * c8mi_em_enable_kernel_timing. */
+int c8mi_em_slot_radio_kernel_time(c8mi_em* em, double* ms, uint64_t* launches);
 
 /* Per-kernel device time (HIP events) since enable: ms[0] photon step,
  * ms[1] lepton transport, ms[2] lepton vertex, ms[3] count, ms[4] scan,

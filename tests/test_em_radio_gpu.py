@@ -194,3 +194,92 @@ def test_detached_backend_projects_nothing(tables):
     radio.close()
     assert counters["valid_tracks"] == 0
     assert not coreas.any() and not zhs.any()
+
+
+# ---- per-shower radio of the pool (every slot its own observers) ----
+
+SLOT_CASES = [  # pid, energy (GeV), shower id, antenna ring radius (m), bins
+    (22, 2.0, 31, 40.0, 384), (11, 1.5, 32, 90.0, 512), (-11, 1.0, 33, 25.0, 256), (22, 1.0, 34, 150.0, 320)]
+
+
+def slot_case(tables, atm, cfg, case):
+    pid, energy, shower, radius, nbins = case
+    prim = em.particle(pid, energy, START, DOWN, history_id=1)
+    tracks = oracle_tracks(cfg, tables, atm, prim, shower)["tracks"]
+    antennas = [(radius * np.cos(a), radius * np.sin(a), 0.0) for a in (0.3, 2.0, 4.4)]
+    observers = window_observers(tracks, antennas, 1.0e9, nbins)
+    return prim, tracks, observers
+
+
+@pytest.mark.parametrize("exact", [True, False])
+def test_pool_slots_have_their_own_radio(tables, exact):
+    """Four showers in four slots of the same fronts, each with its own
+    antennas and windows: every slot's waveforms are its own shower's tracks
+    projected on its own observers (the oracle shower alone), the other slots'
+    steps leave no trace."""
+    atm = em.Atmosphere()
+    cfg = em.make_config(airtables, seed=19, atmosphere=atm, slots=4, moliere=True, moliere_table=True)
+    prop = propagation()
+    cases = [slot_case(tables, atm, cfg, c) for c in SLOT_CASES]
+    dev = em.EmBackend(cfg, tables, atm)
+    dev.enable_slot_radio(4, 512, prop, exact=exact)
+    for slot, (case, (prim, _, observers)) in enumerate(zip(SLOT_CASES, cases)):
+        dev.slot_begin(slot, case[2], 2)
+        dev.slot_set_radio(slot, observers)
+        dev.slot_enqueue(slot, prim)
+    totals = dev.run_rounds(1 << 20)
+    got = [dev.slot_download_radio(slot) for slot in range(4)]
+    dev.close()
+    assert totals["lepton_steps"] == sum(t.shape[1] for _, t, _ in cases)
+    for (prim, tracks, observers), g in zip(cases, got):
+        ref = oracle_lib.radio_project(prop, observers, tracks)
+        assert ref["counters"]["valid_tracks"] > 0
+        assert_same_radio(g, ref, observers)
+
+
+def test_serving_loop_radio_alone_and_in_the_mix(tables):
+    """The serving loop with per-shower radio: each request's waveforms (and a
+    request without radio in between) equal the oracle's shower alone, whether
+    the pool admits them together (budget) or one at a time."""
+    atm = em.Atmosphere()
+    prop = propagation()
+    for slots, budget in ((3, 300.0), (1, 1e9)):
+        cfg = em.make_config(airtables, seed=23, atmosphere=atm, slots=slots, moliere=True, moliere_table=True)
+        cases = [slot_case(tables, atm, cfg, c) for c in SLOT_CASES]
+        requests = []
+        for k, (case, (prim, _, observers)) in enumerate(zip(SLOT_CASES, cases)):
+            requests.append(dict(primary=prim, shower_id=case[2], arrival_s=0.0, footprint=60.0 * case[1],
+                                 radio_observers=None if k == 2 else observers))
+        dev = em.EmBackend(cfg, tables, atm)
+        dev.enable_slot_radio(3, 512, prop)
+        results, _ = dev.serve(requests, budget, realtime=False)
+        dev.close()
+        for k, ((prim, tracks, observers), res) in enumerate(zip(cases, results)):
+            assert res["lepton_steps"] == tracks.shape[1]
+            if k == 2:
+                assert "radio" not in res and res["radio_counters"]["valid_tracks"] == 0
+                continue
+            ref = oracle_lib.radio_project(prop, observers, tracks)
+            assert_same_radio((res["radio"][0], res["radio"][1], res["radio_counters"]), ref, observers)
+
+
+def test_slot_radio_contract(tables):
+    atm = em.Atmosphere()
+    cfg = em.make_config(airtables, seed=3, atmosphere=atm, slots=2)
+    prop = propagation()
+    dev = em.EmBackend(cfg, tables, atm)
+    obs = window_observers(np.zeros((9, 1)), [(0.0, 0.0, 0.0)], 1.0e9, 64)
+    with pytest.raises(c8mi.C8miError):  # not enabled yet
+        dev.slot_set_radio(0, obs)
+    dev.enable_slot_radio(2, 64, prop)
+    with pytest.raises(c8mi.C8miError):  # more observers than a slot holds
+        dev.slot_set_radio(0, np.concatenate([obs, obs, obs]))
+    big = window_observers(np.zeros((9, 1)), [(0.0, 0.0, 0.0)], 1.0e9, 65)
+    with pytest.raises(c8mi.C8miError):  # more bins than an observer holds
+        dev.slot_set_radio(0, big)
+    with pytest.raises(c8mi.C8miError):  # slot outside the pool
+        dev.slot_set_radio(2, obs)
+    dev.slot_set_radio(1, obs)
+    c, z, counters = dev.slot_download_radio(1)
+    assert c.shape == (3, 64) and not c.any() and not z.any() and counters["valid_tracks"] == 0
+    dev.close()
