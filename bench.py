@@ -68,6 +68,13 @@ def parse():
                    help="timed showers of the EM + radio section (0 skips it)")
     p.add_argument("--em-radio-energy", type=float, default=1.0e4, help="primary electron energy (GeV)")
     p.add_argument("--em-radio-only", action="store_true", help="run only the EM + radio section")
+    p.add_argument("--serve-showers", type=int, default=160,
+                   help="requests of the serving-stream section (0 skips it)")
+    p.add_argument("--serve-rate", type=float, default=40.0, help="Poisson arrival rate of the stream (1/s)")
+    p.add_argument("--serve-slots", type=int, default=256, help="pool slots of the serving section")
+    p.add_argument("--serve-plain-showers", type=int, default=24,
+                   help="first requests served again alone, pooled and one at a time (latency comparison)")
+    p.add_argument("--serve-only", action="store_true", help="run only the serving-stream section")
     return p.parse_args()
 
 
@@ -435,6 +442,152 @@ def em_radio_section(showers, energy, cpu_seconds, no_cpu_baseline, moliere=True
     return out
 
 
+def serving_cpu_baseline(tables, atmosphere, propagation, target_seconds, moliere_table=True):
+    """The serving stream's work on the host: each thread serves requests
+    alone, one after another (oracle EM shower with its step tracks, then the
+    FP64 radio oracle on the request's antennas), on all host cores. Sample:
+    requests of the same stream restricted to 100-300 GeV (the per-step and
+    per-update costs do not depend on the energy; the TeV tail would take
+    minutes per shower here)."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from concurrent.futures import ThreadPoolExecutor
+
+    import oracle_lib
+    from c8mi import airtables, em, workloads
+
+    threads = max(1, min(int(os.environ.get("OMP_NUM_THREADS", "0")) or 16, os.cpu_count() or 1))
+    stream = workloads.serving_stream(4 * threads, 10.0, seed=777, e_min_gev=100.0, e_max_gev=300.0)
+
+    def serve_one(k):
+        r = stream[k]
+        cfg = em.make_config(airtables, atmosphere=atmosphere, start=tuple(r["start_m"]),
+                             direction=tuple(r["direction"]), moliere=True,
+                             moliere_table=True if moliere_table else None)
+        prim = em.particle(r["pid"], r["energy_gev"], tuple(r["start_m"]), tuple(r["direction"]), history_id=1)
+        out = oracle_lib.em_run_shower(cfg, tables, atmosphere, prim, shower_id=9000 + k, tracks=True)
+        oracle_lib.radio_project(propagation, r["observers"], out["tracks"], floating=False, threads=1)
+        return out["stats"]["photon_steps"] + out["stats"]["lepton_steps"], out["tracks"].shape[1] * r["observers"].size
+
+    count = threads
+    while True:
+        t0 = time.perf_counter()
+        with ThreadPoolExecutor(threads) as pool:
+            done = list(pool.map(serve_one, range(count)))
+        wall = time.perf_counter() - t0
+        if wall >= 0.5 * target_seconds or count >= len(stream):
+            break
+        count = min(len(stream), 2 * count)
+    steps = sum(d[0] for d in done)
+    updates = sum(d[1] for d in done)
+    return {"value": steps / wall, "unit": "particle-steps/s", "cores": threads, "kind": "port",
+            "updates_per_s": updates / wall,
+            "sample": f"{count} requests of the stream restricted to 100-300 GeV, each served alone by one "
+                      f"thread (oracle EM shower, then the FP64 radio oracle on its 40 antennas); {threads} "
+                      f"threads, {wall:.1f} s wall, {steps} steps"}
+
+
+def serving_section(count, rate, slots, plain_count, cpu_seconds, no_cpu_baseline, moliere_table=True):
+    """The serving stream: photon and electron primaries 50/50, energy
+    log-uniform in 100 GeV - 100 TeV, zenith U(0, 60 deg), azimuth U(0, 360
+    deg), Poisson arrivals at `rate` per second (above what the device serves,
+    so the pool saturates), 1 MeV cut, Molière on, CoREAS+ZHS radio on 40
+    antennas per shower (8 arms x 5 radii, 50-250 m, 1024 bins @ 0.5 ns) kept
+    per shower on the device. Requests are admitted as they arrive, into free
+    slots and within the footprint budget (continuous batching); per-shower
+    latency from arrival to completion. For the latency comparison the first
+    `plain_count` requests of the same stream, same arrivals, are served again
+    alone: through the pool and through a single slot (one shower at a
+    time)."""
+    import c8mi
+    from c8mi import airtables, em, workloads
+
+    tables = airtables.build_air_tables()
+    atm = em.Atmosphere()
+    refractivity = workloads.exponential_refractivity(-1000.0, 113000.0, 1.0)
+    prop = c8mi.Propagation(refractivity=refractivity, minimum_height_m=-1000.0, maximum_height_m=113000.0,
+                            step_m=1.0)
+    stream = workloads.serving_stream(count, rate)
+    axes = [em.shower_axis(atm, tuple(r["start_m"]), tuple(r["direction"])) for r in stream]
+
+    def requests(n, first_id):
+        return [dict(primary=em.particle(r["pid"], r["energy_gev"], tuple(r["start_m"]), tuple(r["direction"]),
+                                         history_id=1),
+                     shower_id=first_id + k, arrival_s=r["arrival_s"], footprint=em.footprint_estimate(r["energy_gev"]),
+                     axis=axes[k], radio_observers=r["observers"]) for k, r in enumerate(stream[:n])]
+
+    table = True if moliere_table else None
+    capacity = 1 << 25
+
+    def run(n, pool_slots, timing):
+        cfg = em.make_config(airtables, atmosphere=atm, slots=pool_slots, queue_capacity=capacity,
+                             observation_capacity=1 << 12, fallback_capacity=1 << 8, moliere=True,
+                             moliere_table=table)
+        dev = em.EmBackend(cfg, tables, atm)
+        dev.enable_slot_radio(40, 1024, prop)
+        # warm-up: one small shower through the same pool
+        warm = dict(primary=em.particle(22, 100.0, tuple(stream[0]["start_m"]), tuple(stream[0]["direction"]),
+                                        history_id=1), shower_id=1, arrival_s=0.0, footprint=1.0, axis=axes[0],
+                    radio_observers=stream[0]["observers"])
+        dev.serve([warm], 0.4 * capacity, realtime=False)
+        dev.enable_kernel_timing(timing)
+        t0 = time.perf_counter()
+        results, totals = dev.serve(requests(n, 100), 0.4 * capacity, realtime=True)
+        wall = time.perf_counter() - t0
+        timing_out = (dev.kernel_time(), dev.slot_radio_kernel_time()) if timing else None
+        dev.close()
+        return results, totals, wall, timing_out
+
+    results, totals, wall, timing = run(count, slots, True)
+    steps = totals["photon_steps"] + totals["lepton_steps"]
+    updates = sum(r["lepton_steps"] for r in results) * 40
+    latency = [r["finished_s"] - s["arrival_s"] for r, s in zip(results, stream)]
+    service = [r["finished_s"] - r["admitted_s"] for r in results]
+    em_timing, (radio_ms, radio_launches) = timing
+    out = {
+        "metric": "serving: particle-steps/s, track·antenna updates/s, per-shower latency",
+        "value": steps / wall,
+        "unit": "particle-steps/s",
+        "updates_per_s": updates / wall,
+        "showers": count,
+        "slots": slots,
+        "arrival_rate_hz": rate,
+        "offered_s": stream[-1]["arrival_s"],
+        "wall_s": wall,
+        "particle_steps": steps,
+        "latency_ms": {"p50": float(np.percentile(latency, 50) * 1e3), "p99": float(np.percentile(latency, 99) * 1e3)},
+        "service_ms": {"p50": float(np.percentile(service, 50) * 1e3), "p99": float(np.percentile(service, 99) * 1e3)},
+        "valid_radio_tracks": int(sum(r["radio_counters"]["valid_tracks"] for r in results)),
+        "kernel_ms": {**{k: v[0] for k, v in em_timing.items()}, "radio_per_shower": radio_ms},
+        "radio_launches": radio_launches,
+        "dtype": "f64",
+        "config": {"workload": f"serving stream: {count} requests, photon/electron 50/50, E log-uniform "
+                               f"100 GeV-100 TeV, zenith U(0,60), azimuth U(0,360), Poisson arrivals at "
+                               f"{rate:g}/s, 1 MeV cut, Moliere on, CoREAS+ZHS on 40 antennas per shower "
+                               f"(1024 bins @ 0.5 ns), {slots} pool slots"},
+    }
+    if plain_count:
+        def summary(res, tot, w):
+            lat = [r["finished_s"] - s["arrival_s"] for r, s in zip(res, stream)]
+            srv = [r["finished_s"] - r["admitted_s"] for r in res]
+            return {"value": (tot["photon_steps"] + tot["lepton_steps"]) / w, "wall_s": w,
+                    "latency_ms": {"p50": float(np.percentile(lat, 50) * 1e3),
+                                   "p99": float(np.percentile(lat, 99) * 1e3)},
+                    "service_ms": {"p50": float(np.percentile(srv, 50) * 1e3),
+                                   "p99": float(np.percentile(srv, 99) * 1e3)}}
+
+        out["latency_comparison"] = {
+            "showers": plain_count,
+            "pool": summary(*run(plain_count, slots, False)[:3]),
+            "one_at_a_time": summary(*run(plain_count, 1, False)[:3]),
+        }
+    if not no_cpu_baseline:
+        base = serving_cpu_baseline(tables, atm, prop, cpu_seconds, moliere_table=moliere_table)
+        out["cpu_baseline"] = base
+        out["speedup_vs_cpu_baseline"] = out["value"] / base["value"]
+        out["updates_speedup_vs_cpu_baseline"] = out["updates_per_s"] / base["updates_per_s"]
+    return out
+
+
 def measured_traffic(kernel):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 PMC summary
     (FETCH_SIZE + WRITE_SIZE, copy-calibrated), or None."""
@@ -464,6 +617,14 @@ def main():
     if c8mi.device_count() < 1:
         print("bench.py: no HIP device visible", file=sys.stderr)
         return 1
+    if args.serve_only:
+        out = serving_section(max(args.serve_showers, 1), args.serve_rate, args.serve_slots,
+                              args.serve_plain_showers, args.cpu_seconds, args.no_cpu_baseline,
+                              moliere_table=not args.em_analytic_moliere)
+        out.update({"n_gpus": 1, "steps": out["showers"], "higher_is_better": True, "scaling": "weak",
+                    "vs_baseline": None, "data": "synthetic", "ms_per_step": out["latency_ms"]["p50"]})
+        print(json.dumps(out), flush=True)
+        return 0
     if args.em_radio_only:
         out = em_radio_section(max(args.em_radio_showers, 1), args.em_radio_energy, args.cpu_seconds,
                                args.no_cpu_baseline, moliere=not args.em_no_moliere,
@@ -579,6 +740,10 @@ def main():
                                   args.no_cpu_baseline, warmup_showers=args.em_warmup_showers,
                                moliere=not args.em_no_moliere,
                                moliere_table=not args.em_analytic_moliere)
+    if args.serve_showers > 0:
+        result["serving"] = serving_section(args.serve_showers, args.serve_rate, args.serve_slots,
+                                            args.serve_plain_showers, args.cpu_seconds, args.no_cpu_baseline,
+                                            moliere_table=not args.em_analytic_moliere)
     if args.em_radio_showers > 0:
         result["em_radio"] = em_radio_section(args.em_radio_showers, args.em_radio_energy, args.cpu_seconds,
                                               args.no_cpu_baseline, moliere=not args.em_no_moliere,

@@ -188,3 +188,40 @@ def tracks_to_records(tracks: np.ndarray, energy_gev: float = 1.0) -> np.ndarray
     rec["start"]["direction"] = (disp / norm).T
     rec["end"]["direction"] = (disp / norm).T
     return rec
+
+
+def serving_stream(count: int, rate_hz: float, seed: int = 4242, e_min_gev: float = 100.0,
+                   e_max_gev: float = 1.0e5, max_zenith_deg: float = 60.0, injection_height_m: float = 112000.0,
+                   arms: int = 8, radii: int = 5, radius_step_m: float = 50.0, bins: int = 1024,
+                   sample_rate_hz: float = 2.0e9, pretrigger_s: float = 10.0e-9,
+                   earth_radius_m: float = 6371000.0):
+    """A stream of shower requests: photon and electron primaries 50/50, energy
+    log-uniform in [e_min, e_max], zenith uniform in [0, max_zenith], azimuth
+    uniform, Poisson arrivals at `rate_hz`; each shower's axis hits the core
+    (0, 0, 0) from `injection_height_m` above the spherical Earth, and its
+    radio antennas (arms x radii star around the core on the ground) open
+    their windows `pretrigger_s` before the primary's plane front.
+
+    Returns a list of dicts: pid, energy_gev, zenith_deg, azimuth_deg,
+    arrival_s, start_m, direction, observers (OBSERVER_DTYPE)."""
+    rng = np.random.default_rng(seed)
+    arrivals = np.cumsum(rng.exponential(1.0 / rate_hz, size=count))
+    arrivals -= arrivals[0]
+    out = []
+    for k in range(count):
+        pid = 22 if rng.random() < 0.5 else 11
+        energy = float(np.exp(rng.uniform(np.log(e_min_gev), np.log(e_max_gev))))
+        zenith = float(rng.uniform(0.0, max_zenith_deg))
+        azimuth = float(rng.uniform(0.0, 360.0))
+        th, ph = math.radians(zenith), math.radians(azimuth)
+        d = np.array([math.sin(th) * math.cos(ph), math.sin(th) * math.sin(ph), -math.cos(th)])
+        # slant distance from the core to the injection height on a sphere
+        r, h, c = earth_radius_m, injection_height_m, math.cos(th)
+        slant = -r * c + math.sqrt(r * r * c * c + h * h + 2.0 * r * h)
+        start = -slant * d
+        obs = star_observers(arms=arms, radii=radii, radius_step_m=radius_step_m, sample_rate_hz=sample_rate_hz,
+                             number_of_bins=bins)
+        plane_front_windows(obs, start, d, pretrigger_s)
+        out.append(dict(pid=pid, energy_gev=energy, zenith_deg=zenith, azimuth_deg=azimuth,
+                        arrival_s=float(arrivals[k]), start_m=start, direction=d, observers=obs))
+    return out

@@ -597,11 +597,12 @@ void EmBackend::front(bool photon, c8mi_em_run_result& out) {
   // Projects this front's tracks on the accumulators' streams once the step kernel is done.
   auto project_tracks = [&] {
     if (!f.tracks) return;
+    std::uint64_t const chunk = slot_radio_ ? slot_radio_->chunk_for(n) : 0;
     if (slot_radio_) {
       std::size_t const keys = static_cast<std::size_t>(slots_) * kMaxRuns;
       unsigned long long* key_begin = segment_keys_[track_buffer].data();
-      launch_radio_segments(f, active_range_, key_begin, key_begin + keys, slot_radio_->observer_counts(),
-                            slot_radio_->chunk_size(), slot_radio_->bin_tiles(), segments_[track_buffer].data(),
+      launch_radio_segments(f, active_range_, key_begin, key_begin + keys, slot_radio_->observer_counts(), chunk,
+                            slot_radio_->bin_tiles(), segments_[track_buffer].data(),
                             segment_count_[track_buffer].data(), item_count_[track_buffer].data(), stream_);
     }
     C8MI_HIP(hipEventRecord(track_ready_[track_buffer], stream_));
@@ -613,8 +614,9 @@ void EmBackend::front(bool photon, c8mi_em_run_result& out) {
     }
     if (slot_radio_) {
       C8MI_HIP(hipStreamWaitEvent(slot_radio_->stream(), track_ready_[track_buffer], 0));
-      slot_radio_->project(f.tracks, n, n, segments_[track_buffer].data(), segment_count_[track_buffer].data(),
-                           item_count_[track_buffer].data(), active_range_ * kMaxRuns);
+      slot_radio_->project(f.tracks, n, n, chunk, segments_[track_buffer].data(),
+                           segment_count_[track_buffer].data(), item_count_[track_buffer].data(),
+                           active_range_ * kMaxRuns);
       C8MI_HIP(hipEventRecord(slot_done_[track_buffer], slot_radio_->stream()));
       slot_busy_[track_buffer] = true;
     }

@@ -160,9 +160,16 @@ void SlotRadio::download(std::uint32_t slot, std::int64_t* coreas, std::int64_t*
   }
 }
 
-void SlotRadio::project(double const* tracks, std::uint64_t stride, std::uint64_t count, SegmentDev const* segments,
-                        std::uint32_t const* segment_count, unsigned long long const* item_count,
-                        std::uint32_t max_segments) {
+std::uint64_t SlotRadio::chunk_for(std::uint64_t count) const noexcept {
+  constexpr std::uint64_t kTargetItems = 8192, kMinimumChunk = 4096, kMaximumChunk = 1 << 20;
+  std::uint64_t const per_item = static_cast<std::uint64_t>(observers_per_slot_) * bin_tiles_;
+  std::uint64_t chunk = (count * per_item + kTargetItems - 1) / kTargetItems;
+  return std::min(kMaximumChunk, std::max(kMinimumChunk, chunk));
+}
+
+void SlotRadio::project(double const* tracks, std::uint64_t stride, std::uint64_t count, std::uint64_t chunk,
+                        SegmentDev const* segments, std::uint32_t const* segment_count,
+                        unsigned long long const* item_count, std::uint32_t max_segments) {
   if (count == 0) return;
   ProjectionLaunch L{};
   L.tracks = tracks;
@@ -178,7 +185,7 @@ void SlotRadio::project(double const* tracks, std::uint64_t stride, std::uint64_
   L.coreas_total_bins = total_bins_;
   L.zhs_total_bins = total_bins_;
   L.counters = counters_.data();
-  L.chunk_size = chunk_size_;
+  L.chunk_size = chunk;
   L.tile_bins = tile_bins_;
   L.bin_tiles = bin_tiles_;
   L.segments = segments;
@@ -188,7 +195,7 @@ void SlotRadio::project(double const* tracks, std::uint64_t stride, std::uint64_
   L.slot_observers = observers_per_slot_;
   // every chunk of every segment on every observer: at most one partial
   // chunk per segment beyond count / chunk_size
-  std::uint64_t const grid = ((count + chunk_size_ - 1) / chunk_size_ + max_segments) *
+  std::uint64_t const grid = ((count + chunk - 1) / chunk + max_segments) *
                              static_cast<std::uint64_t>(observers_per_slot_) * bin_tiles_;
   if (grid > 0x7fffffffULL) throw std::length_error("per-shower radio launch above the grid limit");
   std::array<hipEvent_t, 2> events{nullptr, nullptr};

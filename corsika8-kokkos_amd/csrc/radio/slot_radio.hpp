@@ -36,7 +36,11 @@ class SlotRadio {
 
   hipStream_t stream() const noexcept { return stream_; }
   std::uint32_t observers_per_slot() const noexcept { return observers_per_slot_; }
-  std::uint64_t chunk_size() const noexcept { return chunk_size_; }
+  // Tracks per workgroup for a batch of `count` tracks: about kTargetItems
+  // items over all observers, at least kMinimumChunk tracks each (a
+  // workgroup clears and flushes a whole window; fewer tracks would not pay
+  // for it).
+  std::uint64_t chunk_for(std::uint64_t count) const noexcept;
   std::uint32_t bin_tiles() const noexcept { return bin_tiles_; }
   std::uint32_t const* observer_counts() const noexcept { return observer_counts_.data(); }
 
@@ -55,9 +59,9 @@ class SlotRadio {
   // max_segments, *segment_count of them) give each slot's track ranges and
   // item offsets (chunks x observers x tiles), *item_count their total.
   // Asynchronous on stream().
-  void project(double const* tracks, std::uint64_t stride, std::uint64_t count, SegmentDev const* segments,
-               std::uint32_t const* segment_count, unsigned long long const* item_count,
-               std::uint32_t max_segments);
+  void project(double const* tracks, std::uint64_t stride, std::uint64_t count, std::uint64_t chunk,
+               SegmentDev const* segments, std::uint32_t const* segment_count,
+               unsigned long long const* item_count, std::uint32_t max_segments);
 
   void enable_kernel_timing(bool enable);
   // Projection-kernel time (ms) and launches since timing was enabled.
@@ -69,7 +73,6 @@ class SlotRadio {
   std::uint32_t observers_per_slot_ = 0;
   std::uint32_t bins_per_observer_ = 0;
   std::uint64_t total_bins_ = 0;
-  std::uint64_t chunk_size_ = 1024;
   std::uint32_t tile_bins_ = 0;
   std::uint32_t bin_tiles_ = 1;
   bool fuse_ = false;
