@@ -299,6 +299,10 @@ EmBackend::~EmBackend() {
   slot_radio_.reset();
   for (auto& e : events_)
     if (e) (void)hipEventDestroy(e);
+  if (side_stream_) (void)hipStreamSynchronize(side_stream_);
+  if (open_ready_) (void)hipEventDestroy(open_ready_);
+  if (open_done_) (void)hipEventDestroy(open_done_);
+  if (side_stream_) (void)hipStreamDestroy(side_stream_);
   if (init_copied_) (void)hipEventDestroy(init_copied_);
   if (unstage_copied_) (void)hipEventDestroy(unstage_copied_);
   if (own_stream_ && stream_) (void)hipStreamDestroy(stream_);
@@ -473,7 +477,15 @@ void EmBackend::resize_scratch(std::uint64_t n) {
     moves_.allocate(cap * lepton_move_bytes());
     moved_.allocate(cap);
     moliere_open_.allocate(cap * moliere_open_bytes());
-    if (!moliere_open_count_.data()) moliere_open_count_.allocate(1);
+    open_vertices_.allocate(cap);
+    if (!moliere_open_count_.data()) {
+      moliere_open_count_.allocate(1);
+      open_vertex_count_.allocate(1);
+      C8MI_HIP(hipMemset(open_vertex_count_.data(), 0, sizeof(unsigned long long)));
+      C8MI_HIP(hipStreamCreate(&side_stream_));
+      C8MI_HIP(hipEventCreateWithFlags(&open_ready_, hipEventDisableTiming));
+      C8MI_HIP(hipEventCreateWithFlags(&open_done_, hipEventDisableTiming));
+    }
   }
   scratch_capacity_ = cap;
 }
@@ -578,6 +590,8 @@ void EmBackend::front(bool photon, c8mi_em_run_result& out) {
   f.moved = moved_.data();
   f.moliere_open = moliere_open_.data();
   f.moliere_open_count = moliere_open_count_.data();
+  f.open_vertices = open_vertices_.data();
+  f.open_vertex_count = open_vertex_count_.data();
   int const track_buffer = track_parity_;
   if ((radio_ || slot_radio_) && !photon) {
     // tracks [9][n]; the buffer's previous projections must be done first
@@ -623,7 +637,7 @@ void EmBackend::front(bool photon, c8mi_em_run_result& out) {
     track_parity_ ^= 1;
   };
   // the launches of this front, each followed by an event when timing
-  int kinds[kFrontKernels + 2];
+  int kinds[kFrontKernels + 4];
   int launched = 0;
   auto mark = [&](int kind) {
     if (!timing_) return;
@@ -638,13 +652,22 @@ void EmBackend::front(bool photon, c8mi_em_run_result& out) {
     // Molière scattering runs in the finish kernel, with its own register budget
     launch_lepton_move(context_.data(), config_.profile_bins, f, stream_);
     mark(kKernLeptonTransport);
-    launch_lepton_moliere(context_.data(), host_context_.moliere_polynomials != nullptr, f, stream_);
+    bool const table = host_context_.moliere_polynomials != nullptr;
+    launch_lepton_moliere(context_.data(), table, f, stream_);
     mark(kKernLeptonTransport);
+    // the few open samples continue on the side stream while the other
+    // sources finish and run their vertices here
+    C8MI_HIP(hipEventRecord(open_ready_, stream_));
+    C8MI_HIP(hipStreamWaitEvent(side_stream_, open_ready_, 0));
+    launch_lepton_open(context_.data(), config_.profile_bins, table, f, side_stream_);
+    C8MI_HIP(hipEventRecord(open_done_, side_stream_));
     launch_lepton_finish(context_.data(), config_.profile_bins, f, stream_);
     mark(kKernLeptonTransport);
-    project_tracks();
     launch_lepton_vertex(context_.data(), config_.profile_bins, f, stream_);
     mark(kKernLeptonVertex);
+    C8MI_HIP(hipStreamWaitEvent(stream_, open_done_, 0));
+    mark(kKernLeptonTransport);  // what the open samples add beyond the overlap
+    project_tracks();
   } else {
     launch_lepton_transport(context_.data(), config_.profile_bins, f, stream_);
     mark(kKernLeptonTransport);

@@ -185,6 +185,12 @@ class EmBackend {
   DeviceBuffer<std::uint32_t> moved_;
   DeviceBuffer<unsigned char> moliere_open_;
   DeviceBuffer<unsigned long long> moliere_open_count_;
+  DeviceBuffer<VertexRecord> open_vertices_;
+  DeviceBuffer<unsigned long long> open_vertex_count_;
+  // the open Molière samples' stream (tail, their finish and vertices)
+  hipStream_t side_stream_ = nullptr;
+  hipEvent_t open_ready_ = nullptr;
+  hipEvent_t open_done_ = nullptr;
   DeviceBuffer<unsigned long long> vertex_count_;
   std::uint64_t scratch_capacity_ = 0;
   // radio: tracks of the last two lepton fronts; `track_done_` marks the end
@@ -208,7 +214,7 @@ class EmBackend {
   bool timing_ = false;
   std::array<double, kFrontKernels> ms_{};
   std::array<std::uint64_t, kFrontKernels> launches_{};
-  hipEvent_t events_[kFrontKernels + 3] = {};  // up to 8 launches per front, each followed by one
+  hipEvent_t events_[kFrontKernels + 5] = {};  // up to 9 marks per front, and the one before them
 };
 
 }  // namespace c8mi::em

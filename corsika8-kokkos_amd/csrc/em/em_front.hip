@@ -269,6 +269,7 @@ __global__ __launch_bounds__(kStepBlock) void lepton_moliere_kernel(EmContext co
     f.moved[i] = kMovedMoliereFailed;
   } else if (outcome == phys::kMoliereOpen) {
     open.source = static_cast<std::uint32_t>(i);
+    f.moved[i] = kMovedOpen;  // finished by the open-source kernels
     unsigned long long const k = atomicAdd(f.moliere_open_count, 1ULL);
     static_cast<phys::MoliereOpen*>(f.moliere_open)[k] = open;  // at most one per source
   }
@@ -285,17 +286,59 @@ __global__ __launch_bounds__(moliere::kTailThreads) void lepton_moliere_tail_ker
     phys::LeptonMove mv = static_cast<phys::LeptonMove const*>(f.moves)[open.source];
     int const outcome = phys::moliere_scatter_wave<Table>(*context, mv, open, jobs);
     if (threadIdx.x == 0) {
+      // the outcome goes with the open record: the source stays kMovedOpen for
+      // the finish kernel of the other sources, which may run meanwhile
+      auto* record = static_cast<phys::MoliereOpen*>(f.moliere_open) + w;
       if (outcome == phys::kMoliereFailed) {
-        f.moved[open.source] = kMovedMoliereFailed;
+        record->pad = kMovedMoliereFailed;
       } else {
         auto* out = static_cast<phys::LeptonMove*>(f.moves) + open.source;
         for (int a = 0; a < 3; ++a) out->end.direction[a] = mv.end.direction[a];
+        record->pad = kMovedOk;
       }
     }
   }
 }
 
+// The finish of one moved source (Molière applied): outputs, profile, records
+// and, for an interaction candidate, the vertex to list (returns true).
+__device__ __forceinline__ bool finish_source(EmContext const* local, LdsShowers const& l, FrontArgs const& f,
+                                              std::uint64_t i, std::uint32_t moved, std::uint32_t& s,
+                                              phys::LeptonVertex& vx) {
+  EmContext const& c = *local;
+  s = f.in.shower[i];
+  phys::Accumulator const acc = accumulator(local, l, s);
+  phys::LeptonMove const mv = static_cast<phys::LeptonMove const*>(f.moves)[i];
+  phys::Outputs o = no_outputs(f, i);
+  auto const input = [&] {
+    phys::Particle p;
+    static_cast<c8mi_particle_state&>(p) = load_particle(f.in, i);
+    p.shower_key = mv.start.shower_key;
+    return p;
+  };
+  bool const deferred = phys::lepton_finish(c, acc, input, mv, moved == kMovedMoliereFailed, o, vx);
+  write_outputs(f, i, o, s);
+  if (f.tracks && !o.step) phys::write_no_track(f.tracks + i, f.track_stride);
+  return deferred;
+}
+
+__device__ __forceinline__ VertexRecord vertex_record(phys::LeptonVertex const& vx, std::uint64_t i,
+                                                      std::uint32_t s) {
+  VertexRecord r;
+  r.parent = vx.parent;
+  r.shower_key = vx.parent.shower_key;
+  r.source = i;
+  r.total_rate = vx.total_rate;
+  r.process_uniform = vx.process_uniform;
+  r.particle_mass_gev = vx.particle_mass_gev;
+  r.vertex_density = vx.vertex_density;
+  r.slot = s;
+  r.pad[0] = r.pad[1] = r.pad[2] = 0;
+  return r;
+}
+
 // Two waves per SIMD (a little scratch) measured faster than one for the finish.
+// Sources whose Molière sample is still open are left to lepton_finish_open_kernel.
 __global__ __launch_bounds__(kStepBlock, 2) void lepton_finish_kernel(EmContext const* __restrict__ context,
                                                                    FrontArgs f) {
   extern __shared__ long long lds[];
@@ -311,24 +354,9 @@ __global__ __launch_bounds__(kStepBlock, 2) void lepton_finish_kernel(EmContext 
   bool deferred = false;
   std::uint32_t s = 0;
   phys::LeptonVertex vx;
-  bool stepped = false;
-  std::uint32_t const moved = i < f.count ? f.moved[i] : kMovedNone;
-  if (moved != kMovedNone) {
-    s = f.in.shower[i];
-    phys::Accumulator const acc = accumulator(local, l, s);
-    phys::LeptonMove const mv = static_cast<phys::LeptonMove const*>(f.moves)[i];
-    phys::Outputs o = no_outputs(f, i);
-    auto const input = [&] {
-      phys::Particle p;
-      static_cast<c8mi_particle_state&>(p) = load_particle(f.in, i);
-      p.shower_key = mv.start.shower_key;
-      return p;
-    };
-    deferred = phys::lepton_finish(c, acc, input, mv, moved == kMovedMoliereFailed, o, vx);
-    write_outputs(f, i, o, s);
-    stepped = o.step;
-  }
-  if (f.tracks && i < f.count && !stepped) phys::write_no_track(f.tracks + i, f.track_stride);
+  std::uint32_t const moved = i < f.count ? f.moved[i] : kMovedOpen;
+  if (moved == kMovedOk || moved == kMovedMoliereFailed) deferred = finish_source(local, l, f, i, moved, s, vx);
+  else if (moved == kMovedNone && f.tracks) phys::write_no_track(f.tracks + i, f.track_stride);
   std::uint32_t const wave = threadIdx.x / 64, lane = threadIdx.x % 64;
   unsigned long long const mask = __ballot(deferred);
   if (lane == 0) wave_vertices[wave] = static_cast<std::uint32_t>(__popcll(mask));
@@ -344,17 +372,25 @@ __global__ __launch_bounds__(kStepBlock, 2) void lepton_finish_kernel(EmContext 
     std::uint32_t before = 0;
     for (std::uint32_t w = 0; w < wave; ++w) before += wave_vertices[w];
     unsigned long long const below = lane ? (mask & (~0ULL >> (64 - lane))) : 0ULL;
-    VertexRecord r;
-    r.parent = vx.parent;
-    r.shower_key = vx.parent.shower_key;
-    r.source = i;
-    r.total_rate = vx.total_rate;
-    r.process_uniform = vx.process_uniform;
-    r.particle_mass_gev = vx.particle_mass_gev;
-    r.vertex_density = vx.vertex_density;
-    r.slot = s;
-    r.pad[0] = r.pad[1] = r.pad[2] = 0;
-    f.vertices[block_base + before + static_cast<std::uint32_t>(__popcll(below))] = r;
+    f.vertices[block_base + before + static_cast<std::uint32_t>(__popcll(below))] = vertex_record(vx, i, s);
+  }
+}
+
+// The finish of the sources whose Molière sample went through the tail kernel
+// (few, scattered over the front): one thread each, profile and counters
+// straight to the showers' device totals, vertices into their own list.
+__global__ __launch_bounds__(kStepBlock) void lepton_finish_open_kernel(EmContext const* __restrict__ context,
+                                                                        FrontArgs f) {
+  LdsShowers const l{nullptr, 0u, ~0u, ~0u, 0u};  // no LDS copies: every slot adds to its device totals
+  unsigned long long const n = *f.moliere_open_count;
+  for (unsigned long long w = static_cast<unsigned long long>(blockIdx.x) * kStepBlock + threadIdx.x; w < n;
+       w += static_cast<unsigned long long>(gridDim.x) * kStepBlock) {
+    phys::MoliereOpen const open = static_cast<phys::MoliereOpen const*>(f.moliere_open)[w];
+    std::uint64_t const i = open.source;
+    std::uint32_t s = 0;
+    phys::LeptonVertex vx;
+    if (finish_source(context, l, f, i, open.pad, s, vx))
+      f.open_vertices[atomicAdd(f.open_vertex_count, 1ULL)] = vertex_record(vx, i, s);
   }
 }
 
@@ -418,7 +454,10 @@ __global__ __launch_bounds__(kStepBlock) void count_kernel(FrontArgs f) {
     for (int w = 0; w < kStepBlock / 64; ++w)
       for (int k = 0; k < kShowerFields; ++k) t[k] += wave_sums[w][k];
     static_cast<BlockCounts*>(f.block_counts)[blockIdx.x] = BlockCounts{t[0], t[1], t[2], t[3], t[4], {0, 0, 0}};
-    if (blockIdx.x == 0) *f.vertex_count = 0;  // the vertex kernel of this front has finished
+    if (blockIdx.x == 0) {  // the vertex kernels of this front have finished
+      *f.vertex_count = 0;
+      if (f.open_vertex_count) *f.open_vertex_count = 0;
+    }
   }
   if (key == kNoKey) return;
   std::uint32_t const s = key & ((1u << kRunShift) - 1u), r = key >> kRunShift;
@@ -739,13 +778,23 @@ void launch_lepton_moliere(EmContext const* context, bool table, FrontArgs const
   else
     lepton_moliere_kernel<false><<<grid_for(f.count, kStepBlock), kStepBlock, 0, stream>>>(context, f);
   C8MI_HIP(hipGetLastError());
-  // the open samples are rare (~1e-5 of the steps): a fixed grid of small
+}
+
+void launch_lepton_open(EmContext const* context, std::uint32_t bins, bool table, FrontArgs const& f,
+                        hipStream_t stream) {
+  if (f.count == 0) return;
+  // the open samples are rare (~1e-5 of the steps): fixed grids of small
   // workgroups that leave at once when there are none
-  constexpr unsigned kTailGrid = 256;
+  constexpr unsigned kTailGrid = 256, kOpenGrid = 4;
   if (table)
     lepton_moliere_tail_kernel<true><<<kTailGrid, moliere::kTailThreads, 0, stream>>>(context, f);
   else
     lepton_moliere_tail_kernel<false><<<kTailGrid, moliere::kTailThreads, 0, stream>>>(context, f);
+  lepton_finish_open_kernel<<<kOpenGrid, kStepBlock, 0, stream>>>(context, f);
+  FrontArgs g = f;
+  g.vertices = f.open_vertices;
+  g.vertex_count = f.open_vertex_count;
+  lepton_vertex_kernel<<<1, kStepBlock, step_lds_bytes(bins), stream>>>(context, g);
   C8MI_HIP(hipGetLastError());
 }
 

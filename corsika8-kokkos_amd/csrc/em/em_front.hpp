@@ -62,12 +62,15 @@ struct FrontArgs {
   std::uint64_t track_stride;
   void* moliere_open;                 // split lepton fronts: up to `count` open Molière samples
   unsigned long long* moliere_open_count;  // zeroed by the move kernel
+  VertexRecord* open_vertices;        // vertices of the sources finished after the tail kernel
+  unsigned long long* open_vertex_count;  // zero between fronts
 };
 
 // States of a source between the split lepton kernels (FrontArgs::moved).
 inline constexpr std::uint32_t kMovedNone = 0;           // no move: the move kernel recorded a fallback
 inline constexpr std::uint32_t kMovedOk = 1;             // moved (and scattered, when Molière is on)
 inline constexpr std::uint32_t kMovedMoliereFailed = 2;  // the scattering failed: a transport fallback
+inline constexpr std::uint32_t kMovedOpen = 3;           // the sample continues in the tail kernel
 
 // Start of a shower in a slot: its parameters and first secondary history ID.
 struct SlotInit {
@@ -122,6 +125,11 @@ void launch_radio_segments(FrontArgs const& f, std::uint32_t slots, unsigned lon
 // (the step kernel, then the tail kernel for samples still open);
 // table: the interpolation-view sampler (else the analytical functions).
 void launch_lepton_moliere(EmContext const* context, bool table, FrontArgs const& f, hipStream_t stream);
+// The open Molière samples on a stream of their own, overlapping the finish
+// and vertex kernels of the other sources: tail kernel, their finish, their
+// vertices (FrontArgs::open_vertices).
+void launch_lepton_open(EmContext const* context, std::uint32_t bins, bool table, FrontArgs const& f,
+                        hipStream_t stream);
 void launch_lepton_finish(EmContext const* context, std::uint32_t bins, FrontArgs const& f, hipStream_t stream);
 void launch_lepton_vertex(EmContext const* context, std::uint32_t bins, FrontArgs const& f, hipStream_t stream);
 void launch_count(FrontArgs const& f, hipStream_t stream);

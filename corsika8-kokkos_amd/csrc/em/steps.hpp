@@ -497,7 +497,7 @@ struct MoliereOpen {
   std::uint32_t source;
   std::uint32_t stage;
   std::int32_t iterations;  // done so far for this sample
-  std::uint32_t pad;
+  std::uint32_t pad;        // the tail kernel's outcome (a front source state) for the open finish
   double current;           // the Newton iterate
   double first_angle;       // stage 2: the first projected angle
 };
