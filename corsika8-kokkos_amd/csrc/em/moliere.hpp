@@ -164,6 +164,8 @@ __device__ __forceinline__ double integral_f2_fit(double x) {
 __device__ __forceinline__ double f2_term(c8mi_moliere_snapshot const& m, double x, int i) {
   return m.c2_large[i] * (0.5 * log(x) + m.s2_large[i]) * pow(x, -(i + 0.5));
 }
+// pow(x, -i) with an int exponent is pown here (measured cheaper than pow with
+// a double exponent in the step kernel).
 __device__ __forceinline__ double integral_f2_term(c8mi_moliere_snapshot const& m, Quotients const& q, double x,
                                                    int i) {
   return q.i2_large[i] * (q.half_over[i] + 0.5 * log(x) + m.s2_large[i]) * pow(x, -i);
@@ -370,15 +372,22 @@ __device__ __forceinline__ int one_dimensional(c8mi_moliere_snapshot const& m, Q
 // component and iteration). Serially that is ~1 ms of one lane, and the whole
 // launch waits for it. The step kernels therefore stop after a few iterations
 // (EmContext::moliere_fast_iterations); a sample still open continues here
-// with its iterate: each evaluation spreads
-// the exp, erf, large-x functions and expansion terms of every component over
-// a workgroup of kTailThreads (LDS `jobs`), then every thread sums them in the
-// serial order, so density, CDF and iterates are bit-identical to the serial
-// evaluation.
-
-inline constexpr int kJobSlots = 26;  // per component: exp, erf, f1, integral f1, 11 + 11 terms
-// threads per open sample: one job each for the 4 x 26 jobs of a 4-component medium
-inline constexpr int kTailThreads = 128;
+// with its iterate: each evaluation spreads the transcendental parts of every
+// component over a workgroup of kTailThreads (LDS `jobs`), then every thread
+// combines them in the serial order, so density, CDF and iterates are
+// bit-identical to the serial evaluation.
+//
+// Jobs per component: 13 pow calls (x^1.5 and (1 - 4.5/x)^(2/3) of f1's
+// large-x form, the 11 terms of f2's expansion, each with its log), 11 pown
+// calls (the terms of the expansion of f2's integral, each with its log), then
+// exp(-x), erf(sqrt x) and f1's large-x integral. Each kind runs on a wave of
+// its own (pow jobs on the first, pown jobs on the second, the other three on
+// the third), so every wave runs one code path: a wave that mixed the kinds
+// ran every kind's code one after another (divergence), ~2x the time.
+inline constexpr int kPowJobs = 13;
+inline constexpr int kPownJobs = 11;
+inline constexpr int kJobSlots = kPowJobs + kPownJobs + 3;
+inline constexpr int kTailThreads = 3 * 64;
 
 // Horner of a 70-coefficient power series (the order of each chain of evaluate).
 __device__ __forceinline__ double series70(double const* c, double x) {
@@ -394,22 +403,40 @@ __device__ __forceinline__ void evaluate_wave(c8mi_moliere_snapshot const& m, Qu
   double const t2 = theta * theta;
   int const nc = static_cast<int>(m.component_count);
   double const f1_from = Tab ? kTableUpper : 12.;
-  __syncthreads();  // the previous evaluation has read `jobs`
-  for (int j = threadIdx.x; j < nc * kJobSlots; j += kTailThreads) {
-    int const k = j / kJobSlots, slot = j % kJobSlots;
+  auto scale_of = [&](int k) {  // st.inverse_scale[k] without a dynamic index into registers
     double scale = 0.;
 #pragma unroll
     for (int c = 0; c < N; ++c)
       if (c == k) scale = st.inverse_scale[c];
-    double const x = t2 * scale;
-    double v = 0.;
-    if (slot == 0) v = exp(-x);
-    else if (slot == 1) v = erf(sqrt(x > 0. ? x : 0.));
-    else if (slot == 2) v = x > f1_from ? f1_large(x) : 0.;
-    else if (slot == 3) v = x > f1_from ? integral_f1_large(m, x) : 0.;
-    else if (slot < 15) v = x > 6.5 * 6.5 ? f2_term(m, x, slot - 2) : 0.;
-    else v = x > 6.5 * 6.5 ? integral_f2_term(m, q, x, slot - 13) : 0.;
-    jobs[j] = v;
+    return scale;
+  };
+  __syncthreads();  // the previous evaluation has read `jobs`
+  int const wave = static_cast<int>(threadIdx.x) / 64, lane = static_cast<int>(threadIdx.x) % 64;
+  if (wave == 0) {
+    // slot s: 0 x^1.5, 1 (1 - 4.5/x)^(2/3), 2..12 f2_term(i = s)
+    if (lane < nc * kPowJobs) {
+      int const k = lane / kPowJobs, s = lane % kPowJobs;
+      double const x = t2 * scale_of(k);
+      double const base = s == 1 ? 1. - 4.5 / x : x;
+      double const p = pow(base, s == 0 ? 1.5 : s == 1 ? 2. / 3. : -(s + 0.5));
+      double const term = m.c2_large[s] * (0.5 * log(x) + m.s2_large[s]) * p;
+      jobs[k * kJobSlots + s] = s < 2 ? (x > f1_from ? p : 0.) : (x > 6.5 * 6.5 ? term : 0.);
+    }
+  } else if (wave == 1) {
+    // slot kPowJobs + s: integral_f2_term(i = s + 2)
+    if (lane < nc * kPownJobs) {
+      int const k = lane / kPownJobs, i = lane % kPownJobs + 2;
+      double const x = t2 * scale_of(k);
+      jobs[k * kJobSlots + kPowJobs + i - 2] = x > 6.5 * 6.5 ? integral_f2_term(m, q, x, i) : 0.;
+    }
+  } else if (lane < nc * 3) {
+    int const k = lane / 3, r = lane % 3;
+    double const x = t2 * scale_of(k);
+    double v;
+    if (r == 0) v = exp(-x);
+    else if (r == 1) v = erf(sqrt(x > 0. ? x : 0.));
+    else v = x > f1_from ? integral_f1_large(m, x) : 0.;
+    jobs[k * kJobSlots + kPowJobs + kPownJobs + r] = v;
   }
   __syncthreads();
   double dens = 0., cdf = 0.;
@@ -420,16 +447,18 @@ __device__ __forceinline__ void evaluate_wave(c8mi_moliere_snapshot const& m, Qu
     double const x = t2 * st.inverse_scale[k], ib = st.inverse_b[k], ib2 = ib * ib;
     double const root = sqrt(x > 0. ? x : 0.);
     double f1v, f2v, i1v, i2v;
+    // f1_large's expression over its two pow jobs
+    auto f1_above = [&] { return 0.5 * sqrt(kPi) / (J[0] * J[1]); };
     auto f2_above = [&] {
       if (x <= 6.5 * 6.5) return f2_fit(x);
       double sum = 0.;
-      for (int i = 2; i < 13; ++i) sum += J[2 + i];
+      for (int i = 2; i < 13; ++i) sum += J[i];
       return sum;
     };
     auto i2_above = [&] {
       if (x <= 6.5 * 6.5) return integral_f2_fit(x);
       double sum = 0.;
-      for (int i = 2; i < 13; ++i) sum += J[13 + i];
+      for (int i = 2; i < 13; ++i) sum += J[kPowJobs + i - 2];
       return sum;
     };
     if (Tab && x <= kTableUpper) {
@@ -448,18 +477,18 @@ __device__ __forceinline__ void evaluate_wave(c8mi_moliere_snapshot const& m, Qu
       i1v = cubic(2);
       i2v = cubic(3);
     } else if (Tab) {
-      f1v = J[2];
-      i1v = J[3];
+      f1v = f1_above();
+      i1v = J[kPowJobs + kPownJobs + 2];
       f2v = f2_above();
       i2v = i2_above();
     } else {
-      f1v = x > 12. ? J[2] : series70(m.c1, x);
-      i1v = x > 12. ? J[3] : series70(q.i1, x) * root;
+      f1v = x > 12. ? f1_above() : series70(m.c1, x);
+      i1v = x > 12. ? J[kPowJobs + kPownJobs + 2] : series70(q.i1, x) * root;
       f2v = x > 4.25 * 4.25 ? f2_above() : series70(m.c2, x);
       i2v = x > 4.25 * 4.25 ? i2_above() : series70(q.i2, x) * root;
     }
-    dens += st.density_weight[k] * (J[0] + f1v * ib + f2v * ib2);
-    cdf += m.components[k].weight_zz * (0.5 * J[1] + sqrt(1. / kPi) * (i1v * ib + i2v * ib2));
+    dens += st.density_weight[k] * (J[kPowJobs + kPownJobs] + f1v * ib + f2v * ib2);
+    cdf += m.components[k].weight_zz * (0.5 * J[kPowJobs + kPownJobs + 1] + sqrt(1. / kPi) * (i1v * ib + i2v * ib2));
   }
   density = dens * m.inverse_weight_zz_sum;
   cdf *= m.inverse_weight_zz_sum;
@@ -501,6 +530,7 @@ __device__ __forceinline__ int angle_2d_for(c8mi_moliere_snapshot const& m, Quot
 // Media of up to kDeviceComponents components (air has three; the backend
 // rejects larger Molière snapshots at create).
 inline constexpr int kDeviceComponents = 4;
+static_assert(kDeviceComponents * kPowJobs <= 64 && kDeviceComponents * kPownJobs <= 64 && kDeviceComponents * 3 <= 64);
 
 // apply_moliere_direction_cpu_step_compatible: normalize(tracked + rotate(start) - start).
 __device__ __forceinline__ int deflect_end(double const start[3], double const tracked[3], double angle, double au,
