@@ -47,8 +47,35 @@ AIR_MEAN_EXCITATION_MEV = 85.7e-6
 AIR_DENSITY_SEA_LEVEL = 1.205e-3
 
 
+class Medium:
+    """A medium of the closed-form tables: elements by mass fraction, mean
+    excitation energy, the density its LPM parameters refer to, molar mass and
+    the medium hash written into the table records."""
+
+    def __init__(self, components, mean_excitation_mev, density_g_per_cm3, molar_mass, medium_hash):
+        self.components = components
+        self.z_over_a = sum(w * z / a for _, z, a, w, _, _ in components)
+        self.electrons_per_g = AVOGADRO * self.z_over_a
+        self.mean_excitation_mev = mean_excitation_mev
+        self.density = density_g_per_cm3
+        self.molar_mass = molar_mass
+        self.medium_hash = medium_hash
+
+    def x0(self):
+        return 1.0 / sum(w / x0 for _, _, _, w, x0, _ in self.components)
+
+
+AIR_MEDIUM = Medium(AIR, AIR_MEAN_EXCITATION_MEV, AIR_DENSITY_SEA_LEVEL, 28.96, 0xA1)
+# Ice (H2O) by mass fraction, I = 75 eV, 0.92 g/cm^3
+ICE = [
+    ("H", 1.0, 1.008, 0.1119, 63.04, 0x4800000000000001),
+    ("O", 8.0, 15.999, 0.8881, 34.24, 0x4F00000000000008),
+]
+ICE_MEDIUM = Medium(ICE, 75.0e-6, 0.92, 18.015, 0xA2)
+
+
 def _x0_air():
-    return 1.0 / sum(w / x0 for _, _, _, w, x0, _ in AIR)
+    return AIR_MEDIUM.x0()
 
 
 def pair_dndx_scale(e):
@@ -80,18 +107,18 @@ def klein_nishina_dsigma_dv(e, v):
     return math.pi * RE_CM * RE_CM / k * (1.0 / eps + eps) * (1.0 - eps * sin2 / (1.0 + eps * eps))
 
 
-def restricted_dedx(e, cut_mev):
+def restricted_dedx(e, cut_mev, medium=AIR_MEDIUM):
     """Continuous loss (MeV cm^2/g) of an e+- of total energy e below the cut."""
     e = np.asarray(e, dtype=np.float64)
     gamma = e / ELECTRON_MASS_MEV
     beta2 = np.clip(1.0 - 1.0 / (gamma * gamma), 1e-12, None)
     kinetic = np.clip(e - ELECTRON_MASS_MEV, 1e-12, None)
     t_up = np.minimum(cut_mev, 0.5 * kinetic)
-    k_half = 0.5 * 0.307075 * AIR_Z_OVER_A  # K/2 Z/A, MeV cm^2/g
-    arg = 2.0 * ELECTRON_MASS_MEV * beta2 * gamma * gamma * t_up / (AIR_MEAN_EXCITATION_MEV ** 2)
+    k_half = 0.5 * 0.307075 * medium.z_over_a  # K/2 Z/A, MeV cm^2/g
+    arg = 2.0 * ELECTRON_MASS_MEV * beta2 * gamma * gamma * t_up / (medium.mean_excitation_mev ** 2)
     ionization = k_half / beta2 * (np.log(np.clip(arg, 1.0 + 1e-12, None)) - beta2)
     vc = np.minimum(cut_mev / e, 1.0)
-    brems = e / _x0_air() * ((4.0 / 3.0) * vc - (2.0 / 3.0) * vc * vc + vc ** 3 / 3.0)
+    brems = e / medium.x0() * ((4.0 / 3.0) * vc - (2.0 / 3.0) * vc * vc + vc ** 3 / 3.0)
     return np.clip(ionization, 0.05, None) + brems
 
 
@@ -113,7 +140,7 @@ def _rellog(vc, vmax, t):
     return np.where(vmax > vc, out, vc)
 
 
-def _compton_cdf_function(nodes_per_unit=4000):
+def _compton_cdf_function(nodes_per_unit=4000, medium=AIR_MEDIUM):
     """Cumulative Klein-Nishina rate in the one-minus-log loss coordinate."""
     tt = np.linspace(0.0, 1.0, nodes_per_unit + 1)
 
@@ -131,7 +158,7 @@ def _compton_cdf_function(nodes_per_unit=4000):
             g = klein_nishina_dsigma_dv(ev, v) * dvdt
             cum = np.concatenate(([0.0], np.cumsum(0.5 * (g[1:] + g[:-1]) * np.diff(tt))))
             m = flat_e == ev
-            flat_o[m] = np.interp(np.clip(flat_t[m], 0.0, 1.0), tt, cum) * AIR_ELECTRONS_PER_G
+            flat_o[m] = np.interp(np.clip(flat_t[m], 0.0, 1.0), tt, cum) * medium.electrons_per_g
         return out
 
     return f
@@ -141,24 +168,41 @@ def build_air_tables(cut_mev: float = 1.0, e_max_mev: float = 2.0e9, energy_node
                      loss_nodes: int = 64, polynomial: bool = True):
     """Air tables for photons and e+- with an e+- energy-loss cut of `cut_mev`
     (stochastic losses above it, continuous below)."""
+    return build_tables(AIR_MEDIUM, cut_mev, e_max_mev, energy_nodes, loss_nodes, polynomial)
+
+
+def build_ice_tables(cut_mev: float = 0.1, e_max_mev: float = 2.0e9, energy_nodes: int = 160,
+                     loss_nodes: int = 64, polynomial: bool = True):
+    """Ice (H2O, 0.92 g/cm^3) tables, by default with the 100 keV cut of the
+    in-ice cascade."""
+    return build_tables(ICE_MEDIUM, cut_mev, e_max_mev, energy_nodes, loss_nodes, polynomial)
+
+
+def build_tables(medium, cut_mev, e_max_mev=2.0e9, energy_nodes=160, loss_nodes=64, polynomial=True):
+    """Tables of `medium` for photons and e+-. Below a 0.5 MeV cut the e+-
+    process tables start at 0.55 MeV total energy (else 1 MeV), so every
+    tracked e+- above the cut lies on them."""
+    elements = medium.components
+    electrons_per_g = medium.electrons_per_g
+    lepton_low = 1.0 if cut_mev >= 0.5 else 0.55
     b = T.TableBuilder(constants=dict(T.CONSTANTS, electron_mass_mev=ELECTRON_MASS_MEV))
     me = ELECTRON_MASS_MEV
     lin = T.axis(T.AXIS_LINEAR, 0.0, 1.0, loss_nodes)
     photon_low = 0.05
     ax_pair = T.axis(T.AXIS_EXP, 1.1, e_max_mev, energy_nodes)
     ax_photon = T.axis(T.AXIS_EXP, photon_low, e_max_mev, energy_nodes + 20)
-    ax_lepton = T.axis(T.AXIS_EXP, 1.0, e_max_mev, energy_nodes)
+    ax_lepton = T.axis(T.AXIS_EXP, lepton_low, e_max_mev, energy_nodes)
     kin = T.KIN
     proc = T.PROCESS
     ordinal = {22: 0, 11: 0, -11: 0}
 
     def add(pdg, **fields):
-        b.add_dndx(pdg_id=pdg, selection_ordinal=ordinal[pdg], medium_hash=0xA1,
-                   medium_mean_excitation_energy_mev=AIR_MEAN_EXCITATION_MEV, **fields)
+        b.add_dndx(pdg_id=pdg, selection_ordinal=ordinal[pdg], medium_hash=medium.medium_hash,
+                   medium_mean_excitation_energy_mev=medium.mean_excitation_mev, **fields)
         ordinal[pdg] += 1
 
     # photon pair production, per element
-    for name, z, a, w, x0, h in AIR:
+    for name, z, a, w, x0, h in elements:
         def f(e, t, w=w, x0=x0):
             xmin = me / e
             xmax = 1.0 - xmin
@@ -174,17 +218,17 @@ def build_air_tables(cut_mev: float = 1.0, e_max_mev: float = 2.0e9, energy_node
     # Compton on the medium electrons
     add(22, process_id=proc["compton"], component_hash=0, kinematic_model=kin["compton"],
         loss_transform=T.LOSS_ONEMINUSLOG, rate_model=T.RATE_BICUBIC,
-        spline=b.bicubic(ax_photon, lin, _compton_cdf_function(), polynomial=polynomial),
+        spline=b.bicubic(ax_photon, lin, _compton_cdf_function(medium=medium), polynomial=polynomial),
         component_weight=1.0, particle_mass_mev=0.0, lower_energy_limit_mev=photon_low)
     # photoelectric, per element (Sauter direct rate)
-    for name, z, a, w, x0, h in AIR:
+    for name, z, a, w, x0, h in elements:
         add(22, process_id=proc["photoelectric"], component_hash=h, kinematic_model=kin["only_stochastic"],
             loss_transform=T.LOSS_RELLOG, rate_model=T.RATE_SAUTER, component_weight=w,
             particle_mass_mev=0.0, component_nuclear_charge=z, component_atomic_mass=a,
             direct_rate_factor=AVOGADRO * w / a, lower_energy_limit_mev=1e-3)
 
     for pdg in (11, -11):
-        for name, z, a, w, x0, h in AIR:
+        for name, z, a, w, x0, h in elements:
             def f(e, t, w=w, x0=x0):
                 vc, vmax = _limits_brems(e, cut_mev)
                 v = _rellog(vc, vmax, t)
@@ -193,13 +237,13 @@ def build_air_tables(cut_mev: float = 1.0, e_max_mev: float = 2.0e9, energy_node
                 loss_transform=T.LOSS_RELLOG, rate_model=T.RATE_BICUBIC,
                 spline=b.bicubic(ax_lepton, lin, f, polynomial=polynomial), component_weight=w,
                 particle_mass_mev=me, component_nuclear_charge=z, component_atomic_mass=a,
-                absolute_cut_mev=cut_mev, relative_cut=1.0, lower_energy_limit_mev=1.0)
+                absolute_cut_mev=cut_mev, relative_cut=1.0, lower_energy_limit_mev=lepton_low)
         bhabha = pdg == -11
 
         def g(e, t, bhabha=bhabha):
             vc, vmax = _limits_moller(e, cut_mev, bhabha)
             v = _rellog(vc, vmax, t)
-            k = 2.0 * math.pi * RE_CM * RE_CM * me * AIR_ELECTRONS_PER_G
+            k = 2.0 * math.pi * RE_CM * RE_CM * me * electrons_per_g
             with np.errstate(divide="ignore", invalid="ignore"):
                 val = k / e * (1.0 / vc - 1.0 / v)
             return np.where(vmax > vc, val, 0.0)
@@ -207,36 +251,36 @@ def build_air_tables(cut_mev: float = 1.0, e_max_mev: float = 2.0e9, energy_node
             kinematic_model=kin["bhabha" if bhabha else "moller"], loss_transform=T.LOSS_RELLOG,
             rate_model=T.RATE_BICUBIC, spline=b.bicubic(ax_lepton, lin, g, polynomial=polynomial),
             component_weight=1.0, particle_mass_mev=me, absolute_cut_mev=cut_mev, relative_cut=1.0,
-            lower_energy_limit_mev=1.0)
+            lower_energy_limit_mev=lepton_low)
         if bhabha:
             add(pdg, process_id=proc["annihilation"], component_hash=0,
                 kinematic_model=kin["only_stochastic"], loss_transform=T.LOSS_RELLOG,
                 rate_model=T.RATE_HEITLER, component_weight=1.0, particle_mass_mev=me,
-                direct_rate_factor=math.pi * RE_CM * RE_CM * AIR_ELECTRONS_PER_G,
+                direct_rate_factor=math.pi * RE_CM * RE_CM * electrons_per_g,
                 lower_energy_limit_mev=me)
 
     # total rates: sum of the analytic column rates at the nodes
     def photon_total(e):
         e = np.asarray(e, dtype=np.float64)
-        pair = sum((w / x0) * (7.0 / 9.0) for _, _, _, w, x0, _ in AIR) * pair_dndx_scale(e)
+        pair = sum((w / x0) * (7.0 / 9.0) for _, _, _, w, x0, _ in elements) * pair_dndx_scale(e)
         pair = np.where(e >= 1.1, pair, 0.0)
-        compton = _compton_cdf_function()(e, np.ones_like(e))
-        photo = sum(_sauter_rate(e, z, AVOGADRO * w / a) for _, z, a, w, _, _ in AIR)
+        compton = _compton_cdf_function(medium=medium)(e, np.ones_like(e))
+        photo = sum(_sauter_rate(e, z, AVOGADRO * w / a) for _, z, a, w, _, _ in elements)
         return pair + compton + photo
 
     def lepton_total(e, positron):
         e = np.asarray(e, dtype=np.float64)
         out = np.zeros_like(e)
-        for _, _, _, w, x0, _ in AIR:
+        for _, _, _, w, x0, _ in elements:
             vc, vmax = _limits_brems(e, cut_mev)
             out += (w / x0) * np.where(vmax > vc, brems_cdf(vmax, np.where(vc > 0, vc, 1.0)), 0.0)
         vc, vmax = _limits_moller(e, cut_mev, positron)
-        k = 2.0 * math.pi * RE_CM * RE_CM * me * AIR_ELECTRONS_PER_G
+        k = 2.0 * math.pi * RE_CM * RE_CM * me * electrons_per_g
         with np.errstate(divide="ignore", invalid="ignore"):
             out += np.where(vmax > vc, k / e * (1.0 / vc - 1.0 / vmax), 0.0)
         if positron:
-            out += _heitler_rate(e, math.pi * RE_CM * RE_CM * AIR_ELECTRONS_PER_G)
-        out = np.where(e >= 1.0, out, 0.0)
+            out += _heitler_rate(e, math.pi * RE_CM * RE_CM * electrons_per_g)
+        out = np.where(e >= lepton_low, out, 0.0)
         return out
 
     tax_photon = T.axis(T.AXIS_EXP, 0.01, e_max_mev, 400)
@@ -248,7 +292,7 @@ def build_air_tables(cut_mev: float = 1.0, e_max_mev: float = 2.0e9, energy_node
     # continuous loss and range (utility) of e+-
     ulow = 0.6
     grid = np.exp(np.linspace(math.log(ulow), math.log(e_max_mev), 20001))
-    inv = 1.0 / restricted_dedx(grid, cut_mev)
+    inv = 1.0 / restricted_dedx(grid, cut_mev, medium)
     rng = np.concatenate(([0.0], np.cumsum(0.5 * (inv[1:] + inv[:-1]) * np.diff(grid))))
     lg = np.log(grid)
 
@@ -258,7 +302,7 @@ def build_air_tables(cut_mev: float = 1.0, e_max_mev: float = 2.0e9, energy_node
 
     uax = T.axis(T.AXIS_EXP, ulow, e_max_mev, 600)
     for k, pdg in enumerate((11, -11)):
-        b.add_dedx(pdg, proc["ionization"], b.cubic(uax, lambda e: restricted_dedx(e, cut_mev)),
+        b.add_dedx(pdg, proc["ionization"], b.cubic(uax, lambda e: restricted_dedx(e, cut_mev, medium)),
                    lower=ulow, ordinal=k)
         b.add_utility(pdg, b.cubic(uax, range_of), mass=me, lower=ulow)
     return b.build()
@@ -295,26 +339,27 @@ def _heitler_rate(e, factor):
     return np.where(g > 1.0, out, 0.0)
 
 
-def photon_pair_lpm_snapshot():
-    """Photon-pair LPM parameters of sea-level air (PROPOSAL's PhotoPairLPM inputs)."""
-    x0_cm = _x0_air() / AIR_DENSITY_SEA_LEVEL
+def photon_pair_lpm_snapshot(medium=AIR_MEDIUM):
+    """Photon-pair LPM parameters (PROPOSAL's PhotoPairLPM inputs) of the medium
+    at its reference density (sea-level air by default)."""
+    x0_cm = medium.x0() / medium.density
     e_lpm = 7.7e6 * x0_cm  # MeV: 7.7 TeV per cm of radiation length
-    mol_mass = 28.96
-    return dict(baseline_mass_density_g_per_cm3=AIR_DENSITY_SEA_LEVEL,
-                molecular_density_per_cm3=AIR_DENSITY_SEA_LEVEL * AVOGADRO / mol_mass,
-                sum_charge=AIR_Z_OVER_A * mol_mass, e_lpm_mev=e_lpm,
+    mol_mass = medium.molar_mass
+    return dict(baseline_mass_density_g_per_cm3=medium.density,
+                molecular_density_per_cm3=medium.density * AVOGADRO / mol_mass,
+                sum_charge=medium.z_over_a * mol_mass, e_lpm_mev=e_lpm,
                 classical_electron_radius_cm=RE_CM, fine_structure_constant=ALPHA,
-                components=[(h, z, 183.0) for _, z, _, _, _, h in AIR])
+                components=[(h, z, 183.0) for _, z, _, _, _, h in medium.components])
 
 
-def brems_lpm_snapshot():
-    s = photon_pair_lpm_snapshot()
+def brems_lpm_snapshot(medium=AIR_MEDIUM):
+    s = photon_pair_lpm_snapshot(medium)
     return dict(baseline_mass_density_g_per_cm3=s["baseline_mass_density_g_per_cm3"],
                 molecular_density_per_cm3=s["molecular_density_per_cm3"], sum_charge=s["sum_charge"],
                 e_lpm_mev=s["e_lpm_mev"], lepton_mass_mev=0.5109989461,
                 electron_mass_mev=0.5109989461, muon_mass_mev=105.6583745,
                 classical_electron_radius_cm=RE_CM, fine_structure_constant=ALPHA,
-                components=[(h, z, a, 183.0) for _, z, a, _, _, h in AIR])
+                components=[(h, z, a, 183.0) for _, z, a, _, _, h in medium.components])
 
 
 # ---- Molière multiple scattering (the MoliereSnapshot of the air medium) ----
@@ -356,8 +401,8 @@ def moliere_coefficients():
     return tuple(c1), tuple(c2), tuple(c2l), tuple(s2l), tuple(c1il)
 
 
-def moliere_snapshot(particle_mass_mev=ELECTRON_MASS_MEV):
-    """Electron Molière snapshot of the air medium. Component fields are derived
+def moliere_snapshot(particle_mass_mev=ELECTRON_MASS_MEV, medium=AIR_MEDIUM):
+    """Electron Molière snapshot of the medium (air by default). Component fields are derived
     as the reference's auxiliary export derives them from a medium
     (ProposalNativeAux.cpp:99-135): mass weights, weight * Z^2, the mean
     Z(Z+1)/A for electrons, chi_0 = m_e alpha (128 Z / (9 pi^2))^(1/3) and the
@@ -368,7 +413,7 @@ def moliere_snapshot(particle_mass_mev=ELECTRON_MASS_MEV):
     mean_zz = 0.0
     weight_zz_sum = 0.0
     weights = []
-    for _, z, a, w, _, _ in AIR:
+    for _, z, a, w, _, _ in medium.components:
         weights.append(w)
         mean_a += w * a
         wzz = w * z * z
@@ -386,3 +431,20 @@ def moliere_snapshot(particle_mass_mev=ELECTRON_MASS_MEV):
                 inverse_weight_zz_sum=1.0 / weight_zz_sum, euler_mascheroni=EULER_MASCHERONI,
                 maximum_weight_index=maximum, components=components, c1=c1, c2=c2, c2_large=c2l,
                 s2_large=s2l, c1_integral_large=c1il)
+
+
+class IceMedium:
+    """The ice medium's snapshots in the form make_config takes (this module's
+    functions with medium=ICE_MEDIUM); tables: build_ice_tables()."""
+
+    @staticmethod
+    def moliere_snapshot(particle_mass_mev=ELECTRON_MASS_MEV):
+        return moliere_snapshot(particle_mass_mev, ICE_MEDIUM)
+
+    @staticmethod
+    def photon_pair_lpm_snapshot():
+        return photon_pair_lpm_snapshot(ICE_MEDIUM)
+
+    @staticmethod
+    def brems_lpm_snapshot():
+        return brems_lpm_snapshot(ICE_MEDIUM)

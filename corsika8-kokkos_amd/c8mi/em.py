@@ -290,10 +290,12 @@ class Atmosphere:
     """Layered spherical atmosphere (LayeredSphericalAtmosphereBuilder) as an
     environment tree: universe > outermost layer > ... > innermost layer; each
     layer a sphere around the Earth centre with an exponential (sliding planar)
-    or homogeneous medium model."""
+    or homogeneous medium model. `magnetic_field_t`: a uniform field (tesla) in
+    every layer (e+- then follow leapfrog steps)."""
 
     def __init__(self, layers=USSTDBK, observation_height_m: float = 0.0,
-                 earth_radius_m: float = EARTH_RADIUS_M, observation: bool = True):
+                 earth_radius_m: float = EARTH_RADIUS_M, observation: bool = True,
+                 magnetic_field_t=(0.0, 0.0, 0.0)):
         self.earth_center = np.array([0.0, 0.0, -earth_radius_m])
         n = len(layers)
         self.models = (EnvModel * n)()
@@ -322,6 +324,8 @@ class Atmosphere:
                 m.anchor_m[a] = self.earth_center[a]
             m.axis[2] = 1.0
             m.medium_id = 1
+            for a in range(3):
+                m.magnetic_field_t[a] = magnetic_field_t[a]
             node = self.nodes[k]
             node.shape_kind = SHAPE_SPHERE
             node.parent = k - 1
@@ -378,6 +382,60 @@ class Atmosphere:
         rho = self.density(start[None, :] + s[:, None] * d[None, :])
         cum = np.concatenate(([0.0], np.cumsum(0.5 * (rho[1:] + rho[:-1]) * np.diff(s)))) * 100.0
         return np.ascontiguousarray(cum[::fine][:n])
+
+
+class IceVolume:
+    """In-ice scene (applications/c8_ice_cascade.cpp): the universe holding one
+    homogeneous ice sphere around the Earth centre whose surface is z = 0
+    (0.92 g/cm^3 by default, no magnetic field); the observation plane
+    `observation_depth_m` below the surface, or none."""
+
+    def __init__(self, density_g_per_cm3: float = 0.92, observation_depth_m=None,
+                 earth_radius_m: float = EARTH_RADIUS_M):
+        self.earth_center = np.array([0.0, 0.0, -earth_radius_m])
+        self.rho = density_g_per_cm3
+        self.models = (EnvModel * 1)()
+        self.nodes = (EnvNode * 2)()
+        self.children = (c_int32 * 1)(1)
+        m = self.models[0]
+        m.density_kind = DENSITY_HOMOGENEOUS
+        m.reference_density_g_per_cm3 = density_g_per_cm3
+        m.medium_id = 2
+        m.axis[2] = 1.0
+        for a in range(3):
+            m.anchor_m[a] = self.earth_center[a]
+        n = self.nodes
+        n[0].shape_kind, n[0].parent, n[0].model, n[0].child_begin, n[0].child_count = SHAPE_UNIVERSE, -1, -1, 0, 1
+        n[1].shape_kind, n[1].parent, n[1].model = SHAPE_SPHERE, 0, 0
+        for a in range(3):
+            n[1].center_m[a] = self.earth_center[a]
+        n[1].radius_m = earth_radius_m
+        s = self.struct = Environment()
+        s.nodes, s.node_count, s.children, s.child_index_count = n, 2, self.children, 1
+        s.models, s.model_count = self.models, 1
+        s.observation_enabled = 0 if observation_depth_m is None else 1
+        s.observation_point_m[2] = -(observation_depth_m or 0.0)
+        s.observation_normal[2] = 1.0
+        for a in range(3):
+            s.earth_center_m[a] = self.earth_center[a]
+        s.maximum_magnetic_deflection_rad = 0.2
+        self.earth_radius = earth_radius_m
+
+    def density(self, p):
+        p = np.asarray(p, dtype=np.float64)
+        inside = np.linalg.norm(p - self.earth_center, axis=-1) < self.earth_radius
+        return np.where(inside, self.rho, 0.0)
+
+    def axis_table(self, start, direction, step_m=10.0, length_m=400.0):
+        """Grammage along the axis at `step_m` nodes (homogeneous: rho s)."""
+        n = int(math.ceil(length_m / step_m)) + 2
+        s = np.arange(n) * step_m
+        d = np.asarray(direction, dtype=np.float64)
+        d = d / np.linalg.norm(d)
+        rho = self.density(np.asarray(start, dtype=np.float64)[None, :] + s[:, None] * d[None, :])
+        if not np.all(rho == self.rho):
+            raise ValueError("the axis leaves the ice within the table")
+        return np.ascontiguousarray(s * self.rho * 100.0)
 
 
 def particle(pid, energy_gev, position, direction, history_id=1, weight=1.0, node=-3):

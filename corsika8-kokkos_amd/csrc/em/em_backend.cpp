@@ -55,8 +55,9 @@ void validate_environment(c8mi_environment const& e) {
             "unsupported density model");
     require(std::isfinite(m.reference_density_g_per_cm3) && m.reference_density_g_per_cm3 > 0.,
             "reference density");
-    require(m.magnetic_field_t[0] == 0. && m.magnetic_field_t[1] == 0. && m.magnetic_field_t[2] == 0.,
-            "magnetic fields are not transported by this build (the field must be zero)");
+    require(std::isfinite(m.magnetic_field_t[0]) && std::isfinite(m.magnetic_field_t[1]) &&
+                std::isfinite(m.magnetic_field_t[2]),
+            "magnetic field must be finite");
   }
   if (e.observation_enabled) {
     double const n2 = e.observation_normal[0] * e.observation_normal[0] +

@@ -246,22 +246,4 @@ __device__ __forceinline__ double observation_distance(EmContext const& c, doubl
   return d;
 }
 
-// Field-free lepton distance to the observation plane: the linear root of the
-// separation-plane quadratic in CurvedBoundaryAccess, in time units and back.
-__device__ __forceinline__ double lepton_observation_distance(EmContext const& c, double const p[3],
-                                                     double const dir[3], double speed,
-                                                     double momentum_si) {
-  double const delta[3] = {p[0] - c.observation_point[0], p[1] - c.observation_point[1],
-                           p[2] - c.observation_point[2]};
-  double const b = 2 * momentum_si * dot3(dir, c.observation_normal);
-  double const cc = 2 * momentum_si * dot3(c.observation_normal, delta);
-  double first = __builtin_inf();
-  if (b != 0.) {
-    double const root = -cc / b;
-    if (root >= 0. && root < first) first = root;
-  }
-  double const entry = first / speed;
-  return entry > 0. ? entry * speed : __builtin_inf();
-}
-
 }  // namespace c8mi::em::env

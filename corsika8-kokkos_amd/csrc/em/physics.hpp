@@ -12,7 +12,7 @@
 //   tables/{DiscretePhysicsQueries,ContinuousPhysicsQueries,TableQueryResult}.inl,
 //   corsika/detail/kokkos/output/algorithms/{ProfileAccumulation,ProfileProjection}.inl.
 // Muons and the e+- pair (electron_pair) final state are handed to the host as
-// fallback requests; magnetic fields must be zero.
+// fallback requests.
 #pragma once
 
 #include <hip/hip_runtime.h>

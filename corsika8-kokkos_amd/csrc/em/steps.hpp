@@ -7,6 +7,7 @@
 // scatter kernel replaces it with first_history + child offset + k.
 #pragma once
 
+#include "em/magnetic.hpp"
 #include "em/moliere.hpp"
 #include "em/physics.hpp"
 
@@ -333,24 +334,22 @@ __device__ __forceinline__ bool lepton_move(EmContext const& c, Accumulator cons
     limit = kLlCut;
     cut_deposit = (e0 - tm) / 1000.;
   } else {
-    c8mi_env_model const& model = *env::model_of(c, layer.node);
-    if (model.magnetic_field_t[0] != 0. || model.magnetic_field_t[1] != 0. || model.magnetic_field_t[2] != 0.)
-      return transport_fallback(kFrMagneticTransport);
+    // maximum_uniform_magnetic_step, lepton_boundary, lepton_observation_distance
+    // (LeptonTransportStep.inl:119-160)
+    double const charge = start.pid > 0 ? -1. : 1.;
+    double const* field = env::model_of(c, layer.node)->magnetic_field_t;
     double const m_gev = tm / 1000.;
-    if (!isfinite(start.energy_gev) || !(start.energy_gev > m_gev) || !env::normalized(start.direction))
+    mag::Limit const mlimit = mag::step_limit(start, m_gev, charge, field, c.maximum_magnetic_deflection);
+    if (mlimit.status == mag::kInvalidInput || mlimit.status == mag::kNonFinite)
       return transport_fallback(kFrMagneticTransport);
-    double const p_gev = sqrt((start.energy_gev - m_gev) * (start.energy_gev + m_gev));
-    double const speed = kC * p_gev / start.energy_gev;
-    double const momentum_si = p_gev * kElementaryCharge / kRigidity;
-    double const vel[3] = {speed * start.direction[0], speed * start.direction[1], speed * start.direction[2]};
-    double param;
+    double geometry;
     std::int32_t next_node;
-    env::next_boundary(c, start.position_m, vel, start.environment_node, __builtin_inf() / speed, param, next_node);
-    double geometry = param * speed;
+    if (!mag::lepton_boundary(c, start, m_gev, charge, field, mlimit, geometry, next_node))
+      return transport_fallback(kFrMagneticBoundary);
     bool magnetic_wins = next_node == start.environment_node;
     bool obs_wins = false;
     if (c.observation_enabled) {
-      double const od = env::lepton_observation_distance(c, start.position_m, start.direction, speed, momentum_si);
+      double const od = mag::lepton_observation_distance(c, start, m_gev, charge, field, mlimit);
       if (od < geometry) {
         geometry = od;
         obs_wins = true;
@@ -374,21 +373,11 @@ __device__ __forceinline__ bool lepton_move(EmContext const& c, Accumulator cons
       distance = d.value;
     }
     bool const observation_reached = obs_wins && !continuous_wins && !interaction_wins && !magnetic_wins;
-    if (distance != 0.) {
-      double disp[3], kicked[3];
-      for (int a = 0; a < 3; ++a) {
-        kicked[a] = start.direction[a] + 0.;
-        disp[a] = distance * (start.direction[a] + 0.5 * 0.);
-        end.position_m[a] = start.position_m[a] + disp[a];
-      }
-      double const kn = sqrt(env::dot3(kicked, kicked));
-      double const chord = sqrt(env::dot3(disp, disp));
-      double const beta = p_gev / start.energy_gev;
-      if (!isfinite(kn) || !(kn > 0.) || !isfinite(chord) || !isfinite(beta) || !(beta > 0.))
-        return transport_fallback(kFrMagneticTransport);
-      for (int a = 0; a < 3; ++a) end.direction[a] = kicked[a] / kn;
-      end.time_s = start.time_s + distance / (beta * kC);
-    }
+    // advance_uniform_magnetic_field, with no field where the limit was Linear
+    double const zero_field[3] = {0., 0., 0.};
+    int const advanced = mag::advance(start, m_gev, charge, mlimit.status == mag::kLinear ? zero_field : field,
+                                      distance, end);
+    if (advanced == mag::kInvalidInput || advanced == mag::kNonFinite) return transport_fallback(kFrMagneticTransport);
     double chord_dir[3], chord2 = 0.;
     for (int a = 0; a < 3; ++a) {
       double const dd = end.position_m[a] - start.position_m[a];

@@ -80,7 +80,7 @@ typedef struct c8mi_env_model {
   double reference_height_m;
   double anchor_m[3];
   double axis[3];
-  double magnetic_field_t[3]; /* uniform B; must be zero in this build */
+  double magnetic_field_t[3]; /* uniform B (tesla) of the model's volume; finite */
 } c8mi_env_model;
 
 /* One volume of the environment tree (EnvironmentTreeView): its shape, parent,

@@ -387,24 +387,514 @@ static double observation_distance(const c8mi_environment* e, const double p[3],
   return d;
 }
 
-/* Straight-line lepton observation distance with zero field: the separation-plane
- * branch of CurvedBoundaryAccess::intersectShape (CurvedBoundaryAccess.inl:50-66)
- * reduces to the linear root -c/b of quadratic_roots (PolynomialRoots.inl:98-116),
- * scaled by 1/speed and back (EnvironmentLeptonBoundary.inl:62-79). */
-static double lepton_observation_distance(const c8mi_environment* e, const ps_t* s, double speed,
-                                          double momentum_si) {
-  double delta[3] = {s->position_m[0] - e->observation_point_m[0],
-                     s->position_m[1] - e->observation_point_m[1],
-                     s->position_m[2] - e->observation_point_m[2]};
-  double b = 2 * momentum_si * dot3(s->direction, e->observation_normal);
-  double c = 2 * momentum_si * dot3(e->observation_normal, delta);
-  double first = INFINITY;
-  if (b != 0.) {
-    double root = -c / b;
-    if (root >= 0. && root < first) first = root;
+/* ====================== uniform magnetic field ====================== */
+/* maximum_uniform_magnetic_step / advance_uniform_magnetic_field
+ * (UniformMagneticField.inl:101-195), lepton_boundary /
+ * lepton_observation_distance (EnvironmentLeptonBoundary.inl:16-80) over
+ * CurvedBoundaryAccess::intersectShape (CurvedBoundaryAccess.inl:38-126), and
+ * the root finders (PolynomialRoots.inl:98-134, StableQuartic.inl) in the
+ * double-double arithmetic of the reference's device build
+ * (ExtendedPrecision.inl). */
+
+enum { MAG_SUCCESS = 0, MAG_LINEAR = 1, MAG_INVALID = 2, MAG_NONFINITE = 3 };
+
+typedef struct {
+  double hi, lo;
+} dd_t;
+
+static dd_t dd(double v) {
+  dd_t r = {v, 0.};
+  return r;
+}
+static double dd_double(dd_t a) { return a.hi + a.lo; }
+static dd_t dd_renorm(double a, double b) {
+  double s = a + b;
+  dd_t r = {s, b - (s - a)};
+  return r;
+}
+static dd_t dd_add(dd_t a, dd_t b) {
+  double s = a.hi + b.hi;
+  double bv = s - a.hi;
+  double e = (a.hi - (s - bv)) + (b.hi - bv);
+  return dd_renorm(s, e + (a.lo + b.lo));
+}
+static dd_t dd_neg(dd_t a) {
+  dd_t r = {-a.hi, -a.lo};
+  return r;
+}
+static dd_t dd_sub(dd_t a, dd_t b) { return dd_add(a, dd_neg(b)); }
+static dd_t dd_mul(dd_t a, dd_t b) {
+  double p = a.hi * b.hi;
+  double e = fma(a.hi, b.hi, -p);
+  e = e + (a.hi * b.lo + a.lo * b.hi);
+  e = e + a.lo * b.lo;
+  return dd_renorm(p, e);
+}
+static dd_t dd_div(dd_t a, dd_t b) {
+  double q = a.hi / b.hi;
+  if (!finite(q)) return dd(q);
+  dd_t r = dd_sub(a, dd_mul(b, dd(q)));
+  double c = r.hi / b.hi;
+  r = dd_sub(r, dd_mul(b, dd(c)));
+  return dd_add(dd_add(dd(q), dd(c)), dd(r.hi / b.hi));
+}
+static int dd_lt(dd_t a, dd_t b) { return a.hi < b.hi || (a.hi == b.hi && a.lo < b.lo); }
+static int dd_eq(dd_t a, dd_t b) { return a.hi == b.hi && a.lo == b.lo; }
+static dd_t dd_abs(dd_t a) { return dd_lt(a, dd(0.)) ? dd_neg(a) : a; }
+static dd_t dd_sqrt(dd_t v) {
+  double a = sqrt(v.hi);
+  if (a == 0. || !finite(a)) return dd(a);
+  return dd_add(dd(a), dd_div(dd_sub(v, dd_mul(dd(a), dd(a))), dd(2. * a)));
+}
+static dd_t dd_cbrt(dd_t v) {
+  dd_t a = dd(cbrt(v.hi));
+  if (a.hi == 0. || !finite(a.hi)) return a;
+  for (int it = 0; it < 2; ++it)
+    a = dd_add(a, dd_div(dd_sub(v, dd_mul(dd_mul(a, a), a)), dd_mul(dd_mul(dd(3.), a), a)));
+  return a;
+}
+static dd_t dd_acos(dd_t v) {
+  double angle = acos(v.hi);
+  if (v.lo == 0.) return dd(angle);
+  if (v.hi == 1.) return dd_sqrt(dd_mul(dd(2.), dd_sub(dd(1.), v)));
+  if (v.hi == -1.) return dd_sub(dd(3.14159265358979323846), dd_sqrt(dd_mul(dd(2.), dd_add(dd(1.), v))));
+  return dd_sub(dd(angle), dd(v.lo / sqrt((1. - v.hi) * (1. + v.hi))));
+}
+static dd_t dd_cos(dd_t v) { return dd_sub(dd(cos(v.hi)), dd(sin(v.hi) * v.lo)); }
+static dd_t dd_fma(dd_t a, dd_t b, dd_t c) { return dd_add(dd_mul(a, b), c); }
+
+typedef struct {
+  double v[4];
+  unsigned n;
+} roots_t;
+
+static void roots_append(roots_t* r, double x) { r->v[r->n++] = x; }
+static void roots_append_distinct(roots_t* r, double x) {
+  for (unsigned i = 0; i < r->n; ++i)
+    if (fabs(x - r->v[i]) <= 8 * 2.220446049250313e-16 * fmax(fabs(x), fabs(r->v[i]))) return;
+  if (r->n < 4) roots_append(r, x);
+}
+
+/* quadratic_roots (PolynomialRoots.inl:104-134), epsilon 1e-12 */
+static roots_t quadratic_roots_dd(dd_t a, dd_t b, dd_t c) {
+  const double epsilon = 1.e-12;
+  roots_t r = {{0., 0., 0., 0.}, 0u};
+  if (dd_lt(dd_abs(a), dd(epsilon))) {
+    double bb = dd_double(b), cc = dd_double(c);
+    if (bb != 0.) roots_append(&r, -cc / bb);
+    return r;
   }
-  double entry = first / speed;
-  return entry > 0. ? entry * speed : INFINITY;
+  if (dd_lt(dd_abs(c), dd(epsilon))) {
+    double aa = dd_double(a), bb = dd_double(b);
+    if (aa != 0.) roots_append(&r, -bb / aa);
+    roots_append(&r, 0.);
+    return r;
+  }
+  dd_t a4 = dd_mul(a, dd(4.));
+  dd_t multiplied = dd_mul(a4, c);
+  dd_t error = dd_fma(a4, c, dd_neg(multiplied));
+  dd_t corrected = dd_fma(b, b, dd_neg(multiplied));
+  dd_t radicand = dd_add(corrected, error);
+  if (dd_lt(dd_abs(radicand), dd(epsilon))) {
+    roots_append(&r, dd_double(dd_div(dd_neg(b), dd_mul(dd(2.), a))));
+    return r;
+  }
+  if (dd_lt(radicand, dd(0.))) return r;
+  dd_t den = dd_lt(dd(0.), b) ? dd_sub(dd_neg(b), dd_sqrt(radicand)) : dd_add(dd_neg(b), dd_sqrt(radicand));
+  dd_t root = dd_div(dd_mul(c, dd(2.)), den);
+  roots_append(&r, dd_double(root));
+  roots_append(&r, dd_double(dd_div(c, dd_mul(a, root))));
+  return r;
+}
+
+/* stable_monic_quadratic (StableQuartic.inl) */
+static roots_t monic_quadratic_dd(dd_t b, dd_t c) {
+  roots_t r = {{0., 0., 0., 0.}, 0u};
+  if (dd_eq(c, dd(0.))) {
+    roots_append(&r, 0.);
+    if (!dd_eq(b, dd(0.))) roots_append(&r, dd_double(dd_neg(b)));
+    return r;
+  }
+  dd_t multiplied = dd_mul(dd(4.), c);
+  dd_t disc = dd_fma(b, b, dd_neg(multiplied));
+  if (dd_lt(disc, dd(0.))) return r;
+  if (dd_eq(disc, dd(0.))) {
+    roots_append(&r, dd_double(dd_div(dd_neg(b), dd(2.))));
+    return r;
+  }
+  dd_t s = !dd_lt(b, dd(0.)) ? dd_sqrt(disc) : dd_neg(dd_sqrt(disc));
+  dd_t large = dd_div(dd_neg(dd_add(b, s)), dd(2.));
+  roots_append(&r, dd_double(large));
+  roots_append(&r, dd_double(dd_div(c, large)));
+  return r;
+}
+
+static dd_t halley_dd(dd_t root, dd_t p, dd_t quadratic, dd_t nc) {
+  for (int it = 0; it < 4; ++it) {
+    dd_t value = dd_sub(dd_mul(dd_add(dd_mul(dd_add(root, p), root), quadratic), root), nc);
+    dd_t derivative = dd_add(dd_mul(dd_add(dd_mul(dd(3.), root), dd_mul(dd(2.), p)), root), quadratic);
+    dd_t second = dd_add(dd_mul(dd(6.), root), dd_mul(dd(2.), p));
+    dd_t num = dd_mul(dd_mul(dd(2.), value), derivative);
+    dd_t den = dd_sub(dd_mul(dd_mul(dd(2.), derivative), derivative), dd_mul(value, second));
+    root = dd_sub(root, dd_div(num, den));
+  }
+  return root;
+}
+
+/* stable_resolvent (StableQuartic.inl) */
+static dd_t resolvent_dd(dd_t p, dd_t q, dd_t r) {
+  dd_t quadratic = dd_sub(dd_div(dd_mul(p, p), dd(4.)), r);
+  dd_t nc = dd_div(dd_mul(q, q), dd(8.));
+  dd_t zero = dd(0.);
+  if (dd_lt(p, zero) && dd_lt(r, zero) && dd_lt(nc, dd_div(dd_mul(dd_neg(p), quadratic), dd(64.))) &&
+      dd_lt(nc, dd_div(dd_mul(p, r), dd(8.))))
+    return halley_dd(dd_div(nc, quadratic), p, quadratic, nc);
+  if (!dd_lt(p, zero) && !dd_lt(quadratic, zero)) {
+    dd_t root = dd_cbrt(nc);
+    if (dd_lt(zero, p)) {
+      dd_t candidate = dd_sqrt(dd_div(nc, p));
+      if (dd_lt(candidate, root)) root = candidate;
+    }
+    if (dd_lt(zero, quadratic)) {
+      dd_t candidate = dd_div(nc, quadratic);
+      if (dd_lt(candidate, root)) root = candidate;
+    }
+    return halley_dd(root, p, quadratic, nc);
+  }
+  dd_t dq = dd_div(dd_sub(dd_mul(dd(3.), quadratic), dd_mul(p, p)), dd(9.));
+  dd_t dc = dd_div(dd_add(dd_mul(p, dd_sub(dd_mul(dd(9.), quadratic), dd_mul(dd_mul(dd(2.), p), p))),
+                          dd_mul(dd(27.), nc)),
+                   dd(54.));
+  dd_t disc = dd_fma(dc, dc, dd_mul(dd_mul(dq, dq), dq));
+  if (!dd_lt(disc, zero)) {
+    dd_t s = !dd_lt(dc, zero) ? dd_sqrt(disc) : dd_neg(dd_sqrt(disc));
+    dd_t cube = dd_cbrt(dd_add(dc, s));
+    dd_t t = dd_eq(cube, zero) ? zero : dd_sub(cube, dd_div(dq, cube));
+    return dd_sub(t, dd_div(p, dd(3.)));
+  }
+  dd_t argument = dd_div(dc, dd_sqrt(dd_mul(dd_mul(dd_neg(dq), dq), dq)));
+  if (dd_lt(dd(1.), argument)) argument = dd(1.);
+  if (dd_lt(argument, dd(-1.))) argument = dd(-1.);
+  dd_t t = dd_mul(dd_mul(dd(2.), dd_sqrt(dd_neg(dq))), dd_cos(dd_div(dd_acos(argument), dd(3.))));
+  return dd_sub(t, dd_div(p, dd(3.)));
+}
+
+/* stable_quartic_roots (StableQuartic.inl): x^4 + p x^2 + q x + r */
+static roots_t quartic_roots(double p, double q, double r) {
+  roots_t out = {{0., 0., 0., 0.}, 0u};
+  int pe = -4096, qe = -4096, re = -4096, e;
+  if (p != 0.) {
+    frexp(p, &e);
+    pe = e >= 0 ? (e + 1) / 2 : e / 2;
+  }
+  if (q != 0.) {
+    frexp(q, &e);
+    qe = e >= 0 ? (e + 2) / 3 : e / 3;
+  }
+  if (r != 0.) {
+    frexp(r, &e);
+    re = e >= 0 ? (e + 3) / 4 : e / 4;
+  }
+  int exponent = pe > qe ? pe : qe;
+  exponent = exponent > re ? exponent : re;
+  if (exponent == -4096) {
+    roots_append(&out, 0.);
+    return out;
+  }
+  dd_t sp = dd(ldexp(p, -2 * exponent)), sq = dd(ldexp(q, -3 * exponent)), sr = dd(ldexp(r, -4 * exponent));
+  double scale = ldexp(1., exponent);
+  if (q == 0.) {
+    roots_t squares = monic_quadratic_dd(sp, sr);
+    for (unsigned i = 0; i < squares.n; ++i) {
+      if (squares.v[i] < 0.) continue;
+      if (squares.v[i] == 0.) {
+        roots_append_distinct(&out, 0.);
+      } else {
+        double root = dd_double(dd_sqrt(dd(squares.v[i]))) * scale;
+        roots_append_distinct(&out, -root);
+        roots_append_distinct(&out, root);
+      }
+    }
+  } else {
+    dd_t y = resolvent_dd(sp, sq, sr);
+    if (!dd_lt(dd(0.), y)) {
+      roots_append(&out, INFINITY);
+      return out;
+    }
+    dd_t linear = dd_sqrt(dd_mul(dd(2.), y));
+    dd_t common = dd_add(dd_div(sp, dd(2.)), y);
+    dd_t difference = dd_div(sq, dd_mul(dd(2.), linear));
+    dd_t first, second;
+    if (!dd_lt(dd_abs(dd_add(common, difference)), dd_abs(dd_sub(common, difference)))) {
+      second = dd_add(common, difference);
+      first = dd_div(sr, second);
+    } else {
+      first = dd_sub(common, difference);
+      second = dd_div(sr, first);
+    }
+    roots_t a = monic_quadratic_dd(linear, first);
+    roots_t b = monic_quadratic_dd(dd_neg(linear), second);
+    for (unsigned i = 0; i < a.n; ++i) roots_append_distinct(&out, a.v[i] * scale);
+    for (unsigned i = 0; i < b.n; ++i) roots_append_distinct(&out, b.v[i] * scale);
+  }
+  for (unsigned i = 1; i < out.n; ++i) {
+    double v = out.v[i];
+    unsigned j = i;
+    while (j && v < out.v[j - 1]) {
+      out.v[j] = out.v[j - 1];
+      --j;
+    }
+    out.v[j] = v;
+  }
+  return out;
+}
+
+typedef struct {
+  int status;
+  double distance_m, gyroradius_m;
+} mlimit_t;
+
+static int magnetic_valid_particle(const ps_t* p, double mass_gev, double charge) {
+  if (!finite(p->energy_gev) || !(p->energy_gev > mass_gev) || !finite(mass_gev) || !(mass_gev > 0.) ||
+      !finite(charge))
+    return 0;
+  for (int a = 0; a < 3; ++a)
+    if (!finite(p->position_m[a]) || !finite(p->direction[a])) return 0;
+  double n2 = dot3(p->direction, p->direction);
+  return finite(n2) && fabs(n2 - 1.) <= 1.e-12;
+}
+
+static int magnetic_valid_field(const double b[3]) { return finite(b[0]) && finite(b[1]) && finite(b[2]); }
+
+static mlimit_t magnetic_step_limit(const ps_t* p, double mass_gev, double charge, const double b[3],
+                                    double max_deflection) {
+  mlimit_t out = {MAG_INVALID, 0., 0.};
+  if (!magnetic_valid_particle(p, mass_gev, charge) || !magnetic_valid_field(b) || !finite(max_deflection) ||
+      !(max_deflection > 0.) || !(max_deflection < 0.5 * 3.141592653589793))
+    return out;
+  double b2 = dot3(b, b);
+  out.status = MAG_LINEAR;
+  out.distance_m = INFINITY;
+  out.gyroradius_m = INFINITY;
+  if (charge == 0. || b2 == 0.) return out;
+  double momentum = sqrt((p->energy_gev - mass_gev) * (p->energy_gev + mass_gev));
+  double pv[3] = {momentum * p->direction[0], momentum * p->direction[1], momentum * p->direction[2]};
+  double along = dot3(pv, b) / b2;
+  double perp[3] = {pv[0] - along * b[0], pv[1] - along * b[1], pv[2] - along * b[2]};
+  double p_perp = sqrt(dot3(perp, perp));
+  if (p_perp < 1.e-9) return out;
+  double radius = p_perp / (MAGNETIC_RIGIDITY * fabs(charge) * sqrt(b2));
+  out.gyroradius_m = radius;
+  if (!finite(radius) || radius > 1.e9) return out;
+  double distance = 2. * cos(max_deflection) * sin(max_deflection) * radius;
+  if (!finite(distance) || !(distance > 0.)) {
+    out.status = MAG_NONFINITE;
+    out.distance_m = 0.;
+    return out;
+  }
+  out.status = MAG_SUCCESS;
+  out.distance_m = distance;
+  return out;
+}
+
+static int magnetic_advance(const ps_t* start, double mass_gev, double charge, const double b[3],
+                            double distance, ps_t* end) {
+  if (!magnetic_valid_particle(start, mass_gev, charge) || !magnetic_valid_field(b) || !finite(distance) ||
+      distance < 0.)
+    return MAG_INVALID;
+  *end = *start;
+  if (distance == 0.) return MAG_SUCCESS;
+  double momentum = sqrt((start->energy_gev - mass_gev) * (start->energy_gev + mass_gev));
+  double coefficient = charge * MAGNETIC_RIGIDITY * distance / momentum;
+  const double* u = start->direction;
+  double uxb[3] = {u[1] * b[2] - u[2] * b[1], u[2] * b[0] - u[0] * b[2], u[0] * b[1] - u[1] * b[0]};
+  double kicked[3], disp[3];
+  for (int a = 0; a < 3; ++a) {
+    double kick = coefficient * uxb[a];
+    kicked[a] = u[a] + kick;
+    disp[a] = distance * (u[a] + 0.5 * kick);
+    end->position_m[a] = start->position_m[a] + disp[a];
+  }
+  double kn = sqrt(dot3(kicked, kicked));
+  double chord = sqrt(dot3(disp, disp));
+  double beta = momentum / start->energy_gev;
+  if (!finite(kn) || !(kn > 0.) || !finite(chord) || !finite(beta) || !(beta > 0.)) return MAG_NONFINITE;
+  for (int a = 0; a < 3; ++a) end->direction[a] = kicked[a] / kn;
+  end->time_s = start->time_s + distance / (beta * C_LIGHT);
+  return dot3(uxb, uxb) == 0. || charge == 0. ? MAG_LINEAR : MAG_SUCCESS;
+}
+
+typedef struct {
+  double position[3], velocity[3], direction[3], field[3];
+  double speed, momentum_si, charge_si, gyroradius;
+  int linear, incomplete;
+} curved_t;
+
+static curved_t make_curved(const ps_t* p, double mass_gev, double charge, const double b[3],
+                            const mlimit_t* limit, int linear) {
+  curved_t a;
+  double momentum = sqrt((p->energy_gev - mass_gev) * (p->energy_gev + mass_gev));
+  a.speed = C_LIGHT * momentum / p->energy_gev;
+  a.charge_si = charge * ELEMENTARY_CHARGE;
+  a.momentum_si = momentum * ELEMENTARY_CHARGE / MAGNETIC_RIGIDITY;
+  for (int k = 0; k < 3; ++k) {
+    a.position[k] = p->position_m[k];
+    a.velocity[k] = a.speed * p->direction[k];
+    a.direction[k] = p->direction[k];
+    a.field[k] = b[k];
+  }
+  a.gyroradius = limit->gyroradius_m;
+  a.linear = linear;
+  a.incomplete = 0;
+  return a;
+}
+
+/* intersectShape for spheres and the universe (a node of the tree) */
+static isect_t curved_sphere(curved_t* a, const c8mi_env_node* s) {
+  isect_t none = {0., 0., 0};
+  if (a->linear) return straight_intersect(s, a->position, a->velocity);
+  if (s->shape_kind == C8MI_SHAPE_UNIVERSE) return none;
+  const double* u = a->direction;
+  const double* b = a->field;
+  double projected[3] = {u[1] * b[2] - u[2] * b[1], u[2] * b[0] - u[0] * b[2], u[0] * b[1] - u[1] * b[0]};
+  double delta[3] = {a->position[0] - s->center_m[0], a->position[1] - s->center_m[1],
+                     a->position[2] - s->center_m[2]};
+  double projected2 = dot3(projected, projected);
+  if (projected2 == 0.) return straight_intersect(s, a->position, a->velocity);
+  double delta_norm = sqrt(dot3(delta, delta));
+  double distance = fabs(delta_norm - s->radius_m);
+  if (distance > a->gyroradius) return none;
+  if (a->gyroradius > 1000 * distance) return straight_intersect(s, a->position, a->velocity);
+  double k = a->charge_si / a->momentum_si;
+  double denominator = 4. / (projected2 * k * k);
+  double quadratic = (dot3(projected, delta) * k + 1) * denominator;
+  double linear = dot3(u, delta) * 2 * denominator;
+  double constant = (delta_norm + s->radius_m) * (delta_norm - s->radius_m) * denominator;
+  roots_t roots = quartic_roots(quadratic, linear, constant);
+  int inside = contains(s, a->position);
+  double entry = 0., exit = 0.;
+  unsigned count = 0, entries = 0, exits = 0;
+  for (unsigned i = 0; i < roots.n; ++i) {
+    double d = roots.v[i];
+    if (!finite(d)) {
+      a->incomplete = 1;
+      return none;
+    }
+    if (inside) {
+      if (d < 1.e-4) {
+        if (!entries || entry < d) entry = d;
+        ++entries;
+      } else {
+        if (!exits || d < exit) exit = d;
+        ++exits;
+      }
+      count = (entries > 0) + (exits > 0);
+    } else {
+      if (d < -1.e-4) continue;
+      if (!count) {
+        entry = d;
+      } else if (d < entry) {
+        exit = entry;
+        entry = d;
+      } else {
+        exit = d;
+      }
+      ++count;
+    }
+  }
+  if (!count) return none;
+  if ((inside && (!entries || !exits)) || (!inside && count < 2)) {
+    a->incomplete = 1;
+    return none;
+  }
+  isect_t r = {entry / a->speed, exit / a->speed, 1};
+  return r;
+}
+
+/* lepton_boundary: 1 on success (distance, next node), 0 on a failed boundary */
+static int magnetic_lepton_boundary(const c8mi_environment* e, const ps_t* p, double mass_gev, double charge,
+                                    const double b[3], const mlimit_t* limit, double* distance, int32_t* next) {
+  curved_t a = make_curved(p, mass_gev, charge, b, limit, limit->status == MAG_LINEAR);
+  int32_t current = p->environment_node;
+  double selected = limit->distance_m / a.speed;
+  int32_t selected_node = current;
+  const c8mi_env_node* n = &e->nodes[current];
+  isect_t own = curved_sphere(&a, n);
+  if (own.has && own.exit <= selected) {
+    selected = own.exit;
+    selected_node = n->parent;
+  }
+  for (uint32_t c = 0; c < n->child_count; ++c) {
+    int32_t child = e->children[n->child_begin + c];
+    isect_t x = curved_sphere(&a, &e->nodes[child]);
+    if (!x.has) continue;
+    if (x.exit > 0. && x.entry <= selected && -x.entry < x.exit) {
+      selected = x.entry;
+      selected_node = child;
+    }
+  }
+  if (selected < 0.) selected = 0.;
+  *distance = selected * a.speed;
+  *next = selected_node;
+  return !a.incomplete;
+}
+
+/* lepton_observation_distance: the separation-plane branch of intersectShape
+ * on the curved trajectory (the access is never linear here) */
+static double magnetic_observation_distance(const c8mi_environment* e, const ps_t* p, double mass_gev,
+                                            double charge, const double b[3], const mlimit_t* limit) {
+  curved_t a = make_curved(p, mass_gev, charge, b, limit, 0);
+  const double* u = a.direction;
+  double projected[3] = {u[1] * b[2] - u[2] * b[1], u[2] * b[0] - u[0] * b[2], u[0] * b[1] - u[1] * b[0]};
+  double delta[3] = {a.position[0] - e->observation_point_m[0], a.position[1] - e->observation_point_m[1],
+                     a.position[2] - e->observation_point_m[2]};
+  double denominator = a.charge_si * dot3(e->observation_normal, projected);
+  double linear = 2 * a.momentum_si * dot3(u, e->observation_normal);
+  double constant = 2 * a.momentum_si * dot3(e->observation_normal, delta);
+  roots_t r = quadratic_roots_dd(dd(denominator), dd(linear), dd(constant));
+  double first = INFINITY;
+  for (unsigned i = 0; i < r.n; ++i)
+    if (r.v[i] >= 0. && r.v[i] < first) first = r.v[i];
+  double entry = first / a.speed;
+  return entry > 0. ? entry * a.speed : INFINITY;
+}
+
+/* exported for the tests */
+unsigned oracle_quartic_roots(double p, double q, double r, double out[4]) {
+  roots_t x = quartic_roots(p, q, r);
+  for (unsigned i = 0; i < x.n; ++i) out[i] = x.v[i];
+  return x.n;
+}
+unsigned oracle_quadratic_roots(double a, double b, double c, double out[4]) {
+  roots_t x = quadratic_roots_dd(dd(a), dd(b), dd(c));
+  for (unsigned i = 0; i < x.n; ++i) out[i] = x.v[i];
+  return x.n;
+}
+int oracle_magnetic_step_limit(const c8mi_particle_state* p, double mass_gev, double charge,
+                               const double field[3], double max_deflection, double out[2]) {
+  mlimit_t l = magnetic_step_limit(p, mass_gev, charge, field, max_deflection);
+  out[0] = l.distance_m;
+  out[1] = l.gyroradius_m;
+  return l.status;
+}
+int oracle_magnetic_advance(const c8mi_particle_state* p, double mass_gev, double charge, const double field[3],
+                            double distance, c8mi_particle_state* end) {
+  return magnetic_advance(p, mass_gev, charge, field, distance, end);
+}
+/* lepton_boundary + lepton_observation_distance over an environment;
+ * out = {boundary distance, next node, observation distance}; returns the
+ * limit status, or -1 for a failed boundary. */
+int oracle_lepton_boundary(const c8mi_environment* e, const c8mi_particle_state* p, double mass_gev,
+                           double charge, const double field[3], double out[3]) {
+  mlimit_t l = magnetic_step_limit(p, mass_gev, charge, field, e->maximum_magnetic_deflection_rad);
+  if (l.status == MAG_INVALID || l.status == MAG_NONFINITE) return l.status;
+  double distance;
+  int32_t next;
+  if (!magnetic_lepton_boundary(e, p, mass_gev, charge, field, &l, &distance, &next)) return -1;
+  out[0] = distance;
+  out[1] = (double)next;
+  out[2] = magnetic_observation_distance(e, p, mass_gev, charge, field, &l);
+  return l.status;
 }
 
 /* ========================== table facades ========================== */
@@ -2176,30 +2666,26 @@ static void transport_lepton(const ctx_t* x, const inter_t* in, ltrans_t* out, f
   }
   start.environment_node = layer.node;
   const c8mi_env_model* model = get_model(env, layer.node);
-  if (model->magnetic_field_t[0] != 0. || model->magnetic_field_t[1] != 0. ||
-      model->magnetic_field_t[2] != 0.) {
-    set_fallback(fb, in, FR_MAGNETIC_TRANSPORT);
-    return;
-  }
   double m_gev = mass.value / 1000.;
-  /* maximum_uniform_magnetic_step: zero field gives Linear and an infinite limit
-   * (UniformMagneticField.inl:101-113); its particle check */
-  if (!finite(start.energy_gev) || !(start.energy_gev > m_gev) || !normalized(start.direction)) {
+  /* maximum_uniform_magnetic_step, lepton_boundary, lepton_observation_distance
+   * (LeptonTransportStep.inl:119-160) */
+  double charge = start.pid > 0 ? -1. : 1.;
+  const double* field = model->magnetic_field_t;
+  mlimit_t mlimit = magnetic_step_limit(&start, m_gev, charge, field, env->maximum_magnetic_deflection_rad);
+  if (mlimit.status == MAG_INVALID || mlimit.status == MAG_NONFINITE) {
     set_fallback(fb, in, FR_MAGNETIC_TRANSPORT);
     return;
   }
-  double p_gev = sqrt((start.energy_gev - m_gev) * (start.energy_gev + m_gev));
-  double speed = C_LIGHT * p_gev / start.energy_gev;
-  double momentum_si = p_gev * ELEMENTARY_CHARGE / MAGNETIC_RIGIDITY;
-  double vel[3] = {speed * start.direction[0], speed * start.direction[1], speed * start.direction[2]};
-  double param;
+  double geometry;
   int32_t next_node;
-  next_boundary(env, start.position_m, vel, start.environment_node, INFINITY / speed, &param, &next_node);
-  double geometry = param * speed;
+  if (!magnetic_lepton_boundary(env, &start, m_gev, charge, field, &mlimit, &geometry, &next_node)) {
+    set_fallback(fb, in, FR_MAGNETIC_BOUNDARY);
+    return;
+  }
   int magnetic_wins = next_node == start.environment_node;
   int obs_wins = 0;
   if (env->observation_enabled) {
-    double o = lepton_observation_distance(env, &start, speed, momentum_si);
+    double o = magnetic_observation_distance(env, &start, m_gev, charge, field, &mlimit);
     if (o < geometry) {
       geometry = o;
       obs_wins = 1;
@@ -2232,24 +2718,14 @@ static void transport_lepton(const ctx_t* x, const inter_t* in, ltrans_t* out, f
     distance = d.value;
   }
   int observation_reached = obs_wins && !continuous_wins && !interaction_wins && !magnetic_wins;
-  /* advance_uniform_magnetic_field with zero field (UniformMagneticField.inl:146-195) */
-  out->end = start;
-  if (distance != 0.) {
-    double disp[3], kicked[3];
-    for (int a = 0; a < 3; ++a) {
-      kicked[a] = start.direction[a] + 0.;
-      disp[a] = distance * (start.direction[a] + 0.5 * 0.);
-      out->end.position_m[a] = start.position_m[a] + disp[a];
-    }
-    double kn = sqrt(dot3(kicked, kicked));
-    double chord = sqrt(dot3(disp, disp));
-    double beta = p_gev / start.energy_gev;
-    if (!finite(kn) || !(kn > 0.) || !finite(chord) || !finite(beta) || !(beta > 0.)) {
-      set_fallback(fb, in, FR_MAGNETIC_TRANSPORT);
-      return;
-    }
-    for (int a = 0; a < 3; ++a) out->end.direction[a] = kicked[a] / kn;
-    out->end.time_s = start.time_s + distance / (beta * C_LIGHT);
+  /* advance_uniform_magnetic_field (UniformMagneticField.inl:146-195), no field
+   * where the limit was Linear */
+  const double zero_field[3] = {0., 0., 0.};
+  int advanced = magnetic_advance(&start, m_gev, charge, mlimit.status == MAG_LINEAR ? zero_field : field, distance,
+                                  &out->end);
+  if (advanced == MAG_INVALID || advanced == MAG_NONFINITE) {
+    set_fallback(fb, in, FR_MAGNETIC_TRANSPORT);
+    return;
   }
   /* chord grammage (LeptonTransportStep.inl:301-331) */
   double chord_dir[3], chord2 = 0.;

@@ -380,3 +380,48 @@ def moliere_direction(start, tracked, angle, azimuth_uniform, reference=False):
     out = (c_double * 3)()
     st = fn((c_double * 3)(*start), (c_double * 3)(*tracked), c_double(angle), c_double(azimuth_uniform), out)
     return st, tuple(out)
+
+
+def quartic_roots(p, q, r):
+    """Real roots of x^4 + p x^2 + q x + r (stable_quartic_roots), ascending."""
+    out = (c_double * 4)()
+    n = load().oracle_quartic_roots(c_double(p), c_double(q), c_double(r), out)
+    return [out[i] for i in range(n)]
+
+
+def quadratic_roots(a, b, c):
+    out = (c_double * 4)()
+    n = load().oracle_quadratic_roots(c_double(a), c_double(b), c_double(c), out)
+    return [out[i] for i in range(n)]
+
+
+def _state_ptr(p):
+    return np.ascontiguousarray(np.atleast_1d(p)).ctypes.data_as(c_void_p)
+
+
+def magnetic_step_limit(p, mass_gev, charge, field, max_deflection=0.2):
+    """(status, distance_m, gyroradius_m) of maximum_uniform_magnetic_step."""
+    out = (c_double * 2)()
+    arr = np.ascontiguousarray(np.atleast_1d(p))
+    st = load().oracle_magnetic_step_limit(arr.ctypes.data_as(c_void_p), c_double(mass_gev), c_double(charge),
+                                           (c_double * 3)(*field), c_double(max_deflection), out)
+    return st, out[0], out[1]
+
+
+def magnetic_advance(p, mass_gev, charge, field, distance):
+    """(status, end state) of advance_uniform_magnetic_field."""
+    arr = np.ascontiguousarray(np.atleast_1d(p))
+    end = np.zeros(1, dtype=arr.dtype)
+    st = load().oracle_magnetic_advance(arr.ctypes.data_as(c_void_p), c_double(mass_gev), c_double(charge),
+                                        (c_double * 3)(*field), c_double(distance), end.ctypes.data_as(c_void_p))
+    return st, end[0]
+
+
+def lepton_boundary(environment, p, mass_gev, charge, field):
+    """(status, boundary distance, next node, observation distance): status is
+    the magnetic limit's, -1 for a failed boundary."""
+    arr = np.ascontiguousarray(np.atleast_1d(p))
+    out = (c_double * 3)()
+    st = load().oracle_lepton_boundary(_env_ptr(environment), arr.ctypes.data_as(c_void_p), c_double(mass_gev),
+                                       c_double(charge), (c_double * 3)(*field), out)
+    return st, out[0], int(out[1]), out[2]

@@ -35,8 +35,9 @@ def tables():
 
 
 def run_pair(tables, pid, energy, seed, shower_id=1, thinning=None, start=(0.0, 0.0, 112750.0),
-             direction=(0.0, 0.0, -1.0), primaries=None, moliere=False, moliere_table=None, fast_iterations=0):
-    atm = em.Atmosphere()
+             direction=(0.0, 0.0, -1.0), primaries=None, moliere=False, moliere_table=None, fast_iterations=0,
+             field=(0.0, 0.0, 0.0)):
+    atm = em.Atmosphere(magnetic_field_t=field)
     cfg = em.make_config(airtables, seed=seed, atmosphere=atm, thinning=thinning, start=start,
                          direction=direction, moliere=moliere, moliere_table=moliere_table,
                          moliere_fast_iterations=fast_iterations)
@@ -145,6 +146,22 @@ def test_moliere_table_shower_matches_oracle(tables, pid, energy, seed, start, f
                               moliere_table=True, fast_iterations=fast)
     compare(cfg, want, got, energy)
     assert got["stats"]["lepton_steps"] > 100
+
+
+@pytest.mark.parametrize("pid,energy,seed,start,field,moliere,magnetic_limits", [
+    (22, 10.0, 21, 3000.0, (0.0, 5.0e-5, 0.0), False, False),        # Earth-like field, photon primary
+    (11, 5.0, 22, 1000.0, (2.0e-5, -4.0e-5, 3.0e-5), False, False),  # oblique field, electron
+    (-11, 3.0, 23, 2000.0, (0.0, 5.0e-2, 0.0), True, True),          # strong field: magnetic limits, Molière
+    (22, 100.0, 24, 112750.0, (0.0, 5.0e-5, -2.0e-5), True, True)])  # whole air shower from the top
+def test_magnetic_field_shower_matches_oracle(tables, pid, energy, seed, start, field, moliere, magnetic_limits):
+    """e+- in a uniform field: leapfrog step limits, curved layer boundaries
+    (quartic roots in double-double) and the bent advance on the device equal
+    the oracle's, shower for shower."""
+    cfg, want, got = run_pair(tables, pid, energy, seed, start=(0.0, 0.0, start), moliere=moliere,
+                              moliere_table=True if moliere else None, field=field)
+    compare(cfg, want, got, energy)
+    if magnetic_limits:
+        assert got["ledger"]["lepton_limits"][6] > 0  # steps ended by the magnetic limit
 
 
 def test_mixed_primary_batch(tables):
