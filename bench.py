@@ -12,6 +12,7 @@ Prints one JSON line (see --help for options).
 """
 import argparse
 import json
+import math
 import os
 import sys
 import time
@@ -75,6 +76,8 @@ def parse():
     p.add_argument("--serve-plain-showers", type=int, default=24,
                    help="first requests served again alone, pooled and one at a time (latency comparison)")
     p.add_argument("--serve-only", action="store_true", help="run only the serving-stream section")
+    p.add_argument("--ice-energy", type=float, default=1.0e6, help="in-ice primary electron energy (GeV)")
+    p.add_argument("--ice-only", action="store_true", help="run only the in-ice section")
     return p.parse_args()
 
 
@@ -311,10 +314,12 @@ def em_section(showers, plain_showers, slots, cpu_seconds, no_cpu_baseline, ener
     return out
 
 
-def em_radio_cpu_baseline(tables, atmosphere, cfg, observers, propagation, target_seconds):
-    """Oracle EM showers with their step tracks (one 100 GeV electron shower per
-    host thread) followed by the FP64 radio oracle projecting all their tracks
-    on the same antennas (OpenMP over tracks), on all host cores."""
+def em_radio_cpu_baseline(tables, atmosphere, cfg, observers, propagation, target_seconds, primary=None,
+                          label="100 GeV vertical electron", coreas=True):
+    """Oracle EM showers with their step tracks (one shower of `primary`, by
+    default a 100 GeV electron at 112.75 km, per host thread) followed by the
+    FP64 radio oracle projecting all their tracks on the same antennas (OpenMP
+    over tracks), on all host cores."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     from concurrent.futures import ThreadPoolExecutor
 
@@ -322,7 +327,8 @@ def em_radio_cpu_baseline(tables, atmosphere, cfg, observers, propagation, targe
     from c8mi import em
 
     threads = max(1, min(int(os.environ.get("OMP_NUM_THREADS", "0")) or 16, os.cpu_count() or 1))
-    prim = em.particle(11, 100.0, (0.0, 0.0, 112750.0), (0.0, 0.0, -1.0), history_id=1)
+    prim = primary if primary is not None else em.particle(11, 100.0, (0.0, 0.0, 112750.0), (0.0, 0.0, -1.0),
+                                                          history_id=1)
     count = threads
     while True:
         t0 = time.perf_counter()
@@ -333,7 +339,7 @@ def em_radio_cpu_baseline(tables, atmosphere, cfg, observers, propagation, targe
         t_em = time.perf_counter() - t0
         tracks = np.concatenate([r["tracks"] for r in runs], axis=1)
         t1 = time.perf_counter()
-        oracle_lib.radio_project(propagation, observers, tracks, floating=False, threads=threads)
+        oracle_lib.radio_project(propagation, observers, tracks, coreas=coreas, floating=False, threads=threads)
         t_radio = time.perf_counter() - t1
         if t_em + t_radio >= 0.5 * target_seconds or count >= 8 * threads:
             break
@@ -342,7 +348,7 @@ def em_radio_cpu_baseline(tables, atmosphere, cfg, observers, propagation, targe
     wall = t_em + t_radio
     return {"value": steps / wall, "unit": "particle-steps/s", "cores": threads, "kind": "port",
             "updates_per_s": tracks.shape[1] * observers.size / wall,
-            "sample": f"{count} x 100 GeV vertical electron showers with radio tracks, one per thread "
+            "sample": f"{count} x {label} showers with radio tracks, one per thread "
                       f"(oracle, {t_em:.1f} s), then their {tracks.shape[1]} tracks x {observers.size} "
                       f"antennas by the FP64 radio oracle, OpenMP over tracks ({t_radio:.1f} s); "
                       f"{threads} threads"}
@@ -436,6 +442,90 @@ def em_radio_section(showers, energy, cpu_seconds, no_cpu_baseline, moliere=True
     }
     if not no_cpu_baseline:
         base = em_radio_cpu_baseline(tables, atm, cfg, observers, prop, cpu_seconds)
+        out["cpu_baseline"] = base
+        out["speedup_vs_cpu_baseline"] = out["value"] / base["value"]
+        out["updates_speedup_vs_cpu_baseline"] = out["updates_per_s"] / base["updates_per_s"]
+    return out
+
+
+def ice_section(energy, cpu_seconds, no_cpu_baseline):
+    """In-ice cascade (applications/c8_ice_cascade.cpp scene): a vertical
+    electron of `energy` GeV 100 m below the surface of homogeneous ice (0.92
+    g/cm^3, H2O), 100 keV cut, thinning below 1e-6 E0 with maximum weight 1e4,
+    Molière on (ice table), every e+- step projected by ZHS (homogeneous n =
+    1.78) onto 64 antennas on the Cherenkov cone 1 km from the shower maximum,
+    4096 bins @ 0.1 ns; the whole cascade on the device."""
+    import c8mi
+    from c8mi import airtables, em, workloads
+
+    tables = airtables.build_ice_tables(cut_mev=0.1)
+    ice = em.IceVolume()
+    start, down = (0.0, 0.0, -100.0), (0.0, 0.0, -1.0)
+    n_ice = 1.78
+
+    def config(e0, **kw):
+        return em.make_config(airtables.IceMedium, cut_mev=0.1, start=start, direction=down, atmosphere=ice,
+                              thinning=(1.0e-6 * e0, 1.0e4), moliere=True, moliere_table=True, bins=256, **kw)
+
+    cfg = config(energy, queue_capacity=1 << 24, observation_capacity=1 << 10, fallback_capacity=1 << 16)
+    # shower maximum ~ X0 ln(E / E_c) below the start (X0 = 36.08 g/cm^2, E_c ~ 78 MeV)
+    depth_max_m = 36.08 / 0.92 / 100.0 * math.log(energy * 1.0e3 / 78.0)
+    x_max = np.array(start) + depth_max_m * np.array(down)
+    observers = workloads.cherenkov_ring_observers(x_max, down, 1000.0, n_ice, count=64, sample_rate_hz=1.0e10,
+                                                   number_of_bins=4096,
+                                                   emission_time_s=depth_max_m / workloads.SPEED_OF_LIGHT)
+    prop = c8mi.Propagation(homogeneous_index=n_ice)
+    dev = em.EmBackend(cfg, tables, ice)
+    radio = c8mi.RadioAccumulator(observers, prop, coreas=False, zhs=True)
+    dev.attach_radio(radio)
+
+    def run(shower_id, e):
+        radio.reset()
+        dev.begin_shower(shower_id, 2)
+        dev.enqueue(em.particle(11, e, start, down, history_id=1))
+        t0 = time.perf_counter()
+        totals = dev.run()
+        fixed = radio.download_fixed()
+        return time.perf_counter() - t0, totals, fixed
+
+    run(999, 100.0)  # warm-up
+    dev.enable_kernel_timing(True)
+    radio.enable_kernel_timing(True)
+    elapsed, totals, fixed = run(1, energy)
+    em_timing = dev.kernel_time()
+    radio_timing = radio.kernel_time()
+    _, ledger = dev.download_profile()
+    dev.attach_radio(None)
+    dev.close()
+    radio.close()
+    steps = totals["photon_steps"] + totals["lepton_steps"]
+    updates = totals["lepton_steps"] * observers.size
+    zhs = fixed[1].astype(np.float64)
+    out = {
+        "metric": "EM particle-steps/s and ZHS track·antenna updates/s in ice",
+        "value": steps / elapsed,
+        "unit": "particle-steps/s",
+        "updates_per_s": updates / elapsed,
+        "particle_steps": steps,
+        "radio_tracks": totals["lepton_steps"],
+        "valid_tracks": int(fixed[2]["valid_tracks"]),
+        "shower_s": elapsed,
+        "thinning_vertices": int(ledger["thinning_hillas_vertices"] + ledger["thinning_statistical_vertices"]),
+        "ledger_residual_gev": float(em.ledger_residual(ledger)),
+        "zhs_peak_sum": float(np.abs(zhs).max()),
+        "dtype": "f64",
+        "config": {"workload": f"in-ice cascade: {energy:g} GeV vertical electron in homogeneous ice (0.92 g/cm^3), "
+                               "100 keV cut, thinning 1e-6 E0 / max weight 1e4, Moliere on (table), ZHS at 64 "
+                               "antennas on the Cherenkov cone 1 km from Xmax (n = 1.78), 4096 bins @ 0.1 ns",
+                   "antennas": int(observers.size), "bins": 4096, "sample_rate_hz": 1.0e10},
+        "kernel_ms": {**{k: v[0] for k, v in em_timing.items()},
+                      **{"radio_" + k: v[0] for k, v in radio_timing.items() if v[1]}},
+    }
+    if not no_cpu_baseline:
+        sample_e = 100.0
+        base = em_radio_cpu_baseline(tables, ice, config(sample_e), observers, prop, cpu_seconds,
+                                     primary=em.particle(11, sample_e, start, down, history_id=1),
+                                     label="100 GeV electron in ice (thinning 1e-6 E0)", coreas=False)
         out["cpu_baseline"] = base
         out["speedup_vs_cpu_baseline"] = out["value"] / base["value"]
         out["updates_speedup_vs_cpu_baseline"] = out["updates_per_s"] / base["updates_per_s"]
@@ -625,6 +715,12 @@ def main():
                     "vs_baseline": None, "data": "synthetic", "ms_per_step": out["latency_ms"]["p50"]})
         print(json.dumps(out), flush=True)
         return 0
+    if args.ice_only:
+        out = ice_section(args.ice_energy, args.cpu_seconds, args.no_cpu_baseline)
+        out.update({"n_gpus": 1, "steps": 1, "higher_is_better": True, "scaling": "weak",
+                    "vs_baseline": None, "data": "synthetic", "ms_per_step": out["shower_s"] * 1e3})
+        print(json.dumps(out), flush=True)
+        return 0
     if args.em_radio_only:
         out = em_radio_section(max(args.em_radio_showers, 1), args.em_radio_energy, args.cpu_seconds,
                                args.no_cpu_baseline, moliere=not args.em_no_moliere,
@@ -747,7 +843,9 @@ def main():
     if args.em_radio_showers > 0:
         result["em_radio"] = em_radio_section(args.em_radio_showers, args.em_radio_energy, args.cpu_seconds,
                                               args.no_cpu_baseline, moliere=not args.em_no_moliere,
-                               moliere_table=not args.em_analytic_moliere)
+                                              moliere_table=not args.em_analytic_moliere)
+    if args.ice_energy > 0:
+        result["ice"] = ice_section(args.ice_energy, args.cpu_seconds, args.no_cpu_baseline)
     print(json.dumps(result), flush=True)
     return 0
 

@@ -225,3 +225,30 @@ def serving_stream(count: int, rate_hz: float, seed: int = 4242, e_min_gev: floa
         out.append(dict(pid=pid, energy_gev=energy, zenith_deg=zenith, azimuth_deg=azimuth,
                         arrival_s=float(arrivals[k]), start_m=start, direction=d, observers=obs))
     return out
+
+
+def cherenkov_ring_observers(center_m, axis, distance_m: float, refractive_index: float, count: int = 64,
+                             sample_rate_hz: float = 1.0e10, number_of_bins: int = 4096,
+                             emission_time_s: float = 0.0, pretrigger_s: float = 100.0e-9) -> np.ndarray:
+    """`count` antennas on the Cherenkov cone (cos theta_c = 1/n) around the unit
+    `axis` at `distance_m` from `center_m` (the shower maximum), equally spaced in
+    azimuth; each window opens `pretrigger_s` before the Cherenkov arrival
+    emission_time + n distance / c."""
+    axis = np.asarray(axis, dtype=np.float64)
+    axis = axis / np.linalg.norm(axis)
+    helper = np.array([1.0, 0.0, 0.0]) if abs(axis[0]) < 0.9 else np.array([0.0, 1.0, 0.0])
+    e1 = np.cross(axis, helper)
+    e1 /= np.linalg.norm(e1)
+    e2 = np.cross(axis, e1)
+    cos_c = 1.0 / refractive_index
+    sin_c = math.sqrt(1.0 - cos_c * cos_c)
+    obs = np.zeros(count, dtype=OBSERVER_DTYPE)
+    for k in range(count):
+        phi = 2.0 * math.pi * k / count
+        radial = math.cos(phi) * e1 + math.sin(phi) * e2
+        obs[k]["position_m"] = np.asarray(center_m) + distance_m * (cos_c * axis + sin_c * radial)
+    obs["sample_rate_hz"] = sample_rate_hz
+    obs["number_of_bins"] = number_of_bins
+    obs["duration_s"] = (number_of_bins - 1) / sample_rate_hz
+    obs["start_time_s"] = emission_time_s + refractive_index * distance_m / SPEED_OF_LIGHT - pretrigger_s
+    return obs

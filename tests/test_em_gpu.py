@@ -164,6 +164,37 @@ def test_magnetic_field_shower_matches_oracle(tables, pid, energy, seed, start, 
         assert got["ledger"]["lepton_limits"][6] > 0  # steps ended by the magnetic limit
 
 
+@pytest.fixture(scope="module")
+def ice_tables():
+    return airtables.build_ice_tables(cut_mev=0.1)
+
+
+@pytest.mark.parametrize("pid,energy,seed,thinning,moliere", [
+    (11, 1.0, 41, None, False),
+    (22, 10.0, 42, None, True),
+    (11, 100.0, 43, (1.0e-2, 1.0e4), True)])  # thinning below 1e-4 E0
+def test_ice_shower_matches_oracle(ice_tables, pid, energy, seed, thinning, moliere):
+    """The in-ice scene: homogeneous ice (H2O tables, 0.92 g/cm^3), 100 keV
+    cut, e+- process tables from 0.55 MeV, Molière with the ice snapshot's
+    table, thinning; device and oracle agree shower for shower."""
+    ice = em.IceVolume()
+    start = (0.0, 0.0, -100.0)
+    cfg = em.make_config(airtables.IceMedium, seed=seed, cut_mev=0.1, start=start, atmosphere=ice,
+                         thinning=thinning, moliere=moliere, moliere_table=True if moliere else None, bins=256)
+    prim = em.particle(pid, energy, start, (0.0, 0.0, -1.0), history_id=1)
+    want = oracle_lib.em_run_shower(cfg, ice_tables, ice, prim, shower_id=1, first_history_id=2)
+    dev = em.EmBackend(cfg, ice_tables, ice)
+    dev.begin_shower(1, 2)
+    dev.enqueue(prim)
+    got = dev.run()
+    hist, ledger = dev.download_profile()
+    obs, _ = dev.observations()
+    fbs, _ = dev.fallbacks()
+    dev.close()
+    compare(cfg, want, dict(stats=got, hist=hist, ledger=ledger, observations=obs, fallbacks=fbs), energy)
+    assert got["lepton_steps"] > 1000
+
+
 def test_mixed_primary_batch(tables):
     prim = np.concatenate([np.atleast_1d(em.particle(22, 2.0, (0, 0, 30000.0), (0, 0, -1), history_id=1)),
                            np.atleast_1d(em.particle(11, 1.5, (10, 0, 30000.0), (0, 0, -1), history_id=2)),

@@ -509,3 +509,24 @@ def test_moliere_cache_file_round_trip(tmp_path, moliere, moliere_table):
     other.c1[3] *= 1.0 + 1e-15
     with pytest.raises(c8mi.C8miError):
         em.MoliereInterpolation.read(path, other)
+
+
+def test_oracle_ice_shower_closes_and_peaks_where_expected():
+    """The in-ice scene with the ice tables (H2O, 100 keV cut): a 10 GeV photon
+    shower closes to 1e-9 E0 and its charged-particle profile peaks near
+    X0 (ln(E / E_c) - 1/2) for a photon, X0 = 36.08 g/cm^2, E_c ~ 78 MeV
+    (Rossi approximation B; within two radiation lengths)."""
+    ice = em.IceVolume()
+    start = (0.0, 0.0, -100.0)
+    tables = airtables.build_ice_tables(cut_mev=0.1)
+    cfg = em.make_config(airtables.IceMedium, seed=9, cut_mev=0.1, start=start, atmosphere=ice, bins=256,
+                         bin_width=5.0)
+    prim = em.particle(22, 10.0, start, (0.0, 0.0, -1.0), history_id=1)
+    r = oracle_lib.em_run_shower(cfg, tables, ice, prim)
+    led = em.decode_counters(r["counters"], cfg.struct.profile_energy_scale)
+    assert abs(em.ledger_residual(led)) <= 1e-9 * 10.0
+    ws = cfg.struct.profile_weight_scale
+    charged = (r["hist"]["electrons"] + r["hist"]["positrons"]) / ws
+    xmax = (np.argmax(charged) + 0.5) * 5.0
+    expected = 36.08 * (np.log(10.0e3 / 78.0) - 0.5)
+    assert abs(xmax - expected) < 2 * 36.08, (xmax, expected)
