@@ -180,17 +180,20 @@ def em_cpu_baseline(tables, atmosphere, energy, target_seconds, moliere=True, mo
                       f"{dt:.1f} s wall)"}
 
 
-def em_measured_traffic(kernel):
-    """HBM bytes per particle-step of `kernel` from the committed rocprofv3 PMC
-    summary (FETCH_SIZE + WRITE_SIZE, copy-calibrated), or None."""
+def em_measured_traffic(kernels):
+    """HBM bytes per step of the `kernels` together (each one's bytes per step of
+    its own kind, summed) from the committed rocprofv3 PMC summary
+    (FETCH_SIZE + WRITE_SIZE, copy-calibrated), or None."""
     path = os.path.join(ROOT, "profiles", "em_traffic.json")
     try:
         with open(path) as f:
             data = json.load(f)
     except (OSError, ValueError):
         return None
-    entry = data.get("kernels", {}).get(kernel)
-    return entry.get("traffic_bytes_per_step") if entry else None
+    entries = [data.get("kernels", {}).get(k) for k in kernels]
+    if not entries or any(e is None for e in entries):
+        return None
+    return round(sum(e["traffic_bytes_per_step"] for e in entries), 1)
 
 
 def em_showers(energy, count, first_shower, atm=None, pid=22):
@@ -264,8 +267,15 @@ def em_section(showers, plain_showers, slots, cpu_seconds, no_cpu_baseline, ener
     total_kernel_ms = sum(v[0] for v in timing.values())
     # algorithmic bytes of all particle-steps over the summed step-kernel time
     achieved = steps * EM_BYTES_PER_STEP / (step_kernel_ms * 1e-3) / 1e9
-    kernel_name = {"photon_step": "photon_step_kernel", "lepton_transport": "lepton_transport_kernel",
-                   "lepton_vertex": "lepton_vertex_kernel"}[kind]
+    # the "lepton_transport" timing covers the fused kernel, or with Molière its
+    # move, Molière, tail and finish kernels
+    tag = "true" if moliere_table else "false"
+    transport_kernels = (["lepton_move_kernel", f"lepton_moliere_kernel<{tag}>", f"lepton_moliere_tail_kernel<{tag}>",
+                          "lepton_finish_kernel", "lepton_finish_open_kernel"] if moliere
+                         else ["lepton_transport_kernel"])
+    kernel_names = {"photon_step": ["photon_step_kernel"], "lepton_transport": transport_kernels,
+                    "lepton_vertex": ["lepton_vertex_kernel"]}[kind]
+    kernel_name = "+".join(kernel_names)
     out = {
         "metric": "EM particle-steps/s",
         "value": steps / elapsed,
@@ -297,7 +307,7 @@ def em_section(showers, plain_showers, slots, cpu_seconds, no_cpu_baseline, ener
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": em_measured_traffic(kernel_name),
+            "traffic": em_measured_traffic(kernel_names),
             "kernel": kernel_name,
             "bytes_per_step": EM_BYTES_PER_STEP,
             "kernel_ms": {k: v[0] for k, v in timing.items()},
