@@ -16,6 +16,18 @@ namespace c8mi::radio {
 inline constexpr std::uint32_t kProjectionBlock = 256;
 // Dynamic LDS a projection workgroup may request (gfx950: 160 KiB per CU).
 inline constexpr std::size_t kMaxDynamicLdsBytes = 144 * 1024;
+// Default window tile: at most this many LDS bytes of bins per workgroup, so
+// three workgroups share a CU. A 4096-bin ZHS window in one 96 KiB tile (one
+// workgroup per CU) measured 2x slower than two 2048-bin tiles.
+inline constexpr std::size_t kPreferredTileLdsBytes = 48 * 1024;
+// Fewest tracks per workgroup of the default chunking.
+inline constexpr std::uint64_t kMinimumDefaultChunk = 8192;
+// Bins of the default tile for `per_bin` int64 words per bin (3 per algorithm).
+inline std::uint64_t default_tile_capacity(std::size_t per_bin) {
+  std::uint64_t const most = (kMaxDynamicLdsBytes / sizeof(long long) - 3) / per_bin;
+  std::uint64_t const preferred = kPreferredTileLdsBytes / sizeof(long long) / per_bin;
+  return preferred < most ? preferred : most;
+}
 
 // LDS bytes for one workgroup holding `tile_bins` bins of each enabled mode.
 std::size_t projection_lds_bytes(bool coreas, bool zhs, std::uint32_t tile_bins);

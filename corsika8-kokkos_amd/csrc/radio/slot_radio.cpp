@@ -28,9 +28,8 @@ SlotRadio::SlotRadio(c8mi_radio_config const& config, std::uint32_t slots, std::
   total_bins_ = static_cast<std::uint64_t>(slots) * observers_per_slot * bins_per_observer;
   // Both algorithms share every observer, so they share one launch.
   fuse_ = config.coreas_enabled && config.zhs_enabled;
-  std::size_t const words = kMaxDynamicLdsBytes / sizeof(long long);
   std::size_t const per_bin = (config.coreas_enabled ? 3u : 0u) + (config.zhs_enabled ? 3u : 0u);
-  std::uint64_t const capacity = (words - 3) / per_bin;
+  std::uint64_t const capacity = default_tile_capacity(per_bin);
   tile_bins_ = static_cast<std::uint32_t>(std::min<std::uint64_t>(bins_per_observer, capacity));
   bin_tiles_ = static_cast<std::uint32_t>((bins_per_observer + tile_bins_ - 1) / tile_bins_);
   if (stream == nullptr) {
