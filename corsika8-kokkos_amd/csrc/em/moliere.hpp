@@ -1,6 +1,7 @@
 // Molière multiple scattering of one e+- step on the device: the projected-angle
-// distribution with the analytical correction functions, two Newton-solved
-// projected angles, and the deflection of the step's end direction.
+// distribution with the analytical correction functions or the interpolation
+// table, two Newton-solved projected angles, and the deflection of the step's
+// end direction.
 //
 // Reference: corsika/detail/kokkos/em/models/MoliereScattering.inl:118-688
 // (evaluate_interpolation, improve_moliere_initial_guess, normal_ppf,
