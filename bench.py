@@ -458,6 +458,48 @@ def em_radio_section(showers, energy, cpu_seconds, no_cpu_baseline, moliere=True
     return out
 
 
+def ice_field_error(fast_zhs, exact_zhs, observers, prop, tables, ice, config, start, down):
+    """Relative L2 error of the ZHS vector potential per antenna (the maximum
+    over antennas): the timed fast projection of the whole shower against its
+    exact FP64 projection, and on a 100 GeV sample shower from the oracle
+    (8 of the antennas) both modes against the unquantised FP64 oracle."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import c8mi
+    import oracle_lib
+    from c8mi import em
+
+    def worst(got, want, obs):
+        offsets = np.concatenate([[0], np.cumsum(obs["number_of_bins"].astype(np.int64))])
+        out = 0.0
+        for k in range(obs.size):
+            w = want[:, offsets[k]:offsets[k + 1]]
+            norm = np.linalg.norm(w)
+            if norm > 0:
+                out = max(out, float(np.linalg.norm(got[:, offsets[k]:offsets[k + 1]] - w) / norm))
+        return out
+
+    fast = oracle_lib.radio_decode(fast_zhs, observers, potential=True)
+    exact = oracle_lib.radio_decode(exact_zhs, observers, potential=True)
+    result = {"full_shower_fast_vs_exact": worst(fast, exact, observers)}
+    sample_e = 100.0
+    tracks = oracle_lib.em_run_shower(config(sample_e), tables, ice, em.particle(11, sample_e, start, down, history_id=1),
+                                      tracks=True)["tracks"]
+    subset = observers[::8].copy()
+    ref = oracle_lib.radio_project(prop, subset, tracks, coreas=False, fixed=False, threads=16)
+    for name, exact_mode in (("sample_fast_vs_fp64_oracle", False), ("sample_exact_vs_fp64_oracle", True)):
+        acc = c8mi.RadioAccumulator(subset, prop, coreas=False, zhs=True, exact=exact_mode)
+        dev_tracks = c8mi.DeviceArray.from_host(np.ascontiguousarray(tracks))
+        acc.accumulate_tracks(dev_tracks, tracks.shape[1], tracks.shape[1])
+        got = oracle_lib.radio_decode(acc.download_fixed()[1], subset, potential=True)
+        acc.close()
+        dev_tracks.free()
+        result[name] = worst(got, ref["zhs_float"], subset)
+    result["sample"] = f"{sample_e:g} GeV in-ice electron shower ({tracks.shape[1]} oracle tracks), " \
+                       f"{subset.size} of the {observers.size} ring antennas"
+    result["metric"] = "max over antennas of the relative L2 error of the ZHS vector potential"
+    return result
+
+
 def ice_section(energy, cpu_seconds, no_cpu_baseline):
     """In-ice cascade (applications/c8_ice_cascade.cpp scene): a vertical
     electron of `energy` GeV 100 m below the surface of homogeneous ice (0.92
@@ -505,9 +547,22 @@ def ice_section(energy, cpu_seconds, no_cpu_baseline):
     em_timing = dev.kernel_time()
     radio_timing = radio.kernel_time()
     _, ledger = dev.download_profile()
+    # Field error of the timed (fast) projection against FP64: the same shower
+    # (identical EM history) projected again in the exact mode, which equals the
+    # FP64 oracle's int64 sums bit for bit; and, on a 100 GeV sample shower the
+    # oracle can run here, both modes against the unquantised FP64 oracle.
+    exact_radio = c8mi.RadioAccumulator(observers, prop, coreas=False, zhs=True, exact=True)
+    dev.attach_radio(exact_radio)
+    exact_radio.reset()
+    dev.begin_shower(1, 2)
+    dev.enqueue(em.particle(11, energy, start, down, history_id=1))
+    dev.run()
+    exact_fixed = exact_radio.download_fixed()
     dev.attach_radio(None)
     dev.close()
     radio.close()
+    exact_radio.close()
+    field_error = ice_field_error(fixed[1], exact_fixed[1], observers, prop, tables, ice, config, start, down)
     steps = totals["photon_steps"] + totals["lepton_steps"]
     updates = totals["lepton_steps"] * observers.size
     zhs = fixed[1].astype(np.float64)
@@ -523,6 +578,7 @@ def ice_section(energy, cpu_seconds, no_cpu_baseline):
         "thinning_vertices": int(ledger["thinning_hillas_vertices"] + ledger["thinning_statistical_vertices"]),
         "ledger_residual_gev": float(em.ledger_residual(ledger)),
         "zhs_peak_sum": float(np.abs(zhs).max()),
+        "field_error_vs_fp64": field_error,
         "dtype": "f64",
         "config": {"workload": f"in-ice cascade: {energy:g} GeV vertical electron in homogeneous ice (0.92 g/cm^3), "
                                "100 keV cut, thinning 1e-6 E0 / max weight 1e4, Moliere on (table), ZHS at 64 "
@@ -818,7 +874,7 @@ def main():
             "mode": args.mode,
         },
         "roofline": {
-            "bound": "mfma",
+            "bound": "valu",
             "pipe": "fp64 VALU (no MFMA instruction is issued; MI355X's FP64 vector and "
                     "matrix peaks are both 78.6 TFLOP/s)",
             "achieved": achieved,

@@ -171,6 +171,24 @@ def observers_array(observers) -> np.ndarray:
     return np.ascontiguousarray(np.asarray(observers, dtype=OBSERVER_DTYPE))
 
 
+def zhs_electric_field(observers, potential: np.ndarray):
+    """The radio writer's ZHS field from a decoded vector potential
+    (c8mi_radio_zhs_electric_field): E[i] = -(A[i+1] - A[i]) * sample_rate at
+    the bin midpoints, bins - 1 samples per observer. Returns (field [3][n],
+    times [n]) with n = total bins - observer count."""
+    obs = observers_array(observers)
+    potential = np.ascontiguousarray(potential, dtype=np.float64)
+    total = int(obs["number_of_bins"].sum())
+    if potential.shape != (3, total):
+        raise ValueError(f"potential must be [3][{total}], got {potential.shape}")
+    n = total - obs.size
+    field = np.zeros((3, n))
+    times = np.zeros(n)
+    _check(library().c8mi_radio_zhs_electric_field(
+        obs.ctypes.data_as(c_void_p), c_uint64(obs.size), _dptr(potential), _dptr(field), _dptr(times)))
+    return field, times
+
+
 class DeviceArray:
     """A device allocation made through the C ABI (hipMalloc)."""
 

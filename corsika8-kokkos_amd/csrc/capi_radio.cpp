@@ -108,6 +108,45 @@ int c8mi_radio_download_fixed(c8mi_radio* radio, int64_t* coreas, int64_t* zhs,
   });
 }
 
+int c8mi_radio_zhs_electric_field(c8mi_radio_observer const* observers, uint64_t observer_count,
+                                  double const* potential, double* field, double* times) {
+  return guarded([&] {
+    if ((!observers && observer_count) || !potential || !field)
+      throw std::invalid_argument("null argument");
+    std::uint64_t total = 0;
+    for (std::uint64_t k = 0; k < observer_count; ++k) {
+      c8mi_radio_observer const& o = observers[k];
+      if (o.number_of_bins < 2) throw std::invalid_argument("every observer needs at least two bins");
+      if (!(std::isfinite(o.sample_rate_hz) && o.sample_rate_hz > 0.) || !std::isfinite(o.start_time_s))
+        throw std::invalid_argument("observer sample rate and start time must be finite (rate > 0)");
+      total += o.number_of_bins;
+    }
+    std::uint64_t const out_total = total - observer_count;
+    std::uint64_t in_at = 0, out_at = 0;
+    for (std::uint64_t k = 0; k < observer_count; ++k) {
+      c8mi_radio_observer const& o = observers[k];
+      std::uint64_t const bins = o.number_of_bins;
+      double const rate = o.sample_rate_hz;
+      long double const period = 1.0L / static_cast<long double>(rate);
+      for (int c = 0; c < 3; ++c) {
+        double const* a = potential + c * total + in_at;
+        double* e = field + c * out_total + out_at;
+        for (std::uint64_t i = 0; i + 1 < bins; ++i) e[i] = -(a[i + 1] - a[i]) * rate;
+      }
+      if (times) {
+        long double const start = o.start_time_s;
+        for (std::uint64_t i = 0; i + 1 < bins; ++i) {
+          long double const t0 = start + static_cast<long double>(i) * period;
+          long double const t1 = start + static_cast<long double>(i + 1) * period;
+          times[out_at + i] = static_cast<double>((t0 + t1) / 2.0L);
+        }
+      }
+      in_at += bins;
+      out_at += bins - 1;
+    }
+  });
+}
+
 int c8mi_radio_set_tuning(c8mi_radio* radio, uint64_t chunk_tracks, uint32_t tile_bins,
                           uint32_t target_workgroups, uint32_t block_threads, int split) {
   return guarded([&] {

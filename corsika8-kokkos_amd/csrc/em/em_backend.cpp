@@ -420,12 +420,16 @@ void EmBackend::slot_enqueue(std::uint32_t slot, c8mi_particle_state const* part
   }
 }
 
+std::uint32_t EmBackend::active_slot_count() const {
+  std::uint32_t active = 0;
+  for (auto const& h : slot_host_) active += h.active ? 1u : 0u;
+  return active;
+}
+
 void EmBackend::append_run(int species, std::uint64_t end) {
   auto& r = runs_[species];
   if (!r.empty() && r.back() == end) return;
-  std::uint32_t active = 0;
-  for (auto const& h : slot_host_) active += h.active ? 1u : 0u;
-  if (active <= 1) {  // one shower: its queue is a single run in any order of appends
+  if (active_slot_count() <= 1) {  // one shower: its queue is a single run in any order of appends
     r.assign(1, end);
     return;
   }
@@ -565,6 +569,14 @@ void EmBackend::front(bool photon, c8mi_em_run_result& out) {
   std::uint64_t const cap = config_.queue_capacity;
   if (2 * n > cap || count_[o] + 2 * n > cap)
     throw std::length_error("em: a front's outputs could exceed the device queue capacity");
+  // Everything that can refuse this front is checked before the first launch:
+  // once the kernels run, the device ledger, profile and queues move on.
+  if (runs_[o].size() >= static_cast<std::size_t>(kMaxRuns) && active_slot_count() > 1)
+    throw std::logic_error("em: too many queued runs for a pool of showers; alternate photon and lepton fronts");
+  if (!photon) {
+    if (radio_) radio_->check_accumulate(n);
+    if (slot_radio_) slot_radio_->check_project(n, slot_radio_->chunk_for(n), active_range_ * kMaxRuns);
+  }
   resize_scratch(n);
   FrontArgs f{};
   f.in = queues_[s][current_[s]]->view();

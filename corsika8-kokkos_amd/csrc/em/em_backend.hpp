@@ -123,6 +123,7 @@ class EmBackend {
   void run_fronts(int species, std::uint64_t maximum_fronts, c8mi_em_run_result& out);
   void flush_staged(int species);
   void append_run(int species, std::uint64_t end);
+  std::uint32_t active_slot_count() const;
   void resize_scratch(std::uint64_t n);
   void check_slot(std::uint32_t slot) const;
   void update_active_range();

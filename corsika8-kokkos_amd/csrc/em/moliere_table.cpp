@@ -17,6 +17,8 @@
 #include "em/moliere_table.hpp"
 
 #include <array>
+#include <cerrno>
+#include <memory>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -24,6 +26,9 @@
 #include <stdexcept>
 #include <string>
 #include <vector>
+
+#include <fcntl.h>
+#include <unistd.h>
 
 namespace c8mi::em {
 namespace {
@@ -241,7 +246,12 @@ void initial_guess(c8mi_moliere_snapshot const& m, c8mi_moliere_cubic const* pol
 
 // ---- cache file ----
 
+// The reference's cache magic (its own CubicInterpolation-built tables), and the
+// magic this build writes: its natural-spline tables differ from the
+// reference's in the last digits, so a file written here must not pass the
+// reference's reader (which checks the magic) as one of its own.
 constexpr char kMagic[8] = {'C', '8', 'M', 'O', 'L', 'I', '0', '1'};
+constexpr char kBuilderMagic[8] = {'C', '8', 'M', 'I', 'M', 'O', '0', '1'};
 constexpr std::uint32_t kFormatVersion = 1;
 
 struct CacheHeader {
@@ -316,26 +326,71 @@ bool read_moliere_table(char const* path, c8mi_moliere_snapshot const& snapshot,
   CacheHeader header{};
   in.read(reinterpret_cast<char*>(&header), sizeof header);
   in.read(reinterpret_cast<char*>(&out), sizeof out);
-  return static_cast<bool>(in) && std::memcmp(header.magic, kMagic, sizeof kMagic) == 0 &&
-         header.format_version == kFormatVersion && header.snapshot_bytes == sizeof(c8mi_moliere_snapshot) &&
+  bool const magic = std::memcmp(header.magic, kMagic, sizeof kMagic) == 0 ||
+                     std::memcmp(header.magic, kBuilderMagic, sizeof kBuilderMagic) == 0;
+  return static_cast<bool>(in) && magic && header.format_version == kFormatVersion &&
+         header.snapshot_bytes == sizeof(c8mi_moliere_snapshot) &&
          header.table_bytes == sizeof(c8mi_moliere_interpolation) &&
          std::memcmp(&header.snapshot, &snapshot, sizeof snapshot) == 0 &&
          header.table_checksum == fnv1a(&out, sizeof out) && valid_moliere_table(out);
 }
 
+namespace {
+
+bool write_all(int fd, void const* data, std::size_t size) {
+  auto const* p = static_cast<char const*>(data);
+  while (size) {
+    ssize_t const n = ::write(fd, p, size);
+    if (n < 0) {
+      if (errno == EINTR) continue;
+      return false;
+    }
+    p += n;
+    size -= static_cast<std::size_t>(n);
+  }
+  return true;
+}
+
+}  // namespace
+
+// write_interpolation_cache (MoliereInterpolation.cpp:233-270): the table goes
+// to a private temporary file next to the target, is synced, read back and
+// compared, and only then renamed over the target, so a reader never sees a
+// partial file. Written with this build's magic (see kBuilderMagic).
 void write_moliere_table(char const* path, c8mi_moliere_snapshot const& snapshot,
                          c8mi_moliere_interpolation const& table) {
   CacheHeader header{};
-  std::memcpy(header.magic, kMagic, sizeof kMagic);
+  std::memcpy(header.magic, kBuilderMagic, sizeof kBuilderMagic);
   header.format_version = kFormatVersion;
   header.snapshot_bytes = sizeof(c8mi_moliere_snapshot);
   header.table_bytes = sizeof(c8mi_moliere_interpolation);
   header.table_checksum = fnv1a(&table, sizeof table);
   header.snapshot = snapshot;
-  std::ofstream outf(path, std::ios::binary | std::ios::trunc);
-  outf.write(reinterpret_cast<char const*>(&header), sizeof header);
-  outf.write(reinterpret_cast<char const*>(&table), sizeof table);
-  if (!outf) throw std::runtime_error(std::string("cannot write the Moliere cache file ") + path);
+  std::string const target(path);
+  std::vector<char> name(target.begin(), target.end());
+  char const suffix[] = ".tmp.XXXXXX";
+  name.insert(name.end(), suffix, suffix + sizeof suffix);  // with the terminator
+  int const fd = ::mkstemp(name.data());
+  if (fd < 0) throw std::runtime_error("cannot create a temporary file for the Moliere cache " + target);
+  std::string const temporary(name.data());
+  bool ok = write_all(fd, &header, sizeof header) && write_all(fd, &table, sizeof table) && ::fsync(fd) == 0;
+  ok = (::close(fd) == 0) && ok;
+  if (ok) {
+    auto verified = std::make_unique<c8mi_moliere_interpolation>();
+    ok = read_moliere_table(temporary.c_str(), snapshot, *verified) &&
+         std::memcmp(verified.get(), &table, sizeof table) == 0;
+  }
+  if (ok) ok = ::rename(temporary.c_str(), target.c_str()) == 0;
+  if (!ok) {
+    ::unlink(temporary.c_str());
+    throw std::runtime_error("cannot write the Moliere cache file " + target);
+  }
+  std::string const parent = target.find('/') == std::string::npos ? "." : target.substr(0, target.rfind('/') + 1);
+  int const dir = ::open(parent.c_str(), O_RDONLY | O_DIRECTORY);
+  if (dir >= 0) {
+    (void)::fsync(dir);
+    ::close(dir);
+  }
 }
 
 }  // namespace c8mi::em

@@ -170,6 +170,21 @@ void RadioAccumulator::accumulate_tracks(double const* device_tracks, std::uint6
   project(device_tracks, stride, count);
 }
 
+void RadioAccumulator::check_accumulate(std::uint64_t count) const {
+  if (count == 0) return;
+  if (sealed_) throw std::logic_error("radio endpoint already committed; reset before new tracks");
+  // the launch shapes project() will build (they throw on oversized batches)
+  bool const coreas = !coreas_host_.empty(), zhs = !zhs_host_.empty();
+  if (fuse_ && !split_) {
+    (void)base_launch(true, true, static_cast<std::uint32_t>(coreas_host_.size()),
+                      std::max(coreas_max_bins_, zhs_max_bins_), count);
+    return;
+  }
+  if (coreas)
+    (void)base_launch(true, false, static_cast<std::uint32_t>(coreas_host_.size()), coreas_max_bins_, count);
+  if (zhs) (void)base_launch(false, true, static_cast<std::uint32_t>(zhs_host_.size()), zhs_max_bins_, count);
+}
+
 ProjectionLaunch RadioAccumulator::base_launch(bool coreas, bool zhs, std::uint32_t observers,
                                                std::uint64_t max_bins,
                                                std::uint64_t count) const {
@@ -182,9 +197,14 @@ ProjectionLaunch RadioAccumulator::base_launch(bool coreas, bool zhs, std::uint3
   // tiles of that size (set_tuning may ask for others up to the LDS budget).
   std::size_t const words = kMaxDynamicLdsBytes / sizeof(long long);
   std::size_t const per_bin = (coreas ? 3u : 0u) + (zhs ? 3u : 0u);
-  std::uint64_t capacity = default_tile_capacity(per_bin);
+  std::uint64_t capacity = default_tile_capacity(per_bin, zhs);
   if (tile_bins_limit_ != 0) capacity = std::min<std::uint64_t>((words - 3) / per_bin, tile_bins_limit_);
-  std::uint64_t const tile = std::min<std::uint64_t>(std::max<std::uint64_t>(max_bins, 1), capacity);
+  std::uint64_t const want = std::max<std::uint64_t>(max_bins, 1);
+  std::uint64_t tile = std::min<std::uint64_t>(want, capacity);
+  if (tile_bins_limit_ == 0) {  // equal tiles: a 4096-bin window as two 2048-bin tiles
+    std::uint64_t const tiles = (want + capacity - 1) / capacity;
+    tile = (want + tiles - 1) / tiles;
+  }
   L.tile_bins = static_cast<std::uint32_t>(tile);
   L.bin_tiles = static_cast<std::uint32_t>((max_bins + tile - 1) / tile);
   if (L.bin_tiles == 0) L.bin_tiles = 1;

@@ -53,6 +53,9 @@ class RadioAccumulator {
   // Device-resident compact tracks, [9][stride] FP64 (c8mi_types.h).
   void accumulate_tracks(double const* device_tracks, std::uint64_t stride,
                          std::uint64_t count);
+  // Throws what accumulate_tracks of `count` valid tracks would throw, without
+  // launching anything (callers check before work that cannot be undone).
+  void check_accumulate(std::uint64_t count) const;
 
   // Decoded waveforms, [3][total_bins] per algorithm (x, y, z blocks). Either
   // output may be null. Throws OverflowError when the fixed-point range was

@@ -659,11 +659,25 @@ __device__ __forceinline__ Obs make_obs(ObserverDev const& d, PropagationDev con
   return o;
 }
 
-// Inclusive prefix sum over `count` entries of an LDS row, whole block.
+// Inclusive prefix sum over `count` entries of an LDS row, whole block: each
+// thread sums a contiguous run, the run totals are scanned within each wave by
+// shuffles and across waves by the first wave (wave_totals: one entry per wave).
 template <class T>
-__device__ void block_inclusive_scan(T* row, std::uint32_t count, T* scratch) {
+__device__ __forceinline__ T wave_inclusive_sum(T x, std::uint32_t lane) {
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    T const y = __shfl_up(x, d, 64);
+    if (static_cast<int>(lane) >= d) x += y;
+  }
+  return x;
+}
+
+template <class T>
+__device__ void block_inclusive_scan(T* row, std::uint32_t count, T* wave_totals) {
   std::uint32_t const tid = threadIdx.x;
   std::uint32_t const threads = blockDim.x;
+  std::uint32_t const lane = tid & 63u, wave = tid >> 6;
+  std::uint32_t const waves = (threads + 63u) >> 6;
   std::uint32_t const per = (count + threads - 1) / threads;
   std::uint32_t const begin = tid * per < count ? tid * per : count;
   std::uint32_t const end = begin + per < count ? begin + per : count;
@@ -672,18 +686,16 @@ __device__ void block_inclusive_scan(T* row, std::uint32_t count, T* scratch) {
     running += row[i];
     row[i] = running;
   }
-  scratch[tid] = running;
+  T const inclusive = wave_inclusive_sum(running, lane);
+  if (lane == 63u) wave_totals[wave] = inclusive;
   __syncthreads();
-  if (tid == 0) {
-    T acc = 0;
-    for (std::uint32_t i = 0; i < threads; ++i) {
-      T const v = scratch[i];
-      scratch[i] = acc;
-      acc += v;
-    }
+  if (wave == 0) {
+    T const total = lane < waves ? wave_totals[lane] : T(0);
+    T const before = wave_inclusive_sum(total, lane) - total;
+    if (lane < waves) wave_totals[lane] = before;
   }
   __syncthreads();
-  T const offset = scratch[tid];
+  T const offset = wave_totals[wave] + (inclusive - running);
   for (std::uint32_t i = begin; i < end; ++i) row[i] += offset;
   __syncthreads();
 }
@@ -692,7 +704,7 @@ template <bool Coreas, bool Zhs, bool Fixed, bool Exact, int Block>
 __global__ void __launch_bounds__(Block, 4) radio_project_kernel(ProjectionLaunch L) {
   using Acc = std::conditional_t<Fixed, long long, double>;
   extern __shared__ __align__(16) unsigned char lds_raw[];
-  __shared__ Acc scan_scratch[Zhs ? Block : 1];
+  __shared__ Acc scan_scratch[Zhs ? Block / 64 : 1];
   __shared__ unsigned long long block_counts[5];
   __shared__ double block_abs[2];
 

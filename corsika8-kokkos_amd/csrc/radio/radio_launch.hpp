@@ -16,16 +16,24 @@ namespace c8mi::radio {
 inline constexpr std::uint32_t kProjectionBlock = 256;
 // Dynamic LDS a projection workgroup may request (gfx950: 160 KiB per CU).
 inline constexpr std::size_t kMaxDynamicLdsBytes = 144 * 1024;
-// Default window tile: at most this many LDS bytes of bins per workgroup, so
-// three workgroups share a CU. A 4096-bin ZHS window in one 96 KiB tile (one
-// workgroup per CU) measured 2x slower than two 2048-bin tiles.
-inline constexpr std::size_t kPreferredTileLdsBytes = 48 * 1024;
+// Default window tile: its dynamic LDS plus the kernel's static LDS stay within
+// this budget, so three workgroups share a CU (160 KiB). A 4096-bin ZHS window
+// in one 96 KiB tile (one workgroup per CU) measured 2x slower than two
+// 2048-bin tiles; a 1024-bin CoREAS+ZHS window (48 KiB of bins) and a 2048-bin
+// ZHS window keep one tile.
+inline constexpr std::size_t kPreferredWorkgroupLdsBytes = 53 * 1024;
+// Upper bound of the projection kernel's static LDS: the wave totals of the ZHS
+// scan (16 x 8 B for 1024 threads), block counters and |increment| sums.
+inline constexpr std::size_t kProjectionStaticLdsBytes = 256;
 // Fewest tracks per workgroup of the default chunking.
 inline constexpr std::uint64_t kMinimumDefaultChunk = 8192;
-// Bins of the default tile for `per_bin` int64 words per bin (3 per algorithm).
-inline std::uint64_t default_tile_capacity(std::size_t per_bin) {
-  std::uint64_t const most = (kMaxDynamicLdsBytes / sizeof(long long) - 3) / per_bin;
-  std::uint64_t const preferred = kPreferredTileLdsBytes / sizeof(long long) / per_bin;
+// Bins of the default tile for `per_bin` int64 words per bin (3 per algorithm);
+// a ZHS tile holds one more word per component row (its difference array).
+inline std::uint64_t default_tile_capacity(std::size_t per_bin, bool zhs) {
+  std::size_t const extra = zhs ? 3u : 0u;
+  std::uint64_t const most = (kMaxDynamicLdsBytes / sizeof(long long) - extra) / per_bin;
+  std::uint64_t const preferred =
+      ((kPreferredWorkgroupLdsBytes - kProjectionStaticLdsBytes) / sizeof(long long) - extra) / per_bin;
   return preferred < most ? preferred : most;
 }
 

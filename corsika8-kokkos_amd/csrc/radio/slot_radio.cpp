@@ -29,8 +29,9 @@ SlotRadio::SlotRadio(c8mi_radio_config const& config, std::uint32_t slots, std::
   // Both algorithms share every observer, so they share one launch.
   fuse_ = config.coreas_enabled && config.zhs_enabled;
   std::size_t const per_bin = (config.coreas_enabled ? 3u : 0u) + (config.zhs_enabled ? 3u : 0u);
-  std::uint64_t const capacity = default_tile_capacity(per_bin);
-  tile_bins_ = static_cast<std::uint32_t>(std::min<std::uint64_t>(bins_per_observer, capacity));
+  std::uint64_t const capacity = default_tile_capacity(per_bin, config.zhs_enabled != 0);
+  std::uint64_t const tiles = (bins_per_observer + capacity - 1) / capacity;
+  tile_bins_ = static_cast<std::uint32_t>((bins_per_observer + tiles - 1) / tiles);
   bin_tiles_ = static_cast<std::uint32_t>((bins_per_observer + tile_bins_ - 1) / tile_bins_);
   if (stream == nullptr) {
     C8MI_HIP(hipStreamCreate(&stream_));
@@ -159,6 +160,19 @@ void SlotRadio::download(std::uint32_t slot, std::int64_t* coreas, std::int64_t*
   }
 }
 
+// Every chunk of every segment on every observer: at most one partial chunk
+// per segment beyond count / chunk.
+std::uint64_t SlotRadio::grid_for(std::uint64_t count, std::uint64_t chunk, std::uint64_t max_segments) const {
+  return ((count + chunk - 1) / chunk + max_segments) * static_cast<std::uint64_t>(observers_per_slot_) * bin_tiles_;
+}
+
+void SlotRadio::check_project(std::uint64_t count, std::uint64_t chunk, std::uint64_t max_segments) const {
+  if (count == 0) return;
+  if (chunk == 0) throw std::invalid_argument("per-shower radio chunk of zero tracks");
+  if (grid_for(count, chunk, max_segments) > 0x7fffffffULL)
+    throw std::length_error("per-shower radio launch above the grid limit");
+}
+
 std::uint64_t SlotRadio::chunk_for(std::uint64_t count) const noexcept {
   constexpr std::uint64_t kTargetItems = 8192, kMinimumChunk = 4096, kMaximumChunk = 1 << 20;
   std::uint64_t const per_item = static_cast<std::uint64_t>(observers_per_slot_) * bin_tiles_;
@@ -192,11 +206,8 @@ void SlotRadio::project(double const* tracks, std::uint64_t stride, std::uint64_
   L.item_count = item_count;
   L.slot_observer_counts = observer_counts_.data();
   L.slot_observers = observers_per_slot_;
-  // every chunk of every segment on every observer: at most one partial
-  // chunk per segment beyond count / chunk_size
-  std::uint64_t const grid = ((count + chunk - 1) / chunk + max_segments) *
-                             static_cast<std::uint64_t>(observers_per_slot_) * bin_tiles_;
-  if (grid > 0x7fffffffULL) throw std::length_error("per-shower radio launch above the grid limit");
+  check_project(count, chunk, max_segments);
+  std::uint64_t const grid = grid_for(count, chunk, max_segments);
   std::array<hipEvent_t, 2> events{nullptr, nullptr};
   if (timing_) {
     C8MI_HIP(hipEventCreate(&events[0]));

@@ -59,6 +59,9 @@ class SlotRadio {
   // max_segments, *segment_count of them) give each slot's track ranges and
   // item offsets (chunks x observers x tiles), *item_count their total.
   // Asynchronous on stream().
+  // Throws what project() of `count` tracks in chunks of `chunk` over at most
+  // `max_segments` segments would throw, without launching anything.
+  void check_project(std::uint64_t count, std::uint64_t chunk, std::uint64_t max_segments) const;
   void project(double const* tracks, std::uint64_t stride, std::uint64_t count, std::uint64_t chunk,
                SegmentDev const* segments, std::uint32_t const* segment_count,
                unsigned long long const* item_count, std::uint32_t max_segments);
@@ -68,6 +71,7 @@ class SlotRadio {
   void kernel_time_ms(double& ms, std::uint64_t& launches);
 
  private:
+  std::uint64_t grid_for(std::uint64_t count, std::uint64_t chunk, std::uint64_t max_segments) const;
   c8mi_radio_config config_{};
   std::uint32_t slots_ = 0;
   std::uint32_t observers_per_slot_ = 0;

@@ -227,10 +227,16 @@ int c8mi_em_moliere_interpolation_build(const c8mi_moliere_snapshot* snapshot,
                                         c8mi_moliere_interpolation* out);
 
 /* read_interpolation_cache / write_interpolation_cache
- * (MoliereInterpolation.cpp:204-270): the reference's "C8MOLI01" version-1
- * cache file (header with the snapshot and an FNV-1a checksum of the table,
- * then the table). Reading fails (status 1, invalid argument) unless the file
- * is complete, its checksum and snapshot match and every entry is finite. */
+ * (MoliereInterpolation.cpp:204-270): the reference's version-1 cache file
+ * (header with the snapshot and an FNV-1a checksum of the table, then the
+ * table). Reading accepts the reference's magic "C8MOLI01" and this build's
+ * "C8MIMO01", and fails (status 1, invalid argument) unless the file is
+ * complete, its checksum and snapshot match and every entry is finite.
+ * Writing stamps "C8MIMO01" (the table may be this build's natural-spline
+ * restatement, which the reference's reader must not take for its own), goes
+ * through a temporary file in the same directory that is synced, read back and
+ * compared before it is renamed over `path`; on failure (status 4) `path` is
+ * untouched. */
 int c8mi_em_moliere_interpolation_read(const char* path, const c8mi_moliere_snapshot* snapshot,
                                        c8mi_moliere_interpolation* out);
 int c8mi_em_moliere_interpolation_write(const char* path, const c8mi_moliere_snapshot* snapshot,

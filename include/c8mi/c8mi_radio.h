@@ -75,6 +75,20 @@ int c8mi_radio_download(c8mi_radio* radio, double* coreas, double* zhs,
 int c8mi_radio_download_fixed(c8mi_radio* radio, int64_t* coreas, int64_t* zhs,
                               c8mi_radio_counters* counters);
 
+/* The ZHS output step of the reference's radio writer
+ * (corsika/detail/modules/radio/RadioProcess.inl:212-220; the bin-integrated
+ * potential is described at TimeDomainObserver.inl:107-119): from a decoded
+ * ZHS vector potential A, [3][total_bins] as c8mi_radio_download returns it
+ * (observers' bins in order), the electric field
+ *   E[i] = -(A[i+1] - A[i]) * sample_rate   at the bin midpoint
+ *   t[i] = (t_axis[i] + t_axis[i+1]) / 2,   t_axis[i] = start_time + i / sample_rate,
+ * for i < bins - 1 of every observer. `field` is [3][total_bins - observer_count]
+ * (bins - 1 samples per observer, in observer order); `times` (NULL allowed)
+ * holds the matching [total_bins - observer_count] midpoint times in seconds.
+ * Host only: no device call. */
+int c8mi_radio_zhs_electric_field(const c8mi_radio_observer* observers, uint64_t observer_count,
+                                  const double* potential, double* field, double* times);
+
 /* TuningParameters (track/observer tiles, team size; TuningParameters.hpp)
  * analogue: tracks per workgroup (multiple of the workgroup size,
  * 0 = automatic), LDS window tile in bins (0 = automatic), target workgroups

@@ -493,12 +493,29 @@ def test_moliere_initial_guess_inverts_the_mixture_cdf(moliere, moliere_table):
 
 
 def test_moliere_cache_file_round_trip(tmp_path, moliere, moliere_table):
-    """The reference's C8MOLI01 cache format: written and read back unchanged;
-    a corrupted table or another snapshot is refused."""
+    """The reference's cache format: written and read back unchanged; a
+    corrupted table or another snapshot is refused. Files written here carry
+    this build's magic, files with the reference's magic are read as well, and
+    a failed write leaves the target and no temporary behind."""
     path = tmp_path / "moliere.cache"
     moliere_table.write(path, moliere)
     back = em.MoliereInterpolation.read(path, moliere)
     assert bytes(back) == bytes(moliere_table)
+    raw = path.read_bytes()
+    assert raw[:8] == b"C8MIMO01"
+    assert sorted(p.name for p in tmp_path.iterdir()) == ["moliere.cache"]
+    reference_made = tmp_path / "reference.cache"
+    reference_made.write_bytes(b"C8MOLI01" + raw[8:])
+    assert bytes(em.MoliereInterpolation.read(reference_made, moliere)) == bytes(moliere_table)
+    foreign = tmp_path / "foreign.cache"
+    foreign.write_bytes(b"C8MOLI02" + raw[8:])
+    with pytest.raises(c8mi.C8miError):
+        em.MoliereInterpolation.read(foreign, moliere)
+    missing_dir = tmp_path / "absent" / "moliere.cache"
+    with pytest.raises(c8mi.C8miError) as err:
+        moliere_table.write(missing_dir, moliere)
+    assert err.value.status == 4
+    assert not (tmp_path / "absent").exists()
     raw = bytearray(path.read_bytes())
     raw[-8] ^= 1
     bad = tmp_path / "bad.cache"

@@ -283,3 +283,76 @@ def test_slot_radio_contract(tables):
     c, z, counters = dev.slot_download_radio(1)
     assert c.shape == (3, 64) and not c.any() and not z.any() and counters["valid_tracks"] == 0
     dev.close()
+
+
+MULTI_TILE_CASES = [  # pid, energy (GeV), shower id, antenna ring radius (m), bins
+    (22, 2.0, 41, 40.0, 2500), (11, 1.5, 42, 90.0, 700), (-11, 1.0, 43, 25.0, 3000)]
+
+
+def test_pool_slot_radio_windows_longer_than_one_tile(tables):
+    """Per-shower radio with CoREAS and ZHS on windows of up to 3000 bins: each
+    observer's window is split into several LDS tiles (one workgroup per
+    chunk, observer and tile), with observers of mixed lengths. Every shower
+    equals its oracle projection, in a pool of its own and in the mix."""
+    atm = em.Atmosphere()
+    prop = propagation()
+    for slots in (1, 3):
+        cfg = em.make_config(airtables, seed=29, atmosphere=atm, slots=slots, moliere=True, moliere_table=True)
+        cases = [slot_case(tables, atm, cfg, c) for c in MULTI_TILE_CASES]
+        dev = em.EmBackend(cfg, tables, atm)
+        dev.enable_slot_radio(3, 3000, prop, exact=True)
+        got = []
+        if slots == 1:
+            for case, (prim, _, observers) in zip(MULTI_TILE_CASES, cases):
+                dev.slot_begin(0, case[2], 2)
+                dev.slot_set_radio(0, observers)
+                dev.slot_enqueue(0, prim)
+                dev.run_rounds(1 << 20)
+                got.append(dev.slot_download_radio(0))
+                dev.slot_end(0)
+        else:
+            for slot, (case, (prim, _, observers)) in enumerate(zip(MULTI_TILE_CASES, cases)):
+                dev.slot_begin(slot, case[2], 2)
+                dev.slot_set_radio(slot, observers)
+                dev.slot_enqueue(slot, prim)
+            dev.run_rounds(1 << 20)
+            got = [dev.slot_download_radio(slot) for slot in range(3)]
+        dev.close()
+        for (prim, tracks, observers), g in zip(cases, got):
+            ref = oracle_lib.radio_project(prop, observers, tracks, threads=8)
+            assert ref["counters"]["valid_tracks"] > 0
+            assert_same_radio(g, ref, observers)
+
+
+def test_refused_front_leaves_the_shower_unchanged(tables):
+    """A lepton front that the attached accumulator would refuse (downloaded,
+    not reset) is refused before any launch: pending counts, statistics and the
+    ledger are unchanged, and after a reset the shower runs on to the oracle's
+    result."""
+    atm = em.Atmosphere()
+    cfg = em.make_config(airtables, seed=13, atmosphere=atm, moliere=True)
+    prim = em.particle(11, 2.0, START, DOWN, history_id=1)
+    want = oracle_lib.em_run_shower(cfg, tables, atm, prim, shower_id=1, first_history_id=2)["stats"]
+    observers = window_observers(np.zeros((9, 1)), ground_antennas(), 1.0e9, 64)
+    dev = em.EmBackend(cfg, tables, atm)
+    radio = c8mi.RadioAccumulator(observers, propagation(), exact=True)
+    dev.attach_radio(radio)
+    dev.begin_shower(1, 2)
+    dev.enqueue(prim)
+    first = dev.run(maximum_fronts=3)
+    assert first["pending_leptons"] > 0
+    radio.download_fixed()  # commits the accumulator
+    before = (dev.pending(), dev.slot_state(0), dev.download_profile()[1])
+    with pytest.raises(c8mi.C8miError) as err:
+        dev.run_lepton_wavefront([], 1)  # the lepton front alone: refused first thing
+    assert err.value.status == 2
+    after = (dev.pending(), dev.slot_state(0), dev.download_profile()[1])
+    assert after == before
+    radio.reset()
+    rest = dev.run()
+    dev.attach_radio(None)
+    dev.close()
+    radio.close()
+    for k in ("photon_steps", "lepton_steps", "photon_interactions", "lepton_interactions", "secondaries"):
+        assert first[k] + rest[k] == want[k], k
+    assert rest["next_history_id"] == want["next_history_id"]

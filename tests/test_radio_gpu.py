@@ -304,3 +304,125 @@ def test_fast_mode_random_and_config2(config2_small):
     other = gpu_project(tracks, obs, prop, exact=False,
                         tuning=dict(block_threads=1024, split=False, tile_bins=700))
     assert np.array_equal(other[0], fast[0]) and np.array_equal(other[1], fast[1])
+
+
+# ---- in-ice ZHS: homogeneous index n = 1.78, antennas on the Cherenkov cone ----
+# The in-ice workload (1 PeV electron in ice, ZHS at 64 antennas on the
+# Cherenkov ring 1 km from the shower maximum, 4096 bins @ 0.1 ns) projects
+# through the constant-index branch of propagate (RadioProjection.inl:178-181)
+# and, on the cone, the near-Cherenkov midpoint form of the endpoint formulae
+# (RadioProjectionData.hpp:33). Tracks come from the oracle's in-ice shower.
+
+ICE_INDEX = 1.78
+ICE_START, ICE_DOWN = (0.0, 0.0, -100.0), (0.0, 0.0, -1.0)
+_ICE_TABLES = {}
+
+
+def ice_shower_tracks(energy, seed):
+    from c8mi import airtables, em
+
+    if "tables" not in _ICE_TABLES:
+        _ICE_TABLES["tables"] = airtables.build_ice_tables(cut_mev=0.1)
+    ice = em.IceVolume()
+    cfg = em.make_config(airtables.IceMedium, seed=seed, cut_mev=0.1, start=ICE_START, direction=ICE_DOWN,
+                         atmosphere=ice, thinning=(1.0e-6 * energy, 1.0e4), moliere=True, moliere_table=True,
+                         bins=256)
+    prim = em.particle(11, energy, ICE_START, ICE_DOWN, history_id=1)
+    tracks = oracle_lib.em_run_shower(cfg, _ICE_TABLES["tables"], ice, prim, tracks=True)["tracks"]
+    depth = 36.08 / 0.92 / 100.0 * np.log(energy * 1.0e3 / 78.0)
+    return np.ascontiguousarray(tracks), depth
+
+
+def ice_ring(depth, count=64):
+    x_max = np.asarray(ICE_START) + depth * np.asarray(ICE_DOWN)
+    return workloads.cherenkov_ring_observers(x_max, ICE_DOWN, 1000.0, ICE_INDEX, count=count,
+                                              sample_rate_hz=1.0e10, number_of_bins=4096,
+                                              emission_time_s=depth / workloads.SPEED_OF_LIGHT)
+
+
+def near_cone_fraction(tracks, observers, sample=4000):
+    """Fraction of (track, antenna) pairs with |1 - n beta.e| < 1e-3 (the
+    reference's near-Cherenkov threshold), from the geometry alone."""
+    idx = np.linspace(0, tracks.shape[1] - 1, min(sample, tracks.shape[1])).astype(int)
+    t = tracks[:, idx]
+    mid = 0.5 * (t[0:3] + t[3:6])
+    beta = (t[3:6] - t[0:3]) / (299792458.0 * (t[7] - t[6]))
+    near = 0
+    for p in observers["position_m"]:
+        e = np.asarray(p)[:, None] - mid
+        e /= np.linalg.norm(e, axis=0)
+        near += np.count_nonzero(np.abs(1.0 - ICE_INDEX * np.einsum("ij,ij->j", beta, e)) < 1.0e-3)
+    return near / (idx.size * observers.size)
+
+
+def assert_zhs_fast_close(got, ref, observers):
+    _, z, counters = got
+    nbins = observers["number_of_bins"].astype(np.int64)
+    offsets = np.concatenate([[0], np.cumsum(nbins)])
+    decoded = oracle_lib.radio_decode(z, observers, potential=True)
+    want = ref["zhs_float"]
+    for k in range(observers.size):
+        sl = slice(offsets[k], offsets[k + 1])
+        norm = np.linalg.norm(want[:, sl])
+        assert norm > 0
+        assert np.linalg.norm(decoded[:, sl] - want[:, sl]) <= 1e-6 * norm
+    assert counters["valid_tracks"] == ref["counters"]["valid_tracks"]
+    assert counters["zhs_contributions"] == ref["counters"]["zhs_contributions"]
+    assert counters["fixed_point_overflows"] == 0
+
+
+def test_in_ice_cherenkov_ring_bit_exact_fast_and_tiled():
+    tracks, depth = ice_shower_tracks(10.0, 5)
+    obs = ice_ring(depth)
+    print(f"in-ice ring: {tracks.shape[1]} tracks x {obs.size} antennas x 4096 bins", flush=True)
+    assert near_cone_fraction(tracks, obs) > 1.0e-3
+    prop = c8mi.Propagation(homogeneous_index=ICE_INDEX)
+    ref = oracle_lib.radio_project(prop, obs, tracks, coreas=False, threads=8)
+    assert ref["counters"]["zhs_contributions"] > 0
+    exact = gpu_project(tracks, obs, prop, coreas=False)
+    assert_same(exact, ref, coreas=False)
+    # window tiles forced below, at and above the default split of a 4096-bin window
+    for tuning in (dict(tile_bins=1000), dict(tile_bins=4096), dict(tile_bins=1365, block_threads=1024),
+                   dict(chunk_tracks=512, block_threads=256)):
+        print(f"in-ice tuning {tuning}", flush=True)
+        other = gpu_project(tracks, obs, prop, coreas=False, tuning=tuning)
+        assert np.array_equal(other[1], exact[1]) and other[2] == exact[2]
+    ref_float = oracle_lib.radio_project(prop, obs, tracks, coreas=False, threads=8, fixed=False)
+    assert_zhs_fast_close(gpu_project(tracks, obs, prop, coreas=False, exact=False), ref_float, obs)
+    # CoREAS through the same constant-index branch and near-cone midpoint form
+    cref = oracle_lib.radio_project(prop, obs[:16], tracks, zhs=False, threads=8)
+    assert_same(gpu_project(tracks, obs[:16], prop, zhs=False), cref, zhs=False)
+
+
+def test_in_ice_full_size_superposition_permutation_and_error():
+    """About 1.3e6 tracks of a 450 GeV in-ice shower on the 64-antenna ring:
+    split input and shuffled input give the same int64 sums, and the FP64 error
+    of both projection modes on four antennas is the fixed-point quantisation."""
+    tracks, depth = ice_shower_tracks(450.0, 11)
+    n = tracks.shape[1]
+    assert n > 1_000_000
+    obs = ice_ring(depth)
+    prop = c8mi.Propagation(homogeneous_index=ICE_INDEX)
+    print(f"in-ice full size: {n} tracks x {obs.size} antennas", flush=True)
+    whole = gpu_project(tracks, obs, prop, coreas=False)
+    acc = c8mi.RadioAccumulator(obs, prop, coreas=False, exact=True)
+    half = n // 3
+    first = c8mi.DeviceArray.from_host(np.ascontiguousarray(tracks[:, :half]))
+    second = c8mi.DeviceArray.from_host(np.ascontiguousarray(tracks[:, half:]))
+    acc.accumulate_tracks(first, half, half)
+    acc.accumulate_tracks(second, n - half, n - half)
+    split = acc.download_fixed()
+    acc.close()
+    assert np.array_equal(split[1], whole[1]) and split[2] == whole[2]
+    perm = np.random.default_rng(3).permutation(n)
+    shuffled = gpu_project(np.ascontiguousarray(tracks[:, perm]), obs, prop, coreas=False)
+    assert np.array_equal(shuffled[1], whole[1])
+    assert whole[2]["valid_tracks"] == n
+    subset = obs[::16].copy()
+    ref = oracle_lib.radio_project(prop, subset, tracks, coreas=False, threads=16, fixed=False)
+    for exact in (True, False):
+        _, z, _ = gpu_project(tracks, subset, prop, coreas=False, exact=exact)
+        zd = oracle_lib.radio_decode(z, subset, potential=True)
+        err = np.linalg.norm(zd - ref["zhs_float"]) / np.linalg.norm(ref["zhs_float"])
+        print(f"in-ice FP64 error ({'exact' if exact else 'fast'}): {err:.3e}", flush=True)
+        assert err <= 1e-6

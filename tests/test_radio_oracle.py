@@ -182,13 +182,26 @@ def test_zhs_leading_edge_reproduces_coreas_start_field():
     start_bin = int(np.flatnonzero(np.abs(coreas).sum(axis=0))[0])
     nz = np.flatnonzero(np.abs(zhs).sum(axis=0))
     assert len(nz) > 8
+    # The product's output step (c8mi_radio_zhs_electric_field): across the
+    # rising edge E = -(A[i+1] - A[i]) f_s is minus one full bin's potential
+    # times f_s, which must equal the CoREAS start field; the falling edge the
+    # CoREAS end field.
+    field, times = c8mi.zhs_electric_field(obs, zhs)
+    assert field.shape == (3, zhs.shape[1] - 1)
     full = zhs[:, nz[len(nz) // 2]]
-    # E = -(A[i+1] - A[i]) f_s across the rising edge; the box height is one
-    # full bin's potential.
-    e_from_zhs = -full * rate
-    assert np.allclose(coreas[:, start_bin], e_from_zhs, rtol=3e-3, atol=0)
+    edge = np.flatnonzero(np.abs(field).sum(axis=0))
+    e_start = field[:, edge[0]:edge[0] + 2].sum(axis=1)
+    assert np.allclose(coreas[:, start_bin], e_start, rtol=3e-3, atol=0)
+    assert np.allclose(e_start, -full * rate, rtol=3e-3, atol=0)
     end_bin = int(np.flatnonzero(np.abs(coreas).sum(axis=0))[-1])
-    assert np.allclose(coreas[:, end_bin], -e_from_zhs, rtol=3e-3, atol=0)
+    e_end = field[:, edge[-1] - 1:edge[-1] + 1].sum(axis=1)
+    assert np.allclose(coreas[:, end_bin], e_end, rtol=3e-3, atol=0)
+    # midpoint times: half a bin after each axis time
+    axis = obs[0]["start_time_s"] + np.arange(zhs.shape[1]) / rate
+    assert np.allclose(times, 0.5 * (axis[1:] + axis[:-1]), rtol=0, atol=1e-18)
+    # between the edges the field vanishes (a constant potential)
+    inner = field[:, edge[0] + 2:edge[-1] - 1]
+    assert np.abs(inner).max() <= 1e-9 * np.abs(e_start).max()
 
 
 def test_charge_and_weight_linearity():
