@@ -37,6 +37,22 @@ inline std::uint64_t default_tile_capacity(std::size_t per_bin, bool zhs) {
   return preferred < most ? preferred : most;
 }
 
+// Launch-shape knobs of a projection (set_tuning): tracks per workgroup
+// (0 = automatic), window tile limit in bins (0 = automatic), target
+// workgroups per launch and threads per workgroup.
+struct LaunchTuning {
+  std::uint64_t chunk_tracks = 0;
+  std::uint32_t tile_bins_limit = 0;
+  std::uint32_t target_workgroups = 8192;
+  std::uint32_t block_threads = 512;
+};
+
+// Fills the tile and chunk fields of `L` (tile_bins, bin_tiles, chunk_size,
+// chunks) for `count` tracks on `observers` observers whose longest window has
+// `max_bins` bins. Throws length_error when the grid exceeds 2^32 workgroups.
+void shape_projection(ProjectionLaunch& L, bool coreas, bool zhs, std::uint32_t observers,
+                      std::uint64_t max_bins, std::uint64_t count, LaunchTuning const& tuning);
+
 // LDS bytes for one workgroup holding `tile_bins` bins of each enabled mode.
 std::size_t projection_lds_bytes(bool coreas, bool zhs, std::uint32_t tile_bins);
 // Workgroups launched for `launch` (chunks rounded up to whole XCD rounds).

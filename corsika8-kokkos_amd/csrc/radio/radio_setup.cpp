@@ -2,6 +2,8 @@
 // accumulator and the per-slot radio of the EM pool.
 #include "radio_setup.hpp"
 
+#include "radio_launch.hpp"
+
 #include <algorithm>
 #include <cmath>
 #include <limits>
@@ -118,6 +120,47 @@ PropagationDev propagation_view(c8mi_radio_propagation const& p, double const* r
   d.integrated = integrated;
   d.table_size = p.homogeneous_refractive_index > 0. ? 0 : p.table_size;
   return d;
+}
+
+void shape_projection(ProjectionLaunch& L, bool coreas, bool zhs, std::uint32_t observers,
+                      std::uint64_t max_bins, std::uint64_t count, LaunchTuning const& tuning) {
+  // Window tile: the whole waveform when it fits the preferred LDS size, else
+  // equal tiles of at most that size (a tuning limit may ask for others up to
+  // the LDS budget).
+  std::size_t const words = kMaxDynamicLdsBytes / sizeof(long long);
+  std::size_t const per_bin = (coreas ? 3u : 0u) + (zhs ? 3u : 0u);
+  std::uint64_t capacity = default_tile_capacity(per_bin, zhs);
+  if (tuning.tile_bins_limit != 0)
+    capacity = std::min<std::uint64_t>((words - 3) / per_bin, tuning.tile_bins_limit);
+  std::uint64_t const want = std::max<std::uint64_t>(max_bins, 1);
+  std::uint64_t tile = std::min<std::uint64_t>(want, capacity);
+  if (tuning.tile_bins_limit == 0) {  // equal tiles: a 4096-bin window as two 2048-bin tiles
+    std::uint64_t const tiles = (want + capacity - 1) / capacity;
+    tile = (want + tiles - 1) / tiles;
+  }
+  L.tile_bins = static_cast<std::uint32_t>(tile);
+  L.bin_tiles = static_cast<std::uint32_t>((max_bins + tile - 1) / tile);
+  if (L.bin_tiles == 0) L.bin_tiles = 1;
+  std::uint64_t chunk = tuning.chunk_tracks;
+  if (chunk == 0) {
+    std::uint64_t const per_chunk = std::max<std::uint64_t>(1, std::uint64_t(observers) * L.bin_tiles);
+    std::uint64_t const chunks_wanted = std::max<std::uint64_t>(1, tuning.target_workgroups / per_chunk);
+    chunk = (count + chunks_wanted - 1) / chunks_wanted;
+    // a workgroup clears and flushes a whole window tile: below a few thousand
+    // tracks that overhead dominates (a lepton front's batch of ~1e5 tracks on
+    // 64 antennas x 2 tiles measured 1.6x faster with 4x larger chunks)
+    chunk = std::max<std::uint64_t>(chunk, std::min<std::uint64_t>(count, kMinimumDefaultChunk));
+    std::uint64_t const block = tuning.block_threads;
+    chunk = std::max<std::uint64_t>(block, (chunk + block - 1) / block * block);
+  }
+  L.chunk_size = chunk;
+  std::uint64_t const chunks = (count + chunk - 1) / chunk;
+  if (chunks > std::numeric_limits<std::uint32_t>::max())
+    throw std::length_error("radio track batch too large for one launch");
+  L.chunks = static_cast<std::uint32_t>(chunks);
+  std::uint64_t const grid = (chunks + 7) / 8 * 8 * std::uint64_t(observers) * L.bin_tiles;
+  if (grid > std::numeric_limits<std::uint32_t>::max())
+    throw std::length_error("radio projection grid too large");
 }
 
 }  // namespace c8mi::radio

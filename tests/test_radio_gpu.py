@@ -343,7 +343,8 @@ def ice_ring(depth, count=64):
 def near_cone_fraction(tracks, observers, sample=4000):
     """Fraction of (track, antenna) pairs with |1 - n beta.e| < 1e-3 (the
     reference's near-Cherenkov threshold), from the geometry alone."""
-    idx = np.linspace(0, tracks.shape[1] - 1, min(sample, tracks.shape[1])).astype(int)
+    moving = np.flatnonzero((tracks[7] > tracks[6]) & (tracks[8] != 0.0))
+    idx = moving[np.linspace(0, moving.size - 1, min(sample, moving.size)).astype(int)]
     t = tracks[:, idx]
     mid = 0.5 * (t[0:3] + t[3:6])
     beta = (t[3:6] - t[0:3]) / (299792458.0 * (t[7] - t[6]))
@@ -417,9 +418,11 @@ def test_in_ice_full_size_superposition_permutation_and_error():
     perm = np.random.default_rng(3).permutation(n)
     shuffled = gpu_project(np.ascontiguousarray(tracks[:, perm]), obs, prop, coreas=False)
     assert np.array_equal(shuffled[1], whole[1])
-    assert whole[2]["valid_tracks"] == n
     subset = obs[::16].copy()
     ref = oracle_lib.radio_project(prop, subset, tracks, coreas=False, threads=16, fixed=False)
+    # e+- steps without a track (cut at rest, no movement) carry zero weight and
+    # are counted invalid, as the oracle counts them
+    assert whole[2]["valid_tracks"] == ref["counters"]["valid_tracks"] > 0.9 * n
     for exact in (True, False):
         _, z, _ = gpu_project(tracks, subset, prop, coreas=False, exact=exact)
         zd = oracle_lib.radio_decode(z, subset, potential=True)
