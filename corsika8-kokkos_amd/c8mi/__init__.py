@@ -329,6 +329,42 @@ class RadioAccumulator:
         return {kind: (ms[k], launches[k]) for k, kind in enumerate(("coreas_zhs", "coreas", "zhs"))}
 
 
+def radio_project_fixed_noalloc(observers, propagation: Propagation, device_tracks, stride: int,
+                                count: int, device_coreas=None, device_zhs=None, coreas: bool = True,
+                                zhs: bool = True, fixed_point_limit: float = 1.0,
+                                zhs_refinement: int = 1, zhs_observers=None, exact: bool = False,
+                                stream=None):
+    """One-shot projection without device allocation
+    (c8mi_radio_project_fixed_noalloc): int64 waveforms are written into the
+    caller's device buffers (DeviceArray or raw pointers, [3][total bins] each),
+    which the call clears. Returns the counters as a dict."""
+    obs = observers_array(observers)
+    zobs = obs if zhs_observers is None else observers_array(zhs_observers)
+    cfg = RadioConfig()
+    cfg.coreas_enabled = 1 if coreas else 0
+    cfg.zhs_enabled = 1 if zhs else 0
+    cfg.deterministic = 1
+    cfg.zhs_subtrack_refinement = zhs_refinement
+    cfg.exact_fp64 = 1 if exact else 0
+    cfg.fixed_point_field_limit_v_per_m = fixed_point_limit
+    cfg.propagation = propagation.struct
+    cfg.coreas_observers = obs.ctypes.data_as(POINTER(RadioObserver))
+    cfg.coreas_observer_count = obs.size if coreas else 0
+    cfg.zhs_observers = zobs.ctypes.data_as(POINTER(RadioObserver))
+    cfg.zhs_observer_count = zobs.size if zhs else 0
+
+    def pointer(buffer):
+        if buffer is None:
+            return None
+        return buffer.pointer if isinstance(buffer, DeviceArray) else c_void_p(buffer)
+
+    counters = RadioCounters()
+    _check(library().c8mi_radio_project_fixed_noalloc(
+        byref(cfg), pointer(device_tracks), c_uint64(stride), c_uint64(count),
+        pointer(device_coreas), pointer(device_zhs), byref(counters), c_void_p(stream)))
+    return counters.as_dict()
+
+
 # ---- EM: device tables, batched queries, Philox (c8mi_em.h) -----------------
 
 RANDOM_KEY_DTYPE = np.dtype(

@@ -10,6 +10,7 @@
 
 #include "c8mi/c8mi_radio.h"
 #include "radio/radio_accumulator.hpp"
+#include "radio/radio_noalloc.hpp"
 #include "capi_guard.hpp"
 #include "capi_handles.hpp"
 #include "runtime.hpp"
@@ -167,6 +168,17 @@ int c8mi_radio_kernel_time(c8mi_radio* radio, double ms[3], uint64_t launches[3]
   return guarded([&] {
     if (!radio) throw std::invalid_argument("null radio handle");
     radio->accumulator.kernel_time_ms(ms, launches);
+  });
+}
+
+int c8mi_radio_project_fixed_noalloc(c8mi_radio_config const* config, double const* device_tracks,
+                                     uint64_t stride, uint64_t count, int64_t* device_coreas,
+                                     int64_t* device_zhs, c8mi_radio_counters* counters,
+                                     void* stream) {
+  return guarded([&] {
+    if (config == nullptr) throw std::invalid_argument("null radio config");
+    c8mi::radio::project_fixed_noalloc(*config, device_tracks, stride, count, device_coreas,
+                                       device_zhs, counters, static_cast<hipStream_t>(stream));
   });
 }
 

@@ -114,6 +114,29 @@ int c8mi_radio_build_flat_table(const double* refractivity, uint64_t count,
                                 double step_m, double* integrated,
                                 c8mi_radio_propagation* out);
 
+/* One-shot projection that allocates no device memory. No counterpart in the
+ * reference: it is for callers that cannot allocate (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc). It computes
+ * what c8mi_radio_create + c8mi_radio_accumulate_tracks +
+ * c8mi_radio_download_fixed compute for `config` (deterministic mode only),
+ * but writes the int64 fixed-point waveforms into the caller's DEVICE buffers
+ * `device_coreas` / `device_zhs` ([3][total bins] each, cleared by the call;
+ * either may be NULL when its algorithm is disabled) and keeps its own device
+ * copies of the observers, the propagation table and the counters in a fixed
+ * library scratch array (one per GPU, held until the process ends). Limits:
+ * at most C8MI_RADIO_NOALLOC_MAX_OBSERVERS observers per algorithm and
+ * C8MI_RADIO_NOALLOC_MAX_TABLE_ENTRIES propagation-table entries; larger
+ * configurations return 3 (length_error). Work is ordered on `stream` (NULL:
+ * the default stream) and the call returns after it completed, filling
+ * `counters` (NULL allowed); 6 on fixed-point overflow. Calls that overlap on
+ * one GPU (other threads or streams) share the scratch array and race. */
+#define C8MI_RADIO_NOALLOC_MAX_OBSERVERS 1024
+#define C8MI_RADIO_NOALLOC_MAX_TABLE_ENTRIES 131072
+int c8mi_radio_project_fixed_noalloc(const c8mi_radio_config* config, const double* device_tracks,
+                                     uint64_t stride, uint64_t count, int64_t* device_coreas,
+                                     int64_t* device_zhs, c8mi_radio_counters* counters,
+                                     void* stream);
+
 /* Device memory and synchronisation helpers for hosts without their own HIP
  * allocator (tests, ctypes callers). */
 int c8mi_device_count(int* count);
