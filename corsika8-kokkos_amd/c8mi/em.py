@@ -180,7 +180,8 @@ class EmConfig(ctypes.Structure):
                 ("profile_weight_scale", c_double), ("profile_energy_scale", c_double),
                 ("axis_start_m", c_double * 3), ("axis_direction", c_double * 3),
                 ("axis_step_m", c_double), ("axis_grammage_g_per_cm2", POINTER(c_double)),
-                ("axis_support_count", c_uint64), ("shower_slots", c_uint32), ("reserved_", c_uint32)]
+                ("axis_support_count", c_uint64), ("shower_slots", c_uint32), ("reserved_", c_uint32),
+                ("minimum_batch_size", c_uint64), ("resident_minimum_batch", c_uint64)]
 
 
 class ShowerAxis(ctypes.Structure):
@@ -219,7 +220,42 @@ class RunResult(ctypes.Structure):
     _fields_ = [(k, c_uint64) for k in (
         "photon_fronts", "lepton_fronts", "photon_steps", "lepton_steps", "photon_interactions",
         "lepton_interactions", "secondaries", "observations", "fallbacks", "pending_photons",
-        "pending_leptons", "next_history_id", "max_front")]
+        "pending_leptons", "next_history_id", "max_front", "photon_calls", "lepton_calls",
+        "bucketed_fronts", "bucketed_particles", "below_minimum_batch_checkpoints",
+        "history_range_checkpoints", "reserved_history_ids")]
+
+    def as_dict(self):
+        return {k: int(getattr(self, k)) for k, _ in self._fields_}
+
+
+class WavefrontResult(ctypes.Structure):
+    _fields_ = [(k, c_uint64) for k in (
+        "input_particles", "wavefronts", "transport_records", "interactions", "secondaries",
+        "next_secondary_history_id", "peak_resident", "remaining", "other_species_pending",
+        "bucketed_fronts", "bucketed_particles", "small_fronts", "small_particles", "observations",
+        "fallbacks")] + [(k, c_uint32) for k in ("completed", "below_minimum_batch_checkpoint",
+                                                 "history_range_exhausted", "pad_")]
+
+    def as_dict(self):
+        return {k: int(getattr(self, k)) for k, _ in self._fields_ if k != "pad_"}
+
+
+class Limits(ctypes.Structure):
+    _fields_ = [(k, c_uint64) for k in (
+        "minimum_batch_size", "maximum_resident_photon_batch", "maximum_resident_lepton_batch",
+        "maximum_resident_input_batch", "queue_capacity")] + [("shower_slots", c_uint32), ("pad_", c_uint32)]
+
+    def as_dict(self):
+        return {k: int(getattr(self, k)) for k, _ in self._fields_ if k != "pad_"}
+
+
+class Statistics(ctypes.Structure):
+    _fields_ = [(k, c_uint64) for k in (
+        "shower_ordinal", "wavefronts", "photon_wavefronts", "lepton_wavefronts", "particles_enqueued",
+        "particles_advanced", "particles_produced", "interactions_selected", "bucketed_fronts",
+        "bucketed_particles", "small_fronts", "small_particles", "ordered_fronts", "photon_calls",
+        "lepton_calls", "below_minimum_batch_checkpoints", "history_range_checkpoints",
+        "reserved_history_ids", "remaining_downloaded", "host_to_device_bytes", "device_to_host_bytes")]
 
     def as_dict(self):
         return {k: int(getattr(self, k)) for k, _ in self._fields_}
@@ -450,7 +486,7 @@ def make_config(tables_module, seed=1, cut_mev=1.0, start=(0.0, 0.0, 112750.0),
                 direction=(0.0, 0.0, -1.0), atmosphere=None, bins=128, bin_width=10.0,
                 queue_capacity=1 << 22, observation_capacity=1 << 20, fallback_capacity=1 << 16,
                 thinning=None, crossing_mode=0, slots=1, moliere=False, moliere_table=None,
-                moliere_fast_iterations=0):
+                moliere_fast_iterations=0, resident_minimum_batch=0, minimum_batch_size=0):
     """EmConfig + the arrays it points to (kept alive on the returned object).
     moliere: apply Molière multiple scattering to e+- steps (the reference's
     production setting); off, e+- directions change only at vertices.
@@ -458,9 +494,13 @@ def make_config(tables_module, seed=1, cut_mev=1.0, start=(0.0, 0.0, 112750.0),
     interpolation table of the snapshot, as the reference backend uses) or a
     MoliereInterpolation. moliere_fast_iterations: Newton iterations per angle
     in the step kernel before a sample moves to its own wavefront (0: 12;
-    results do not depend on it)."""
+    results do not depend on it). resident_minimum_batch: below-minimum-batch
+    checkpoint of the device-resident schedule (0: none); minimum_batch_size:
+    the handoff batch a host router stages towards (0: 16384)."""
     cfg = EmConfig()
     cfg.shower_slots = slots
+    cfg.resident_minimum_batch = resident_minimum_batch
+    cfg.minimum_batch_size = minimum_batch_size
     cfg.moliere_fast_iterations = moliere_fast_iterations
     table_ref = None
     if moliere:
@@ -613,23 +653,50 @@ class EmBackend:
         p = np.ascontiguousarray(particles, dtype=PARTICLE_STATE_DTYPE).reshape(-1)
         _check(library().c8mi_em_enqueue(self.handle, p.ctypes.data_as(c_void_p), c_uint64(p.size)))
 
-    def run(self, maximum_fronts=1 << 40) -> dict:
+    def run(self, maximum_rounds=1 << 40) -> dict:
+        """Rounds of the reference's call schedule (c8mi_em_run)."""
         r = RunResult()
-        _check(library().c8mi_em_run(self.handle, c_uint64(maximum_fronts), byref(r)))
+        _check(library().c8mi_em_run(self.handle, c_uint64(maximum_rounds), byref(r)))
         return r.as_dict()
 
-    def run_photon_wavefront(self, particles, maximum_fronts=16) -> dict:
+    def run_photon_wavefront(self, particles, first_history_id, maximum_wavefronts=16,
+                             minimum_batch=1) -> dict:
+        """IEmBackend::runPhotonWavefront (c8mi_em_run_photon_wavefront)."""
         p = np.ascontiguousarray(particles, dtype=PARTICLE_STATE_DTYPE).reshape(-1)
-        r = RunResult()
-        _check(library().c8mi_em_run_photon_wavefront(self.handle, p.ctypes.data_as(c_void_p),
-                                                      c_uint64(p.size), c_uint64(maximum_fronts), byref(r)))
+        r = WavefrontResult()
+        _check(library().c8mi_em_run_photon_wavefront(
+            self.handle, p.ctypes.data_as(c_void_p), c_uint64(p.size), c_uint64(first_history_id),
+            c_uint64(maximum_wavefronts), c_uint64(minimum_batch), byref(r)))
         return r.as_dict()
 
-    def run_lepton_wavefront(self, particles, maximum_fronts=1024) -> dict:
+    def run_lepton_wavefront(self, particles, first_history_id, history_limit, maximum_wavefronts=1024,
+                             minimum_batch=1) -> dict:
+        """IEmBackend::runLeptonWavefront (c8mi_em_run_lepton_wavefront)."""
         p = np.ascontiguousarray(particles, dtype=PARTICLE_STATE_DTYPE).reshape(-1)
-        r = RunResult()
-        _check(library().c8mi_em_run_lepton_wavefront(self.handle, p.ctypes.data_as(c_void_p),
-                                                      c_uint64(p.size), c_uint64(maximum_fronts), byref(r)))
+        r = WavefrontResult()
+        _check(library().c8mi_em_run_lepton_wavefront(
+            self.handle, p.ctypes.data_as(c_void_p), c_uint64(p.size), c_uint64(first_history_id),
+            c_uint64(maximum_wavefronts), c_uint64(history_limit), c_uint64(minimum_batch), byref(r)))
+        return r.as_dict()
+
+    def remaining(self):
+        """remainingPhotons_ / remainingLeptons_ of the last wavefront call."""
+        n = c_uint64()
+        _check(library().c8mi_em_download_remaining(self.handle, None, c_uint64(0), byref(n)))
+        out = np.zeros(n.value, dtype=PARTICLE_STATE_DTYPE)
+        if n.value:
+            _check(library().c8mi_em_download_remaining(self.handle, out.ctypes.data_as(c_void_p),
+                                                        c_uint64(out.size), byref(n)))
+        return out
+
+    def limits(self) -> dict:
+        r = Limits()
+        _check(library().c8mi_em_get_limits(self.handle, byref(r)))
+        return r.as_dict()
+
+    def statistics(self) -> dict:
+        r = Statistics()
+        _check(library().c8mi_em_get_statistics(self.handle, byref(r)))
         return r.as_dict()
 
     def pending(self):

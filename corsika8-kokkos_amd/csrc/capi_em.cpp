@@ -118,35 +118,67 @@ int c8mi_em_enqueue(c8mi_em* em, c8mi_particle_state const* particles, uint64_t 
 }
 
 int c8mi_em_run_photon_wavefront(c8mi_em* em, c8mi_particle_state const* particles, uint64_t count,
-                                 uint64_t maximum_fronts, c8mi_em_run_result* out) {
+                                 uint64_t first_secondary_history_id, uint64_t maximum_wavefronts,
+                                 uint64_t minimum_resident_batch, c8mi_em_wavefront_result* out) {
   return guarded([&] {
     require_handle(em);
     if (!out) throw std::invalid_argument("null result");
+    if (count && !particles) throw std::invalid_argument("null particle array");
     for (uint64_t i = 0; i < count; ++i)
       if (particles[i].pid != 22) throw std::invalid_argument("runPhotonWavefront takes photons only");
-    em->backend.enqueue(particles, count);
-    em->backend.run(1, maximum_fronts, *out);
+    em->backend.run_call(true, particles, count, first_secondary_history_id, maximum_wavefronts, 0,
+                         minimum_resident_batch, *out);
   });
 }
 
 int c8mi_em_run_lepton_wavefront(c8mi_em* em, c8mi_particle_state const* particles, uint64_t count,
-                                 uint64_t maximum_fronts, c8mi_em_run_result* out) {
+                                 uint64_t first_secondary_history_id, uint64_t maximum_wavefronts,
+                                 uint64_t secondary_history_id_limit_exclusive,
+                                 uint64_t minimum_resident_batch, c8mi_em_wavefront_result* out) {
   return guarded([&] {
     require_handle(em);
     if (!out) throw std::invalid_argument("null result");
+    if (count && !particles) throw std::invalid_argument("null particle array");
     for (uint64_t i = 0; i < count; ++i)
       if (particles[i].pid != 11 && particles[i].pid != -11)
         throw std::invalid_argument("runLeptonWavefront takes e+- only");
-    em->backend.enqueue(particles, count);
-    em->backend.run(2, maximum_fronts, *out);
+    em->backend.run_call(false, particles, count, first_secondary_history_id, maximum_wavefronts,
+                         secondary_history_id_limit_exclusive, minimum_resident_batch, *out);
   });
 }
 
-int c8mi_em_run(c8mi_em* em, uint64_t maximum_fronts, c8mi_em_run_result* out) {
+int c8mi_em_download_remaining(c8mi_em* em, c8mi_particle_state* out, uint64_t capacity, uint64_t* count) {
+  return guarded([&] {
+    require_handle(em);
+    auto const& r = em->backend.remaining();
+    uint64_t const n = r.size() < capacity ? r.size() : capacity;
+    if (n && !out) throw std::invalid_argument("null output array");
+    for (uint64_t i = 0; i < n; ++i) out[i] = r[i];
+    if (count) *count = r.size();
+  });
+}
+
+int c8mi_em_get_limits(c8mi_em const* em, c8mi_em_limits* out) {
+  return guarded([&] {
+    require_handle(em);
+    if (!out) throw std::invalid_argument("null output");
+    em->backend.limits(*out);
+  });
+}
+
+int c8mi_em_get_statistics(c8mi_em const* em, c8mi_em_statistics* out) {
+  return guarded([&] {
+    require_handle(em);
+    if (!out) throw std::invalid_argument("null output");
+    em->backend.statistics(*out);
+  });
+}
+
+int c8mi_em_run(c8mi_em* em, uint64_t maximum_rounds, c8mi_em_run_result* out) {
   return guarded([&] {
     require_handle(em);
     if (!out) throw std::invalid_argument("null result");
-    em->backend.run(0, maximum_fronts, *out);
+    em->backend.run(maximum_rounds, *out);
   });
 }
 
@@ -205,7 +237,7 @@ int c8mi_em_run_rounds(c8mi_em* em, uint64_t maximum_rounds, c8mi_em_run_result*
   return guarded([&] {
     require_handle(em);
     if (!out) throw std::invalid_argument("null result");
-    em->backend.run(0, maximum_rounds, *out);
+    em->backend.run(maximum_rounds, *out);
   });
 }
 

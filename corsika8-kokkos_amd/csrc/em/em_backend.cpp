@@ -3,6 +3,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <limits>
 #include <thread>
 #include <cstring>
 #include <stdexcept>
@@ -162,8 +163,10 @@ EmBackend::EmBackend(c8mi_em_config const& config, c8mi_native_table_set const& 
   stats_.allocate(slots_ * static_cast<std::uint64_t>(kSlotStats));
   shower_totals_.allocate(static_cast<std::size_t>(kMaxShowerSlots) * kMaxRuns * kShowerFields);
   adjust_.allocate(slots_ * static_cast<std::uint64_t>(kMaxRuns) * 3u);
-  report_.allocate(kReportHeader + 2u * slots_);
-  host_report_ = PinnedBuffer<unsigned long long>(kReportHeader + 2u * slots_);
+  report_.allocate(kReportHeader + kReportPerSlot * static_cast<std::size_t>(slots_));
+  host_report_ = PinnedBuffer<unsigned long long>(kReportHeader + kReportPerSlot * static_cast<std::size_t>(slots_));
+  control_dev_.allocate(slots_);
+  control_host_ = PinnedBuffer<SlotControl>(slots_);
   for (auto* b : {&history_, &stats_, &shower_totals_, &record_counts_})
     C8MI_HIP(hipMemset(b->data(), 0, b->bytes()));
   C8MI_HIP(hipMemset(counters_.data(), 0, counters_.bytes()));
@@ -254,7 +257,20 @@ EmBackend::EmBackend(c8mi_em_config const& config, c8mi_native_table_set const& 
   for (auto& e : events_) C8MI_HIP(hipEventCreate(&e));
   C8MI_HIP(hipEventCreateWithFlags(&init_copied_, hipEventDisableTiming));
   C8MI_HIP(hipEventCreateWithFlags(&unstage_copied_, hipEventDisableTiming));
+  C8MI_HIP(hipEventCreateWithFlags(&control_copied_, hipEventDisableTiming));
+  // media a particle can carry: the environment's, and any a host particle brings
+  for (std::uint32_t i = 0; i < environment.model_count; ++i) media_.push_back(environment.models[i].medium_id);
+  std::sort(media_.begin(), media_.end());
+  media_.erase(std::unique(media_.begin(), media_.end()), media_.end());
+  order_.set_media(media_.data(), static_cast<std::uint32_t>(media_.size()), stream_);
   reset_pool();
+}
+
+void EmBackend::add_medium(std::int32_t medium) {
+  auto const at = std::lower_bound(media_.begin(), media_.end(), medium);
+  if (at != media_.end() && *at == medium) return;
+  media_.insert(at, medium);
+  order_.set_media(media_.data(), static_cast<std::uint32_t>(media_.size()), stream_);
 }
 
 PoolArrays EmBackend::pool_arrays() const {
@@ -280,12 +296,15 @@ void EmBackend::start_slots(SlotInit const* inits, std::uint32_t count) {
   for (std::uint32_t k = 0; k < count; ++k) host_showers_[inits[k].slot] = inits[k].shower;
 }
 
-void EmBackend::activate(std::uint32_t slot, std::uint64_t shower_id) {
+void EmBackend::activate(std::uint32_t slot, std::uint64_t shower_id, std::uint64_t first_history_id) {
   SlotHost& h = slot_host_[slot];
   h.active = true;
   h.shower_id = shower_id;
   h.fronts[0] = h.fronts[1] = 0;
   h.injected = 0;
+  h.call = CallState{};
+  h.call.counter = first_history_id;
+  ++totals_.shower_ordinal;
 }
 
 EmBackend::~EmBackend() {
@@ -306,6 +325,7 @@ EmBackend::~EmBackend() {
   if (side_stream_) (void)hipStreamDestroy(side_stream_);
   if (init_copied_) (void)hipEventDestroy(init_copied_);
   if (unstage_copied_) (void)hipEventDestroy(unstage_copied_);
+  if (control_copied_) (void)hipEventDestroy(control_copied_);
   if (own_stream_ && stream_) (void)hipStreamDestroy(stream_);
 }
 
@@ -335,9 +355,10 @@ void EmBackend::reset_pool() {
   for (int s = 0; s < 2; ++s) {
     count_[s] = 0;
     current_[s] = 0;
-    runs_[s].clear();
+    clean_[s] = true;
     staged_[s].clear();
   }
+  remaining_.clear();
   update_active_range();
 }
 
@@ -375,7 +396,7 @@ void EmBackend::slot_begin(std::uint32_t slot, std::uint64_t shower_id, std::uin
   init.slot = slot;
   start_slots(&init, 1);
   C8MI_HIP(hipStreamSynchronize(stream_));
-  activate(slot, shower_id);
+  activate(slot, shower_id, first_history_id);
   update_active_range();
 }
 
@@ -408,6 +429,8 @@ void EmBackend::slot_enqueue(std::uint32_t slot, c8mi_particle_state const* part
       throw std::invalid_argument("em: particle " + std::to_string(i) + " (pid " +
                                   std::to_string(particles[i].pid) +
                                   ") cannot be transported by the device path");
+  for (std::uint64_t i = 0; i < count; ++i) add_medium(particles[i].medium_id);
+  totals_.particles_enqueued += count;
   for (std::uint64_t i = 0; i < count; ++i) {
     c8mi_particle_state const& p = particles[i];
     int const s = p.pid == 22 ? 0 : 1;
@@ -426,25 +449,13 @@ std::uint32_t EmBackend::active_slot_count() const {
   return active;
 }
 
-void EmBackend::append_run(int species, std::uint64_t end) {
-  auto& r = runs_[species];
-  if (!r.empty() && r.back() == end) return;
-  if (active_slot_count() <= 1) {  // one shower: its queue is a single run in any order of appends
-    r.assign(1, end);
-    return;
-  }
-  if (r.size() >= static_cast<std::size_t>(kMaxRuns))
-    throw std::logic_error("em: too many queued runs for a pool of showers; alternate photon and lepton fronts");
-  r.push_back(end);
-}
-
-// Uploads the host-staged particles of one species as one run, sorted by slot
-// (stable, so each slot keeps its enqueue order): one copy from pinned memory
-// and one kernel that writes the queue's fields.
+// Uploads the host-staged particles of one species after the queue's particles
+// (class host; the ordering step of the slot's next call places them after its
+// pending and remaining particles, each in enqueue order): one copy from pinned
+// memory and one kernel that writes the queue's fields.
 void EmBackend::flush_staged(int species) {
   auto& v = staged_[species];
   if (v.empty()) return;
-  std::stable_sort(v.begin(), v.end(), [](Staged const& a, Staged const& b) { return a.slot < b.slot; });
   std::size_t const n = v.size();
   if (count_[species] + n > config_.queue_capacity)
     throw std::length_error("em: enqueue exceeds the device queue capacity");
@@ -465,7 +476,8 @@ void EmBackend::flush_staged(int species) {
     ++slot_host_[e.slot].pending[species];
   }
   count_[species] += n;
-  append_run(species, count_[species]);
+  clean_[species] = false;
+  totals_.host_to_device_bytes += n * sizeof(StagedParticle);
   v.clear();
 }
 
@@ -558,31 +570,185 @@ void EmBackend::slot_radio_time(double& ms, std::uint64_t& launches) {
   if (slot_radio_) slot_radio_->kernel_time_ms(ms, launches);
 }
 
-void EmBackend::front(bool photon, c8mi_em_run_result& out) {
+namespace {
+
+constexpr std::uint64_t kPhotonCallFronts = 16;   // EmRouter.inl:368
+constexpr std::uint64_t kLeptonCallFronts = 1024; // EmRouter.inl:405
+
+std::uint32_t bits_for(std::uint64_t values) {
+  std::uint32_t b = 0;
+  while (b < 64 && (std::uint64_t{1} << b) < values) ++b;
+  return b;
+}
+
+}  // namespace
+
+// History reservation of a device-resident call of n particles
+// (reservePhotonSecondaryRange, EmRouter.inl:640-656: 2 n per front of the
+// call; reserveResidentSecondaryRange, EmRouter.inl:658-679:
+// max(3 n, min(2^20, 64 * 3 n)) with its exclusive limit).
+void EmBackend::reserve_call(CallState& c, int species, std::uint64_t n, c8mi_em_run_result& out) const {
+  std::uint64_t count;
+  if (species == 0) {
+    if (n > std::numeric_limits<std::uint64_t>::max() / (2 * kPhotonCallFronts))
+      throw std::overflow_error("em: photon history reservation overflow");
+    count = n * 2 * kPhotonCallFronts;
+  } else {
+    if (n > std::numeric_limits<std::uint64_t>::max() / 3)
+      throw std::overflow_error("em: lepton history reservation overflow");
+    std::uint64_t const minimum = 3 * n;
+    count = minimum;
+    if (minimum <= std::numeric_limits<std::uint64_t>::max() / 64)
+      count = std::max(std::min<std::uint64_t>(std::uint64_t{1} << 20, minimum * 64), minimum);
+  }
+  if (count > std::numeric_limits<std::uint64_t>::max() - c.counter)
+    throw std::overflow_error("em: history range limit overflow");
+  c.next = c.counter;
+  c.limit = c.counter + count;
+  c.counter += count;
+  c.open = c.opening = true;
+  c.call_fronts = 0;
+  c.cap = species == 0 ? kPhotonCallFronts : kLeptonCallFronts;
+  c.min_batch = config_.resident_minimum_batch;
+  out.reserved_history_ids += count;
+  ++(species == 0 ? out.photon_calls : out.lepton_calls);
+}
+
+// The loop head of run_photon_cascade / run_lepton_cascade for one shower
+// (PhotonCascade.inl:286-300, LeptonCascade.inl:240-252): a call runs while its
+// queue holds particles and fewer than `cap` fronts ran; after its first front
+// it ends below the minimum batch; a lepton call ends when the front could
+// exhaust its history range. An ended call passes to the other species.
+bool EmBackend::schedule(CallState& c, int species, std::uint64_t n, c8mi_em_run_result& out) const {
+  if (c.phase != species) return false;
+  if (!c.open) {
+    if (n == 0) {
+      c.phase ^= 1;
+      return false;
+    }
+    reserve_call(c, species, n, out);
+  }
+  int stop = -1;
+  if (n == 0) stop = kStopEmpty;
+  else if (c.call_fronts >= c.cap) stop = kStopCap;
+  else if (c.call_fronts != 0 && n < c.min_batch) stop = kStopMinimumBatch;
+  else if (species == 1 && n > (c.limit - c.next) / 3) stop = kStopHistory;
+  if (stop >= 0) {
+    if (stop == kStopMinimumBatch) ++out.below_minimum_batch_checkpoints;
+    if (stop == kStopHistory) ++out.history_range_checkpoints;
+    c.open = c.opening = false;
+    c.stop = stop;
+    c.phase ^= 1;
+    return false;
+  }
+  ++c.call_fronts;
+  return true;
+}
+
+bool EmBackend::front(bool photon, c8mi_em_run_result& out) {
   int const s = photon ? 0 : 1, o = 1 - s;
-  // staged particles of both species join their queues now, ahead of the
-  // secondaries this front appends (the order of a serial run)
-  flush_staged(0);
-  flush_staged(1);
+  // host particles join the queue now; the ordering step of their shower's
+  // call places them after its pending and remaining particles
+  flush_staged(s);
   std::uint64_t const n = count_[s];
-  if (n == 0) return;
+  // Decide every active shower's part in this front on copies of their call
+  // states; they are committed once nothing can refuse the front.
+  plan_.assign(active_range_, CallState{});
+  front_counts_.assign(active_range_, 0);
+  c8mi_em_run_result decided{};
+  std::uint64_t m = 0;
+  bool any_open = false, any_bucket = false;
+  C8MI_HIP(hipEventSynchronize(control_copied_));  // the previous front's control upload has left
+  for (std::uint32_t k = 0; k < active_range_; ++k) {
+    SlotControl& c = control_host_.data()[k];
+    c = SlotControl{};
+    SlotHost const& h = slot_host_[k];
+    plan_[k] = h.call;
+    if (!h.active) continue;
+    std::uint64_t const cnt = h.pending[s];
+    if (!schedule(plan_[k], s, cnt, decided)) continue;
+    c.flags = kControlRun;
+    if (plan_[k].opening) {
+      c.flags |= kControlOpen;
+      c.first_history = plan_[k].next;
+      any_open = true;
+    }
+    if (cnt >= kBucketMinimum) {
+      c.flags |= kControlBucket;
+      any_bucket = true;
+    }
+    front_counts_[k] = cnt;
+    m += cnt;
+  }
+  auto commit = [&] {
+    for (std::uint32_t k = 0; k < active_range_; ++k) {
+      if (!slot_host_[k].active) continue;
+      slot_host_[k].call = plan_[k];
+      if (control_host_.data()[k].flags & kControlRun) slot_host_[k].call.opening = false;
+    }
+    out.photon_calls += decided.photon_calls;
+    out.lepton_calls += decided.lepton_calls;
+    out.reserved_history_ids += decided.reserved_history_ids;
+    out.below_minimum_batch_checkpoints += decided.below_minimum_batch_checkpoints;
+    out.history_range_checkpoints += decided.history_range_checkpoints;
+    totals_.photon_calls += decided.photon_calls;
+    totals_.lepton_calls += decided.lepton_calls;
+    totals_.reserved_history_ids += decided.reserved_history_ids;
+    totals_.below_minimum_batch_checkpoints += decided.below_minimum_batch_checkpoints;
+    totals_.history_range_checkpoints += decided.history_range_checkpoints;
+  };
+  if (m == 0) {
+    commit();
+    return false;
+  }
+  bool const parked = m < n;
+  bool const ordered = any_open || any_bucket || parked || !clean_[s];
   std::uint64_t const cap = config_.queue_capacity;
-  if (2 * n > cap || count_[o] + 2 * n > cap)
-    throw std::length_error("em: a front's outputs could exceed the device queue capacity");
   // Everything that can refuse this front is checked before the first launch:
   // once the kernels run, the device ledger, profile and queues move on.
-  if (runs_[o].size() >= static_cast<std::size_t>(kMaxRuns) && active_slot_count() > 1)
-    throw std::logic_error("em: too many queued runs for a pool of showers; alternate photon and lepton fronts");
+  if (n + m > cap || count_[o] + 2 * m > cap)
+    throw std::length_error("em: a front's outputs could exceed the device queue capacity");
   if (!photon) {
-    if (radio_) radio_->check_accumulate(n);
-    if (slot_radio_) slot_radio_->check_project(n, slot_radio_->chunk_for(n), active_range_ * kMaxRuns);
+    if (radio_) radio_->check_accumulate(m);
+    if (slot_radio_) slot_radio_->check_project(m, slot_radio_->chunk_for(m), active_range_ * kMaxRuns);
   }
-  resize_scratch(n);
+  resize_scratch(m);
+  commit();
+  ParticleQueue const cur = queues_[s][current_[s]]->view();
+  ParticleQueue const alt = queues_[s][1 - current_[s]]->view();
+  if (ordered) {
+    C8MI_HIP(hipMemcpyAsync(control_dev_.data(), control_host_.data(), active_range_ * sizeof(SlotControl),
+                            hipMemcpyHostToDevice, stream_));
+    C8MI_HIP(hipEventRecord(control_copied_, stream_));
+    totals_.host_to_device_bytes += active_range_ * sizeof(SlotControl);
+    OrderLayout layout{};
+    layout.class_bits = any_open ? 2u : 0u;
+    layout.medium_bits = order_.medium_bits();
+    layout.bucket_bits = any_bucket ? 18u + layout.medium_bits : 0u;
+    layout.slot_bits = bits_for(active_range_);
+    layout.parked_bits = parked ? 1u : 0u;
+    order_.order(cur, alt, n, control_dev_.data(), active_range_, history_.data(), layout, stream_);
+    ++totals_.ordered_fronts;
+  }
+  // ordered: the front reads the gathered buffer and writes its continuing
+  // particles back into the current one; else it reads the current buffer
+  ParticleQueue const front_in = ordered ? alt : cur;
+  ParticleQueue const next = ordered ? cur : alt;
+  ParticleQueue const other = queues_[o][current_[o]]->view();
+  for (std::uint32_t k = 0; k < active_range_; ++k) {
+    if (!(control_host_.data()[k].flags & kControlRun)) continue;
+    ++slot_host_[k].fronts[s];
+    bool const bucketed = front_counts_[k] >= kBucketMinimum;
+    out.bucketed_fronts += bucketed ? 1u : 0u;
+    out.bucketed_particles += bucketed ? front_counts_[k] : 0u;
+    (bucketed ? totals_.bucketed_fronts : totals_.small_fronts) += 1;
+    (bucketed ? totals_.bucketed_particles : totals_.small_particles) += front_counts_[k];
+  }
   FrontArgs f{};
-  f.in = queues_[s][current_[s]]->view();
-  f.count = n;
-  f.runs = static_cast<std::uint32_t>(runs_[s].size());
-  for (std::uint32_t r = 0; r < f.runs; ++r) f.run_end[r] = runs_[s][r];
+  f.in = front_in;
+  f.count = m;
+  f.runs = 1;
+  f.run_end[0] = m;
   f.slots = active_range_;
   f.staging = staging_.data();
   f.packed = packed_.data();
@@ -593,10 +759,6 @@ void EmBackend::front(bool photon, c8mi_em_run_result& out) {
   f.history = history_.data();
   f.stats = stats_.data();
   f.report = report_.data();
-  ParticleQueue const next = queues_[s][1 - current_[s]]->view();
-  ParticleQueue const other = queues_[o][current_[o]]->view();
-  for (std::uint32_t k = 0; k < active_range_; ++k)
-    if (slot_host_[k].pending[s]) ++slot_host_[k].fronts[s];
   f.vertices = vertices_.data();
   f.vertex_count = vertex_count_.data();
   f.moves = moves_.data();
@@ -609,22 +771,22 @@ void EmBackend::front(bool photon, c8mi_em_run_result& out) {
   if ((radio_ || slot_radio_) && !photon) {
     // tracks [9][n]; the buffer's previous projections must be done first
     std::uint64_t& capacity = track_capacity_[track_buffer];
-    if (capacity < n) {
+    if (capacity < m) {
       if (track_busy_[track_buffer]) C8MI_HIP(hipEventSynchronize(track_done_[track_buffer]));
       if (slot_busy_[track_buffer]) C8MI_HIP(hipEventSynchronize(slot_done_[track_buffer]));
       track_busy_[track_buffer] = slot_busy_[track_buffer] = false;
-      capacity = std::max<std::uint64_t>(n, 2 * capacity);
+      capacity = std::max<std::uint64_t>(m, 2 * capacity);
       tracks_[track_buffer].allocate(C8MI_RADIO_TRACK_FIELDS * capacity);
     }
     if (track_busy_[track_buffer]) C8MI_HIP(hipStreamWaitEvent(stream_, track_done_[track_buffer], 0));
     if (slot_busy_[track_buffer]) C8MI_HIP(hipStreamWaitEvent(stream_, slot_done_[track_buffer], 0));
     f.tracks = tracks_[track_buffer].data();
-    f.track_stride = n;
+    f.track_stride = m;
   }
   // Projects this front's tracks on the accumulators' streams once the step kernel is done.
   auto project_tracks = [&] {
     if (!f.tracks) return;
-    std::uint64_t const chunk = slot_radio_ ? slot_radio_->chunk_for(n) : 0;
+    std::uint64_t const chunk = slot_radio_ ? slot_radio_->chunk_for(m) : 0;
     if (slot_radio_) {
       std::size_t const keys = static_cast<std::size_t>(slots_) * kMaxRuns;
       unsigned long long* key_begin = segment_keys_[track_buffer].data();
@@ -635,13 +797,13 @@ void EmBackend::front(bool photon, c8mi_em_run_result& out) {
     C8MI_HIP(hipEventRecord(track_ready_[track_buffer], stream_));
     if (radio_) {
       C8MI_HIP(hipStreamWaitEvent(radio_->stream(), track_ready_[track_buffer], 0));
-      radio_->accumulate_tracks(f.tracks, n, n);
+      radio_->accumulate_tracks(f.tracks, m, m);
       C8MI_HIP(hipEventRecord(track_done_[track_buffer], radio_->stream()));
       track_busy_[track_buffer] = true;
     }
     if (slot_radio_) {
       C8MI_HIP(hipStreamWaitEvent(slot_radio_->stream(), track_ready_[track_buffer], 0));
-      slot_radio_->project(f.tracks, n, n, chunk, segments_[track_buffer].data(),
+      slot_radio_->project(f.tracks, m, m, chunk, segments_[track_buffer].data(),
                            segment_count_[track_buffer].data(), item_count_[track_buffer].data(),
                            active_range_ * kMaxRuns);
       C8MI_HIP(hipEventRecord(slot_done_[track_buffer], slot_radio_->stream()));
@@ -694,12 +856,16 @@ void EmBackend::front(bool photon, c8mi_em_run_result& out) {
   mark(kKernScan);
   launch_scatter(f, next, other, count_[o], stream_);
   mark(kKernScatter);
+  // showers that sat this front out follow the continuing particles unchanged
+  if (parked) launch_carry(front_in, m, n, next, report_.data(), stream_);
   C8MI_HIP(hipMemcpyAsync(host_report_.data(), report_.data(),
-                          (kReportHeader + 2u * active_range_) * sizeof(unsigned long long),
+                          (kReportHeader + kReportPerSlot * static_cast<std::size_t>(active_range_)) *
+                              sizeof(unsigned long long),
                           hipMemcpyDeviceToHost, stream_));
   C8MI_HIP(hipStreamSynchronize(stream_));
   FrontTotals t;
   std::memcpy(&t, host_report_.data(), sizeof t);
+  totals_.device_to_host_bytes += (kReportHeader + kReportPerSlot * active_range_) * sizeof(unsigned long long);
   for (int k = 0; k < launched; ++k) {
     float ms = 0.f;
     C8MI_HIP(hipEventElapsedTime(&ms, events_[k], events_[k + 1]));
@@ -708,31 +874,40 @@ void EmBackend::front(bool photon, c8mi_em_run_result& out) {
   }
   unsigned long long const* per_slot = host_report_.data() + kReportHeader;
   for (std::uint32_t k = 0; k < active_range_; ++k) {
-    slot_host_[k].pending[s] = per_slot[2 * k];
-    slot_host_[k].pending[o] += per_slot[2 * k + 1];
+    if (!(control_host_.data()[k].flags & kControlRun)) continue;
+    SlotHost& h = slot_host_[k];
+    h.pending[s] = per_slot[kReportPerSlot * k];
+    h.pending[o] += per_slot[kReportPerSlot * k + 1];
+    h.call.next += per_slot[kReportPerSlot * k + 2];
   }
-  count_[s] = t.next;
-  current_[s] ^= 1;
-  runs_[s].clear();
-  if (t.next) runs_[s].push_back(t.next);
+  count_[s] = t.next + (n - m);
+  if (!ordered) current_[s] ^= 1;
+  clean_[s] = !parked;
   count_[o] += t.other;
-  if (t.other) append_run(o, count_[o]);
+  if (t.other) clean_[o] = false;
   if (photon) {
     out.photon_fronts += 1;
     out.photon_steps += t.steps;
     out.photon_interactions += t.interactions;
+    totals_.photon_wavefronts += 1;
   } else {
     out.lepton_fronts += 1;
     out.lepton_steps += t.steps;
     out.lepton_interactions += t.interactions;
+    totals_.lepton_wavefronts += 1;
   }
   out.secondaries += t.children;
-  if (n > out.max_front) out.max_front = n;
+  if (m > out.max_front) out.max_front = m;
+  totals_.wavefronts += 1;
+  totals_.particles_advanced += t.steps;
+  totals_.particles_produced += t.children;
+  totals_.interactions_selected += t.interactions;
+  return true;
 }
 
-void EmBackend::run(int species, std::uint64_t maximum_fronts, c8mi_em_run_result& out) {
+void EmBackend::run(std::uint64_t maximum_rounds, c8mi_em_run_result& out) {
   std::memset(&out, 0, sizeof out);
-  run_fronts(species, maximum_fronts, out);
+  run_rounds_loop(maximum_rounds, out);
   std::vector<unsigned long long> records(2u * slots_);
   C8MI_HIP(hipMemcpy(records.data(), record_counts_.data(), records.size() * sizeof(unsigned long long),
                      hipMemcpyDeviceToHost));
@@ -741,26 +916,119 @@ void EmBackend::run(int species, std::uint64_t maximum_fronts, c8mi_em_run_resul
     out.fallbacks += records[2 * k + 1];
   }
   pending(out.pending_photons, out.pending_leptons);
-  unsigned long long next_history = 0;
-  C8MI_HIP(hipMemcpy(&next_history, history_.data(), sizeof next_history, hipMemcpyDeviceToHost));
-  out.next_history_id = next_history;
+  out.next_history_id = slot_host_[0].call.counter;
 }
 
-// The front loop alone: no record or history read-back.
-void EmBackend::run_fronts(int species, std::uint64_t maximum_fronts, c8mi_em_run_result& out) {
+// Rounds of the schedule: every shower's photon front (if its call runs one
+// now), then its lepton front. No record or history read-back.
+void EmBackend::run_rounds_loop(std::uint64_t maximum_rounds, c8mi_em_run_result& out) {
   auto has = [&](int s) { return count_[s] + staged_[s].size() > 0; };
-  for (std::uint64_t round = 0; round < maximum_fronts; ++round) {
-    if (species == 0) {
-      if (!has(0) && !has(1)) break;
-      if (has(0)) front(true, out);
-      if (has(1)) front(false, out);
-    } else {
-      int const s = species == 1 ? 0 : 1;
-      if (!has(s)) break;
-      front(s == 0, out);
-    }
+  for (std::uint64_t round = 0; round < maximum_rounds; ++round) {
+    if (!has(0) && !has(1)) break;
+    front(true, out);
+    front(false, out);
   }
 }
+
+void EmBackend::run_call(bool photon, c8mi_particle_state const* particles, std::uint64_t count,
+                         std::uint64_t first_secondary_history_id, std::uint64_t maximum_wavefronts,
+                         std::uint64_t history_limit_exclusive, std::uint64_t minimum_resident_batch,
+                         c8mi_em_wavefront_result& out) {
+  std::memset(&out, 0, sizeof out);
+  int const s = photon ? 0 : 1, o = 1 - s;
+  SlotHost& h = slot_host_[0];
+  if (!h.active || active_slot_count() != 1)
+    throw std::logic_error("em: the wavefront calls act on the one shower begun with begin_shower");
+  if (count && !particles) throw std::invalid_argument("em: null particle array");
+  remaining_.clear();
+  std::uint64_t const total = h.pending[s] + h.staged[s] + count;
+  out.input_particles = total;
+  out.other_species_pending = h.pending[o] + h.staged[o];
+  if (total == 0) {  // PhotonCascade.inl:206-209
+    out.completed = 1;
+    return;
+  }
+  // PhotonCascade.inl:211-214, LeptonCascade.inl:127-132
+  if (first_secondary_history_id == 0 || maximum_wavefronts == 0 || minimum_resident_batch == 0)
+    throw std::invalid_argument("em: a wavefront call requires nonzero history ID, wavefronts and batch");
+  if (!photon && history_limit_exclusive <= first_secondary_history_id)
+    throw std::invalid_argument("em: the lepton history limit must exceed the first history ID");
+  if (2 * total > config_.queue_capacity)
+    throw std::length_error("em: wavefront input exceeds the advertised resident batch limit");
+  slot_enqueue(0, particles, count);
+  unsigned long long records_before[2] = {0, 0};
+  C8MI_HIP(hipStreamSynchronize(stream_));
+  C8MI_HIP(hipMemcpy(records_before, record_counts_.data(), sizeof records_before, hipMemcpyDeviceToHost));
+  c8mi_em_statistics const before = totals_;
+  CallState& c = h.call;
+  CallState const saved = c;
+  c.phase = s;
+  c.open = c.opening = true;
+  c.stop = kStopEmpty;
+  c.call_fronts = 0;
+  c.next = first_secondary_history_id;
+  c.limit = photon ? std::numeric_limits<std::uint64_t>::max() : history_limit_exclusive;
+  c.cap = maximum_wavefronts;
+  c.min_batch = minimum_resident_batch;
+  c8mi_em_run_result acc{};
+  try {
+    while (c.open && c.phase == s) front(photon, acc);
+  } catch (...) {
+    if (c.open && c.call_fronts == 0) c = saved;  // refused before its first front
+    throw;
+  }
+  out.wavefronts = c.call_fronts;
+  out.transport_records = photon ? acc.photon_steps : acc.lepton_steps;
+  out.interactions = photon ? acc.photon_interactions : acc.lepton_interactions;
+  out.secondaries = acc.secondaries;
+  out.next_secondary_history_id = c.next;
+  out.peak_resident = acc.max_front;
+  out.bucketed_fronts = totals_.bucketed_fronts - before.bucketed_fronts;
+  out.bucketed_particles = totals_.bucketed_particles - before.bucketed_particles;
+  out.small_fronts = totals_.small_fronts - before.small_fronts;
+  out.small_particles = totals_.small_particles - before.small_particles;
+  out.below_minimum_batch_checkpoint = c.stop == kStopMinimumBatch ? 1u : 0u;
+  out.history_range_exhausted = c.stop == kStopHistory ? 1u : 0u;
+  // the species' particles left on the device are the call's remaining ones
+  std::uint64_t const left = count_[s];
+  if (left) {
+    resize_scratch(left);
+    launch_pack(queues_[s][current_[s]]->view(), left, staging_.data(), stream_);
+    remaining_.resize(left);
+    C8MI_HIP(hipMemcpyAsync(remaining_.data(), staging_.data(), left * sizeof(c8mi_particle_state),
+                            hipMemcpyDeviceToHost, stream_));
+    totals_.device_to_host_bytes += left * sizeof(c8mi_particle_state);
+    totals_.remaining_downloaded += left;
+  }
+  unsigned long long records_after[2] = {0, 0};
+  C8MI_HIP(hipMemcpyAsync(records_after, record_counts_.data(), sizeof records_after, hipMemcpyDeviceToHost,
+                          stream_));
+  C8MI_HIP(hipStreamSynchronize(stream_));
+  count_[s] = 0;
+  h.pending[s] = 0;
+  clean_[s] = true;
+  // the remaining particles leave the device ledger: their energy no longer
+  // counts as injected (it is injected again when the router sends them back)
+  for (auto const& p : remaining_)
+    h.injected -= static_cast<long double>(std::llrint(p.energy_gev * p.weight * config_.profile_energy_scale));
+  out.remaining = left;
+  out.completed = left == 0 ? 1u : 0u;
+  out.other_species_pending = h.pending[o] + h.staged[o];
+  out.observations = records_after[0] - records_before[0];
+  out.fallbacks = records_after[1] - records_before[1];
+}
+
+void EmBackend::limits(c8mi_em_limits& out) const {
+  std::memset(&out, 0, sizeof out);
+  out.minimum_batch_size = config_.minimum_batch_size ? config_.minimum_batch_size : 16384;
+  out.maximum_resident_photon_batch = config_.queue_capacity / 2;
+  out.maximum_resident_lepton_batch = config_.queue_capacity / 2;
+  out.maximum_resident_input_batch = config_.queue_capacity / 2;
+  out.queue_capacity = config_.queue_capacity;
+  out.shower_slots = slots_;
+}
+
+void EmBackend::statistics(c8mi_em_statistics& out) const { out = totals_; }
 
 void EmBackend::pending(std::uint64_t& photons, std::uint64_t& leptons) const {
   photons = count_[0] + staged_[0].size();
@@ -771,16 +1039,15 @@ void EmBackend::slot_state(std::uint32_t slot, c8mi_em_slot_state& out) const {
   check_slot(slot);
   std::memset(&out, 0, sizeof out);
   SlotHost const& h = slot_host_[slot];
-  unsigned long long st[kSlotStats], records[2], history = 0;
+  unsigned long long st[kSlotStats], records[2];
   C8MI_HIP(hipStreamSynchronize(stream_));
   C8MI_HIP(hipMemcpy(st, stats_.data() + static_cast<std::uint64_t>(slot) * kSlotStats, sizeof st,
                      hipMemcpyDeviceToHost));
   C8MI_HIP(hipMemcpy(records, record_counts_.data() + 2u * slot, sizeof records, hipMemcpyDeviceToHost));
-  C8MI_HIP(hipMemcpy(&history, history_.data() + slot, sizeof history, hipMemcpyDeviceToHost));
   out.shower_id = h.shower_id;
   out.pending_photons = h.pending[0] + h.staged[0];
   out.pending_leptons = h.pending[1] + h.staged[1];
-  out.next_history_id = history;
+  out.next_history_id = h.call.counter;
   out.photon_fronts = h.fronts[0];
   out.lepton_fronts = h.fronts[1];
   out.photon_steps = st[0];
@@ -807,7 +1074,7 @@ long long EmBackend::pending_energy(std::uint32_t slot) {
     C8MI_HIP(hipMemcpy(w.data(), q.weight, n * 8, hipMemcpyDeviceToHost));
     C8MI_HIP(hipMemcpy(owner.data(), q.shower, n * 4, hipMemcpyDeviceToHost));
     for (std::uint64_t k = 0; k < n; ++k)
-      if (owner[k] == slot) pending += e[k] * w[k];
+      if ((owner[k] & kSlotBits) == slot) pending += e[k] * w[k];
   }
   for (int s = 0; s < 2; ++s)
     for (auto const& p : staged_[s])
@@ -961,7 +1228,7 @@ void EmBackend::serve(c8mi_em_request const* requests, std::uint64_t count, c8mi
       init.first_history = r.first_history_id;
       init.slot = slot;
       inits.push_back(init);
-      activate(slot, r.shower_id);
+      activate(slot, r.shower_id, r.first_history_id);
       if (slot_radio_) slot_radio_->set_slot(slot, r.radio_observers, r.radio_observer_count);
       slot_enqueue(slot, &r.primary, 1);
       owner[slot] = static_cast<std::int64_t>(next);
@@ -981,7 +1248,7 @@ void EmBackend::serve(c8mi_em_request const* requests, std::uint64_t count, c8mi
       continue;
     }
     c8mi_em_run_result round{};
-    run_fronts(0, 1, round);
+    run_rounds_loop(1, round);
     totals.photon_fronts += round.photon_fronts;
     totals.lepton_fronts += round.lepton_fronts;
     totals.photon_steps += round.photon_steps;
@@ -990,6 +1257,13 @@ void EmBackend::serve(c8mi_em_request const* requests, std::uint64_t count, c8mi
     totals.lepton_interactions += round.lepton_interactions;
     totals.secondaries += round.secondaries;
     totals.max_front = std::max(totals.max_front, round.max_front);
+    totals.photon_calls += round.photon_calls;
+    totals.lepton_calls += round.lepton_calls;
+    totals.bucketed_fronts += round.bucketed_fronts;
+    totals.bucketed_particles += round.bucketed_particles;
+    totals.below_minimum_batch_checkpoints += round.below_minimum_batch_checkpoints;
+    totals.history_range_checkpoints += round.history_range_checkpoints;
+    totals.reserved_history_ids += round.reserved_history_ids;
     finished.clear();
     for (std::uint32_t slot = 0; slot < slots_; ++slot) {
       if (owner[slot] < 0) continue;
@@ -1018,7 +1292,7 @@ void EmBackend::serve(c8mi_em_request const* requests, std::uint64_t count, c8mi
       out.photon_interactions = g[kGatherStats + 2];
       out.lepton_interactions = g[kGatherStats + 3];
       out.secondaries = g[kGatherStats + 4];
-      out.next_history_id = g[kGatherHistory];
+      out.next_history_id = slot_host_[slot].call.counter;
       out.observations = g[kGatherRecords];
       out.fallbacks = g[kGatherRecords + 1];
       long long c[kCounters];

@@ -20,6 +20,7 @@
 #include "em/device_tables.hpp"
 #include "em/em_device.hpp"
 #include "em/em_front.hpp"
+#include "em/em_order.hpp"
 #include "radio/radio_accumulator.hpp"
 #include "radio/slot_radio.hpp"
 #include "runtime.hpp"
@@ -51,8 +52,24 @@ class EmBackend {
   void begin_shower(std::uint64_t shower_id, std::uint64_t first_history_id);
   bool can_transport(c8mi_particle_state const& p) const;
   void enqueue(c8mi_particle_state const* particles, std::uint64_t count) { slot_enqueue(0, particles, count); }
-  // species: 0 both (alternating), 1 photons only, 2 leptons only
-  void run(int species, std::uint64_t maximum_fronts, c8mi_em_run_result& out);
+  // Rounds of the reference's call schedule over every active slot (each
+  // shower: a photon call of at most 16 fronts, then a lepton call of at most
+  // 1024, with history ranges reserved from its counter), until nothing is
+  // pending or `maximum_rounds` rounds (a photon front, then a lepton front)
+  // have run.
+  void run(std::uint64_t maximum_rounds, c8mi_em_run_result& out);
+  // IEmBackend::runPhotonWavefront / runLeptonWavefront for the shower of slot
+  // 0 (run_photon_cascade, PhotonCascade.inl:177-574; run_lepton_cascade,
+  // LeptonCascade.inl:86-565): one call over the device pending prefix and
+  // `particles`; the particles of the species left when the call ends are
+  // downloaded and removed (remaining()).
+  void run_call(bool photon, c8mi_particle_state const* particles, std::uint64_t count,
+                std::uint64_t first_secondary_history_id, std::uint64_t maximum_wavefronts,
+                std::uint64_t history_limit_exclusive, std::uint64_t minimum_resident_batch,
+                c8mi_em_wavefront_result& out);
+  std::vector<c8mi_particle_state> const& remaining() const noexcept { return remaining_; }
+  void limits(c8mi_em_limits& out) const;
+  void statistics(c8mi_em_statistics& out) const;
   void pending(std::uint64_t& photons, std::uint64_t& leptons) const;
   void download_profile(double* photons, double* electrons, double* positrons, double* energy_loss,
                         c8mi_em_ledger* ledger) {
@@ -103,6 +120,19 @@ class EmBackend {
 
  private:
   using Staged = StagedParticle;
+  // A shower's place in the reference's call schedule (EmRouter.inl:339-416).
+  struct CallState {
+    int phase = 0;                  // species of the current or next call
+    bool open = false;              // a call of `phase` holds a history range
+    bool opening = false;           // its first front has not run yet
+    int stop = 0;                   // how the last call ended (kStop*)
+    std::uint64_t call_fronts = 0;  // fronts of the open call
+    std::uint64_t counter = 0;      // next history ID to reserve
+    std::uint64_t next = 0;         // next secondary history ID of the open call
+    std::uint64_t limit = 0;        // exclusive limit of the open call
+    std::uint64_t cap = 0;          // fronts per call
+    std::uint64_t min_batch = 0;    // below-minimum-batch checkpoint
+  };
   struct SlotHost {
     bool active = false;
     std::uint64_t shower_id = 0;
@@ -111,18 +141,26 @@ class EmBackend {
     std::uint64_t fronts[2] = {0, 0};
     long double injected = 0;
     DeviceBuffer<double> axis;          // own projection table, if any
+    CallState call;
   };
+  enum : int { kStopEmpty = 0, kStopCap = 1, kStopMinimumBatch = 2, kStopHistory = 3 };
 
   void reset_pool();
   PoolArrays pool_arrays() const;
   // Enqueues the start of the given slots (no host synchronisation).
   void start_slots(SlotInit const* inits, std::uint32_t count);
-  void activate(std::uint32_t slot, std::uint64_t shower_id);
+  void activate(std::uint32_t slot, std::uint64_t shower_id, std::uint64_t first_history_id);
   void decode_ledger(long long* counters, std::uint32_t slot, c8mi_em_ledger& out) const;
-  void front(bool photon, c8mi_em_run_result& out);
-  void run_fronts(int species, std::uint64_t maximum_fronts, c8mi_em_run_result& out);
+  // One front of one species over the slots whose call runs now; false when
+  // no slot's call ran a front.
+  bool front(bool photon, c8mi_em_run_result& out);
+  // Whether slot `h`'s call of `species` runs a front over its n particles
+  // now (opening or ending calls as the schedule says).
+  bool schedule(CallState& c, int species, std::uint64_t n, c8mi_em_run_result& out) const;
+  void reserve_call(CallState& c, int species, std::uint64_t n, c8mi_em_run_result& out) const;
+  void run_rounds_loop(std::uint64_t maximum_rounds, c8mi_em_run_result& out);
   void flush_staged(int species);
-  void append_run(int species, std::uint64_t end);
+  void add_medium(std::int32_t medium);
   std::uint32_t active_slot_count() const;
   void resize_scratch(std::uint64_t n);
   void check_slot(std::uint32_t slot) const;
@@ -174,8 +212,22 @@ class EmBackend {
   std::unique_ptr<QueueStorage> queues_[2][2];
   int current_[2] = {0, 0};
   std::uint64_t count_[2] = {0, 0};
-  std::vector<std::uint64_t> runs_[2];  // exclusive run ends in the current buffer
+  // the queue holds only the last front's continuing particles (clean slot
+  // words, in front order): a front needs no ordering step unless a call
+  // opens, a shower is bucketed or sits out
+  bool clean_[2] = {true, true};
   std::vector<Staged> staged_[2];
+  // front ordering (bucketing, call-input order, parked showers)
+  FrontOrder order_;
+  std::vector<std::int32_t> media_;  // sorted distinct medium IDs
+  DeviceBuffer<SlotControl> control_dev_;
+  PinnedBuffer<SlotControl> control_host_;
+  hipEvent_t control_copied_ = nullptr;
+  std::vector<c8mi_particle_state> remaining_;
+  std::vector<CallState> plan_;
+  std::vector<std::uint64_t> front_counts_;
+  // cumulative statistics (IEmBackend::statistics)
+  c8mi_em_statistics totals_{};
   // front scratch
   DeviceBuffer<c8mi_particle_state> staging_;
   DeviceBuffer<std::uint32_t> packed_;

@@ -36,6 +36,7 @@
 
 #include "em/em_device.hpp"
 #include "em/em_front.hpp"
+#include "em/em_order.hpp"
 #include "em/steps.hpp"
 #include "runtime.hpp"
 
@@ -570,8 +571,9 @@ __global__ __launch_bounds__(kScanBlock) void scan_kernel(FrontArgs f, std::uint
     st[photon ? 0 : 1] += steps;
     st[photon ? 2 : 3] += interactions;
     st[4] += within[2];
-    f.report[kReportHeader + 2 * static_cast<std::uint64_t>(s)] = within[0];
-    f.report[kReportHeader + 2 * static_cast<std::uint64_t>(s) + 1] = within[1];
+    f.report[kReportHeader + kReportPerSlot * static_cast<std::uint64_t>(s)] = within[0];
+    f.report[kReportHeader + kReportPerSlot * static_cast<std::uint64_t>(s) + 1] = within[1];
+    f.report[kReportHeader + kReportPerSlot * static_cast<std::uint64_t>(s) + 2] = within[2];
   }
   if (threadIdx.x == 0) {
     FrontTotals out{};
@@ -616,7 +618,7 @@ __global__ __launch_bounds__(kStepBlock) void scatter_kernel(FrontArgs f, Partic
     if (kind != kSlotNextContinue) p.history_id = child_base + p.history_id;
     if (kind == kSlotOtherChild) {
       store_particle(other_queue, other_pos, p);
-      other_queue.shower[other_pos++] = s;
+      other_queue.shower[other_pos++] = s | (kClassPending << kClassShift);
     } else {
       store_particle(next_queue, next_pos, p);
       next_queue.shower[next_pos++] = s;
@@ -705,7 +707,13 @@ __global__ __launch_bounds__(256) void unstage_kernel(StagedParticle const* __re
   std::uint64_t const i = static_cast<std::uint64_t>(blockIdx.x) * 256 + threadIdx.x;
   if (i >= n) return;
   store_particle(queue, at + i, staged[i].particle);
-  queue.shower[at + i] = staged[i].slot;
+  queue.shower[at + i] = staged[i].slot | (kClassHost << kClassShift);
+}
+
+__global__ __launch_bounds__(256) void pack_kernel(ParticleQueue queue, std::uint64_t n,
+                                                   c8mi_particle_state* __restrict__ out) {
+  std::uint64_t const i = static_cast<std::uint64_t>(blockIdx.x) * 256 + threadIdx.x;
+  if (i < n) out[i] = load_particle(queue, i);
 }
 
 __global__ __launch_bounds__(256) void slot_init_kernel(PoolArrays pool, SlotInit const* __restrict__ inits) {
@@ -848,6 +856,12 @@ void launch_unstage(StagedParticle const* staged, std::uint64_t n, ParticleQueue
                     hipStream_t stream) {
   if (n == 0) return;
   unstage_kernel<<<grid_for(n, 256), 256, 0, stream>>>(staged, n, queue, at);
+  C8MI_HIP(hipGetLastError());
+}
+
+void launch_pack(ParticleQueue const& queue, std::uint64_t n, c8mi_particle_state* out, hipStream_t stream) {
+  if (n == 0) return;
+  pack_kernel<<<grid_for(n, 256), 256, 0, stream>>>(queue, n, out);
   C8MI_HIP(hipGetLastError());
 }
 

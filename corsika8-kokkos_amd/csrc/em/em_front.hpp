@@ -11,9 +11,9 @@
 
 namespace c8mi::em {
 
-// A queue holds up to kMaxRuns runs, each sorted by pool slot: the previous
-// front's stable output, then the secondaries appended by the other species'
-// front, then particles staged by the host.
+// A front's input is one run sorted by pool slot (the previous front's stable
+// output, or the gather of the ordering step); the (slot, run) arrays keep
+// room for kMaxRuns runs.
 inline constexpr int kMaxRuns = 4;
 // Per-(slot, run) output totals of a front: next, other, children, steps,
 // interactions.
@@ -22,8 +22,10 @@ inline constexpr int kShowerFields = 5;
 // lepton interactions, secondaries.
 inline constexpr int kSlotStats = 8;
 // Front report read back by the host: FrontTotals (8 words), then per slot the
-// new count of the stepped species and the count appended to the other one.
+// new count of the stepped species, the count appended to the other one and
+// the children given history IDs.
 inline constexpr int kReportHeader = 8;
+inline constexpr int kReportPerSlot = 3;
 
 // An e+- step that reached its interaction vertex, handed from the transport
 // kernel to the vertex kernel (compacted, in source order within each block).
@@ -143,7 +145,10 @@ struct StagedParticle {
   std::uint32_t pad[3];
 };
 
-// Writes n staged particles into queue entries [at, at + n).
+// Copies queue entries [0, n) into `out` as particle records.
+void launch_pack(ParticleQueue const& queue, std::uint64_t n, c8mi_particle_state* out, hipStream_t stream);
+
+// Writes n staged particles into queue entries [at, at + n), class host.
 void launch_unstage(StagedParticle const* staged, std::uint64_t n, ParticleQueue const& queue, std::uint64_t at,
                     hipStream_t stream);
 // Clears the profile, ledger, records and statistics of each started slot and

@@ -298,6 +298,17 @@ typedef struct c8mi_em_config {
    * particle queues (queue_capacity per species and buffer) are shared. */
   uint32_t shower_slots;
   uint32_t reserved_;
+  /* IEmBackend::minimumBatchSize (BackendConfig::minBatchSize_,
+   * corsika/kokkos/em/execution/BackendConfig.hpp:26; 0 selects its default
+   * 16384): the handoff batch a host router stages towards
+   * (EmRouter.inl:246-260). */
+  uint64_t minimum_batch_size;
+  /* Below-minimum-batch checkpoint of the device-resident drivers (c8mi_em_run,
+   * c8mi_em_run_rounds, c8mi_em_serve): a shower's call of one species ends
+   * after its first front once that shower's next front would hold fewer
+   * particles (PhotonCascade.inl:296-300); the rest waits for the shower's
+   * next call of the species. 0 or 1: never. */
+  uint64_t resident_minimum_batch;
 } c8mi_em_config;
 
 /* Shower axis of one pool slot (the ProfileProjectionView of its ShowerConfig):
@@ -344,13 +355,25 @@ typedef struct c8mi_em_run_result {
   uint64_t pending_leptons;
   uint64_t next_history_id;
   uint64_t max_front;       /* largest single front processed */
+  /* The reference's call schedule (EmRouter.inl:368-407): per shower, a photon
+   * call of at most 16 fronts, then a lepton call of at most 1024 fronts, each
+   * with its own reserved history range. */
+  uint64_t photon_calls;    /* calls that reserved a history range */
+  uint64_t lepton_calls;
+  uint64_t bucketed_fronts;     /* (shower, front) pairs of >= 256 particles, stably
+                                   sorted by the bucket key (WavefrontBucketing.inl:72-79) */
+  uint64_t bucketed_particles;
+  uint64_t below_minimum_batch_checkpoints;
+  uint64_t history_range_checkpoints;
+  uint64_t reserved_history_ids;
 } c8mi_em_run_result;
 
 /* Energy ledger of one shower (ProfileCounters / ProfileResult ledger terms,
  * corsika/kokkos/output/ProfileCounters.hpp; closure ShowerReport.inl:1236-1276).
  * Weighted energies in GeV, decoded from the device's int64 fixed point. */
 typedef struct c8mi_em_ledger {
-  double injected_energy;               /* weighted total energy enqueued */
+  double injected_energy;               /* weighted total energy enqueued, less the
+                                           remaining particles wavefront calls returned */
   double continuous_deposit;            /* dE/dX along lepton steps */
   double cut_deposit;                   /* kinetic (photon: total) energy at particle cuts */
   double cut_rest_mass;                 /* lepton rest mass at particle cuts */
@@ -377,6 +400,71 @@ typedef struct c8mi_em_ledger {
 
 /* FallbackRequest subset: the particle at the failing stage and the reason
  * (FallbackReason, corsika/kokkos/em/FallbackRequest.hpp:22-65). */
+/* PhotonCascadeResult / LeptonCascadeResult of one IEmBackend wavefront call
+ * (corsika/kokkos/em/execution/{Photon,Lepton}CascadeResult.hpp). */
+typedef struct c8mi_em_wavefront_result {
+  uint64_t input_particles;      /* device pending prefix + host particles */
+  uint64_t wavefronts;
+  uint64_t transport_records;    /* steps (PhotonCascade.inl:499) */
+  uint64_t interactions;
+  uint64_t secondaries;          /* children given history IDs */
+  uint64_t next_secondary_history_id;
+  uint64_t peak_resident;        /* largest front of the call */
+  uint64_t remaining;            /* particles of the species left when the call
+                                    ended (remainingPhotons_ / remainingLeptons_),
+                                    c8mi_em_download_remaining */
+  uint64_t other_species_pending; /* the other species' device pending queue */
+  uint64_t bucketed_fronts;      /* wavefrontBucketingBatches_ (fronts of >= 256) */
+  uint64_t bucketed_particles;
+  uint64_t small_fronts;         /* wavefrontBucketingSmallBatches_ */
+  uint64_t small_particles;
+  uint64_t observations;         /* recorded during the call */
+  uint64_t fallbacks;
+  uint32_t completed;            /* no particle of the species left */
+  uint32_t below_minimum_batch_checkpoint;
+  uint32_t history_range_exhausted;
+  uint32_t pad_;
+} c8mi_em_wavefront_result;
+
+/* IEmBackend::minimumBatchSize and maximumResident{Photon,Lepton,Input}BatchSize
+ * (IEmBackend.hpp:31-35). */
+typedef struct c8mi_em_limits {
+  uint64_t minimum_batch_size;
+  uint64_t maximum_resident_photon_batch;
+  uint64_t maximum_resident_lepton_batch;
+  uint64_t maximum_resident_input_batch;
+  uint64_t queue_capacity;
+  uint32_t shower_slots;
+  uint32_t pad_;
+} c8mi_em_limits;
+
+/* The transport counters of IEmBackend::statistics (BackendStatistics,
+ * corsika/kokkos/em/execution/BackendStatistics.hpp:20-150), accumulated since
+ * c8mi_em_create over every call and slot. */
+typedef struct c8mi_em_statistics {
+  uint64_t shower_ordinal;        /* showers begun (begin_shower, slot starts, served requests) */
+  uint64_t wavefronts;
+  uint64_t photon_wavefronts;
+  uint64_t lepton_wavefronts;
+  uint64_t particles_enqueued;
+  uint64_t particles_advanced;    /* transport records */
+  uint64_t particles_produced;    /* children given history IDs */
+  uint64_t interactions_selected;
+  uint64_t bucketed_fronts;
+  uint64_t bucketed_particles;
+  uint64_t small_fronts;
+  uint64_t small_particles;
+  uint64_t ordered_fronts;        /* fronts that needed the ordering step */
+  uint64_t photon_calls;
+  uint64_t lepton_calls;
+  uint64_t below_minimum_batch_checkpoints;
+  uint64_t history_range_checkpoints;
+  uint64_t reserved_history_ids;
+  uint64_t remaining_downloaded; /* particles returned by wavefront calls */
+  uint64_t host_to_device_bytes;
+  uint64_t device_to_host_bytes;
+} c8mi_em_statistics;
+
 typedef struct c8mi_em_fallback {
   c8mi_particle_state particle;
   int32_t reason;
@@ -403,19 +491,58 @@ int c8mi_em_can_transport(const c8mi_em* em, const c8mi_particle_state* particle
 /* Adds host particles to the device queues (photons and e+- split by PID). */
 int c8mi_em_enqueue(c8mi_em* em, const c8mi_particle_state* particles, uint64_t count);
 
-/* IEmBackend::runPhotonWavefront / runLeptonWavefront: enqueue `count`
- * particles, then run at most `maximum_fronts` fronts of that species;
- * secondaries of the other species stay queued on the device. */
+/* IEmBackend::runPhotonWavefront (IEmBackend.hpp:51-53; run_photon_cascade,
+ * corsika/detail/kokkos/em/execution/PhotonCascade.inl:177-574) for the shower
+ * begun with c8mi_em_begin_shower. The call's input is the device pending
+ * photon prefix (photons the lepton fronts appended) followed by `particles`
+ * (photons only). At most `maximum_wavefronts` fronts run; each front of at
+ * least 256 photons is first stably sorted by the bucket key
+ * (WavefrontBucketing.inl:72-79); children take history IDs from
+ * first_secondary_history_id on, in front order; after the first front the
+ * call stops when fewer than `minimum_resident_batch` photons remain. Charged
+ * children stay in the device pending lepton queue. Photons left when the call
+ * ends are downloaded (c8mi_em_download_remaining) and no longer queued.
+ * Errors as the reference: zero first ID, wavefronts or batch -> 1
+ * (invalid_argument); more input than the queue holds -> 3 (length_error);
+ * other pool slots active -> 2 (logic_error). */
 int c8mi_em_run_photon_wavefront(c8mi_em* em, const c8mi_particle_state* particles,
-                                 uint64_t count, uint64_t maximum_fronts,
-                                 c8mi_em_run_result* out);
+                                 uint64_t count, uint64_t first_secondary_history_id,
+                                 uint64_t maximum_wavefronts, uint64_t minimum_resident_batch,
+                                 c8mi_em_wavefront_result* out);
+/* IEmBackend::runLeptonWavefront (IEmBackend.hpp:55-58; run_lepton_cascade,
+ * LeptonCascade.inl:86-565): as above for e+-, with the history range
+ * [first_secondary_history_id, secondary_history_id_limit_exclusive): before
+ * each front the call stops when the front holds more than a third of the IDs
+ * left (historyRangeExhausted_, LeptonCascade.inl:248-252). Photons join the
+ * device pending photon queue. limit <= first -> 1 (invalid_argument). */
 int c8mi_em_run_lepton_wavefront(c8mi_em* em, const c8mi_particle_state* particles,
-                                 uint64_t count, uint64_t maximum_fronts,
-                                 c8mi_em_run_result* out);
+                                 uint64_t count, uint64_t first_secondary_history_id,
+                                 uint64_t maximum_wavefronts,
+                                 uint64_t secondary_history_id_limit_exclusive,
+                                 uint64_t minimum_resident_batch, c8mi_em_wavefront_result* out);
+/* The remaining particles of the last wavefront call (remainingPhotons_ /
+ * remainingLeptons_): up to `capacity` copied into `out`; *count the total. */
+int c8mi_em_download_remaining(c8mi_em* em, c8mi_particle_state* out, uint64_t capacity,
+                               uint64_t* count);
 
-/* Runs photon and lepton fronts alternately (photon front, then lepton front)
- * until both queues are empty or `maximum_fronts` pairs have run. */
-int c8mi_em_run(c8mi_em* em, uint64_t maximum_fronts, c8mi_em_run_result* out);
+/* IEmBackend limits (IEmBackend.hpp:31-35). */
+int c8mi_em_get_limits(const c8mi_em* em, c8mi_em_limits* out);
+/* IEmBackend::statistics (IEmBackend.hpp:73): cumulative transport counters. */
+int c8mi_em_get_statistics(const c8mi_em* em, c8mi_em_statistics* out);
+
+/* The device-resident driver: rounds of the reference's call schedule over
+ * every active slot (EmRouter.inl:339-416 with the device pending queues, as
+ * the router runs a shower whose fronts all stay on the device): per shower a
+ * photon call of at most 16 fronts, then a lepton call of at most 1024 fronts,
+ * each reserving its history range from the shower's counter
+ * (first_history_id of the begin call; photons 2 n per front of the call,
+ * leptons max(3 n, min(2^20, 192 n)), EmRouter.inl:640-679), with the
+ * bucketing, the call-input order (pending, then remaining, then host) and the
+ * exhaustion and resident_minimum_batch checkpoints of the wavefront calls.
+ * Runs until nothing is pending or `maximum_rounds` rounds (a photon front,
+ * then a lepton front) have run. out->next_history_id: the next ID slot 0's
+ * counter would reserve. */
+int c8mi_em_run(c8mi_em* em, uint64_t maximum_rounds, c8mi_em_run_result* out);
 
 /* IEmBackend::pendingPhotonCount / pendingLeptonCount. */
 int c8mi_em_pending_counts(const c8mi_em* em, uint64_t* photons, uint64_t* leptons);

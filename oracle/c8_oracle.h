@@ -137,8 +137,14 @@ enum {
 
 typedef c8mi_em_run_result or_em_stats;
 
-/* Runs one shower to completion (or maximum_fronts photon+lepton front pairs)
- * with the serial front schedule of c8mi_em_run. Histograms hold profile_bins
+/* Runs one shower to completion (or maximum_fronts rounds) with the reference's
+ * call schedule as c8mi_em_run runs it: per round the shower's photon front,
+ * then its lepton front, when its schedule runs one (a photon call of at most
+ * 16 fronts, then a lepton call of at most 1024, each with its own reserved
+ * history range starting from first_history_id; fronts of at least 256
+ * particles stably sorted by oracle_em_bucket_key; below-minimum-batch
+ * checkpoint config->resident_minimum_batch). stats->next_history_id is the
+ * next ID the shower's counter would reserve. Histograms hold profile_bins
  * int64 each; counters OR_EM_COUNTERS; observations / fallbacks up to the
  * configured capacities (may be NULL). */
 int oracle_em_run_shower(const c8mi_em_config* config, const c8mi_native_table_set* tables,
@@ -161,6 +167,14 @@ int oracle_em_run_shower_tracks(const c8mi_em_config* config, const c8mi_native_
                                 c8mi_em_fallback* fallbacks, or_em_stats* stats, double** tracks,
                                 uint64_t* track_count);
 void oracle_free(void* p);
+
+/* Test hook: the lepton history reservation max(3 n, min(chunk, multiplier *
+ * 3 n)) of oracle_em_run_shower (0, 0: the reference's 64 and 2^20). Process
+ * wide; not thread safe. */
+void oracle_em_set_lepton_reservation(uint64_t multiplier, uint64_t chunk);
+
+/* wavefront_bucket_key (WavefrontBucketing.inl:72-79). */
+uint64_t oracle_em_bucket_key(const c8mi_particle_state* p);
 
 /* oracle_em_run_shower over shower_count single-primary showers on `threads`
  * host threads (shower k: primaries[k], shower_id first_shower + k, first

@@ -1826,15 +1826,43 @@ static int brems_lpm(const c8mi_brems_lpm* s, uint64_t comp, double e, double v,
 
 /* ========================== shower state ========================== */
 
+/* Class of a queued particle, which orders the input of a call: the reference
+ * uploads the device pending-queue prefix (particles the other species' fronts
+ * appended) ahead of the host particles (WavefrontQueue.inl:207-260), and the
+ * host particles are the ones left over by the species' previous call
+ * (remainingPhotons_/remainingLeptons_, staged by EmRouter.inl:520-521,570-571)
+ * followed by newly staged ones. */
+enum { CLS_PENDING = 0, CLS_REMAINING = 1, CLS_HOST = 2 };
+
 typedef struct {
   ps_t* items;
+  uint8_t* cls;
   uint64_t count, capacity;
 } queue_t;
+
+/* The reference's call schedule for one shower (EmRouter.inl:339-416,
+ * PhotonCascade.inl:286-300, LeptonCascade.inl:240-260): a photon call of at
+ * most 16 fronts, then a lepton call of at most 1024 fronts, repeated; each
+ * call reserves its own history range from the shower's counter (the stack's
+ * reserveTransportHistoryIds, EmRouter.inl:627-679) and ends early below the
+ * minimum batch (after its first front) or, for leptons, when the front could
+ * exhaust the range. */
+typedef struct {
+  int phase;         /* 0 photon call, 1 lepton call */
+  int open;          /* a call of `phase` holds a reservation */
+  uint64_t fronts;   /* fronts of the open call */
+  uint64_t counter;  /* next history ID to reserve */
+  uint64_t next;     /* next secondary history ID of the open call */
+  uint64_t limit;    /* exclusive limit of the open lepton call */
+  uint64_t min_batch;
+} sched_t;
+
+enum { PHOTON_CALL_FRONTS = 16, LEPTON_CALL_FRONTS = 1024, BUCKET_MINIMUM = 256 };
 
 typedef struct {
   ctx_t x;
   queue_t photons, leptons;
-  uint64_t next_history;
+  sched_t sched;
   prof_t prof;
   ps_t* observations;
   uint64_t observation_count;
@@ -1872,12 +1900,20 @@ static void record_track(shower_t* s, const ps_t* start, const ps_t* end) {
   t[8] = start->pid == 11 ? -start->weight : start->weight;
 }
 
-static void push(queue_t* q, const ps_t* p) {
+static void push(queue_t* q, const ps_t* p, int cls) {
   if (q->count == q->capacity) {
     q->capacity = q->capacity ? 2 * q->capacity : 1024;
     q->items = (ps_t*)realloc(q->items, q->capacity * sizeof(ps_t));
+    q->cls = (uint8_t*)realloc(q->cls, q->capacity);
   }
+  q->cls[q->count] = (uint8_t)cls;
   q->items[q->count++] = *p;
+}
+
+static void queue_free(queue_t* q) {
+  free(q->items);
+  free(q->cls);
+  memset(q, 0, sizeof *q);
 }
 
 static void record_observation(shower_t* s, const ps_t* p) {
@@ -3251,45 +3287,189 @@ static void lepton_source(shower_t* s, const ps_t* p, lout_t* o, uint64_t child_
 
 /* ========================== driver ========================== */
 
-/* One front of one species: every source in order; continuing particles of the
- * species go to `next`, the other species is appended to its queue. History IDs
- * come from the running child count (the stable prefix sum of the device). */
+/* wavefront_bucket_key (WavefrontBucketing.inl:24-79): PID bucket (photon 0,
+ * e- 1, e+ 2, other 3) << 48 | (medium ^ 2^31) << 16 | energy bin, the bin
+ * floor(log2(E/GeV) * 16 + 32768) clamped to [0, 65534], 65535 for a
+ * non-positive or non-finite energy. */
+uint64_t oracle_em_bucket_key(const c8mi_particle_state* p) {
+  uint64_t const pid = p->pid == 22 ? 0u : p->pid == 11 ? 1u : p->pid == -11 ? 2u : 3u;
+  uint64_t const medium = (uint64_t)((uint32_t)p->medium_id ^ 0x80000000u);
+  uint64_t bin;
+  double const e = p->energy_gev;
+  if (!(e > 0.) || !finite(e)) {
+    bin = 65535u;
+  } else {
+    double const c = floor(log2(e) * 16. + 32768.);
+    bin = !(c > 0.) ? 0u : c >= 65534. ? 65534u : (uint64_t)c;
+  }
+  return pid << 48 | medium << 16 | bin;
+}
+
+typedef struct {
+  uint64_t key;
+  uint64_t index;
+} sort_item_t;
+
+static int sort_item_less(const void* a, const void* b) {
+  const sort_item_t* x = (const sort_item_t*)a;
+  const sort_item_t* y = (const sort_item_t*)b;
+  if (x->key != y->key) return x->key < y->key ? -1 : 1;
+  return x->index < y->index ? -1 : x->index > y->index;
+}
+
+/* Stable reorder of a queue by key(i): ties keep the queue order. */
+static void stable_reorder(queue_t* q, uint64_t (*key)(const queue_t*, uint64_t)) {
+  uint64_t const n = q->count;
+  sort_item_t* order = (sort_item_t*)malloc(n * sizeof *order);
+  ps_t* items = (ps_t*)malloc(n * sizeof *items);
+  uint8_t* cls = (uint8_t*)malloc(n);
+  for (uint64_t i = 0; i < n; ++i) {
+    order[i].key = key(q, i);
+    order[i].index = i;
+  }
+  qsort(order, n, sizeof *order, sort_item_less);
+  for (uint64_t j = 0; j < n; ++j) {
+    items[j] = q->items[order[j].index];
+    cls[j] = q->cls[order[j].index];
+  }
+  free(q->items);
+  free(q->cls);
+  q->items = items;
+  q->cls = cls;
+  q->capacity = n;
+  free(order);
+}
+
+static uint64_t class_key(const queue_t* q, uint64_t i) { return q->cls[i]; }
+static uint64_t bucket_key(const queue_t* q, uint64_t i) { return oracle_em_bucket_key(&q->items[i]); }
+
+/* One front of one species over the whole queue, in queue order (after the
+ * bucketing of PhotonCascade.inl:318-332 / LeptonCascade.inl:296-307 for
+ * fronts of at least 256 particles): continuing particles of the species form
+ * the next front, the other species' particles join its queue as pending;
+ * children take history IDs from the call's running counter in source order. */
+static void bucket_front(shower_t* s, queue_t* q) {
+  if (q->count >= BUCKET_MINIMUM) {
+    stable_reorder(q, bucket_key);
+    s->stats.bucketed_fronts++;
+    s->stats.bucketed_particles += q->count;
+  }
+}
+
 static void photon_front(shower_t* s) {
+  bucket_front(s, &s->photons);
   queue_t in = s->photons;
-  queue_t next = {NULL, 0, 0};
+  queue_t next;
+  memset(&next, 0, sizeof next);
   uint64_t child = 0;
-  uint64_t first = s->next_history;
+  uint64_t first = s->sched.next;
   for (uint64_t i = 0; i < in.count; ++i) {
     pout_t o;
     photon_source(s, &in.items[i], &o, child, first);
-    for (uint32_t k = 0; k < o.next_count; ++k) push(&next, &o.next[k]);
-    for (uint32_t k = 0; k < o.charged_count; ++k) push(&s->leptons, &o.charged[k]);
+    for (uint32_t k = 0; k < o.next_count; ++k) push(&next, &o.next[k], CLS_REMAINING);
+    for (uint32_t k = 0; k < o.charged_count; ++k) push(&s->leptons, &o.charged[k], CLS_PENDING);
     child += o.child_count;
   }
-  s->next_history = first + child;
-  free(in.items);
-  s->photons = next;
+  s->sched.next = first + child;
   s->stats.photon_fronts++;
   if (in.count > s->stats.max_front) s->stats.max_front = in.count;
+  queue_free(&in);
+  s->photons = next;
 }
 
 static void lepton_front(shower_t* s) {
+  bucket_front(s, &s->leptons);
   queue_t in = s->leptons;
-  queue_t next = {NULL, 0, 0};
+  queue_t next;
+  memset(&next, 0, sizeof next);
   uint64_t child = 0;
-  uint64_t first = s->next_history;
+  uint64_t first = s->sched.next;
   for (uint64_t i = 0; i < in.count; ++i) {
     lout_t o;
     lepton_source(s, &in.items[i], &o, child, first);
-    for (uint32_t k = 0; k < o.next_count; ++k) push(&next, &o.next[k]);
-    for (uint32_t k = 0; k < o.photon_count; ++k) push(&s->photons, &o.photons[k]);
+    for (uint32_t k = 0; k < o.next_count; ++k) push(&next, &o.next[k], CLS_REMAINING);
+    for (uint32_t k = 0; k < o.photon_count; ++k) push(&s->photons, &o.photons[k], CLS_PENDING);
     child += o.child_count;
   }
-  s->next_history = first + child;
-  free(in.items);
-  s->leptons = next;
+  s->sched.next = first + child;
   s->stats.lepton_fronts++;
   if (in.count > s->stats.max_front) s->stats.max_front = in.count;
+  queue_free(&in);
+  s->leptons = next;
+}
+
+/* The lepton reservation's multiplier and chunk (64 and 2^20 in the
+ * reference); tests lower them to force history-range checkpoints. */
+static uint64_t g_lepton_multiplier = 64u;
+static uint64_t g_lepton_chunk = (uint64_t)1 << 20;
+
+void oracle_em_set_lepton_reservation(uint64_t multiplier, uint64_t chunk) {
+  g_lepton_multiplier = multiplier ? multiplier : 64u;
+  g_lepton_chunk = chunk ? chunk : ((uint64_t)1 << 20);
+}
+
+/* History reservation of a call of n input particles: photons 2 n per front of
+ * the call (reservePhotonSecondaryRange, EmRouter.inl:640-656); leptons
+ * max(3 n, min(2^20, 64 * 3 n)) with its exclusive limit
+ * (reserveResidentSecondaryRange, EmRouter.inl:658-679). */
+static void open_call(shower_t* s, int lepton, uint64_t n) {
+  sched_t* c = &s->sched;
+  uint64_t count;
+  if (!lepton) {
+    count = n * 2u * PHOTON_CALL_FRONTS;
+  } else {
+    uint64_t const minimum = 3u * n;
+    count = minimum;
+    if (minimum <= UINT64_MAX / g_lepton_multiplier) {
+      count = minimum * g_lepton_multiplier < g_lepton_chunk ? minimum * g_lepton_multiplier : g_lepton_chunk;
+      if (count < minimum) count = minimum;
+    }
+  }
+  c->next = c->counter;
+  c->limit = c->counter + count;
+  c->counter += count;
+  c->fronts = 0;
+  c->open = 1;
+  s->stats.reserved_history_ids += count;
+  if (lepton) s->stats.lepton_calls++;
+  else s->stats.photon_calls++;
+}
+
+/* Whether the shower's next front of `lepton` runs now: opens a call when the
+ * shower enters that phase with particles (input ordered pending, remaining,
+ * host), ends the call when its queue is empty, after the call's last front,
+ * below the minimum batch after its first front, or (leptons) when the front
+ * could exhaust the reserved range; an ended call passes to the other species. */
+static int schedule_front(shower_t* s, int lepton) {
+  sched_t* c = &s->sched;
+  queue_t* q = lepton ? &s->leptons : &s->photons;
+  if (c->phase != lepton) return 0;
+  if (!c->open) {
+    if (q->count == 0) {
+      c->phase ^= 1;
+      return 0;
+    }
+    stable_reorder(q, class_key);
+    open_call(s, lepton, q->count);
+  }
+  uint64_t const n = q->count;
+  int stop = n == 0 || c->fronts >= (uint64_t)(lepton ? LEPTON_CALL_FRONTS : PHOTON_CALL_FRONTS);
+  if (!stop && c->fronts != 0 && n < c->min_batch) {
+    stop = 1;
+    s->stats.below_minimum_batch_checkpoints++;
+  }
+  if (!stop && lepton && n > (c->limit - c->next) / 3u) {
+    stop = 1;
+    s->stats.history_range_checkpoints++;
+  }
+  if (stop) {
+    c->open = 0;
+    c->phase ^= 1;
+    for (uint64_t i = 0; i < q->count; ++i) q->cls[i] = CLS_REMAINING;
+    return 0;
+  }
+  c->fronts++;
+  return 1;
 }
 
 static int run_shower(const c8mi_em_config* config, const c8mi_native_table_set* tables,
@@ -3311,7 +3491,8 @@ static int run_shower(const c8mi_em_config* config, const c8mi_native_table_set*
   s.x.cfg = config;
   s.x.seed = config->random_seed;
   s.x.shower = shower_id;
-  s.next_history = first_history_id;
+  s.sched.counter = first_history_id;
+  s.sched.min_batch = config->resident_minimum_batch;
   s.prof.photons = photons;
   s.prof.electrons = electrons;
   s.prof.positrons = positrons;
@@ -3321,17 +3502,19 @@ static int run_shower(const c8mi_em_config* config, const c8mi_native_table_set*
   for (uint64_t i = 0; i < primary_count; ++i) {
     const ps_t* p = &primaries[i];
     ledger_add(&s.prof, config, OR_EMC_INJECTED, p->energy_gev * p->weight);
-    if (p->pid == 22) push(&s.photons, p);
-    else if (is_ee(p->pid)) push(&s.leptons, p);
+    if (p->pid == 22) push(&s.photons, p, CLS_HOST);
+    else if (is_ee(p->pid)) push(&s.leptons, p, CLS_HOST);
     else {
       fb_t f = {1, FR_UNSUPPORTED_PARTICLE, *p, 0};
       record_fallback(&s, &f, STAGE_SELECTION);
     }
   }
   uint64_t rounds = 0;
+  /* rounds of the device pool: the shower's photon front (if its schedule
+   * runs one now), then its lepton front */
   while ((s.photons.count || s.leptons.count) && rounds < maximum_fronts) {
-    if (s.photons.count) photon_front(&s);
-    if (s.leptons.count) lepton_front(&s);
+    if (schedule_front(&s, 0)) photon_front(&s);
+    if (schedule_front(&s, 1)) lepton_front(&s);
     ++rounds;
   }
   double pending = 0.;
@@ -3342,12 +3525,12 @@ static int run_shower(const c8mi_em_config* config, const c8mi_native_table_set*
   ledger_add(&s.prof, config, OR_EMC_PENDING, pending);
   s.stats.pending_photons = s.photons.count;
   s.stats.pending_leptons = s.leptons.count;
-  s.stats.next_history_id = s.next_history;
+  s.stats.next_history_id = s.sched.counter;
   s.stats.observations = s.observation_count;
   memcpy(counters, s.prof.counters, sizeof s.prof.counters);
   *stats = s.stats;
-  free(s.photons.items);
-  free(s.leptons.items);
+  queue_free(&s.photons);
+  queue_free(&s.leptons);
   if (tracks) {
     if (s.track_overflow) {
       free(s.tracks);

@@ -229,6 +229,33 @@ def em_run_shower(config, tables, environment, primaries, shower_id=1, first_his
     return out
 
 
+def em_bucket_key(particle) -> int:
+    """oracle_em_bucket_key: the reference's 50-bit wavefront bucket key."""
+    from c8mi import PARTICLE_STATE_DTYPE
+
+    lib = load()
+    lib.oracle_em_bucket_key.restype = ctypes.c_uint64
+    p = np.ascontiguousarray(particle, dtype=PARTICLE_STATE_DTYPE).reshape(-1)[:1]
+    return int(lib.oracle_em_bucket_key(p.ctypes.data_as(c_void_p)))
+
+
+class lepton_reservation:
+    """Context manager: the oracle's lepton history reservation
+    max(3 n, min(chunk, multiplier * 3 n)) inside the block (the reference's
+    64 and 2^20 outside)."""
+
+    def __init__(self, multiplier, chunk=1 << 20):
+        self.multiplier, self.chunk = multiplier, chunk
+
+    def __enter__(self):
+        load().oracle_em_set_lepton_reservation(c_uint64(self.multiplier), c_uint64(self.chunk))
+        return self
+
+    def __exit__(self, *exc):
+        load().oracle_em_set_lepton_reservation(c_uint64(0), c_uint64(0))
+        return False
+
+
 def em_run_batch(config, tables, environment, primaries, first_shower=1, threads=1,
                  maximum_fronts=1 << 40):
     """Independent single-primary showers on `threads` host threads; returns

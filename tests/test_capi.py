@@ -168,7 +168,8 @@ def test_em_binding_structs_match_c_layout(tmp_path):
              "c8mi_em_slot_state": em.SlotState, "c8mi_em_request": em.Request,
              "c8mi_em_serve_options": em.ServeOptions, "c8mi_em_request_result": em.RequestResult,
              "c8mi_environment": em.Environment, "c8mi_env_node": em.EnvNode,
-             "c8mi_env_model": em.EnvModel}
+             "c8mi_env_model": em.EnvModel, "c8mi_em_wavefront_result": em.WavefrontResult,
+             "c8mi_em_limits": em.Limits, "c8mi_em_statistics": em.Statistics}
     src = tmp_path / "sizes.c"
     src.write_text('#include <stdio.h>\n#include "c8mi/c8mi_em.h"\nint main(void){\n' +
                    "".join(f'printf("%zu\\n", sizeof({n}));\n' for n in names) + "return 0;}\n")

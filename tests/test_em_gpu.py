@@ -2,9 +2,12 @@
 serial restatement (oracle/em_oracle.c) on the same tables, atmosphere,
 primaries and Philox keys.
 
-Both run the same front schedule (photon front, then lepton front) with the
-stable source order of the reference's scans, so history IDs, every Philox
-draw and therefore whole showers coincide. The device evaluates log/exp/pow
+Both run the reference's call schedule (per shower a photon call of at most
+16 fronts, then a lepton call of at most 1024, each with its own reserved
+history range; fronts of at least 256 particles stably sorted by the bucket
+key; call input ordered pending, remaining, host) with the stable source
+order of the reference's scans, so history IDs, every Philox draw and
+therefore whole showers coincide. The device evaluates log/exp/pow
 with the GPU math library, whose results may differ from the host's in the
 last ulp; the bar is:
   * identical integer statistics (fronts, steps, interactions, secondaries,
@@ -26,7 +29,9 @@ pytestmark = pytest.mark.gpu
 
 STAT_KEYS = ("photon_fronts", "lepton_fronts", "photon_steps", "lepton_steps", "photon_interactions",
              "lepton_interactions", "secondaries", "observations", "fallbacks", "pending_photons",
-             "pending_leptons", "next_history_id", "max_front")
+             "pending_leptons", "next_history_id", "max_front", "photon_calls", "lepton_calls",
+             "bucketed_fronts", "bucketed_particles", "below_minimum_batch_checkpoints",
+             "history_range_checkpoints", "reserved_history_ids")
 
 
 @pytest.fixture(scope="module")
@@ -228,13 +233,43 @@ def test_backend_contract(tables):
     with pytest.raises(c8mi.C8miError) as err:
         dev.enqueue(em.particle(2212, 1.0, (0, 0, 1e4), (0, 0, -1)))
     assert err.value.status == 1
-    r = dev.run_photon_wavefront(em.particle(22, 1.0, (0, 0, 1e4), (0, 0, -1), history_id=1), 1)
-    assert r["photon_fronts"] == 1 and r["lepton_fronts"] == 0
+    photon = em.particle(22, 1.0, (0, 0, 1e4), (0, 0, -1), history_id=1)
+    # the reference's argument checks (PhotonCascade.inl:211-214, LeptonCascade.inl:127-132)
+    for first, fronts, batch in ((0, 16, 1), (2, 0, 1), (2, 16, 0)):
+        with pytest.raises(c8mi.C8miError) as err:
+            dev.run_photon_wavefront(photon, first, fronts, batch)
+        assert err.value.status == 1
+    with pytest.raises(c8mi.C8miError) as err:
+        dev.run_lepton_wavefront(em.particle(11, 1.0, (0, 0, 1e4), (0, 0, -1)), 10, 10)
+    assert err.value.status == 1
+    with pytest.raises(c8mi.C8miError) as err:  # species checked at the boundary
+        dev.run_photon_wavefront(em.particle(11, 1.0, (0, 0, 1e4), (0, 0, -1)), 2)
+    assert err.value.status == 1
+    # an empty call completes without checking its limits (PhotonCascade.inl:206-209)
+    empty = dev.run_lepton_wavefront(np.zeros(0, dtype=c8mi.PARTICLE_STATE_DTYPE), 0, 0, 0, 0)
+    assert empty["completed"] == 1 and empty["wavefronts"] == 0
+    r = dev.run_photon_wavefront(photon, 100, 1)
+    assert r["wavefronts"] == 1 and r["input_particles"] == 1 and r["transport_records"] == 1
+    assert r["next_secondary_history_id"] == 100 + r["secondaries"]
     photons, leptons = dev.pending()
-    assert (photons, leptons) == (r["pending_photons"], r["pending_leptons"])
-    r2 = dev.run_lepton_wavefront(np.zeros(0, dtype=c8mi.PARTICLE_STATE_DTYPE), 4)
-    assert r2["photon_fronts"] == 0
+    # the photon left after the one front came back to the host
+    assert photons == 0 and r["remaining"] == len(dev.remaining()) == 1 - r["interactions"]
+    assert leptons == r["other_species_pending"]
+    assert r["completed"] == (r["remaining"] == 0)
+    lim = dev.limits()
+    assert lim["minimum_batch_size"] == 16384 and lim["maximum_resident_input_batch"] == cfg.struct.queue_capacity // 2
+    st = dev.statistics()
+    assert st["photon_wavefronts"] == 1 and st["particles_advanced"] == 1 and st["shower_ordinal"] == 1
+    # the wavefront calls act on one shower: refused while other pool slots are active
     dev.close()
+    pool_cfg = em.make_config(airtables, seed=3, atmosphere=atm, slots=2)
+    pool = em.EmBackend(pool_cfg, tables, atm)
+    pool.slot_begin(0, 1)
+    pool.slot_begin(1, 2)
+    with pytest.raises(c8mi.C8miError) as err:
+        pool.run_photon_wavefront(photon, 2)
+    assert err.value.status == 2
+    pool.close()
 
 
 # ---- shower pool: several showers in the same fronts ----

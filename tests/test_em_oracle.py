@@ -220,7 +220,10 @@ def test_oracle_shower_closes_energy(tables, atmosphere, pid, energy, seed, thin
     assert led["invalid_records"] == 0 and led["fixed_point_overflows"] == 0
     st = r["stats"]
     assert st["pending_photons"] == 0 and st["pending_leptons"] == 0
-    assert st["next_history_id"] == 2 + st["secondaries"]
+    # every call reserved its range from the shower's counter, which starts at 2
+    assert st["next_history_id"] == 2 + st["reserved_history_ids"]
+    assert st["secondaries"] <= st["reserved_history_ids"]
+    assert st["photon_calls"] + st["lepton_calls"] >= 1
     if thinning:
         assert led["thinning_hillas_vertices"] + led["thinning_statistical_vertices"] > 0
     # the shower develops: most energy is deposited in the air
@@ -529,21 +532,60 @@ def test_moliere_cache_file_round_trip(tmp_path, moliere, moliere_table):
 
 
 def test_oracle_ice_shower_closes_and_peaks_where_expected():
-    """The in-ice scene with the ice tables (H2O, 100 keV cut): a 10 GeV photon
-    shower closes to 1e-9 E0 and its charged-particle profile peaks near
-    X0 (ln(E / E_c) - 1/2) for a photon, X0 = 36.08 g/cm^2, E_c ~ 78 MeV
-    (Rossi approximation B; within two radiation lengths)."""
+    """The in-ice scene with the ice tables (H2O, 100 keV cut): 10 GeV photon
+    showers close to 1e-9 E0 and their charged-particle profiles peak, on
+    average over 12 showers, near X0 (ln(E / E_c) - 1/2) for a photon,
+    X0 = 36.08 g/cm^2, E_c ~ 78 MeV (Rossi approximation B; within one
+    radiation length, about 3 standard errors of the mean of single-shower
+    maxima that scatter by ~1.4 X0)."""
     ice = em.IceVolume()
     start = (0.0, 0.0, -100.0)
     tables = airtables.build_ice_tables(cut_mev=0.1)
-    cfg = em.make_config(airtables.IceMedium, seed=9, cut_mev=0.1, start=start, atmosphere=ice, bins=256,
-                         bin_width=5.0)
-    prim = em.particle(22, 10.0, start, (0.0, 0.0, -1.0), history_id=1)
-    r = oracle_lib.em_run_shower(cfg, tables, ice, prim)
-    led = em.decode_counters(r["counters"], cfg.struct.profile_energy_scale)
-    assert abs(em.ledger_residual(led)) <= 1e-9 * 10.0
-    ws = cfg.struct.profile_weight_scale
-    charged = (r["hist"]["electrons"] + r["hist"]["positrons"]) / ws
-    xmax = (np.argmax(charged) + 0.5) * 5.0
+    maxima = []
+    for seed in range(1, 13):
+        cfg = em.make_config(airtables.IceMedium, seed=seed, cut_mev=0.1, start=start, atmosphere=ice,
+                             bins=256, bin_width=5.0)
+        prim = em.particle(22, 10.0, start, (0.0, 0.0, -1.0), history_id=1)
+        r = oracle_lib.em_run_shower(cfg, tables, ice, prim)
+        led = em.decode_counters(r["counters"], cfg.struct.profile_energy_scale)
+        assert abs(em.ledger_residual(led)) <= 1e-9 * 10.0
+        ws = cfg.struct.profile_weight_scale
+        charged = (r["hist"]["electrons"] + r["hist"]["positrons"]) / ws
+        maxima.append((np.argmax(charged) + 0.5) * 5.0)
     expected = 36.08 * (np.log(10.0e3 / 78.0) - 0.5)
-    assert abs(xmax - expected) < 2 * 36.08, (xmax, expected)
+    assert abs(np.mean(maxima) - expected) < 36.08, (maxima, expected)
+
+
+def test_bucket_key_layout_pinned_by_hand_values():
+    """wavefront_bucket_key (WavefrontBucketing.inl:24-79): PID bucket in bits
+    48-49 (photon 0, e- 1, e+ 2, anything else 3), medium ^ 2^31 in bits 16-47
+    (so signed medium order), energy bin floor(16 log2(E/GeV)) + 32768 in bits
+    0-15, 0 below the range, 65535 for non-positive or non-finite energies. The
+    expected keys are written out from those rules for inputs whose logarithm
+    is exact or far from a bin edge."""
+    def key(pid, medium, energy):
+        p = em.particle(pid, energy, (0.0, 0.0, 0.0), (0.0, 0.0, -1.0))
+        p["medium_id"] = medium
+        return oracle_lib.em_bucket_key(p)
+
+    assert key(22, 0, 1.0) == (0x80000000 << 16) | 32768
+    assert key(11, 0, 2.0) == (1 << 48) | (0x80000000 << 16) | (32768 + 16)
+    assert key(-11, 0, 0.5) == (2 << 48) | (0x80000000 << 16) | (32768 - 16)
+    assert key(13, 0, 1.0) == (3 << 48) | (0x80000000 << 16) | 32768
+    assert key(22, -1, 1.0) == (0x7FFFFFFF << 16) | 32768
+    assert key(22, 7, 1.0) == (0x80000007 << 16) | 32768
+    # 1 MeV: log2(1e-3) = -9.9658 -> 16 * -9.9658 = -159.45 -> bin 32608
+    assert key(22, 0, 1.0e-3) & 0xFFFF == 32608
+    # 1 PeV: log2(1e6) = 19.9316 -> 318.9 -> bin 33086
+    assert key(22, 0, 1.0e6) & 0xFFFF == 33086
+    # one bin is a factor 2^(1/16): just below and above 2^(5/16)
+    edge = 2.0 ** (5.0 / 16.0)
+    assert key(22, 0, edge * (1 - 1e-12)) & 0xFFFF == 32768 + 4
+    assert key(22, 0, edge * (1 + 1e-12)) & 0xFFFF == 32768 + 5
+    # the smallest double, 2^-1074: 32768 - 16 * 1074 (no finite double reaches either clamp)
+    assert key(22, 0, 5e-324) & 0xFFFF == 32768 - 16 * 1074
+    for bad in (0.0, -1.0, float("nan"), float("inf")):
+        assert key(22, 0, bad) & 0xFFFF == 65535
+    # the key orders by PID first, then signed medium, then energy
+    keys = [key(22, -5, 50.0), key(22, 3, 0.01), key(11, -100, 1e-3), key(11, -100, 1e3), key(-11, 0, 1.0)]
+    assert keys == sorted(keys)

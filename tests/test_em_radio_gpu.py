@@ -339,12 +339,12 @@ def test_refused_front_leaves_the_shower_unchanged(tables):
     dev.attach_radio(radio)
     dev.begin_shower(1, 2)
     dev.enqueue(prim)
-    first = dev.run(maximum_fronts=3)
+    first = dev.run(3)
     assert first["pending_leptons"] > 0
     radio.download_fixed()  # commits the accumulator
     before = (dev.pending(), dev.slot_state(0), dev.download_profile()[1])
     with pytest.raises(c8mi.C8miError) as err:
-        dev.run_lepton_wavefront([], 1)  # the lepton front alone: refused first thing
+        dev.run_lepton_wavefront([], 1 << 40, (1 << 40) + (1 << 30))  # refused before its first front
     assert err.value.status == 2
     after = (dev.pending(), dev.slot_state(0), dev.download_profile()[1])
     assert after == before
