@@ -59,6 +59,11 @@ def parse():
     p.add_argument("--em-plain-showers", type=int, default=20,
                    help="timed 100 GeV showers of the one-at-a-time EM form")
     p.add_argument("--em-slots", type=int, default=512, help="concurrent showers of the EM pool")
+    p.add_argument("--em-moliere-fast-iterations", type=int, default=0,
+                   help="Newton iterations of a Moliere angle in the step kernel before the sample moves to "
+                        "the tail kernel (0: the library default; results do not depend on it)")
+    p.add_argument("--em-pools", type=int, default=1,
+                   help="backends the EM pool's slots are split over (own stream and host thread each)")
     p.add_argument("--em-warmup-showers", type=int, default=64, help="untimed showers before the EM pool run")
     p.add_argument("--em-no-moliere", action="store_true",
                    help="EM without Moliere multiple scattering (on by default, as in the reference's production)")
@@ -205,34 +210,59 @@ def em_showers(energy, count, first_shower, atm=None, pid=22):
 
 
 def em_section(showers, plain_showers, slots, cpu_seconds, no_cpu_baseline, energy=100.0, warmup_showers=64,
-               moliere=True, moliere_table=True):
+               moliere=True, moliere_table=True, pools=1, fast_iterations=0):
     """EM particle-steps/s on the plain 100 GeV vertical photon shower in the
     USStdBK atmosphere (1 MeV cut, no thinning, FP64), the whole cascade on the
     device from the primary on. Two forms: the pool (`showers` showers served
     through `slots` concurrent slots, continuous batching; the throughput
     figure) and one shower at a time (`plain_showers` showers, per-shower
-    latency)."""
+    latency). With pools > 1 the slots are split over that many backends, each
+    on its own stream and served from its own host thread, so one pool's
+    kernels run while another's host turns its front report around."""
+    import threading
+
     from c8mi import airtables, em
 
     tables = airtables.build_air_tables()
     atm = em.Atmosphere()
-    capacity = 1 << 23
+    pool_slots = max(1, slots // pools)
+    capacity = 1 << 23 if pool_slots <= 1024 else 1 << 24 if pool_slots <= 2048 else 1 << 25
     table = True if moliere_table else None
-    cfg = em.make_config(airtables, atmosphere=atm, slots=slots, queue_capacity=capacity,
+    cfg = em.make_config(airtables, atmosphere=atm, slots=pool_slots, queue_capacity=capacity,
                          observation_capacity=1 << 12, fallback_capacity=1 << 8, moliere=moliere,
-                         moliere_table=table)
-    dev = em.EmBackend(cfg, tables, atm)
+                         moliere_table=table, moliere_fast_iterations=fast_iterations)
+    devs = [em.EmBackend(cfg, tables, atm) for _ in range(pools)]
     budget = 0.4 * capacity
     zero = {"photon_steps": 0, "lepton_steps": 0, "lepton_interactions": 0}
     warm_totals = zero
     if warmup_showers:
-        _, warm_totals = dev.serve(em_showers(energy, warmup_showers, 1), budget, realtime=False)
-    dev.enable_kernel_timing(True)
+        for dev in devs:
+            _, warm_totals = dev.serve(em_showers(energy, warmup_showers, 1), budget, realtime=False)
+    for dev in devs:
+        dev.enable_kernel_timing(True)
+    requests = em_showers(energy, showers, 1000)
+    parts = [requests[k::pools] for k in range(pools)]
+    outcome = [None] * pools
+
+    def serve_part(k):
+        outcome[k] = devs[k].serve(parts[k], budget, realtime=False)
+
+    workers = [threading.Thread(target=serve_part, args=(k,)) for k in range(pools)]
     t0 = time.perf_counter()
-    results, totals = dev.serve(em_showers(energy, showers, 1000), budget, realtime=False)
+    for w in workers:
+        w.start()
+    for w in workers:
+        w.join()
     elapsed = time.perf_counter() - t0
-    timing = dev.kernel_time()
-    dev.close()
+    if any(o is None for o in outcome):
+        raise RuntimeError("an EM pool failed")
+    timings = [dev.kernel_time() for dev in devs]
+    timing = {k: (sum(t[k][0] for t in timings), sum(t[k][1] for t in timings)) for k in timings[0]}
+    for dev in devs:
+        dev.close()
+    results = [r for o in outcome for r in o[0]]
+    totals = {k: sum(o[1][k] for o in outcome) if k != "max_front" else max(o[1][k] for o in outcome)
+              for k in outcome[0][1]}
     steps = totals["photon_steps"] + totals["lepton_steps"]
     # Closure holds to 1e-9 E0 except in showers with Molière-failed steps: that
     # transport fallback hands over the vertex state, so the failed step's
@@ -282,6 +312,7 @@ def em_section(showers, plain_showers, slots, cpu_seconds, no_cpu_baseline, ener
         "unit": "particle-steps/s",
         "showers": showers,
         "slots": slots,
+        "pools": pools,
         "particle_steps": steps,
         "ms_per_shower": elapsed * 1e3 / showers,
         "latency_ms": {"p50": float(np.percentile(pool_latency, 50) * 1e3),
@@ -300,7 +331,8 @@ def em_section(showers, plain_showers, slots, cpu_seconds, no_cpu_baseline, ener
                                "spherical air, observation level 0 m, 1 MeV cut, no thinning, no radio, "
                                f"Moliere scattering {('on (interpolation table)' if moliere_table else 'on (analytical functions)') if moliere else 'off'}, "
                                "closed-form v7 air tables, whole cascade on the device; "
-                               f"{showers} showers through {slots} pool slots"},
+                               f"{showers} showers through {slots} pool slots"
+                               + (f" in {pools} pools on their own streams" if pools > 1 else "")},
         "roofline": {
             "bound": "hbm",
             "achieved": achieved,
@@ -800,7 +832,8 @@ def main():
         em_result = em_section(max(args.em_showers, 1), args.em_plain_showers, args.em_slots, args.cpu_seconds,
                                args.no_cpu_baseline, warmup_showers=args.em_warmup_showers,
                                moliere=not args.em_no_moliere,
-                               moliere_table=not args.em_analytic_moliere)
+                               moliere_table=not args.em_analytic_moliere, pools=args.em_pools,
+                               fast_iterations=args.em_moliere_fast_iterations)
         em_result.update({"n_gpus": 1, "steps": em_result["showers"], "higher_is_better": True,
                           "scaling": "weak", "vs_baseline": None, "data": "synthetic",
                           "ms_per_step": em_result["ms_per_shower"]})
@@ -900,8 +933,9 @@ def main():
     if args.em_showers > 0:
         result["em"] = em_section(args.em_showers, args.em_plain_showers, args.em_slots, args.cpu_seconds,
                                   args.no_cpu_baseline, warmup_showers=args.em_warmup_showers,
-                               moliere=not args.em_no_moliere,
-                               moliere_table=not args.em_analytic_moliere)
+                                  moliere=not args.em_no_moliere,
+                                  moliere_table=not args.em_analytic_moliere, pools=args.em_pools,
+                               fast_iterations=args.em_moliere_fast_iterations)
     if args.serve_showers > 0:
         result["serving"] = serving_section(args.serve_showers, args.serve_rate, args.serve_slots,
                                             args.serve_plain_showers, args.cpu_seconds, args.no_cpu_baseline,

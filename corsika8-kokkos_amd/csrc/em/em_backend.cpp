@@ -79,7 +79,7 @@ void validate_config(c8mi_em_config const& c) {
   require(c.axis_grammage_g_per_cm2 != nullptr && c.axis_support_count > 0, "profile projection table");
   require(std::isfinite(c.axis_step_m) && c.axis_step_m > 0., "projection step");
   require(c.profile_crossing_mode <= C8MI_PROFILE_CROSSING_FORWARD, "profile crossing mode");
-  require(c.shower_slots <= kMaxShowerSlots, "shower slots above the supported maximum (1024)");
+  require(c.shower_slots <= kMaxShowerSlots, "shower slots above the supported maximum (4096)");
   require(c.moliere_enabled <= 1, "moliere_enabled must be 0 or 1");
   require(c.moliere_fast_iterations <= 100, "moliere_fast_iterations above the Newton limit (100)");
   if (c.moliere_enabled) {
@@ -615,34 +615,39 @@ void EmBackend::reserve_call(CallState& c, int species, std::uint64_t n, c8mi_em
 }
 
 // The loop head of run_photon_cascade / run_lepton_cascade for one shower
-// (PhotonCascade.inl:286-300, LeptonCascade.inl:240-252): a call runs while its
-// queue holds particles and fewer than `cap` fronts ran; after its first front
-// it ends below the minimum batch; a lepton call ends when the front could
-// exhaust its history range. An ended call passes to the other species.
-bool EmBackend::schedule(CallState& c, int species, std::uint64_t n, c8mi_em_run_result& out) const {
-  if (c.phase != species) return false;
-  if (!c.open) {
-    if (n == 0) {
-      c.phase ^= 1;
-      return false;
+// (PhotonCascade.inl:286-300, LeptonCascade.inl:240-252), applied until the
+// shower's next front is known: a call runs while its queue holds particles and
+// fewer than `cap` fronts ran; after its first front it ends below the minimum
+// batch; a lepton call ends when the front could exhaust its history range. An
+// ended call passes to the other species, where the device-resident schedule
+// opens the next call (reserving its range) if that species has particles.
+// n[s]: the shower's particles of each species. Returns true when the shower
+// runs a front of species c.phase now.
+bool EmBackend::settle(CallState& c, std::uint64_t const n[2], c8mi_em_run_result& out) const {
+  for (int step = 0; step < 4; ++step) {
+    int const s = c.phase;
+    if (!c.open) {
+      if (!c.resident) return false;  // a wavefront call ended: the caller decides
+      if (n[s] == 0) {
+        if (n[1 - s] == 0) return false;
+        c.phase ^= 1;
+        continue;
+      }
+      reserve_call(c, s, n[s], out);
     }
-    reserve_call(c, species, n, out);
-  }
-  int stop = -1;
-  if (n == 0) stop = kStopEmpty;
-  else if (c.call_fronts >= c.cap) stop = kStopCap;
-  else if (c.call_fronts != 0 && n < c.min_batch) stop = kStopMinimumBatch;
-  else if (species == 1 && n > (c.limit - c.next) / 3) stop = kStopHistory;
-  if (stop >= 0) {
+    int stop = -1;
+    if (n[s] == 0) stop = kStopEmpty;
+    else if (c.call_fronts >= c.cap) stop = kStopCap;
+    else if (c.call_fronts != 0 && n[s] < c.min_batch) stop = kStopMinimumBatch;
+    else if (s == 1 && n[s] > (c.limit - c.next) / 3) stop = kStopHistory;
+    if (stop < 0) return true;
     if (stop == kStopMinimumBatch) ++out.below_minimum_batch_checkpoints;
     if (stop == kStopHistory) ++out.history_range_checkpoints;
     c.open = c.opening = false;
     c.stop = stop;
     c.phase ^= 1;
-    return false;
   }
-  ++c.call_fronts;
-  return true;
+  return false;
 }
 
 bool EmBackend::front(bool photon, c8mi_em_run_result& out) {
@@ -665,8 +670,9 @@ bool EmBackend::front(bool photon, c8mi_em_run_result& out) {
     SlotHost const& h = slot_host_[k];
     plan_[k] = h.call;
     if (!h.active) continue;
+    std::uint64_t const counts[2] = {h.pending[0] + h.staged[0], h.pending[1] + h.staged[1]};
+    if (!settle(plan_[k], counts, decided) || plan_[k].phase != s) continue;
     std::uint64_t const cnt = h.pending[s];
-    if (!schedule(plan_[k], s, cnt, decided)) continue;
     c.flags = kControlRun;
     if (plan_[k].opening) {
       c.flags |= kControlOpen;
@@ -684,7 +690,10 @@ bool EmBackend::front(bool photon, c8mi_em_run_result& out) {
     for (std::uint32_t k = 0; k < active_range_; ++k) {
       if (!slot_host_[k].active) continue;
       slot_host_[k].call = plan_[k];
-      if (control_host_.data()[k].flags & kControlRun) slot_host_[k].call.opening = false;
+      if (control_host_.data()[k].flags & kControlRun) {
+        slot_host_[k].call.opening = false;
+        ++slot_host_[k].call.call_fronts;
+      }
     }
     out.photon_calls += decided.photon_calls;
     out.lepton_calls += decided.lepton_calls;
@@ -922,11 +931,18 @@ void EmBackend::run(std::uint64_t maximum_rounds, c8mi_em_run_result& out) {
 // Rounds of the schedule: every shower's photon front (if its call runs one
 // now), then its lepton front. No record or history read-back.
 void EmBackend::run_rounds_loop(std::uint64_t maximum_rounds, c8mi_em_run_result& out) {
-  auto has = [&](int s) { return count_[s] + staged_[s].size() > 0; };
+  // Showers are independent: any interleaving that keeps each shower's own
+  // front sequence gives the same results. Every round offers each shower its
+  // next front (a photon front, then a lepton front), so every shower advances
+  // every round; letting one species wait for a larger front instead measured
+  // 1.9x more fronts on a pool of 100 GeV showers.
   for (std::uint64_t round = 0; round < maximum_rounds; ++round) {
-    if (!has(0) && !has(1)) break;
-    front(true, out);
-    front(false, out);
+    bool const photons = front(true, out);
+    bool const leptons = front(false, out);
+    if (photons || leptons) continue;
+    if (count_[0] + staged_[0].size() + count_[1] + staged_[1].size() == 0) break;
+    // a shower with particles always has a next front once its calls settle
+    throw std::logic_error("em: particles queued but no shower's schedule runs a front");
   }
 }
 
@@ -964,6 +980,7 @@ void EmBackend::run_call(bool photon, c8mi_particle_state const* particles, std:
   CallState const saved = c;
   c.phase = s;
   c.open = c.opening = true;
+  c.resident = false;
   c.stop = kStopEmpty;
   c.call_fronts = 0;
   c.next = first_secondary_history_id;
@@ -975,8 +992,10 @@ void EmBackend::run_call(bool photon, c8mi_particle_state const* particles, std:
     while (c.open && c.phase == s) front(photon, acc);
   } catch (...) {
     if (c.open && c.call_fronts == 0) c = saved;  // refused before its first front
+    c.resident = true;
     throw;
   }
+  c.resident = true;
   out.wavefronts = c.call_fronts;
   out.transport_records = photon ? acc.photon_steps : acc.lepton_steps;
   out.interactions = photon ? acc.photon_interactions : acc.lepton_interactions;

@@ -132,6 +132,7 @@ class EmBackend {
     std::uint64_t limit = 0;        // exclusive limit of the open call
     std::uint64_t cap = 0;          // fronts per call
     std::uint64_t min_batch = 0;    // below-minimum-batch checkpoint
+    bool resident = true;           // calls open by themselves (false during a wavefront call)
   };
   struct SlotHost {
     bool active = false;
@@ -156,7 +157,7 @@ class EmBackend {
   bool front(bool photon, c8mi_em_run_result& out);
   // Whether slot `h`'s call of `species` runs a front over its n particles
   // now (opening or ending calls as the schedule says).
-  bool schedule(CallState& c, int species, std::uint64_t n, c8mi_em_run_result& out) const;
+  bool settle(CallState& c, std::uint64_t const n[2], c8mi_em_run_result& out) const;
   void reserve_call(CallState& c, int species, std::uint64_t n, c8mi_em_run_result& out) const;
   void run_rounds_loop(std::uint64_t maximum_rounds, c8mi_em_run_result& out);
   void flush_staged(int species);

@@ -188,7 +188,7 @@ enum SlotKind : std::uint32_t { kSlotNone = 0, kSlotNextContinue = 1, kSlotNextC
 // Packed per-source counts: bits 0-1 next, 2-3 other, 4-5 children, 6-7 slot0
 // kind, 8-9 slot1 kind, 10 a transport step was made, 11 an interaction
 // vertex was reached, 12-31 pool slot of the source's shower.
-inline constexpr std::uint32_t kMaxShowerSlots = 1024;
+inline constexpr std::uint32_t kMaxShowerSlots = 4096;
 inline constexpr int kPackedSlotShift = 12;
 __host__ __device__ __forceinline__ std::uint32_t pack_counts(std::uint32_t next, std::uint32_t other,
                                                               std::uint32_t child, std::uint32_t k0,

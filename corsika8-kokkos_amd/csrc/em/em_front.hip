@@ -513,67 +513,49 @@ __global__ __launch_bounds__(kScanBlock) void scan_kernel(FrontArgs f, std::uint
     __syncthreads();
   }
 
-  // ---- per slot: offsets that turn a global output rank into the rank inside
-  // the shower, new queue counts, history counters and statistics. One
-  // exclusive scan over the slots per (run, stream). ----
+  // ---- per slot (a front is one run sorted by slot, so a global output rank
+  // is already the output's queue position): the offset that turns a child's
+  // global rank into its history ID inside its shower, new queue counts,
+  // history counters and statistics. One exclusive scan of the children
+  // counts over the slots, a workgroup's worth of slots at a time. ----
   __shared__ unsigned long long slot_sums[2][kScanBlock / 64];
-  std::uint32_t const s = threadIdx.x;
-  bool const mine = s < f.slots;
-  unsigned long long* row = f.shower_totals + static_cast<std::uint64_t>(s) * kMaxRuns * kShowerFields;
-  // partial[r][k] = this slot's outputs in earlier runs - the rank offset of
-  // the slot inside run r - the outputs of all slots in earlier runs
-  unsigned long long partial[kMaxRuns][3];
-  unsigned long long base[3] = {0, 0, 0}, within[3] = {0, 0, 0}, run_start[3] = {0, 0, 0};
+  unsigned long long children_before = 0;  // children of the slots of earlier chunks
   int parity = 0;
-#pragma unroll
-  for (int r = 0; r < kMaxRuns; ++r) {
-#pragma unroll
-    for (int k = 0; k < 3; ++k) {
-      partial[r][k] = 0;
-      if (r >= static_cast<int>(f.runs)) continue;  // uniform
-      unsigned long long const x = mine ? row[r * kShowerFields + k] : 0ULL;
-      unsigned long long const inc = wave_inclusive(x, lane);
-      if (lane == 63) slot_sums[parity][wave] = inc;
-      __syncthreads();
-      unsigned long long before = 0, all = 0;
-      for (std::uint32_t w = 0; w < kScanBlock / 64; ++w) {
-        unsigned long long const y = slot_sums[parity][w];
-        before += w < wave ? y : 0ULL;
-        all += y;
-      }
-      parity ^= 1;  // the next scan writes the other buffer; the one after is past a barrier
-      unsigned long long const excl = before + inc - x;
-      partial[r][k] = within[k] - excl - run_start[k];
-      base[k] += excl;
-      within[k] += x;
-      run_start[k] += all;
+  for (std::uint32_t chunk = 0; chunk < f.slots; chunk += kScanBlock) {
+    std::uint32_t const s = chunk + threadIdx.x;
+    bool const mine = s < f.slots;
+    unsigned long long* row = f.shower_totals + static_cast<std::uint64_t>(mine ? s : 0) * kMaxRuns * kShowerFields;
+    unsigned long long const x = mine ? row[2] : 0ULL;
+    unsigned long long const inc = wave_inclusive(x, lane);
+    if (lane == 63) slot_sums[parity][wave] = inc;
+    __syncthreads();
+    unsigned long long before = 0, all = 0;
+    for (std::uint32_t w = 0; w < kScanBlock / 64; ++w) {
+      unsigned long long const y = slot_sums[parity][w];
+      before += w < wave ? y : 0ULL;
+      all += y;
     }
-  }
-  if (mine) {
-    unsigned long long const history = f.history[s];
-    base[2] = history;
+    parity ^= 1;  // the next chunk writes the other buffer; the one after is past a barrier
+    if (mine) {
+      unsigned long long const excl = children_before + before + inc - x;
+      unsigned long long const history = f.history[s];
+      unsigned long long* adj = f.adjust + static_cast<std::uint64_t>(s) * kMaxRuns * 3;
+      adj[0] = 0;
+      adj[1] = 0;
+      adj[2] = history - excl;
+      unsigned long long const next = row[0], other = row[1], steps = row[3], interactions = row[4];
 #pragma unroll
-    for (int r = 0; r < kMaxRuns; ++r)
-#pragma unroll
-      for (int k = 0; k < 3; ++k)
-        if (r < static_cast<int>(f.runs))
-          f.adjust[(static_cast<std::uint64_t>(s) * kMaxRuns + r) * 3 + k] = base[k] + partial[r][k];
-    unsigned long long steps = 0, interactions = 0;
-#pragma unroll
-    for (int r = 0; r < kMaxRuns; ++r) {
-      steps += row[r * kShowerFields + 3];
-      interactions += row[r * kShowerFields + 4];
-#pragma unroll
-      for (int k = 0; k < kShowerFields; ++k) row[r * kShowerFields + k] = 0;  // ready for the next front
+      for (int k = 0; k < kShowerFields; ++k) row[k] = 0;  // ready for the next front
+      f.history[s] = history + x;
+      unsigned long long* st = f.stats + static_cast<std::uint64_t>(s) * kSlotStats;
+      st[photon ? 0 : 1] += steps;
+      st[photon ? 2 : 3] += interactions;
+      st[4] += x;
+      f.report[kReportHeader + kReportPerSlot * static_cast<std::uint64_t>(s)] = next;
+      f.report[kReportHeader + kReportPerSlot * static_cast<std::uint64_t>(s) + 1] = other;
+      f.report[kReportHeader + kReportPerSlot * static_cast<std::uint64_t>(s) + 2] = x;
     }
-    f.history[s] = history + within[2];
-    unsigned long long* st = f.stats + static_cast<std::uint64_t>(s) * kSlotStats;
-    st[photon ? 0 : 1] += steps;
-    st[photon ? 2 : 3] += interactions;
-    st[4] += within[2];
-    f.report[kReportHeader + kReportPerSlot * static_cast<std::uint64_t>(s)] = within[0];
-    f.report[kReportHeader + kReportPerSlot * static_cast<std::uint64_t>(s) + 1] = within[1];
-    f.report[kReportHeader + kReportPerSlot * static_cast<std::uint64_t>(s) + 2] = within[2];
+    children_before += all;
   }
   if (threadIdx.x == 0) {
     FrontTotals out{};
@@ -837,8 +819,8 @@ void launch_count(FrontArgs const& f, hipStream_t stream) {
 }
 
 void launch_scan(bool photon, FrontArgs const& f, hipStream_t stream) {
-  if (f.slots > static_cast<std::uint32_t>(kScanBlock) || f.slots > kMaxShowerSlots)
-    throw std::length_error("em: more pool slots than one scan workgroup covers");
+  if (f.slots > kMaxShowerSlots) throw std::length_error("em: more pool slots than the scan covers");
+  if (f.runs != 1) throw std::logic_error("em: a front's input must be one run sorted by slot");
   std::uint32_t const blocks = f.count == 0 ? 0u : grid_for(f.count, kStepBlock);
   scan_kernel<<<1, kScanBlock, 0, stream>>>(f, blocks, photon ? 1 : 0);
   C8MI_HIP(hipGetLastError());

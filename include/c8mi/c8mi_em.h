@@ -292,7 +292,7 @@ typedef struct c8mi_em_config {
   double axis_step_m;
   const double* axis_grammage_g_per_cm2;
   uint64_t axis_support_count;
-  /* Concurrent showers (pool slots, 1..1024; 0 means 1). Each slot has its
+  /* Concurrent showers (pool slots, 1..4096; 0 means 1). Each slot has its
    * own Philox shower key, history counter, projection axis, profile, ledger,
    * observation_capacity observations and fallback_capacity fallbacks; the
    * particle queues (queue_capacity per species and buffer) are shared. */

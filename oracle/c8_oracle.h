@@ -168,6 +168,58 @@ int oracle_em_run_shower_tracks(const c8mi_em_config* config, const c8mi_native_
                                 uint64_t* track_count);
 void oracle_free(void* p);
 
+/* Vertex log of a shower (test probe): one record per lepton vertex attempt
+ * (kind 0: the reselection of LeptonVertexSelection.inl:41-199 at the
+ * post-loss energy, accepted when process_uniform * rate_start < rate_vertex)
+ * and per materialised final state (kind 1), with the kinematic energies and
+ * directions of its children before thinning. */
+typedef struct or_vertex {
+  int32_t kind;
+  int32_t pid;          /* parent */
+  int32_t process;      /* process id (kind 1) */
+  uint32_t accepted;    /* kind 0 */
+  uint32_t count;       /* children logged (kind 1) */
+  uint32_t pad;
+  double energy;        /* parent total energy at the vertex, GeV */
+  double mass;          /* projectile mass the kinematics used, GeV */
+  double target_mass;   /* target electron mass the kinematics used, GeV (0: none) */
+  double direction[3];
+  double v;             /* energy fraction */
+  double rate_start;    /* kind 0: total rate at the step start */
+  double rate_vertex;   /* kind 0: total rate at the vertex */
+  double process_uniform;
+  int32_t child_pid[3];
+  int32_t pad2;
+  double child_energy[3];        /* total energies, GeV */
+  double child_direction[3][3];
+} or_vertex;
+
+/* oracle_em_run_shower with the vertex log, released with oracle_free. */
+int oracle_em_run_shower_vertices(const c8mi_em_config* config, const c8mi_native_table_set* tables,
+                                  const c8mi_environment* environment, uint64_t shower_id,
+                                  uint64_t first_history_id, const c8mi_particle_state* primaries,
+                                  uint64_t primary_count, or_vertex** vertices, uint64_t* vertex_count,
+                                  or_em_stats* stats);
+
+/* select_discrete_interaction of one particle (InteractionSelection.inl:12-215):
+ * status, selected process and component (photons), sampled interaction
+ * grammage, total rate and loss fraction. */
+typedef struct or_selection {
+  int32_t status;
+  int32_t process;
+  uint64_t component;
+  double interaction_grammage;
+  double total_rate;
+  double energy_fraction;
+  double loss_quantile;
+  double process_uniform;
+  int32_t fallback;     /* 1 when the selection asked for a fallback */
+  int32_t fallback_reason;
+} or_selection;
+int oracle_em_select(const c8mi_em_config* config, const c8mi_native_table_set* tables,
+                     const c8mi_environment* environment, uint64_t shower_id,
+                     const c8mi_particle_state* particle, or_selection* out);
+
 /* Test hook: the lepton history reservation max(3 n, min(chunk, multiplier *
  * 3 n)) of oracle_em_run_shower (0, 0: the reference's 64 and 2^20). Process
  * wide; not thread safe. */

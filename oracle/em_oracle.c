@@ -1873,7 +1873,40 @@ typedef struct {
   double* tracks;
   uint64_t track_count, track_capacity;
   int record_tracks, track_overflow;
+  /* optional vertex log (test probe) */
+  or_vertex* vertices;
+  uint64_t vertex_count, vertex_capacity;
+  int record_vertices, vertex_overflow;
 } shower_t;
+
+static or_vertex* log_vertex(shower_t* s, int kind, const ps_t* parent) {
+  if (!s->record_vertices) return NULL;
+  if (s->vertex_count == s->vertex_capacity) {
+    uint64_t cap = s->vertex_capacity ? 2 * s->vertex_capacity : 1024;
+    or_vertex* grown = realloc(s->vertices, cap * sizeof *grown);
+    if (!grown) {
+      s->vertex_overflow = 1;
+      return NULL;
+    }
+    s->vertices = grown;
+    s->vertex_capacity = cap;
+  }
+  or_vertex* v = &s->vertices[s->vertex_count++];
+  memset(v, 0, sizeof *v);
+  v->kind = kind;
+  v->pid = parent->pid;
+  v->energy = parent->energy_gev;
+  for (int a = 0; a < 3; ++a) v->direction[a] = parent->direction[a];
+  return v;
+}
+
+static void log_child(or_vertex* v, int32_t pid, double energy, const double direction[3]) {
+  if (!v || v->count >= 3) return;
+  v->child_pid[v->count] = pid;
+  v->child_energy[v->count] = energy;
+  for (int a = 0; a < 3; ++a) v->child_direction[v->count][a] = direction[a];
+  v->count++;
+}
 
 /* Compact radio track of one e+- step (the make_radio_track_kinematics input,
  * RadioProjection.inl:145-168): start and end positions and times, weight
@@ -2075,6 +2108,16 @@ static void photon_source(shower_t* s, const ps_t* p, pout_t* o, uint64_t child_
     el.reserved = 0;
     el.weight = th.w2;
     deflect(parent->direction, ec, fmod(az + PHOTON_PI, PHOTON_TWO_PI), el.direction, 0);
+    {
+      or_vertex* lv = log_vertex(s, 1, parent);
+      if (lv) {
+        lv->process = vi->process_id;
+        lv->target_mass = ELECTRON_MASS_GEV;
+        lv->v = v;
+        log_child(lv, 22, e1, ph.direction);
+        log_child(lv, 11, ELECTRON_MASS_GEV + parent->energy_gev * v, el.direction);
+      }
+    }
     uint32_t k = 0;
     if (th.keep & 1U) {
       ph.history_id = first_history + child_offset + k++;
@@ -2193,6 +2236,15 @@ static void photon_source(shower_t* s, const ps_t* p, pout_t* o, uint64_t child_
   po.reserved = 0;
   po.weight = th.w2;
   deflect(parent->direction, pcos, fmod(az + PHOTON_PI, PHOTON_TWO_PI), po.direction, 0);
+  {
+    or_vertex* lv = log_vertex(s, 1, parent);
+    if (lv) {
+      lv->process = vi->process_id;
+      lv->v = rho;
+      log_child(lv, 11, ee, el.direction);
+      log_child(lv, -11, pe, po.direction);
+    }
+  }
   uint32_t k = 0;
   if (th.keep & 1U) {
     el.history_id = first_history + child_offset + k++;
@@ -3017,6 +3069,15 @@ static void lepton_source(shower_t* s, const ps_t* p, lout_t* o, uint64_t child_
     record_fallback(s, &f, STAGE_VERTEX);
     return;
   }
+  {
+    or_vertex* lv = log_vertex(s, 0, vp);
+    if (lv) {
+      lv->accepted = vt.value > 0. && outer < vt.value;
+      lv->rate_start = r.total_rate;
+      lv->rate_vertex = vt.value;
+      lv->process_uniform = r.process_uniform;
+    }
+  }
   if (!(vt.value > 0.) || outer >= vt.value) {
     if (vp->step_id == 0xffffffffffffffffULL) {
       set_fallback(&f, &r, FR_INVALID_FINAL_STATE);
@@ -3121,6 +3182,17 @@ static void lepton_source(shower_t* s, const ps_t* p, lout_t* o, uint64_t child_
     g2.weight = th.w2;
     deflect(parent->direction, c1, az, g1.direction, 1);
     deflect(parent->direction, c2, fmod(az + LEPTON_PI, LEPTON_TWO_PI), g2.direction, 1);
+    {
+      or_vertex* lv = log_vertex(s, 1, parent);
+      if (lv) {
+        lv->process = r.process_id;
+        lv->mass = lm;
+        lv->target_mass = lm;
+        lv->v = v;
+        log_child(lv, 22, e1, g1.direction);
+        log_child(lv, 22, e2, g2.direction);
+      }
+    }
     uint32_t k = 0;
     if (th.keep & 1U) {
       g1.history_id = first_history + child_offset + k++;
@@ -3183,6 +3255,17 @@ static void lepton_source(shower_t* s, const ps_t* p, lout_t* o, uint64_t child_
     dl.weight = th.w2;
     deflect(parent->direction, oc, az, out.direction, 1);
     deflect(parent->direction, dc, fmod(az + LEPTON_PI, LEPTON_TWO_PI), dl.direction, 1);
+    {
+      or_vertex* lv = log_vertex(s, 1, parent);
+      if (lv) {
+        lv->process = r.process_id;
+        lv->mass = lm;
+        lv->target_mass = fs_me;
+        lv->v = r.energy_fraction;
+        log_child(lv, out.pid, oe, out.direction);
+        log_child(lv, 11, de, dl.direction);
+      }
+    }
     uint32_t k = 0;
     if (th.keep & 1U) {
       out.history_id = first_history + child_offset + k++;
@@ -3268,6 +3351,16 @@ static void lepton_source(shower_t* s, const ps_t* p, lout_t* o, uint64_t child_
   ph.reserved = 0;
   ph.weight = th.w2;
   for (int a = 0; a < 3; ++a) ph.direction[a] = pdir[a];
+  {
+    or_vertex* lv = log_vertex(s, 1, parent);
+    if (lv) {
+      lv->process = r.process_id;
+      lv->mass = lm;
+      lv->v = v;
+      log_child(lv, parent->pid, le, lep.direction);
+      log_child(lv, 22, pe, pdir);
+    }
+  }
   uint32_t k = 0;
   if (th.keep & 1U) {
     lep.history_id = first_history + child_offset + k++;
@@ -3479,10 +3572,11 @@ static int run_shower(const c8mi_em_config* config, const c8mi_native_table_set*
                       int64_t* electrons, int64_t* positrons, int64_t* energy_loss,
                       int64_t* counters, c8mi_particle_state* observations,
                       c8mi_em_fallback* fallbacks, or_em_stats* stats, double** tracks,
-                      uint64_t* track_count) {
+                      uint64_t* track_count, or_vertex** vertices, uint64_t* vertex_count) {
   shower_t s;
   memset(&s, 0, sizeof s);
   s.record_tracks = tracks != NULL;
+  s.record_vertices = vertices != NULL;
   s.x.t = tables;
   s.x.cont.t = tables;
   s.x.cont.em_cut_mev = config->em_transport_cut_mev;
@@ -3534,10 +3628,19 @@ static int run_shower(const c8mi_em_config* config, const c8mi_native_table_set*
   if (tracks) {
     if (s.track_overflow) {
       free(s.tracks);
+      free(s.vertices);
       return -1;
     }
     *tracks = s.tracks;
     *track_count = s.track_count;
+  }
+  if (vertices) {
+    if (s.vertex_overflow) {
+      free(s.vertices);
+      return -1;
+    }
+    *vertices = s.vertices;
+    *vertex_count = s.vertex_count;
   }
   return 0;
 }
@@ -3551,7 +3654,7 @@ int oracle_em_run_shower(const c8mi_em_config* config, const c8mi_native_table_s
                          c8mi_em_fallback* fallbacks, or_em_stats* stats) {
   return run_shower(config, tables, environment, shower_id, first_history_id, primaries,
                     primary_count, maximum_fronts, photons, electrons, positrons, energy_loss,
-                    counters, observations, fallbacks, stats, NULL, NULL);
+                    counters, observations, fallbacks, stats, NULL, NULL, NULL, NULL);
 }
 
 int oracle_em_run_shower_tracks(const c8mi_em_config* config, const c8mi_native_table_set* tables,
@@ -3567,10 +3670,59 @@ int oracle_em_run_shower_tracks(const c8mi_em_config* config, const c8mi_native_
   *track_count = 0;
   return run_shower(config, tables, environment, shower_id, first_history_id, primaries,
                     primary_count, maximum_fronts, photons, electrons, positrons, energy_loss,
-                    counters, observations, fallbacks, stats, tracks, track_count);
+                    counters, observations, fallbacks, stats, tracks, track_count, NULL, NULL);
 }
 
 void oracle_free(void* p) { free(p); }
+
+int oracle_em_run_shower_vertices(const c8mi_em_config* config, const c8mi_native_table_set* tables,
+                                  const c8mi_environment* environment, uint64_t shower_id,
+                                  uint64_t first_history_id, const c8mi_particle_state* primaries,
+                                  uint64_t primary_count, or_vertex** vertices, uint64_t* vertex_count,
+                                  or_em_stats* stats) {
+  if (!vertices || !vertex_count || !stats) return -1;
+  *vertices = NULL;
+  *vertex_count = 0;
+  uint64_t const bins = config->profile_bins ? config->profile_bins : 1;
+  int64_t* hist = calloc(4 * bins, sizeof(int64_t));
+  int64_t counters[OR_EM_COUNTERS];
+  if (!hist) return -1;
+  int rc = run_shower(config, tables, environment, shower_id, first_history_id, primaries, primary_count,
+                      UINT64_MAX, hist, hist + bins, hist + 2 * bins, hist + 3 * bins, counters, NULL, NULL,
+                      stats, NULL, NULL, vertices, vertex_count);
+  free(hist);
+  return rc;
+}
+
+int oracle_em_select(const c8mi_em_config* config, const c8mi_native_table_set* tables,
+                     const c8mi_environment* environment, uint64_t shower_id,
+                     const c8mi_particle_state* particle, or_selection* out) {
+  ctx_t x;
+  memset(&x, 0, sizeof x);
+  x.t = tables;
+  x.cont.t = tables;
+  x.cont.em_cut_mev = config->em_transport_cut_mev;
+  x.cont.muon_cut_mev = config->muon_transport_cut_mev;
+  x.env = environment;
+  x.cfg = config;
+  x.seed = config->random_seed;
+  x.shower = shower_id;
+  inter_t in;
+  fb_t fb = {0, 0, {0}, 0};
+  select_interaction(&x, particle, &in, &fb);
+  memset(out, 0, sizeof *out);
+  out->status = in.status;
+  out->process = in.process_id;
+  out->component = in.component_hash;
+  out->interaction_grammage = in.interaction_grammage;
+  out->total_rate = in.total_rate;
+  out->energy_fraction = in.energy_fraction;
+  out->loss_quantile = in.loss_quantile;
+  out->process_uniform = in.process_uniform;
+  out->fallback = fb.flag;
+  out->fallback_reason = fb.reason;
+  return 0;
+}
 
 /* Independent showers in parallel over host threads (the CPU baseline of the
  * benchmark): shower k runs primaries[k] alone with shower_id first_shower + k,

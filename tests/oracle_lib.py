@@ -229,6 +229,55 @@ def em_run_shower(config, tables, environment, primaries, shower_id=1, first_his
     return out
 
 
+VERTEX_DTYPE = np.dtype([
+    ("kind", "<i4"), ("pid", "<i4"), ("process", "<i4"), ("accepted", "<u4"), ("count", "<u4"),
+    ("pad", "<u4"), ("energy", "<f8"), ("mass", "<f8"), ("target_mass", "<f8"), ("direction", "<f8", 3),
+    ("v", "<f8"), ("rate_start", "<f8"), ("rate_vertex", "<f8"), ("process_uniform", "<f8"),
+    ("child_pid", "<i4", 3), ("pad2", "<i4"), ("child_energy", "<f8", 3), ("child_direction", "<f8", (3, 3))])
+
+SELECTION_DTYPE = np.dtype([
+    ("status", "<i4"), ("process", "<i4"), ("component", "<u8"), ("interaction_grammage", "<f8"),
+    ("total_rate", "<f8"), ("energy_fraction", "<f8"), ("loss_quantile", "<f8"), ("process_uniform", "<f8"),
+    ("fallback", "<i4"), ("fallback_reason", "<i4")])
+
+
+def em_vertices(config, tables, environment, primaries, shower_id=1, first_history_id=2):
+    """Vertex log of an oracle shower (oracle_em_run_shower_vertices): lepton
+    vertex attempts (kind 0) and materialised final states (kind 1)."""
+    from c8mi import PARTICLE_STATE_DTYPE
+    from c8mi.em import RunResult
+
+    lib = load()
+    cfg = config.struct if hasattr(config, "struct") else config
+    env = environment.struct if hasattr(environment, "struct") else environment
+    prim = np.ascontiguousarray(primaries, dtype=PARTICLE_STATE_DTYPE).reshape(-1)
+    buf, n, stats = c_void_p(), c_uint64(), RunResult()
+    if lib.oracle_em_run_shower_vertices(byref(cfg), byref(tables.struct), byref(env), c_uint64(shower_id),
+                                         c_uint64(first_history_id), prim.ctypes.data_as(c_void_p),
+                                         c_uint64(prim.size), byref(buf), byref(n), byref(stats)) != 0:
+        raise RuntimeError("oracle_em_run_shower_vertices failed")
+    try:
+        raw = ctypes.string_at(buf, n.value * VERTEX_DTYPE.itemsize) if n.value else b""
+    finally:
+        lib.oracle_free(buf)
+    return np.frombuffer(raw, dtype=VERTEX_DTYPE).copy(), stats.as_dict()
+
+
+def em_select(config, tables, environment, particles, shower_id=1):
+    """select_discrete_interaction of each particle (oracle_em_select)."""
+    from c8mi import PARTICLE_STATE_DTYPE
+
+    lib = load()
+    cfg = config.struct if hasattr(config, "struct") else config
+    env = environment.struct if hasattr(environment, "struct") else environment
+    prim = np.ascontiguousarray(particles, dtype=PARTICLE_STATE_DTYPE).reshape(-1)
+    out = np.zeros(prim.size, dtype=SELECTION_DTYPE)
+    for i in range(prim.size):
+        lib.oracle_em_select(byref(cfg), byref(tables.struct), byref(env), c_uint64(shower_id),
+                             prim[i:i + 1].ctypes.data_as(c_void_p), out[i:i + 1].ctypes.data_as(c_void_p))
+    return out
+
+
 def em_bucket_key(particle) -> int:
     """oracle_em_bucket_key: the reference's 50-bit wavefront bucket key."""
     from c8mi import PARTICLE_STATE_DTYPE
