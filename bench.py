@@ -54,15 +54,15 @@ def parse():
                    help="run CoREAS and ZHS as two launches instead of one fused launch")
     p.add_argument("--exact", action="store_true",
                    help="reference FP64 operation order (bit-identical to the FP64 oracle)")
-    p.add_argument("--em-showers", type=int, default=2048,
+    p.add_argument("--em-showers", type=int, default=8192,
                    help="timed 100 GeV showers of the EM pool (0 disables the EM section)")
     p.add_argument("--em-plain-showers", type=int, default=20,
                    help="timed 100 GeV showers of the one-at-a-time EM form")
-    p.add_argument("--em-slots", type=int, default=512, help="concurrent showers of the EM pool")
-    p.add_argument("--em-moliere-fast-iterations", type=int, default=0,
+    p.add_argument("--em-slots", type=int, default=8192, help="concurrent showers of the EM pools")
+    p.add_argument("--em-moliere-fast-iterations", type=int, default=5,
                    help="Newton iterations of a Moliere angle in the step kernel before the sample moves to "
                         "the tail kernel (0: the library default; results do not depend on it)")
-    p.add_argument("--em-pools", type=int, default=1,
+    p.add_argument("--em-pools", type=int, default=2,
                    help="backends the EM pool's slots are split over (own stream and host thread each)")
     p.add_argument("--em-warmup-showers", type=int, default=64, help="untimed showers before the EM pool run")
     p.add_argument("--em-no-moliere", action="store_true",

@@ -167,7 +167,7 @@ EmBackend::EmBackend(c8mi_em_config const& config, c8mi_native_table_set const& 
   host_report_ = PinnedBuffer<unsigned long long>(kReportHeader + kReportPerSlot * static_cast<std::size_t>(slots_));
   control_dev_.allocate(slots_);
   control_host_ = PinnedBuffer<SlotControl>(slots_);
-  for (auto* b : {&history_, &stats_, &shower_totals_, &record_counts_})
+  for (auto* b : {&history_, &stats_, &shower_totals_, &record_counts_, &adjust_})
     C8MI_HIP(hipMemset(b->data(), 0, b->bytes()));
   C8MI_HIP(hipMemset(counters_.data(), 0, counters_.bytes()));
   C8MI_HIP(hipMemset(profile_.data(), 0, profile_.bytes()));

@@ -48,6 +48,10 @@ constexpr int kStepBlock = 256;
 constexpr int kScanBlock = 1024;
 constexpr std::uint32_t kNoKey = 0xffffffffu;
 constexpr int kRunShift = 20;
+// Words of a slot's adjust row beyond the run offsets: its first source in the
+// front + 1 (0: none) and that source's in-workgroup exclusive children prefix.
+constexpr int kAdjustFirst = 3;
+constexpr int kAdjustChildPrefix = 4;
 // Vertex kernel grid: one workgroup per 2048 front particles (a chunk of 256
 // vertices per workgroup and pass), more vertices are covered by the stride loop.
 constexpr std::uint64_t kSourcesPerVertexBlock = 2048;
@@ -473,6 +477,13 @@ __global__ __launch_bounds__(kStepBlock) void count_kernel(FrontArgs f) {
     std::uint32_t const incl = before + inc[k], excl = incl - v[k];
     if (start && excl != 0) atomicAdd(&t[k], 0ULL - excl);
     if (end && incl != 0) atomicAdd(&t[k], static_cast<unsigned long long>(incl));
+    // the shower's first source in the front: its in-workgroup exclusive
+    // children prefix, from which the scan forms the shower's child offset
+    if (k == 2 && start && (i == 0 || f.in.shower[i - 1] != s)) {
+      unsigned long long* a = f.adjust + static_cast<std::uint64_t>(s) * kMaxRuns * 3;
+      a[kAdjustFirst] = i + 1;
+      a[kAdjustChildPrefix] = excl;
+    }
   }
 }
 
@@ -513,50 +524,6 @@ __global__ __launch_bounds__(kScanBlock) void scan_kernel(FrontArgs f, std::uint
     __syncthreads();
   }
 
-  // ---- per slot (a front is one run sorted by slot, so a global output rank
-  // is already the output's queue position): the offset that turns a child's
-  // global rank into its history ID inside its shower, new queue counts,
-  // history counters and statistics. One exclusive scan of the children
-  // counts over the slots, a workgroup's worth of slots at a time. ----
-  __shared__ unsigned long long slot_sums[2][kScanBlock / 64];
-  unsigned long long children_before = 0;  // children of the slots of earlier chunks
-  int parity = 0;
-  for (std::uint32_t chunk = 0; chunk < f.slots; chunk += kScanBlock) {
-    std::uint32_t const s = chunk + threadIdx.x;
-    bool const mine = s < f.slots;
-    unsigned long long* row = f.shower_totals + static_cast<std::uint64_t>(mine ? s : 0) * kMaxRuns * kShowerFields;
-    unsigned long long const x = mine ? row[2] : 0ULL;
-    unsigned long long const inc = wave_inclusive(x, lane);
-    if (lane == 63) slot_sums[parity][wave] = inc;
-    __syncthreads();
-    unsigned long long before = 0, all = 0;
-    for (std::uint32_t w = 0; w < kScanBlock / 64; ++w) {
-      unsigned long long const y = slot_sums[parity][w];
-      before += w < wave ? y : 0ULL;
-      all += y;
-    }
-    parity ^= 1;  // the next chunk writes the other buffer; the one after is past a barrier
-    if (mine) {
-      unsigned long long const excl = children_before + before + inc - x;
-      unsigned long long const history = f.history[s];
-      unsigned long long* adj = f.adjust + static_cast<std::uint64_t>(s) * kMaxRuns * 3;
-      adj[0] = 0;
-      adj[1] = 0;
-      adj[2] = history - excl;
-      unsigned long long const next = row[0], other = row[1], steps = row[3], interactions = row[4];
-#pragma unroll
-      for (int k = 0; k < kShowerFields; ++k) row[k] = 0;  // ready for the next front
-      f.history[s] = history + x;
-      unsigned long long* st = f.stats + static_cast<std::uint64_t>(s) * kSlotStats;
-      st[photon ? 0 : 1] += steps;
-      st[photon ? 2 : 3] += interactions;
-      st[4] += x;
-      f.report[kReportHeader + kReportPerSlot * static_cast<std::uint64_t>(s)] = next;
-      f.report[kReportHeader + kReportPerSlot * static_cast<std::uint64_t>(s) + 1] = other;
-      f.report[kReportHeader + kReportPerSlot * static_cast<std::uint64_t>(s) + 2] = x;
-    }
-    children_before += all;
-  }
   if (threadIdx.x == 0) {
     FrontTotals out{};
     out.next = carry[0];
@@ -566,6 +533,41 @@ __global__ __launch_bounds__(kScanBlock) void scan_kernel(FrontArgs f, std::uint
     out.interactions = carry[4];
     *reinterpret_cast<FrontTotals*>(f.report) = out;
   }
+}
+
+// Per slot (a front is one run sorted by slot, so a global output rank is
+// already the output's queue position): the offset that turns a child's global
+// rank into its history ID inside its shower (the shower's first children
+// rank is its first source's workgroup offset plus the in-workgroup prefix the
+// count kernel kept), new queue counts, history counters and statistics. One
+// thread per slot.
+__global__ __launch_bounds__(kStepBlock) void slot_update_kernel(FrontArgs f, int photon) {
+  std::uint32_t const s = blockIdx.x * kStepBlock + threadIdx.x;
+  if (s >= f.slots) return;
+  unsigned long long* row = f.shower_totals + static_cast<std::uint64_t>(s) * kMaxRuns * kShowerFields;
+  unsigned long long* adj = f.adjust + static_cast<std::uint64_t>(s) * kMaxRuns * 3;
+  unsigned long long const next = row[0], other = row[1], children = row[2], steps = row[3],
+                           interactions = row[4];
+#pragma unroll
+  for (int k = 0; k < kShowerFields; ++k) row[k] = 0;  // ready for the next front
+  unsigned long long const history = f.history[s];
+  unsigned long long const first = adj[kAdjustFirst];
+  if (first != 0) {
+    std::uint64_t const block = (first - 1) / kStepBlock;
+    unsigned long long const excl = f.offsets[3 * block + 2] + adj[kAdjustChildPrefix];
+    adj[0] = 0;
+    adj[1] = 0;
+    adj[2] = history - excl;
+    adj[kAdjustFirst] = 0;
+  }
+  f.history[s] = history + children;
+  unsigned long long* st = f.stats + static_cast<std::uint64_t>(s) * kSlotStats;
+  st[photon ? 0 : 1] += steps;
+  st[photon ? 2 : 3] += interactions;
+  st[4] += children;
+  f.report[kReportHeader + kReportPerSlot * static_cast<std::uint64_t>(s)] = next;
+  f.report[kReportHeader + kReportPerSlot * static_cast<std::uint64_t>(s) + 1] = other;
+  f.report[kReportHeader + kReportPerSlot * static_cast<std::uint64_t>(s) + 2] = children;
 }
 
 __global__ __launch_bounds__(kStepBlock) void scatter_kernel(FrontArgs f, ParticleQueue next_queue,
@@ -824,6 +826,10 @@ void launch_scan(bool photon, FrontArgs const& f, hipStream_t stream) {
   std::uint32_t const blocks = f.count == 0 ? 0u : grid_for(f.count, kStepBlock);
   scan_kernel<<<1, kScanBlock, 0, stream>>>(f, blocks, photon ? 1 : 0);
   C8MI_HIP(hipGetLastError());
+  if (f.slots) {
+    slot_update_kernel<<<grid_for(f.slots, kStepBlock), kStepBlock, 0, stream>>>(f, photon ? 1 : 0);
+    C8MI_HIP(hipGetLastError());
+  }
 }
 
 void launch_scatter(FrontArgs const& f, ParticleQueue const& next_queue, ParticleQueue const& other_queue,

@@ -236,6 +236,13 @@ int oracle_em_run_batch(const c8mi_em_config* config, const c8mi_native_table_se
                         const c8mi_particle_state* primaries, uint64_t shower_count,
                         uint64_t maximum_fronts, int threads, or_em_stats* stats);
 
+/* oracle_em_run_batch keeping each shower's histograms (4 x profile_bins
+ * int64 per shower) and counters (OR_EM_COUNTERS per shower). */
+int oracle_em_run_batch_profiles(const c8mi_em_config* config, const c8mi_native_table_set* tables,
+                                 const c8mi_environment* environment, uint64_t first_shower,
+                                 const c8mi_particle_state* primaries, uint64_t shower_count, int threads,
+                                 or_em_stats* stats, int64_t* hist, int64_t* counters);
+
 /* Molière probes (analytical correction functions). */
 double oracle_moliere_f1(const c8mi_moliere_snapshot* m, double x);
 double oracle_moliere_f2(const c8mi_moliere_snapshot* m, double x);

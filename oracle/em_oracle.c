@@ -3761,6 +3761,39 @@ int oracle_em_run_batch(const c8mi_em_config* config, const c8mi_native_table_se
   return failed ? -1 : 0;
 }
 
+/* oracle_em_run_batch that also keeps every shower's four histograms,
+ * hist[k * 4 * profile_bins + h * profile_bins + b] (h: photons, electrons,
+ * positrons, energy loss), and its counters[k * OR_EM_COUNTERS + c]. */
+int oracle_em_run_batch_profiles(const c8mi_em_config* config, const c8mi_native_table_set* tables,
+                                 const c8mi_environment* environment, uint64_t first_shower,
+                                 const c8mi_particle_state* primaries, uint64_t shower_count, int threads,
+                                 or_em_stats* stats, int64_t* hist, int64_t* counters) {
+  int failed = 0;
+  uint64_t const bins = config->profile_bins ? config->profile_bins : 1;
+#pragma omp parallel num_threads(threads > 0 ? threads : 1) reduction(| : failed)
+  {
+    c8mi_particle_state* obs = config->observation_capacity
+                                   ? malloc(config->observation_capacity * sizeof *obs) : NULL;
+    c8mi_em_fallback* fb = config->fallback_capacity
+                               ? malloc(config->fallback_capacity * sizeof *fb) : NULL;
+    if ((config->observation_capacity && !obs) || (config->fallback_capacity && !fb)) {
+      failed = 1;
+    } else {
+#pragma omp for schedule(dynamic, 1)
+      for (uint64_t k = 0; k < shower_count; ++k) {
+        int64_t* h = hist + k * 4 * bins;
+        memset(h, 0, 4 * bins * sizeof(int64_t));
+        failed |= oracle_em_run_shower(config, tables, environment, first_shower + k, 2, &primaries[k], 1,
+                                       UINT64_MAX, h, h + bins, h + 2 * bins, h + 3 * bins,
+                                       counters + k * OR_EM_COUNTERS, obs, fb, &stats[k]) != 0;
+      }
+    }
+    free(obs);
+    free(fb);
+  }
+  return failed ? -1 : 0;
+}
+
 /* ---- single-function probes for the tests ---- */
 
 int oracle_em_environment_query(const c8mi_environment* e, const double position[3],

@@ -278,6 +278,29 @@ def em_select(config, tables, environment, particles, shower_id=1):
     return out
 
 
+def em_run_batch_profiles(config, tables, environment, primaries, first_shower=1, threads=1):
+    """Independent single-primary showers on `threads` host threads; per
+    shower the stats, the four int64 histograms [count][4][bins] and the raw
+    counters [count][COUNTERS]."""
+    from c8mi import PARTICLE_STATE_DTYPE
+    from c8mi.em import COUNTERS, RunResult
+
+    lib = load()
+    cfg = config.struct if hasattr(config, "struct") else config
+    env = environment.struct if hasattr(environment, "struct") else environment
+    prim = np.ascontiguousarray(primaries, dtype=PARTICLE_STATE_DTYPE).reshape(-1)
+    bins = max(cfg.profile_bins, 1)
+    stats = (RunResult * prim.size)()
+    hist = np.zeros((prim.size, 4, bins), dtype=np.int64)
+    counters = np.zeros((prim.size, COUNTERS), dtype=np.int64)
+    if lib.oracle_em_run_batch_profiles(byref(cfg), byref(tables.struct), byref(env), c_uint64(first_shower),
+                                        prim.ctypes.data_as(c_void_p), c_uint64(prim.size), ctypes.c_int(threads),
+                                        stats, hist.ctypes.data_as(c_void_p),
+                                        counters.ctypes.data_as(c_void_p)) != 0:
+        raise RuntimeError("oracle_em_run_batch_profiles failed")
+    return [s.as_dict() for s in stats], hist, counters
+
+
 def em_bucket_key(particle) -> int:
     """oracle_em_bucket_key: the reference's 50-bit wavefront bucket key."""
     from c8mi import PARTICLE_STATE_DTYPE
