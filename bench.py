@@ -70,6 +70,10 @@ def parse():
     p.add_argument("--em-analytic-moliere", action="store_true",
                    help="Moliere with the analytical correction functions instead of the interpolation table")
     p.add_argument("--em-only", action="store_true", help="run only the EM section")
+    p.add_argument("--em-no-prefix", action="store_true",
+                   help="EM showers start on the device from the primary (no CPU prefix / handoff)")
+    p.add_argument("--em-handoff-showers", type=int, default=4,
+                   help="showers run one at a time through the host handoff driver (0 skips)")
     p.add_argument("--em-radio-showers", type=int, default=2,
                    help="timed showers of the EM + radio section (0 skips it)")
     p.add_argument("--em-radio-energy", type=float, default=1.0e4, help="primary electron energy (GeV)")
@@ -209,16 +213,58 @@ def em_showers(energy, count, first_shower, atm=None, pid=22):
                  footprint=em.footprint_estimate(energy)) for k in range(count)]
 
 
+def em_handoff_showers(tables, atm, energy, count, moliere, table):
+    """`count` showers one after another through the host handoff
+    (c8mi_em_hybrid_run, the reference's constants): throughput, latency and
+    the host / device split of the wall time."""
+    from c8mi import airtables, em
+
+    cfg = em.make_config(airtables, atmosphere=atm, slots=1, queue_capacity=1 << 20, observation_capacity=1 << 14,
+                         fallback_capacity=1 << 10, moliere=moliere, moliere_table=table)
+    dev = em.EmBackend(cfg, tables, atm)
+    stepper = em.ScalarStepper(cfg, tables, atm)
+    cascade = em.HybridCascade(dev, stepper)
+    prim = em.particle(22, energy, (0.0, 0.0, 112750.0), (0.0, 0.0, -1.0), history_id=1)
+    cascade.run(prim, 7000)  # warm-up
+    runs = []
+    for k in range(count):
+        t0 = time.perf_counter()
+        r = cascade.run(prim, 7001 + k)
+        runs.append((time.perf_counter() - t0, r))
+    cascade.close()
+    stepper.close()
+    dev.close()
+    wall = sum(t for t, _ in runs)
+    steps = sum(r["cpu_photon_steps"] + r["cpu_lepton_steps"] + r["device_photon_steps"] + r["device_lepton_steps"]
+                for _, r in runs)
+    return {"value": steps / wall, "showers": count,
+            "latency_ms": {"p50": float(np.percentile([t for t, _ in runs], 50) * 1e3),
+                           "p99": float(np.percentile([t for t, _ in runs], 99) * 1e3)},
+            "host_step_s": sum(r["host_seconds"] for _, r in runs) / count,
+            "device_call_s": sum(r["device_seconds"] for _, r in runs) / count,
+            "calls_per_shower": sum(r["photon_calls"] + r["lepton_calls"] for _, r in runs) / count,
+            "expansion_rounds_per_shower": sum(r["small_batch_expansions"] for _, r in runs) / count,
+            "host_steps_per_shower": sum(r["cpu_photon_steps"] + r["cpu_lepton_steps"] for _, r in runs) / count}
+
+
 def em_section(showers, plain_showers, slots, cpu_seconds, no_cpu_baseline, energy=100.0, warmup_showers=64,
-               moliere=True, moliere_table=True, pools=1, fast_iterations=0):
+               moliere=True, moliere_table=True, pools=1, fast_iterations=0, prefix=True, handoff_showers=4):
     """EM particle-steps/s on the plain 100 GeV vertical photon shower in the
-    USStdBK atmosphere (1 MeV cut, no thinning, FP64), the whole cascade on the
-    device from the primary on. Two forms: the pool (`showers` showers served
-    through `slots` concurrent slots, continuous batching; the throughput
-    figure) and one shower at a time (`plain_showers` showers, per-shower
-    latency). With pools > 1 the slots are split over that many backends, each
-    on its own stream and served from its own host thread, so one pool's
-    kernels run while another's host turns its front report around."""
+    USStdBK atmosphere (1 MeV cut, no thinning, FP64). With `prefix` every
+    shower starts on the host as the reference's router starts it: scalar
+    expansion rounds of the host stepper up to the first device call (the
+    16384-particle handoff batch, at most 8 rounds; c8mi_em_hybrid_prefixes on
+    host threads, inside the timed region), then the handed-off batch runs on
+    the device with the router's calls (a below-16384 checkpoint after every
+    first front). Without it the whole cascade runs on the device from the
+    primary on. Forms: the pool (`showers` showers served through `slots`
+    concurrent slots, continuous batching; the throughput figure), one shower
+    at a time on the device (`plain_showers` showers, per-shower latency) and
+    `handoff_showers` showers driven one at a time through the host handoff
+    (c8mi_em_hybrid_run: every device call returns to the host). With pools >
+    1 the slots are split over that many backends, each on its own stream and
+    served from its own host thread, so one pool's kernels run while
+    another's host turns its front report around."""
     import threading
 
     from c8mi import airtables, em
@@ -228,9 +274,11 @@ def em_section(showers, plain_showers, slots, cpu_seconds, no_cpu_baseline, ener
     pool_slots = max(1, slots // pools)
     capacity = 1 << 23 if pool_slots <= 1024 else 1 << 24 if pool_slots <= 2048 else 1 << 25
     table = True if moliere_table else None
+    handoff = 16384  # BackendConfig minBatchSize: the router's handoff batch and the calls' minimum
     cfg = em.make_config(airtables, atmosphere=atm, slots=pool_slots, queue_capacity=capacity,
                          observation_capacity=1 << 12, fallback_capacity=1 << 8, moliere=moliere,
-                         moliere_table=table, moliere_fast_iterations=fast_iterations)
+                         moliere_table=table, moliere_fast_iterations=fast_iterations,
+                         resident_minimum_batch=handoff if prefix else 0)
     devs = [em.EmBackend(cfg, tables, atm) for _ in range(pools)]
     budget = 0.4 * capacity
     zero = {"photon_steps": 0, "lepton_steps": 0, "lepton_interactions": 0}
@@ -243,12 +291,22 @@ def em_section(showers, plain_showers, slots, cpu_seconds, no_cpu_baseline, ener
     requests = em_showers(energy, showers, 1000)
     parts = [requests[k::pools] for k in range(pools)]
     outcome = [None] * pools
+    host_threads = max(1, min(int(os.environ.get("OMP_NUM_THREADS", "0")) or 16, os.cpu_count() or 1))
 
     def serve_part(k):
         outcome[k] = devs[k].serve(parts[k], budget, realtime=False)
 
     workers = [threading.Thread(target=serve_part, args=(k,)) for k in range(pools)]
     t0 = time.perf_counter()
+    host_ledgers, host_totals, prefix_s = None, None, 0.0
+    if prefix:
+        prims = np.concatenate([r["primary"].reshape(1) for r in requests])
+        batches, next_ids, host_ledgers, host_totals = em.hybrid_prefixes(
+            cfg, tables, atm, prims, [r["shower_id"] for r in requests], threads=host_threads,
+            hybrid=em.hybrid_config(minimum_batch=handoff))
+        for r, b, h in zip(requests, batches, next_ids):
+            r["particles"], r["first_history_id"] = b, h
+        prefix_s = time.perf_counter() - t0
     for w in workers:
         w.start()
     for w in workers:
@@ -263,13 +321,21 @@ def em_section(showers, plain_showers, slots, cpu_seconds, no_cpu_baseline, ener
     results = [r for o in outcome for r in o[0]]
     totals = {k: sum(o[1][k] for o in outcome) if k != "max_front" else max(o[1][k] for o in outcome)
               for k in outcome[0][1]}
-    steps = totals["photon_steps"] + totals["lepton_steps"]
+    host_steps = host_totals["cpu_photon_steps"] + host_totals["cpu_lepton_steps"] if prefix else 0
+    steps = totals["photon_steps"] + totals["lepton_steps"] + host_steps
+    # per request: its device ledger and, with the prefix, its host ledger
+    # (each side closes; the host books the primary as injected less the batch)
+    ledgers = [None] * len(requests)
+    for k in range(pools):
+        for j, r in enumerate(outcome[k][0]):
+            ledgers[k + j * pools] = [r["ledger"]] + ([host_ledgers[k + j * pools]] if prefix else [])
     # Closure holds to 1e-9 E0 except in showers with Molière-failed steps: that
     # transport fallback hands over the vertex state, so the failed step's
     # continuous deposit is not booked (MoliereStep.inl:60-64).
-    moliere_failed = [r for r in results if r["ledger"]["fallback_reasons"][21]]
-    residual = max([abs(em.ledger_residual(r["ledger"])) for r in results
-                    if not r["ledger"]["fallback_reasons"][21]] or [0.0])
+    failed = [any(l["fallback_reasons"][21] for l in ls) for ls in ledgers]
+    residual = max([abs(sum(em.ledger_residual(l) for l in ls)) for ls, f in zip(ledgers, failed) if not f]
+                   or [0.0])
+    injected = max(abs(sum(l["injected_energy"] for l in ls) - energy) for ls in ledgers)
     pool_latency = [r["finished_s"] - r["admitted_s"] for r in results]
 
     plain_warm_totals = plain_totals = zero
@@ -330,8 +396,10 @@ def em_section(showers, plain_showers, slots, cpu_seconds, no_cpu_baseline, ener
         "config": {"workload": f"EM plain: {energy:g} GeV vertical photon at 112.75 km, USStdBK 5-layer "
                                "spherical air, observation level 0 m, 1 MeV cut, no thinning, no radio, "
                                f"Moliere scattering {('on (interpolation table)' if moliere_table else 'on (analytical functions)') if moliere else 'off'}, "
-                               "closed-form v7 air tables, whole cascade on the device; "
-                               f"{showers} showers through {slots} pool slots"
+                               "closed-form v7 air tables, "
+                               + ("CPU prefix (host scalar expansion, at most 8 rounds) to the 16384 handoff, "
+                                  "then the device; " if prefix else "whole cascade on the device; ")
+                               + f"{showers} showers through {slots} pool slots"
                                + (f" in {pools} pools on their own streams" if pools > 1 else "")},
         "roofline": {
             "bound": "hbm",
@@ -347,8 +415,22 @@ def em_section(showers, plain_showers, slots, cpu_seconds, no_cpu_baseline, ener
             "device_busy_fraction": total_kernel_ms * 1e-3 / elapsed,
         },
         "ledger_residual_gev_max": residual,
-        "showers_with_moliere_fallbacks": len(moliere_failed),
+        "injected_energy_error_gev_max": injected,
+        "showers_with_moliere_fallbacks": int(sum(failed)),
     }
+    if prefix:
+        out["handoff"] = {
+            "minimum_batch": handoff,
+            "host_prefix_s": prefix_s,
+            "host_threads": host_threads,
+            "host_steps": host_steps,
+            "device_steps": totals["photon_steps"] + totals["lepton_steps"],
+            "mean_batch": float(np.mean([r["particles"].size for r in requests])),
+            "expansion_rounds": host_totals["small_batch_expansions"],
+            "budget_flushes": host_totals["scalar_expansion_budget_flushes"],
+        }
+    if handoff_showers:
+        out["handoff_one_at_a_time"] = em_handoff_showers(tables, atm, energy, handoff_showers, moliere, table)
     if not no_cpu_baseline:
         base = em_cpu_baseline(tables, atm, energy, cpu_seconds, moliere=moliere, moliere_table=moliere_table)
         out["cpu_baseline"] = base
@@ -396,14 +478,17 @@ def em_radio_cpu_baseline(tables, atmosphere, cfg, observers, propagation, targe
                       f"{threads} threads"}
 
 
-def em_radio_section(showers, energy, cpu_seconds, no_cpu_baseline, moliere=True, moliere_table=True):
+def em_radio_section(showers, energy, cpu_seconds, no_cpu_baseline, moliere=True, moliere_table=True, prefix=True):
     """EM transport with radio: a vertical electron of `energy` GeV at 112.75 km
-    in USStdBK air (1 MeV cut, no thinning, Molière on), the whole cascade on
-    the device, every e+- step projected by CoREAS+ZHS onto the 160-antenna
-    star (8 arms x 20 radii, 25-500 m) at 1024 bins @ 0.5 ns, n(h) = 1 +
-    3.25e-4 exp(-h / 8 km). Showers run one after another, each with its own
-    radio reset and fixed-point download; the projection of a front overlaps
-    the next fronts on its own stream."""
+    in USStdBK air (1 MeV cut, no thinning, Molière on), every e+- step
+    projected by CoREAS+ZHS onto the 160-antenna star (8 arms x 20 radii,
+    25-500 m) at 1024 bins @ 0.5 ns, n(h) = 1 + 3.25e-4 exp(-h / 8 km). With
+    `prefix` each shower runs through the host handoff (c8mi_em_hybrid_run:
+    host scalar expansion down to the 16384 handoff batch, then the router's
+    device calls; the host e+- tracks reach the radio as host tracks), else the
+    whole cascade on the device. Showers run one after another, each with its
+    own radio reset and fixed-point download; the projection of a front
+    overlaps the next fronts on its own stream."""
     import c8mi
     from c8mi import airtables, em, workloads
 
@@ -419,18 +504,31 @@ def em_radio_section(showers, energy, cpu_seconds, no_cpu_baseline, moliere=True
     prop = c8mi.Propagation(refractivity=refractivity, minimum_height_m=-1000.0, maximum_height_m=113000.0,
                             step_m=1.0)
     dev = em.EmBackend(cfg, tables, atm)
+    stepper = em.ScalarStepper(cfg, tables, atm) if prefix else None
+    cascade = em.HybridCascade(dev, stepper) if prefix else None
 
     def run(radio, shower_id, e):
         radio.reset()
-        dev.begin_shower(shower_id, 2)
-        dev.enqueue(em.particle(11, e, start, down, history_id=1))
+        prim = em.particle(11, e, start, down, history_id=1)
         t0 = time.perf_counter()
-        totals = dev.run()
+        if prefix:
+            r = cascade.run(prim, shower_id, 2)
+            totals = {"photon_steps": r["cpu_photon_steps"] + r["device_photon_steps"],
+                      "lepton_steps": r["cpu_lepton_steps"] + r["device_lepton_steps"], "handoff": r}
+        else:
+            dev.begin_shower(shower_id, 2)
+            dev.enqueue(prim)
+            totals = dev.run()
         fixed = radio.download_fixed()  # waits for the last projections
         return time.perf_counter() - t0, totals, fixed
 
+    def attach(radio):
+        dev.attach_radio(radio)
+        if prefix:
+            cascade.attach_radio(radio)
+
     radio = c8mi.RadioAccumulator(observers, prop)
-    dev.attach_radio(radio)
+    attach(radio)
     run(radio, 999, 100.0)  # warm-up
     dev.enable_kernel_timing(True)
     radio.enable_kernel_timing(True)
@@ -440,9 +538,12 @@ def em_radio_section(showers, energy, cpu_seconds, no_cpu_baseline, moliere=True
     # field error of this (fast FP64) projection: the same first shower again
     # with the exact projection, whose samples equal the FP64 oracle's
     exact = c8mi.RadioAccumulator(observers, prop, exact=True)
-    dev.attach_radio(exact)
+    attach(exact)
     _, _, exact_fixed = run(exact, 1, energy)
-    dev.attach_radio(None)
+    attach(None)
+    if prefix:
+        cascade.close()
+        stepper.close()
     dev.close()
     radio.close()
     exact.close()
@@ -474,7 +575,9 @@ def em_radio_section(showers, energy, cpu_seconds, no_cpu_baseline, moliere=True
         "field_error_vs_fp64": {"coreas": errors[0], "zhs": errors[1]},
         "config": {"workload": f"EM + radio: {energy:g} GeV vertical electron at 112.75 km, USStdBK, "
                                f"1 MeV cut, Moliere {('on (table)' if moliere_table else 'on (analytical)') if moliere else 'off'}, CoREAS+ZHS at "
-                               f"{observers.size} antennas, 1024 bins @ 0.5 ns, whole cascade on the device",
+                               f"{observers.size} antennas, 1024 bins @ 0.5 ns, "
+                               + ("serial CPU prefix down to the 16384 handoff, then the device (host handoff "
+                                  "driver, one shower at a time)" if prefix else "whole cascade on the device"),
                    "antennas": int(observers.size), "bins": 1024, "sample_rate_hz": 2.0e9},
         "kernel_ms": {**{k: v[0] for k, v in em_timing.items()},
                       **{"radio_" + k: v[0] for k, v in radio_timing.items() if v[1]}},
@@ -482,6 +585,12 @@ def em_radio_section(showers, energy, cpu_seconds, no_cpu_baseline, moliere=True
         # projection overlaps the fronts
         "device_busy_fraction": (em_ms + radio_ms) * 1e-3 / elapsed,
     }
+    if prefix:
+        hs = [r[1]["handoff"] for r in results]
+        out["handoff"] = {k: float(np.mean([h[k] for h in hs])) for k in (
+            "cpu_photon_steps", "cpu_lepton_steps", "cpu_tracks", "small_batch_expansions", "wavefronts",
+            "photon_calls", "lepton_calls", "maximum_input_batch", "below_minimum_batch_checkpoints",
+            "host_seconds", "device_seconds")}
     if not no_cpu_baseline:
         base = em_radio_cpu_baseline(tables, atm, cfg, observers, prop, cpu_seconds)
         out["cpu_baseline"] = base
@@ -822,7 +931,7 @@ def main():
     if args.em_radio_only:
         out = em_radio_section(max(args.em_radio_showers, 1), args.em_radio_energy, args.cpu_seconds,
                                args.no_cpu_baseline, moliere=not args.em_no_moliere,
-                               moliere_table=not args.em_analytic_moliere)
+                               moliere_table=not args.em_analytic_moliere, prefix=not args.em_no_prefix)
         out.update({"n_gpus": 1, "steps": out["showers"], "higher_is_better": True, "scaling": "weak",
                     "vs_baseline": None, "data": "synthetic",
                     "ms_per_step": out["latency_ms"]["p50"]})
@@ -833,7 +942,8 @@ def main():
                                args.no_cpu_baseline, warmup_showers=args.em_warmup_showers,
                                moliere=not args.em_no_moliere,
                                moliere_table=not args.em_analytic_moliere, pools=args.em_pools,
-                               fast_iterations=args.em_moliere_fast_iterations)
+                               fast_iterations=args.em_moliere_fast_iterations, prefix=not args.em_no_prefix,
+                               handoff_showers=args.em_handoff_showers)
         em_result.update({"n_gpus": 1, "steps": em_result["showers"], "higher_is_better": True,
                           "scaling": "weak", "vs_baseline": None, "data": "synthetic",
                           "ms_per_step": em_result["ms_per_shower"]})
@@ -935,7 +1045,8 @@ def main():
                                   args.no_cpu_baseline, warmup_showers=args.em_warmup_showers,
                                   moliere=not args.em_no_moliere,
                                   moliere_table=not args.em_analytic_moliere, pools=args.em_pools,
-                               fast_iterations=args.em_moliere_fast_iterations)
+                                  fast_iterations=args.em_moliere_fast_iterations, prefix=not args.em_no_prefix,
+                                  handoff_showers=args.em_handoff_showers)
     if args.serve_showers > 0:
         result["serving"] = serving_section(args.serve_showers, args.serve_rate, args.serve_slots,
                                             args.serve_plain_showers, args.cpu_seconds, args.no_cpu_baseline,
@@ -943,7 +1054,8 @@ def main():
     if args.em_radio_showers > 0:
         result["em_radio"] = em_radio_section(args.em_radio_showers, args.em_radio_energy, args.cpu_seconds,
                                               args.no_cpu_baseline, moliere=not args.em_no_moliere,
-                                              moliere_table=not args.em_analytic_moliere)
+                                              moliere_table=not args.em_analytic_moliere,
+                                              prefix=not args.em_no_prefix)
     if args.ice_energy > 0:
         result["ice"] = ice_section(args.ice_energy, args.cpu_seconds, args.no_cpu_baseline)
     print(json.dumps(result), flush=True)

@@ -209,6 +209,7 @@ class Request(ctypes.Structure):
                 ("arrival_s", c_double), ("footprint_particles", c_double), ("axis", ShowerAxis),
                 ("radio_observers", c_void_p), ("radio_observer_count", c_uint64),
                 ("radio_coreas", c_void_p), ("radio_zhs", c_void_p),
+                ("particles", c_void_p), ("particle_count", c_uint64),
                 ("align_", c_uint64)]  # the C struct is 16-byte aligned
 
 
@@ -779,8 +780,9 @@ class EmBackend:
 
     def serve(self, requests, footprint_budget, realtime=True):
         """requests: list of dicts with primary (particle), shower_id, arrival_s,
-        footprint, axis (ShowerAxis or None) and, with per-shower radio, radio
-        observers (an OBSERVER_DTYPE array); returns (per-request dicts,
+        footprint, axis (ShowerAxis or None), particles (a handed-off batch
+        enqueued instead of the primary, or None) and, with per-shower radio,
+        radio observers (an OBSERVER_DTYPE array); returns (per-request dicts,
         totals). A request with observers gets its raw fixed-point waveforms
         as result["radio"] = (coreas [3][bins], zhs [3][bins])."""
         from . import observers_array
@@ -796,6 +798,11 @@ class EmBackend:
             arr[k].first_history_id = r.get("first_history_id", 2)
             arr[k].arrival_s = r.get("arrival_s", 0.0)
             arr[k].footprint_particles = r.get("footprint", 0.0)
+            if r.get("particles") is not None:
+                batch = np.ascontiguousarray(r["particles"], dtype=PARTICLE_STATE_DTYPE).reshape(-1)
+                keep.append(batch)
+                arr[k].particles = batch.ctypes.data
+                arr[k].particle_count = batch.size
             if r.get("axis") is not None:
                 arr[k].axis = r["axis"]
                 keep.append(r["axis"])
@@ -877,3 +884,183 @@ class EmBackend:
         _check(library().c8mi_em_kernel_time(self.handle, ms, n))
         return {k: (ms[i], n[i]) for i, k in enumerate(("photon_step", "lepton_transport", "lepton_vertex",
                                                         "count", "scan", "scatter"))}
+
+
+# ---- host handoff (include/c8mi/c8mi_hybrid.h) ----
+
+HYBRID_COUNTS = ("cpu_particles", "cpu_photon_steps", "cpu_lepton_steps", "cpu_interactions", "cpu_secondaries",
+                 "cpu_tracks", "particles_staged", "small_batch_expansions", "particles_returned_for_expansion",
+                 "scalar_expansion_budget_flushes", "stalled_boundary_flushes", "wavefronts", "input_batch_splits",
+                 "maximum_input_batch", "reserved_history_ids", "below_minimum_batch_checkpoints",
+                 "history_range_checkpoints", "photon_calls", "lepton_calls", "device_photon_fronts",
+                 "device_lepton_fronts", "device_photon_steps", "device_lepton_steps", "device_interactions",
+                 "device_secondaries", "bucketed_fronts", "next_history_id")
+
+
+class HybridConfig(ctypes.Structure):
+    """EmRouter's scheduling constants (zero: the reference's value)."""
+    _fields_ = [("minimum_batch_size", c_uint64), ("maximum_expansion_rounds", c_uint32),
+                ("photon_wavefronts", c_uint32), ("lepton_wavefronts", c_uint32),
+                ("lepton_reservation_multiplier", c_uint32), ("lepton_reservation_chunk", c_uint64)]
+
+
+class HybridResult(ctypes.Structure):
+    _fields_ = [(k, c_uint64) for k in HYBRID_COUNTS] + [("host_seconds", c_double), ("device_seconds", c_double)]
+
+    def as_dict(self):
+        out = {k: int(getattr(self, k)) for k in HYBRID_COUNTS}
+        out["host_seconds"] = self.host_seconds
+        out["device_seconds"] = self.device_seconds
+        return out
+
+
+def hybrid_config(minimum_batch=0, expansion_rounds=0, photon_wavefronts=0, lepton_wavefronts=0,
+                  multiplier=0, chunk=0) -> HybridConfig:
+    return HybridConfig(minimum_batch, expansion_rounds, photon_wavefronts, lepton_wavefronts, multiplier, chunk)
+
+
+class ScalarStepper:
+    """Binding of the host scalar stepper (c8mi_em_scalar_*): one step of one
+    particle on the CPU with the device path's step functions."""
+
+    def __init__(self, config, tables, environment):
+        cfg = config.struct if hasattr(config, "struct") else config
+        env = environment.struct if hasattr(environment, "struct") else environment
+        self.config, self.tables, self.environment = config, tables, environment
+        self.bins = cfg.profile_bins
+        self.observation_capacity = cfg.observation_capacity
+        self.fallback_capacity = cfg.fallback_capacity
+        self.handle = c_void_p()
+        _check(library().c8mi_em_scalar_create(byref(cfg), byref(tables.struct), byref(env), byref(self.handle)))
+
+    def close(self):
+        if self.handle:
+            _check(library().c8mi_em_scalar_destroy(self.handle))
+            self.handle = c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def begin_shower(self, shower_id=1):
+        _check(library().c8mi_em_scalar_begin_shower(self.handle, c_uint64(shower_id)))
+
+    def step(self, particle_state, first_history_id):
+        """One host step: (outputs, children, track or None)."""
+        p = np.ascontiguousarray(particle_state, dtype=PARTICLE_STATE_DTYPE).reshape(-1)[:1]
+        out = np.zeros(2, dtype=PARTICLE_STATE_DTYPE)
+        count, children, has_track = c_uint32(), c_uint32(), c_int()
+        track = np.zeros(9)
+        _check(library().c8mi_em_scalar_step(self.handle, p.ctypes.data_as(c_void_p), c_uint64(first_history_id),
+                                             out.ctypes.data_as(c_void_p), byref(count), byref(children),
+                                             track.ctypes.data_as(POINTER(c_double)), byref(has_track)))
+        return out[:count.value], children.value, (track if has_track.value else None)
+
+    def download_profile(self):
+        hist = {k: np.zeros(self.bins) for k in ("photons", "electrons", "positrons", "energy_loss")}
+        led = Ledger()
+        dp = lambda a: a.ctypes.data_as(POINTER(c_double))
+        _check(library().c8mi_em_scalar_download_profile(self.handle, dp(hist["photons"]), dp(hist["electrons"]),
+                                                         dp(hist["positrons"]), dp(hist["energy_loss"]),
+                                                         byref(led)))
+        return hist, led.as_dict()
+
+    def observations(self):
+        out = np.zeros(self.observation_capacity, dtype=PARTICLE_STATE_DTYPE)
+        n = c_uint64()
+        _check(library().c8mi_em_scalar_download_observations(self.handle, out.ctypes.data_as(c_void_p),
+                                                              c_uint64(out.size), byref(n)))
+        return out[:min(n.value, out.size)], n.value
+
+    def fallbacks(self):
+        out = np.zeros(self.fallback_capacity, dtype=FALLBACK_DTYPE)
+        n = c_uint64()
+        _check(library().c8mi_em_scalar_download_fallbacks(self.handle, out.ctypes.data_as(c_void_p),
+                                                           c_uint64(out.size), byref(n)))
+        return out[:min(n.value, out.size)], n.value
+
+    def prefix(self, primaries, shower_id, first_history_id=2, config=None, capacity=1 << 16,
+               track_capacity=1 << 16):
+        """The CPU prefix of a shower (c8mi_em_hybrid_prefix): (batch handed to
+        the device, next history ID, host tracks [n][9], counts)."""
+        p = np.ascontiguousarray(primaries, dtype=PARTICLE_STATE_DTYPE).reshape(-1)
+        batch = np.zeros(capacity, dtype=PARTICLE_STATE_DTYPE)
+        tracks = np.zeros((track_capacity, 9))
+        n, nxt, nt = c_uint64(), c_uint64(), c_uint64()
+        r = HybridResult()
+        _check(library().c8mi_em_hybrid_prefix(
+            self.handle, byref(config) if config is not None else None, p.ctypes.data_as(c_void_p), c_uint64(p.size),
+            c_uint64(shower_id), c_uint64(first_history_id), batch.ctypes.data_as(c_void_p), c_uint64(capacity),
+            byref(n), byref(nxt), tracks.ctypes.data_as(POINTER(c_double)), c_uint64(track_capacity), byref(nt),
+            byref(r)))
+        if n.value > capacity or nt.value > track_capacity:
+            raise RuntimeError("prefix: batch or track capacity exceeded")
+        return batch[:n.value], nxt.value, tracks[:nt.value], r.as_dict()
+
+
+class HybridCascade:
+    """Binding of the host handoff driver (c8mi_em_hybrid_*; HybridCascade +
+    EmRouter for EM particles) over an EmBackend and a ScalarStepper built
+    from the same configuration, tables and environment."""
+
+    def __init__(self, backend, stepper, config=None):
+        self.backend, self.stepper = backend, stepper
+        self.handle = c_void_p()
+        _check(library().c8mi_em_hybrid_create(backend.handle, stepper.handle,
+                                               byref(config) if config is not None else None, byref(self.handle)))
+        self._radio = None
+
+    def close(self):
+        if self.handle:
+            _check(library().c8mi_em_hybrid_destroy(self.handle))
+            self.handle = c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def attach_radio(self, radio):
+        """Host e+- tracks go to `radio` (attach it to the backend too)."""
+        _check(library().c8mi_em_hybrid_attach_radio(self.handle, radio.handle if radio is not None else None))
+        self._radio = radio
+
+    def run(self, primaries, shower_id, first_history_id=2) -> dict:
+        p = np.ascontiguousarray(primaries, dtype=PARTICLE_STATE_DTYPE).reshape(-1)
+        r = HybridResult()
+        _check(library().c8mi_em_hybrid_run(self.handle, p.ctypes.data_as(c_void_p), c_uint64(p.size),
+                                            c_uint64(shower_id), c_uint64(first_history_id), byref(r)))
+        return r.as_dict()
+
+
+def hybrid_prefixes(config, tables, environment, primaries, shower_ids, first_history_id=2, threads=16,
+                    hybrid=None, capacity=None, histograms=False):
+    """CPU prefixes of many showers on host threads (c8mi_em_hybrid_prefixes):
+    (list of handed-off batches, next history IDs, host ledgers, totals[, host
+    histograms [n][4][bins]])."""
+    cfg = config.struct if hasattr(config, "struct") else config
+    env = environment.struct if hasattr(environment, "struct") else environment
+    prim = np.ascontiguousarray(primaries, dtype=PARTICLE_STATE_DTYPE).reshape(-1)
+    ids = np.ascontiguousarray(shower_ids, dtype=np.uint64).reshape(-1)
+    if ids.size != prim.size:
+        raise ValueError("one shower ID per primary")
+    n = prim.size
+    capacity = capacity if capacity is not None else 512 * max(n, 1)
+    staged = np.zeros(capacity, dtype=PARTICLE_STATE_DTYPE)
+    offsets = np.zeros(n + 1, dtype=np.uint64)
+    nxt = np.zeros(n, dtype=np.uint64)
+    ledgers = (Ledger * max(n, 1))()
+    hist = np.zeros((n, 4, cfg.profile_bins)) if histograms else None
+    totals = HybridResult()
+    _check(library().c8mi_em_hybrid_prefixes(
+        byref(cfg), byref(tables.struct), byref(env), byref(hybrid) if hybrid is not None else None,
+        prim.ctypes.data_as(c_void_p), c_uint64(n), ids.ctypes.data_as(c_void_p), c_uint64(first_history_id),
+        c_uint32(threads), staged.ctypes.data_as(c_void_p), c_uint64(capacity), offsets.ctypes.data_as(c_void_p),
+        nxt.ctypes.data_as(c_void_p), ledgers, hist.ctypes.data_as(c_void_p) if hist is not None else None,
+        byref(totals)))
+    batches = [staged[int(offsets[k]):int(offsets[k + 1])] for k in range(n)]
+    out = (batches, [int(x) for x in nxt], [ledgers[k].as_dict() for k in range(n)], totals.as_dict())
+    return out + (hist,) if histograms else out

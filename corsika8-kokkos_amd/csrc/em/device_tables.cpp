@@ -4,6 +4,7 @@
 #include <cstring>
 #include <stdexcept>
 #include <string>
+#include <utility>
 #include <vector>
 
 namespace c8mi::em {
@@ -99,68 +100,86 @@ void validate_table_bounds(c8mi_native_table_set const& t) {
     check_cubic(t.utility_columns[i].spline, t.cubic_coefficient_count, "utility spline");
 }
 
-DeviceTables::DeviceTables(c8mi_native_table_set const& host) {
-  validate_table_bounds(host);
+namespace {
 
+// The arrays of a table set packed into one 256-byte-aligned image; `at`
+// gives each array's offset (npos: empty).
+struct Packed {
+  static constexpr std::size_t npos = ~std::size_t{0};
+  std::vector<unsigned char> image;
+  std::size_t at[13];
+};
+
+Packed pack(c8mi_native_table_set const& host) {
+  Packed p;
+  std::size_t total = 0;
   struct Part {
     void const* source;
     std::size_t bytes;
-    std::size_t offset;
-  };
-  std::vector<Part> parts;
-  std::size_t total = 0;
-  auto add = [&](void const* source, std::size_t bytes) {
-    total = (total + 255u) & ~std::size_t{255u};
-    parts.push_back({source, bytes, total});
-    total += bytes;
-    return parts.size() - 1;
   };
   std::uint32_t const nd = host.dndx_column_count;
-  std::size_t const i_dndx = add(host.dndx_columns, nd * sizeof(c8mi_native_dndx_column));
-  std::size_t const i_total = add(host.total_rate_columns,
-                                  host.total_rate_column_count * sizeof(c8mi_native_total_rate_column));
-  std::size_t const i_sel = add(host.selection_column_indices, nd * sizeof(std::uint32_t));
-  std::size_t const i_dedx =
-      add(host.dedx_columns, host.dedx_column_count * sizeof(c8mi_native_dedx_column));
-  std::size_t const i_dedx_index =
-      add(host.dedx_accumulation_indices, host.dedx_column_count * sizeof(std::uint32_t));
-  std::size_t const i_util =
-      add(host.utility_columns, host.utility_column_count * sizeof(c8mi_native_utility_column));
   std::size_t const nb = host.bicubic_coefficient_count * sizeof(double);
-  std::size_t const i_b0 = add(host.bicubic_values, nb);
-  std::size_t const i_b1 = add(host.bicubic_derivative_energy, nb);
-  std::size_t const i_b2 = add(host.bicubic_derivative_loss, nb);
-  std::size_t const i_b3 = add(host.bicubic_mixed_derivative, nb);
-  std::size_t const i_poly =
-      add(host.bicubic_polynomial_coefficients, host.bicubic_polynomial_coefficient_count * sizeof(double));
   std::size_t const nc = host.cubic_coefficient_count * sizeof(double);
-  std::size_t const i_c0 = add(host.cubic_values, nc);
-  std::size_t const i_c1 = add(host.cubic_node_derivatives, nc);
+  Part const parts[13] = {
+      {host.dndx_columns, nd * sizeof(c8mi_native_dndx_column)},
+      {host.total_rate_columns, host.total_rate_column_count * sizeof(c8mi_native_total_rate_column)},
+      {host.selection_column_indices, nd * sizeof(std::uint32_t)},
+      {host.dedx_columns, host.dedx_column_count * sizeof(c8mi_native_dedx_column)},
+      {host.dedx_accumulation_indices, host.dedx_column_count * sizeof(std::uint32_t)},
+      {host.utility_columns, host.utility_column_count * sizeof(c8mi_native_utility_column)},
+      {host.bicubic_values, nb},
+      {host.bicubic_derivative_energy, nb},
+      {host.bicubic_derivative_loss, nb},
+      {host.bicubic_mixed_derivative, nb},
+      {host.bicubic_polynomial_coefficients, host.bicubic_polynomial_coefficient_count * sizeof(double)},
+      {host.cubic_values, nc},
+      {host.cubic_node_derivatives, nc}};
+  for (int i = 0; i < 13; ++i) {
+    total = (total + 255u) & ~std::size_t{255u};
+    p.at[i] = parts[i].bytes ? total : Packed::npos;
+    total += parts[i].bytes;
+  }
+  p.image.assign(total, 0);
+  for (int i = 0; i < 13; ++i)
+    if (parts[i].bytes) std::memcpy(p.image.data() + p.at[i], parts[i].source, parts[i].bytes);
+  return p;
+}
 
-  std::vector<unsigned char> staging(total, 0);
-  for (auto const& p : parts)
-    if (p.bytes) std::memcpy(staging.data() + p.offset, p.source, p.bytes);
-  storage_.allocate(total);
-  C8MI_HIP(hipMemcpy(storage_.data(), staging.data(), total, hipMemcpyHostToDevice));
+// The table set's view with every array at its offset from `base`.
+c8mi_native_table_set rebase(c8mi_native_table_set const& host, Packed const& p, unsigned char* base) {
+  auto at = [&](int i) -> void* { return p.at[i] == Packed::npos ? nullptr : static_cast<void*>(base + p.at[i]); };
+  c8mi_native_table_set v = host;
+  v.dndx_columns = static_cast<c8mi_native_dndx_column const*>(at(0));
+  v.total_rate_columns = static_cast<c8mi_native_total_rate_column const*>(at(1));
+  v.selection_column_indices = static_cast<std::uint32_t const*>(at(2));
+  v.dedx_columns = static_cast<c8mi_native_dedx_column const*>(at(3));
+  v.dedx_accumulation_indices = static_cast<std::uint32_t const*>(at(4));
+  v.utility_columns = static_cast<c8mi_native_utility_column const*>(at(5));
+  v.bicubic_values = static_cast<double const*>(at(6));
+  v.bicubic_derivative_energy = static_cast<double const*>(at(7));
+  v.bicubic_derivative_loss = static_cast<double const*>(at(8));
+  v.bicubic_mixed_derivative = static_cast<double const*>(at(9));
+  v.bicubic_polynomial_coefficients = static_cast<double const*>(at(10));
+  v.cubic_values = static_cast<double const*>(at(11));
+  v.cubic_node_derivatives = static_cast<double const*>(at(12));
+  return v;
+}
 
-  unsigned char* base = storage_.data();
-  auto at = [&](std::size_t i) -> void* {
-    return parts[i].bytes ? static_cast<void*>(base + parts[i].offset) : nullptr;
-  };
-  view_ = host;
-  view_.dndx_columns = static_cast<c8mi_native_dndx_column const*>(at(i_dndx));
-  view_.total_rate_columns = static_cast<c8mi_native_total_rate_column const*>(at(i_total));
-  view_.selection_column_indices = static_cast<std::uint32_t const*>(at(i_sel));
-  view_.dedx_columns = static_cast<c8mi_native_dedx_column const*>(at(i_dedx));
-  view_.dedx_accumulation_indices = static_cast<std::uint32_t const*>(at(i_dedx_index));
-  view_.utility_columns = static_cast<c8mi_native_utility_column const*>(at(i_util));
-  view_.bicubic_values = static_cast<double const*>(at(i_b0));
-  view_.bicubic_derivative_energy = static_cast<double const*>(at(i_b1));
-  view_.bicubic_derivative_loss = static_cast<double const*>(at(i_b2));
-  view_.bicubic_mixed_derivative = static_cast<double const*>(at(i_b3));
-  view_.bicubic_polynomial_coefficients = static_cast<double const*>(at(i_poly));
-  view_.cubic_values = static_cast<double const*>(at(i_c0));
-  view_.cubic_node_derivatives = static_cast<double const*>(at(i_c1));
+}  // namespace
+
+DeviceTables::DeviceTables(c8mi_native_table_set const& host) {
+  validate_table_bounds(host);
+  Packed const p = pack(host);
+  storage_.allocate(p.image.size());
+  C8MI_HIP(hipMemcpy(storage_.data(), p.image.data(), p.image.size(), hipMemcpyHostToDevice));
+  view_ = rebase(host, p, storage_.data());
+}
+
+HostTables::HostTables(c8mi_native_table_set const& host) {
+  validate_table_bounds(host);
+  Packed p = pack(host);
+  storage_ = std::move(p.image);
+  view_ = rebase(host, p, storage_.data());
 }
 
 }  // namespace c8mi::em

@@ -6,6 +6,7 @@
 #pragma once
 
 #include <cstdint>
+#include <vector>
 
 #include "c8mi/c8mi_tables.h"
 #include "runtime.hpp"
@@ -25,6 +26,21 @@ class DeviceTables {
 
  private:
   DeviceBuffer<unsigned char> storage_;
+  c8mi_native_table_set view_{};
+};
+
+// Host copy of a table set in the same packed layout (the host stepper's
+// tables); needs no device.
+class HostTables {
+ public:
+  explicit HostTables(c8mi_native_table_set const& host);
+  HostTables(HostTables const&) = delete;
+  HostTables& operator=(HostTables const&) = delete;
+
+  c8mi_native_table_set const& view() const noexcept { return view_; }
+
+ private:
+  std::vector<unsigned char> storage_;
   c8mi_native_table_set view_{};
 };
 

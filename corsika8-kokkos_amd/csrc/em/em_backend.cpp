@@ -26,6 +26,8 @@ void upload(DeviceBuffer<T>& buffer, T const* host, std::size_t count) {
   if (count) C8MI_HIP(hipMemcpy(buffer.data(), host, count * sizeof(T), hipMemcpyHostToDevice));
 }
 
+}  // namespace
+
 void validate_environment(c8mi_environment const& e) {
   require(e.node_count > 0 && e.nodes != nullptr, "environment has no nodes");
   require(e.model_count > 0 && e.models != nullptr, "environment has no medium models");
@@ -90,6 +92,8 @@ void validate_config(c8mi_em_config const& c) {
   }
 }
 
+namespace {
+
 void validate_axis(c8mi_em_shower_axis const& a) {
   require(a.grammage_g_per_cm2 != nullptr && a.support_count > 0, "shower axis projection table");
   require(std::isfinite(a.step_m) && a.step_m > 0., "shower axis projection step");
@@ -100,6 +104,42 @@ void validate_axis(c8mi_em_shower_axis const& a) {
 std::uint64_t nonzero(std::uint64_t n) { return n ? n : 1; }
 
 }  // namespace
+
+void fill_context(c8mi_em_config const& config, c8mi_environment const& environment, EmContext& h) {
+  h.node_count = environment.node_count;
+  h.model_count = environment.model_count;
+  h.observation_enabled = environment.observation_enabled;
+  for (int a = 0; a < 3; ++a) {
+    h.observation_point[a] = environment.observation_point_m[a];
+    h.observation_normal[a] = environment.observation_normal[a];
+  }
+  h.maximum_magnetic_deflection = environment.maximum_magnetic_deflection_rad;
+  h.seed = config.random_seed;
+  h.em_cut_mev = config.em_transport_cut_mev;
+  h.muon_cut_mev = config.muon_transport_cut_mev;
+  h.pair_lpm = config.photon_pair_lpm;
+  h.brems_lpm = config.brems_lpm;
+  h.thinning = config.thinning;
+  h.moliere = config.electron_moliere;
+  h.moliere_enabled = config.moliere_enabled;
+  h.moliere_fast_iterations = config.moliere_fast_iterations ? config.moliere_fast_iterations : 12;
+  for (int i = 0; i < 70; ++i) {
+    h.moliere_i1[i] = config.electron_moliere.c1[i] / (2. * i + 1.);
+    h.moliere_i2[i] = config.electron_moliere.c2[i] / (2. * i + 1.);
+  }
+  for (int i = 0; i < 13; ++i) {
+    h.moliere_i2_large[i] = i ? -0.5 * config.electron_moliere.c2_large[i] / i : 0.;
+    h.moliere_half_over[i] = i ? 0.5 / i : 0.;
+  }
+  h.profile_bins = config.profile_bins;
+  h.crossing_mode = config.profile_crossing_mode;
+  h.bin_width = config.profile_bin_width_g_per_cm2;
+  h.energy_loss_threshold = config.profile_energy_loss_threshold_g_per_cm2;
+  h.weight_scale = config.profile_weight_scale;
+  h.energy_scale = config.profile_energy_scale;
+  h.observation_capacity = config.observation_capacity;
+  h.fallback_capacity = config.fallback_capacity;
+}
 
 QueueStorage::QueueStorage(std::uint64_t capacity) : capacity_(capacity) {
   // 4-byte fields first, then 8-byte fields, each array 256-byte aligned
@@ -188,35 +228,11 @@ EmBackend::EmBackend(c8mi_em_config const& config, c8mi_native_table_set const& 
   slot_host_ = std::vector<SlotHost>(slots_);
 
   EmContext& h = host_context_;
+  fill_context(config, environment, h);
   h.tables = tables_.view();
   h.nodes = nodes_.data();
   h.children = children_.data();
   h.models = models_.data();
-  h.node_count = environment.node_count;
-  h.model_count = environment.model_count;
-  h.observation_enabled = environment.observation_enabled;
-  for (int a = 0; a < 3; ++a) {
-    h.observation_point[a] = environment.observation_point_m[a];
-    h.observation_normal[a] = environment.observation_normal[a];
-  }
-  h.maximum_magnetic_deflection = environment.maximum_magnetic_deflection_rad;
-  h.seed = config.random_seed;
-  h.em_cut_mev = config.em_transport_cut_mev;
-  h.muon_cut_mev = config.muon_transport_cut_mev;
-  h.pair_lpm = config.photon_pair_lpm;
-  h.brems_lpm = config.brems_lpm;
-  h.thinning = config.thinning;
-  h.moliere = config.electron_moliere;
-  h.moliere_enabled = config.moliere_enabled;
-  h.moliere_fast_iterations = config.moliere_fast_iterations ? config.moliere_fast_iterations : 12;
-  for (int i = 0; i < 70; ++i) {
-    h.moliere_i1[i] = config.electron_moliere.c1[i] / (2. * i + 1.);
-    h.moliere_i2[i] = config.electron_moliere.c2[i] / (2. * i + 1.);
-  }
-  for (int i = 0; i < 13; ++i) {
-    h.moliere_i2_large[i] = i ? -0.5 * config.electron_moliere.c2_large[i] / i : 0.;
-    h.moliere_half_over[i] = i ? 0.5 / i : 0.;
-  }
   if (config.moliere_enabled && config.electron_moliere_interpolation) {
     c8mi_moliere_interpolation const& t = *config.electron_moliere_interpolation;
     constexpr std::size_t cubics = sizeof(t.polynomials) / sizeof(t.polynomials[0]);
@@ -229,18 +245,10 @@ EmBackend::EmBackend(c8mi_em_config const& config, c8mi_native_table_set const& 
     h.moliere_polynomials = moliere_polynomials_.data();
     h.moliere_guess = moliere_guess_.data();
   }
-  h.profile_bins = config.profile_bins;
-  h.crossing_mode = config.profile_crossing_mode;
-  h.bin_width = config.profile_bin_width_g_per_cm2;
-  h.energy_loss_threshold = config.profile_energy_loss_threshold_g_per_cm2;
-  h.weight_scale = config.profile_weight_scale;
-  h.energy_scale = config.profile_energy_scale;
   h.showers = showers_.data();
   h.shower_slots = slots_;
   h.observations = observations_.data();
-  h.observation_capacity = config.observation_capacity;
   h.fallbacks = fallbacks_.data();
-  h.fallback_capacity = config.fallback_capacity;
   h.record_counts = record_counts_.data();
   h.counters = counters_.data();
   h.profile = profile_.data();
@@ -1127,11 +1135,14 @@ void EmBackend::slot_download_profile(std::uint32_t slot, double* photons, doubl
 // Ledger of a slot from its device counters (the host adds the injected energy
 // it accumulated at enqueue).
 void EmBackend::decode_ledger(long long* c, std::uint32_t slot, c8mi_em_ledger& out) const {
-  c8mi_em_ledger* ledger = &out;
   c[kInjected] += static_cast<long long>(slot_host_[slot].injected);
-  double const es = config_.profile_energy_scale;
+  ledger_from_counters(c, config_.profile_energy_scale, out);
+}
+
+void ledger_from_counters(long long const* c, double energy_scale, c8mi_em_ledger& out) {
+  c8mi_em_ledger* ledger = &out;
   double* energy = &ledger->injected_energy;
-  for (int k = 0; k < kEnergyTerms; ++k) energy[k] = static_cast<double>(c[k]) / es;
+  for (int k = 0; k < kEnergyTerms; ++k) energy[k] = static_cast<double>(c[k]) / energy_scale;
   ledger->steps = static_cast<std::uint64_t>(c[kSteps]);
   ledger->photon_cuts = static_cast<std::uint64_t>(c[kPhotonCuts]);
   for (int k = 0; k < 8; ++k) ledger->lepton_limits[k] = static_cast<std::uint64_t>(c[kLeptonLimit0 + k]);
@@ -1185,8 +1196,14 @@ void EmBackend::serve(c8mi_em_request const* requests, std::uint64_t count, c8mi
       throw std::invalid_argument("em: request arrival times must be finite and non-decreasing");
     if (!std::isfinite(r.footprint_particles) || r.footprint_particles < 0.)
       throw std::invalid_argument("em: request footprint");
-    if (!can_transport(r.primary))
+    if (r.particle_count) {
+      if (!r.particles) throw std::invalid_argument("em: request " + std::to_string(k) + " has a null batch");
+      for (std::uint64_t i = 0; i < r.particle_count; ++i)
+        if (!can_transport(r.particles[i]))
+          throw std::invalid_argument("em: request " + std::to_string(k) + " has a particle the device cannot transport");
+    } else if (!can_transport(r.primary)) {
       throw std::invalid_argument("em: request " + std::to_string(k) + " has a primary the device cannot transport");
+    }
     if (r.axis.grammage_g_per_cm2) {
       validate_axis(r.axis);
       axis_words += r.axis.support_count;
@@ -1249,7 +1266,8 @@ void EmBackend::serve(c8mi_em_request const* requests, std::uint64_t count, c8mi
       inits.push_back(init);
       activate(slot, r.shower_id, r.first_history_id);
       if (slot_radio_) slot_radio_->set_slot(slot, r.radio_observers, r.radio_observer_count);
-      slot_enqueue(slot, &r.primary, 1);
+      if (r.particle_count) slot_enqueue(slot, r.particles, r.particle_count);
+      else slot_enqueue(slot, &r.primary, 1);
       owner[slot] = static_cast<std::int64_t>(next);
       results[next].admitted_s = now;
       results[next].slot = slot;

@@ -27,6 +27,15 @@
 
 namespace c8mi::em {
 
+// Checks of the backend's construction arguments (std::invalid_argument).
+void validate_environment(c8mi_environment const& environment);
+void validate_config(c8mi_em_config const& config);
+// The context's scalar fields from the configuration and environment (every
+// pointer field is left to the caller).
+void fill_context(c8mi_em_config const& config, c8mi_environment const& environment, EmContext& context);
+// The ledger of a shower's int64 counters (energies divided by the scale).
+void ledger_from_counters(long long const* counters, double energy_scale, c8mi_em_ledger& out);
+
 class QueueStorage {
  public:
   explicit QueueStorage(std::uint64_t capacity);

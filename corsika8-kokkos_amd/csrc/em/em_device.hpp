@@ -8,6 +8,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include "em/hd.hpp"
+
 #include <cstdint>
 
 #include "c8mi/c8mi_em.h"
@@ -134,8 +136,8 @@ struct ParticleQueue {
   std::uint32_t* shower;
 };
 
-__device__ __forceinline__ c8mi_particle_state load_particle(ParticleQueue const& q,
-                                                             std::uint64_t i) {
+C8MI_HD c8mi_particle_state load_particle(ParticleQueue const& q,
+                                          std::uint64_t i) {
   c8mi_particle_state p;
   p.pid = q.pid[i];
   p.medium_id = q.medium[i];
@@ -159,8 +161,8 @@ __device__ __forceinline__ c8mi_particle_state load_particle(ParticleQueue const
   return p;
 }
 
-__device__ __forceinline__ void store_particle(ParticleQueue const& q, std::uint64_t i,
-                                               c8mi_particle_state const& p) {
+C8MI_HD void store_particle(ParticleQueue const& q, std::uint64_t i,
+                            c8mi_particle_state const& p) {
   q.pid[i] = p.pid;
   q.medium[i] = p.medium_id;
   q.generation[i] = p.generation;
@@ -190,10 +192,10 @@ enum SlotKind : std::uint32_t { kSlotNone = 0, kSlotNextContinue = 1, kSlotNextC
 // vertex was reached, 12-31 pool slot of the source's shower.
 inline constexpr std::uint32_t kMaxShowerSlots = 4096;
 inline constexpr int kPackedSlotShift = 12;
-__host__ __device__ __forceinline__ std::uint32_t pack_counts(std::uint32_t next, std::uint32_t other,
-                                                              std::uint32_t child, std::uint32_t k0,
-                                                              std::uint32_t k1, bool step, bool interaction,
-                                                              std::uint32_t shower) {
+__host__ C8MI_HD std::uint32_t pack_counts(std::uint32_t next, std::uint32_t other,
+                                           std::uint32_t child, std::uint32_t k0,
+                                           std::uint32_t k1, bool step, bool interaction,
+                                           std::uint32_t shower) {
   return next | (other << 2) | (child << 4) | (k0 << 6) | (k1 << 8) | (step ? 1u << 10 : 0u) |
          (interaction ? 1u << 11 : 0u) | (shower << kPackedSlotShift);
 }

@@ -18,35 +18,35 @@ namespace c8mi::em::env {
 
 enum Status : int { kSuccess = 0, kInvalid = 1, kNonFinite = 2, kOutside = 3, kNoForward = 5, kGrammageOutOfRange = 6 };
 
-__device__ __forceinline__ double dot3(double const a[3], double const b[3]) {
+C8MI_HD double dot3(double const a[3], double const b[3]) {
   return a[0] * b[0] + a[1] * b[1] + a[2] * b[2];
 }
 
-__device__ __forceinline__ bool normalized(double const d[3]) {
+C8MI_HD bool normalized(double const d[3]) {
   double const n = dot3(d, d);
   return isfinite(n) && fabs(n - 1.) <= 1.e-12;
 }
 
-__device__ __forceinline__ c8mi_env_model const* model_of(EmContext const& c, std::int32_t node) {
+C8MI_HD c8mi_env_model const* model_of(EmContext const& c, std::int32_t node) {
   if (node < 0 || static_cast<std::uint32_t>(node) >= c.node_count) return nullptr;
   std::int32_t const m = c.nodes[node].model;
   if (m < 0 || static_cast<std::uint32_t>(m) >= c.model_count) return nullptr;
   return &c.models[m];
 }
 
-__device__ __forceinline__ bool valid_environment(EmContext const& c) {
+C8MI_HD bool valid_environment(EmContext const& c) {
   return c.node_count > 0 && c.model_count > 0 && isfinite(c.maximum_magnetic_deflection) &&
          c.maximum_magnetic_deflection > 0. &&
          c.maximum_magnetic_deflection < 0.5 * 3.141592653589793;
 }
 
-__device__ __forceinline__ bool contains(c8mi_env_node const& s, double const p[3]) {
+C8MI_HD bool contains(c8mi_env_node const& s, double const p[3]) {
   if (s.shape_kind == C8MI_SHAPE_UNIVERSE) return true;
   double const d[3] = {p[0] - s.center_m[0], p[1] - s.center_m[1], p[2] - s.center_m[2]};
   return dot3(d, d) < s.radius_m * s.radius_m;
 }
 
-__device__ __forceinline__ std::int32_t locate(EmContext const& c, double const p[3]) {
+C8MI_HD std::int32_t locate(EmContext const& c, double const p[3]) {
   std::int32_t current = 0;
   for (std::uint32_t hop = 0; hop < c.node_count; ++hop) {
     if (current < 0 || static_cast<std::uint32_t>(current) >= c.node_count) return -2;
@@ -71,8 +71,8 @@ struct Intersections {
   bool has;
 };
 
-__device__ __forceinline__ Intersections intersect(c8mi_env_node const& s, double const p[3],
-                                                   double const v[3]) {
+C8MI_HD Intersections intersect(c8mi_env_node const& s, double const p[3],
+                                double const v[3]) {
   if (s.shape_kind == C8MI_SHAPE_UNIVERSE) return {-__builtin_inf(), __builtin_inf(), true};
   double const d[3] = {p[0] - s.center_m[0], p[1] - s.center_m[1], p[2] - s.center_m[2]};
   double const vv = dot3(v, v), vd = dot3(v, d);
@@ -83,9 +83,9 @@ __device__ __forceinline__ Intersections intersect(c8mi_env_node const& s, doubl
 }
 
 // next_environment_boundary: own exit or nearest child entry.
-__device__ __forceinline__ void next_boundary(EmContext const& c, double const p[3], double const v[3],
-                                     std::int32_t current, double limit, double& parameter,
-                                     std::int32_t& node) {
+C8MI_HD void next_boundary(EmContext const& c, double const p[3], double const v[3],
+                  std::int32_t current, double limit, double& parameter,
+                  std::int32_t& node) {
   double selected = limit;
   std::int32_t selected_node = current;
   c8mi_env_node const& n = c.nodes[current];
@@ -108,10 +108,10 @@ __device__ __forceinline__ void next_boundary(EmContext const& c, double const p
   node = selected_node;
 }
 
-__device__ __forceinline__ double safe_expm1(double v) { return v == 0. ? v : expm1(v); }
-__device__ __forceinline__ double safe_log1p(double v) { return v == 0. ? v : log1p(v); }
+C8MI_HD double safe_expm1(double v) { return v == 0. ? v : expm1(v); }
+C8MI_HD double safe_log1p(double v) { return v == 0. ? v : log1p(v); }
 
-__device__ __forceinline__ double density(c8mi_env_model const& m, double const p[3]) {
+C8MI_HD double density(c8mi_env_model const& m, double const p[3]) {
   if (m.density_kind == C8MI_DENSITY_HOMOGENEOUS) return m.reference_density_g_per_cm3;
   double const d[3] = {p[0] - m.anchor_m[0], p[1] - m.anchor_m[1], p[2] - m.anchor_m[2]};
   double const h = m.density_kind == C8MI_DENSITY_SLIDING_PLANAR_EXPONENTIAL ? sqrt(dot3(d, d))
@@ -119,15 +119,15 @@ __device__ __forceinline__ double density(c8mi_env_model const& m, double const 
   return m.reference_density_g_per_cm3 * exp(m.inverse_scale_length_per_m * (h - m.reference_height_m));
 }
 
-__device__ __forceinline__ double projected_axis(c8mi_env_model const& m, double const p[3],
-                                                 double const dir[3]) {
+C8MI_HD double projected_axis(c8mi_env_model const& m, double const p[3],
+                              double const dir[3]) {
   if (m.density_kind == C8MI_DENSITY_FLAT_EXPONENTIAL) return dot3(dir, m.axis);
   double const d[3] = {p[0] - m.anchor_m[0], p[1] - m.anchor_m[1], p[2] - m.anchor_m[2]};
   return dot3(d, dir) / sqrt(dot3(d, d));
 }
 
-__device__ __forceinline__ double model_grammage(c8mi_env_model const& m, double const p[3],
-                                                 double const dir[3], double length) {
+C8MI_HD double model_grammage(c8mi_env_model const& m, double const p[3],
+                              double const dir[3], double length) {
   if (length == 0.) return 0.;
   double const rho = density(m, p);
   if (m.density_kind == C8MI_DENSITY_HOMOGENEOUS) return length * rho * 100.;
@@ -136,8 +136,8 @@ __device__ __forceinline__ double model_grammage(c8mi_env_model const& m, double
   return rho * (m.scale_length_m / proj) * safe_expm1(proj * length * m.inverse_scale_length_per_m) * 100.;
 }
 
-__device__ __forceinline__ double model_arclength(c8mi_env_model const& m, double const p[3],
-                                                  double const dir[3], double grammage) {
+C8MI_HD double model_arclength(c8mi_env_model const& m, double const p[3],
+                               double const dir[3], double grammage) {
   double const rho = density(m, p);
   if (m.density_kind == C8MI_DENSITY_HOMOGENEOUS) return grammage / (rho * 100.);
   double const proj = projected_axis(m, p, dir);
@@ -152,8 +152,8 @@ struct Layer {
   double density;
 };
 
-__device__ __forceinline__ Layer query(EmContext const& c, double const p[3], double const* dir,
-                              std::int32_t logical) {
+C8MI_HD Layer query(EmContext const& c, double const p[3], double const* dir,
+           std::int32_t logical) {
   if (!valid_environment(c)) return {kInvalid, -1, 0.};
   for (int a = 0; a < 3; ++a)
     if (!isfinite(p[a]) || (dir && !isfinite(dir[a]))) return {kNonFinite, -1, 0.};
@@ -171,8 +171,8 @@ struct Boundary {
   std::int32_t next_node;
 };
 
-__device__ __forceinline__ Boundary distance_to_boundary(EmContext const& c, double const p[3],
-                                                double const dir[3], std::int32_t logical) {
+C8MI_HD Boundary distance_to_boundary(EmContext const& c, double const p[3],
+                             double const dir[3], std::int32_t logical) {
   Layer const l = query(c, p, dir, logical);
   if (l.status != kSuccess) return {l.status, 0., -3};
   double param;
@@ -187,8 +187,8 @@ struct Grammage {
   double value;
 };
 
-__device__ __forceinline__ Grammage grammage(EmContext const& c, std::int32_t node, double const p[3],
-                                    double const dir[3], double distance) {
+C8MI_HD Grammage grammage(EmContext const& c, std::int32_t node, double const p[3],
+                 double const dir[3], double distance) {
   c8mi_env_model const* m = model_of(c, node);
   if (!valid_environment(c) || !m || !isfinite(distance) || distance < 0. || !normalized(dir))
     return {kNonFinite, 0.};
@@ -196,8 +196,8 @@ __device__ __forceinline__ Grammage grammage(EmContext const& c, std::int32_t no
   return {isfinite(x) && x >= 0. ? kSuccess : kGrammageOutOfRange, x};
 }
 
-__device__ __forceinline__ Grammage arclength(EmContext const& c, std::int32_t node, double const p[3],
-                                     double const dir[3], double x) {
+C8MI_HD Grammage arclength(EmContext const& c, std::int32_t node, double const p[3],
+                  double const dir[3], double x) {
   c8mi_env_model const* m = model_of(c, node);
   if (!valid_environment(c) || !m || !isfinite(x) || x < 0. || !normalized(dir))
     return {kNonFinite, 0.};
@@ -205,9 +205,9 @@ __device__ __forceinline__ Grammage arclength(EmContext const& c, std::int32_t n
   return {isfinite(l) && l >= 0. ? kSuccess : kGrammageOutOfRange, l};
 }
 
-__device__ __forceinline__ Grammage competition_grammage(EmContext const& c, std::int32_t node,
-                                                double const p[3], double const dir[3],
-                                                double distance) {
+C8MI_HD Grammage competition_grammage(EmContext const& c, std::int32_t node,
+                             double const p[3], double const dir[3],
+                             double distance) {
   Grammage r = grammage(c, node, p, dir, distance);
   if (r.status == kGrammageOutOfRange && r.value == __builtin_huge_val() && isfinite(distance) &&
       distance > 0.) {
@@ -225,8 +225,8 @@ __device__ __forceinline__ Grammage competition_grammage(EmContext const& c, std
 }
 
 // Straight intersection with the observation plane; -1 when there is none.
-__device__ __forceinline__ double observation_distance(EmContext const& c, double const p[3],
-                                              double const dir[3]) {
+C8MI_HD double observation_distance(EmContext const& c, double const p[3],
+                           double const dir[3]) {
   if (!c.observation_enabled) return -1.;
   double ns = 0.;
   for (int a = 0; a < 3; ++a) {

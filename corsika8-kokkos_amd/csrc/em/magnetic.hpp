@@ -33,22 +33,22 @@ struct DD {
   double hi, lo;
 };
 
-__device__ __forceinline__ DD dd(double v) { return {v, 0.}; }
-__device__ __forceinline__ double to_double(DD a) { return a.hi + a.lo; }
+C8MI_HD DD dd(double v) { return {v, 0.}; }
+C8MI_HD double to_double(DD a) { return a.hi + a.lo; }
 // Fast two-sum of a and b, |a| >= |b| or a = 0.
-__device__ __forceinline__ DD renormalize(double a, double b) {
+C8MI_HD DD renormalize(double a, double b) {
   double const s = a + b;
   return {s, b - (s - a)};
 }
-__device__ __forceinline__ DD operator+(DD a, DD b) {
+C8MI_HD DD operator+(DD a, DD b) {
   double const s = a.hi + b.hi;
   double const bv = s - a.hi;
   double const e = (a.hi - (s - bv)) + (b.hi - bv);
   return renormalize(s, e + (a.lo + b.lo));
 }
-__device__ __forceinline__ DD operator-(DD a) { return {-a.hi, -a.lo}; }
-__device__ __forceinline__ DD operator-(DD a, DD b) { return a + (-b); }
-__device__ __forceinline__ DD operator*(DD a, DD b) {
+C8MI_HD DD operator-(DD a) { return {-a.hi, -a.lo}; }
+C8MI_HD DD operator-(DD a, DD b) { return a + (-b); }
+C8MI_HD DD operator*(DD a, DD b) {
   double const p = a.hi * b.hi;
   double e = fma(a.hi, b.hi, -p);  // the exact rounding error of p
   e = e + (a.hi * b.lo + a.lo * b.hi);
@@ -56,7 +56,7 @@ __device__ __forceinline__ DD operator*(DD a, DD b) {
   return renormalize(p, e);
 }
 // Long division with two correction terms.
-__device__ __forceinline__ DD operator/(DD a, DD b) {
+C8MI_HD DD operator/(DD a, DD b) {
   double const q = a.hi / b.hi;
   if (!isfinite(q)) return dd(q);
   DD r = a - b * dd(q);
@@ -64,33 +64,33 @@ __device__ __forceinline__ DD operator/(DD a, DD b) {
   r = r - b * dd(c);
   return (dd(q) + dd(c)) + dd(r.hi / b.hi);
 }
-__device__ __forceinline__ bool operator<(DD a, DD b) { return a.hi < b.hi || (a.hi == b.hi && a.lo < b.lo); }
-__device__ __forceinline__ bool operator>(DD a, DD b) { return b < a; }
-__device__ __forceinline__ bool operator==(DD a, DD b) { return a.hi == b.hi && a.lo == b.lo; }
-__device__ __forceinline__ DD dd_abs(DD a) { return a < dd(0.) ? -a : a; }
+C8MI_HD bool operator<(DD a, DD b) { return a.hi < b.hi || (a.hi == b.hi && a.lo < b.lo); }
+C8MI_HD bool operator>(DD a, DD b) { return b < a; }
+C8MI_HD bool operator==(DD a, DD b) { return a.hi == b.hi && a.lo == b.lo; }
+C8MI_HD DD dd_abs(DD a) { return a < dd(0.) ? -a : a; }
 // One Newton correction of the double square root.
-__device__ __forceinline__ DD dd_sqrt(DD v) {
+C8MI_HD DD dd_sqrt(DD v) {
   double const a = sqrt(v.hi);
   if (a == 0. || !isfinite(a)) return dd(a);
   return dd(a) + (v - dd(a) * dd(a)) / dd(2. * a);
 }
 // Two Newton corrections of the double cube root.
-__device__ __forceinline__ DD dd_cbrt(DD v) {
+C8MI_HD DD dd_cbrt(DD v) {
   DD a = dd(cbrt(v.hi));
   if (a.hi == 0. || !isfinite(a.hi)) return a;
   for (int it = 0; it < 2; ++it) a = a + (v - a * a * a) / (dd(3.) * a * a);
   return a;
 }
-__device__ __forceinline__ DD dd_acos(DD v) {
+C8MI_HD DD dd_acos(DD v) {
   double const angle = acos(v.hi);
   if (v.lo == 0.) return dd(angle);
   if (v.hi == 1.) return dd_sqrt(dd(2.) * (dd(1.) - v));
   if (v.hi == -1.) return dd(3.14159265358979323846) - dd_sqrt(dd(2.) * (dd(1.) + v));
   return dd(angle) - dd(v.lo / sqrt((1. - v.hi) * (1. + v.hi)));
 }
-__device__ __forceinline__ DD dd_cos(DD v) { return dd(cos(v.hi)) - dd(sin(v.hi) * v.lo); }
+C8MI_HD DD dd_cos(DD v) { return dd(cos(v.hi)) - dd(sin(v.hi) * v.lo); }
 // fma of the double-double type: a product and a sum, each double-double.
-__device__ __forceinline__ DD dd_fma(DD a, DD b, DD c) { return a * b + c; }
+C8MI_HD DD dd_fma(DD a, DD b, DD c) { return a * b + c; }
 
 // ---- real roots (at most four, as doubles) ----
 // The general solvers stay out of line: they run only near a boundary in a
@@ -101,9 +101,9 @@ struct Roots {
   unsigned n;
 };
 
-__device__ __forceinline__ void append(Roots& r, double x) { r.v[r.n++] = x; }
+C8MI_HD void append(Roots& r, double x) { r.v[r.n++] = x; }
 // Skips a value within 8 ulp-scale of one already present; keeps at most four.
-__device__ __forceinline__ void append_distinct(Roots& r, double x) {
+C8MI_HD void append_distinct(Roots& r, double x) {
   for (unsigned i = 0; i < r.n; ++i)
     if (fabs(x - r.v[i]) <= 8 * 2.220446049250313e-16 * fmax(fabs(x), fabs(r.v[i]))) return;
   if (r.n < 4) append(r, x);
@@ -112,7 +112,7 @@ __device__ __forceinline__ void append_distinct(Roots& r, double x) {
 // a x^2 + b x + c = 0 (PolynomialRoots.inl:104-134): the linear cases when a or
 // c is below epsilon (inline: the observation plane's SI coefficients always
 // take the first), else the cancellation-free pair (out of line).
-__device__ __noinline__ Roots quadratic_pair(DD a, DD b, DD c, double epsilon) {
+C8MI_HD_NOINLINE Roots quadratic_pair(DD a, DD b, DD c, double epsilon) {
   Roots r{{0., 0., 0., 0.}, 0u};
   DD const multiplied = a * dd(4.) * c;
   DD const error = dd_fma(a * dd(4.), c, -multiplied);
@@ -129,7 +129,7 @@ __device__ __noinline__ Roots quadratic_pair(DD a, DD b, DD c, double epsilon) {
   return r;
 }
 
-__device__ __forceinline__ Roots quadratic_roots(DD a, DD b, DD c, double epsilon) {
+C8MI_HD Roots quadratic_roots(DD a, DD b, DD c, double epsilon) {
   Roots r{{0., 0., 0., 0.}, 0u};
   if (dd_abs(a) < dd(epsilon)) {
     double const bb = to_double(b), cc = to_double(c);
@@ -146,7 +146,7 @@ __device__ __forceinline__ Roots quadratic_roots(DD a, DD b, DD c, double epsilo
 }
 
 // x^2 + b x + c = 0 without cancellation (StableQuartic.inl stable_monic_quadratic).
-__device__ __forceinline__ Roots monic_quadratic(DD b, DD c) {
+C8MI_HD Roots monic_quadratic(DD b, DD c) {
   Roots r{{0., 0., 0., 0.}, 0u};
   if (c == dd(0.)) {
     append(r, 0.);
@@ -166,7 +166,7 @@ __device__ __forceinline__ Roots monic_quadratic(DD b, DD c) {
   return r;
 }
 
-__device__ __forceinline__ DD halley(DD root, DD p, DD quadratic, DD negative_constant) {
+C8MI_HD DD halley(DD root, DD p, DD quadratic, DD negative_constant) {
   for (int it = 0; it < 4; ++it) {
     DD const value = ((root + p) * root + quadratic) * root - negative_constant;
     DD const derivative = (dd(3.) * root + dd(2.) * p) * root + quadratic;
@@ -179,7 +179,7 @@ __device__ __forceinline__ DD halley(DD root, DD p, DD quadratic, DD negative_co
 // The positive root of Ferrari's resolvent cubic of x^4 + p x^2 + q x + r
 // (stable_resolvent): a Halley-refined small root where it is the only positive
 // one, else Cardano or the trigonometric form.
-__device__ __forceinline__ DD resolvent(DD p, DD q, DD r) {
+C8MI_HD DD resolvent(DD p, DD q, DD r) {
   DD const quadratic = p * p / dd(4.) - r;
   DD const negative_constant = q * q / dd(8.);
   if (p < dd(0.) && r < dd(0.) && negative_constant < (-p) * quadratic / dd(64.) &&
@@ -214,7 +214,7 @@ __device__ __forceinline__ DD resolvent(DD p, DD q, DD r) {
 // by a power of two, biquadratic when q = 0, else Ferrari's factorisation into
 // two monic quadratics; a non-positive resolvent yields the single root +inf
 // (the caller's incomplete-roots signal).
-__device__ __noinline__ Roots quartic_roots(double p, double q, double r) {
+C8MI_HD_NOINLINE Roots quartic_roots(double p, double q, double r) {
   Roots out{{0., 0., 0., 0.}, 0u};
   int pe = -4096, qe = -4096, re = -4096;
   if (p != 0.) {
@@ -295,7 +295,7 @@ struct Limit {
   double gyroradius_m;
 };
 
-__device__ __forceinline__ bool valid_particle(c8mi_particle_state const& p, double mass_gev, double charge) {
+C8MI_HD bool valid_particle(c8mi_particle_state const& p, double mass_gev, double charge) {
   if (!isfinite(p.energy_gev) || !(p.energy_gev > mass_gev) || !isfinite(mass_gev) || !(mass_gev > 0.) ||
       !isfinite(charge))
     return false;
@@ -305,15 +305,15 @@ __device__ __forceinline__ bool valid_particle(c8mi_particle_state const& p, dou
   return isfinite(n2) && fabs(n2 - 1.) <= 1.e-12;
 }
 
-__device__ __forceinline__ bool valid_field(double const b[3]) {
+C8MI_HD bool valid_field(double const b[3]) {
   return isfinite(b[0]) && isfinite(b[1]) && isfinite(b[2]);
 }
 
 // maximum_uniform_magnetic_step: the leapfrog length that turns the direction
 // by at most `max_deflection` on the gyro circle, 2 cos(d) sin(d) R; Linear
 // (infinite) for neutral particles, no field, p_perp < 1 eV or R > 1e9 m.
-__device__ __forceinline__ Limit step_limit(c8mi_particle_state const& p, double mass_gev, double charge,
-                                            double const b[3], double max_deflection) {
+C8MI_HD Limit step_limit(c8mi_particle_state const& p, double mass_gev, double charge,
+                         double const b[3], double max_deflection) {
   if (!valid_particle(p, mass_gev, charge) || !valid_field(b) || !isfinite(max_deflection) ||
       !(max_deflection > 0.) || !(max_deflection < 0.5 * 3.141592653589793))
     return {kInvalidInput, 0., 0.};
@@ -336,8 +336,8 @@ __device__ __forceinline__ Limit step_limit(c8mi_particle_state const& p, double
 // advance_uniform_magnetic_field: one leapfrog step of length `distance`: the
 // direction gets the full kick (q/p) L (u x B), the position moves along the
 // half-kicked direction; the time advances by L / (beta c).
-__device__ __forceinline__ int advance(c8mi_particle_state const& start, double mass_gev, double charge,
-                                       double const b[3], double distance, c8mi_particle_state& end) {
+C8MI_HD int advance(c8mi_particle_state const& start, double mass_gev, double charge,
+                    double const b[3], double distance, c8mi_particle_state& end) {
   if (!valid_particle(start, mass_gev, charge) || !valid_field(b) || !isfinite(distance) || distance < 0.)
     return kInvalidInput;
   end = start;
@@ -372,8 +372,8 @@ struct Curved {
   mutable bool unsupported, incomplete;
 };
 
-__device__ __forceinline__ Curved make_curved(c8mi_particle_state const& p, double mass_gev, double charge,
-                                              double const b[3], Limit const& limit, bool linear) {
+C8MI_HD Curved make_curved(c8mi_particle_state const& p, double mass_gev, double charge,
+                           double const b[3], Limit const& limit, bool linear) {
   Curved a;
   double const momentum = sqrt((p.energy_gev - mass_gev) * (p.energy_gev + mass_gev));
   a.speed = kSpeedOfLight * momentum / p.energy_gev;
@@ -393,9 +393,9 @@ __device__ __forceinline__ Curved make_curved(c8mi_particle_state const& p, doub
 
 // Time parameters of the entry and exit of a sphere (or the universe), or of a
 // plane (center/normal) when `plane` is set.
-__device__ __forceinline__ env::Intersections curved_intersect(Curved const& a, std::uint32_t shape_kind,
-                                                               double const center[3], double radius,
-                                                               double const normal[3], bool plane) {
+C8MI_HD env::Intersections curved_intersect(Curved const& a, std::uint32_t shape_kind,
+                                            double const center[3], double radius,
+                                            double const normal[3], bool plane) {
   env::Intersections const none{0., 0., false};
   double const inf = __builtin_inf();
   auto straight = [&] {
@@ -473,7 +473,7 @@ __device__ __forceinline__ env::Intersections curved_intersect(Curved const& a, 
   return {entry / a.speed, exit / a.speed, true};
 }
 
-__device__ __forceinline__ env::Intersections curved_node(EmContext const& c, Curved const& a, std::int32_t node) {
+C8MI_HD env::Intersections curved_node(EmContext const& c, Curved const& a, std::int32_t node) {
   c8mi_env_node const& n = c.nodes[node];
   double const zero[3] = {0., 0., 0.};
   return curved_intersect(a, n.shape_kind, n.center_m, n.radius_m, zero, false);
@@ -481,9 +481,9 @@ __device__ __forceinline__ env::Intersections curved_node(EmContext const& c, Cu
 
 // lepton_boundary: next_environment_boundary over the curved access with the
 // magnetic limit as the parameter bound; distance, next node and failure.
-__device__ __forceinline__ bool lepton_boundary(EmContext const& c, c8mi_particle_state const& p, double mass_gev,
-                                                double charge, double const b[3], Limit const& limit,
-                                                double& distance, std::int32_t& next) {
+C8MI_HD bool lepton_boundary(EmContext const& c, c8mi_particle_state const& p, double mass_gev,
+                             double charge, double const b[3], Limit const& limit,
+                             double& distance, std::int32_t& next) {
   Curved const a = make_curved(p, mass_gev, charge, b, limit, limit.status == kLinear);
   std::int32_t const current = p.environment_node;
   double selected = limit.distance_m / a.speed;
@@ -511,9 +511,9 @@ __device__ __forceinline__ bool lepton_boundary(EmContext const& c, c8mi_particl
 
 // lepton_observation_distance: the curved trajectory always (also where the
 // boundary search ran straight); +inf when the plane is behind or missed.
-__device__ __forceinline__ double lepton_observation_distance(EmContext const& c, c8mi_particle_state const& p,
-                                                              double mass_gev, double charge, double const b[3],
-                                                              Limit const& limit) {
+C8MI_HD double lepton_observation_distance(EmContext const& c, c8mi_particle_state const& p,
+                                           double mass_gev, double charge, double const b[3],
+                                           Limit const& limit) {
   Curved const a = make_curved(p, mass_gev, charge, b, limit, false);
   env::Intersections const x =
       curved_intersect(a, 0u, c.observation_point, 0., c.observation_normal, true);

@@ -15,6 +15,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include "em/hd.hpp"
+
 #include <cstdint>
 
 #include "c8mi/c8mi_em.h"
@@ -41,7 +43,7 @@ struct State {
 };
 
 // Acklam's rational approximation of the normal quantile (the Newton seed).
-__device__ __forceinline__ double normal_ppf(double p) {
+C8MI_HD double normal_ppf(double p) {
   constexpr double a0 = -3.969683028665376e+01, a1 = 2.209460984245205e+02, a2 = -2.759285104469687e+02,
                    a3 = 1.383577518672690e+02, a4 = -3.066479806614716e+01, a5 = 2.506628277459239e+00;
   constexpr double b0 = -5.447609879822406e+01, b1 = 1.615858368580409e+02, b2 = -1.556989798598866e+02,
@@ -65,7 +67,7 @@ __device__ __forceinline__ double normal_ppf(double p) {
 }
 
 // B - ln B = rhs on the B > 1 branch: fourth-order asymptotic seed, one Halley step.
-__device__ __forceinline__ double solve_b(double rhs) {
+C8MI_HD double solve_b(double rhs) {
   if (!isfinite(rhs) || rhs < kMinimumBRhs) return 0.;
   double const l = log(rhs), ir = 1. / rhs, ir2 = ir * ir, l2 = l * l, l3 = l2 * l;
   double b = rhs + l + l * ir + l * (2. - l) * 0.5 * ir2 + l * (6. - 9. * l + 2. * l2) / 6. * ir2 * ir +
@@ -100,7 +102,7 @@ inline constexpr int kIntervals = C8MI_MOLIERE_INTERVALS;
 
 // The snapshot was validated when the backend was created.
 template <int Cap>
-__device__ __forceinline__ int build(c8mi_moliere_snapshot const& m, double grammage, double e_mev, State<Cap>& st) {
+C8MI_HD int build(c8mi_moliere_snapshot const& m, double grammage, double e_mev, State<Cap>& st) {
   if (!isfinite(grammage) || grammage < 0. || !isfinite(e_mev) || !(e_mev > m.particle_mass_mev))
     return kInvalidInput;
   if (grammage == 0.) return kNoDeflection;
@@ -144,40 +146,40 @@ __device__ __forceinline__ int build(c8mi_moliere_snapshot const& m, double gram
 // Large-x branches of the four functions (the power series cover x <= 12 for
 // f1 and its integral, x <= 4.25^2 for f2 and its integral): rare tails, kept
 // out of line so the step kernel's code stays small.
-__device__ __noinline__ double f1_large(double x) {
+C8MI_HD_NOINLINE double f1_large(double x) {
   return 0.5 * sqrt(kPi) / (pow(x, 1.5) * pow(1. - 4.5 / x, 2. / 3.));
 }
-__device__ __noinline__ double integral_f1_large(c8mi_moliere_snapshot const& m, double x) {
+C8MI_HD_NOINLINE double integral_f1_large(c8mi_moliere_snapshot const& m, double x) {
   double sum = m.c1_integral_large[14];
   for (int i = 13; i >= 0; --i) sum = m.c1_integral_large[i] + sum / x;
   return sum;
 }
 // f2 and its integral between 4.25^2 and 6.5^2 (fitted), and the terms of
 // their expansions above.
-__device__ __forceinline__ double f2_fit(double x) {
+C8MI_HD double f2_fit(double x) {
   return 0.00013567765224589459194769192063035 * x - 0.0022635502525409950842771866774683 * sqrt(x) +
          0.0098037758070269476889935233998585;
 }
-__device__ __forceinline__ double integral_f2_fit(double x) {
+C8MI_HD double integral_f2_fit(double x) {
   return -0.00026360133958801203364619158975302 * x + 0.0039965027465457608410459577896745 * sqrt(x) -
          0.016305842044996649714549974419242;
 }
-__device__ __forceinline__ double f2_term(c8mi_moliere_snapshot const& m, double x, int i) {
+C8MI_HD double f2_term(c8mi_moliere_snapshot const& m, double x, int i) {
   return m.c2_large[i] * (0.5 * log(x) + m.s2_large[i]) * pow(x, -(i + 0.5));
 }
 // pow(x, -i) with an int exponent is pown here (measured cheaper than pow with
 // a double exponent in the step kernel).
-__device__ __forceinline__ double integral_f2_term(c8mi_moliere_snapshot const& m, Quotients const& q, double x,
-                                                   int i) {
+C8MI_HD double integral_f2_term(c8mi_moliere_snapshot const& m, Quotients const& q, double x,
+                                int i) {
   return q.i2_large[i] * (q.half_over[i] + 0.5 * log(x) + m.s2_large[i]) * pow(x, -i);
 }
-__device__ __noinline__ double f2_large(c8mi_moliere_snapshot const& m, double x) {
+C8MI_HD_NOINLINE double f2_large(c8mi_moliere_snapshot const& m, double x) {
   if (x <= 6.5 * 6.5) return f2_fit(x);
   double sum = 0.;
   for (int i = 2; i < 13; ++i) sum += f2_term(m, x, i);
   return sum;
 }
-__device__ __noinline__ double integral_f2_large(c8mi_moliere_snapshot const& m, Quotients const& q, double x) {
+C8MI_HD_NOINLINE double integral_f2_large(c8mi_moliere_snapshot const& m, Quotients const& q, double x) {
   if (x <= 6.5 * 6.5) return integral_f2_fit(x);
   double sum = 0.;
   for (int i = 2; i < 13; ++i) sum += integral_f2_term(m, q, x, i);
@@ -189,8 +191,8 @@ __device__ __noinline__ double integral_f2_large(c8mi_moliere_snapshot const& m,
 // 4N independent Horner chains share each coefficient load and fill the FP64
 // pipeline, each chain with the same operations in the same order as alone.
 template <int N>
-__device__ __forceinline__ void evaluate(c8mi_moliere_snapshot const& m, Quotients const& q, State<N> const& st,
-                                         double theta, double& density, double& cumulative) {
+C8MI_HD void evaluate(c8mi_moliere_snapshot const& m, Quotients const& q, State<N> const& st,
+                      double theta, double& density, double& cumulative) {
   double const t2 = theta * theta;
   double x[N], s1[N], s2[N], si1[N], si2[N];
 #pragma unroll
@@ -233,9 +235,9 @@ __device__ __forceinline__ void evaluate(c8mi_moliere_snapshot const& m, Quotien
 // for x <= 20 (the four functions share the interval), the large-x branches
 // above (x > 20 is past both branch points of the series).
 template <int N>
-__device__ __forceinline__ void evaluate_table(c8mi_moliere_snapshot const& m, Quotients const& q, Table const& t,
-                                               State<N> const& st, double theta, double& density,
-                                               double& cumulative) {
+C8MI_HD void evaluate_table(c8mi_moliere_snapshot const& m, Quotients const& q, Table const& t,
+                            State<N> const& st, double theta, double& density,
+                            double& cumulative) {
   double const t2 = theta * theta;
   double dens = 0., cdf = 0.;
 #pragma unroll 1
@@ -276,7 +278,7 @@ __device__ __forceinline__ void evaluate_table(c8mi_moliere_snapshot const& m, Q
 
 // improve_moliere_initial_guess (MoliereScattering.inl:136-216): the Gaussian
 // coordinate plus the trilinear interpolation of the tabulated correction.
-__device__ __forceinline__ double improve_guess(double const* g, double reference_b, double beta2, double gaussian) {
+C8MI_HD double improve_guess(double const* g, double reference_b, double beta2, double gaussian) {
   constexpr double b_lo = 4.5, b_hi = 64., beta_lo = 0.25, beta_hi = 1., c_lo = 0., c_hi = 5.;
   constexpr std::uint32_t nb = C8MI_MOLIERE_GUESS_B_NODES, nt = C8MI_MOLIERE_GUESS_BETA2_NODES,
                           nc = C8MI_MOLIERE_GUESS_COORDINATE_NODES;
@@ -311,8 +313,8 @@ __device__ __forceinline__ double improve_guess(double const* g, double referenc
 // Newton start (the Gaussian quantile, corrected by the tabulated guess).
 inline constexpr int kStarted = -1;  // Newton iteration to follow
 template <bool Tab, int Cap>
-__device__ __forceinline__ int newton_start(c8mi_moliere_snapshot const& m, Table const& t, State<Cap> const& st,
-                                            double u, double& current) {
+C8MI_HD int newton_start(c8mi_moliere_snapshot const& m, Table const& t, State<Cap> const& st,
+                         double u, double& current) {
   current = 0.;
   if (!isfinite(u) || !(u > 0.) || !(u < 1.)) return kInvalidInput;
   if (u == 0.5) return kSuccess;
@@ -328,7 +330,7 @@ __device__ __forceinline__ int newton_start(c8mi_moliere_snapshot const& m, Tabl
 
 // One Newton update of sample_one_dimensional: `current` becomes the next
 // iterate; kStarted while not converged.
-__device__ __forceinline__ int newton_update(double density, double cdf, double target, double& current) {
+C8MI_HD int newton_update(double density, double cdf, double target, double& current) {
   double const previous = current;
   if (!isfinite(density) || density == 0.) return kNonFinite;  // current stays the previous iterate
   current = previous - (cdf - target) / density;
@@ -344,9 +346,9 @@ inline constexpr int kPending = -2;  // not converged within the iterations allo
 // kSuccess, kNonFinite, kNonConvergent after iteration 100, or kPending after
 // `last` < 100.
 template <bool Tab, int Cap>
-__device__ __forceinline__ int newton_iterate(c8mi_moliere_snapshot const& m, Quotients const& q, Table const& t,
-                                              State<Cap> const& st, double u, int first, int last,
-                                              double& current) {
+C8MI_HD int newton_iterate(c8mi_moliere_snapshot const& m, Quotients const& q, Table const& t,
+                           State<Cap> const& st, double u, int first, int last,
+                           double& current) {
   double const target = u - 0.5;
   for (int it = first; it <= last; ++it) {
     double density, cdf;
@@ -359,8 +361,8 @@ __device__ __forceinline__ int newton_iterate(c8mi_moliere_snapshot const& m, Qu
 }
 
 template <bool Tab, int Cap>
-__device__ __forceinline__ int one_dimensional(c8mi_moliere_snapshot const& m, Quotients const& q, Table const& t,
-                                               State<Cap> const& st, double u, double& angle) {
+C8MI_HD int one_dimensional(c8mi_moliere_snapshot const& m, Quotients const& q, Table const& t,
+                            State<Cap> const& st, double u, double& angle) {
   int const status = newton_start<Tab>(m, t, st, u, angle);
   if (status != kStarted) return status;
   return newton_iterate<Tab>(m, q, t, st, u, 1, kMaximumIterations, angle);
@@ -391,7 +393,7 @@ inline constexpr int kJobSlots = kPowJobs + kPownJobs + 3;
 inline constexpr int kTailThreads = 3 * 64;
 
 // Horner of a 70-coefficient power series (the order of each chain of evaluate).
-__device__ __forceinline__ double series70(double const* c, double x) {
+C8MI_HD double series70(double const* c, double x) {
   double s = c[69];
   for (int i = 68; i >= 0; --i) s = s * x + c[i];
   return s;
@@ -513,8 +515,8 @@ __device__ __forceinline__ int newton_iterate_wave(c8mi_moliere_snapshot const& 
 
 // Space angle of two projected draws (sample_moliere_scattering_angle_2d).
 template <bool Tab, int Cap>
-__device__ __forceinline__ int angle_2d_for(c8mi_moliere_snapshot const& m, Quotients const& q, Table const& t,
-                                            double grammage, double e_mev, double u1, double u2, double& angle) {
+C8MI_HD int angle_2d_for(c8mi_moliere_snapshot const& m, Quotients const& q, Table const& t,
+                         double grammage, double e_mev, double u1, double u2, double& angle) {
   State<Cap> st;
   angle = 0.;
   int status = build(m, grammage, e_mev, st);
@@ -534,8 +536,8 @@ inline constexpr int kDeviceComponents = 4;
 static_assert(kDeviceComponents * kPowJobs <= 64 && kDeviceComponents * kPownJobs <= 64 && kDeviceComponents * 3 <= 64);
 
 // apply_moliere_direction_cpu_step_compatible: normalize(tracked + rotate(start) - start).
-__device__ __forceinline__ int deflect_end(double const start[3], double const tracked[3], double angle, double au,
-                                           double out[3]) {
+C8MI_HD int deflect_end(double const start[3], double const tracked[3], double angle, double au,
+                        double out[3]) {
   if (!isfinite(angle) || angle < 0. || !isfinite(au) || !(au > 0.) || !(au < 1.)) return kInvalidInput;
   constexpr double lo = (1. - 1.e-12) * (1. - 1.e-12), hi = (1. + 1.e-12) * (1. + 1.e-12);
   double const n2 = start[0] * start[0] + start[1] * start[1] + start[2] * start[2];

@@ -12,6 +12,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include "em/hd.hpp"
+
 #include <cmath>
 #include <cstdint>
 
@@ -28,8 +30,8 @@ struct QueryResult {
   double value;
 };
 
-__device__ __forceinline__ QueryResult qr(std::uint32_t status, double value = 0., std::uint32_t newton = 0,
-                                 std::uint32_t bisection = 0) {
+C8MI_HD QueryResult qr(std::uint32_t status, double value = 0., std::uint32_t newton = 0,
+              std::uint32_t bisection = 0) {
   return {status, newton, bisection, value};
 }
 
@@ -37,7 +39,7 @@ namespace detail {
 
 constexpr double kLn2 = 0.693147180559945309417232121458176568;
 
-__device__ __forceinline__ double axis_transform(c8mi_native_axis const& a, double x) {
+C8MI_HD double axis_transform(c8mi_native_axis const& a, double x) {
   switch (a.type) {
     case C8MI_AXIS_LINEAR: return (x - a.low) / a.step;
     case C8MI_AXIS_EXPONENTIAL: return log(x / a.low) / a.step;
@@ -46,7 +48,7 @@ __device__ __forceinline__ double axis_transform(c8mi_native_axis const& a, doub
   return __builtin_nan("");
 }
 
-__device__ __forceinline__ double axis_back_transform(c8mi_native_axis const& a, double c) {
+C8MI_HD double axis_back_transform(c8mi_native_axis const& a, double c) {
   switch (a.type) {
     case C8MI_AXIS_LINEAR: return c * a.step + a.low;
     case C8MI_AXIS_EXPONENTIAL: return a.low * exp(c * a.step);
@@ -55,7 +57,7 @@ __device__ __forceinline__ double axis_back_transform(c8mi_native_axis const& a,
   return __builtin_nan("");
 }
 
-__device__ __forceinline__ double axis_derivative(c8mi_native_axis const& a, double x) {
+C8MI_HD double axis_derivative(c8mi_native_axis const& a, double x) {
   switch (a.type) {
     case C8MI_AXIS_LINEAR: return 1. / a.step;
     case C8MI_AXIS_EXPONENTIAL: return 1. / (x * a.step);
@@ -64,7 +66,7 @@ __device__ __forceinline__ double axis_derivative(c8mi_native_axis const& a, dou
   return __builtin_nan("");
 }
 
-__device__ __forceinline__ double axis_back_derivative(c8mi_native_axis const& a, double c) {
+C8MI_HD double axis_back_derivative(c8mi_native_axis const& a, double c) {
   switch (a.type) {
     case C8MI_AXIS_LINEAR: return a.step;
     case C8MI_AXIS_EXPONENTIAL: return a.low * a.step * exp(c * a.step);
@@ -77,13 +79,13 @@ __device__ __forceinline__ double axis_back_derivative(c8mi_native_axis const& a
 // maps to 0 (the host form indexes with (uint32)-1 there); the fraction is NaN
 // either way, so the interpolated value is NaN as on the host, and no load
 // leaves the spline's coefficients.
-__device__ __forceinline__ std::uint32_t interval_node(double coordinate, std::uint32_t nodes) {
+C8MI_HD std::uint32_t interval_node(double coordinate, std::uint32_t nodes) {
   if (!(coordinate >= 0.)) return 0;
   if (coordinate >= static_cast<double>(nodes - 2u)) return nodes - 2u;
   return static_cast<std::uint32_t>(coordinate);
 }
 
-__device__ __forceinline__ void hermite(double x, double b[4]) {
+C8MI_HD void hermite(double x, double b[4]) {
   double const x2 = x * x, x3 = x2 * x;
   b[0] = 1. - 3. * x2 + 2. * x3;
   b[1] = 3. * x2 - 2. * x3;
@@ -91,7 +93,7 @@ __device__ __forceinline__ void hermite(double x, double b[4]) {
   b[3] = -x2 + x3;
 }
 
-__device__ __forceinline__ void hermite_derivative(double x, double b[4]) {
+C8MI_HD void hermite_derivative(double x, double b[4]) {
   double const x2 = x * x;
   b[0] = -6. * x + 6. * x2;
   b[1] = 6. * x - 6. * x2;
@@ -109,7 +111,7 @@ struct EnergyRow {
   double f0;
 };
 
-__device__ __forceinline__ EnergyRow energy_row(c8mi_native_bicubic const& s, double e) {
+C8MI_HD EnergyRow energy_row(c8mi_native_bicubic const& s, double e) {
   double const x0 = axis_transform(s.energy_axis, e);
   std::uint32_t const n0 = interval_node(x0, s.rows);
   return {x0, n0, x0 - n0};
@@ -117,8 +119,8 @@ __device__ __forceinline__ EnergyRow energy_row(c8mi_native_bicubic const& s, do
 
 // Four coefficient planes, column-major per spline: (row, column) at
 // offset + row + rows * column.
-__device__ __forceinline__ double bicubic_sum(Tables const& t, c8mi_native_bicubic const& s, EnergyRow const& er,
-                                              double loss, bool derivative) {
+C8MI_HD double bicubic_sum(Tables const& t, c8mi_native_bicubic const& s, EnergyRow const& er,
+                           double loss, bool derivative) {
   double const x1 = axis_transform(s.loss_axis, loss);
   std::uint32_t const n0 = er.n0;
   std::uint32_t const n1 = interval_node(x1, s.columns);
@@ -144,15 +146,15 @@ __device__ __forceinline__ double bicubic_sum(Tables const& t, c8mi_native_bicub
   return derivative ? r / s.loss_axis.step : r;
 }
 
-__device__ __forceinline__ double dot4(double a0, double a1, double a2, double a3, double b0, double b1,
-                              double b2, double b3) {
+C8MI_HD double dot4(double a0, double a1, double a2, double a3, double b0, double b1,
+           double b2, double b3) {
   double const p0 = a0 * b0, p1 = a1 * b1, p2 = a2 * b2, p3 = a3 * b3;
   return ((p0 + p1) + p2) + p3;
 }
 
 // Host-precomputed 16-coefficient cell polynomial.
-__device__ __forceinline__ double bicubic_precomputed(Tables const& t, c8mi_native_bicubic const& s,
-                                                      EnergyRow const& er, double loss) {
+C8MI_HD double bicubic_precomputed(Tables const& t, c8mi_native_bicubic const& s,
+                                   EnergyRow const& er, double loss) {
   if (s.polynomial_coefficient_offset == C8MI_NO_POLYNOMIAL_COEFFICIENTS ||
       t.bicubic_polynomial_coefficients == nullptr)
     return __builtin_nan("");
@@ -175,23 +177,23 @@ __device__ __forceinline__ double bicubic_precomputed(Tables const& t, c8mi_nati
   return dot4(v0[0], v0[1], v0[2], v0[3], p[0], p[1], p[2], p[3]);
 }
 
-__device__ __forceinline__ double bicubic76(Tables const& t, c8mi_native_bicubic const& s, EnergyRow const& er,
-                                            double loss) {
+C8MI_HD double bicubic76(Tables const& t, c8mi_native_bicubic const& s, EnergyRow const& er,
+                         double loss) {
   if (s.polynomial_coefficient_offset != C8MI_NO_POLYNOMIAL_COEFFICIENTS &&
       t.bicubic_polynomial_coefficients != nullptr)
     return bicubic_precomputed(t, s, er, loss);
   return bicubic_sum(t, s, er, loss, false);
 }
 
-__device__ __forceinline__ double bicubic76(Tables const& t, c8mi_native_bicubic const& s, double e,
-                                            double loss) {
+C8MI_HD double bicubic76(Tables const& t, c8mi_native_bicubic const& s, double e,
+                         double loss) {
   return bicubic76(t, s, energy_row(s, e), loss);
 }
 
 // CubicInterpolation 0.1.5's prime(): Boost's sixth-order finite difference.
 // The step is (epsilon / 168)^(1/7) rounded to double, 0x1.6deb70960cf79p-9.
-__device__ __forceinline__ double bicubic_loss_derivative76(Tables const& t, c8mi_native_bicubic const& s,
-                                                            EnergyRow const& er, double loss) {
+C8MI_HD double bicubic_loss_derivative76(Tables const& t, c8mi_native_bicubic const& s,
+                                         EnergyRow const& er, double loss) {
   double h = 0x1.6deb70960cf79p-9;
   double const coordinate = axis_transform(s.loss_axis, loss);
   double const representable = coordinate + h;
@@ -204,7 +206,7 @@ __device__ __forceinline__ double bicubic_loss_derivative76(Tables const& t, c8m
   return cd * axis_derivative(s.loss_axis, loss);
 }
 
-__device__ __forceinline__ double cubic(Tables const& t, c8mi_native_cubic const& s, double x) {
+C8MI_HD double cubic(Tables const& t, c8mi_native_cubic const& s, double x) {
   double const coordinate = axis_transform(s.axis, x);
   std::uint32_t const node = interval_node(coordinate, s.coefficient_count);
   double const fraction = coordinate - node;
@@ -217,8 +219,8 @@ __device__ __forceinline__ double cubic(Tables const& t, c8mi_native_cubic const
   return value;
 }
 
-__device__ __forceinline__ double cubic_derivative76(Tables const& t, c8mi_native_cubic const& s,
-                                            double x) {
+C8MI_HD double cubic_derivative76(Tables const& t, c8mi_native_cubic const& s,
+                         double x) {
   double const coordinate = axis_transform(s.axis, x);
   std::uint32_t const node = interval_node(coordinate, s.coefficient_count);
   double const fraction = coordinate - node;
@@ -236,7 +238,7 @@ __device__ __forceinline__ double cubic_derivative76(Tables const& t, c8mi_nativ
   return derivative * axis_derivative(s.axis, x);
 }
 
-__device__ __forceinline__ int sgn(double x) { return (0. < x ? 1 : 0) - (x < 0. ? 1 : 0); }
+C8MI_HD int sgn(double x) { return (0. < x ? 1 : 0) - (x < 0. ? 1 : 0); }
 
 struct NewtonResult {
   double value;
@@ -246,8 +248,8 @@ struct NewtonResult {
 
 // Boost 1.79 newton_raphson_iterate(f, guess, lower, upper, digits = 20).
 template <class F, class D>
-__device__ __forceinline__ NewtonResult newton76(F const& f, D const& df, double initial, double lower,
-                                        double upper) {
+C8MI_HD NewtonResult newton76(F const& f, D const& df, double initial, double lower,
+                     double upper) {
   double residual = 0., last_residual = 0., result = initial, guess = initial;
   double delta = 1.7976931348623157e308, delta1 = delta, delta2 = delta;
   double maximum_residual = 0., minimum_residual = 0.;
@@ -311,7 +313,7 @@ __device__ __forceinline__ NewtonResult newton76(F const& f, D const& df, double
   return {result, it, it == 256};
 }
 
-__device__ __forceinline__ bool valid_view(Tables const& t) {
+C8MI_HD bool valid_view(Tables const& t) {
   return t.dndx_column_count > 0 && t.dndx_columns != nullptr && t.total_rate_column_count > 0 &&
          t.total_rate_columns != nullptr && t.selection_column_indices != nullptr &&
          (t.dedx_column_count == 0 ||
@@ -321,9 +323,9 @@ __device__ __forceinline__ bool valid_view(Tables const& t) {
          t.bicubic_coefficient_count > 0;
 }
 
-__device__ __forceinline__ c8mi_native_dndx_column const* find_dndx(Tables const& t, std::int32_t pid,
-                                                           std::int32_t process,
-                                                           std::uint64_t component) {
+C8MI_HD c8mi_native_dndx_column const* find_dndx(Tables const& t, std::int32_t pid,
+                                        std::int32_t process,
+                                        std::uint64_t component) {
   std::uint32_t lo = 0, hi = t.dndx_column_count;
   while (lo < hi) {
     std::uint32_t const mid = lo + (hi - lo) / 2;
@@ -342,8 +344,8 @@ __device__ __forceinline__ c8mi_native_dndx_column const* find_dndx(Tables const
   return nullptr;
 }
 
-__device__ __forceinline__ void dndx_range(Tables const& t, std::int32_t pid, std::uint32_t& begin,
-                                  std::uint32_t& end) {
+C8MI_HD void dndx_range(Tables const& t, std::int32_t pid, std::uint32_t& begin,
+               std::uint32_t& end) {
   std::uint32_t lo = 0, hi = t.dndx_column_count;
   while (lo < hi) {
     std::uint32_t const mid = lo + (hi - lo) / 2;
@@ -360,16 +362,16 @@ __device__ __forceinline__ void dndx_range(Tables const& t, std::int32_t pid, st
   end = lo;
 }
 
-__device__ __forceinline__ c8mi_native_dndx_column const* selection_dndx(Tables const& t,
-                                                                std::uint32_t position) {
+C8MI_HD c8mi_native_dndx_column const* selection_dndx(Tables const& t,
+                                             std::uint32_t position) {
   if (position >= t.dndx_column_count) return nullptr;
   std::uint32_t const index = t.selection_column_indices[position];
   if (index >= t.dndx_column_count) return nullptr;
   return &t.dndx_columns[index];
 }
 
-__device__ __forceinline__ c8mi_native_total_rate_column const* find_total_rate(Tables const& t,
-                                                                      std::int32_t pid) {
+C8MI_HD c8mi_native_total_rate_column const* find_total_rate(Tables const& t,
+                                                   std::int32_t pid) {
   std::uint32_t lo = 0, hi = t.total_rate_column_count;
   while (lo < hi) {
     std::uint32_t const mid = lo + (hi - lo) / 2;
@@ -381,8 +383,8 @@ __device__ __forceinline__ c8mi_native_total_rate_column const* find_total_rate(
   return nullptr;
 }
 
-__device__ __forceinline__ c8mi_native_utility_column const* find_utility(Tables const& t,
-                                                                 std::int32_t pid) {
+C8MI_HD c8mi_native_utility_column const* find_utility(Tables const& t,
+                                              std::int32_t pid) {
   for (std::uint32_t i = 0; i < t.utility_column_count; ++i)
     if (t.utility_columns[i].pdg_id == pid) return &t.utility_columns[i];
   return nullptr;
@@ -393,8 +395,8 @@ struct Limits {
   bool valid;
 };
 
-__device__ __forceinline__ Limits kinematic_limits(Tables const& t, c8mi_native_dndx_column const& c,
-                                          double e) {
+C8MI_HD Limits kinematic_limits(Tables const& t, c8mi_native_dndx_column const& c,
+                       double e) {
   if (!(e > 0.)) return {0., 0., false};
   double const mass = c.particle_mass_mev;
   double minimum = 0., maximum = 0.;
@@ -474,8 +476,8 @@ __device__ __forceinline__ Limits kinematic_limits(Tables const& t, c8mi_native_
   return {cut, maximum, true};
 }
 
-__device__ __forceinline__ double physical_loss(Tables const& t, c8mi_native_dndx_column const& c,
-                                       double e, double loss) {
+C8MI_HD double physical_loss(Tables const& t, c8mi_native_dndx_column const& c,
+                    double e, double loss) {
   Limits const l = kinematic_limits(t, c, e);
   if (!l.valid) return __builtin_nan("");
   double const lower = l.minimum, upper = l.maximum;
@@ -497,15 +499,15 @@ __device__ __forceinline__ double physical_loss(Tables const& t, c8mi_native_dnd
   return __builtin_nan("");
 }
 
-__device__ __forceinline__ bool rate_below_threshold(c8mi_native_dndx_column const& c, double e) {
+C8MI_HD bool rate_below_threshold(c8mi_native_dndx_column const& c, double e) {
   return c.rate_model == C8MI_RATE_BICUBIC &&
          (e < c.lower_energy_limit_mev || e < c.spline.energy_axis.low);
 }
 
 }  // namespace detail
 
-__device__ __forceinline__ QueryResult cumulative_rate(Tables const& t, c8mi_native_dndx_column const& c,
-                                                       double e, double loss) {
+C8MI_HD QueryResult cumulative_rate(Tables const& t, c8mi_native_dndx_column const& c,
+                                    double e, double loss) {
   using namespace detail;
   if (!valid_view(t)) return qr(C8MI_QUERY_INVALID_VIEW);
   if (!isfinite(e) || !isfinite(loss)) return qr(C8MI_QUERY_NON_FINITE_INPUT);
@@ -519,8 +521,8 @@ __device__ __forceinline__ QueryResult cumulative_rate(Tables const& t, c8mi_nat
   return qr(C8MI_QUERY_SUCCESS, value);
 }
 
-__device__ __forceinline__ QueryResult rate_for_column(Tables const& t, c8mi_native_dndx_column const& c,
-                                                       double e) {
+C8MI_HD QueryResult rate_for_column(Tables const& t, c8mi_native_dndx_column const& c,
+                                    double e) {
   using namespace detail;
   if (!valid_view(t)) return qr(C8MI_QUERY_INVALID_VIEW);
   if (rate_below_threshold(c, e)) return qr(C8MI_QUERY_SUCCESS);
@@ -593,8 +595,8 @@ __device__ __forceinline__ QueryResult rate_for_column(Tables const& t, c8mi_nat
   return cumulative_rate(t, c, e, 1.);
 }
 
-__device__ __forceinline__ QueryResult query_rate(Tables const& t, std::int32_t pid, std::int32_t process,
-                                         std::uint64_t component, double e) {
+C8MI_HD QueryResult query_rate(Tables const& t, std::int32_t pid, std::int32_t process,
+                      std::uint64_t component, double e) {
   using namespace detail;
   if (!valid_view(t)) return qr(C8MI_QUERY_INVALID_VIEW);
   auto const* c = find_dndx(t, pid, process, component);
@@ -603,7 +605,7 @@ __device__ __forceinline__ QueryResult query_rate(Tables const& t, std::int32_t 
   return rate_for_column(t, *c, e);
 }
 
-__device__ __forceinline__ QueryResult total_rate(Tables const& t, std::int32_t pid, double e) {
+C8MI_HD QueryResult total_rate(Tables const& t, std::int32_t pid, double e) {
   using namespace detail;
   if (!valid_view(t)) return qr(C8MI_QUERY_INVALID_VIEW);
   if (!isfinite(e)) return qr(C8MI_QUERY_NON_FINITE_INPUT);
@@ -616,7 +618,7 @@ __device__ __forceinline__ QueryResult total_rate(Tables const& t, std::int32_t 
   return qr(C8MI_QUERY_SUCCESS, value < 0. ? 0. : value);
 }
 
-__device__ __forceinline__ QueryResult selection_total_rate(Tables const& t, std::int32_t pid, double e) {
+C8MI_HD QueryResult selection_total_rate(Tables const& t, std::int32_t pid, double e) {
   using namespace detail;
   if (!valid_view(t)) return qr(C8MI_QUERY_INVALID_VIEW);
   double total = 0.;
@@ -634,7 +636,7 @@ __device__ __forceinline__ QueryResult selection_total_rate(Tables const& t, std
   return qr(C8MI_QUERY_SUCCESS, total);
 }
 
-__device__ __forceinline__ QueryResult dedx(Tables const& t, std::int32_t pid, double e) {
+C8MI_HD QueryResult dedx(Tables const& t, std::int32_t pid, double e) {
   using namespace detail;
   if (!isfinite(e)) return qr(C8MI_QUERY_NON_FINITE_INPUT);
   double total = 0.;
@@ -653,9 +655,9 @@ __device__ __forceinline__ QueryResult dedx(Tables const& t, std::int32_t pid, d
   return qr(C8MI_QUERY_SUCCESS, total);
 }
 
-__device__ __forceinline__ QueryResult loss_fraction(Tables const& t, std::int32_t pid, std::int32_t process,
-                                            std::uint64_t component, double e, double q,
-                                            bool photon_split) {
+C8MI_HD QueryResult loss_fraction(Tables const& t, std::int32_t pid, std::int32_t process,
+                         std::uint64_t component, double e, double q,
+                         bool photon_split) {
   using namespace detail;
   if (!valid_view(t)) return qr(C8MI_QUERY_INVALID_VIEW);
   if (!isfinite(e) || !isfinite(q)) return qr(C8MI_QUERY_NON_FINITE_INPUT);
@@ -851,7 +853,7 @@ __device__ __forceinline__ QueryResult loss_fraction(Tables const& t, std::int32
   return qr(C8MI_QUERY_SUCCESS, physical, newton, bisection);
 }
 
-__device__ __forceinline__ QueryResult range(Tables const& t, std::int32_t pid, double e) {
+C8MI_HD QueryResult range(Tables const& t, std::int32_t pid, double e) {
   using namespace detail;
   auto const* c = find_utility(t, pid);
   if (!c) return qr(C8MI_QUERY_PARTICLE_NOT_FOUND);
@@ -865,8 +867,8 @@ __device__ __forceinline__ QueryResult range(Tables const& t, std::int32_t pid, 
   return qr(C8MI_QUERY_SUCCESS, value);
 }
 
-__device__ __forceinline__ QueryResult energy_after_loss(Tables const& t, std::int32_t pid, double e0,
-                                                double grammage) {
+C8MI_HD QueryResult energy_after_loss(Tables const& t, std::int32_t pid, double e0,
+                             double grammage) {
   using namespace detail;
   auto const* c = find_utility(t, pid);
   if (!c) return qr(C8MI_QUERY_PARTICLE_NOT_FOUND);
@@ -913,7 +915,7 @@ __device__ __forceinline__ QueryResult energy_after_loss(Tables const& t, std::i
   return qr(C8MI_QUERY_SUCCESS, lower, n.iterations, bisection);
 }
 
-__device__ __forceinline__ QueryResult energy_of_range(Tables const& t, std::int32_t pid, double r) {
+C8MI_HD QueryResult energy_of_range(Tables const& t, std::int32_t pid, double r) {
   using namespace detail;
   auto const* c = find_utility(t, pid);
   if (!c) return qr(C8MI_QUERY_PARTICLE_NOT_FOUND);
@@ -936,7 +938,7 @@ __device__ __forceinline__ QueryResult energy_of_range(Tables const& t, std::int
   return qr(C8MI_QUERY_SUCCESS, 0.5 * (lower + upper), 0, it);
 }
 
-__device__ __forceinline__ QueryResult run_query(Tables const& t, c8mi_native_query const& q) {
+C8MI_HD QueryResult run_query(Tables const& t, c8mi_native_query const& q) {
   switch (q.kind) {
     case C8MI_NATIVE_TOTAL_RATE: return total_rate(t, q.pdg_id, q.energy_mev);
     case C8MI_NATIVE_SELECTION_TOTAL_RATE: return selection_total_rate(t, q.pdg_id, q.energy_mev);

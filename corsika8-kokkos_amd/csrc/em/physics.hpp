@@ -82,26 +82,26 @@ enum LeptonLimit : int {
 };
 enum Stage : std::uint32_t { kStageSelection = 1, kStageTransport = 2, kStageVertex = 3, kStageFinal = 4 };
 
-__device__ __forceinline__ int sgn(double v) { return (0. < v) - (v < 0.); }
+C8MI_HD int sgn(double v) { return (0. < v) - (v < 0.); }
 
-__device__ __forceinline__ double uniform(EmContext const& c, Particle const& p,
-                                          std::uint32_t process, std::uint64_t draw) {
+C8MI_HD double uniform(EmContext const& c, Particle const& p,
+                       std::uint32_t process, std::uint64_t draw) {
   return uniform_open01(RandomKey{c.seed, p.shower_key, p.history_id, p.step_id, process, draw});
 }
 
-__device__ __forceinline__ double transport_mass(std::int32_t pid) {
+C8MI_HD double transport_mass(std::int32_t pid) {
   if (pid == 11 || pid == -11) return kTransportElectronMassGeV;
   if (pid == 13 || pid == -13) return kTransportMuonMassGeV;
   return 0.;
 }
-__device__ __forceinline__ double secondary_energy(std::int32_t pid, double total, double mass) {
+C8MI_HD double secondary_energy(std::int32_t pid, double total, double mass) {
   return (total - mass) + transport_mass(pid);
 }
-__device__ __forceinline__ double mass_correction(std::int32_t pid, double mass, double weight) {
+C8MI_HD double mass_correction(std::int32_t pid, double mass, double weight) {
   return weight * (transport_mass(pid) - mass);
 }
-__device__ __forceinline__ bool is_ee(std::int32_t pid) { return pid == 11 || pid == -11; }
-__device__ __forceinline__ bool is_charged_lepton(std::int32_t pid) {
+C8MI_HD bool is_ee(std::int32_t pid) { return pid == 11 || pid == -11; }
+C8MI_HD bool is_charged_lepton(std::int32_t pid) {
   return pid == 11 || pid == -11 || pid == 13 || pid == -13;
 }
 
@@ -118,26 +118,26 @@ struct Accumulator {
   std::uint32_t slot;
 };
 
-__device__ __forceinline__ void add_fixed(Accumulator const& a, long long* address, double value,
-                                          double scale) {
+C8MI_HD void add_fixed(Accumulator const& a, long long* address, double value,
+                       double scale) {
   double const scaled = value * scale;
   if (!isfinite(scaled) || fabs(scaled) >= 9.2233720368547748e18) {
-    atomicAdd(reinterpret_cast<unsigned long long*>(&a.counters[kOverflows]), 1ULL);
+    fetch_add(reinterpret_cast<unsigned long long*>(&a.counters[kOverflows]), 1ULL);
     return;
   }
   long long const inc = static_cast<long long>(rint(scaled));
-  if (inc != 0) atomicAdd(reinterpret_cast<unsigned long long*>(address), static_cast<unsigned long long>(inc));
+  if (inc != 0) fetch_add(reinterpret_cast<unsigned long long*>(address), static_cast<unsigned long long>(inc));
 }
 
-__device__ __forceinline__ void count(Accumulator const& a, int index, unsigned long long n = 1) {
-  atomicAdd(reinterpret_cast<unsigned long long*>(&a.counters[index]), n);
+C8MI_HD void count(Accumulator const& a, int index, unsigned long long n = 1) {
+  fetch_add(reinterpret_cast<unsigned long long*>(&a.counters[index]), n);
 }
 
-__device__ __forceinline__ void ledger(Accumulator const& a, int term, double value) {
+C8MI_HD void ledger(Accumulator const& a, int term, double value) {
   add_fixed(a, &a.counters[term], value, a.ctx->energy_scale);
 }
 
-__device__ __forceinline__ double project_grammage(Accumulator const& a, double const p[3]) {
+C8MI_HD double project_grammage(Accumulator const& a, double const p[3]) {
   ShowerSlot const& c = *a.shower;
   double proj = 0.;
   for (int k = 0; k < 3; ++k) proj += (p[k] - c.axis_start[k]) * c.axis_direction[k];
@@ -151,8 +151,8 @@ __device__ __forceinline__ double project_grammage(Accumulator const& a, double 
   return c.axis_grammage[upper] * f + c.axis_grammage[lower] * (1. - f);
 }
 
-__device__ __forceinline__ void particle_profile(Accumulator const& a, std::int32_t pid, double start,
-                                        double end, double weight) {
+C8MI_HD void particle_profile(Accumulator const& a, std::int32_t pid, double start,
+                     double end, double weight) {
   EmContext const& c = *a.ctx;
   double const s = start / c.bin_width, e = end / c.bin_width;
   if (c.crossing_mode == C8MI_PROFILE_CROSSING_FORWARD && !(e > s)) return;
@@ -175,7 +175,7 @@ __device__ __forceinline__ void particle_profile(Accumulator const& a, std::int3
     add_fixed(a, hist + k, weight, c.weight_scale);
 }
 
-__device__ __forceinline__ void energy_profile(Accumulator const& a, double start, double end, double deposit) {
+C8MI_HD void energy_profile(Accumulator const& a, double start, double end, double deposit) {
   EmContext const& c = *a.ctx;
   if (!(deposit > 0.) || c.profile_bins == 0) return;
   count(a, kDepositedSteps);
@@ -208,7 +208,7 @@ __device__ __forceinline__ void energy_profile(Accumulator const& a, double star
   for (int bin = first + 1; bin < last; ++bin) add_fixed(a, hist + bin, dens * w, c.energy_scale);
 }
 
-__device__ __forceinline__ void point_energy_profile(Accumulator const& a, double x, double deposit) {
+C8MI_HD void point_energy_profile(Accumulator const& a, double x, double deposit) {
   EmContext const& c = *a.ctx;
   if (!(deposit > 0.) || c.profile_bins == 0) return;
   int const maximum_bin = static_cast<int>(c.profile_bins - 1);
@@ -220,7 +220,7 @@ __device__ __forceinline__ void point_energy_profile(Accumulator const& a, doubl
             c.energy_scale);
 }
 
-__device__ __forceinline__ void terminal_energy(Accumulator const& a, double e, bool observed) {
+C8MI_HD void terminal_energy(Accumulator const& a, double e, bool observed) {
   if (!(e >= 0.) || !isfinite(e)) {
     count(a, kInvalid);
     return;
@@ -228,9 +228,9 @@ __device__ __forceinline__ void terminal_energy(Accumulator const& a, double e, 
   ledger(a, observed ? kObserved : kEscaped, e);
 }
 
-__device__ __forceinline__ void thinning_ledger(Accumulator const& a, int status, std::uint32_t multiplicity,
-                                       std::uint32_t kept, double parent_energy_weight,
-                                       double kept_energy_weight) {
+C8MI_HD void thinning_ledger(Accumulator const& a, int status, std::uint32_t multiplicity,
+                    std::uint32_t kept, double parent_energy_weight,
+                    double kept_energy_weight) {
   if (status == 1) count(a, kThinHillas);
   else if (status == 2) count(a, kThinStatistical);
   if (multiplicity > kept) count(a, kThinDiscarded, multiplicity - kept);
@@ -239,18 +239,18 @@ __device__ __forceinline__ void thinning_ledger(Accumulator const& a, int status
 
 // Observation / fallback records: appended to the shower's own region with one
 // atomic per record (rare).
-__device__ __forceinline__ void record_observation(Accumulator const& a, Particle const& p) {
+C8MI_HD void record_observation(Accumulator const& a, Particle const& p) {
   EmContext const& c = *a.ctx;
-  unsigned long long const n = atomicAdd(&c.record_counts[2u * a.slot], 1ULL);
+  unsigned long long const n = fetch_add(&c.record_counts[2u * a.slot], 1ULL);
   if (n < c.observation_capacity) c.observations[a.slot * c.observation_capacity + n] = p;
 }
 
-__device__ __forceinline__ void record_fallback(Accumulator const& a, Particle const& p, int reason,
-                                       std::int32_t process, std::uint32_t stage) {
+C8MI_HD void record_fallback(Accumulator const& a, Particle const& p, int reason,
+                    std::int32_t process, std::uint32_t stage) {
   EmContext const& c = *a.ctx;
   count(a, kFallbackReason0 + (reason & 31));
   ledger(a, kFallbackEnergy, p.energy_gev * p.weight);
-  unsigned long long const n = atomicAdd(&c.record_counts[2u * a.slot + 1u], 1ULL);
+  unsigned long long const n = fetch_add(&c.record_counts[2u * a.slot + 1u], 1ULL);
   if (n < c.fallback_capacity) {
     c8mi_em_fallback f;
     f.particle = p;
@@ -264,7 +264,7 @@ __device__ __forceinline__ void record_fallback(Accumulator const& a, Particle c
 
 // ---- table facades ----
 
-__device__ __forceinline__ int from_native(std::uint32_t status, bool continuous) {
+C8MI_HD int from_native(std::uint32_t status, bool continuous) {
   switch (status) {
     case C8MI_QUERY_SUCCESS: return kTsSuccess;
     case C8MI_QUERY_PARTICLE_NOT_FOUND: return continuous ? kTsContParticle : kTsParticleNotFound;
@@ -278,7 +278,7 @@ __device__ __forceinline__ int from_native(std::uint32_t status, bool continuous
   }
 }
 
-__device__ __forceinline__ int fallback_reason(int status) {
+C8MI_HD int fallback_reason(int status) {
   switch (status) {
     case kTsParticleNotFound: return kFrParticleTable;
     case kTsColumnNotFound: return kFrComponent;
@@ -294,7 +294,7 @@ __device__ __forceinline__ int fallback_reason(int status) {
   }
 }
 
-__device__ __forceinline__ bool requires_selected_loss(int reason) {
+C8MI_HD bool requires_selected_loss(int reason) {
   return reason == kFrSelectionReplay || reason == kFrInverseCdf || reason == kFrLossEnergy ||
          reason == kFrLossQuantile;
 }
@@ -304,24 +304,24 @@ struct Tq {
   double value;
 };
 
-__device__ __forceinline__ Tq native(QueryResult const& r, bool continuous) {
+C8MI_HD Tq native(QueryResult const& r, bool continuous) {
   return {from_native(r.status, continuous), r.status == C8MI_QUERY_SUCCESS ? r.value : 0.};
 }
 
-__device__ __forceinline__ Tq cq_mass(EmContext const& c, std::int32_t pid) {
+C8MI_HD Tq cq_mass(EmContext const& c, std::int32_t pid) {
   auto const* u = detail::find_utility(c.tables, pid);
   if (!u) return {kTsContParticle, 0.};
   if (!isfinite(u->particle_mass_mev) || !(u->particle_mass_mev > 0.)) return {kTsInvalidView, 0.};
   return {kTsSuccess, u->particle_mass_mev};
 }
 
-__device__ __forceinline__ Tq cq_transport_cut(EmContext const& c, std::int32_t pid) {
+C8MI_HD Tq cq_transport_cut(EmContext const& c, std::int32_t pid) {
   double const cut = (pid == 13 || pid == -13) ? c.muon_cut_mev : c.em_cut_mev;
   if (!isfinite(cut) || !(cut > 0.)) return {kTsInvalidView, 0.};
   return {kTsSuccess, cut};
 }
 
-__device__ __forceinline__ Tq cq_minimum_energy(EmContext const& c, std::int32_t pid) {
+C8MI_HD Tq cq_minimum_energy(EmContext const& c, std::int32_t pid) {
   auto const* u = detail::find_utility(c.tables, pid);
   if (!u) return {kTsContParticle, 0.};
   Tq const cut = cq_transport_cut(c, pid);
@@ -334,7 +334,7 @@ __device__ __forceinline__ Tq cq_minimum_energy(EmContext const& c, std::int32_t
   return {kTsSuccess, minimum};
 }
 
-__device__ __forceinline__ Tq cq_energy_after_loss(EmContext const& c, std::int32_t pid, double e0, double x) {
+C8MI_HD Tq cq_energy_after_loss(EmContext const& c, std::int32_t pid, double e0, double x) {
   if (!isfinite(x) || x < 0.) return {kTsNonFinite, 0.};
   Tq const initial = native(range(c.tables, pid, e0), true);
   if (initial.status != kTsSuccess) return initial;
@@ -342,11 +342,11 @@ __device__ __forceinline__ Tq cq_energy_after_loss(EmContext const& c, std::int3
   return native(energy_after_loss(c.tables, pid, e0, x), true);
 }
 
-__device__ __forceinline__ bool pk_finite(double v) {
+C8MI_HD bool pk_finite(double v) {
   return v == v && v <= 1.7976931348623157e308 && v >= -1.7976931348623157e308;
 }
 
-__device__ __forceinline__ bool decode_pair_split(double e, double s, double& rho) {
+C8MI_HD bool decode_pair_split(double e, double s, double& rho) {
   if (!pk_finite(e) || !pk_finite(s) || !(e > 2. * kElectronMassGeV)) return false;
   double const tol = 64. * 2.2204460492503131e-16;
   if (s < -tol || s > 1. + tol) return false;
@@ -357,7 +357,7 @@ __device__ __forceinline__ bool decode_pair_split(double e, double s, double& rh
   return pk_finite(rho) && rho >= lower && rho <= 1. - lower;
 }
 
-__device__ __forceinline__ bool encode_pair_split(double e, double rho, double& s) {
+C8MI_HD bool encode_pair_split(double e, double rho, double& s) {
   if (!pk_finite(e) || !pk_finite(rho) || !(e > 2. * kElectronMassGeV)) return false;
   double const lower = kElectronMassGeV / e;
   double const width = 1. - 2. * lower;
@@ -369,8 +369,8 @@ __device__ __forceinline__ bool encode_pair_split(double e, double rho, double& 
   return pk_finite(s) && s >= 0. && s <= 1.;
 }
 
-__device__ __forceinline__ Tq dq_loss_fraction(EmContext const& c, std::int32_t pid, std::int32_t process,
-                                      std::uint64_t comp, double e, double q) {
+C8MI_HD Tq dq_loss_fraction(EmContext const& c, std::int32_t pid, std::int32_t process,
+                   std::uint64_t comp, double e, double q) {
   if (process == kPhotonPairFinalState && pid == 22) {
     Tq r = native(loss_fraction(c.tables, pid, kPhotonPair, comp, e, q, true), false);
     if (r.status != kTsSuccess) return r;
@@ -389,7 +389,7 @@ struct Selection {
   bool selected;
 };
 
-__device__ __forceinline__ Selection select_column(EmContext const& c, std::int32_t pid, double e, double u) {
+C8MI_HD Selection select_column(EmContext const& c, std::int32_t pid, double e, double u) {
   Selection r{kTsNonFinite, 0, 0, 0., false};
   if (!isfinite(u) || u < 0. || u >= 1.) return r;
   Tq const total = native(selection_total_rate(c.tables, pid, e), false);
@@ -437,7 +437,7 @@ __device__ __forceinline__ Selection select_column(EmContext const& c, std::int3
 enum Capability : int { kCapPair, kCapMuPair, kCapBrems, kCapCompton, kCapPhotoelectric,
                         kCapAnnihilation, kCapIonization, kCapCpuOnly, kCapNotImplemented };
 
-__device__ __forceinline__ int capability(std::int32_t pid, std::int32_t process) {
+C8MI_HD int capability(std::int32_t pid, std::int32_t process) {
   if (pid == 22 && process == kPhotonMuPair) return kCapMuPair;
   if (pid == 22 && process == kPhotonPair) return kCapPair;
   if (is_ee(pid) && process == kBrems) return kCapBrems;
@@ -455,7 +455,7 @@ __device__ __forceinline__ int capability(std::int32_t pid, std::int32_t process
   }
 }
 
-__device__ __forceinline__ bool requires_replay(std::int32_t pid, std::int32_t process, double q) {
+C8MI_HD bool requires_replay(std::int32_t pid, std::int32_t process, double q) {
   int const cap = capability(pid, process);
   if (cap == kCapCpuOnly) return true;
   if (cap == kCapMuPair || cap == kCapNotImplemented) return false;
@@ -485,8 +485,8 @@ struct Fallback {
   int reason;
 };
 
-__device__ __forceinline__ void select_interaction(EmContext const& c, Particle const& p, Interaction& in,
-                                          Fallback& fb) {
+C8MI_HD void select_interaction(EmContext const& c, Particle const& p, Interaction& in,
+                       Fallback& fb) {
   in.particle = p;
   in.deferred_reason = -1;
   in.status = kSelected;
@@ -577,12 +577,12 @@ __device__ __forceinline__ void select_interaction(EmContext const& c, Particle 
 
 // ---- final-state helpers ----
 
-__device__ __forceinline__ double bounded_cosine(double v) {
+C8MI_HD double bounded_cosine(double v) {
   double const m = v < 1. ? v : 1.;
   return -1. > m ? -1. : m;
 }
 
-__device__ __forceinline__ double sauter_cosine(double e, double u) {
+C8MI_HD double sauter_cosine(double e, double u) {
   double const m2 = kElectronMassGeV * kElectronMassGeV;
   double const t = e * e - m2;
   double const p = sqrt(0. > t ? 0. : t);
@@ -592,7 +592,7 @@ __device__ __forceinline__ double sauter_cosine(double e, double u) {
   return bounded_cosine(1. - omc);
 }
 
-__device__ __forceinline__ bool compton_cosines(double e, double v, double& pc, double& ec) {
+C8MI_HD bool compton_cosines(double e, double v, double& pc, double& ec) {
   double const scattered = e * (1. - v);
   double const pe2 = 2. * v * e * kElectronMassGeV + v * v * e * e;
   if (!isfinite(e) || !(e > 0.) || !isfinite(v) || !(v > 0.) || !(v < 1.) || !(scattered > 0.) ||
@@ -608,8 +608,8 @@ __device__ __forceinline__ bool compton_cosines(double e, double v, double& pc, 
   return true;
 }
 
-__device__ __forceinline__ bool photoelectric_energy(c8mi_photon_pair_lpm const& s, std::uint64_t comp,
-                                            double e, double& total, double& fraction) {
+C8MI_HD bool photoelectric_energy(c8mi_photon_pair_lpm const& s, std::uint64_t comp,
+                         double e, double& total, double& fraction) {
   if (!isfinite(e) || !(e > 0.) || s.component_count == 0 || s.component_count > 16 ||
       !isfinite(s.fine_structure_constant) || !(s.fine_structure_constant > 0.))
     return false;
@@ -630,8 +630,8 @@ __device__ __forceinline__ bool photoelectric_energy(c8mi_photon_pair_lpm const&
          fraction <= 1.;
 }
 
-__device__ __forceinline__ void deflect(double const in[3], double cosine, double azimuth, double out[3],
-                               bool bound) {
+C8MI_HD void deflect(double const in[3], double cosine, double azimuth, double out[3],
+            bool bound) {
   double const t = in[0] * in[0] + in[1] * in[1];
   double const tr = sqrt(0. > t ? 0. : t);
   double cp = 1., sp = 0.;
@@ -662,8 +662,8 @@ struct Thin {
   double w1, w2;
 };
 
-__device__ __forceinline__ Thin em_thinning(c8mi_em_thinning const& c, bool enabled, double pe, double pw,
-                                   double e1, double e2, double u1, double u2) {
+C8MI_HD Thin em_thinning(c8mi_em_thinning const& c, bool enabled, double pe, double pw,
+                double e1, double e2, double u1, double u2) {
   if (!isfinite(pe) || !(pe > 0.) || !isfinite(pw) || pw < 0. || !isfinite(e1) || !(e1 > 0.) ||
       !isfinite(e2) || !(e2 > 0.) || !isfinite(u1) || !(u1 > 0.) || !(u1 < 1.) || !isfinite(u2) ||
       !(u2 > 0.) || !(u2 < 1.))
@@ -708,9 +708,9 @@ __device__ __forceinline__ Thin em_thinning(c8mi_em_thinning const& c, bool enab
   return o;
 }
 
-__device__ __forceinline__ bool two_child_thinning(EmContext const& c, Particle const& parent,
-                                          std::int32_t process, double e1, double e2, bool lepton,
-                                          Thin& t) {
+C8MI_HD bool two_child_thinning(EmContext const& c, Particle const& parent,
+                       std::int32_t process, double e1, double e2, bool lepton,
+                       Thin& t) {
   double const u1 = uniform(c, parent, static_cast<std::uint32_t>(process), 0x100);
   double const u2 = uniform(c, parent, static_cast<std::uint32_t>(process), 0x101);
   bool const enabled = c.thinning.enabled != 0 && !(lepton && !is_ee(parent.pid));
@@ -719,8 +719,8 @@ __device__ __forceinline__ bool two_child_thinning(EmContext const& c, Particle 
 }
 
 // LPM tails shared by the photon-pair and bremsstrahlung suppressions.
-__device__ __forceinline__ void lpm_functions(double sp, double s1, double gamma, double& xi, double& fi,
-                                     double& g) {
+C8MI_HD void lpm_functions(double sp, double s1, double gamma, double& xi, double& fi,
+                  double& g) {
   constexpr double Pi = 3.141592653589793238462643383279502884;
   double const h = log(sp) / log(s1);
   xi = 1.;
@@ -743,8 +743,8 @@ __device__ __forceinline__ void lpm_functions(double sp, double s1, double gamma
   }
 }
 
-__device__ __forceinline__ int pair_lpm(c8mi_photon_pair_lpm const& s, std::uint64_t comp, double e, double x,
-                               double rho, double& prob) {
+C8MI_HD int pair_lpm(c8mi_photon_pair_lpm const& s, std::uint64_t comp, double e, double x,
+            double rho, double& prob) {
   if (s.component_count == 0 || s.component_count > 16 || !isfinite(s.baseline_mass_density_g_per_cm3) ||
       !(s.baseline_mass_density_g_per_cm3 > 0.) || !isfinite(s.molecular_density_per_cm3) ||
       !(s.molecular_density_per_cm3 > 0.) || !isfinite(s.sum_charge) || !(s.sum_charge > 0.) ||
@@ -782,8 +782,8 @@ __device__ __forceinline__ int pair_lpm(c8mi_photon_pair_lpm const& s, std::uint
   return 0;
 }
 
-__device__ __forceinline__ int brems_lpm(c8mi_brems_lpm const& s, std::uint64_t comp, double e, double v,
-                                double rho, double& prob) {
+C8MI_HD int brems_lpm(c8mi_brems_lpm const& s, std::uint64_t comp, double e, double v,
+             double rho, double& prob) {
   if (s.component_count == 0 || s.component_count > 16 || !isfinite(s.baseline_mass_density_g_per_cm3) ||
       !(s.baseline_mass_density_g_per_cm3 > 0.) || !isfinite(s.molecular_density_per_cm3) ||
       !(s.molecular_density_per_cm3 > 0.) || !isfinite(s.sum_charge) || !(s.sum_charge > 0.) ||
@@ -830,11 +830,11 @@ __device__ __forceinline__ int brems_lpm(c8mi_brems_lpm const& s, std::uint64_t 
   return 0;
 }
 
-__device__ __forceinline__ double annihilation_term(double a1, double a2, double rho) {
+C8MI_HD double annihilation_term(double a1, double a2, double rho) {
   return a1 * log(rho) + a2 / rho - rho;
 }
 
-__device__ __forceinline__ bool annihilation_rho(double e, double m, double u, double& rho) {
+C8MI_HD bool annihilation_rho(double e, double m, double u, double& rho) {
   double const gamma = e / m;
   if (!isfinite(gamma) || !(gamma > 1.) || !isfinite(u) || !(u > 0.) || !(u < 1.)) return false;
   double const aux = sqrt((gamma - 1.) / (gamma + 1.));
@@ -862,8 +862,8 @@ __device__ __forceinline__ bool annihilation_rho(double e, double m, double u, d
   return isfinite(rho) && rho > 0. && rho < 1.;
 }
 
-__device__ __forceinline__ Particle child_of(Particle const& parent, std::int32_t pid, double energy,
-                                             double weight) {
+C8MI_HD Particle child_of(Particle const& parent, std::int32_t pid, double energy,
+                          double weight) {
   Particle c = parent;
   c.pid = pid;
   c.energy_gev = energy;
@@ -892,8 +892,8 @@ struct Outputs {
 // Compact radio track of one e+- step (the make_radio_track_kinematics input,
 // RadioProjection.inl:145-168): start and end positions, start and end times,
 // weight signed by charge (negative for e-).
-__device__ __forceinline__ void write_track(double* t, std::uint64_t stride, c8mi_particle_state const& start,
-                                            c8mi_particle_state const& end) {
+C8MI_HD void write_track(double* t, std::uint64_t stride, c8mi_particle_state const& start,
+                         c8mi_particle_state const& end) {
   t[0] = start.position_m[0];
   t[stride] = start.position_m[1];
   t[2 * stride] = start.position_m[2];
@@ -906,11 +906,11 @@ __device__ __forceinline__ void write_track(double* t, std::uint64_t stride, c8m
 }
 
 // A source without an e+- step record: a zero-length track, which the projection rejects.
-__device__ __forceinline__ void write_no_track(double* t, std::uint64_t stride) {
+C8MI_HD void write_no_track(double* t, std::uint64_t stride) {
   for (int k = 0; k < 9; ++k) t[k * stride] = 0.;
 }
 
-__device__ __forceinline__ void emit(Outputs& o, Particle const& p, std::uint32_t kind) {
+C8MI_HD void emit(Outputs& o, Particle const& p, std::uint32_t kind) {
   std::uint32_t const k = (o.kind[0] == kSlotNone) ? 0u : 1u;
   o.staging[k] = p;
   o.kind[k] = kind;

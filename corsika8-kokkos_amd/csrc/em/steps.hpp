@@ -15,8 +15,8 @@ namespace c8mi::em::phys {
 
 // ---------------------------------------------------------------- photon ----
 
-__device__ __forceinline__ void photon_step(EmContext const& c, Accumulator const& acc, Particle const& p,
-                                   Outputs& o) {
+C8MI_HD void photon_step(EmContext const& c, Accumulator const& acc, Particle const& p,
+                Outputs& o) {
   Interaction in;
   Fallback fb{false, 0};
   select_interaction(c, p, in, fb);
@@ -274,8 +274,8 @@ struct LeptonMove {
 
 // The move of one e+- step (selection, continuous step, transport): its end
 // state and limit. Returns false after recording a fallback.
-__device__ __forceinline__ bool lepton_move(EmContext const& c, Accumulator const& acc, Particle const& p,
-                                            LeptonMove& mv) {
+C8MI_HD bool lepton_move(EmContext const& c, Accumulator const& acc, Particle const& p,
+                         LeptonMove& mv) {
   Interaction in;
   Fallback fb{false, 0};
   select_interaction(c, p, in, fb);
@@ -495,13 +495,13 @@ struct MoliereDraws {
   double u1, u2, ua;
 };
 
-__device__ __forceinline__ MoliereDraws moliere_draws(EmContext const& c, Particle const& start) {
+C8MI_HD MoliereDraws moliere_draws(EmContext const& c, Particle const& start) {
   return {uniform(c, start, kScatteringProcess, 0), uniform(c, start, kScatteringProcess, 1),
           uniform(c, start, kScatteringProcess, 2)};
 }
 
 // The space angle of the two projected angles, then the deflection of the end direction.
-__device__ __forceinline__ int moliere_apply(LeptonMove& mv, double a1, double a2, double ua) {
+C8MI_HD int moliere_apply(LeptonMove& mv, double a1, double a2, double ua) {
   double const angle = sqrt(a1 * a1 + a2 * a2);
   double dir[3];
   if (!isfinite(angle) || moliere::deflect_end(mv.start.direction, mv.end.direction, angle, ua, dir) != 0)
@@ -514,7 +514,7 @@ __device__ __forceinline__ int moliere_apply(LeptonMove& mv, double a1, double a
 // sample still open is described in `open` (kMoliereOpen) for
 // moliere_scatter_wave.
 template <bool Table>
-__device__ __forceinline__ int moliere_scatter(EmContext const& c, LeptonMove& mv, MoliereOpen& open) {
+C8MI_HD int moliere_scatter(EmContext const& c, LeptonMove& mv, MoliereOpen& open) {
   if (mv.particle_cut) return kMoliereDone;
   MoliereDraws const d = moliere_draws(c, mv.start);
   moliere::Quotients const q{c.moliere_i1, c.moliere_i2, c.moliere_i2_large, c.moliere_half_over};
@@ -576,9 +576,9 @@ __device__ __forceinline__ int moliere_scatter_wave(EmContext const& c, LeptonMo
 // Returns true when the step ends at an interaction vertex: `vx` then holds
 // what lepton_vertex needs.
 template <class Input>
-__device__ __forceinline__ bool lepton_finish(EmContext const& c, Accumulator const& acc, Input const& input,
-                                              LeptonMove const& mv, bool moliere_failed, Outputs& o,
-                                              LeptonVertex& vx) {
+C8MI_HD bool lepton_finish(EmContext const& c, Accumulator const& acc, Input const& input,
+                           LeptonMove const& mv, bool moliere_failed, Outputs& o,
+                           LeptonVertex& vx) {
   Particle const& start = mv.start;
   Particle end;
   static_cast<c8mi_particle_state&>(end) = mv.end;
@@ -636,8 +636,8 @@ __device__ __forceinline__ bool lepton_finish(EmContext const& c, Accumulator co
 
 // The step of one e+- up to its interaction vertex (move, then finish),
 // without Molière scattering.
-__device__ __forceinline__ bool lepton_transport(EmContext const& c, Accumulator const& acc, Particle const& p,
-                                                 Outputs& o, LeptonVertex& vx) {
+C8MI_HD bool lepton_transport(EmContext const& c, Accumulator const& acc, Particle const& p,
+                              Outputs& o, LeptonVertex& vx) {
   LeptonMove mv;
   if (!lepton_move(c, acc, p, mv)) return false;
   return lepton_finish(c, acc, [&] { return p; }, mv, false, o, vx);
@@ -645,8 +645,8 @@ __device__ __forceinline__ bool lepton_transport(EmContext const& c, Accumulator
 
 // Interaction vertex of an e+- step: vertex reselection at the post-loss
 // energy, then the final state.
-__device__ __forceinline__ void lepton_vertex(EmContext const& c, Accumulator const& acc, LeptonVertex const& vx,
-                                              Outputs& o) {
+C8MI_HD void lepton_vertex(EmContext const& c, Accumulator const& acc, LeptonVertex const& vx,
+                           Outputs& o) {
   // select_lepton_vertex (LeptonVertexSelection.inl:41-197)
   Particle parent = vx.parent;
   double const w = parent.weight;

@@ -620,6 +620,12 @@ typedef struct c8mi_em_request {
   uint64_t radio_observer_count;
   int64_t* radio_coreas;
   int64_t* radio_zhs;
+  /* A batch the host handed off (the CPU prefix of the shower,
+   * c8mi_em_hybrid_prefix): with particle_count > 0 these particles are
+   * enqueued at admission instead of `primary`, and first_history_id is the
+   * prefix's counter. Read during the serve call only. */
+  const c8mi_particle_state* particles;
+  uint64_t particle_count;
 } c8mi_em_request;
 
 typedef struct c8mi_em_serve_options {

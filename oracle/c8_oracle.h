@@ -17,6 +17,7 @@
 
 #include "c8mi/c8mi_tables.h"
 #include "c8mi/c8mi_em.h"
+#include "c8mi/c8mi_hybrid.h"
 #include "c8mi/c8mi_types.h"
 
 #ifdef __cplusplus
@@ -242,6 +243,37 @@ int oracle_em_run_batch_profiles(const c8mi_em_config* config, const c8mi_native
                                  const c8mi_environment* environment, uint64_t first_shower,
                                  const c8mi_particle_state* primaries, uint64_t shower_count, int threads,
                                  or_em_stats* stats, int64_t* hist, int64_t* counters);
+
+/* Host handoff of one shower (HybridCascade::run, HybridCascade.inl:55-175,
+ * with EmRouter::{canRoute, stage, advanceOneWavefrontAndReturn},
+ * EmRouter.inl:64-423): LIFO host stack, host steps of the particles the
+ * router returns (at most maximum_expansion_rounds consecutive expansion
+ * rounds of batches below the minimum), device calls of the staged batch as
+ * IEmBackend wavefront calls (device pending prefix, then host particles;
+ * remaining particles back to the batch). Host steps are this oracle's step
+ * of one source with history IDs from the shower's counter. Each side's
+ * histograms (4 x profile_bins, caller's), counters, records (caller's
+ * arrays of the configured capacities; counts returned) and compact tracks
+ * (oracle-allocated [track_count][9], released with oracle_free).
+ * prefix_only: stop at the first device call and return its batch in
+ * *staged (oracle_free), the host ledger less the batch; -2 when particles
+ * are left on the host stack at that point. limits NULL: unlimited. */
+typedef struct or_hybrid_side {
+  int64_t* hist;
+  int64_t counters[OR_EM_COUNTERS];
+  c8mi_particle_state* observations;
+  uint64_t observation_count;
+  c8mi_em_fallback* fallbacks;
+  uint64_t fallback_count;
+  double* tracks;
+  uint64_t track_count;
+} or_hybrid_side;
+int oracle_em_run_hybrid(const c8mi_em_config* config, const c8mi_native_table_set* tables,
+                         const c8mi_environment* environment, uint64_t shower_id, uint64_t first_history_id,
+                         const c8mi_particle_state* primaries, uint64_t primary_count,
+                         const c8mi_em_hybrid_config* hc, const c8mi_em_limits* limits, int prefix_only,
+                         or_hybrid_side* host, or_hybrid_side* device, c8mi_particle_state** staged,
+                         uint64_t* staged_count, c8mi_em_hybrid_result* out);
 
 /* Molière probes (analytical correction functions). */
 double oracle_moliere_f1(const c8mi_moliere_snapshot* m, double x);

@@ -1855,6 +1855,10 @@ typedef struct {
   uint64_t next;     /* next secondary history ID of the open call */
   uint64_t limit;    /* exclusive limit of the open lepton call */
   uint64_t min_batch;
+  uint64_t photon_call_fronts;  /* fronts per call (16 and 1024 in the reference) */
+  uint64_t lepton_call_fronts;
+  uint64_t lepton_multiplier;   /* lepton reservation max(3 n, min(chunk, multiplier 3 n)) */
+  uint64_t lepton_chunk;
 } sched_t;
 
 enum { PHOTON_CALL_FRONTS = 16, LEPTON_CALL_FRONTS = 1024, BUCKET_MINIMUM = 256 };
@@ -3509,12 +3513,12 @@ static void open_call(shower_t* s, int lepton, uint64_t n) {
   sched_t* c = &s->sched;
   uint64_t count;
   if (!lepton) {
-    count = n * 2u * PHOTON_CALL_FRONTS;
+    count = n * 2u * c->photon_call_fronts;
   } else {
     uint64_t const minimum = 3u * n;
     count = minimum;
-    if (minimum <= UINT64_MAX / g_lepton_multiplier) {
-      count = minimum * g_lepton_multiplier < g_lepton_chunk ? minimum * g_lepton_multiplier : g_lepton_chunk;
+    if (minimum <= UINT64_MAX / c->lepton_multiplier) {
+      count = minimum * c->lepton_multiplier < c->lepton_chunk ? minimum * c->lepton_multiplier : c->lepton_chunk;
       if (count < minimum) count = minimum;
     }
   }
@@ -3546,7 +3550,7 @@ static int schedule_front(shower_t* s, int lepton) {
     open_call(s, lepton, q->count);
   }
   uint64_t const n = q->count;
-  int stop = n == 0 || c->fronts >= (uint64_t)(lepton ? LEPTON_CALL_FRONTS : PHOTON_CALL_FRONTS);
+  int stop = n == 0 || c->fronts >= (lepton ? c->lepton_call_fronts : c->photon_call_fronts);
   if (!stop && c->fronts != 0 && n < c->min_batch) {
     stop = 1;
     s->stats.below_minimum_batch_checkpoints++;
@@ -3587,6 +3591,10 @@ static int run_shower(const c8mi_em_config* config, const c8mi_native_table_set*
   s.x.shower = shower_id;
   s.sched.counter = first_history_id;
   s.sched.min_batch = config->resident_minimum_batch;
+  s.sched.photon_call_fronts = PHOTON_CALL_FRONTS;
+  s.sched.lepton_call_fronts = LEPTON_CALL_FRONTS;
+  s.sched.lepton_multiplier = g_lepton_multiplier;
+  s.sched.lepton_chunk = g_lepton_chunk;
   s.prof.photons = photons;
   s.prof.electrons = electrons;
   s.prof.positrons = positrons;
@@ -3792,6 +3800,356 @@ int oracle_em_run_batch_profiles(const c8mi_em_config* config, const c8mi_native
     free(fb);
   }
   return failed ? -1 : 0;
+}
+
+/* ===================== host handoff (CPU prefix + device calls) ===================== */
+
+/* One side's accumulators (host steps or device fronts), swapped into the
+ * shower around host steps. */
+typedef struct {
+  prof_t prof;
+  ps_t* observations;
+  uint64_t observation_count;
+  c8mi_em_fallback* fallbacks;
+  uint64_t fallback_count;
+  double* tracks;
+  uint64_t track_count, track_capacity;
+  int track_overflow;
+} side_t;
+
+static void swap_side(shower_t* s, side_t* d) {
+  side_t cur = {s->prof, s->observations, s->observation_count, s->fallbacks, s->fallback_count,
+                s->tracks, s->track_count, s->track_capacity, s->track_overflow};
+  s->prof = d->prof;
+  s->observations = d->observations;
+  s->observation_count = d->observation_count;
+  s->fallbacks = d->fallbacks;
+  s->fallback_count = d->fallback_count;
+  s->tracks = d->tracks;
+  s->track_count = d->track_count;
+  s->track_capacity = d->track_capacity;
+  s->track_overflow = d->track_overflow;
+  *d = cur;
+}
+
+/* HybridCascade::run (HybridCascade.inl:55-175) with EmRouter
+ * (EmRouter.inl:64-423) for an EM shower: the host stack is LIFO
+ * (CpuParticleScheduler::acquireNext, CpuParticleScheduler.inl:27-52); a
+ * particle the router can route is staged, any other (and one the router
+ * returned for a host step, marked cls = 1) takes one host step whose outputs
+ * go on the stack in emission order. The router hands the staged batch to the
+ * device when it holds at least the minimum batch, when device pending queues
+ * are not empty, or after the maximum number of consecutive expansion rounds
+ * (EmRouter.inl:246-423); a smaller batch is put back on the stack for one
+ * host step each (scalar expansion, EmRouter.inl:260-293). */
+typedef struct {
+  shower_t s;      /* device side: queues, schedule, accumulators */
+  side_t host;     /* host side (swapped in for host steps) */
+  queue_t stack;   /* cls 1: returned by an expansion, steps on the host */
+  queue_t staged;
+  queue_t expanded; /* states of the last expansion round */
+  int force_small;
+  uint64_t consecutive;
+  uint64_t min_batch, max_rounds;
+  c8mi_em_limits limits;
+  c8mi_em_hybrid_result r;
+  int prefix_only, stop;
+} hybrid_t;
+
+static int host_can_route(const hybrid_t* h, const ps_t* p) {
+  /* EmRouter::canRoute (EmRouter.inl:64-105): an EM particle the backend can
+   * transport (EmBackend canTransport: finite state, positive energy,
+   * non-negative weight) inside the environment. */
+  if (!(p->pid == 22 || is_ee(p->pid))) return 0;
+  if (!finite(p->energy_gev) || !(p->energy_gev > 0.) || !finite(p->weight) || p->weight < 0.) return 0;
+  for (int a = 0; a < 3; ++a)
+    if (!finite(p->position_m[a]) || !finite(p->direction[a])) return 0;
+  layer_t l = query_environment(h->s.x.env, p->position_m, p->direction, p->environment_node);
+  return l.status == ES_SUCCESS;
+}
+
+static int same_transport_state(const ps_t* a, const ps_t* b) {
+  if (a->pid != b->pid || a->environment_node != b->environment_node || a->medium_id != b->medium_id ||
+      a->energy_gev != b->energy_gev || a->time_s != b->time_s || a->weight != b->weight)
+    return 0;
+  for (int k = 0; k < 3; ++k)
+    if (a->position_m[k] != b->position_m[k] || a->direction[k] != b->direction[k]) return 0;
+  return 1;
+}
+
+static void host_stage(hybrid_t* h, const ps_t* p) {
+  /* EmRouter::stage (EmRouter.inl:115-137) */
+  for (uint64_t i = 0; i < h->expanded.count; ++i)
+    if (h->expanded.items[i].history_id == p->history_id && same_transport_state(&h->expanded.items[i], p)) {
+      h->force_small = 1;
+      break;
+    }
+  push(&h->staged, p, 0);
+  h->r.particles_staged++;
+}
+
+/* One host step: outputs with history IDs from the shower's counter, a
+ * continuation alone or the children in ID order. */
+static uint32_t host_step(hybrid_t* h, const ps_t* p, ps_t out[4]) {
+  shower_t* s = &h->s;
+  or_em_stats const before = s->stats;
+  uint64_t const first = s->sched.counter;
+  uint32_t n = 0, children;
+  swap_side(s, &h->host);
+  uint64_t const tracks_before = s->track_count;
+  if (p->pid == 22) {
+    pout_t o;
+    photon_source(s, p, &o, 0, first);
+    for (uint32_t k = 0; k < o.next_count; ++k) out[n++] = o.next[k];
+    for (uint32_t k = 0; k < o.charged_count; ++k) out[n++] = o.charged[k];
+    children = o.child_count;
+  } else {
+    lout_t o;
+    lepton_source(s, p, &o, 0, first);
+    for (uint32_t k = 0; k < o.next_count; ++k) out[n++] = o.next[k];
+    for (uint32_t k = 0; k < o.photon_count; ++k) out[n++] = o.photons[k];
+    children = o.child_count;
+  }
+  h->r.cpu_tracks += s->track_count - tracks_before;
+  swap_side(s, &h->host);
+  if (children && n == 2 && out[0].history_id > out[1].history_id) {
+    ps_t t = out[0];
+    out[0] = out[1];
+    out[1] = t;
+  }
+  s->sched.counter += children;
+  h->r.cpu_particles++;
+  h->r.cpu_photon_steps += s->stats.photon_steps - before.photon_steps;
+  h->r.cpu_lepton_steps += s->stats.lepton_steps - before.lepton_steps;
+  h->r.cpu_interactions += (s->stats.photon_interactions - before.photon_interactions) +
+                           (s->stats.lepton_interactions - before.lepton_interactions);
+  h->r.cpu_secondaries += s->stats.secondaries - before.secondaries;
+  s->stats = before;
+  return n;
+}
+
+/* A wavefront call (IEmBackend::run{Photon,Lepton}Wavefront): the device
+ * pending prefix and the host particles, the call's fronts, its remaining
+ * particles back to the staged batch. The energy the device keeps leaves the
+ * host ledger for the device's. */
+static void device_call(hybrid_t* h, int lepton, const queue_t* host) {
+  shower_t* s = &h->s;
+  const c8mi_em_config* cfg = s->x.cfg;
+  queue_t* q = lepton ? &s->leptons : &s->photons;
+  for (uint64_t i = 0; i < host->count; ++i) {
+    const ps_t* p = &host->items[i];
+    push(q, p, CLS_HOST);
+    ledger_add(&s->prof, cfg, OR_EMC_INJECTED, p->energy_gev * p->weight);
+    ledger_add(&h->host.prof, cfg, OR_EMC_INJECTED, -(p->energy_gev * p->weight));
+  }
+  uint64_t const n = q->count;
+  if (n > h->r.maximum_input_batch) h->r.maximum_input_batch = n;
+  s->sched.phase = lepton;
+  s->sched.open = 0;
+  while (schedule_front(s, lepton)) {
+    if (lepton) lepton_front(s);
+    else photon_front(s);
+  }
+  for (uint64_t i = 0; i < q->count; ++i) {
+    const ps_t* p = &q->items[i];
+    push(&h->staged, p, 0);
+    ledger_add(&s->prof, cfg, OR_EMC_INJECTED, -(p->energy_gev * p->weight));
+    ledger_add(&h->host.prof, cfg, OR_EMC_INJECTED, p->energy_gev * p->weight);
+  }
+  q->count = 0;
+}
+
+static uint64_t device_pending(const hybrid_t* h) { return h->s.photons.count + h->s.leptons.count; }
+
+/* EmRouter::advanceOneWavefrontAndReturn (EmRouter.inl:246-423). */
+static void router_advance(hybrid_t* h) {
+  uint64_t const pending = device_pending(h);
+  if (h->staged.count == 0 && pending == 0) return;
+  if (h->staged.count < h->min_batch && !h->force_small && pending == 0 && h->consecutive < h->max_rounds) {
+    /* scalar expansion: the batch goes back to the stack for one host step each */
+    h->expanded.count = 0;
+    for (uint64_t i = 0; i < h->staged.count; ++i) {
+      push(&h->stack, &h->staged.items[i], 1);
+      push(&h->expanded, &h->staged.items[i], 0);
+    }
+    h->r.particles_returned_for_expansion += h->staged.count;
+    h->staged.count = 0;
+    h->force_small = 0;
+    h->consecutive++;
+    h->r.small_batch_expansions++;
+    return;
+  }
+  if (h->staged.count < h->min_batch) {
+    if (h->force_small) h->r.stalled_boundary_flushes++;
+    else if (pending == 0 && h->consecutive >= h->max_rounds) h->r.scalar_expansion_budget_flushes++;
+  }
+  h->force_small = 0;
+  h->consecutive = 0;
+  h->expanded.count = 0;
+  if (h->prefix_only) {
+    h->stop = 1;
+    return;
+  }
+  queue_t current = h->staged;
+  memset(&h->staged, 0, sizeof h->staged);
+  if (current.count > h->limits.maximum_resident_input_batch) {
+    for (uint64_t i = h->limits.maximum_resident_input_batch; i < current.count; ++i)
+      push(&h->staged, &current.items[i], 0);
+    current.count = h->limits.maximum_resident_input_batch;
+    h->r.input_batch_splits++;
+  }
+  h->r.wavefronts++;
+  queue_t photons, leptons;
+  memset(&photons, 0, sizeof photons);
+  memset(&leptons, 0, sizeof leptons);
+  for (uint64_t i = 0; i < current.count; ++i)
+    push(current.items[i].pid == 22 ? &photons : &leptons, &current.items[i], 0);
+  queue_free(&current);
+  uint64_t const max_photons = h->limits.maximum_resident_photon_batch;
+  uint64_t const pending_photons = h->s.photons.count < max_photons ? h->s.photons.count : max_photons;
+  if (photons.count > max_photons - pending_photons) {
+    for (uint64_t i = max_photons - pending_photons; i < photons.count; ++i) push(&h->staged, &photons.items[i], 0);
+    photons.count = max_photons - pending_photons;
+    h->r.input_batch_splits++;
+  }
+  if (photons.count + pending_photons) device_call(h, 0, &photons);
+  uint64_t const max_leptons = h->limits.maximum_resident_lepton_batch;
+  uint64_t const pending_leptons = h->s.leptons.count < max_leptons ? h->s.leptons.count : max_leptons;
+  if (leptons.count + h->s.leptons.count) {
+    if (leptons.count > max_leptons - pending_leptons) {
+      for (uint64_t i = max_leptons - pending_leptons; i < leptons.count; ++i)
+        push(&h->staged, &leptons.items[i], 0);
+      leptons.count = max_leptons - pending_leptons;
+      h->r.input_batch_splits++;
+    }
+    device_call(h, 1, &leptons);
+  }
+  queue_free(&photons);
+  queue_free(&leptons);
+}
+
+static int router_ready(const hybrid_t* h) {
+  /* EmRouter::readyForScalarInterleave (EmRouter.inl:150-167) */
+  return h->force_small || device_pending(h) != 0 || h->staged.count >= h->min_batch;
+}
+
+static int router_pending(const hybrid_t* h) { return h->staged.count != 0 || device_pending(h) != 0; }
+
+static void hybrid_cascade(hybrid_t* h) {
+  while (!h->stop && (h->stack.count || router_pending(h))) {
+    while (!h->stop && h->stack.count) {
+      ps_t const p = h->stack.items[--h->stack.count];
+      int const marked = h->stack.cls[h->stack.count];
+      if (!marked && host_can_route(h, &p)) {
+        host_stage(h, &p);
+        while (!h->stop && router_ready(h)) router_advance(h);
+        continue;
+      }
+      ps_t out[4];
+      uint32_t const n = host_step(h, &p, out);
+      for (uint32_t k = 0; k < n; ++k) push(&h->stack, &out[k], 0);
+    }
+    if (!h->stop && router_pending(h)) router_advance(h);
+  }
+}
+
+int oracle_em_run_hybrid(const c8mi_em_config* config, const c8mi_native_table_set* tables,
+                         const c8mi_environment* environment, uint64_t shower_id, uint64_t first_history_id,
+                         const c8mi_particle_state* primaries, uint64_t primary_count,
+                         const c8mi_em_hybrid_config* hc, const c8mi_em_limits* limits, int prefix_only,
+                         or_hybrid_side* host, or_hybrid_side* device, c8mi_particle_state** staged,
+                         uint64_t* staged_count, c8mi_em_hybrid_result* out) {
+  hybrid_t* h = calloc(1, sizeof *h);
+  if (!h) return -1;
+  shower_t* s = &h->s;
+  s->record_tracks = 1;
+  s->x.t = tables;
+  s->x.cont.t = tables;
+  s->x.cont.em_cut_mev = config->em_transport_cut_mev;
+  s->x.cont.muon_cut_mev = config->muon_transport_cut_mev;
+  s->x.env = environment;
+  s->x.cfg = config;
+  s->x.seed = config->random_seed;
+  s->x.shower = shower_id;
+  s->sched.counter = first_history_id;
+  h->min_batch = hc && hc->minimum_batch_size ? hc->minimum_batch_size : 16384u;
+  h->max_rounds = hc && hc->maximum_expansion_rounds ? hc->maximum_expansion_rounds : 8u;
+  s->sched.min_batch = h->min_batch;
+  s->sched.photon_call_fronts = hc && hc->photon_wavefronts ? hc->photon_wavefronts : PHOTON_CALL_FRONTS;
+  s->sched.lepton_call_fronts = hc && hc->lepton_wavefronts ? hc->lepton_wavefronts : LEPTON_CALL_FRONTS;
+  s->sched.lepton_multiplier = hc && hc->lepton_reservation_multiplier ? hc->lepton_reservation_multiplier : 64u;
+  s->sched.lepton_chunk = hc && hc->lepton_reservation_chunk ? hc->lepton_reservation_chunk : ((uint64_t)1 << 20);
+  if (limits) {
+    h->limits = *limits;
+  } else {
+    h->limits.maximum_resident_photon_batch = UINT64_MAX;
+    h->limits.maximum_resident_lepton_batch = UINT64_MAX;
+    h->limits.maximum_resident_input_batch = UINT64_MAX;
+  }
+  h->prefix_only = prefix_only;
+  uint64_t const bins = config->profile_bins;
+  s->prof.photons = device->hist;
+  s->prof.electrons = device->hist + bins;
+  s->prof.positrons = device->hist + 2 * bins;
+  s->prof.energy_loss = device->hist + 3 * bins;
+  s->observations = device->observations;
+  s->fallbacks = device->fallbacks;
+  h->host.prof.photons = host->hist;
+  h->host.prof.electrons = host->hist + bins;
+  h->host.prof.positrons = host->hist + 2 * bins;
+  h->host.prof.energy_loss = host->hist + 3 * bins;
+  h->host.observations = host->observations;
+  h->host.fallbacks = host->fallbacks;
+  for (uint64_t i = 0; i < primary_count; ++i) {
+    ledger_add(&h->host.prof, config, OR_EMC_INJECTED, primaries[i].energy_gev * primaries[i].weight);
+    push(&h->stack, &primaries[i], 0);
+  }
+  hybrid_cascade(h);
+  int const stranded = prefix_only && h->stack.count != 0;
+  if (prefix_only) {
+    /* the handed batch leaves the host ledger */
+    for (uint64_t i = 0; i < h->staged.count; ++i)
+      ledger_add(&h->host.prof, config, OR_EMC_INJECTED,
+                 -(h->staged.items[i].energy_gev * h->staged.items[i].weight));
+  }
+  c8mi_em_hybrid_result* r = &h->r;
+  r->photon_calls = s->stats.photon_calls;
+  r->lepton_calls = s->stats.lepton_calls;
+  r->device_photon_fronts = s->stats.photon_fronts;
+  r->device_lepton_fronts = s->stats.lepton_fronts;
+  r->device_photon_steps = s->stats.photon_steps;
+  r->device_lepton_steps = s->stats.lepton_steps;
+  r->device_interactions = s->stats.photon_interactions + s->stats.lepton_interactions;
+  r->device_secondaries = s->stats.secondaries;
+  r->bucketed_fronts = s->stats.bucketed_fronts;
+  r->reserved_history_ids = s->stats.reserved_history_ids;
+  r->below_minimum_batch_checkpoints = s->stats.below_minimum_batch_checkpoints;
+  r->history_range_checkpoints = s->stats.history_range_checkpoints;
+  r->next_history_id = s->sched.counter;
+  *out = *r;
+  memcpy(device->counters, s->prof.counters, sizeof s->prof.counters);
+  memcpy(host->counters, h->host.prof.counters, sizeof h->host.prof.counters);
+  device->observation_count = s->observation_count;
+  device->fallback_count = s->fallback_count;
+  device->tracks = s->tracks;
+  device->track_count = s->track_count;
+  host->observation_count = h->host.observation_count;
+  host->fallback_count = h->host.fallback_count;
+  host->tracks = h->host.tracks;
+  host->track_count = h->host.track_count;
+  int const overflow = s->track_overflow || h->host.track_overflow;
+  if (staged && staged_count) {
+    *staged = h->staged.items;
+    *staged_count = h->staged.count;
+    h->staged.items = NULL;
+  }
+  queue_free(&h->staged);
+  queue_free(&h->stack);
+  queue_free(&h->expanded);
+  queue_free(&s->photons);
+  queue_free(&s->leptons);
+  free(h);
+  return overflow ? -1 : stranded ? -2 : 0;
 }
 
 /* ---- single-function probes for the tests ---- */

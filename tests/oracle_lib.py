@@ -229,6 +229,77 @@ def em_run_shower(config, tables, environment, primaries, shower_id=1, first_his
     return out
 
 
+class OrHybridSide(ctypes.Structure):
+    _fields_ = [("hist", ctypes.c_void_p), ("counters", ctypes.c_int64 * 64),
+                ("observations", ctypes.c_void_p), ("observation_count", c_uint64),
+                ("fallbacks", ctypes.c_void_p), ("fallback_count", c_uint64),
+                ("tracks", ctypes.c_void_p), ("track_count", c_uint64)]
+
+
+def em_run_hybrid(config, tables, environment, primaries, shower_id=1, first_history_id=2, hybrid=None,
+                  limits=None, prefix_only=False):
+    """The oracle's restatement of the host handoff (oracle_em_run_hybrid):
+    per side ("host" steps, "device" fronts) the int64 histograms, counters,
+    records and compact tracks ([9][n]); the staged batch (prefix_only) and
+    the counts."""
+    from c8mi import PARTICLE_STATE_DTYPE
+    from c8mi.em import FALLBACK_DTYPE, HybridResult, Limits
+
+    lib = load()
+    cfg = config.struct if hasattr(config, "struct") else config
+    env = environment.struct if hasattr(environment, "struct") else environment
+    bins = cfg.profile_bins
+    sides, keep = {}, []
+    for name in ("host", "device"):
+        hist = np.zeros(4 * bins, dtype=np.int64)
+        obs = np.zeros(cfg.observation_capacity, dtype=PARTICLE_STATE_DTYPE)
+        fbs = np.zeros(cfg.fallback_capacity, dtype=FALLBACK_DTYPE)
+        side = OrHybridSide()
+        side.hist = hist.ctypes.data_as(c_void_p)
+        side.observations = obs.ctypes.data_as(c_void_p)
+        side.fallbacks = fbs.ctypes.data_as(c_void_p)
+        sides[name] = (side, hist, obs, fbs)
+    prim = np.ascontiguousarray(primaries, dtype=PARTICLE_STATE_DTYPE).reshape(-1)
+    staged, n_staged = c_void_p(), c_uint64()
+    result = HybridResult()
+    lim = None
+    if limits is not None:
+        lim = Limits()
+        for k, v in limits.items():
+            setattr(lim, k, v)
+    status = lib.oracle_em_run_hybrid(
+        byref(cfg), byref(tables.struct), byref(env), c_uint64(shower_id), c_uint64(first_history_id),
+        prim.ctypes.data_as(c_void_p), c_uint64(prim.size), byref(hybrid) if hybrid is not None else None,
+        byref(lim) if lim is not None else None, ctypes.c_int(1 if prefix_only else 0), byref(sides["host"][0]),
+        byref(sides["device"][0]), byref(staged), byref(n_staged), byref(result))
+    out = {"result": result.as_dict()}
+    for name, (side, hist, obs, fbs) in sides.items():
+        tracks = np.zeros((9, 0))
+        if side.tracks:
+            if side.track_count:
+                rows = np.ctypeslib.as_array(ctypes.cast(side.tracks, ctypes.POINTER(ctypes.c_double)),
+                                             shape=(side.track_count * 9,))
+                tracks = np.ascontiguousarray(rows.reshape(-1, 9).T.copy())
+            lib.oracle_free(side.tracks)
+        h = hist.reshape(4, bins)
+        out[name] = dict(hist=dict(photons=h[0].copy(), electrons=h[1].copy(), positrons=h[2].copy(),
+                                   energy_loss=h[3].copy()),
+                         counters=np.array(side.counters, dtype=np.int64),
+                         observations=obs[:min(side.observation_count, obs.size)],
+                         fallbacks=fbs[:min(side.fallback_count, fbs.size)], tracks=tracks)
+    batch = np.zeros(0, dtype=PARTICLE_STATE_DTYPE)
+    if staged:
+        if n_staged.value:
+            raw = np.ctypeslib.as_array(ctypes.cast(staged, ctypes.POINTER(ctypes.c_uint8)),
+                                        shape=(n_staged.value * PARTICLE_STATE_DTYPE.itemsize,))
+            batch = raw.view(PARTICLE_STATE_DTYPE).copy()
+        lib.oracle_free(staged)
+    out["staged"] = batch
+    if status != 0:
+        raise RuntimeError(f"oracle_em_run_hybrid failed ({status})")
+    return out
+
+
 VERTEX_DTYPE = np.dtype([
     ("kind", "<i4"), ("pid", "<i4"), ("process", "<i4"), ("accepted", "<u4"), ("count", "<u4"),
     ("pad", "<u4"), ("energy", "<f8"), ("mass", "<f8"), ("target_mass", "<f8"), ("direction", "<f8", 3),

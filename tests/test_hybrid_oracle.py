@@ -1,0 +1,166 @@
+"""Host handoff on the CPU: the product's host stepper and prefix against the
+oracle, and the oracle's restatement of the router pinned by what its
+definition fixes.
+
+* The CPU prefix of a shower (HybridCascade::run with EmRouter's scalar
+  expansion, EmRouter.inl:260-293, up to the first device call): the
+  product runs the device path's step functions compiled for the host
+  (c8mi_em_hybrid_prefix); the oracle runs its own serial restatement of the
+  step. The batch handed to the device, the history counter, every host
+  e+- track, the host ledger and histograms, observations and fallbacks are
+  equal, bit for bit: both sides compute in FP64 without contraction with the
+  same libm, and the profile is int64 fixed point.
+* The router with a minimum batch of one hands every staged particle to the
+  device at once, so no host step happens and the router's calls are the
+  device-resident schedule: the oracle's hybrid equals its resident run
+  (oracle_em_run_shower with resident_minimum_batch 1) in every count and in
+  the ledger.
+* Each side closes its own energy ledger: the host books the primaries as
+  injected and gives the device what its calls keep.
+"""
+import numpy as np
+import pytest
+
+import oracle_lib
+from c8mi import airtables, em
+
+START = (0.0, 0.0, 112750.0)
+DOWN = (0.0, 0.0, -1.0)
+FIELDS = ("pid", "medium_id", "generation", "energy_gev", "position_m", "direction", "time_s", "weight",
+          "history_id", "parent_history_id", "step_id", "environment_node")
+
+
+@pytest.fixture(scope="module")
+def air():
+    return airtables.build_air_tables()
+
+
+def same_particles(a, b):
+    assert a.size == b.size
+    for f in FIELDS:
+        assert np.array_equal(a[f], b[f]), f
+
+
+def scene(air, kind, seed):
+    if kind == "ice":
+        ice = em.IceVolume()
+        tables = airtables.build_ice_tables(cut_mev=0.1)
+        cfg = em.make_config(airtables.IceMedium, seed=seed, cut_mev=0.1, start=(0.0, 0.0, -100.0),
+                             atmosphere=ice, moliere=True, moliere_table=True)
+        return cfg, tables, ice, (0.0, 0.0, -100.0)
+    field = (0.0, 5.0e-5, 0.0) if kind == "field" else (0.0, 0.0, 0.0)
+    atm = em.Atmosphere(magnetic_field_t=field)
+    moliere = kind in ("table", "thinned", "field", "analytic")
+    cfg = em.make_config(airtables, seed=seed, atmosphere=atm, moliere=moliere,
+                         moliere_table=True if kind in ("table", "thinned", "field") else None,
+                         thinning=(0.5, 50.0) if kind == "thinned" else None)
+    return cfg, air, atm, START
+
+
+CASES = [("table", 22, 100.0, 1), ("table", 11, 10000.0, 2), ("none", -11, 5.0, 3), ("analytic", 22, 30.0, 4),
+         ("thinned", 22, 300.0, 5), ("field", 11, 50.0, 6), ("ice", 11, 1000.0, 7)]
+
+
+@pytest.mark.parametrize("kind,pid,energy,seed", CASES)
+def test_host_prefix_equals_oracle(air, kind, pid, energy, seed):
+    cfg, tables, env, start = scene(air, kind, seed)
+    prim = em.particle(pid, energy, start, DOWN, history_id=1)
+    stepper = em.ScalarStepper(cfg, tables, env)
+    batch, next_history, tracks, result = stepper.prefix(prim, seed)
+    hist, ledger = stepper.download_profile()
+    obs, n_obs = stepper.observations()
+    fbs, n_fb = stepper.fallbacks()
+    stepper.close()
+    want = oracle_lib.em_run_hybrid(cfg, tables, env, prim, shower_id=seed, prefix_only=True)
+    same_particles(batch, want["staged"])
+    assert next_history == want["result"]["next_history_id"]
+    for k in em.HYBRID_COUNTS:
+        assert result[k] == want["result"][k], k
+    assert result["small_batch_expansions"] == 8 and result["scalar_expansion_budget_flushes"] == 1
+    assert result["cpu_particles"] == result["particles_returned_for_expansion"] > 8
+    host = want["host"]
+    assert np.array_equal(tracks.T, host["tracks"])
+    assert tracks.shape[0] == result["cpu_tracks"] > 0
+    es, ws = cfg.struct.profile_energy_scale, cfg.struct.profile_weight_scale
+    got = em.decode_counters(host["counters"], es)
+    for k in em.LEDGER_ENERGY_TERMS:
+        assert ledger[k] == got[k], k
+    for k in ("steps", "photon_cuts", "lepton_limits", "deposited_steps", "fallback_reasons",
+              "thinning_hillas_vertices", "thinning_statistical_vertices", "thinning_particles_discarded"):
+        assert ledger[k] == got[k], k
+    for k in ("photons", "electrons", "positrons"):
+        assert np.array_equal(hist[k], host["hist"][k] / ws), k
+    assert np.array_equal(hist["energy_loss"], host["hist"]["energy_loss"] / es)
+    same_particles(obs, host["observations"])
+    assert n_fb == host["fallbacks"].size
+    same_particles(fbs["particle"], host["fallbacks"]["particle"])
+    # the host side closes: what it kept of the primary is deposited or handed on
+    if ledger["fallback_reasons"][21] == 0:
+        assert abs(em.ledger_residual(ledger)) <= 1e-9 * energy
+
+
+def test_host_steps_take_the_next_history_ids(air):
+    cfg, tables, env, start = scene(air, "table", 11)
+    stepper = em.ScalarStepper(cfg, tables, env)
+    stepper.begin_shower(11)
+    p = em.particle(22, 50.0, start, DOWN, history_id=1)
+    counter, seen = 1000, 0
+    for _ in range(200):
+        out, children, track = stepper.step(p, counter)
+        assert track is None  # photons leave no radio track
+        if children:
+            assert list(out["history_id"]) == list(range(counter, counter + children))
+            assert np.all(out["parent_history_id"] == p["history_id"])
+            seen += 1
+            break
+        assert out.size == 1 and out["history_id"][0] == p["history_id"]
+        assert out["step_id"][0] == p["step_id"] + 1
+        p = out[0]
+    assert seen == 1
+    stepper.close()
+
+
+@pytest.mark.parametrize("pid,energy,seed", [(22, 30.0, 21), (11, 10.0, 22)])
+def test_router_with_unit_batch_is_the_resident_schedule(air, pid, energy, seed):
+    atm = em.Atmosphere()
+    cfg = em.make_config(airtables, seed=seed, atmosphere=atm, moliere=True, moliere_table=True,
+                         resident_minimum_batch=1)
+    prim = em.particle(pid, energy, START, DOWN, history_id=1)
+    hybrid = oracle_lib.em_run_hybrid(cfg, air, atm, prim, shower_id=seed, hybrid=em.hybrid_config(minimum_batch=1))
+    resident = oracle_lib.em_run_shower(cfg, air, atm, prim, shower_id=seed)
+    r, w = hybrid["result"], resident["stats"]
+    assert r["cpu_particles"] == 0 and r["small_batch_expansions"] == 0
+    assert r["device_photon_steps"] == w["photon_steps"] and r["device_lepton_steps"] == w["lepton_steps"]
+    assert r["device_photon_fronts"] == w["photon_fronts"] and r["device_lepton_fronts"] == w["lepton_fronts"]
+    assert r["photon_calls"] == w["photon_calls"] and r["lepton_calls"] == w["lepton_calls"]
+    assert r["device_secondaries"] == w["secondaries"] and r["next_history_id"] == w["next_history_id"]
+    assert r["reserved_history_ids"] == w["reserved_history_ids"]
+    assert r["bucketed_fronts"] == w["bucketed_fronts"]
+    dev = hybrid["device"]["counters"]
+    # every energy term but the injected one (the host books the primary, the
+    # device what it keeps) is the resident run's
+    assert np.array_equal(dev[1:], resident["counters"][1:])
+    assert dev[0] + hybrid["host"]["counters"][0] == resident["counters"][0]
+    for k in ("photons", "electrons", "positrons", "energy_loss"):
+        assert np.array_equal(hybrid["device"]["hist"][k], resident["hist"][k]), k
+
+
+@pytest.mark.parametrize("pid,energy,seed,minimum", [(22, 100.0, 31, 16384), (11, 30.0, 32, 64),
+                                                     (-11, 20.0, 33, 8)])
+def test_each_side_closes_its_ledger(air, pid, energy, seed, minimum):
+    atm = em.Atmosphere()
+    cfg = em.make_config(airtables, seed=seed, atmosphere=atm)
+    prim = em.particle(pid, energy, START, DOWN, history_id=1)
+    out = oracle_lib.em_run_hybrid(cfg, air, atm, prim, shower_id=seed,
+                                   hybrid=em.hybrid_config(minimum_batch=minimum))
+    r = out["result"]
+    es = cfg.struct.profile_energy_scale
+    host = em.decode_counters(out["host"]["counters"], es)
+    dev = em.decode_counters(out["device"]["counters"], es)
+    assert abs(host["injected_energy"] + dev["injected_energy"] - energy) <= 1e-9 * energy
+    for side in (host, dev):
+        assert side["pending_energy"] == 0.0
+        assert abs(em.ledger_residual(side)) <= 1e-9 * energy
+    assert r["small_batch_expansions"] >= 8 and r["cpu_particles"] > 0 and r["wavefronts"] > 0
+    assert r["cpu_tracks"] == out["host"]["tracks"].shape[1]
+    assert r["device_lepton_steps"] == out["device"]["tracks"].shape[1]
