@@ -299,6 +299,18 @@ class RequestResult(ctypes.Structure):
         return out
 
 
+FIRST_INTERACTION_DTYPE = np.dtype({"names": ["parent_at_vertex", "secondaries", "process_id", "secondary_count"],
+                                    "formats": [PARTICLE_STATE_DTYPE, (PARTICLE_STATE_DTYPE, 3), "<i4", "<u4"],
+                                    "offsets": [0, 128, 512, 516], "itemsize": 528})
+
+
+def _first_interaction(call):
+    out = np.zeros(1, dtype=FIRST_INTERACTION_DTYPE)
+    captured = c_int(0)
+    _check(call(out.ctypes.data_as(c_void_p), byref(captured)))
+    return out[0] if captured.value else None
+
+
 FALLBACK_DTYPE = np.dtype({"names": ["particle", "reason", "process_id", "stage"],
                            "formats": [PARTICLE_STATE_DTYPE, "<i4", "<i4", "<u4"],
                            "offsets": [0, 128, 132, 136], "itemsize": 144})
@@ -727,6 +739,15 @@ class EmBackend:
                                                     c_uint64(out.size), byref(n)))
         return out[:min(n.value, out.size)], n.value
 
+    def first_interaction(self):
+        """IEmBackend::downloadFirstInteractionSnapshot: a FIRST_INTERACTION_DTYPE
+        record, or None."""
+        return _first_interaction(lambda o, c: library().c8mi_em_download_first_interaction(self.handle, o, c))
+
+    def slot_first_interaction(self, slot):
+        return _first_interaction(
+            lambda o, c: library().c8mi_em_slot_download_first_interaction(self.handle, c_uint32(slot), o, c))
+
     # ---- shower pool (c8mi_em_slot_*) ----
 
     def slot_begin(self, slot, shower_id, first_history_id=2, axis=None):
@@ -980,6 +1001,10 @@ class ScalarStepper:
         _check(library().c8mi_em_scalar_download_fallbacks(self.handle, out.ctypes.data_as(c_void_p),
                                                            c_uint64(out.size), byref(n)))
         return out[:min(n.value, out.size)], n.value
+
+    def first_interaction(self):
+        return _first_interaction(
+            lambda o, c: library().c8mi_em_scalar_download_first_interaction(self.handle, o, c))
 
     def prefix(self, primaries, shower_id, first_history_id=2, config=None, capacity=1 << 16,
                track_capacity=1 << 16):

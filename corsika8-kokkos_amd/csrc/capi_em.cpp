@@ -218,6 +218,19 @@ int c8mi_em_download_fallbacks(c8mi_em* em, c8mi_em_fallback* out, uint64_t capa
   });
 }
 
+int c8mi_em_download_first_interaction(c8mi_em* em, c8mi_em_first_interaction* out, int* captured) {
+  return c8mi_em_slot_download_first_interaction(em, 0, out, captured);
+}
+
+int c8mi_em_slot_download_first_interaction(c8mi_em* em, uint32_t slot, c8mi_em_first_interaction* out,
+                                            int* captured) {
+  return guarded([&] {
+    require_handle(em);
+    if (!out || !captured) throw std::invalid_argument("null argument");
+    *captured = em->backend.slot_download_first_interaction(slot, *out) ? 1 : 0;
+  });
+}
+
 int c8mi_em_slot_begin(c8mi_em* em, uint32_t slot, uint64_t shower_id, uint64_t first_history_id,
                        c8mi_em_shower_axis const* axis) {
   return guarded([&] {

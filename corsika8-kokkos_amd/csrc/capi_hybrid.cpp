@@ -116,6 +116,15 @@ int c8mi_em_scalar_download_fallbacks(c8mi_em_scalar const* scalar, c8mi_em_fall
   });
 }
 
+int c8mi_em_scalar_download_first_interaction(c8mi_em_scalar const* scalar, c8mi_em_first_interaction* out,
+                                              int* captured) {
+  return guarded([&] {
+    require_handle(scalar);
+    if (!out || !captured) throw std::invalid_argument("null argument");
+    *captured = scalar->stepper.first_interaction(*out) ? 1 : 0;
+  });
+}
+
 int c8mi_em_hybrid_create(c8mi_em* em, c8mi_em_scalar* scalar, c8mi_em_hybrid_config const* config,
                           c8mi_em_hybrid** out) {
   return guarded([&] {

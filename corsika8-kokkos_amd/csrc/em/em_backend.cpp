@@ -199,6 +199,8 @@ EmBackend::EmBackend(c8mi_em_config const& config, c8mi_native_table_set const& 
   observations_.allocate(slots_ * nonzero(config.observation_capacity));
   fallbacks_.allocate(slots_ * nonzero(config.fallback_capacity));
   record_counts_.allocate(2u * slots_);
+  first_interactions_.allocate(slots_);
+  first_candidates_.allocate(slots_);
   history_.allocate(slots_);
   stats_.allocate(slots_ * static_cast<std::uint64_t>(kSlotStats));
   shower_totals_.allocate(static_cast<std::size_t>(kMaxShowerSlots) * kMaxRuns * kShowerFields);
@@ -207,7 +209,7 @@ EmBackend::EmBackend(c8mi_em_config const& config, c8mi_native_table_set const& 
   host_report_ = PinnedBuffer<unsigned long long>(kReportHeader + kReportPerSlot * static_cast<std::size_t>(slots_));
   control_dev_.allocate(slots_);
   control_host_ = PinnedBuffer<SlotControl>(slots_);
-  for (auto* b : {&history_, &stats_, &shower_totals_, &record_counts_, &adjust_})
+  for (auto* b : {&history_, &stats_, &shower_totals_, &record_counts_, &adjust_, &first_candidates_})
     C8MI_HIP(hipMemset(b->data(), 0, b->bytes()));
   C8MI_HIP(hipMemset(counters_.data(), 0, counters_.bytes()));
   C8MI_HIP(hipMemset(profile_.data(), 0, profile_.bytes()));
@@ -250,6 +252,8 @@ EmBackend::EmBackend(c8mi_em_config const& config, c8mi_native_table_set const& 
   h.observations = observations_.data();
   h.fallbacks = fallbacks_.data();
   h.record_counts = record_counts_.data();
+  h.first_interactions = first_interactions_.data();
+  h.first_candidates = first_candidates_.data();
   h.counters = counters_.data();
   h.profile = profile_.data();
   context_.allocate(1);
@@ -287,6 +291,7 @@ PoolArrays EmBackend::pool_arrays() const {
   p.profile = profile_.data();
   p.counters = counters_.data();
   p.record_counts = record_counts_.data();
+  p.first_candidates = first_candidates_.data();
   p.stats = stats_.data();
   p.history = history_.data();
   p.bins = static_cast<std::uint32_t>(nonzero(config_.profile_bins));
@@ -1170,6 +1175,16 @@ std::uint64_t EmBackend::slot_download_records(std::uint32_t slot, int kind, voi
     C8MI_HIP(hipMemcpy(out, base + slot * region * record_bytes, copy * record_bytes, hipMemcpyDeviceToHost));
   }
   return n;
+}
+
+bool EmBackend::slot_download_first_interaction(std::uint32_t slot, c8mi_em_first_interaction& out) {
+  check_slot(slot);
+  C8MI_HIP(hipStreamSynchronize(stream_));
+  unsigned long long candidates = 0;
+  C8MI_HIP(hipMemcpy(&candidates, first_candidates_.data() + slot, sizeof candidates, hipMemcpyDeviceToHost));
+  if (candidates == 0) return false;
+  C8MI_HIP(hipMemcpy(&out, first_interactions_.data() + slot, sizeof out, hipMemcpyDeviceToHost));
+  return true;
 }
 
 void EmBackend::slot_pending(std::uint32_t slot, std::uint64_t& photons, std::uint64_t& leptons) const {

@@ -101,6 +101,8 @@ class EmBackend {
                              double* energy_loss, c8mi_em_ledger* ledger);
   std::uint64_t slot_download_records(std::uint32_t slot, int kind, void* out, std::size_t record_bytes,
                                       std::uint64_t capacity);
+  // The slot's first-interaction snapshot; false when none was captured.
+  bool slot_download_first_interaction(std::uint32_t slot, c8mi_em_first_interaction& out);
   void slot_end(std::uint32_t slot);
   void slot_pending(std::uint32_t slot, std::uint64_t& photons, std::uint64_t& leptons) const;
   void serve(c8mi_em_request const* requests, std::uint64_t count, c8mi_em_serve_options const& options,
@@ -201,6 +203,8 @@ class EmBackend {
   DeviceBuffer<c8mi_particle_state> observations_;
   DeviceBuffer<c8mi_em_fallback> fallbacks_;
   DeviceBuffer<unsigned long long> record_counts_;
+  DeviceBuffer<c8mi_em_first_interaction> first_interactions_;
+  DeviceBuffer<unsigned long long> first_candidates_;
   DeviceBuffer<unsigned long long> history_;
   DeviceBuffer<unsigned long long> stats_;
   DeviceBuffer<unsigned long long> shower_totals_;

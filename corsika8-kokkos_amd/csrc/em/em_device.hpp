@@ -110,6 +110,8 @@ struct EmContext {
   unsigned long long* record_counts;  // per slot: [0] observations, [1] fallbacks
   long long* counters;                // per slot: kCounters
   long long* profile;                 // per slot: 4 * profile_bins
+  c8mi_em_first_interaction* first_interactions;  // per slot (null: not captured)
+  unsigned long long* first_candidates;           // per slot: final states of generation zero so far
 };
 
 // Struct-of-arrays particle queue (ParticleSoA, corsika/kokkos/data/ParticleSoA.hpp:27-82):

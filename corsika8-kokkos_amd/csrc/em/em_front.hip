@@ -709,6 +709,7 @@ __global__ __launch_bounds__(256) void slot_init_kernel(PoolArrays pool, SlotIni
   if (threadIdx.x < kSlotStats) pool.stats[slot * kSlotStats + threadIdx.x] = 0;
   if (threadIdx.x < 2) pool.record_counts[2 * slot + threadIdx.x] = 0;
   if (threadIdx.x == 0) {
+    pool.first_candidates[slot] = 0;
     pool.history[slot] = init.first_history;
     pool.showers[slot] = init.shower;
   }

@@ -95,6 +95,7 @@ struct PoolArrays {
   long long* profile;
   long long* counters;
   unsigned long long* record_counts;
+  unsigned long long* first_candidates;
   unsigned long long* stats;
   unsigned long long* history;
   std::uint32_t bins;

@@ -61,6 +61,8 @@ HostStepper::HostStepper(c8mi_em_config const& config, c8mi_native_table_set con
   h.observations = observations_.data();
   h.fallbacks = fallbacks_.data();
   h.record_counts = record_counts_;
+  h.first_interactions = &first_interaction_;
+  h.first_candidates = &first_candidates_;
   h.counters = counters_.data();
   h.profile = profile_.data();
 }
@@ -70,6 +72,7 @@ void HostStepper::begin_shower(std::uint64_t shower_id) {
   std::fill(counters_.begin(), counters_.end(), 0);
   std::fill(profile_.begin(), profile_.end(), 0);
   record_counts_[0] = record_counts_[1] = 0;
+  first_candidates_ = 0;
 }
 
 bool HostStepper::can_route(c8mi_particle_state const& p) const {

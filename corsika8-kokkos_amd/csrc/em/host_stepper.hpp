@@ -51,6 +51,11 @@ class HostStepper {
   std::uint64_t fallback_count() const noexcept { return record_counts_[1]; }
   c8mi_particle_state const* observations() const noexcept { return observations_.data(); }
   c8mi_em_fallback const* fallbacks() const noexcept { return fallbacks_.data(); }
+  bool first_interaction(c8mi_em_first_interaction& out) const {
+    if (first_candidates_ == 0) return false;
+    out = first_interaction_;
+    return true;
+  }
   c8mi_em_config const& config() const noexcept { return config_; }
 
  private:
@@ -67,6 +72,8 @@ class HostStepper {
   std::vector<c8mi_particle_state> observations_;
   std::vector<c8mi_em_fallback> fallbacks_;
   unsigned long long record_counts_[2] = {0, 0};
+  c8mi_em_first_interaction first_interaction_{};
+  unsigned long long first_candidates_ = 0;
   ShowerSlot shower_{};
   EmContext context_{};
 };

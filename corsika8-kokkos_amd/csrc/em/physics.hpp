@@ -262,6 +262,27 @@ C8MI_HD void record_fallback(Accumulator const& a, Particle const& p, int reason
   }
 }
 
+// The writers' first-interaction snapshot (PhotonFront.inl:95-111,
+// LeptonFinalStateStep.inl:609-615): the first final state of a
+// generation-zero parent, its children before thinning with the parent's
+// weight. The first candidate of the shower writes it.
+C8MI_HD void first_interaction(Accumulator const& a, Particle const& parent, std::int32_t process,
+                               Particle const& first, Particle const* second) {
+  EmContext const& c = *a.ctx;
+  if (parent.generation != 0 || c.first_interactions == nullptr) return;
+  if (fetch_add(&c.first_candidates[a.slot], 1ULL) != 0) return;
+  c8mi_em_first_interaction& r = c.first_interactions[a.slot];
+  r.parent_at_vertex = parent;
+  r.secondaries[0] = first;
+  r.secondaries[0].weight = parent.weight;
+  c8mi_particle_state none{};
+  r.secondaries[1] = second ? static_cast<c8mi_particle_state const&>(*second) : none;
+  r.secondaries[1].weight = second ? parent.weight : 0.;
+  r.secondaries[2] = none;
+  r.process_id = process;
+  r.secondary_count = second ? 2u : 1u;
+}
+
 // ---- table facades ----
 
 C8MI_HD int from_native(std::uint32_t status, bool continuous) {

@@ -160,6 +160,7 @@ C8MI_HD void photon_step(EmContext const& c, Accumulator const& acc, Particle co
     deflect(parent.direction, pc, az, ph.direction, false);
     Particle el = child_of(parent, 11, e2, th.w2);
     deflect(parent.direction, ec, fmod(az + kPhotonPi, kPhotonTwoPi), el.direction, false);
+    first_interaction(acc, parent, kCompton, ph, &el);
     std::uint32_t k = 0;
     if (th.keep & 1U) {
       ph.history_id = k++;
@@ -184,6 +185,7 @@ C8MI_HD void photon_step(EmContext const& c, Accumulator const& acc, Particle co
     Particle el = child_of(parent, 11,
                            secondary_energy(11, kElectronMassGeV + parent.energy_gev * frac, kElectronMassGeV),
                            parent.weight);
+    first_interaction(acc, parent, kPhotoelectric, el, nullptr);
     el.history_id = 0;
     emit(o, el, kSlotOtherChild);
     ledger(acc, kMassCorrection, mass_correction(11, kElectronMassGeV, parent.weight));
@@ -229,6 +231,7 @@ C8MI_HD void photon_step(EmContext const& c, Accumulator const& acc, Particle co
   deflect(parent.direction, sauter_cosine(ee, eu), az, el.direction, false);
   Particle po = child_of(parent, -11, e2, th.w2);
   deflect(parent.direction, sauter_cosine(pe, pu), fmod(az + kPhotonPi, kPhotonTwoPi), po.direction, false);
+  first_interaction(acc, parent, kPhotonPair, el, &po);
   std::uint32_t k = 0;
   if (th.keep & 1U) {
     el.history_id = k++;
@@ -717,6 +720,7 @@ C8MI_HD void lepton_vertex(EmContext const& c, Accumulator const& acc, LeptonVer
     Particle g2 = child_of(parent, 22, e2, th.w2);
     deflect(parent.direction, c1, az, g1.direction, true);
     deflect(parent.direction, c2, fmod(az + kLeptonPi, kLeptonTwoPi), g2.direction, true);
+    first_interaction(acc, parent, kAnnihilation, g1, &g2);
     std::uint32_t k = 0;
     if (th.keep & 1U) {
       g1.history_id = k++;
@@ -759,6 +763,7 @@ C8MI_HD void lepton_vertex(EmContext const& c, Accumulator const& acc, LeptonVer
     Particle dl = child_of(parent, 11, secondary_energy(11, de, fs_me), th.w2);
     deflect(parent.direction, oc, az, out.direction, true);
     deflect(parent.direction, dc, fmod(az + kLeptonPi, kLeptonTwoPi), dl.direction, true);
+    first_interaction(acc, parent, kIonization, out, &dl);
     std::uint32_t k = 0;
     if (th.keep & 1U) {
       out.history_id = k++;
@@ -817,6 +822,7 @@ C8MI_HD void lepton_vertex(EmContext const& c, Accumulator const& acc, LeptonVer
   }
   Particle ph = child_of(parent, 22, pe, th.w2);
   for (int a = 0; a < 3; ++a) ph.direction[a] = pdir[a];
+  first_interaction(acc, parent, kBrems, lep, &ph);
   std::uint32_t k = 0;
   if (th.keep & 1U) {
     lep.history_id = k++;

@@ -473,6 +473,19 @@ typedef struct c8mi_em_fallback {
   uint32_t pad_;
 } c8mi_em_fallback;
 
+/* FirstInteractionRecord (corsika/kokkos/em/FirstInteractionRecord.hpp:22-27):
+ * the first final state of the shower's generation-zero particle, its
+ * children before thinning, each with the parent's weight (the writers'
+ * snapshot, FirstInteractionRecord.hpp:15-19; PhotonFront.inl:95-111). Children
+ * carry the parent's history ID (the IDs assigned afterwards are not part of
+ * the snapshot). secondary_count entries are valid (1 or 2). */
+typedef struct c8mi_em_first_interaction {
+  c8mi_particle_state parent_at_vertex;
+  c8mi_particle_state secondaries[3];
+  int32_t process_id;
+  uint32_t secondary_count;
+} c8mi_em_first_interaction;
+
 typedef struct c8mi_em c8mi_em;
 
 /* EmBackend::initialize (corsika/kokkos/em/execution/EmBackend.hpp:29-31):
@@ -559,6 +572,10 @@ int c8mi_em_download_observations(c8mi_em* em, c8mi_particle_state* out, uint64_
                                   uint64_t* count);
 int c8mi_em_download_fallbacks(c8mi_em* em, c8mi_em_fallback* out, uint64_t capacity,
                                uint64_t* count);
+/* IEmBackend::downloadFirstInteractionSnapshot (IEmBackend.hpp:87): *captured
+ * 0 when the shower's generation-zero particle has had no final state on the
+ * device (out then untouched). */
+int c8mi_em_download_first_interaction(c8mi_em* em, c8mi_em_first_interaction* out, int* captured);
 
 /* ---- Shower pool: several showers in the same device fronts ----
  * No counterpart in IEmBackend, which holds one shower per backend; this is the
@@ -592,6 +609,8 @@ int c8mi_em_slot_download_observations(c8mi_em* em, uint32_t slot, c8mi_particle
                                        uint64_t capacity, uint64_t* count);
 int c8mi_em_slot_download_fallbacks(c8mi_em* em, uint32_t slot, c8mi_em_fallback* out,
                                     uint64_t capacity, uint64_t* count);
+int c8mi_em_slot_download_first_interaction(c8mi_em* em, uint32_t slot, c8mi_em_first_interaction* out,
+                                            int* captured);
 
 /* Marks a slot idle. A slot that still has pending particles is refused
  * (logic error): run it to completion first. */

@@ -61,6 +61,10 @@ int c8mi_em_scalar_download_observations(const c8mi_em_scalar* scalar, c8mi_part
                                          uint64_t capacity, uint64_t* count);
 int c8mi_em_scalar_download_fallbacks(const c8mi_em_scalar* scalar, c8mi_em_fallback* out,
                                       uint64_t capacity, uint64_t* count);
+/* The shower's first interaction when its generation-zero particle reached a
+ * final state in a host step (as c8mi_em_download_first_interaction). */
+int c8mi_em_scalar_download_first_interaction(const c8mi_em_scalar* scalar, c8mi_em_first_interaction* out,
+                                              int* captured);
 
 /* ---- cascade driver (HybridCascade + EmRouter for EM particles) ---- */
 

@@ -169,6 +169,15 @@ int oracle_em_run_shower_tracks(const c8mi_em_config* config, const c8mi_native_
                                 uint64_t* track_count);
 void oracle_free(void* p);
 
+/* oracle_em_run_shower returning the writers' first-interaction snapshot of
+ * the shower (*captured 0: its generation-zero particle reached no final
+ * state). */
+int oracle_em_first_interaction(const c8mi_em_config* config, const c8mi_native_table_set* tables,
+                                const c8mi_environment* environment, uint64_t shower_id,
+                                uint64_t first_history_id, const c8mi_particle_state* primaries,
+                                uint64_t primary_count, c8mi_em_first_interaction* out, int* captured,
+                                or_em_stats* stats);
+
 /* Vertex log of a shower (test probe): one record per lepton vertex attempt
  * (kind 0: the reselection of LeptonVertexSelection.inl:41-199 at the
  * post-loss energy, accepted when process_uniform * rate_start < rate_vertex)
@@ -267,6 +276,8 @@ typedef struct or_hybrid_side {
   uint64_t fallback_count;
   double* tracks;
   uint64_t track_count;
+  c8mi_em_first_interaction first;
+  uint64_t first_captured;
 } or_hybrid_side;
 int oracle_em_run_hybrid(const c8mi_em_config* config, const c8mi_native_table_set* tables,
                          const c8mi_environment* environment, uint64_t shower_id, uint64_t first_history_id,

@@ -1881,6 +1881,9 @@ typedef struct {
   or_vertex* vertices;
   uint64_t vertex_count, vertex_capacity;
   int record_vertices, vertex_overflow;
+  /* the writers' first-interaction snapshot */
+  c8mi_em_first_interaction first;
+  uint64_t first_candidates;
 } shower_t;
 
 static or_vertex* log_vertex(shower_t* s, int kind, const ps_t* parent) {
@@ -1910,6 +1913,26 @@ static void log_child(or_vertex* v, int32_t pid, double energy, const double dir
   v->child_energy[v->count] = energy;
   for (int a = 0; a < 3; ++a) v->child_direction[v->count][a] = direction[a];
   v->count++;
+}
+
+/* The first final state of a generation-zero parent (PhotonFront.inl:95-111,
+ * LeptonFinalStateStep.inl:609-615): children before thinning, with the
+ * parent's weight and history ID; the first candidate of the shower wins. */
+static void capture_first(shower_t* s, const ps_t* parent, int32_t process, const ps_t* a, const ps_t* b) {
+  if (parent->generation != 0) return;
+  if (s->first_candidates++ != 0) return;
+  memset(&s->first, 0, sizeof s->first);
+  s->first.parent_at_vertex = *parent;
+  s->first.secondaries[0] = *a;
+  s->first.secondaries[0].weight = parent->weight;
+  s->first.secondaries[0].history_id = parent->history_id;
+  if (b) {
+    s->first.secondaries[1] = *b;
+    s->first.secondaries[1].weight = parent->weight;
+    s->first.secondaries[1].history_id = parent->history_id;
+  }
+  s->first.process_id = process;
+  s->first.secondary_count = b ? 2u : 1u;
 }
 
 /* Compact radio track of one e+- step (the make_radio_track_kinematics input,
@@ -2122,6 +2145,7 @@ static void photon_source(shower_t* s, const ps_t* p, pout_t* o, uint64_t child_
         log_child(lv, 11, ELECTRON_MASS_GEV + parent->energy_gev * v, el.direction);
       }
     }
+    capture_first(s, parent, P_COMPTON, &ph, &el);
     uint32_t k = 0;
     if (th.keep & 1U) {
       ph.history_id = first_history + child_offset + k++;
@@ -2158,6 +2182,8 @@ static void photon_source(shower_t* s, const ps_t* p, pout_t* o, uint64_t child_
     el.generation = parent->generation + 1;
     el.step_id = 0;
     el.reserved = 0;
+    el.weight = parent->weight;
+    capture_first(s, parent, P_PHOTOELECTRIC, &el, NULL);
     o->charged[o->charged_count++] = el;
     o->child_count = 1;
     s->stats.secondaries += 1;
@@ -2249,6 +2275,7 @@ static void photon_source(shower_t* s, const ps_t* p, pout_t* o, uint64_t child_
       log_child(lv, -11, pe, po.direction);
     }
   }
+  capture_first(s, parent, P_PHOTON_PAIR, &el, &po);
   uint32_t k = 0;
   if (th.keep & 1U) {
     el.history_id = first_history + child_offset + k++;
@@ -3197,6 +3224,7 @@ static void lepton_source(shower_t* s, const ps_t* p, lout_t* o, uint64_t child_
         log_child(lv, 22, e2, g2.direction);
       }
     }
+    capture_first(s, parent, P_ANNIHILATION, &g1, &g2);
     uint32_t k = 0;
     if (th.keep & 1U) {
       g1.history_id = first_history + child_offset + k++;
@@ -3270,6 +3298,7 @@ static void lepton_source(shower_t* s, const ps_t* p, lout_t* o, uint64_t child_
         log_child(lv, 11, de, dl.direction);
       }
     }
+    capture_first(s, parent, P_IONIZATION, &out, &dl);
     uint32_t k = 0;
     if (th.keep & 1U) {
       out.history_id = first_history + child_offset + k++;
@@ -3365,6 +3394,7 @@ static void lepton_source(shower_t* s, const ps_t* p, lout_t* o, uint64_t child_
       log_child(lv, 22, pe, pdir);
     }
   }
+  capture_first(s, parent, P_BREMS, &lep, &ph);
   uint32_t k = 0;
   if (th.keep & 1U) {
     lep.history_id = first_history + child_offset + k++;
@@ -3576,7 +3606,8 @@ static int run_shower(const c8mi_em_config* config, const c8mi_native_table_set*
                       int64_t* electrons, int64_t* positrons, int64_t* energy_loss,
                       int64_t* counters, c8mi_particle_state* observations,
                       c8mi_em_fallback* fallbacks, or_em_stats* stats, double** tracks,
-                      uint64_t* track_count, or_vertex** vertices, uint64_t* vertex_count) {
+                      uint64_t* track_count, or_vertex** vertices, uint64_t* vertex_count,
+                      c8mi_em_first_interaction* first, int* first_captured) {
   shower_t s;
   memset(&s, 0, sizeof s);
   s.record_tracks = tracks != NULL;
@@ -3631,6 +3662,10 @@ static int run_shower(const c8mi_em_config* config, const c8mi_native_table_set*
   s.stats.observations = s.observation_count;
   memcpy(counters, s.prof.counters, sizeof s.prof.counters);
   *stats = s.stats;
+  if (first && first_captured) {
+    *first_captured = s.first_candidates != 0;
+    if (*first_captured) *first = s.first;
+  }
   queue_free(&s.photons);
   queue_free(&s.leptons);
   if (tracks) {
@@ -3662,7 +3697,7 @@ int oracle_em_run_shower(const c8mi_em_config* config, const c8mi_native_table_s
                          c8mi_em_fallback* fallbacks, or_em_stats* stats) {
   return run_shower(config, tables, environment, shower_id, first_history_id, primaries,
                     primary_count, maximum_fronts, photons, electrons, positrons, energy_loss,
-                    counters, observations, fallbacks, stats, NULL, NULL, NULL, NULL);
+                    counters, observations, fallbacks, stats, NULL, NULL, NULL, NULL, NULL, NULL);
 }
 
 int oracle_em_run_shower_tracks(const c8mi_em_config* config, const c8mi_native_table_set* tables,
@@ -3678,10 +3713,29 @@ int oracle_em_run_shower_tracks(const c8mi_em_config* config, const c8mi_native_
   *track_count = 0;
   return run_shower(config, tables, environment, shower_id, first_history_id, primaries,
                     primary_count, maximum_fronts, photons, electrons, positrons, energy_loss,
-                    counters, observations, fallbacks, stats, tracks, track_count, NULL, NULL);
+                    counters, observations, fallbacks, stats, tracks, track_count, NULL, NULL, NULL, NULL);
 }
 
 void oracle_free(void* p) { free(p); }
+
+int oracle_em_first_interaction(const c8mi_em_config* config, const c8mi_native_table_set* tables,
+                                const c8mi_environment* environment, uint64_t shower_id,
+                                uint64_t first_history_id, const c8mi_particle_state* primaries,
+                                uint64_t primary_count, c8mi_em_first_interaction* out, int* captured,
+                                or_em_stats* stats) {
+  uint64_t const bins = config->profile_bins ? config->profile_bins : 1;
+  int64_t* hist = calloc(4 * bins, sizeof(int64_t));
+  int64_t counters[OR_EM_COUNTERS];
+  if (!hist || !out || !captured) {
+    free(hist);
+    return -1;
+  }
+  int rc = run_shower(config, tables, environment, shower_id, first_history_id, primaries, primary_count,
+                      UINT64_MAX, hist, hist + bins, hist + 2 * bins, hist + 3 * bins, counters, NULL, NULL,
+                      stats, NULL, NULL, NULL, NULL, out, captured);
+  free(hist);
+  return rc;
+}
 
 int oracle_em_run_shower_vertices(const c8mi_em_config* config, const c8mi_native_table_set* tables,
                                   const c8mi_environment* environment, uint64_t shower_id,
@@ -3697,7 +3751,7 @@ int oracle_em_run_shower_vertices(const c8mi_em_config* config, const c8mi_nativ
   if (!hist) return -1;
   int rc = run_shower(config, tables, environment, shower_id, first_history_id, primaries, primary_count,
                       UINT64_MAX, hist, hist + bins, hist + 2 * bins, hist + 3 * bins, counters, NULL, NULL,
-                      stats, NULL, NULL, vertices, vertex_count);
+                      stats, NULL, NULL, vertices, vertex_count, NULL, NULL);
   free(hist);
   return rc;
 }
@@ -3815,11 +3869,13 @@ typedef struct {
   double* tracks;
   uint64_t track_count, track_capacity;
   int track_overflow;
+  c8mi_em_first_interaction first;
+  uint64_t first_candidates;
 } side_t;
 
 static void swap_side(shower_t* s, side_t* d) {
   side_t cur = {s->prof, s->observations, s->observation_count, s->fallbacks, s->fallback_count,
-                s->tracks, s->track_count, s->track_capacity, s->track_overflow};
+                s->tracks, s->track_count, s->track_capacity, s->track_overflow, s->first, s->first_candidates};
   s->prof = d->prof;
   s->observations = d->observations;
   s->observation_count = d->observation_count;
@@ -3829,6 +3885,8 @@ static void swap_side(shower_t* s, side_t* d) {
   s->track_count = d->track_count;
   s->track_capacity = d->track_capacity;
   s->track_overflow = d->track_overflow;
+  s->first = d->first;
+  s->first_candidates = d->first_candidates;
   *d = cur;
 }
 
@@ -4137,6 +4195,10 @@ int oracle_em_run_hybrid(const c8mi_em_config* config, const c8mi_native_table_s
   host->fallback_count = h->host.fallback_count;
   host->tracks = h->host.tracks;
   host->track_count = h->host.track_count;
+  host->first = h->host.first;
+  host->first_captured = h->host.first_candidates != 0;
+  device->first = s->first;
+  device->first_captured = s->first_candidates != 0;
   int const overflow = s->track_overflow || h->host.track_overflow;
   if (staged && staged_count) {
     *staged = h->staged.items;

@@ -233,7 +233,9 @@ class OrHybridSide(ctypes.Structure):
     _fields_ = [("hist", ctypes.c_void_p), ("counters", ctypes.c_int64 * 64),
                 ("observations", ctypes.c_void_p), ("observation_count", c_uint64),
                 ("fallbacks", ctypes.c_void_p), ("fallback_count", c_uint64),
-                ("tracks", ctypes.c_void_p), ("track_count", c_uint64)]
+                ("tracks", ctypes.c_void_p), ("track_count", c_uint64),
+                ("align_", c_uint64),  # the snapshot is 16-byte aligned in C
+                ("first", ctypes.c_uint8 * 528), ("first_captured", c_uint64), ("tail_", c_uint64)]
 
 
 def em_run_hybrid(config, tables, environment, primaries, shower_id=1, first_history_id=2, hybrid=None,
@@ -282,7 +284,9 @@ def em_run_hybrid(config, tables, environment, primaries, shower_id=1, first_his
                 tracks = np.ascontiguousarray(rows.reshape(-1, 9).T.copy())
             lib.oracle_free(side.tracks)
         h = hist.reshape(4, bins)
-        out[name] = dict(hist=dict(photons=h[0].copy(), electrons=h[1].copy(), positrons=h[2].copy(),
+        from c8mi.em import FIRST_INTERACTION_DTYPE
+        first = np.frombuffer(bytes(side.first), dtype=FIRST_INTERACTION_DTYPE)[0].copy() if side.first_captured else None
+        out[name] = dict(first=first, hist=dict(photons=h[0].copy(), electrons=h[1].copy(), positrons=h[2].copy(),
                                    energy_loss=h[3].copy()),
                          counters=np.array(side.counters, dtype=np.int64),
                          observations=obs[:min(side.observation_count, obs.size)],
@@ -298,6 +302,26 @@ def em_run_hybrid(config, tables, environment, primaries, shower_id=1, first_his
     if status != 0:
         raise RuntimeError(f"oracle_em_run_hybrid failed ({status})")
     return out
+
+
+def em_first_interaction(config, tables, environment, primaries, shower_id=1, first_history_id=2):
+    """The oracle's first-interaction snapshot of a shower (or None) and its stats."""
+    from c8mi import PARTICLE_STATE_DTYPE
+    from c8mi.em import FIRST_INTERACTION_DTYPE, RunResult
+
+    lib = load()
+    cfg = config.struct if hasattr(config, "struct") else config
+    env = environment.struct if hasattr(environment, "struct") else environment
+    prim = np.ascontiguousarray(primaries, dtype=PARTICLE_STATE_DTYPE).reshape(-1)
+    out = np.zeros(1, dtype=FIRST_INTERACTION_DTYPE)
+    captured = c_int(0)
+    stats = RunResult()
+    if lib.oracle_em_first_interaction(byref(cfg), byref(tables.struct), byref(env), c_uint64(shower_id),
+                                       c_uint64(first_history_id), prim.ctypes.data_as(c_void_p),
+                                       c_uint64(prim.size), out.ctypes.data_as(c_void_p), byref(captured),
+                                       byref(stats)) != 0:
+        raise RuntimeError("oracle_em_first_interaction failed")
+    return (out[0] if captured.value else None), stats.as_dict()
 
 
 VERTEX_DTYPE = np.dtype([

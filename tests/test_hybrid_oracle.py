@@ -70,6 +70,7 @@ def test_host_prefix_equals_oracle(air, kind, pid, energy, seed):
     hist, ledger = stepper.download_profile()
     obs, n_obs = stepper.observations()
     fbs, n_fb = stepper.fallbacks()
+    first = stepper.first_interaction()
     stepper.close()
     want = oracle_lib.em_run_hybrid(cfg, tables, env, prim, shower_id=seed, prefix_only=True)
     same_particles(batch, want["staged"])
@@ -94,6 +95,15 @@ def test_host_prefix_equals_oracle(air, kind, pid, energy, seed):
     same_particles(obs, host["observations"])
     assert n_fb == host["fallbacks"].size
     same_particles(fbs["particle"], host["fallbacks"]["particle"])
+    # the first final state of the primary happened in a host step
+    want_first = host["first"]
+    assert first is not None and want_first is not None
+    assert first["process_id"] == want_first["process_id"]
+    assert first["secondary_count"] == want_first["secondary_count"]
+    same_particles(first["parent_at_vertex"].reshape(1), want_first["parent_at_vertex"].reshape(1))
+    same_particles(first["secondaries"][:first["secondary_count"]],
+                   want_first["secondaries"][:want_first["secondary_count"]])
+    assert np.all(first["secondaries"]["weight"][:first["secondary_count"]] == prim["weight"])
     # the host side closes: what it kept of the primary is deposited or handed on
     if ledger["fallback_reasons"][21] == 0:
         assert abs(em.ledger_residual(ledger)) <= 1e-9 * energy
