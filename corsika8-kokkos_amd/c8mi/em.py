@@ -915,7 +915,7 @@ HYBRID_COUNTS = ("cpu_particles", "cpu_photon_steps", "cpu_lepton_steps", "cpu_i
                  "maximum_input_batch", "reserved_history_ids", "below_minimum_batch_checkpoints",
                  "history_range_checkpoints", "photon_calls", "lepton_calls", "device_photon_fronts",
                  "device_lepton_fronts", "device_photon_steps", "device_lepton_steps", "device_interactions",
-                 "device_secondaries", "bucketed_fronts", "next_history_id")
+                 "device_secondaries", "bucketed_fronts", "next_history_id", "fallbacks_returned", "fallbacks_kept")
 
 
 class HybridConfig(ctypes.Structure):

@@ -89,7 +89,7 @@ void HostStepper::book_injected(double energy_weight) {
   phys::ledger(acc, kInjected, energy_weight);
 }
 
-HostStep HostStepper::step(c8mi_particle_state const& particle, std::uint64_t first_history_id) {
+HostStep HostStepper::step(c8mi_particle_state const& particle, std::uint64_t first_history_id, bool moliere) {
   HostStep r{};
   phys::Accumulator const acc{profile_.data(), counters_.data(), &context_, &shower_, 0};
   phys::Particle p;
@@ -111,7 +111,7 @@ HostStep HostStepper::step(c8mi_particle_state const& particle, std::uint64_t fi
     phys::LeptonMove mv;
     if (phys::lepton_move(c, acc, p, mv)) {
       bool failed = false;
-      if (c.moliere_enabled) {
+      if (c.moliere_enabled && moliere) {
         phys::MoliereOpen open;
         int const outcome = c.moliere_polynomials ? phys::moliere_scatter<true>(c, mv, open)
                                                   : phys::moliere_scatter<false>(c, mv, open);

@@ -36,7 +36,9 @@ class HostStepper {
 
   // Starts a shower (Philox showerId) and zeroes the accumulators.
   void begin_shower(std::uint64_t shower_id);
-  HostStep step(c8mi_particle_state const& particle, std::uint64_t first_history_id);
+  // moliere false: the step without Molière scattering (the host's step of
+  // a particle whose device step failed the Molière sampling).
+  HostStep step(c8mi_particle_state const& particle, std::uint64_t first_history_id, bool moliere = true);
   // EmRouter::canRoute for EM particles (EmRouter.inl:64-105): an e+-/photon
   // the device can transport (EmBackend::can_transport) inside the environment.
   bool can_route(c8mi_particle_state const& particle) const;

@@ -14,6 +14,9 @@ namespace c8mi::scheduling {
 
 namespace {
 
+// FallbackReason::MoliereSamplingFailed (FallbackRequest.hpp:22-65).
+constexpr std::int32_t kMoliereSamplingFailed = 21;
+
 // A status of the EM / radio C ABI back into the exception it stands for.
 void check(int status) {
   if (status == kOk) return;
@@ -74,6 +77,8 @@ void HybridCascade::start(c8mi_particle_state const* primaries, std::uint64_t co
   stack_.clear();
   staged_.clear();
   expansion_steps_.clear();
+  fallback_steps_.clear();
+  fallback_cursor_ = 0;
   expanded_.clear();
   tracks_.clear();
   force_small_ = false;
@@ -90,13 +95,53 @@ void HybridCascade::start(c8mi_particle_state const* primaries, std::uint64_t co
   }
 }
 
-bool HybridCascade::can_route(c8mi_particle_state const& p) {
-  auto const found = expansion_steps_.find({p.history_id, p.step_id});
-  if (found != expansion_steps_.end()) {
-    expansion_steps_.erase(found);
-    return false;
+HybridCascade::Route HybridCascade::route(c8mi_particle_state const& p) {
+  // EmRouter::canRoute (EmRouter.inl:64-105): a step returned for a host
+  // expansion or a fallback is declined once
+  auto const expansion = expansion_steps_.find({p.history_id, p.step_id});
+  if (expansion != expansion_steps_.end()) {
+    expansion_steps_.erase(expansion);
+    return Route::kHostStep;
   }
-  return stepper_.can_route(p);
+  auto const fallback = fallback_steps_.find({p.history_id, p.step_id});
+  if (fallback != fallback_steps_.end()) {
+    fallback_steps_.erase(fallback);
+    return Route::kFallbackStep;
+  }
+  return stepper_.can_route(p) ? Route::kDevice : Route::kHostStep;
+}
+
+void HybridCascade::return_fallbacks() {
+  // EmRouter::returnFallbacks (EmRouter.inl:709-800): the shower's new device
+  // fallback records; a Molière sampling failure goes back to the host stack
+  // for a host step without Molière scattering (the reference hands it to
+  // C8's scalar transport), in (history, step) order so that the host's
+  // history IDs do not depend on the order the device appended the records
+  std::uint64_t total = 0;
+  check(c8mi_em_download_fallbacks(em_, nullptr, 0, &total));
+  if (total <= fallback_cursor_) return;
+  std::uint64_t const stored = std::min<std::uint64_t>(total, stepper_.config().fallback_capacity);
+  fallbacks_.resize(stored);
+  std::uint64_t n = 0;
+  if (stored) check(c8mi_em_download_fallbacks(em_, fallbacks_.data(), fallbacks_.size(), &n));
+  std::vector<c8mi_particle_state> returned;
+  for (std::uint64_t i = fallback_cursor_; i < stored; ++i) {
+    c8mi_em_fallback const& f = fallbacks_[i];
+    if (f.reason == kMoliereSamplingFailed) returned.push_back(f.particle);
+    else ++r_.fallbacks_kept;
+  }
+  if (total > std::max(stored, fallback_cursor_)) r_.fallbacks_kept += total - std::max(stored, fallback_cursor_);
+  std::sort(returned.begin(), returned.end(), [](c8mi_particle_state const& a, c8mi_particle_state const& b) {
+    return a.history_id != b.history_id ? a.history_id < b.history_id : a.step_id < b.step_id;
+  });
+  for (auto const& p : returned) {
+    if (!fallback_steps_.insert({p.history_id, p.step_id}).second)
+      throw std::runtime_error("hybrid: duplicate host fallback for one transport step");
+    stack_.push_back(p);
+    stepper_.book_injected(p.energy_gev * p.weight);
+    ++r_.fallbacks_returned;
+  }
+  fallback_cursor_ = total;
 }
 
 void HybridCascade::stage(c8mi_particle_state const& p) {
@@ -106,9 +151,9 @@ void HybridCascade::stage(c8mi_particle_state const& p) {
   ++r_.particles_staged;
 }
 
-void HybridCascade::host_step(c8mi_particle_state const& p) {
+void HybridCascade::host_step(c8mi_particle_state const& p, bool moliere) {
   auto const t0 = Clock::now();
-  em::HostStep const s = stepper_.step(p, counter_);
+  em::HostStep const s = stepper_.step(p, counter_, moliere);
   counter_ += s.children;
   ++r_.cpu_particles;
   if (s.step) {
@@ -213,6 +258,7 @@ void HybridCascade::device_call(bool lepton, std::vector<c8mi_particle_state> co
     staged_.push_back(p);
     stepper_.book_injected(p.energy_gev * p.weight);
   }
+  return_fallbacks();
   device_seconds_ += seconds_since(t0);
 }
 
@@ -286,12 +332,13 @@ void HybridCascade::cascade() {
     while (!stop_ && !stack_.empty()) {
       c8mi_particle_state const p = stack_.back();
       stack_.pop_back();
-      if (can_route(p)) {
+      Route const r = route(p);
+      if (r == Route::kDevice) {
         stage(p);
         while (!stop_ && ready()) advance();
         continue;
       }
-      host_step(p);
+      host_step(p, r != Route::kFallbackStep);
     }
     if (!stop_ && pending()) advance();
   }

@@ -46,9 +46,11 @@ class HybridCascade {
   void start(c8mi_particle_state const* primaries, std::uint64_t count, std::uint64_t shower_id,
              std::uint64_t first_history_id);
   void cascade();
-  bool can_route(c8mi_particle_state const& p);
+  enum class Route { kDevice, kHostStep, kFallbackStep };
+  Route route(c8mi_particle_state const& p);
+  void return_fallbacks();
   void stage(c8mi_particle_state const& p);
-  void host_step(c8mi_particle_state const& p);
+  void host_step(c8mi_particle_state const& p, bool moliere);
   std::uint64_t device_pending() const;
   bool ready() const;
   bool pending() const;
@@ -67,6 +69,11 @@ class HybridCascade {
   // cpuWavefrontExpansionSteps_: (history, step) of the particles returned
   // for a host step; canRoute declines each once.
   std::set<std::pair<std::uint64_t, std::uint64_t>> expansion_steps_;
+  // cpuFallbackSteps_: device fallbacks returned for a host step, which runs
+  // without Molière scattering.
+  std::set<std::pair<std::uint64_t, std::uint64_t>> fallback_steps_;
+  std::vector<c8mi_em_fallback> fallbacks_;
+  std::uint64_t fallback_cursor_ = 0;
   // lastCpuExpansionStates_: a staged particle in the state of its last
   // expansion forces the next small batch to the device.
   std::unordered_map<std::uint64_t, c8mi_particle_state> expanded_;

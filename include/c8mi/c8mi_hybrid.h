@@ -112,6 +112,12 @@ typedef struct c8mi_em_hybrid_result {
   uint64_t device_secondaries;
   uint64_t bucketed_fronts;
   uint64_t next_history_id;        /* the shower's history counter at the end */
+  /* fallback round trip (EmRouter.inl:709-800): device fallback records whose
+   * particle went back to the host stack for a host step (Molière sampling
+   * failures: the step again without Molière scattering), and those kept as
+   * records (specified final states the host has no model for) */
+  uint64_t fallbacks_returned;
+  uint64_t fallbacks_kept;
   double host_seconds;             /* wall time of host steps */
   double device_seconds;           /* wall time of device calls (remaining downloads included) */
 } c8mi_em_hybrid_result;

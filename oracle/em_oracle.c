@@ -3912,6 +3912,8 @@ typedef struct {
   c8mi_em_limits limits;
   c8mi_em_hybrid_result r;
   int prefix_only, stop;
+  uint64_t fallback_cursor;       /* device fallback records already returned or kept */
+  c8mi_em_config without_moliere; /* the configuration of a fallback's host step */
 } hybrid_t;
 
 static int host_can_route(const hybrid_t* h, const ps_t* p) {
@@ -3948,8 +3950,10 @@ static void host_stage(hybrid_t* h, const ps_t* p) {
 
 /* One host step: outputs with history IDs from the shower's counter, a
  * continuation alone or the children in ID order. */
-static uint32_t host_step(hybrid_t* h, const ps_t* p, ps_t out[4]) {
+static uint32_t host_step(hybrid_t* h, const ps_t* p, ps_t out[4], int moliere) {
   shower_t* s = &h->s;
+  const c8mi_em_config* const cfg = s->x.cfg;
+  if (!moliere) s->x.cfg = &h->without_moliere;
   or_em_stats const before = s->stats;
   uint64_t const first = s->sched.counter;
   uint32_t n = 0, children;
@@ -3970,6 +3974,7 @@ static uint32_t host_step(hybrid_t* h, const ps_t* p, ps_t out[4]) {
   }
   h->r.cpu_tracks += s->track_count - tracks_before;
   swap_side(s, &h->host);
+  s->x.cfg = cfg;
   if (children && n == 2 && out[0].history_id > out[1].history_id) {
     ps_t t = out[0];
     out[0] = out[1];
@@ -3984,6 +3989,42 @@ static uint32_t host_step(hybrid_t* h, const ps_t* p, ps_t out[4]) {
   h->r.cpu_secondaries += s->stats.secondaries - before.secondaries;
   s->stats = before;
   return n;
+}
+
+static int history_step_less(const void* a, const void* b) {
+  const ps_t* x = a;
+  const ps_t* y = b;
+  if (x->history_id != y->history_id) return x->history_id < y->history_id ? -1 : 1;
+  return x->step_id < y->step_id ? -1 : x->step_id > y->step_id;
+}
+
+/* EmRouter::returnFallbacks (EmRouter.inl:709-800): the device fallbacks of the
+ * call; a Molière sampling failure goes back to the host stack for a host step
+ * without Molière scattering (marked cls 2), in (history, step) order; the
+ * others stay records. */
+static void return_fallbacks(hybrid_t* h) {
+  shower_t* s = &h->s;
+  uint64_t const total = s->fallback_count;
+  if (total <= h->fallback_cursor) return;
+  uint64_t const stored = total < s->x.cfg->fallback_capacity ? total : s->x.cfg->fallback_capacity;
+  queue_t returned;
+  memset(&returned, 0, sizeof returned);
+  for (uint64_t i = h->fallback_cursor; i < stored; ++i) {
+    const c8mi_em_fallback* f = &s->fallbacks[i];
+    if (f->reason == FR_MOLIERE_FAILED) push(&returned, &f->particle, 0);
+    else h->r.fallbacks_kept++;
+  }
+  uint64_t const seen = stored > h->fallback_cursor ? stored : h->fallback_cursor;
+  if (total > seen) h->r.fallbacks_kept += total - seen;
+  if (returned.count) qsort(returned.items, returned.count, sizeof(ps_t), history_step_less);
+  for (uint64_t i = 0; i < returned.count; ++i) {
+    const ps_t* p = &returned.items[i];
+    push(&h->stack, p, 2);
+    ledger_add(&h->host.prof, s->x.cfg, OR_EMC_INJECTED, p->energy_gev * p->weight);
+    h->r.fallbacks_returned++;
+  }
+  queue_free(&returned);
+  h->fallback_cursor = total;
 }
 
 /* A wavefront call (IEmBackend::run{Photon,Lepton}Wavefront): the device
@@ -4015,6 +4056,7 @@ static void device_call(hybrid_t* h, int lepton, const queue_t* host) {
     ledger_add(&h->host.prof, cfg, OR_EMC_INJECTED, p->energy_gev * p->weight);
   }
   q->count = 0;
+  return_fallbacks(h);
 }
 
 static uint64_t device_pending(const hybrid_t* h) { return h->s.photons.count + h->s.leptons.count; }
@@ -4104,7 +4146,7 @@ static void hybrid_cascade(hybrid_t* h) {
         continue;
       }
       ps_t out[4];
-      uint32_t const n = host_step(h, &p, out);
+      uint32_t const n = host_step(h, &p, out, marked != 2);
       for (uint32_t k = 0; k < n; ++k) push(&h->stack, &out[k], 0);
     }
     if (!h->stop && router_pending(h)) router_advance(h);
@@ -4145,6 +4187,8 @@ int oracle_em_run_hybrid(const c8mi_em_config* config, const c8mi_native_table_s
     h->limits.maximum_resident_input_batch = UINT64_MAX;
   }
   h->prefix_only = prefix_only;
+  h->without_moliere = *config;
+  h->without_moliere.moliere_enabled = 0;
   uint64_t const bins = config->profile_bins;
   s->prof.photons = device->hist;
   s->prof.electrons = device->hist + bins;

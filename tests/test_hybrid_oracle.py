@@ -174,3 +174,38 @@ def test_each_side_closes_its_ledger(air, pid, energy, seed, minimum):
     assert r["small_batch_expansions"] >= 8 and r["cpu_particles"] > 0 and r["wavefronts"] > 0
     assert r["cpu_tracks"] == out["host"]["tracks"].shape[1]
     assert r["device_lepton_steps"] == out["device"]["tracks"].shape[1]
+
+
+def test_fallback_round_trip_returns_moliere_failures(air):
+    """Device Molière sampling failures go back to the host stack and take a
+    host step without Molière scattering (EmRouter::returnFallbacks,
+    EmRouter.inl:709-800, with the host's scalar transport): every such record
+    is returned, the host steps fail nothing, and the particles' energy moves
+    from the device's fallback term to the host's injected term. The shower
+    then closes over both sides, except the continuous deposit of a failed
+    step that ended at a vertex, which the reference's transport fallback
+    does not book (MoliereStep.inl:50-54)."""
+    atm = em.Atmosphere()
+    returned = closed = 0
+    for seed in range(1, 13):
+        cfg = em.make_config(airtables, seed=seed, atmosphere=atm, moliere=True, moliere_table=True)
+        prim = em.particle(22, 100.0, START, DOWN, history_id=1)
+        out = oracle_lib.em_run_hybrid(cfg, air, atm, prim, shower_id=seed,
+                                       hybrid=em.hybrid_config(minimum_batch=16384))
+        r = out["result"]
+        es = cfg.struct.profile_energy_scale
+        host = em.decode_counters(out["host"]["counters"], es)
+        dev = em.decode_counters(out["device"]["counters"], es)
+        assert r["fallbacks_returned"] == dev["fallback_reasons"][21]
+        assert r["fallbacks_kept"] == sum(dev["fallback_reasons"]) - dev["fallback_reasons"][21]
+        assert host["fallback_reasons"][21] == 0
+        # the returned particles' energy is the device's fallback term and the
+        # host's injected term at once
+        moved = dev["fallback_energy"] if r["fallbacks_kept"] == 0 else 0.0
+        assert abs(host["injected_energy"] + dev["injected_energy"] - moved - 100.0) <= 1e-9 * 100.0
+        residual = em.ledger_residual(host) + em.ledger_residual(dev)
+        assert residual >= -1e-9 * 100.0 and residual <= 1e-2  # an unbooked continuous deposit is >= 0
+        if r["fallbacks_returned"]:
+            returned += 1
+            closed += abs(residual) <= 1e-9 * 100.0
+    assert returned >= 4 and closed >= returned // 2
