@@ -415,3 +415,80 @@ def random_queries(tables: NativeTables, count: int, seed: int = 2) -> np.ndarra
             arg = 0.0
         q[i]["argument"] = arg
     return q
+
+
+# ---- table files and content hash (c8mi_native_table_*) ----
+
+class NativeInteractionIdentity(ctypes.Structure):
+    _fields_ = [("pdg_id", c_int32), ("reserved", c_uint32), ("medium_hash", c_uint64),
+                ("interaction_hash", c_uint64)]
+
+
+class NativeTableMeta(ctypes.Structure):
+    _fields_ = [("format_version", c_uint32), ("pad_", c_uint32), ("proposal_version", ctypes.c_char * 32),
+                ("cubic_interpolation_version", ctypes.c_char * 32), ("stochastic_cut_mev", c_double),
+                ("interaction_identities", c_void_p), ("interaction_identity_count", c_uint64)]
+
+
+def table_meta(proposal_version="7.6.2", cubic_interpolation_version="0.1.5", stochastic_cut_mev=0.0,
+               identities=()):
+    """Metadata of a table set (ProposalNativeTableSet's version strings, cut and
+    Interaction identities [(pdg_id, medium_hash, interaction_hash)])."""
+    m = NativeTableMeta()
+    m.format_version = 7
+    m.proposal_version = proposal_version.encode()
+    m.cubic_interpolation_version = cubic_interpolation_version.encode()
+    m.stochastic_cut_mev = stochastic_cut_mev
+    ids = (NativeInteractionIdentity * max(len(identities), 1))()
+    for k, (pid, medium, interaction) in enumerate(identities):
+        ids[k].pdg_id, ids[k].medium_hash, ids[k].interaction_hash = pid, medium, interaction
+    m.interaction_identities = ctypes.cast(ids, c_void_p) if identities else None
+    m.interaction_identity_count = len(identities)
+    m._keep = ids
+    return m
+
+
+def table_hash(tables, meta) -> bytes:
+    """The content hash (calculate_proposal_native_hash) of a table set."""
+    from . import _check, library
+
+    digest = (ctypes.c_uint8 * 32)()
+    _check(library().c8mi_native_table_hash(ctypes.byref(tables.struct), ctypes.byref(meta), digest))
+    return bytes(digest)
+
+
+def write_table_file(path, tables, meta):
+    from . import _check, library
+
+    _check(library().c8mi_native_table_write(str(path).encode(), ctypes.byref(tables.struct), ctypes.byref(meta)))
+
+
+class TableFile:
+    """A table file read back (c8mi_native_table_read): .struct is the set (use
+    it where a NativeTables is expected), .meta the metadata, .digest the
+    verified content hash. Keep the object alive while the set is in use."""
+
+    def __init__(self, path):
+        from . import _check, library
+
+        self.handle = c_void_p()
+        _check(library().c8mi_native_table_read(str(path).encode(), ctypes.byref(self.handle)))
+        self.struct = NativeTableSet()
+        self.meta = NativeTableMeta()
+        digest = (ctypes.c_uint8 * 32)()
+        _check(library().c8mi_native_table_file_view(self.handle, ctypes.byref(self.struct),
+                                                     ctypes.byref(self.meta), digest))
+        self.digest = bytes(digest)
+
+    def close(self):
+        from . import _check, library
+
+        if self.handle:
+            _check(library().c8mi_native_table_file_destroy(self.handle))
+            self.handle = c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

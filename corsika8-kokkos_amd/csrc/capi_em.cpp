@@ -2,6 +2,8 @@
 // batched native-table queries and batched Philox draws.
 #include <hip/hip_runtime.h>
 
+#include <cstring>
+#include <memory>
 #include <stdexcept>
 #include <string>
 
@@ -12,11 +14,16 @@
 #include "em/em_backend.hpp"
 #include "em/em_launch.hpp"
 #include "em/moliere_table.hpp"
+#include "em/native_table_file.hpp"
 #include "runtime.hpp"
 
 struct c8mi_em_tables {
   c8mi::em::DeviceTables tables;
   explicit c8mi_em_tables(c8mi_native_table_set const& host) : tables(host) {}
+};
+
+struct c8mi_native_table_file {
+  std::unique_ptr<c8mi::em::TableFile> file;
 };
 
 struct c8mi_em {
@@ -42,6 +49,54 @@ int c8mi_em_tables_create(c8mi_native_table_set const* host_tables, c8mi_em_tabl
     *out = nullptr;
     *out = new c8mi_em_tables(*host_tables);
   });
+}
+
+int c8mi_sha256(uint8_t const* data, uint64_t size, uint8_t digest[32]) {
+  return guarded([&] {
+    if ((size && !data) || !digest) throw std::invalid_argument("null argument");
+    c8mi::em::Digest const d = c8mi::em::sha256(data, size);
+    std::memcpy(digest, d.data(), d.size());
+  });
+}
+
+int c8mi_native_table_hash(c8mi_native_table_set const* set, c8mi_native_table_meta const* meta,
+                           uint8_t digest[32]) {
+  return guarded([&] {
+    if (!set || !meta || !digest) throw std::invalid_argument("null argument");
+    c8mi::em::Digest const d = c8mi::em::native_table_hash(*set, *meta);
+    std::memcpy(digest, d.data(), d.size());
+  });
+}
+
+int c8mi_native_table_write(char const* path, c8mi_native_table_set const* set, c8mi_native_table_meta const* meta) {
+  return guarded([&] {
+    if (!path || !set || !meta) throw std::invalid_argument("null argument");
+    c8mi::em::write_native_table(path, *set, *meta);
+  });
+}
+
+int c8mi_native_table_read(char const* path, c8mi_native_table_file** out) {
+  return guarded([&] {
+    if (!path || !out) throw std::invalid_argument("null argument");
+    *out = nullptr;
+    auto handle = std::make_unique<c8mi_native_table_file>();
+    handle->file = c8mi::em::read_native_table(path);
+    *out = handle.release();
+  });
+}
+
+int c8mi_native_table_file_view(c8mi_native_table_file const* file, c8mi_native_table_set* set,
+                                c8mi_native_table_meta* meta, uint8_t digest[32]) {
+  return guarded([&] {
+    require_handle(file);
+    if (set) *set = file->file->set();
+    if (meta) *meta = file->file->meta();
+    if (digest) std::memcpy(digest, file->file->digest().data(), 32);
+  });
+}
+
+int c8mi_native_table_file_destroy(c8mi_native_table_file* file) {
+  return guarded([&] { delete file; });
 }
 
 int c8mi_em_tables_destroy(c8mi_em_tables* tables) {

@@ -232,6 +232,56 @@ typedef struct c8mi_native_query {
   double argument; /* quantile, grammage or range, by kind */
 } c8mi_native_query;
 
+/* ---- the table set on disk ----
+ * The metadata a ProposalNativeTableSet carries beside its arrays
+ * (corsika/kokkos/em/tables/ProposalNativeTable.hpp:21-56): format version,
+ * dependency versions, stochastic cut and the Interaction identities. */
+typedef struct c8mi_native_interaction_identity { /* NativeInteractionIdentity */
+  int32_t pdg_id;
+  uint32_t reserved;
+  uint64_t medium_hash;
+  uint64_t interaction_hash;
+} c8mi_native_interaction_identity;
+
+typedef struct c8mi_native_table_meta {
+  uint32_t format_version;               /* 7 */
+  uint32_t pad_;
+  char proposal_version[32];             /* NUL-terminated, "7.6.2" for a PROPOSAL export */
+  char cubic_interpolation_version[32];  /* "0.1.5" */
+  double stochastic_cut_mev;
+  const c8mi_native_interaction_identity* interaction_identities;
+  uint64_t interaction_identity_count;
+} c8mi_native_table_meta;
+
+/* sha256 (corsika/kokkos/utility/Sha256.hpp:21): FIPS 180-4 SHA-256 of a byte
+ * string. */
+int c8mi_sha256(const uint8_t* data, uint64_t size, uint8_t digest[32]);
+
+/* calculate_proposal_native_hash (src/kokkos/em/tables/ProposalNativeTable.cpp:
+ * 570-666): SHA-256 of the canonical little-endian encoding of the metadata
+ * and every array, in the reference's order; the table's content hash (the
+ * `table_sha256` of the shower report). */
+int c8mi_native_table_hash(const c8mi_native_table_set* set, const c8mi_native_table_meta* meta,
+                           uint8_t digest[32]);
+
+/* The table file: 8 bytes "C8MINT07", the payload length (u64 little
+ * endian), the canonical encoding above as payload, its SHA-256. Written to a
+ * temporary file in the target's directory, read back and verified, then
+ * renamed over `path`. */
+int c8mi_native_table_write(const char* path, const c8mi_native_table_set* set,
+                            const c8mi_native_table_meta* meta);
+
+/* Reads and verifies a table file (magic, length, digest, every descriptor
+ * within its arrays): an owning handle whose set and metadata views stay
+ * valid until it is destroyed; pass the set to c8mi_em_create or
+ * c8mi_em_tables_create. A bad file is status 1 (invalid_argument), an
+ * unreadable one status 4. */
+typedef struct c8mi_native_table_file c8mi_native_table_file;
+int c8mi_native_table_read(const char* path, c8mi_native_table_file** out);
+int c8mi_native_table_file_view(const c8mi_native_table_file* file, c8mi_native_table_set* set,
+                                c8mi_native_table_meta* meta, uint8_t digest[32]);
+int c8mi_native_table_file_destroy(c8mi_native_table_file* file);
+
 #ifdef __cplusplus
 static_assert(sizeof(c8mi_native_axis) == 32, "NativeAxisDescriptor layout");
 static_assert(sizeof(c8mi_native_bicubic) == 88, "NativeBicubicDescriptor layout");

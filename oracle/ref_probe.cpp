@@ -12,6 +12,8 @@
 #include <corsika/kokkos/em/random/Philox.hpp>
 #include <corsika/kokkos/em/tables/MoliereInterpolation.hpp>
 #include <corsika/kokkos/em/tables/ProposalNativeQueries.hpp>
+#include <corsika/kokkos/em/tables/ProposalNativeTable.hpp>
+#include <corsika/kokkos/utility/Sha256.hpp>
 #include <corsika/kokkos/environment/EnvironmentBorrowedView.hpp>
 #include <corsika/kokkos/environment/EnvironmentQuery.hpp>
 #include <corsika/kokkos/environment/ObservationPlane.hpp>
@@ -19,6 +21,8 @@
 #include <cstddef>
 #include <cstdint>
 #include <cstring>
+#include <exception>
+#include <string>
 #include <vector>
 
 #include "c8mi/c8mi_em.h"
@@ -416,6 +420,73 @@ int ref_em_thinning(c8mi_em_thinning const* c, double pe, double pw, double e1, 
 int ref_em_pair_split(double e_gev, double x, double* normalized, double* decoded) {
   if (!ck::encode_photon_pair_normalized_split(e_gev, x, *normalized)) return 0;
   return ck::decode_photon_pair_normalized_split(e_gev, *normalized, *decoded) ? 1 : 0;
+}
+
+
+/* The reference's content hash of a table set with its metadata
+ * (calculate_proposal_native_hash, ProposalNativeTable.cpp:570-666) and its
+ * validation (validate_proposal_native_table, :234-568; returns 0 when valid,
+ * else 1 with the message in `message`). */
+static ck::ProposalNativeTableSet owning_set(c8mi_native_table_set const* t, c8mi_native_table_meta const* m) {
+  ck::ProposalNativeTableSet s;
+  s.formatVersion_ = m->format_version;
+  s.proposalVersion_ = m->proposal_version;
+  s.cubicInterpolationVersion_ = m->cubic_interpolation_version;
+  std::memcpy(&s.constants_, &t->constants, sizeof(s.constants_));
+  s.stochasticCutMeV_ = m->stochastic_cut_mev;
+  for (uint64_t i = 0; i < m->interaction_identity_count; ++i) {
+    ck::NativeInteractionIdentity id{};
+    id.pdgId_ = m->interaction_identities[i].pdg_id;
+    id.reserved_ = m->interaction_identities[i].reserved;
+    id.mediumHash_ = m->interaction_identities[i].medium_hash;
+    id.interactionHash_ = m->interaction_identities[i].interaction_hash;
+    s.interactionIdentities_.push_back(id);
+  }
+  auto copy = [](auto& out, auto const* in, std::size_t n) {
+    out.resize(n);
+    if (n) std::memcpy(out.data(), in, n * sizeof(out[0]));
+  };
+  copy(s.dndxColumns_, t->dndx_columns, t->dndx_column_count);
+  copy(s.totalRateColumns_, t->total_rate_columns, t->total_rate_column_count);
+  copy(s.selectionColumnIndices_, t->selection_column_indices, t->dndx_column_count);
+  copy(s.dedxColumns_, t->dedx_columns, t->dedx_column_count);
+  copy(s.dedxAccumulationIndices_, t->dedx_accumulation_indices, t->dedx_column_count);
+  copy(s.utilityColumns_, t->utility_columns, t->utility_column_count);
+  copy(s.bicubicValues_, t->bicubic_values, t->bicubic_coefficient_count);
+  copy(s.bicubicDerivativeEnergy_, t->bicubic_derivative_energy, t->bicubic_coefficient_count);
+  copy(s.bicubicDerivativeLoss_, t->bicubic_derivative_loss, t->bicubic_coefficient_count);
+  copy(s.bicubicMixedDerivative_, t->bicubic_mixed_derivative, t->bicubic_coefficient_count);
+  copy(s.bicubicPolynomialCoefficients_, t->bicubic_polynomial_coefficients, t->bicubic_polynomial_coefficient_count);
+  copy(s.cubicValues_, t->cubic_values, t->cubic_coefficient_count);
+  copy(s.cubicNodeDerivatives_, t->cubic_node_derivatives, t->cubic_coefficient_count);
+  return s;
+}
+
+int ref_native_table_hash(c8mi_native_table_set const* t, c8mi_native_table_meta const* m, uint8_t out[32]) {
+  auto const digest = ck::calculate_proposal_native_hash(owning_set(t, m));
+  std::memcpy(out, digest.data(), 32);
+  return 0;
+}
+
+int ref_native_table_validate(c8mi_native_table_set const* t, c8mi_native_table_meta const* m, char* message,
+                              uint64_t capacity) {
+  try {
+    ck::validate_proposal_native_table(owning_set(t, m));
+    return 0;
+  } catch (std::exception const& e) {
+    if (message && capacity) {
+      std::strncpy(message, e.what(), capacity - 1);
+      message[capacity - 1] = '\0';
+    }
+    return 1;
+  }
+}
+
+int ref_sha256(uint8_t const* data, uint64_t size, uint8_t out[32]) {
+  std::vector<std::uint8_t> bytes(data, data + size);
+  auto const digest = ck::sha256(bytes);
+  std::memcpy(out, digest.data(), 32);
+  return 0;
 }
 
 }  // extern "C"
