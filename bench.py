@@ -30,6 +30,8 @@ UNIT = "track·antenna updates/s"
 FLOPS_PER_UPDATE = {"coreas_zhs": 285.0, "zhs": 155.0, "coreas": 130.0}
 FP64_PEAK_TFLOPS = 78.6
 HBM_PEAK_GBS = 8000.0
+# VALU issue: 256 CUs x 4 SIMDs, each a wave64 FP64/FP32 instruction every 4 clocks, at 2.4 GHz
+VALU_ISSUE_PEAK_G = 256 * 2.4
 TRACK_BYTES = 72
 
 
@@ -205,6 +207,31 @@ def em_measured_traffic(kernels):
     return round(sum(e["traffic_bytes_per_step"] for e in entries), 1)
 
 
+def radio_valu_issue(kernel, updates, kernel_ms):
+    """The projection kernel's share of the VALU issue rate: its counted VALU
+    wave-instructions per track-antenna update (committed rocprofv3 counter
+    summary) times the launch's updates over the live launch time, or None."""
+    try:
+        with open(os.path.join(ROOT, "profiles", "radio_valu.json")) as f:
+            per = json.load(f)["kernels"][kernel]["valu_per_update"]
+    except (OSError, ValueError, KeyError):
+        return None
+    rate = per * updates / (kernel_ms * 1e-3) / 1e9
+    return {"valu_per_update": per, "achieved": rate, "peak": VALU_ISSUE_PEAK_G,
+            "unit": "G VALU wave-instructions/s", "frac": rate / VALU_ISSUE_PEAK_G}
+
+
+def em_valu_per_unit():
+    """VALU wave-instructions per unit of work (photon step, e+- step, e+-
+    interaction) of each EM kernel from the committed rocprofv3 counter summary
+    (SQ_INSTS_VALU over the profiled run's device steps), or {}."""
+    try:
+        with open(os.path.join(ROOT, "profiles", "em_valu.json")) as f:
+            return {k: v["valu_per_unit"] for k, v in json.load(f)["kernels"].items()}
+    except (OSError, ValueError, KeyError):
+        return {}
+
+
 def em_showers(energy, count, first_shower, atm=None, pid=22):
     from c8mi import em
 
@@ -335,6 +362,7 @@ def em_section(showers, plain_showers, slots, cpu_seconds, no_cpu_baseline, ener
     failed = [any(l["fallback_reasons"][21] for l in ls) for ls in ledgers]
     residual = max([abs(sum(em.ledger_residual(l) for l in ls)) for ls, f in zip(ledgers, failed) if not f]
                    or [0.0])
+    residual_all = max(abs(sum(em.ledger_residual(l) for l in ls)) for ls in ledgers)
     injected = max(abs(sum(l["injected_energy"] for l in ls) - energy) for ls in ledgers)
     pool_latency = [r["finished_s"] - r["admitted_s"] for r in results]
 
@@ -361,8 +389,9 @@ def em_section(showers, plain_showers, slots, cpu_seconds, no_cpu_baseline, ener
     kind = max(step_kinds, key=lambda k: timing[k][0])
     step_kernel_ms = sum(timing[k][0] for k in step_kinds)
     total_kernel_ms = sum(v[0] for v in timing.values())
-    # algorithmic bytes of all particle-steps over the summed step-kernel time
-    achieved = steps * EM_BYTES_PER_STEP / (step_kernel_ms * 1e-3) / 1e9
+    device_steps = totals["photon_steps"] + totals["lepton_steps"]
+    # algorithmic bytes of all device particle-steps over the summed step-kernel time
+    hbm_achieved = device_steps * EM_BYTES_PER_STEP / (step_kernel_ms * 1e-3) / 1e9
     # the "lepton_transport" timing covers the fused kernel, or with Molière its
     # move, Molière, tail and finish kernels
     tag = "true" if moliere_table else "false"
@@ -372,6 +401,20 @@ def em_section(showers, plain_showers, slots, cpu_seconds, no_cpu_baseline, ener
     kernel_names = {"photon_step": ["photon_step_kernel"], "lepton_transport": transport_kernels,
                     "lepton_vertex": ["lepton_vertex_kernel"]}[kind]
     kernel_name = "+".join(kernel_names)
+    # The step kernels are bound by VALU issue (FP64 arithmetic with divergent
+    # loops), not HBM: the roofline is the VALU issue rate of the dominant
+    # group, its counted instructions per unit of work over the live HIP-event
+    # time of its launches.
+    valu = em_valu_per_unit()
+    units = {"photon_step": totals["photon_steps"], "lepton_transport": totals["lepton_steps"],
+             "lepton_vertex": totals["lepton_interactions"]}
+    group_valu = {k: (sum(valu[n] for n in names) if all(n in valu for n in names) else None)
+                  for k, names in (("photon_step", ["photon_step_kernel"]), ("lepton_transport", transport_kernels),
+                                   ("lepton_vertex", ["lepton_vertex_kernel"]))}
+    achieved = (group_valu[kind] * units[kind] / (timing[kind][0] * 1e-3) / 1e9
+                if group_valu[kind] is not None and timing[kind][0] > 0 else None)
+    chip_valu = (sum(group_valu[k] * units[k] for k in step_kinds) / elapsed / 1e9
+                 if all(group_valu[k] is not None for k in step_kinds) else None)
     out = {
         "metric": "EM particle-steps/s",
         "value": steps / elapsed,
@@ -402,19 +445,29 @@ def em_section(showers, plain_showers, slots, cpu_seconds, no_cpu_baseline, ener
                                + f"{showers} showers through {slots} pool slots"
                                + (f" in {pools} pools on their own streams" if pools > 1 else "")},
         "roofline": {
-            "bound": "hbm",
+            "bound": "valu",
+            "pipe": "fp64 VALU issue: one wave64 instruction per CU per clock, 256 CUs at 2.4 GHz "
+                    "(no MFMA instruction is issued)",
             "achieved": achieved,
-            "peak": HBM_PEAK_GBS,
-            "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS,
+            "peak": VALU_ISSUE_PEAK_G,
+            "unit": "G VALU wave-instructions/s",
+            "frac": achieved / VALU_ISSUE_PEAK_G if achieved is not None else None,
             "traffic": em_measured_traffic(kernel_names),
             "kernel": kernel_name,
-            "bytes_per_step": EM_BYTES_PER_STEP,
+            "valu_per_unit": group_valu[kind],
+            "unit_of_work": {"photon_step": "photon step", "lepton_transport": "e+- step",
+                             "lepton_vertex": "e+- interaction"}[kind],
+            "chip_valu_frac": chip_valu / VALU_ISSUE_PEAK_G if chip_valu is not None else None,
+            "hbm": {"achieved_gbs": hbm_achieved, "frac": hbm_achieved / HBM_PEAK_GBS,
+                    "bytes_per_step": EM_BYTES_PER_STEP},
             "kernel_ms": {k: v[0] for k, v in timing.items()},
             "launches": {k: v[1] for k, v in timing.items()},
             "device_busy_fraction": total_kernel_ms * 1e-3 / elapsed,
         },
         "ledger_residual_gev_max": residual,
+        # every shower, those with a Molière-failed step included: their residual is
+        # the failed step's continuous deposit, which the reference does not book
+        "ledger_residual_gev_max_all_showers": residual_all,
         "injected_energy_error_gev_max": injected,
         "showers_with_moliere_fallbacks": int(sum(failed)),
     }
@@ -1030,6 +1083,7 @@ def main():
             "kernel_ms_per_step": {k: v[0] / args.steps for k, v in timing.items() if v[1]},
             "flops_per_update": FLOPS_PER_UPDATE[kind],
             "algorithmic_hbm_gbs": (n * TRACK_BYTES) / (kernel_avg_ms * 1e-3) / 1e9,
+            "valu_issue": radio_valu_issue(kernel_name, updates, kernel_avg_ms),
         },
         "counters": counters,
     }

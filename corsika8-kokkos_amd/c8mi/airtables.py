@@ -165,10 +165,11 @@ def _compton_cdf_function(nodes_per_unit=4000, medium=AIR_MEDIUM):
 
 
 def build_air_tables(cut_mev: float = 1.0, e_max_mev: float = 2.0e9, energy_nodes: int = 160,
-                     loss_nodes: int = 64, polynomial: bool = True):
+                     loss_nodes: int = 64, polynomial: bool = True, muon_pair: float = 0.0):
     """Air tables for photons and e+- with an e+- energy-loss cut of `cut_mev`
-    (stochastic losses above it, continuous below)."""
-    return build_tables(AIR_MEDIUM, cut_mev, e_max_mev, energy_nodes, loss_nodes, polynomial)
+    (stochastic losses above it, continuous below); `muon_pair` > 0 adds the
+    photon mu-pair columns at that multiple of their rate."""
+    return build_tables(AIR_MEDIUM, cut_mev, e_max_mev, energy_nodes, loss_nodes, polynomial, muon_pair)
 
 
 def build_ice_tables(cut_mev: float = 0.1, e_max_mev: float = 2.0e9, energy_nodes: int = 160,
@@ -178,10 +179,34 @@ def build_ice_tables(cut_mev: float = 0.1, e_max_mev: float = 2.0e9, energy_node
     return build_tables(ICE_MEDIUM, cut_mev, e_max_mev, energy_nodes, loss_nodes, polynomial)
 
 
-def build_tables(medium, cut_mev, e_max_mev=2.0e9, energy_nodes=160, loss_nodes=64, polynomial=True):
+def muon_pair_threshold_mev():
+    """Lowest photon energy of the native mu-pair split: 2 m_mu sqrt(e)."""
+    return 2.0 * T.CONSTANTS["muon_mass_mev"] * T.CONSTANTS["sqrt_e"]
+
+
+def muon_pair_limits(e):
+    """The split's x = E(mu+)/E range at photon energy e (MeV): the mu-pair limits
+    of the native loss-fraction query."""
+    width = np.sqrt(np.clip(1.0 - muon_pair_threshold_mev() / np.asarray(e, dtype=np.float64), 0.0, None))
+    return 0.5 * (1.0 - width), 0.5 * (1.0 + width)
+
+
+def muon_pair_scale(e):
+    """Threshold factor of the synthetic mu-pair rate: zero at the split's
+    threshold, the pair rate's (m_e/m_mu)^2 far above it."""
+    e = np.asarray(e, dtype=np.float64)
+    r = muon_pair_threshold_mev() / e
+    ratio = (ELECTRON_MASS_MEV / T.CONSTANTS["muon_mass_mev"]) ** 2
+    return np.where(r < 1.0, ratio * np.clip(1.0 - r, 0.0, None) ** 3, 0.0)
+
+
+def build_tables(medium, cut_mev, e_max_mev=2.0e9, energy_nodes=160, loss_nodes=64, polynomial=True,
+                 muon_pair=0.0):
     """Tables of `medium` for photons and e+-. Below a 0.5 MeV cut the e+-
     process tables start at 0.55 MeV total energy (else 1 MeV), so every
-    tracked e+- above the cut lies on them."""
+    tracked e+- above the cut lies on them. `muon_pair` > 0: photon mu-pair
+    columns per element (the pair rate's shape in x over the mu-pair limits,
+    (m_e/m_mu)^2 of its size, times `muon_pair`)."""
     elements = medium.components
     electrons_per_g = medium.electrons_per_g
     lepton_low = 1.0 if cut_mev >= 0.5 else 0.55
@@ -215,6 +240,22 @@ def build_tables(medium, cut_mev, e_max_mev=2.0e9, energy_nodes=160, loss_nodes=
             spline=b.bicubic(ax_pair, lin, f, polynomial=polynomial), component_weight=w,
             particle_mass_mev=0.0, component_nuclear_charge=z, component_atomic_mass=a,
             component_average_nucleon_mass_mev=931.494, lower_energy_limit_mev=1.1)
+    mu_low = 1.001 * muon_pair_threshold_mev()
+    if muon_pair > 0.0:
+        ax_mu = T.axis(T.AXIS_EXP, mu_low, e_max_mev, energy_nodes)
+        for name, z, a, w, x0, h in elements:
+            def fm(e, t, w=w, x0=x0):
+                xmin, xmax = muon_pair_limits(e)
+                x = _rellog(xmin, xmax, t)
+                norm = pair_cdf(xmax) - pair_cdf(xmin)
+                frac = np.where(norm > 0, (pair_cdf(x) - pair_cdf(xmin)) / np.where(norm > 0, norm, 1), 0)
+                return w * (w / x0) * (7.0 / 9.0) * muon_pair * muon_pair_scale(e) * frac
+            add(22, process_id=proc["photon_muon_pair"], component_hash=h,
+                kinematic_model=kin["only_stochastic"], loss_transform=T.LOSS_RELLOG,
+                rate_model=T.RATE_BICUBIC, spline=b.bicubic(ax_mu, lin, fm, polynomial=polynomial),
+                component_weight=w, particle_mass_mev=0.0, component_nuclear_charge=z,
+                component_atomic_mass=a, component_average_nucleon_mass_mev=931.494,
+                lower_energy_limit_mev=mu_low)
     # Compton on the medium electrons
     add(22, process_id=proc["compton"], component_hash=0, kinematic_model=kin["compton"],
         loss_transform=T.LOSS_ONEMINUSLOG, rate_model=T.RATE_BICUBIC,
@@ -266,7 +307,11 @@ def build_tables(medium, cut_mev, e_max_mev=2.0e9, energy_nodes=160, loss_nodes=
         pair = np.where(e >= 1.1, pair, 0.0)
         compton = _compton_cdf_function(medium=medium)(e, np.ones_like(e))
         photo = sum(_sauter_rate(e, z, AVOGADRO * w / a) for _, z, a, w, _, _ in elements)
-        return pair + compton + photo
+        mu = 0.0
+        if muon_pair > 0.0:
+            mu = sum((w / x0) * (7.0 / 9.0) for _, _, _, w, x0, _ in elements) * muon_pair * muon_pair_scale(e)
+            mu = np.where(e >= mu_low, mu, 0.0)
+        return pair + compton + photo + mu
 
     def lepton_total(e, positron):
         e = np.asarray(e, dtype=np.float64)

@@ -262,20 +262,22 @@ class Statistics(ctypes.Structure):
         return {k: int(getattr(self, k)) for k, _ in self._fields_}
 
 
-LEDGER_ENERGY_TERMS = ("injected_energy", "continuous_deposit", "cut_deposit", "cut_rest_mass",
-                       "observed_energy", "escaped_energy", "observation_cut_overlap",
-                       "mass_convention_correction", "medium_rest_mass_input",
-                       "photoelectric_binding", "fallback_energy", "thinning_discarded_energy",
-                       "pending_energy")
+# energy terms at counters 0..12, then the native μ-pair cut energy at counter 61
+_COUNTER_ENERGY_TERMS = ("injected_energy", "continuous_deposit", "cut_deposit", "cut_rest_mass",
+                         "observed_energy", "escaped_energy", "observation_cut_overlap",
+                         "mass_convention_correction", "medium_rest_mass_input",
+                         "photoelectric_binding", "fallback_energy", "thinning_discarded_energy",
+                         "pending_energy")
+LEDGER_ENERGY_TERMS = _COUNTER_ENERGY_TERMS + ("native_muon_pair_cut_energy",)
 
 
 class Ledger(ctypes.Structure):
-    _fields_ = ([(k, c_double) for k in LEDGER_ENERGY_TERMS] +
+    _fields_ = ([(k, c_double) for k in _COUNTER_ENERGY_TERMS] +
                 [("steps", c_uint64), ("photon_cuts", c_uint64), ("lepton_limits", c_uint64 * 8),
                  ("thinning_hillas_vertices", c_uint64), ("thinning_statistical_vertices", c_uint64),
                  ("thinning_particles_discarded", c_uint64), ("deposited_steps", c_uint64),
                  ("fixed_point_overflows", c_uint64), ("invalid_records", c_uint64),
-                 ("fallback_reasons", c_uint64 * 32)])
+                 ("fallback_reasons", c_uint64 * 32), ("native_muon_pair_cut_energy", c_double)])
 
     def as_dict(self):
         out = {}
@@ -316,10 +318,10 @@ FALLBACK_DTYPE = np.dtype({"names": ["particle", "reason", "process_id", "stage"
                            "offsets": [0, 128, 132, 136], "itemsize": 144})
 
 # Counter layout of the device ledger (and of the oracle's counter array).
-COUNTER_INDEX = {name: k for k, name in enumerate(LEDGER_ENERGY_TERMS)}
+COUNTER_INDEX = {name: k for k, name in enumerate(_COUNTER_ENERGY_TERMS)}
 COUNTER_INDEX.update(steps=13, photon_cuts=14, lepton_limit0=15, thin_hillas=23,
                      thin_statistical=24, thin_discarded=25, deposited_steps=26, overflows=27,
-                     invalid=28, fallback_reason0=29)
+                     invalid=28, fallback_reason0=29, native_muon_pair_cut_energy=61)
 COUNTERS = 64
 
 
@@ -331,7 +333,7 @@ def ledger_residual(ledger: dict) -> float:
              ledger["observed_energy"] + ledger["escaped_energy"] - ledger["observation_cut_overlap"] +
              ledger["fallback_energy"] + ledger["pending_energy"] + ledger["thinning_discarded_energy"] +
              ledger["photoelectric_binding"] - ledger["medium_rest_mass_input"] -
-             ledger["mass_convention_correction"])
+             ledger["mass_convention_correction"] + ledger.get("native_muon_pair_cut_energy", 0.0))
     return ledger["injected_energy"] - sinks
 
 
@@ -604,8 +606,8 @@ def shower_axis(atmosphere, start, direction, step_m=10.0):
 
 def decode_counters(counters, energy_scale):
     counters = np.asarray(counters, dtype=np.int64)
-    out = {name: float(counters[k]) / energy_scale for k, name in enumerate(LEDGER_ENERGY_TERMS)}
     ci = COUNTER_INDEX
+    out = {name: float(counters[ci[name]]) / energy_scale for name in LEDGER_ENERGY_TERMS}
     out["steps"] = int(counters[ci["steps"]])
     out["photon_cuts"] = int(counters[ci["photon_cuts"]])
     out["lepton_limits"] = [int(x) for x in counters[ci["lepton_limit0"]:ci["lepton_limit0"] + 8]]

@@ -1158,6 +1158,7 @@ void ledger_from_counters(long long const* c, double energy_scale, c8mi_em_ledge
   ledger->fixed_point_overflows = static_cast<std::uint64_t>(c[kOverflows]);
   ledger->invalid_records = static_cast<std::uint64_t>(c[kInvalid]);
   for (int k = 0; k < 32; ++k) ledger->fallback_reasons[k] = static_cast<std::uint64_t>(c[kFallbackReason0 + k]);
+  ledger->native_muon_pair_cut_energy = static_cast<double>(c[kNativeMuonCut]) / energy_scale;
 }
 
 std::uint64_t EmBackend::slot_download_records(std::uint32_t slot, int kind, void* out, std::size_t record_bytes,

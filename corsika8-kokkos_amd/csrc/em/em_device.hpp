@@ -40,7 +40,8 @@ enum Counter : int {
   kDepositedSteps = 26,
   kOverflows = 27,
   kInvalid = 28,
-  kFallbackReason0 = 29,
+  kFallbackReason0 = 29,  // 32 entries
+  kNativeMuonCut = 61,    // signed energy of μ-pair muons cut at their first step
   kCounters = 64
 };
 

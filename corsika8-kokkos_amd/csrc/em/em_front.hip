@@ -143,7 +143,7 @@ __device__ __forceinline__ phys::Outputs no_outputs(FrontArgs const& f, std::uin
   o.track_stride = f.track_stride;
   o.kind[0] = o.kind[1] = kSlotNone;
   o.next = o.other = o.children = 0;
-  o.step = o.interaction = false;
+  o.step = o.interaction = o.tracked = false;
   return o;
 }
 

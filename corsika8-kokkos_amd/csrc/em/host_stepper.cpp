@@ -102,7 +102,7 @@ HostStep HostStepper::step(c8mi_particle_state const& particle, std::uint64_t fi
   o.track_stride = 1;
   o.kind[0] = o.kind[1] = kSlotNone;
   o.next = o.other = o.children = 0;
-  o.step = o.interaction = false;
+  o.step = o.interaction = o.tracked = false;
   EmContext const& c = context_;
   if (p.pid == 22) {
     phys::photon_step(c, acc, p, o);
@@ -121,7 +121,7 @@ HostStep HostStepper::step(c8mi_particle_state const& particle, std::uint64_t fi
       phys::LeptonVertex vx;
       if (phys::lepton_finish(c, acc, [&] { return p; }, mv, failed, o, vx)) phys::lepton_vertex(c, acc, vx, o);
     }
-    r.has_track = o.step;
+    r.has_track = o.tracked;
   }
   r.step = o.step;
   r.interaction = o.interaction;

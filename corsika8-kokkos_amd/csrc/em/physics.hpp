@@ -58,8 +58,14 @@ enum Process : std::int32_t {
   kPhotonPair = 1000000013,
   kPhotonMuPair = 1000000015,
   kPhotoelectric = 1000000016,
-  kPhotonPairFinalState = 2000000013
+  kPhotonPairFinalState = 2000000013,
+  kPhotonMuPairFinalState = 2000000015
 };
+// Photon μ-pair draws (PhotonFinalStateRandomDomains.hpp:25-27).
+inline constexpr std::uint64_t kMuPairSplitDraw = std::uint64_t{1} << 32;
+inline constexpr std::uint64_t kMuPairAzimuthDraw = kMuPairSplitDraw + 1;
+// reserved_ tag of a μ-pair muon below its transport mass (NativeMuonPairCut.hpp:17).
+inline constexpr std::uint32_t kNativeMuonPairCutFlag = 1u;
 
 enum Inter : int { kSelected = 0, kNoDiscrete = 1, kReselect = 2, kDistanceSampled = 3, kParticleCut = 4 };
 enum Lookup : int {
@@ -103,6 +109,12 @@ C8MI_HD double mass_correction(std::int32_t pid, double mass, double weight) {
 C8MI_HD bool is_ee(std::int32_t pid) { return pid == 11 || pid == -11; }
 C8MI_HD bool is_charged_lepton(std::int32_t pid) {
   return pid == 11 || pid == -11 || pid == 13 || pid == -13;
+}
+// is_native_muon_pair_cut (NativeMuonPairCut.inl:12-16): a muon of a photon
+// μ-pair made at or below its transport mass, cut at its first step.
+C8MI_HD bool native_muon_pair_cut(c8mi_particle_state const& p) {
+  return p.reserved == kNativeMuonPairCutFlag && (p.pid == 13 || p.pid == -13) && p.generation > 0 &&
+         p.step_id == 0 && p.energy_gev > 0. && p.energy_gev <= transport_mass(p.pid);
 }
 
 // ---- accumulation: the histograms and counters of the particle's shower ----
@@ -392,6 +404,8 @@ C8MI_HD bool encode_pair_split(double e, double rho, double& s) {
 
 C8MI_HD Tq dq_loss_fraction(EmContext const& c, std::int32_t pid, std::int32_t process,
                    std::uint64_t comp, double e, double q) {
+  if (process == kPhotonMuPairFinalState && pid == 22)
+    return native(loss_fraction(c.tables, pid, kPhotonMuPair, comp, e, q, true), false);
   if (process == kPhotonPairFinalState && pid == 22) {
     Tq r = native(loss_fraction(c.tables, pid, kPhotonPair, comp, e, q, true), false);
     if (r.status != kTsSuccess) return r;
@@ -522,7 +536,7 @@ C8MI_HD void select_interaction(EmContext const& c, Particle const& p, Interacti
   in.process_uniform = 0.;
   in.loss_quantile = 0.;
   double const e_mev = p.energy_gev * 1000.;
-  if (p.time_s > kCutTime) {
+  if (native_muon_pair_cut(p) || p.time_s > kCutTime) {
     in.status = kParticleCut;
     in.interaction_grammage = __builtin_inf();
     return;
@@ -908,6 +922,7 @@ struct Outputs {
   std::uint32_t kind[2];  // SlotKind
   std::uint32_t next, other, children;
   bool step, interaction;
+  bool tracked;  // `track` holds this step's e+- track
 };
 
 // Compact radio track of one e+- step (the make_radio_track_kinematics input,

@@ -141,8 +141,43 @@ C8MI_HD void photon_step(EmContext const& c, Accumulator const& acc, Particle co
     return final_fallback(requires_selected_loss(in.deferred_reason) ? in.deferred_reason
                                                                      : kFrInvalidFinalState);
   int const cap = capability(parent.pid, in.process_id);
-  if (cap != kCapPair && cap != kCapCompton && cap != kCapPhotoelectric)
-    return final_fallback(cap == kCapCpuOnly || cap == kCapMuPair ? kFrCpuOnly : kFrNotImplemented);
+  if (cap != kCapPair && cap != kCapMuPair && cap != kCapCompton && cap != kCapPhotoelectric)
+    return final_fallback(cap == kCapCpuOnly ? kFrCpuOnly : kFrNotImplemented);
+  if (cap == kCapMuPair) {
+    // muon pair (PhotonFinalStateStep.inl:326-382, 679-731): the split x = E(mu+)/E
+    // from the native CDF, polar angle m/E for both muons, opposite azimuths; no
+    // thinning (the muons are not EM) and no LPM veto
+    double const mu_mass = c.tables.constants.muon_mass_mev / 1000.;
+    if (!isfinite(parent.energy_gev) || !isfinite(parent.weight) || parent.weight < 0. || !(mu_mass > 0.) ||
+        parent.generation == 0xffffffffU || parent.step_id == 0xffffffffffffffffULL)
+      return final_fallback(kFrInvalidFinalState);
+    double const su = uniform(c, parent, kPhotonMuPair, kMuPairSplitDraw);
+    Tq const split = dq_loss_fraction(c, 22, kPhotonMuPairFinalState, in.component_hash,
+                                      parent.energy_gev * 1000., su);
+    if (split.status != kTsSuccess) return final_fallback(kFrInvalidFinalState);
+    double const minus_total = (1. - split.value) * parent.energy_gev;
+    double const plus_total = split.value * parent.energy_gev;
+    if (!isfinite(split.value) || !(minus_total > 0.) || !(plus_total > 0.))
+      return final_fallback(kFrInvalidFinalState);
+    double const az = uniform(c, parent, kPhotonMuPair, kMuPairAzimuthDraw) * kPhotonTwoPi;
+    double const k = (parent.energy_gev * 1000.) / (mu_mass * 1000.);
+    double cosine = cos(1. / k);
+    if (cosine == -1.) cosine *= -1.;
+    Particle mu = child_of(parent, 13, secondary_energy(13, minus_total, mu_mass), parent.weight);
+    Particle mp = child_of(parent, -13, secondary_energy(-13, plus_total, mu_mass), parent.weight);
+    if (mu.energy_gev <= transport_mass(13)) mu.reserved = kNativeMuonPairCutFlag;
+    if (mp.energy_gev <= transport_mass(-13)) mp.reserved = kNativeMuonPairCutFlag;
+    deflect(parent.direction, cosine, az, mu.direction, false);
+    deflect(parent.direction, cosine, fmod(az + kPhotonPi, kPhotonTwoPi), mp.direction, false);
+    first_interaction(acc, parent, kPhotonMuPair, mu, &mp);
+    mu.history_id = 0;
+    emit(o, mu, kSlotOtherChild);
+    mp.history_id = 1;
+    emit(o, mp, kSlotOtherChild);
+    ledger(acc, kMassCorrection, mass_correction(13, mu_mass, 2. * parent.weight));
+    thinning_ledger(acc, 0, 2, 2, 0., 0.);
+    return;
+  }
   if (cap == kCapCompton) {
     double pc, ec;
     if (parent.generation == 0xffffffffU || parent.step_id == 0xffffffffffffffffULL ||
@@ -273,6 +308,7 @@ struct LeptonMove {
   std::int32_t process_id;
   std::uint32_t particle_cut;  // cut at the start: no scattering
   std::uint32_t observed_before_cut;
+  std::uint32_t native_cut;    // a μ-pair muon cut at its first step (signed deposit, no track)
 };
 
 // The move of one e+- step (selection, continuous step, transport): its end
@@ -291,13 +327,26 @@ C8MI_HD bool lepton_move(EmContext const& c, Accumulator const& acc, Particle co
     return false;
   };
   Particle start = in.particle;
-  if (!is_ee(start.pid)) return transport_fallback(kFrUnsupportedParticle);
+  if (!is_charged_lepton(start.pid)) return transport_fallback(kFrUnsupportedParticle);
   if (in.status != kSelected && in.status != kDistanceSampled && in.status != kNoDiscrete &&
       in.status != kParticleCut)
     return transport_fallback(kFrInvalidFinalState);
   if ((in.status == kSelected || in.status == kDistanceSampled) &&
       (!isfinite(in.interaction_grammage) || in.interaction_grammage < 0.))
     return transport_fallback(kFrInvalidFinalState);
+  if (in.status == kParticleCut && native_muon_pair_cut(start)) {
+    // native_muon_pair_cut_record (NativeMuonPairCutRecord.inl:12-25): a
+    // zero-length cut, no table query, no scattering; the deposit E - m is <= 0
+    mv = LeptonMove{};
+    mv.start = start;
+    mv.end = start;
+    mv.end.step_id++;
+    mv.cut_deposit = start.energy_gev - transport_mass(start.pid);
+    mv.limit = kLlCut;
+    mv.particle_cut = 1u;
+    mv.native_cut = 1u;
+    return true;
+  }
   // prepare_lepton_continuous_step (LeptonContinuousStep.inl:43-120)
   Tq const native_mass = cq_mass(c, start.pid);
   double const tm = 1000. * transport_mass(start.pid);
@@ -309,6 +358,9 @@ C8MI_HD bool lepton_move(EmContext const& c, Accumulator const& acc, Particle co
   if (!(e0 > tm)) return transport_fallback(kFrInvalidFinalState);
   Tq const tcut = cq_transport_cut(c, start.pid);
   if (tcut.status != kTsSuccess) return transport_fallback(kFrInvalidQuery);
+  // muons with continuous tables: this backend has no muon decay or muon
+  // scattering, so they stay with the host
+  if (!is_ee(start.pid)) return transport_fallback(kFrUnsupportedParticle);
   bool const particle_cut = in.status == kParticleCut || e0 - tm < tcut.value;
   double max_grammage = 0.;
   if (!particle_cut) {
@@ -472,6 +524,7 @@ C8MI_HD bool lepton_move(EmContext const& c, Accumulator const& acc, Particle co
   mv.process_id = in.process_id;
   mv.particle_cut = particle_cut ? 1u : 0u;
   mv.observed_before_cut = observed_before_cut ? 1u : 0u;
+  mv.native_cut = 0u;
   return true;
 }
 
@@ -589,7 +642,6 @@ C8MI_HD bool lepton_finish(EmContext const& c, Accumulator const& acc, Input con
   int const limit = mv.limit;
   double const continuous_deposit = mv.continuous_deposit, cut_deposit = mv.cut_deposit;
   bool const observed_before_cut = mv.observed_before_cut != 0;
-  (void)c;
 
   if (moliere_failed) {
     if (limit == kLlInteraction) record_fallback(acc, end, kFrMoliereFailed, 0, kStageTransport);
@@ -597,9 +649,26 @@ C8MI_HD bool lepton_finish(EmContext const& c, Accumulator const& acc, Input con
     return false;
   }
   o.step = true;
-  if (o.track) write_track(o.track, o.track_stride, start, end);  // e+- only reach here
-  // accumulate_lepton_profile_step
   double const w = start.weight;
+  double const tmg = transport_mass(start.pid);
+  if (mv.native_cut) {
+    // accumulate_lepton_profile_step of the native cut (ProfileAccumulation.inl:300-345):
+    // no track, a zero-length profile step, the signed deposit into the muon
+    // energy loss (its own ledger term here), the rest mass as at any cut
+    if (o.track) write_no_track(o.track, o.track_stride);
+    double const deposit = cut_deposit * w;
+    if (deposit < 0. && c.profile_bins != 0) count(acc, kDepositedSteps);
+    ledger(acc, kNativeMuonCut, deposit);
+    count(acc, kSteps);
+    count(acc, kLeptonLimit0 + kLlCut);
+    ledger(acc, kCutRestMass, tmg * w);
+    return false;
+  }
+  if (o.track) {
+    write_track(o.track, o.track_stride, start, end);  // e+- only reach here
+    o.tracked = true;
+  }
+  // accumulate_lepton_profile_step
   double const gs = project_grammage(acc, start.position_m);
   double const ge = project_grammage(acc, end.position_m);
   particle_profile(acc, start.pid, gs, ge, w);
@@ -608,7 +677,6 @@ C8MI_HD bool lepton_finish(EmContext const& c, Accumulator const& acc, Input con
   ledger(acc, kContinuous, continuous_deposit * w);
   count(acc, kSteps);
   count(acc, kLeptonLimit0 + limit);
-  double const tmg = transport_mass(start.pid);
   if (limit == kLlCut) {
     ledger(acc, kCutDeposit, cut_deposit * w);
     ledger(acc, kCutRestMass, tmg * w);

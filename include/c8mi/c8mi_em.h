@@ -396,6 +396,10 @@ typedef struct c8mi_em_ledger {
   uint64_t fixed_point_overflows;
   uint64_t invalid_records;
   uint64_t fallback_reasons[32];
+  double native_muon_pair_cut_energy;   /* signed E - m of photon μ-pair muons made at or
+                                           below their transport mass and cut at their
+                                           first step (weightedNativeMuonPairCutEnergy_,
+                                           a sink of the muon energy loss) */
 } c8mi_em_ledger;
 
 /* FallbackRequest subset: the particle at the failing stage and the reason

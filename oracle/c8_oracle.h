@@ -133,6 +133,7 @@ enum {
   OR_EMC_OVERFLOWS = 27,
   OR_EMC_INVALID = 28,
   OR_EMC_FALLBACK_REASON0 = 29, /* 32 entries */
+  OR_EMC_NATIVE_MUON_CUT = 61,  /* signed E - m of photon mu-pair muons cut at their first step */
   OR_EM_COUNTERS = 64
 };
 

@@ -60,6 +60,9 @@
 #define D_PAIR_E_POLAR 3ULL
 #define D_PAIR_P_POLAR 4ULL
 #define D_PAIR_LPM 5ULL
+#define D_MU_PAIR_SPLIT (1ULL << 32)              /* PhotonFinalStateRandomDomains.hpp:25-27 */
+#define D_MU_PAIR_AZIMUTH ((1ULL << 32) + 1ULL)
+#define NATIVE_MUON_PAIR_CUT_FLAG 1U             /* NativeMuonPairCut.hpp:17 */
 #define D_COMPTON_AZIMUTH 1ULL
 #define D_BREMS_AZIMUTH 1ULL
 #define D_BREMS_LPM 2ULL
@@ -72,7 +75,7 @@
 enum { P_BREMS = 1000000002, P_IONIZATION = 1000000003, P_EPAIR = 1000000004,
        P_COMPTON = 1000000010, P_ANNIHILATION = 1000000012, P_PHOTON_PAIR = 1000000013,
        P_PHOTON_MU_PAIR = 1000000015, P_PHOTOELECTRIC = 1000000016,
-       P_PHOTON_PAIR_FS = 2000000013 };
+       P_PHOTON_PAIR_FS = 2000000013, P_PHOTON_MU_PAIR_FS = 2000000015 };
 
 /* InteractionStatus (InteractionRecord.hpp:17-45) */
 enum { IS_SELECTED = 0, IS_NO_DISCRETE = 1, IS_RESELECT = 2, IS_DISTANCE_SAMPLED = 3,
@@ -961,6 +964,11 @@ static double mass_correction(int32_t pid, double native_mass, double weight) {
 static int is_ee(int32_t pid) { return pid == 11 || pid == -11; }
 static int is_muon(int32_t pid) { return pid == 13 || pid == -13; }
 static int is_charged_lepton(int32_t pid) { return is_ee(pid) || is_muon(pid); }
+/* is_native_muon_pair_cut (NativeMuonPairCut.inl:12-16) */
+static int native_muon_pair_cut(const ps_t* p) {
+  return p->reserved == NATIVE_MUON_PAIR_CUT_FLAG && is_muon(p->pid) && p->generation > 0 &&
+         p->step_id == 0 && p->energy_gev > 0. && p->energy_gev <= transport_mass_gev(p->pid);
+}
 
 /* query_continuous_* (ContinuousPhysicsQueries.inl:42-130) */
 static tq_t cq_mass(const cont_t* c, int32_t pid) {
@@ -1066,6 +1074,12 @@ static int encode_pair_split(double e_gev, double rho, double* s) {
 static tq_t dq_loss_fraction(const cont_t* c, int32_t pid, int32_t process, uint64_t comp,
                              double e, double q) {
   tq_t r;
+  if (process == P_PHOTON_MU_PAIR_FS && pid == 22) {
+    c8mi_native_result n = oracle_nq_loss_fraction(c->t, pid, P_PHOTON_MU_PAIR, comp, e, q, 1);
+    r.status = from_native(n.status, 0);
+    r.value = n.status == C8MI_QUERY_SUCCESS ? n.value : 0.;
+    return r;
+  }
   if (process == P_PHOTON_PAIR_FS && pid == 22) {
     c8mi_native_result n = oracle_nq_loss_fraction(c->t, pid, P_PHOTON_PAIR, comp, e, q, 1);
     r.status = from_native(n.status, 0);
@@ -1224,6 +1238,11 @@ static void select_interaction(const ctx_t* x, const ps_t* p, inter_t* in, fb_t*
   in->particle = *p;
   in->deferred_reason = -1;
   double e_mev = p->energy_gev * 1000.;
+  if (native_muon_pair_cut(p)) { /* InteractionSelection.inl:28-32 */
+    in->status = IS_PARTICLE_CUT;
+    in->interaction_grammage = INFINITY;
+    return;
+  }
   if (p->time_s > PARTICLE_CUT_TIME_S) {
     in->status = IS_PARTICLE_CUT;
     in->interaction_grammage = INFINITY;
@@ -2091,11 +2110,74 @@ static void photon_source(shower_t* s, const ps_t* p, pout_t* o, uint64_t child_
   }
   int cap = capability(parent->pid, vi->process_id);
   fb_t f;
-  if (cap != CAP_PHOTON_PAIR && cap != CAP_COMPTON && cap != CAP_PHOTOELECTRIC) {
-    /* photon muon pairs are left to the host in this build */
-    set_fallback(&f, vi, cap == CAP_CPU_ONLY || cap == CAP_PHOTON_MU_PAIR ? FR_CPU_ONLY
-                                                                          : FR_NOT_IMPLEMENTED);
+  if (cap != CAP_PHOTON_PAIR && cap != CAP_PHOTON_MU_PAIR && cap != CAP_COMPTON &&
+      cap != CAP_PHOTOELECTRIC) {
+    set_fallback(&f, vi, cap == CAP_CPU_ONLY ? FR_CPU_ONLY : FR_NOT_IMPLEMENTED);
     record_fallback(s, &f, STAGE_FINAL);
+    return;
+  }
+  if (cap == CAP_PHOTON_MU_PAIR) {
+    /* PhotonFinalStateStep.inl:326-382 (classification), 679-731 (children):
+     * x = E(mu+)/E from the native split CDF, both muons at polar angle m/E,
+     * opposite azimuths, the parent's weight, no thinning */
+    double mass = x->cont.t->constants.muon_mass_mev / 1000.;
+    if (!finite(parent->energy_gev) || !finite(parent->weight) || parent->weight < 0. || !(mass > 0.) ||
+        parent->generation == 0xffffffffU || parent->step_id == 0xffffffffffffffffULL) {
+      set_fallback(&f, vi, FR_INVALID_FINAL_STATE);
+      record_fallback(s, &f, STAGE_FINAL);
+      return;
+    }
+    double su = uniform(x->seed, x->shower, parent, P_PHOTON_MU_PAIR, D_MU_PAIR_SPLIT);
+    tq_t split = dq_loss_fraction(&x->cont, 22, P_PHOTON_MU_PAIR_FS, vi->component_hash,
+                                  parent->energy_gev * 1000., su);
+    if (split.status != TS_SUCCESS) {
+      set_fallback(&f, vi, FR_INVALID_FINAL_STATE);
+      record_fallback(s, &f, STAGE_FINAL);
+      return;
+    }
+    double minus_total = (1. - split.value) * parent->energy_gev;
+    double plus_total = split.value * parent->energy_gev;
+    if (!finite(split.value) || !(minus_total > 0.) || !(plus_total > 0.)) {
+      set_fallback(&f, vi, FR_INVALID_FINAL_STATE);
+      record_fallback(s, &f, STAGE_FINAL);
+      return;
+    }
+    double az = uniform(x->seed, x->shower, parent, P_PHOTON_MU_PAIR, D_MU_PAIR_AZIMUTH) * PHOTON_TWO_PI;
+    double k = (parent->energy_gev * 1000.) / (mass * 1000.);
+    double cosine = cos(1. / k);
+    if (cosine == -1.) cosine *= -1.;
+    ps_t mu = *parent, mp = *parent;
+    mu.pid = 13;
+    mp.pid = -13;
+    mu.energy_gev = secondary_transport_energy(13, minus_total, mass);
+    mp.energy_gev = secondary_transport_energy(-13, plus_total, mass);
+    mu.parent_history_id = mp.parent_history_id = parent->history_id;
+    mu.generation = mp.generation = parent->generation + 1;
+    mu.step_id = mp.step_id = 0;
+    mu.reserved = mu.energy_gev <= transport_mass_gev(13) ? NATIVE_MUON_PAIR_CUT_FLAG : 0U;
+    mp.reserved = mp.energy_gev <= transport_mass_gev(-13) ? NATIVE_MUON_PAIR_CUT_FLAG : 0U;
+    mu.weight = mp.weight = parent->weight;
+    deflect(parent->direction, cosine, az, mu.direction, 0);
+    deflect(parent->direction, cosine, fmod(az + PHOTON_PI, PHOTON_TWO_PI), mp.direction, 0);
+    {
+      or_vertex* lv = log_vertex(s, 1, parent);
+      if (lv) {
+        lv->process = vi->process_id;
+        lv->target_mass = mass;
+        lv->v = split.value;
+        log_child(lv, 13, minus_total, mu.direction);
+        log_child(lv, -13, plus_total, mp.direction);
+      }
+    }
+    capture_first(s, parent, P_PHOTON_MU_PAIR, &mu, &mp);
+    mu.history_id = first_history + child_offset;
+    mp.history_id = first_history + child_offset + 1;
+    o->charged[o->charged_count++] = mu;
+    o->charged[o->charged_count++] = mp;
+    o->child_count = 2;
+    s->stats.secondaries += 2;
+    ledger_add(&s->prof, cfg, OR_EMC_MASS_CORRECTION, mass_correction(13, mass, 2. * parent->weight));
+    thinning_ledger(s, 0, 2, 2, 0., 0.);
     return;
   }
   const c8mi_photon_pair_lpm* lpm = &cfg->photon_pair_lpm;
@@ -2690,6 +2772,7 @@ typedef struct {
   int observed_before_cut;
   double traversed;   /* chord grammage of the step (g/cm^2) */
   int zero_length;    /* particle cut at the start: no scattering stage */
+  int native_cut;     /* native_muon_pair_cut_record: signed deposit, no track */
 } ltrans_t;
 
 /* prepare_lepton_continuous_step + transport_lepton (LeptonContinuousStep.inl:43-120,
@@ -2700,7 +2783,7 @@ static void transport_lepton(const ctx_t* x, const inter_t* in, ltrans_t* out, f
   const cont_t* c = &x->cont;
   ps_t start = in->particle;
   memset(out, 0, sizeof *out);
-  if (!is_ee(start.pid)) {
+  if (!is_charged_lepton(start.pid)) {
     set_fallback(fb, in, FR_UNSUPPORTED_PARTICLE);
     return;
   }
@@ -2712,6 +2795,18 @@ static void transport_lepton(const ctx_t* x, const inter_t* in, ltrans_t* out, f
   if ((in->status == IS_SELECTED || in->status == IS_DISTANCE_SAMPLED) &&
       (!finite(in->interaction_grammage) || in->interaction_grammage < 0.)) {
     set_fallback(fb, in, FR_INVALID_FINAL_STATE);
+    return;
+  }
+  if (in->status == IS_PARTICLE_CUT && native_muon_pair_cut(&start)) {
+    /* native_muon_pair_cut_record (NativeMuonPairCutRecord.inl:12-25) */
+    out->in = *in;
+    out->in.vertex_reached = 0;
+    out->start = out->end = start;
+    out->end.step_id++;
+    out->limit = LL_CUT;
+    out->cut_deposit = start.energy_gev - transport_mass_gev(start.pid);
+    out->zero_length = 1;
+    out->native_cut = 1;
     return;
   }
   /* prepare_lepton_continuous_step */
@@ -2733,6 +2828,12 @@ static void transport_lepton(const ctx_t* x, const inter_t* in, ltrans_t* out, f
   tq_t tcut = cq_transport_cut(c, start.pid);
   if (tcut.status != TS_SUCCESS || !finite(tcut.value) || !(tcut.value > 0.)) {
     set_fallback(fb, in, FR_INVALID_TABLE_QUERY);
+    return;
+  }
+  /* muons with continuous tables stay with the host: no muon decay or muon
+   * scattering in this restatement */
+  if (!is_ee(start.pid)) {
+    set_fallback(fb, in, FR_UNSUPPORTED_PARTICLE);
     return;
   }
   int particle_cut = in->status == IS_PARTICLE_CUT || e0 - mass.value < tcut.value;
@@ -3048,11 +3149,22 @@ static void lepton_source(shower_t* s, const ps_t* p, lout_t* o, uint64_t child_
     if (tr.limit == LL_INTERACTION) tr.in.particle = tr.end;
   }
   s->stats.lepton_steps++;
+  double w = tr.start.weight;
+  if (tr.native_cut) {
+    /* the native cut's profile step (ProfileAccumulation.inl:300-345): no
+     * track, the signed deposit to the muon energy loss, the rest mass */
+    double deposit = tr.cut_deposit * w;
+    if (deposit < 0. && cfg->profile_bins != 0) s->prof.counters[OR_EMC_DEPOSITED_STEPS]++;
+    ledger_add(&s->prof, cfg, OR_EMC_NATIVE_MUON_CUT, deposit);
+    s->prof.counters[OR_EMC_STEPS]++;
+    s->prof.counters[OR_EMC_LEPTON_LIMIT0 + LL_CUT]++;
+    ledger_add(&s->prof, cfg, OR_EMC_CUT_REST_MASS, transport_mass_gev(tr.start.pid) * w);
+    return;
+  }
   /* the front's step record feeds the radio accumulator
    * (LeptonCascade.inl:81-82) */
   record_track(s, &tr.start, &tr.end);
   /* accumulate_lepton_profile_step (ProfileAccumulation.inl:300-375) */
-  double w = tr.start.weight;
   double gs = project_grammage(cfg, tr.start.position_m);
   double ge = project_grammage(cfg, tr.end.position_m);
   particle_profile(&s->prof, cfg, tr.start.pid, gs, ge, w);

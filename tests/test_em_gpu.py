@@ -126,6 +126,27 @@ def test_inclined_shower_reaches_ground_and_matches(tables):
     assert got["stats"]["observations"] > 0
 
 
+@pytest.mark.parametrize("seed", [2, 5, 8])
+def test_muon_pair_shower_matches_oracle(seed):
+    """Photon showers with the mu-pair columns (at 1e4 times their rate): the
+    device's mu-pair final states (PhotonFinalStateStep.inl:326-382, 679-731)
+    put both muons on the lepton queue; each is cut at its first step (native
+    cut, its signed energy in the ledger) or handed to the host as a fallback
+    (no muon tables here), as in the oracle."""
+    mu_tables = airtables.build_air_tables(muon_pair=1.0e4)
+    cfg, want, got = run_pair(mu_tables, 22, 100.0, seed, moliere=True, moliere_table=True)
+    compare(cfg, want, got, 100.0)
+    fb = got["fallbacks"]
+    muons = fb[np.abs(fb["particle"]["pid"]) == 13]
+    assert muons.size >= 2 and np.all(muons["reason"] == 3)
+    wf = want["fallbacks"]
+    wm = wf[np.abs(wf["particle"]["pid"]) == 13]
+    key = lambda a: np.lexsort((a["particle"]["pid"], a["particle"]["history_id"]))
+    muons, wm = muons[key(muons)], wm[key(wm)]
+    assert np.array_equal(muons["particle"]["history_id"], wm["particle"]["history_id"])
+    np.testing.assert_allclose(muons["particle"]["energy_gev"], wm["particle"]["energy_gev"], rtol=1e-9)
+
+
 @pytest.mark.parametrize("pid,energy,seed,start,fast", [(11, 5.0, 13, 1000.0, 0), (22, 10.0, 14, 112750.0, 0),
                                                         (-11, 3.0, 15, 4000.0, 0), (11, 5.0, 13, 1000.0, 1),
                                                         (22, 30.0, 17, 112750.0, 2)])
