@@ -325,6 +325,94 @@ COUNTER_INDEX.update(steps=13, photon_cuts=14, lepton_limit0=15, thin_hillas=23,
 COUNTERS = 64
 
 
+def ledger_struct(ledger: dict) -> "Ledger":
+    """A Ledger struct from a ledger dict (download_profile, decode_counters)."""
+    out = Ledger()
+    for k, _ in Ledger._fields_:
+        v = ledger.get(k, 0)
+        if isinstance(v, (list, tuple, np.ndarray)):
+            arr = getattr(out, k)
+            for i, x in enumerate(v):
+                arr[i] = int(x)
+        else:
+            setattr(out, k, v)
+    return out
+
+
+class OutputStatistics(ctypes.Structure):
+    _fields_ = [("steps", c_uint64), ("deposited_steps", c_uint64),
+                ("weighted_deposited_energy_gev", c_double),
+                ("weighted_native_muon_pair_cut_energy_gev", c_double),
+                ("weighted_muon_parent_productions", c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+LONGITUDINAL_COLUMNS = ("photon", "electron", "positron", "mu_minus", "mu_plus")
+
+
+def output_profile(hist: dict, ledger: dict) -> dict:
+    """CorsikaOutputSink::onGpuProfile (c8mi_em_output_profile): the checked
+    profile in the writers' layout: longitudinal [bins][5] (LONGITUDINAL_COLUMNS),
+    EM and muon energy loss per bin (GeV) and the sink's statistics."""
+    arrays = [np.ascontiguousarray(hist[k], dtype=np.float64)
+              for k in ("photons", "electrons", "positrons", "energy_loss")]
+    bins = arrays[0].size
+    longitudinal = np.zeros((bins, len(LONGITUDINAL_COLUMNS)))
+    em_loss, mu_loss = np.zeros(bins), np.zeros(bins)
+    st = OutputStatistics()
+    led = ledger_struct(ledger)
+    dp = lambda a: a.ctypes.data_as(POINTER(c_double))
+    _check(library().c8mi_em_output_profile(*[dp(a) for a in arrays], ctypes.c_uint32(bins), byref(led),
+                                            dp(longitudinal), dp(em_loss), dp(mu_loss), byref(st)))
+    return dict(longitudinal=longitudinal, energy_loss_em=em_loss, energy_loss_muon=mu_loss,
+                statistics=st.as_dict())
+
+
+class ReportContext(ctypes.Structure):
+    _fields_ = [("initial_total_gev", c_double), ("thinning_fraction", c_double),
+                ("acceptance_tolerance", c_double), ("electromagnetic_primary", c_uint32), ("pad_", c_uint32),
+                ("scalar_em_steps", c_uint64), ("particles_returned_for_cpu_fallback", c_uint64),
+                ("particles_returned_for_cpu_memory_spill", c_uint64), ("specified_cpu_final_states", c_uint64)]
+
+
+class EnergyLedger(ctypes.Structure):
+    _fields_ = [(k, c_double) for k in (
+        "initial_total_gev", "medium_rest_mass_input_gev", "deposited_gev", "cut_rest_mass_energy_gev",
+        "observed_total_energy_gev", "escaped_total_energy_gev", "unwritten_photoelectric_binding_energy_gev",
+        "observation_cut_overlap_energy_gev", "mass_convention_correction_gev", "source_gev", "terminal_gev",
+        "residual_gev", "relative_closure_error", "acceptance_tolerance", "native_muon_pair_cut_energy_gev",
+        "fallback_energy_gev", "pending_energy_gev", "thinning_discarded_energy_gev", "extended_residual_gev",
+        "injected_gev")] + [("scalar_em_steps", c_uint64), ("fallbacks", c_uint64),
+                            ("complete_coverage", c_uint32), ("accepted", c_uint32)]
+
+
+ENERGY_LEDGER_DEFINITION = ("initial_total + medium_electron_rest + mass_convention_correction = "
+                            "deposit + cut_rest + observed_total + escaped_total + "
+                            "unwritten_photoelectric_binding - observation_cut_overlap")
+
+
+def energy_ledger_report(ledgers, initial_total_gev, electromagnetic_primary=True, thinning_fraction=0.0,
+                         scalar_em_steps=0, cpu_fallbacks=0, memory_spills=0, specified_final_states=0,
+                         tolerance=0.0) -> dict:
+    """The energy_ledger block of the reference's shower report
+    (c8mi_em_energy_ledger_report) over the ledgers of a shower's sides, with
+    the report's key names (ShowerReport.inl:1254-1276) plus the terms a
+    complete-coverage shower never has."""
+    ledgers = [ledgers] if isinstance(ledgers, dict) else list(ledgers)
+    arr = (Ledger * max(len(ledgers), 1))(*[ledger_struct(l) for l in ledgers])
+    ctx = ReportContext(initial_total_gev, thinning_fraction, tolerance, int(bool(electromagnetic_primary)), 0,
+                        scalar_em_steps, cpu_fallbacks, memory_spills, specified_final_states)
+    out = EnergyLedger()
+    _check(library().c8mi_em_energy_ledger_report(arr, c_uint64(len(ledgers)), byref(ctx), byref(out)))
+    r = {k: getattr(out, k) for k, _ in out._fields_}
+    r["complete_coverage"] = bool(r["complete_coverage"])
+    r["accepted"] = bool(r["accepted"])
+    r["definition"] = ENERGY_LEDGER_DEFINITION
+    return r
+
+
 def ledger_residual(ledger: dict) -> float:
     """Energy closure of a shower (GeV): injected minus every sink, with the
     native/transport mass convention, atomic-electron rest mass, photoelectric

@@ -124,6 +124,14 @@ def test_hybrid_shower_equals_oracle(tables, pid, energy, seed, minimum, multipl
     moved = got["device"][1]["fallback_energy"] if r["fallbacks_kept"] == 0 else 0.0
     total = got["host"][1]["injected_energy"] + got["device"][1]["injected_energy"] - moved
     assert abs(total - energy) <= 1e-9 * energy
+    # the shower report over both sides (ShowerReport's energy_ledger block):
+    # host EM steps make the coverage incomplete by the reference's rule; the
+    # closure with every sink booked holds unless a Molière step failed
+    report = em.energy_ledger_report([got["host"][1], got["device"][1]], energy,
+                                     scalar_em_steps=r["cpu_photon_steps"] + r["cpu_lepton_steps"])
+    assert not report["complete_coverage"]
+    if got["host"][1]["fallback_reasons"][21] == 0 and got["device"][1]["fallback_reasons"][21] == 0:
+        assert abs(report["extended_residual_gev"] + moved) <= 1e-9 * energy
 
 
 def test_hybrid_radio_combines_host_and_device_tracks(tables):
