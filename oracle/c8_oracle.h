@@ -203,7 +203,15 @@ typedef struct or_vertex {
   int32_t pad2;
   double child_energy[3];        /* total energies, GeV */
   double child_direction[3][3];
+  uint32_t thin_status;          /* two-child thinning of the vertex (kind 1): status, */
+  uint32_t keep;                 /* keep mask, */
+  double parent_weight;          /* the parent's weight and */
+  double child_weight[2];        /* the two children's weights */
 } or_vertex;
+
+/* project_profile_grammage (ProfileProjection.inl:12-29) of a point on the
+ * configuration's shower axis table. */
+double oracle_em_project_grammage(const c8mi_em_config* config, const double position[3]);
 
 /* oracle_em_run_shower with the vertex log, released with oracle_free. */
 int oracle_em_run_shower_vertices(const c8mi_em_config* config, const c8mi_native_table_set* tables,

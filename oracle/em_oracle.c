@@ -1359,6 +1359,8 @@ static double project_grammage(const c8mi_em_config* c, const double p[3]) {
   return c->axis_grammage_g_per_cm2[upper] * f + c->axis_grammage_g_per_cm2[lower] * (1. - f);
 }
 
+double oracle_em_project_grammage(const c8mi_em_config* c, const double p[3]) { return project_grammage(c, p); }
+
 /* longitudinal_profile_bins (LongitudinalCrossings.inl:12-57) for the Both and
  * Forward modes */
 static void crossing_bins(double start, double end, size_t count, uint32_t mode, size_t* b,
@@ -1934,6 +1936,16 @@ static void log_child(or_vertex* v, int32_t pid, double energy, const double dir
   v->count++;
 }
 
+static void log_thinning(or_vertex* v, uint32_t status, uint32_t keep, double parent_weight, double w1,
+                         double w2) {
+  if (!v) return;
+  v->thin_status = status;
+  v->keep = keep;
+  v->parent_weight = parent_weight;
+  v->child_weight[0] = w1;
+  v->child_weight[1] = w2;
+}
+
 /* The first final state of a generation-zero parent (PhotonFront.inl:95-111,
  * LeptonFinalStateStep.inl:609-615): children before thinning, with the
  * parent's weight and history ID; the first candidate of the shower wins. */
@@ -2168,6 +2180,7 @@ static void photon_source(shower_t* s, const ps_t* p, pout_t* o, uint64_t child_
         log_child(lv, 13, minus_total, mu.direction);
         log_child(lv, -13, plus_total, mp.direction);
       }
+      log_thinning(lv, 0U, 3U, parent->weight, parent->weight, parent->weight);
     }
     capture_first(s, parent, P_PHOTON_MU_PAIR, &mu, &mp);
     mu.history_id = first_history + child_offset;
@@ -2226,6 +2239,7 @@ static void photon_source(shower_t* s, const ps_t* p, pout_t* o, uint64_t child_
         log_child(lv, 22, e1, ph.direction);
         log_child(lv, 11, ELECTRON_MASS_GEV + parent->energy_gev * v, el.direction);
       }
+      log_thinning(lv, (uint32_t)th.status, th.keep, parent->weight, th.w1, th.w2);
     }
     capture_first(s, parent, P_COMPTON, &ph, &el);
     uint32_t k = 0;
@@ -2356,6 +2370,7 @@ static void photon_source(shower_t* s, const ps_t* p, pout_t* o, uint64_t child_
       log_child(lv, 11, ee, el.direction);
       log_child(lv, -11, pe, po.direction);
     }
+    log_thinning(lv, (uint32_t)th.status, th.keep, parent->weight, th.w1, th.w2);
   }
   capture_first(s, parent, P_PHOTON_PAIR, &el, &po);
   uint32_t k = 0;
@@ -3335,6 +3350,7 @@ static void lepton_source(shower_t* s, const ps_t* p, lout_t* o, uint64_t child_
         log_child(lv, 22, e1, g1.direction);
         log_child(lv, 22, e2, g2.direction);
       }
+      log_thinning(lv, (uint32_t)th.status, th.keep, parent->weight, th.w1, th.w2);
     }
     capture_first(s, parent, P_ANNIHILATION, &g1, &g2);
     uint32_t k = 0;
@@ -3409,6 +3425,7 @@ static void lepton_source(shower_t* s, const ps_t* p, lout_t* o, uint64_t child_
         log_child(lv, out.pid, oe, out.direction);
         log_child(lv, 11, de, dl.direction);
       }
+      log_thinning(lv, (uint32_t)th.status, th.keep, parent->weight, th.w1, th.w2);
     }
     capture_first(s, parent, P_IONIZATION, &out, &dl);
     uint32_t k = 0;
@@ -3505,6 +3522,7 @@ static void lepton_source(shower_t* s, const ps_t* p, lout_t* o, uint64_t child_
       log_child(lv, parent->pid, le, lep.direction);
       log_child(lv, 22, pe, pdir);
     }
+    log_thinning(lv, (uint32_t)th.status, th.keep, parent->weight, th.w1, th.w2);
   }
   capture_first(s, parent, P_BREMS, &lep, &ph);
   uint32_t k = 0;

@@ -328,7 +328,8 @@ VERTEX_DTYPE = np.dtype([
     ("kind", "<i4"), ("pid", "<i4"), ("process", "<i4"), ("accepted", "<u4"), ("count", "<u4"),
     ("pad", "<u4"), ("energy", "<f8"), ("mass", "<f8"), ("target_mass", "<f8"), ("direction", "<f8", 3),
     ("v", "<f8"), ("rate_start", "<f8"), ("rate_vertex", "<f8"), ("process_uniform", "<f8"),
-    ("child_pid", "<i4", 3), ("pad2", "<i4"), ("child_energy", "<f8", 3), ("child_direction", "<f8", (3, 3))])
+    ("child_pid", "<i4", 3), ("pad2", "<i4"), ("child_energy", "<f8", 3), ("child_direction", "<f8", (3, 3)),
+    ("thin_status", "<u4"), ("keep", "<u4"), ("parent_weight", "<f8"), ("child_weight", "<f8", 2)])
 
 SELECTION_DTYPE = np.dtype([
     ("status", "<i4"), ("process", "<i4"), ("component", "<u8"), ("interaction_grammage", "<f8"),
