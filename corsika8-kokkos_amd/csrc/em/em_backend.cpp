@@ -789,6 +789,8 @@ bool EmBackend::front(bool photon, c8mi_em_run_result& out) {
   f.moliere_open_count = moliere_open_count_.data();
   f.open_vertices = open_vertices_.data();
   f.open_vertex_count = open_vertex_count_.data();
+  f.first_interactions = first_interactions_.data();
+  f.first_candidates = first_candidates_.data();
   int const track_buffer = track_parity_;
   if ((radio_ || slot_radio_) && !photon) {
     // tracks [9][n]; the buffer's previous projections must be done first

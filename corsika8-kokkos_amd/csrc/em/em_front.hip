@@ -599,7 +599,12 @@ __global__ __launch_bounds__(kStepBlock) void scatter_kernel(FrontArgs f, Partic
     std::uint32_t const kind = (word >> (6 + 2 * k)) & 3u;
     if (kind == kSlotNone) continue;
     c8mi_particle_state p = f.staging[2 * i + k];
-    if (kind != kSlotNextContinue) p.history_id = child_base + p.history_id;
+    if (kind != kSlotNextContinue) {
+      p.history_id = child_base + p.history_id;
+      // children of generation-zero parents only (a handful per shower)
+      if (p.generation == 1 && f.first_interactions)
+        phys::patch_first_interaction(f.first_interactions[s], f.first_candidates[s], p);
+    }
     if (kind == kSlotOtherChild) {
       store_particle(other_queue, other_pos, p);
       other_queue.shower[other_pos++] = s | (kClassPending << kClassShift);

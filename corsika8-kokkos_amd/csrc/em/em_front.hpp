@@ -66,6 +66,8 @@ struct FrontArgs {
   unsigned long long* moliere_open_count;  // zeroed by the move kernel
   VertexRecord* open_vertices;        // vertices of the sources finished after the tail kernel
   unsigned long long* open_vertex_count;  // zero between fronts
+  c8mi_em_first_interaction* first_interactions;  // per slot; the scatter gives a photoelectric
+  unsigned long long const* first_candidates;     // snapshot's child its assigned history ID
 };
 
 // States of a source between the split lepton kernels (FrontArgs::moved).

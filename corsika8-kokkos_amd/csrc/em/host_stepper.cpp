@@ -133,6 +133,7 @@ HostStep HostStepper::step(c8mi_particle_state const& particle, std::uint64_t fi
     q.pad_ = 0;
     q.pad2_ = 0;
     r.out[r.count++] = q;
+    phys::patch_first_interaction(first_interaction_, first_candidates_, q);
   }
   return r;
 }

@@ -274,6 +274,19 @@ C8MI_HD void record_fallback(Accumulator const& a, Particle const& p, int reason
   }
 }
 
+// The snapshot's photoelectric electron carries the history ID it is assigned
+// (PhotonFinalStateStep.inl:649-660 sets it before the snapshot; every other
+// process snapshots its children before their IDs): once the child's ID is
+// known, the one child of the captured photoelectric vertex takes it.
+C8MI_HD void patch_first_interaction(c8mi_em_first_interaction& r, unsigned long long candidates,
+                                     c8mi_particle_state const& child) {
+  if (candidates == 0 || child.generation != 1 || child.pid != 11 || r.process_id != kPhotoelectric ||
+      r.secondary_count != 1 || child.parent_history_id != r.parent_at_vertex.history_id ||
+      r.secondaries[0].history_id != r.parent_at_vertex.history_id)
+    return;
+  r.secondaries[0].history_id = child.history_id;
+}
+
 // The writers' first-interaction snapshot (PhotonFront.inl:95-111,
 // LeptonFinalStateStep.inl:609-615): the first final state of a
 // generation-zero parent, its children before thinning with the parent's

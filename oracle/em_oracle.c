@@ -1956,7 +1956,9 @@ static void capture_first(shower_t* s, const ps_t* parent, int32_t process, cons
   s->first.parent_at_vertex = *parent;
   s->first.secondaries[0] = *a;
   s->first.secondaries[0].weight = parent->weight;
-  s->first.secondaries[0].history_id = parent->history_id;
+  /* the photoelectric electron is snapshot with its assigned history ID
+   * (PhotonFinalStateStep.inl:649-660); other children before their IDs */
+  if (process != P_PHOTOELECTRIC) s->first.secondaries[0].history_id = parent->history_id;
   if (b) {
     s->first.secondaries[1] = *b;
     s->first.secondaries[1].weight = parent->weight;

@@ -83,3 +83,31 @@ def test_pool_slots_keep_their_own_snapshots(tables):
         same_snapshot(dev.slot_first_interaction(slot), want)
     assert dev.slot_first_interaction(3) is None
     dev.close()
+
+
+def test_photoelectric_snapshot_child_takes_its_history_id():
+    """A photon whose first final state is photoelectric: the snapshot's electron
+    carries the history ID the front assigns it (PhotonFinalStateStep.inl:649-660),
+    patched in by the scatter once the ID is known; equal to the oracle's."""
+    tables = airtables.build_air_tables(cut_mev=0.05)
+    atm = em.Atmosphere()
+    prim = em.particle(22, 6e-5, (0.0, 0.0, 20000.0), DOWN, history_id=1)
+    found = 0
+    for seed in range(1, 200):
+        cfg = em.make_config(airtables, seed=seed, atmosphere=atm, cut_mev=0.05)
+        want, _ = oracle_lib.em_first_interaction(cfg, tables, atm, prim, shower_id=seed)
+        if want is None or want["process_id"] != 1000000016:
+            continue
+        print(f"photoelectric snapshot: seed {seed}", flush=True)
+        dev = em.EmBackend(cfg, tables, atm)
+        dev.begin_shower(seed, 2)
+        dev.enqueue(prim)
+        dev.run()
+        got = dev.first_interaction()
+        dev.close()
+        same_snapshot(got, want)
+        assert got["secondaries"][0]["history_id"] == 2  # the first ID of the call
+        found += 1
+        if found == 2:
+            break
+    assert found == 2

@@ -209,3 +209,45 @@ def test_fallback_round_trip_returns_moliere_failures(air):
             returned += 1
             closed += abs(residual) <= 1e-9 * 100.0
     assert returned >= 4 and closed >= returned // 2
+
+
+def photoelectric_seeds(cfg_of, tables, atm, prim, count=2):
+    """Seeds whose first final state (oracle) is photoelectric."""
+    seeds = []
+    for seed in range(1, 200):
+        want, _ = oracle_lib.em_first_interaction(cfg_of(seed), tables, atm, prim, shower_id=seed)
+        if want is not None and want["process_id"] == 1000000016:
+            seeds.append(seed)
+            if len(seeds) == count:
+                break
+    return seeds
+
+
+def test_photoelectric_snapshot_carries_the_assigned_history_id():
+    """PhotonFinalStateStep.inl:649-660 gives the photoelectric electron its
+    history ID before the snapshot (every other process snapshots its children
+    before their IDs): the host stepper's snapshot equals the oracle's."""
+    tables = airtables.build_air_tables(cut_mev=0.05)
+    atm = em.Atmosphere()
+    cfg_of = lambda seed: em.make_config(airtables, seed=seed, atmosphere=atm, cut_mev=0.05)
+    prim = em.particle(22, 6e-5, (0.0, 0.0, 20000.0), DOWN, history_id=1)
+    seeds = photoelectric_seeds(cfg_of, tables, atm, prim)
+    assert len(seeds) == 2
+    for seed in seeds:
+        cfg = cfg_of(seed)
+        want, _ = oracle_lib.em_first_interaction(cfg, tables, atm, prim, shower_id=seed)
+        stepper = em.ScalarStepper(cfg, tables, atm)
+        stepper.begin_shower(seed)
+        counter, stack = 40, [prim]
+        while stack:
+            out, children, _ = stepper.step(stack.pop(), counter)
+            counter += children
+            stack.extend(out)
+        first = stepper.first_interaction()
+        stepper.close()
+        assert first["process_id"] == 1000000016 and first["secondary_count"] == 1
+        child = first["secondaries"][0]
+        assert child["history_id"] == 40 != first["parent_at_vertex"]["history_id"]
+        # the oracle's shower gives the photon's first child the first ID of its call
+        assert want["secondaries"][0]["history_id"] != want["parent_at_vertex"]["history_id"]
+        same_particles(first["parent_at_vertex"].reshape(1), want["parent_at_vertex"].reshape(1))
