@@ -127,9 +127,12 @@ def test_hybrid_shower_equals_oracle(tables, pid, energy, seed, minimum, multipl
     # the shower report over both sides (ShowerReport's energy_ledger block):
     # host EM steps make the coverage incomplete by the reference's rule; the
     # closure with every sink booked holds unless a Molière step failed
-    report = em.energy_ledger_report([got["host"][1], got["device"][1]], energy,
-                                     scalar_em_steps=r["cpu_photon_steps"] + r["cpu_lepton_steps"])
-    assert not report["complete_coverage"]
+    host_steps = r["cpu_photon_steps"] + r["cpu_lepton_steps"]
+    report = em.energy_ledger_report([got["host"][1], got["device"][1]], energy, scalar_em_steps=host_steps)
+    fallbacks = sum(got["host"][1]["fallback_reasons"]) + sum(got["device"][1]["fallback_reasons"])
+    assert report["complete_coverage"] == (host_steps == 0 and fallbacks == 0)
+    if report["complete_coverage"]:
+        assert report["accepted"] and report["relative_closure_error"] <= 1e-9
     if got["host"][1]["fallback_reasons"][21] == 0 and got["device"][1]["fallback_reasons"][21] == 0:
         assert abs(report["extended_residual_gev"] + moved) <= 1e-9 * energy
 
