@@ -934,10 +934,7 @@ class EmBackend:
         results = (RequestResult * n)()
         totals = RunResult()
         _check(library().c8mi_em_serve(self.handle, arr, c_uint64(n), byref(opts), results, byref(totals)))
-        out = [results[k].as_dict() for k in range(n)]
-        for k, waves in radio_out.items():
-            out[k]["radio"] = waves
-        return out, totals.as_dict()
+        return _ResultList(results, n, radio_out), totals.as_dict()
 
     def enable_slot_radio(self, observers_per_slot, bins_per_observer, propagation, coreas=True, zhs=True,
                           exact=False, fixed_point_limit=1.0, zhs_refinement=1):
@@ -1164,7 +1161,8 @@ def hybrid_prefixes(config, tables, environment, primaries, shower_ids, first_hi
         raise ValueError("one shower ID per primary")
     n = prim.size
     capacity = capacity if capacity is not None else 512 * max(n, 1)
-    staged = np.zeros(capacity, dtype=PARTICLE_STATE_DTYPE)
+    # written only where batches land (the library fills [0, offsets[n]))
+    staged = np.empty(capacity, dtype=PARTICLE_STATE_DTYPE)
     offsets = np.zeros(n + 1, dtype=np.uint64)
     nxt = np.zeros(n, dtype=np.uint64)
     ledgers = (Ledger * max(n, 1))()
@@ -1177,5 +1175,56 @@ def hybrid_prefixes(config, tables, environment, primaries, shower_ids, first_hi
         nxt.ctypes.data_as(c_void_p), ledgers, hist.ctypes.data_as(c_void_p) if hist is not None else None,
         byref(totals)))
     batches = [staged[int(offsets[k]):int(offsets[k + 1])] for k in range(n)]
-    out = (batches, [int(x) for x in nxt], [ledgers[k].as_dict() for k in range(n)], totals.as_dict())
+    out = (batches, [int(x) for x in nxt], _LedgerList(ledgers, n), totals.as_dict())
     return out + (hist,) if histograms else out
+
+
+class _ResultList:
+    """The per-request results of serve: each turned into a dict when first
+    read (and kept), with its radio waveforms when it had observers."""
+
+    def __init__(self, array, count, radio):
+        self._array, self._count, self._radio = array, count, radio
+        self._dicts = [None] * count
+
+    def __len__(self):
+        return self._count
+
+    def __getitem__(self, k):
+        if isinstance(k, slice):
+            return [self[i] for i in range(*k.indices(self._count))]
+        if k < 0:
+            k += self._count
+        if not 0 <= k < self._count:
+            raise IndexError(k)
+        if self._dicts[k] is None:
+            d = self._array[k].as_dict()
+            if k in self._radio:
+                d["radio"] = self._radio[k]
+            self._dicts[k] = d
+        return self._dicts[k]
+
+    def __iter__(self):
+        return (self[k] for k in range(self._count))
+
+
+class _LedgerList:
+    """The host ledgers of hybrid_prefixes, each turned into a dict when read."""
+
+    def __init__(self, array, count):
+        self._array, self._count = array, count
+
+    def __len__(self):
+        return self._count
+
+    def __getitem__(self, k):
+        if isinstance(k, slice):
+            return [self[i] for i in range(*k.indices(self._count))]
+        if k < 0:
+            k += self._count
+        if not 0 <= k < self._count:
+            raise IndexError(k)
+        return self._array[k].as_dict()
+
+    def __iter__(self):
+        return (self[k] for k in range(self._count))

@@ -195,12 +195,15 @@ int c8mi_em_hybrid_prefixes(c8mi_em_config const* config, c8mi_native_table_set 
     std::vector<std::vector<c8mi_particle_state>> batches(count);
     std::vector<c8mi_em_hybrid_result> results(count);
     std::vector<std::exception_ptr> errors(workers);
-    // one stepper per worker, built here so that argument errors surface at once
-    std::vector<std::unique_ptr<c8mi::em::HostStepper>> steppers;
-    for (std::uint32_t w = 0; w < workers; ++w)
-      steppers.push_back(std::make_unique<c8mi::em::HostStepper>(*config, *tables, *environment));
+    // one stepper per worker over one shared host copy of the tables; the
+    // first is built here so that argument errors surface at once, the others
+    // by their workers
+    auto const host_tables = std::make_shared<c8mi::em::HostTables const>(*tables);
+    std::vector<std::unique_ptr<c8mi::em::HostStepper>> steppers(workers);
+    steppers[0] = std::make_unique<c8mi::em::HostStepper>(*config, host_tables, *environment);
     auto work = [&](std::uint32_t w) {
       try {
+        if (!steppers[w]) steppers[w] = std::make_unique<c8mi::em::HostStepper>(*config, host_tables, *environment);
         c8mi::em::HostStepper& stepper = *steppers[w];
         c8mi::scheduling::HybridCascade cascade(nullptr, stepper, hybrid);
         std::vector<double> tracks;

@@ -20,7 +20,12 @@ std::vector<T> copy_of(T const* host, std::size_t count) {
 
 HostStepper::HostStepper(c8mi_em_config const& config, c8mi_native_table_set const& tables,
                          c8mi_environment const& environment)
-    : config_(config), tables_(tables) {
+    : HostStepper(config, std::make_shared<HostTables const>(tables), environment) {}
+
+HostStepper::HostStepper(c8mi_em_config const& config, std::shared_ptr<HostTables const> tables,
+                         c8mi_environment const& environment)
+    : config_(config), tables_(std::move(tables)) {
+  if (!tables_) throw std::invalid_argument("em: null host tables");
   validate_config(config);
   validate_environment(environment);
   nodes_ = copy_of(environment.nodes, environment.node_count);
@@ -30,12 +35,14 @@ HostStepper::HostStepper(c8mi_em_config const& config, c8mi_native_table_set con
   std::uint64_t const bins = config.profile_bins ? config.profile_bins : 1u;
   counters_.assign(kCounters, 0);
   profile_.assign(4u * bins, 0);
-  observations_.resize(config.observation_capacity);
-  fallbacks_.resize(config.fallback_capacity);
+  // the record arrays grow with use up to the configured capacities (a host
+  // prefix writes a handful; the device-sized capacity would be mostly idle)
+  observations_.resize(std::min<std::uint64_t>(config.observation_capacity, kInitialRecords));
+  fallbacks_.resize(std::min<std::uint64_t>(config.fallback_capacity, kInitialRecords));
 
   EmContext& h = context_;
   fill_context(config, environment, h);
-  h.tables = tables_.view();
+  h.tables = tables_->view();
   h.nodes = nodes_.data();
   h.children = children_.data();
   h.models = models_.data();
@@ -60,6 +67,8 @@ HostStepper::HostStepper(c8mi_em_config const& config, c8mi_native_table_set con
   h.shower_slots = 1;
   h.observations = observations_.data();
   h.fallbacks = fallbacks_.data();
+  h.observation_capacity = observations_.size();
+  h.fallback_capacity = fallbacks_.size();
   h.record_counts = record_counts_;
   h.first_interactions = &first_interaction_;
   h.first_candidates = &first_candidates_;
@@ -89,7 +98,26 @@ void HostStepper::book_injected(double energy_weight) {
   phys::ledger(acc, kInjected, energy_weight);
 }
 
+void HostStepper::reserve_records() {
+  // one step records at most one observation or one fallback: room for two
+  // more keeps every record up to the configured capacity
+  auto grow = [](auto& records, std::uint64_t count, std::uint64_t limit) {
+    if (count + 2 <= records.size() || records.size() >= limit) return false;
+    records.resize(std::min<std::uint64_t>(limit, std::max<std::uint64_t>(2 * records.size(), count + 2)));
+    return true;
+  };
+  if (grow(observations_, record_counts_[0], config_.observation_capacity)) {
+    context_.observations = observations_.data();
+    context_.observation_capacity = observations_.size();
+  }
+  if (grow(fallbacks_, record_counts_[1], config_.fallback_capacity)) {
+    context_.fallbacks = fallbacks_.data();
+    context_.fallback_capacity = fallbacks_.size();
+  }
+}
+
 HostStep HostStepper::step(c8mi_particle_state const& particle, std::uint64_t first_history_id, bool moliere) {
+  reserve_records();
   HostStep r{};
   phys::Accumulator const acc{profile_.data(), counters_.data(), &context_, &shower_, 0};
   phys::Particle p;

@@ -9,6 +9,7 @@
 #pragma once
 
 #include <cstdint>
+#include <memory>
 #include <vector>
 
 #include "c8mi/c8mi_em.h"
@@ -30,6 +31,9 @@ struct HostStep {
 class HostStepper {
  public:
   HostStepper(c8mi_em_config const& config, c8mi_native_table_set const& tables,
+              c8mi_environment const& environment);
+  // Shares a host copy of the tables (read only) with other steppers.
+  HostStepper(c8mi_em_config const& config, std::shared_ptr<HostTables const> tables,
               c8mi_environment const& environment);
   HostStepper(HostStepper const&) = delete;
   HostStepper& operator=(HostStepper const&) = delete;
@@ -61,8 +65,11 @@ class HostStepper {
   c8mi_em_config const& config() const noexcept { return config_; }
 
  private:
+  static constexpr std::uint64_t kInitialRecords = 64;
+  void reserve_records();
+
   c8mi_em_config config_;
-  HostTables tables_;
+  std::shared_ptr<HostTables const> tables_;
   std::vector<c8mi_env_node> nodes_;
   std::vector<std::int32_t> children_;
   std::vector<c8mi_env_model> models_;
