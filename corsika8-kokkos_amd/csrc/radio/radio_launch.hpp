@@ -25,8 +25,11 @@ inline constexpr std::size_t kPreferredWorkgroupLdsBytes = 53 * 1024;
 // Upper bound of the projection kernel's static LDS: the wave totals of the ZHS
 // scan (16 x 8 B for 1024 threads), block counters and |increment| sums.
 inline constexpr std::size_t kProjectionStaticLdsBytes = 256;
-// Fewest tracks per workgroup of the default chunking.
-inline constexpr std::uint64_t kMinimumDefaultChunk = 8192;
+// Tracks per workgroup the default chunking gives a batch large enough for
+// eight such chunks, at least (a smaller batch is cut into eight chunks).
+inline constexpr std::uint64_t kMinimumDefaultChunk = 32768;
+// XCDs of the chip: the projection grid maps chunk c to XCD c % kXcdCount.
+inline constexpr std::uint64_t kXcdCount = 8;
 // Bins of the default tile for `per_bin` int64 words per bin (3 per algorithm);
 // a ZHS tile holds one more word per component row (its difference array).
 inline std::uint64_t default_tile_capacity(std::size_t per_bin, bool zhs) {
