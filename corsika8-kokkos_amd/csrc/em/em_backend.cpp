@@ -748,8 +748,8 @@ bool EmBackend::front(bool photon, c8mi_em_run_result& out) {
     layout.medium_bits = order_.medium_bits();
     layout.bucket_bits = any_bucket ? 18u + layout.medium_bits : 0u;
     layout.slot_bits = bits_for(active_range_);
-    layout.parked_bits = parked ? 1u : 0u;
-    order_.order(cur, alt, n, control_dev_.data(), active_range_, history_.data(), layout, stream_);
+    layout.parked = parked ? 1u : 0u;
+    order_.order(cur, alt, n, m, control_dev_.data(), active_range_, history_.data(), layout, stream_);
     ++totals_.ordered_fronts;
   }
   // ordered: the front reads the gathered buffer and writes its continuing

@@ -1,8 +1,12 @@
-// Front ordering: sort keys, stable radix sort, gather; parked-tail carry.
-// See em_order.hpp for the key layout.
+// Front ordering: parked partition, sort keys, stable radix sort, gather;
+// parked-tail carry. See em_order.hpp for the key layout.
 #include <hip/hip_runtime.h>
 
+#include <rocprim/device/device_partition.hpp>
 #include <rocprim/device/device_radix_sort.hpp>
+#include <rocprim/iterator/counting_iterator.hpp>
+
+#include <algorithm>
 
 #include <cmath>
 #include <cstdint>
@@ -57,13 +61,33 @@ __device__ __forceinline__ std::uint32_t class_rank(std::uint32_t cls) {
   return cls == kClassPending ? 0u : cls == kClassRemaining ? 1u : 2u;
 }
 
-__global__ __launch_bounds__(kOrderBlock) void order_keys_kernel(
-    ParticleQueue in, std::uint64_t n, SlotControl const* __restrict__ control, std::uint32_t slot_count,
-    unsigned long long* __restrict__ history, std::int32_t const* __restrict__ media, std::uint32_t media_count,
-    OrderLayout layout, unsigned long long* __restrict__ keys, std::uint32_t* __restrict__ values) {
+// Applies the kControlOpen history IDs and flags the entries of running
+// showers for the partition.
+__global__ __launch_bounds__(kOrderBlock) void order_flags_kernel(ParticleQueue in, std::uint64_t n,
+                                                                  SlotControl const* __restrict__ control,
+                                                                  std::uint32_t slot_count,
+                                                                  unsigned long long* __restrict__ history,
+                                                                  std::uint8_t* __restrict__ run) {
   std::uint64_t const i = static_cast<std::uint64_t>(blockIdx.x) * kOrderBlock + threadIdx.x;
   if (i < slot_count && (control[i].flags & kControlOpen)) history[i] = control[i].first_history;
   if (i >= n) return;
+  run[i] = (control[in.shower[i] & kSlotBits].flags & kControlRun) ? 1u : 0u;
+}
+
+// Sort keys of `count` entries: entry j is queue entry selected[j] (the
+// partition's participants, whose indices are already in `values`), or j
+// itself when `selected` is null (then `values` gets j, and `history` the
+// kControlOpen IDs).
+template <class Key>
+__global__ __launch_bounds__(kOrderBlock) void order_keys_kernel(
+    ParticleQueue in, std::uint64_t count, std::uint32_t const* __restrict__ selected,
+    SlotControl const* __restrict__ control, std::uint32_t slot_count, unsigned long long* __restrict__ history,
+    std::int32_t const* __restrict__ media, std::uint32_t media_count, OrderLayout layout, Key* __restrict__ keys,
+    std::uint32_t* __restrict__ values) {
+  std::uint64_t const j = static_cast<std::uint64_t>(blockIdx.x) * kOrderBlock + threadIdx.x;
+  if (history && j < slot_count && (control[j].flags & kControlOpen)) history[j] = control[j].first_history;
+  if (j >= count) return;
+  std::uint64_t const i = selected ? selected[j] : j;
   std::uint32_t const word = in.shower[i];
   std::uint32_t const slot = word & kSlotBits;
   std::uint32_t const flags = control[slot].flags;
@@ -84,13 +108,9 @@ __global__ __launch_bounds__(kOrderBlock) void order_keys_kernel(
     }
     shift += layout.bucket_bits;
   }
-  if (layout.slot_bits) {
-    key |= static_cast<unsigned long long>(slot) << shift;
-    shift += layout.slot_bits;
-  }
-  if (layout.parked_bits && !run) key |= 1ull << shift;
-  keys[i] = key;
-  values[i] = static_cast<std::uint32_t>(i);
+  if (layout.slot_bits) key |= static_cast<unsigned long long>(slot) << shift;
+  keys[j] = static_cast<Key>(key);
+  if (!selected) values[j] = static_cast<std::uint32_t>(i);
 }
 
 __global__ __launch_bounds__(kOrderBlock) void order_gather_kernel(ParticleQueue in, ParticleQueue out,
@@ -121,7 +141,8 @@ __global__ __launch_bounds__(kOrderBlock) void carry_kernel(ParticleQueue front,
 
 FrontOrder::~FrontOrder() {
   for (auto* p : {static_cast<void*>(media_), static_cast<void*>(keys_[0]), static_cast<void*>(keys_[1]),
-                  static_cast<void*>(values_[0]), static_cast<void*>(values_[1]), temporary_})
+                  static_cast<void*>(values_[0]), static_cast<void*>(values_[1]), static_cast<void*>(run_flags_),
+                  static_cast<void*>(selected_count_), temporary_})
     if (p) (void)hipFree(p);
 }
 
@@ -140,7 +161,7 @@ void FrontOrder::set_media(std::int32_t const* media, std::uint32_t count, hipSt
   while ((1u << medium_bits_) < count) ++medium_bits_;
 }
 
-void FrontOrder::reserve(std::uint64_t n, std::uint32_t bits, hipStream_t stream) {
+void FrontOrder::reserve(std::uint64_t n, std::uint32_t bits, bool partition, hipStream_t stream) {
   if (n > 0xffffffffULL) throw std::length_error("em: front of 2^32 or more particles");
   if (n > capacity_) {
     C8MI_HIP(hipStreamSynchronize(stream));  // earlier fronts may still read the arrays
@@ -154,12 +175,26 @@ void FrontOrder::reserve(std::uint64_t n, std::uint32_t bits, hipStream_t stream
       C8MI_HIP(hipMalloc(reinterpret_cast<void**>(&keys_[b]), cap * sizeof(unsigned long long)));
       C8MI_HIP(hipMalloc(reinterpret_cast<void**>(&values_[b]), cap * sizeof(std::uint32_t)));
     }
+    if (run_flags_) C8MI_HIP(hipFree(run_flags_));
+    run_flags_ = nullptr;
+    C8MI_HIP(hipMalloc(reinterpret_cast<void**>(&run_flags_), cap));
     capacity_ = cap;
   }
-  if (bits == 0) return;
-  std::size_t need = 0;
-  C8MI_HIP(rocprim::radix_sort_pairs(nullptr, need, keys_[0], keys_[1], values_[0], values_[1],
-                                     static_cast<std::uint32_t>(capacity_), 0u, 64u, stream));
+  if (!selected_count_) C8MI_HIP(hipMalloc(reinterpret_cast<void**>(&selected_count_), sizeof(unsigned int)));
+  std::size_t need = 0, part = 0;
+  auto const items = static_cast<std::uint32_t>(capacity_);
+  if (bits > 32) {
+    C8MI_HIP(rocprim::radix_sort_pairs(nullptr, need, keys_[0], keys_[1], values_[0], values_[1], items, 0u, 64u,
+                                       stream));
+  } else if (bits > 0) {
+    auto* k0 = reinterpret_cast<std::uint32_t*>(keys_[0]);
+    auto* k1 = reinterpret_cast<std::uint32_t*>(keys_[1]);
+    C8MI_HIP(rocprim::radix_sort_pairs(nullptr, need, k0, k1, values_[0], values_[1], items, 0u, 32u, stream));
+  }
+  if (partition)
+    C8MI_HIP(rocprim::partition_two_way(nullptr, part, rocprim::counting_iterator<std::uint32_t>(0u), run_flags_,
+                                        values_[0], values_[1], selected_count_, capacity_, stream));
+  need = std::max(need, part);
   if (need > temporary_bytes_) {
     C8MI_HIP(hipStreamSynchronize(stream));
     if (temporary_) C8MI_HIP(hipFree(temporary_));
@@ -170,21 +205,56 @@ void FrontOrder::reserve(std::uint64_t n, std::uint32_t bits, hipStream_t stream
 }
 
 void FrontOrder::order(ParticleQueue const& in, ParticleQueue const& out, std::uint64_t n,
-                       SlotControl const* control, std::uint32_t slot_count, unsigned long long* history,
-                       OrderLayout const& layout, hipStream_t stream) {
+                       std::uint64_t participants, SlotControl const* control, std::uint32_t slot_count,
+                       unsigned long long* history, OrderLayout const& layout, hipStream_t stream) {
   std::uint32_t const bits = layout.total();
   if (bits > 64) throw std::logic_error("em: front sort key wider than 64 bits");
-  reserve(n, bits, stream);
-  std::uint64_t const threads = n > slot_count ? n : slot_count;
-  order_keys_kernel<<<order_grid(threads), kOrderBlock, 0, stream>>>(in, n, control, slot_count, history, media_,
-                                                                       media_count_, layout, keys_[0], values_[0]);
-  C8MI_HIP(hipGetLastError());
-  if (n == 0) return;
-  std::uint32_t const* order = values_[0];
-  if (bits != 0) {
+  bool const partition = layout.parked != 0;
+  if (participants > n || (!partition && participants != n))
+    throw std::logic_error("em: front participants do not match the queued entries");
+  reserve(n, bits, partition, stream);
+  std::uint64_t const m = partition ? participants : n;
+  bool const narrow = bits <= 32;
+  auto* narrow_keys = reinterpret_cast<std::uint32_t*>(keys_[0]);
+  auto keys = [&](std::uint64_t count, std::uint32_t const* selected, unsigned long long* ids, std::uint64_t threads) {
+    if (threads == 0) return;
+    if (narrow)
+      order_keys_kernel<std::uint32_t><<<order_grid(threads), kOrderBlock, 0, stream>>>(
+          in, count, selected, control, slot_count, ids, media_, media_count_, layout, narrow_keys, values_[0]);
+    else
+      order_keys_kernel<unsigned long long><<<order_grid(threads), kOrderBlock, 0, stream>>>(
+          in, count, selected, control, slot_count, ids, media_, media_count_, layout, keys_[0], values_[0]);
+    C8MI_HIP(hipGetLastError());
+  };
+  if (partition) {
+    // participants' indices to values_[0][0, m), the parked ones' (in queue
+    // order) straight to their final place values_[1][m, n)
+    order_flags_kernel<<<order_grid(std::max<std::uint64_t>(n, slot_count)), kOrderBlock, 0, stream>>>(
+        in, n, control, slot_count, history, run_flags_);
+    C8MI_HIP(hipGetLastError());
+    if (n == 0) return;
     std::size_t bytes = temporary_bytes_;
-    C8MI_HIP(rocprim::radix_sort_pairs(temporary_, bytes, keys_[0], keys_[1], values_[0], values_[1],
-                                       static_cast<std::uint32_t>(n), 0u, bits, stream));
+    C8MI_HIP(rocprim::partition_two_way(temporary_, bytes, rocprim::counting_iterator<std::uint32_t>(0u), run_flags_,
+                                        values_[0], values_[1] + m, selected_count_, n, stream));
+    if (bits != 0) keys(m, values_[0], nullptr, m);
+  } else {
+    keys(n, nullptr, history, std::max<std::uint64_t>(n, slot_count));
+    if (n == 0) return;
+  }
+  std::uint32_t const* order = values_[0];
+  if (bits != 0 && m != 0) {
+    std::size_t bytes = temporary_bytes_;
+    auto const items = static_cast<std::uint32_t>(m);
+    if (narrow)
+      C8MI_HIP(rocprim::radix_sort_pairs(temporary_, bytes, narrow_keys, reinterpret_cast<std::uint32_t*>(keys_[1]),
+                                         values_[0], values_[1], items, 0u, bits, stream));
+    else
+      C8MI_HIP(rocprim::radix_sort_pairs(temporary_, bytes, keys_[0], keys_[1], values_[0], values_[1], items, 0u,
+                                         bits, stream));
+    order = values_[1];
+  } else if (partition) {
+    if (m != 0)
+      C8MI_HIP(hipMemcpyAsync(values_[1], values_[0], m * sizeof(std::uint32_t), hipMemcpyDeviceToDevice, stream));
     order = values_[1];
   }
   order_gather_kernel<<<order_grid(n), kOrderBlock, 0, stream>>>(in, out, order, n, control);

@@ -4,10 +4,11 @@
 // before every front at PhotonCascade.inl:318-332 and LeptonCascade.inl:296-307)
 // extended to a pool of showers and to the order of a call's input.
 //
-// One stable LSD radix sort of (key, queue index) pairs over the species'
-// queue, then a gather into the front buffer. The key, most significant first:
-//   parked (1 bit)  showers whose call does not run this front go last and are
-//                   carried over to the next queue unchanged;
+// A stable two-way partition of the queue indices moves the parked entries
+// (showers whose call does not run this front) behind the participants, in
+// queue order; one stable LSD radix sort of (key, queue index) pairs over the
+// participants follows (32-bit keys when the fields fit, else 64-bit), then a
+// gather into the front buffer. The key, most significant first:
 //   slot            the pool slot: every shower's particles are contiguous;
 //   bucket          wavefront_bucket_key (PID bucket, medium, energy bin) for
 //                   showers whose front holds at least 256 particles, else 0;
@@ -16,7 +17,9 @@
 //                   previous call) before host-staged, the order of the
 //                   reference's call input (WavefrontQueue.inl:207-260).
 // Ties keep the queue order. Field widths are chosen per front by the host;
-// absent fields take no bits.
+// absent fields take no bits. The parked entries are carried over to the next
+// queue unchanged; their order across showers does not matter (a front after a
+// parked one is always ordered again) and within a shower it is kept.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -56,8 +59,8 @@ struct OrderLayout {
   std::uint32_t bucket_bits;  // 16 energy + medium rank bits + 2 PID, when a shower is bucketed
   std::uint32_t medium_bits;
   std::uint32_t slot_bits;
-  std::uint32_t parked_bits;  // 1 when a shower sits this front out
-  std::uint32_t total() const { return class_bits + bucket_bits + slot_bits + parked_bits; }
+  std::uint32_t parked;       // 1 when a shower sits this front out: partition first
+  std::uint32_t total() const { return class_bits + bucket_bits + slot_bits; }
 };
 
 class FrontOrder {
@@ -71,21 +74,24 @@ class FrontOrder {
   void set_media(std::int32_t const* media, std::uint32_t count, hipStream_t stream);
   std::uint32_t medium_bits() const noexcept { return medium_bits_; }
 
-  // Orders queue entries [0, n) of `in` into `out`: participants first (the
-  // host counted `participants`), then the parked entries, which keep their
-  // slot words. Also applies kControlOpen history IDs (slots < slot_count).
-  void order(ParticleQueue const& in, ParticleQueue const& out, std::uint64_t n,
+  // Orders queue entries [0, n) of `in` into `out`: the `participants`
+  // entries of running showers first (the host counts them from the same
+  // control words), then the parked entries, which keep their slot words. Also
+  // applies kControlOpen history IDs (slots < slot_count).
+  void order(ParticleQueue const& in, ParticleQueue const& out, std::uint64_t n, std::uint64_t participants,
              SlotControl const* control, std::uint32_t slot_count, unsigned long long* history,
              OrderLayout const& layout, hipStream_t stream);
 
  private:
-  void reserve(std::uint64_t n, std::uint32_t bits, hipStream_t stream);
+  void reserve(std::uint64_t n, std::uint32_t bits, bool partition, hipStream_t stream);
 
   std::int32_t* media_ = nullptr;
   std::uint32_t media_count_ = 0;
   std::uint32_t medium_bits_ = 0;
-  unsigned long long* keys_[2] = {nullptr, nullptr};
+  unsigned long long* keys_[2] = {nullptr, nullptr};  // 32-bit keys use the same arrays
   std::uint32_t* values_[2] = {nullptr, nullptr};
+  std::uint8_t* run_flags_ = nullptr;           // partition flags, one per queue entry
+  unsigned int* selected_count_ = nullptr;      // the partition's count (the host knows it)
   void* temporary_ = nullptr;
   std::size_t temporary_bytes_ = 0;
   std::uint64_t capacity_ = 0;
