@@ -7,8 +7,8 @@
 // A stable two-way partition of the queue indices moves the parked entries
 // (showers whose call does not run this front) behind the participants, in
 // queue order; one stable LSD radix sort of (key, queue index) pairs over the
-// participants follows (32-bit keys when the fields fit, else 64-bit), then a
-// gather into the front buffer. The key, most significant first:
+// participants follows, then a gather into the front buffer. The key, most
+// significant first:
 //   slot            the pool slot: every shower's particles are contiguous;
 //   bucket          wavefront_bucket_key (PID bucket, medium, energy bin) for
 //                   showers whose front holds at least 256 particles, else 0;
@@ -88,7 +88,7 @@ class FrontOrder {
   std::int32_t* media_ = nullptr;
   std::uint32_t media_count_ = 0;
   std::uint32_t medium_bits_ = 0;
-  unsigned long long* keys_[2] = {nullptr, nullptr};  // 32-bit keys use the same arrays
+  unsigned long long* keys_[2] = {nullptr, nullptr};
   std::uint32_t* values_[2] = {nullptr, nullptr};
   std::uint8_t* run_flags_ = nullptr;           // partition flags, one per queue entry
   unsigned int* selected_count_ = nullptr;      // the partition's count (the host knows it)
