@@ -21,6 +21,13 @@ namespace {
 
 constexpr int kOrderBlock = 256;
 
+// rocPRIM's radix sort: fronts below this many participants take its merge
+// sort, larger ones its onesweep radix passes. Lower limits (2^16, 2^13) were
+// measured slower on every EM workload (10 TeV with radio: up to 10%).
+constexpr std::size_t kMergeSortLimit = std::size_t{1} << 20;
+using SortConfig = rocprim::radix_sort_config<rocprim::default_config, rocprim::default_config,
+                                              rocprim::default_config, kMergeSortLimit>;
+
 unsigned order_grid(std::uint64_t n) {
   std::uint64_t const g = (n + kOrderBlock - 1) / kOrderBlock;
   if (g > 0x7fffffffULL) throw std::length_error("em: front exceeds the ordering grid");
@@ -184,7 +191,7 @@ void FrontOrder::reserve(std::uint64_t n, std::uint32_t bits, bool partition, hi
   auto const items = static_cast<std::uint32_t>(capacity_);
   // 64-bit keys also for keys of 32 bits or fewer: measured no slower
   if (bits > 0)
-    C8MI_HIP(rocprim::radix_sort_pairs(nullptr, need, keys_[0], keys_[1], values_[0], values_[1], items, 0u, 64u,
+    C8MI_HIP(rocprim::radix_sort_pairs<SortConfig>(nullptr, need, keys_[0], keys_[1], values_[0], values_[1], items, 0u, 64u,
                                        stream));
   if (partition)
     C8MI_HIP(rocprim::partition_two_way(nullptr, part, rocprim::counting_iterator<std::uint32_t>(0u), run_flags_,
@@ -235,7 +242,7 @@ void FrontOrder::order(ParticleQueue const& in, ParticleQueue const& out, std::u
   if (bits != 0 && m != 0) {
     std::size_t bytes = temporary_bytes_;
     auto const items = static_cast<std::uint32_t>(m);
-    C8MI_HIP(rocprim::radix_sort_pairs(temporary_, bytes, keys_[0], keys_[1], values_[0], values_[1], items, 0u, bits,
+    C8MI_HIP(rocprim::radix_sort_pairs<SortConfig>(temporary_, bytes, keys_[0], keys_[1], values_[0], values_[1], items, 0u, bits,
                                        stream));
     order = values_[1];
   } else if (partition) {
