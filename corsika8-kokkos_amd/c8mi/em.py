@@ -256,7 +256,8 @@ class Statistics(ctypes.Structure):
         "particles_advanced", "particles_produced", "interactions_selected", "bucketed_fronts",
         "bucketed_particles", "small_fronts", "small_particles", "ordered_fronts", "photon_calls",
         "lepton_calls", "below_minimum_batch_checkpoints", "history_range_checkpoints",
-        "reserved_history_ids", "remaining_downloaded", "host_to_device_bytes", "device_to_host_bytes")]
+        "reserved_history_ids", "remaining_downloaded", "host_to_device_bytes", "device_to_host_bytes",
+        "parked_fronts", "parked_particles")]
 
     def as_dict(self):
         return {k: int(getattr(self, k)) for k, _ in self._fields_}

@@ -751,6 +751,10 @@ bool EmBackend::front(bool photon, c8mi_em_run_result& out) {
     layout.parked = parked ? 1u : 0u;
     order_.order(cur, alt, n, m, control_dev_.data(), active_range_, history_.data(), layout, stream_);
     ++totals_.ordered_fronts;
+    if (parked) {
+      ++totals_.parked_fronts;
+      totals_.parked_particles += n - m;
+    }
   }
   // ordered: the front reads the gathered buffer and writes its continuing
   // particles back into the current one; else it reads the current buffer

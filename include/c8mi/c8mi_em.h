@@ -467,6 +467,8 @@ typedef struct c8mi_em_statistics {
   uint64_t remaining_downloaded; /* particles returned by wavefront calls */
   uint64_t host_to_device_bytes;
   uint64_t device_to_host_bytes;
+  uint64_t parked_fronts;         /* ordered fronts with showers sitting them out (partitioned behind) */
+  uint64_t parked_particles;      /* entries of those showers carried over unchanged */
 } c8mi_em_statistics;
 
 typedef struct c8mi_em_fallback {

@@ -432,7 +432,14 @@ def test_serving_loop_results_equal_solo_runs(tables, slots, budget, moliere_tab
                          fallback_capacity=1 << 10, moliere=moliere_table is not None, moliere_table=moliere_table)
     dev = em.EmBackend(cfg, tables, atm)
     results, totals = dev.serve(serve_requests(atm), budget, realtime=False)
+    stats = dev.statistics()
     dev.close()
+    if slots > 1:
+        # fronts that showers of the other species' call sat out: their entries
+        # were partitioned behind the participants and carried over
+        assert stats["parked_fronts"] > 0 and stats["parked_particles"] > 0, stats
+    else:
+        assert stats["parked_fronts"] == 0 and stats["parked_particles"] == 0, stats
     steps = 0
     for case, res in zip(POOL_CASES, results):
         ocfg, want = solo_oracle(tables, atm, case, moliere_table=moliere_table)

@@ -183,6 +183,8 @@ def test_pool_of_large_and_small_showers_equals_solo_runs(tables):
         dev.slot_enqueue(slot + 1, em.particle(pid, energy, START, DOWN, history_id=1))
     totals = dev.run_rounds()
     assert totals["bucketed_fronts"] > 0
+    stats = dev.statistics()
+    assert stats["parked_fronts"] > 0 and stats["parked_particles"] > 0, stats
     for slot, (pid, energy, shower) in enumerate(cases):
         prim = em.particle(pid, energy, START, DOWN, history_id=1)
         want = oracle_lib.em_run_shower(cfg, tables, atm, prim, shower_id=shower, first_history_id=2)
